@@ -1,0 +1,85 @@
+"""Builds the gfx950 HIP library (and, for tests, the CPU oracle) in-tree.
+
+The library is compiled with hipcc for gfx950 only, with FP contraction off
+(ray geometry keeps the IEEE operation sequence of its source expressions)
+and packed-FP32 code generation disabled.
+"""
+from __future__ import annotations
+
+import concurrent.futures as futures
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PACKAGE = Path(__file__).resolve().parent
+ROOT = PACKAGE.parent
+CSRC = PACKAGE / "csrc"
+LIBDIR = PACKAGE / "lib"
+OBJDIR = PACKAGE / "_build"
+LIBRARY = LIBDIR / "libdvs_hip.so"
+
+ARCH = "gfx950"
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-fPIC",
+    "-std=c++17",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    f"-I{ROOT / 'include'}",
+    f"-I{CSRC}",
+]
+HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
+HIP_SOURCES = ["dvs_kernels.hip", "dvs_capi.hip", "events_sort.hip"]
+HOST_SOURCES = ["bvh_host.cpp"]
+
+
+def _hipcc() -> str:
+    for cand in (shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if cand and Path(cand).exists():
+            return cand
+    raise RuntimeError("hipcc not found; the gfx950 library cannot be built")
+
+
+def _run(cmd: list[str]) -> None:
+    proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError(f"build step failed ({proc.returncode}): {' '.join(cmd)}\n{proc.stdout}")
+
+
+def build_library(jobs: int = 4) -> Path:
+    """Compiles every HIP/C++ source of the library and links libdvs_hip.so."""
+    hipcc = _hipcc()
+    OBJDIR.mkdir(exist_ok=True)
+    LIBDIR.mkdir(exist_ok=True)
+    steps = []
+    objects = []
+    for src in HIP_SOURCES:
+        obj = OBJDIR / (src + ".o")
+        objects.append(obj)
+        steps.append([hipcc, *HIP_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
+    for src in HOST_SOURCES:
+        obj = OBJDIR / (src + ".o")
+        objects.append(obj)
+        steps.append(["g++", *HOST_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
+    with futures.ThreadPoolExecutor(max_workers=max(1, jobs)) as pool:
+        for f in [pool.submit(_run, s) for s in steps]:
+            f.result()
+    tmp = LIBRARY.with_suffix(".so.tmp")
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objects)])
+    os.replace(tmp, LIBRARY)
+    return LIBRARY
+
+
+def build_oracle() -> Path:
+    """Builds the CPU oracle (test infrastructure) with its Makefile."""
+    _run(["make", "-s", "-C", str(ROOT / "oracle")])
+    return ROOT / "oracle" / "_build" / "liboracle_dvs.so"
+
+
+if __name__ == "__main__":
+    print(build_library(), file=sys.stderr)
+    if "--with-oracle" in sys.argv:
+        print(build_oracle(), file=sys.stderr)

@@ -1,0 +1,415 @@
+"""ctypes binding of the gfx950 library's C ABI (include/dvs_hip.h).
+
+Mirrors the reference's C++ objects used on the hot path:
+``Scene`` ~ DeviceScene + CamRayCaster (include/dvs/device_scene.hpp,
+include/dvs/raycast.hpp:64-86), ``Sensor`` ~ EventSensor
+(include/dvs/event_sensor.hpp:40-79), ``bvh_build`` ~ BVHBuilder::build
+(include/bvh/build.h:19-24).  Errors are raised as the reference's exception
+classes would be: ``InvalidArgument`` (std::invalid_argument) and
+``DvsError`` (std::runtime_error), with the same messages.
+
+There is no CPU fallback: if the HIP library is missing, importing
+``load_library`` raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+from pathlib import Path
+
+import numpy as np
+
+LIBRARY_PATH = Path(__file__).resolve().parent / "lib" / "libdvs_hip.so"
+
+DVS_OK, DVS_E_CAPACITY, DVS_E_OPTICAL, DVS_E_ARGUMENT, DVS_E_HIP, DVS_E_STATE = range(6)
+
+NODE_DTYPE = np.dtype([("min", "<f4", (3,)), ("max", "<f4", (3,)), ("left", "<i4"), ("right", "<i4"),
+                       ("start", "<i4"), ("count", "<i4")])
+STATE_DTYPE = np.dtype([("position", "<f4", (3,)), ("velocity", "<f4", (3,)), ("light", "<f4", (3,)),
+                        ("light_rate", "<f4", (3,)), ("rotation", "<f4", (4,)), ("intensity", "<f4"),
+                        ("intensity_rate", "<f4")])
+INTERVAL_DTYPE = np.dtype([("start", STATE_DTYPE), ("end", STATE_DTYPE), ("t0", "<f8"), ("t1", "<f8")])
+EVENT_DTYPE = np.dtype([("timestamp", "<f8"), ("x", "<i2"), ("y", "<i2"), ("polarity", "i1"),
+                        ("depth", "<f4")])
+PIXEL_STATE_DTYPE = np.dtype([("receptor_log", "<f8"), ("reference_log", "<f8"), ("refractory_until", "<f8"),
+                              ("threshold_pos", "<f8"), ("threshold_neg", "<f8"), ("reset_pending", "<i4"),
+                              ("reset_block", "<u4"), ("shot_block", "<u8")])
+assert NODE_DTYPE.itemsize == 40 and STATE_DTYPE.itemsize == 72 and INTERVAL_DTYPE.itemsize == 160
+assert EVENT_DTYPE.itemsize == 17 and PIXEL_STATE_DTYPE.itemsize == 56
+
+
+class DvsError(RuntimeError):
+    """Failure reported by the library (std::runtime_error in the reference)."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+
+
+class InvalidArgument(DvsError, ValueError):
+    """Rejected input (std::invalid_argument in the reference)."""
+
+
+@dataclasses.dataclass(frozen=True)
+class Hapke:
+    """HapkeConfig (include/dvs/scene_types.hpp:89-99) with its defaults."""
+    b: float = 0.24
+    c: float = 0.305
+    hS: float = 0.08
+    hC: float = 1.0
+    thetap: float = 26.0 * np.float32(np.pi) / 180.0
+    w: float = 0.324
+    B_S0: float = 2.42
+    B_C0: float = 0.0
+    phi: float = 0.3
+
+
+@dataclasses.dataclass(frozen=True)
+class Camera:
+    """CameraConfig (scene_types.hpp:59-62) plus scene.supersample_factor."""
+    width: int
+    height: int
+    fx: float
+    fy: float
+    cx: float
+    cy: float
+    samples: int = 1
+
+
+@dataclasses.dataclass(frozen=True)
+class Physic:
+    """PhysicConfig (scene_types.hpp:64-79), same fields and defaults."""
+    threshold_pos: float = 0.3
+    threshold_neg: float = 0.3
+    refractory_time: float = 0.0002
+    readout_bandwidth: float = 5e6
+    readout_latency: float = 0.0
+    threshold_std: float = 0.03
+    threshold_min: float = 0.01
+    wavelength_nm: float = 550
+    quantum_efficiency: float = 0.3
+    pixel_pitch_um: float = 15
+    fill_factor: float = 0.2
+    f_number: float = 2
+    lens_transmission: float = 0.8
+    radiance_scale: float = 0.001
+    temperature_k: float = 300
+    subthreshold_kappa: float = 0.8
+    receptor_capacitance_f: float = 1e-15
+    reset_capacitance_f: float = 20e-15
+    dark_current_a: float = 1e-18
+    reset_leakage_a: float = 0.0
+    circuit_step: float = 1e-6
+    photon_shot_noise: int = 1
+    reset_thermal_noise: int = 0
+    random_seed: int = 114514
+    max_events_per_step: int = 16
+    readout_queue_capacity: int = 1000000
+
+
+class _HapkeC(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_float) for n in ("b", "c", "hS", "hC", "thetap", "w", "B_S0", "B_C0", "phi")]
+
+
+class _CameraC(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("fx", ctypes.c_float),
+                ("fy", ctypes.c_float), ("cx", ctypes.c_float), ("cy", ctypes.c_float),
+                ("samples", ctypes.c_int32)]
+
+
+_PHYSIC_INT = {"photon_shot_noise", "reset_thermal_noise", "random_seed", "max_events_per_step",
+               "readout_queue_capacity"}
+
+
+class _PhysicC(ctypes.Structure):
+    _fields_ = [(f.name, ctypes.c_int32 if f.name in _PHYSIC_INT else ctypes.c_float)
+                for f in dataclasses.fields(Physic)]
+
+
+class _StatsC(ctypes.Structure):
+    _fields_ = [("trace_ms", ctypes.c_double), ("circuit_ms", ctypes.c_double), ("events_ms", ctypes.c_double),
+                ("total_ms", ctypes.c_double), ("trace_launches", ctypes.c_double),
+                ("primary_rays", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64),
+                ("event_rays", ctypes.c_uint64), ("pixel_cells", ctypes.c_uint64),
+                ("comparator_events", ctypes.c_uint64), ("delivered_events", ctypes.c_uint64)]
+
+
+def _struct(cls, obj):
+    return cls(**dataclasses.asdict(obj))
+
+
+_lib = None
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_I32 = ctypes.c_int32
+
+
+def load_library() -> ctypes.CDLL:
+    """Loads libdvs_hip.so; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIBRARY_PATH.exists():
+        raise RuntimeError(f"gfx950 library not built: {LIBRARY_PATH} (run __graft_entry__.build())")
+    lib = ctypes.CDLL(str(LIBRARY_PATH))
+    sig = {
+        "dvs_last_error": (ctypes.c_char_p, []),
+        "dvs_device_count": (ctypes.c_int, []),
+        "dvs_bvh_build": (ctypes.c_int, [ctypes.c_int, ctypes.c_float, ctypes.c_float, _P, _I64, _P, _I64, _P,
+                                         ctypes.POINTER(_I64), ctypes.POINTER(_I32)]),
+        "dvs_bvh_validate": (ctypes.c_int, [_P, _I64, _I32]),
+        "dvs_scene_create": (ctypes.c_int, [_P, _I64, _I32, _P, _P, _I64, _P, _I64, _P, ctypes.POINTER(_CameraC),
+                                            ctypes.POINTER(_HapkeC), ctypes.POINTER(_P)]),
+        "dvs_scene_destroy": (ctypes.c_int, [_P]),
+        "dvs_scene_info": (ctypes.c_int, [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I64), ctypes.POINTER(_I64),
+                                          ctypes.POINTER(_I64)]),
+        "dvs_frame_render": (ctypes.c_int, [_P, _P, _I32, _P, _P, _P]),
+        "dvs_frame_render_device": (ctypes.c_int, [_P, _P, _I32, _P, _P]),
+        "dvs_scene_stats": (ctypes.c_int, [_P, ctypes.POINTER(_StatsC)]),
+        "dvs_sensor_create": (ctypes.c_int, [_P, ctypes.POINTER(_PhysicC), _I32, _I32, ctypes.POINTER(_P)]),
+        "dvs_sensor_destroy": (ctypes.c_int, [_P]),
+        "dvs_sensor_initialize": (ctypes.c_int, [_P, _P, ctypes.c_double]),
+        "dvs_sensor_process": (ctypes.c_int, [_P, _P, _I32, ctypes.POINTER(_I64)]),
+        "dvs_sensor_take": (ctypes.c_int, [_P, _P, _I64, ctypes.POINTER(_I64)]),
+        "dvs_sensor_raw_events": (ctypes.c_int, [_P, _P, _I64, ctypes.POINTER(_I64)]),
+        "dvs_sensor_drain": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),
+        "dvs_sensor_readout_stats": (ctypes.c_int, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
+        "dvs_sensor_states": (ctypes.c_int, [_P, _P]),
+        "dvs_sensor_stats": (ctypes.c_int, [_P, ctypes.POINTER(_StatsC)]),
+        "dvs_sensor_reset_stats": (ctypes.c_int, [_P]),
+        "dvs_hapke_eval": (ctypes.c_int, [_P, _P, _P, _P, _I64, ctypes.POINTER(_HapkeC), _P]),
+        "dvs_receptor_eval": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, _I64, _P]),
+        "dvs_noise_eval": (ctypes.c_int, [ctypes.c_uint64, _I64, ctypes.c_double, _I32, _P, _P]),
+        "dvs_trace_rays": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P, _P, _P, _P]),
+        "dvs_trace_shadow_rays": (ctypes.c_int, [_P, _P, _P, _P, _I64, _P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols() -> list[str]:
+    """Names of every C entry point declared in include/dvs_hip.h."""
+    header = (Path(__file__).resolve().parent.parent / "include" / "dvs_hip.h").read_text()
+    import re
+    return sorted(set(re.findall(r"\b(dvs_[a-z0-9_]+)\s*\(", header)))
+
+
+def _check(code: int) -> None:
+    if code == DVS_OK:
+        return
+    message = load_library().dvs_last_error().decode()
+    if code == DVS_E_ARGUMENT:
+        raise InvalidArgument(code, message)
+    raise DvsError(code, message)
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def device_count() -> int:
+    return int(load_library().dvs_device_count())
+
+
+def bvh_build(vertices: np.ndarray, indices: np.ndarray, max_leaf_size: int = 8, node_cost: float = 1.0,
+              primitive_cost: float = 1.0):
+    """BVHBuilder::build: returns (nodes, root, indices in leaf order)."""
+    lib = load_library()
+    v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+    idx = np.array(indices, dtype=np.uint32, copy=True).reshape(-1, 3)
+    nodes = np.zeros(max(1, 2 * len(idx)), dtype=NODE_DTYPE)
+    count, root = _I64(0), _I32(0)
+    _check(lib.dvs_bvh_build(int(max_leaf_size), float(node_cost), float(primitive_cost), _ptr(v), len(v), _ptr(idx),
+                             len(idx), _ptr(nodes), ctypes.byref(count), ctypes.byref(root)))
+    return nodes[: count.value].copy(), int(root.value), idx
+
+
+def bvh_validate(nodes: np.ndarray, root: int) -> None:
+    _check(load_library().dvs_bvh_validate(_ptr(np.ascontiguousarray(nodes)), len(nodes), int(root)))
+
+
+class Stats(dict):
+    pass
+
+
+def _stats(c: _StatsC) -> Stats:
+    return Stats({name: getattr(c, name) for name, _ in c._fields_})
+
+
+class Scene:
+    """DeviceScene + CamRayCaster over one uploaded BVH and mesh."""
+
+    def __init__(self, nodes, root, vertices, normals, indices, camera: Camera, hapke: Hapke | None = None,
+                 albedos=None):
+        lib = load_library()
+        self._keep = [np.ascontiguousarray(nodes, dtype=NODE_DTYPE),
+                      np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3),
+                      np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 3),
+                      np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1, 3),
+                      None if albedos is None else np.ascontiguousarray(albedos, dtype=np.float32)]
+        n, v, nr, idx, alb = self._keep
+        self.camera = camera
+        self.hapke = hapke or Hapke()
+        handle = _P()
+        cam = _struct(_CameraC, camera)
+        hk = _struct(_HapkeC, self.hapke)
+        _check(lib.dvs_scene_create(_ptr(n), len(n), int(root), _ptr(v), _ptr(nr), len(v), _ptr(idx), len(idx),
+                                    _ptr(alb), ctypes.byref(cam), ctypes.byref(hk), ctypes.byref(handle)))
+        self.handle = handle
+        self._keep = None
+
+    @property
+    def pixels(self) -> int:
+        return self.camera.width * self.camera.height
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load_library().dvs_scene_destroy(self.handle)
+            self.handle = None
+
+    __del__ = close
+
+    def info(self) -> dict:
+        d, i, l, b = _I32(), _I64(), _I64(), _I64()
+        _check(load_library().dvs_scene_info(self.handle, ctypes.byref(d), ctypes.byref(i), ctypes.byref(l),
+                                             ctypes.byref(b)))
+        return {"max_depth": d.value, "internal_nodes": i.value, "leaves": l.value, "device_bytes": b.value}
+
+    def render(self, poses: np.ndarray, with_radiance: bool = False):
+        """render_main over poses: (image, depth[, radiance]), each (n, H, W)."""
+        p = np.ascontiguousarray(np.atleast_1d(poses), dtype=STATE_DTYPE)
+        shape = (len(p), self.camera.height, self.camera.width)
+        img = np.empty(shape, np.float32)
+        dep = np.empty(shape, np.float32)
+        rad = np.empty(shape, np.float64) if with_radiance else None
+        _check(load_library().dvs_frame_render(self.handle, _ptr(p), len(p), _ptr(img), _ptr(dep), _ptr(rad)))
+        return (img, dep, rad) if with_radiance else (img, dep)
+
+    def render_device(self, device_poses_ptr: int, count: int, device_image_ptr: int, device_depth_ptr: int):
+        _check(load_library().dvs_frame_render_device(self.handle, ctypes.c_void_p(device_poses_ptr), count,
+                                                      ctypes.c_void_p(device_image_ptr),
+                                                      ctypes.c_void_p(device_depth_ptr)))
+
+    def stats(self) -> Stats:
+        s = _StatsC()
+        _check(load_library().dvs_scene_stats(self.handle, ctypes.byref(s)))
+        return _stats(s)
+
+    def trace(self, origins, directions):
+        o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(directions, np.float32).reshape(-1, 3)
+        n = len(o)
+        hit, tri = np.empty(n, np.int32), np.empty(n, np.int32)
+        t, u, v = (np.empty(n, np.float32) for _ in range(3))
+        _check(load_library().dvs_trace_rays(self.handle, _ptr(o), _ptr(d), n, _ptr(hit), _ptr(tri), _ptr(t),
+                                             _ptr(u), _ptr(v)))
+        return hit, tri, t, u, v
+
+    def trace_shadow(self, origins, directions, self_triangles):
+        o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(directions, np.float32).reshape(-1, 3)
+        s = np.ascontiguousarray(self_triangles, np.int32)
+        occ = np.empty(len(o), np.int32)
+        _check(load_library().dvs_trace_shadow_rays(self.handle, _ptr(o), _ptr(d), _ptr(s), len(o), _ptr(occ)))
+        return occ
+
+
+class Sensor:
+    """EventSensor on a Scene (native solver)."""
+
+    def __init__(self, scene: Scene, physic: Physic, physic_effects: bool = True, aer_delay: bool = True):
+        lib = load_library()
+        self.scene = scene
+        self.physic = physic
+        handle = _P()
+        ph = _struct(_PhysicC, physic)
+        _check(lib.dvs_sensor_create(scene.handle, ctypes.byref(ph), int(physic_effects), int(aer_delay),
+                                     ctypes.byref(handle)))
+        self.handle = handle
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load_library().dvs_sensor_destroy(self.handle)
+            self.handle = None
+
+    __del__ = close
+
+    def initialize(self, pose: np.ndarray, timestamp: float) -> None:
+        p = np.ascontiguousarray(np.atleast_1d(pose)[:1], dtype=STATE_DTYPE)
+        _check(load_library().dvs_sensor_initialize(self.handle, _ptr(p), float(timestamp)))
+
+    def _take(self, count: int) -> np.ndarray:
+        out = np.empty(count, EVENT_DTYPE)
+        got = _I64()
+        _check(load_library().dvs_sensor_take(self.handle, _ptr(out), count, ctypes.byref(got)))
+        return out[: got.value]
+
+    def process(self, intervals: np.ndarray) -> np.ndarray:
+        """process_many: returns the readout-delivered events of this call."""
+        iv = np.ascontiguousarray(np.atleast_1d(intervals), dtype=INTERVAL_DTYPE)
+        ready = _I64()
+        _check(load_library().dvs_sensor_process(self.handle, _ptr(iv), len(iv), ctypes.byref(ready)))
+        return self._take(ready.value)
+
+    def raw_events(self) -> np.ndarray:
+        """Comparator events of the last process call, (t, y, x) order, before readout."""
+        lib = load_library()
+        n = _I64()
+        # Size query: capacity 0 reports the count (and an argument error if any exist).
+        code = lib.dvs_sensor_raw_events(self.handle, None, 0, ctypes.byref(n))
+        if code not in (DVS_OK, DVS_E_ARGUMENT):
+            _check(code)
+        out = np.empty(n.value, EVENT_DTYPE)
+        _check(lib.dvs_sensor_raw_events(self.handle, _ptr(out), n.value, ctypes.byref(n)))
+        return out
+
+    def drain(self) -> np.ndarray:
+        ready = _I64()
+        _check(load_library().dvs_sensor_drain(self.handle, ctypes.byref(ready)))
+        return self._take(ready.value)
+
+    def readout_stats(self) -> tuple[int, int]:
+        d, p = _I64(), _I64()
+        _check(load_library().dvs_sensor_readout_stats(self.handle, ctypes.byref(d), ctypes.byref(p)))
+        return d.value, p.value
+
+    def states(self) -> np.ndarray:
+        out = np.empty(self.scene.pixels, PIXEL_STATE_DTYPE)
+        _check(load_library().dvs_sensor_states(self.handle, _ptr(out)))
+        return out
+
+    def stats(self) -> Stats:
+        s = _StatsC()
+        _check(load_library().dvs_sensor_stats(self.handle, ctypes.byref(s)))
+        return _stats(s)
+
+    def reset_stats(self) -> None:
+        _check(load_library().dvs_sensor_reset_stats(self.handle))
+
+
+def hapke_eval(cosi, cose, cosg, albedo, hapke: Hapke) -> np.ndarray:
+    a, b, c, d = (np.ascontiguousarray(x, np.float32) for x in np.broadcast_arrays(cosi, cose, cosg, albedo))
+    a, b, c, d = (np.ascontiguousarray(x) for x in (a, b, c, d))
+    out = np.empty(a.shape, np.float32)
+    hk = _struct(_HapkeC, hapke)
+    _check(load_library().dvs_hapke_eval(_ptr(a), _ptr(b), _ptr(c), _ptr(d), a.size, ctypes.byref(hk), _ptr(out)))
+    return out
+
+
+def receptor_eval(u, rate, dt, charge: float) -> np.ndarray:
+    a, b, c = (np.ascontiguousarray(x, np.float64) for x in np.broadcast_arrays(u, rate, dt))
+    a, b, c = (np.ascontiguousarray(x) for x in (a, b, c))
+    out = np.empty(a.shape, np.float64)
+    _check(load_library().dvs_receptor_eval(_ptr(a), _ptr(b), _ptr(c), float(charge), a.size, _ptr(out)))
+    return out
+
+
+def noise_eval(seed: int, pixels: int, lam: float, draws: int):
+    p = np.empty((pixels, draws), np.float64)
+    q = np.empty((pixels, draws), np.float64)
+    _check(load_library().dvs_noise_eval(int(seed), int(pixels), float(lam), int(draws), _ptr(p), _ptr(q)))
+    return p, q

@@ -1,0 +1,186 @@
+// gfx950 device pixel circuit (FP64) and counter-based noise.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dvs_hip.h"
+
+namespace dvsdev {
+
+// PixelParameters (include/dvs/pixel_model.hpp:7-10).
+struct CircuitParams {
+  double threshold_pos, threshold_neg, charge_scale, refractory_time;
+  double reference_leak, reset_noise_std;
+};
+// PhotoParameters (include/dvs/optical_sensor.hpp:21-25).
+struct PhotoParams {
+  double electrons_per_radiance, dark_electrons, integration_step;
+  int shot_noise;
+  int pad;
+};
+struct CrossingEvent { double time; int polarity; int pad; };
+
+// Exact constant-rate receptor solution (receptor_log_at, pixel_model.hpp:18-26).
+__device__ __forceinline__ double receptor_log_at(double u, double rate, double dt, double charge) {
+  if (dt <= 0) return u;
+  rate = fmax(0.0, rate);
+  const double x = exp(u);
+  if (rate == 0) return u - log1p(x * dt / charge);
+  const double a = exp(-rate * dt / charge), b = -expm1(-rate * dt / charge);
+  return -log(a / x + b / rate);
+}
+
+// ---------------------------------------------------------------------------
+// Noise.  Philox4x64-10 (Salmon, Moraes, Dror, Shaw, SC'11) keyed by
+// (seed, tag ^ stream), counter (block, pixel, 0, 0).  Each draw starts a
+// fresh block, so a pixel's stream depends only on how many draws it has
+// made: batching, launch shape and the presence of other requests cannot
+// change it.
+// ---------------------------------------------------------------------------
+enum { kStreamShot = 0, kStreamThreshold = 1, kStreamReset = 2 };
+constexpr uint64_t kStreamTag = 0x4456535F4E4F4953ULL;
+
+__device__ __forceinline__ void philox4x64(uint64_t c0, uint64_t c1, uint64_t c2, uint64_t c3, uint64_t k0,
+                                           uint64_t k1, uint64_t out[4]) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r) { k0 += 0x9E3779B97F4A7C15ULL; k1 += 0xBB67AE8584CAA73BULL; }
+    const uint64_t lo0 = 0xD2E7470EE14C6C93ULL * c0, hi0 = __umul64hi(0xD2E7470EE14C6C93ULL, c0);
+    const uint64_t lo1 = 0xCA5A826395121157ULL * c2, hi1 = __umul64hi(0xCA5A826395121157ULL, c2);
+    const uint64_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+    c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+struct NoiseStream {
+  uint64_t seed, pixel, stream, block;
+  uint64_t buf[4];
+  int used;
+  __device__ NoiseStream(uint64_t s, uint64_t p, uint64_t st, uint64_t b)
+      : seed(s), pixel(p), stream(st), block(b), used(4) {}
+  __device__ double uniform() {  // open interval (0, 1), 53 bits
+    if (used == 4) { philox4x64(block++, pixel, 0, 0, seed, kStreamTag ^ stream, buf); used = 0; }
+    return ((double)(buf[used++] >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  }
+  // Box-Muller, one normal per fresh block.
+  __device__ double normal() {
+    used = 4;
+    const double u1 = uniform(), u2 = uniform();
+    return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+  }
+  // Poisson(lambda): sequential-search inversion below 10 (Devroye 1986,
+  // ch. X.3), PTRS transformed rejection (Hoermann 1993) up to 1e6, and the
+  // reference's rounded normal approximation above (photon_noise.cuh:4-9).
+  __device__ double poisson(double lambda) {
+    if (lambda <= 0) return 0;
+    if (lambda > 1e6) return fmax(0.0, floor(lambda + sqrt(lambda) * normal() + .5));
+    used = 4;
+    if (lambda < 10) {
+      const double u = uniform();
+      double p = exp(-lambda), F = p;
+      int k = 0;
+      while (u > F && k < 1000) { ++k; p *= lambda / k; F += p; }
+      return k;
+    }
+    const double slam = sqrt(lambda), loglam = log(lambda);
+    const double b = 0.931 + 2.53 * slam, a = -0.059 + 0.02483 * b;
+    const double invalpha = 1.1239 + 1.1328 / (b - 3.4), vr = 0.9277 - 3.6224 / (b - 2);
+    for (;;) {
+      const double U = uniform() - 0.5, V = uniform();
+      const double us = 0.5 - fabs(U);
+      const double k = floor((2 * a / us + b) * U + lambda + 0.43);
+      if (us >= 0.07 && V <= vr) return k;
+      if (k < 0 || (us < 0.013 && V > us)) continue;
+      if (log(V) + log(invalpha) - log(a / (us * us) + b) <= -lambda + k * loglam - lgamma(k + 1)) return k;
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// advance_pixel (include/dvs/pixel_model.hpp:35-102) in FP64: refractory hold,
+// leakage-extremum split, analytic crossing root without leakage and a
+// 48-step bisection with it, reset at the crossing voltage.
+// ---------------------------------------------------------------------------
+struct PixelRegs {
+  double receptor_log, reference_log, refractory_until, threshold_pos, threshold_neg;
+  int reset_pending;
+};
+template <class Normal>
+__device__ inline int advance_pixel(PixelRegs &s, const CircuitParams &p, double rate, double t0, double t1,
+                                    CrossingEvent *out, int capacity, Normal &&normal) {
+  if (t1 <= t0) return 0;
+  int count = 0;
+  double cursor = t0, u = s.receptor_log;
+  while (cursor < t1) {
+    if (cursor < s.refractory_until) {
+      const double end = fmin(t1, s.refractory_until);
+      u = receptor_log_at(u, rate, end - cursor, p.charge_scale);
+      cursor = end;
+      s.reference_log = u;
+      if ((cursor < t1 || cursor == s.refractory_until) && cursor >= s.refractory_until && s.reset_pending) {
+        if (p.reset_noise_std > 0) s.reference_log += p.reset_noise_std * normal();
+        s.reset_pending = 0;
+      }
+      if (cursor >= t1) break;
+    } else if (s.reset_pending) {
+      if (p.reset_noise_std > 0) s.reference_log += p.reset_noise_std * normal();
+      s.reset_pending = 0;
+    }
+    const double remaining = t1 - cursor, c0 = u - s.reference_log;
+    double boundary = remaining;
+    const double x = exp(u), target = rate + p.charge_scale * p.reference_leak;
+    if (p.reference_leak > 0 && x > target && target > rate) {
+      double extremum;
+      if (rate > 0) extremum = -p.charge_scale / rate * log((1.0 / target - 1.0 / rate) / (1.0 / x - 1.0 / rate));
+      else extremum = p.charge_scale * (1.0 / target - 1.0 / x);
+      if (extremum > 1e-15 && extremum < boundary) boundary = extremum;
+    }
+    const double ub = receptor_log_at(u, rate, boundary, p.charge_scale);
+    const double cb = ub - s.reference_log + p.reference_leak * boundary;
+    int polarity = -1;
+    double level = 0;
+    if (c0 >= s.threshold_pos) { polarity = 1; level = s.threshold_pos; }
+    else if (c0 <= -s.threshold_neg) { polarity = 0; level = -s.threshold_neg; }
+    else if (cb >= s.threshold_pos) { polarity = 1; level = s.threshold_pos; }
+    else if (cb <= -s.threshold_neg) { polarity = 0; level = -s.threshold_neg; }
+    if (polarity < 0) {
+      u = ub;
+      s.reference_log -= p.reference_leak * boundary;
+      cursor += boundary;
+      continue;
+    }
+    if (count == capacity) return -1;
+    double lo = 0, hi = boundary;
+    if (c0 >= s.threshold_pos || c0 <= -s.threshold_neg) {
+      hi = 0;
+    } else if (p.reference_leak == 0) {
+      const double target_inverse = exp(-(s.reference_log + level));
+      const double initial_inverse = exp(-u);
+      if (rate > 0) {
+        const double stationary_inverse = 1.0 / rate;
+        hi = -p.charge_scale / rate * log1p((target_inverse - initial_inverse) / (initial_inverse - stationary_inverse));
+      } else {
+        hi = p.charge_scale * (target_inverse - initial_inverse);
+      }
+      hi = fmin(boundary, fmax(0.0, hi));
+    } else {
+      for (int j = 0; j < 48; ++j) {
+        const double mid = (lo + hi) * .5;
+        const double c = receptor_log_at(u, rate, mid, p.charge_scale) - s.reference_log + p.reference_leak * mid;
+        if (polarity ? c >= level : c <= level) hi = mid; else lo = mid;
+      }
+    }
+    u = receptor_log_at(u, rate, hi, p.charge_scale);
+    cursor += hi;
+    out[count].time = cursor;
+    out[count].polarity = polarity;
+    ++count;
+    s.reference_log = u;
+    s.refractory_until = cursor + p.refractory_time;
+    s.reset_pending = 1;
+  }
+  s.receptor_log = u;
+  return count;
+}
+
+}  // namespace dvsdev

@@ -1,0 +1,398 @@
+// gfx950 device optics: SoA BVH traversal, Moller-Trumbore, shadow origin,
+// Hapke shading and the per-subray sample.  Every function keeps the
+// reference's IEEE operation sequence (the library is built with
+// -ffp-contract=off) so that ray geometry is bit-identical to a host
+// evaluation of the same source expressions; only the float transcendentals
+// of the Hapke BRDF (ocml vs. a host libm) differ in the last bits.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dvs_hip.h"
+
+namespace dvsdev {
+
+struct f3 { float x, y, z; };
+__device__ __forceinline__ f3 mk(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 add(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ f3 sub(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ f3 neg(f3 a) { return mk(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ f3 scale(f3 a, float b) { return mk(a.x * b, a.y * b, a.z * b); }
+__device__ __forceinline__ f3 divide(f3 a, float b) { return mk(a.x / b, a.y / b, a.z / b); }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 cross(f3 a, f3 b) {
+  return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ f3 normalize(f3 v) {
+  const float len = sqrtf(dot(v, v));
+  return len > 0.0f ? divide(v, len) : mk(0.f, 0.f, 0.f);
+}
+__device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+
+constexpr float kRayEps = 1e-6f;                  // include/helper_math.hpp:11
+constexpr float kPi = 3.14159265358979323846f;    // include/dvs/math_types.hpp
+
+// Device view of one scene (SoA BVH + indexed mesh), passed by value.
+struct DevScene {
+  const float4 *pairs;       // 4 x float4 per internal node
+  const int2 *leaves;        // (start, count)
+  const float4 *triangles;   // 3 x float4 per BVH-ordered triangle: v0 | e1 | e2
+  const uint32_t *indices;   // 3 per BVH-ordered triangle
+  const float *normals;      // 3 per vertex
+  const float *albedos;      // per vertex or nullptr
+  int root_ref, stack_depth;
+  int width, height, samples;
+  float fx, fy, cx, cy;
+  dvs_hapke hapke;
+};
+
+// Pose used by the ray kernels: the interpolated OpticalState fields rays read.
+struct Pose {
+  float px, py, pz;
+  float lx, ly, lz;
+  float qx, qy, qz, qw;
+  float intensity;
+  float pad;
+};
+
+// ---------------------------------------------------------------------------
+// Pose interpolation: optical_state_at (include/dvs/device_optics.cuh:6-26)
+// with the quaternion renormalised by 1/sqrtf (the reference's rsqrtf is an
+// approximate intrinsic).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float hermite(float a, float va, float b, float vb, float u, float dt) {
+  const float u2 = u * u, u3 = u2 * u;
+  return a + (-2 * u3 + 3 * u2) * (b - a) + (u3 - 2 * u2 + u) * dt * va + (u3 - u2) * dt * vb;
+}
+__device__ inline Pose pose_at(const dvs_optical_interval &iv, double t) {
+  const float dt = (float)(iv.t1 - iv.t0);
+  const float u = dt > 0 ? (float)fmin(1.0, fmax(0.0, (t - iv.t0) / dt)) : 0.f;
+  const dvs_optical_state &a = iv.start, &b = iv.end;
+  Pose p;
+  p.px = hermite(a.position[0], a.velocity[0], b.position[0], b.velocity[0], u, dt);
+  p.py = hermite(a.position[1], a.velocity[1], b.position[1], b.velocity[1], u, dt);
+  p.pz = hermite(a.position[2], a.velocity[2], b.position[2], b.velocity[2], u, dt);
+  p.lx = hermite(a.light[0], a.light_rate[0], b.light[0], b.light_rate[0], u, dt);
+  p.ly = hermite(a.light[1], a.light_rate[1], b.light[1], b.light_rate[1], u, dt);
+  p.lz = hermite(a.light[2], a.light_rate[2], b.light[2], b.light_rate[2], u, dt);
+  p.intensity = fmaxf(0.f, hermite(a.intensity, a.intensity_rate, b.intensity, b.intensity_rate, u, dt));
+  float q0 = b.rotation[0], q1 = b.rotation[1], q2 = b.rotation[2], q3 = b.rotation[3];
+  float d = a.rotation[0] * q0 + a.rotation[1] * q1 + a.rotation[2] * q2 + a.rotation[3] * q3;
+  if (d < 0) { q0 = -q0; q1 = -q1; q2 = -q2; q3 = -q3; d = -d; }
+  float wa = 1 - u, wb = u;
+  if (d < .9995f) {
+    const float angle = acosf(fminf(1.f, d)), inv = 1 / sinf(angle);
+    wa = sinf((1 - u) * angle) * inv;
+    wb = sinf(u * angle) * inv;
+  }
+  const float x = wa * a.rotation[0] + wb * q0, y = wa * a.rotation[1] + wb * q1;
+  const float z = wa * a.rotation[2] + wb * q2, w = wa * a.rotation[3] + wb * q3;
+  const float norm = 1.0f / sqrtf(x * x + y * y + z * z + w * w);
+  p.qx = x * norm; p.qy = y * norm; p.qz = z * norm; p.qw = w * norm;
+  p.pad = 0;
+  return p;
+}
+__device__ __forceinline__ Pose pose_of(const dvs_optical_state &s) {
+  Pose p;
+  p.px = s.position[0]; p.py = s.position[1]; p.pz = s.position[2];
+  p.lx = s.light[0]; p.ly = s.light[1]; p.lz = s.light[2];
+  p.qx = s.rotation[0]; p.qy = s.rotation[1]; p.qz = s.rotation[2]; p.qw = s.rotation[3];
+  p.intensity = s.intensity; p.pad = 0;
+  return p;
+}
+// rotate_ray (device_optics.cuh:27-29)
+__device__ __forceinline__ f3 rotate_ray(const Pose &q, f3 v) {
+  const f3 qv = mk(q.qx, q.qy, q.qz);
+  const f3 c = scale(cross(qv, v), 2.0f);
+  return add(add(v, scale(c, q.qw)), cross(qv, c));
+}
+
+// ---------------------------------------------------------------------------
+// Traversal over the SoA layout.  Order of node visits is the reference's
+// near-first order (device_ray.hpp:120-127,153): the nearer child is taken at
+// once, the farther one goes on the per-lane LDS stack, so the first triangle
+// found at a tied distance is the same one the reference finds.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ f3 inverse_direction(f3 d) {  // device_ray.hpp:6-12
+  f3 inv;
+  inv.x = 1.0f / (fabsf(d.x) > 1e-20f ? d.x : copysignf(1e-20f, d.x));
+  inv.y = 1.0f / (fabsf(d.y) > 1e-20f ? d.y : copysignf(1e-20f, d.y));
+  inv.z = 1.0f / (fabsf(d.z) > 1e-20f ? d.z : copysignf(1e-20f, d.z));
+  return inv;
+}
+// Slab test of both children of one pair record (device_ray.hpp:13-53).
+__device__ __forceinline__ int test_pair(const float4 *rec, f3 P, f3 idir, float t_min, float t_max,
+                                         float &d0, float &d1, int &left, int &right) {
+  const float4 a = rec[0], b = rec[1], c = rec[2], r = rec[3];
+  const float t1x0 = (a.x - P.x) * idir.x, t1x1 = (a.w - P.x) * idir.x;
+  const float t1y0 = (a.y - P.y) * idir.y, t1y1 = (b.x - P.y) * idir.y;
+  const float t1z0 = (a.z - P.z) * idir.z, t1z1 = (b.y - P.z) * idir.z;
+  float t1min = fmaxf(fmaxf(fminf(t1x0, t1x1), fminf(t1y0, t1y1)), fminf(t1z0, t1z1));
+  float t1max = fminf(fminf(fmaxf(t1x0, t1x1), fmaxf(t1y0, t1y1)), fmaxf(t1z0, t1z1));
+  t1min = fmaxf(t_min, t1min); t1max = fminf(t_max, t1max);
+  const float t2x0 = (b.z - P.x) * idir.x, t2x1 = (c.y - P.x) * idir.x;
+  const float t2y0 = (b.w - P.y) * idir.y, t2y1 = (c.z - P.y) * idir.y;
+  const float t2z0 = (c.x - P.z) * idir.z, t2z1 = (c.w - P.z) * idir.z;
+  float t2min = fmaxf(fmaxf(fminf(t2x0, t2x1), fminf(t2y0, t2y1)), fminf(t2z0, t2z1));
+  float t2max = fminf(fminf(fmaxf(t2x0, t2x1), fmaxf(t2y0, t2y1)), fmaxf(t2z0, t2z1));
+  t2min = fmaxf(t_min, t2min); t2max = fminf(t_max, t2max);
+  d0 = t1min; d1 = t2min;
+  left = __float_as_int(r.x); right = __float_as_int(r.y);
+  return (t1max >= t1min ? 1 : 0) | (t2max >= t2min ? 2 : 0);
+}
+// Moller-Trumbore on a pre-gathered record (device_ray.hpp:54-80).
+__device__ __forceinline__ bool intersect(const float4 *rec, f3 ray, f3 o, float &ot, float &ou, float &ov) {
+  const float4 a = rec[0], b = rec[1], c = rec[2];
+  const f3 v0 = mk(a.x, a.y, a.z), e1 = mk(a.w, b.x, b.y), e2 = mk(b.z, b.w, c.x);
+  const f3 pvec = cross(ray, e2);
+  const float det = dot(e1, pvec);
+  if (fabsf(det) < kRayEps) return false;
+  const float inv_det = 1.0f / det;
+  const f3 tvec = sub(o, v0);
+  const float u = dot(tvec, pvec) * inv_det;
+  if (u < 0.0f || u > 1.0f) return false;
+  const f3 qvec = cross(tvec, e1);
+  const float v = dot(ray, qvec) * inv_det;
+  if (v < 0.0f || (u + v) > 1.0f) return false;
+  const float t = dot(e2, qvec) * inv_det;
+  if (t <= kRayEps) return false;
+  ot = t; ou = u; ov = v;
+  return true;
+}
+
+struct Hit { int hit, tri; float t, u, v; };
+
+// Nearest hit (trace_nearest, device_ray.hpp:136-158).  `stack` points at
+// this lane's first LDS slot, consecutive slots `stride` ints apart.
+__device__ inline Hit trace_nearest(const DevScene &s, f3 origin, f3 ray, int *stack, int stride) {
+  Hit h{0, -1, 0.f, 0.f, 0.f};
+  float closest = 1e30f;
+  const f3 idir = inverse_direction(ray);
+  int ref = s.root_ref, sp = 0;
+  for (;;) {
+    if (ref < 0) {
+      const int2 leaf = s.leaves[~ref];
+      for (int i = 0; i < leaf.y; ++i) {
+        const int id = leaf.x + i;
+        float t, u, v;
+        if (intersect(s.triangles + 3 * (size_t)id, ray, origin, t, u, v) && t < closest) {
+          closest = t; h.hit = 1; h.tri = id; h.t = t; h.u = u; h.v = v;
+        }
+      }
+    } else {
+      float d0, d1; int l, r;
+      const int mask = test_pair(s.pairs + 4 * (size_t)ref, origin, idir, kRayEps, closest, d0, d1, l, r);
+      if (mask == 3) {
+        if (sp >= s.stack_depth) { h.hit = -1; return h; }
+        const bool left_first = d0 < d1;
+        stack[sp * stride] = left_first ? r : l;
+        ++sp;
+        ref = left_first ? l : r;
+        continue;
+      }
+      if (mask) { ref = mask == 1 ? l : r; continue; }
+    }
+    if (sp == 0) break;
+    --sp;
+    ref = stack[sp * stride];
+  }
+  return h;
+}
+// Any hit toward the light skipping the origin's own triangle
+// (trace_shadow_ray_any_hit, device_ray.hpp:81-135).  Overflow counts as
+// occluded, as in the reference.
+__device__ inline bool trace_any(const DevScene &s, f3 origin, f3 dir, int self, float t_min,
+                                 float t_max, int *stack, int stride) {
+  const f3 idir = inverse_direction(dir);
+  int ref = s.root_ref, sp = 0;
+  for (;;) {
+    if (ref < 0) {
+      const int2 leaf = s.leaves[~ref];
+      for (int i = 0; i < leaf.y; ++i) {
+        const int id = leaf.x + i;
+        if (id == self) continue;
+        float t, u, v;
+        if (intersect(s.triangles + 3 * (size_t)id, dir, origin, t, u, v) && t > t_min && t < t_max) return true;
+      }
+    } else {
+      float d0, d1; int l, r;
+      const int mask = test_pair(s.pairs + 4 * (size_t)ref, origin, idir, t_min, t_max, d0, d1, l, r);
+      if (mask == 3) {
+        if (sp >= s.stack_depth) return true;
+        const bool left_first = d0 < d1;
+        stack[sp * stride] = left_first ? r : l;
+        ++sp;
+        ref = left_first ? l : r;
+        continue;
+      }
+      if (mask) { ref = mask == 1 ? l : r; continue; }
+    }
+    if (sp == 0) break;
+    --sp;
+    ref = stack[sp * stride];
+  }
+  return false;
+}
+
+// shadow_origin (device_optics.cuh:37-51)
+__device__ inline f3 shadow_origin(const float4 *rec, float u, float v, f3 light) {
+  const float4 ra = rec[0], rb = rec[1], rc = rec[2];
+  const f3 a = mk(ra.x, ra.y, ra.z), e1 = mk(ra.w, rb.x, rb.y), e2 = mk(rb.z, rb.w, rc.x);
+  f3 point = add(add(a, scale(e1, u)), scale(e2, v));
+  f3 normal = normalize(cross(e1, e2));
+  if (dot(normal, light) < 0) normal = neg(normal);
+  constexpr float epsilon = 5.960464477539063e-8f, gamma = 8 * epsilon / (1 - 8 * epsilon);
+  const f3 err = mk(gamma * (fabsf(a.x) + fabsf(u * e1.x) + fabsf(v * e2.x)),
+                    gamma * (fabsf(a.y) + fabsf(u * e1.y) + fabsf(v * e2.y)),
+                    gamma * (fabsf(a.z) + fabsf(u * e1.z) + fabsf(v * e2.z)));
+  const float distance = fabsf(normal.x) * err.x + fabsf(normal.y) * err.y + fabsf(normal.z) * err.z + 1e-6f;
+  const f3 offset = scale(normal, distance);
+  point = add(point, offset);
+  if (offset.x != 0) point.x = nextafterf(point.x, copysignf(INFINITY, offset.x));
+  if (offset.y != 0) point.y = nextafterf(point.y, copysignf(INFINITY, offset.y));
+  if (offset.z != 0) point.z = nextafterf(point.z, copysignf(INFINITY, offset.z));
+  return point;
+}
+
+// ---------------------------------------------------------------------------
+// Hapke BRDF x cos(i) (include/dvs/hapke.hpp:14-157), float.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float hk_safe_tan(float x) { return tanf(clampf(x, -kPi / 2.0f + 1e-6f, kPi / 2.0f - 1e-6f)); }
+__device__ __forceinline__ float hk_safe_acos(float x) { return acosf(clampf(x, -1.0f, 1.0f)); }
+__device__ __forceinline__ float hk_phase(float cosg, float b, float c) {
+  const float omb = 1.0f - b * b;
+  const float t1 = ((1.0f + c) * 0.5f) * omb / powf(1.0f - 2.0f * b * cosg + b * b, 1.5f);
+  const float t2 = ((1.0f - c) * 0.5f) * omb / powf(1.0f + 2.0f * b * cosg + b * b, 1.5f);
+  return t1 + t2;
+}
+__device__ __forceinline__ float hk_shoe(float tg, float h) {
+  if (h <= 0.0f) return tg <= 0.0f ? 1.0f : 0.0f;
+  return 1.0f / (1.0f + tg / h);
+}
+__device__ __forceinline__ float hk_cboe(float tg, float h) {
+  if (h <= 0.0f) return tg <= 0.0f ? 1.0f : 0.0f;
+  const float T = tg / h;
+  if (T < 1e-8f) return 1.0f;
+  return (1.0f + (1.0f - expf(-T)) / T) / (2.0f * (1.0f + T) * (1.0f + T));
+}
+__device__ __forceinline__ float hk_H(float x, float w) {
+  if (x <= 1e-8f) return 1.0f;
+  const float root = sqrtf(fmaxf(0.0f, 1.0f - w));
+  const float r0 = (1.0f - root) / (1.0f + root);
+  const float term = r0 + (1.0f - 2.0f * r0 * x) * 0.5f * logf(1.0f + 1.0f / x);
+  return 1.0f / (1.0f - w * x * term);
+}
+__device__ __forceinline__ float hk_porosity(float phi) {
+  if (phi <= 1e-8f) return 1.0f;
+  const float p = 1.209f * powf(phi, 2.0f / 3.0f);
+  return -logf(1.0f - p) / p;
+}
+// The roughness helpers take tan(thetap) and chi(thetap) precomputed; the
+// reference recomputes the same float expressions (tanf of the same
+// argument), so the values are identical.
+__device__ __forceinline__ float hk_E1(float cot_tp, float cy, float sy) {
+  return expf((-2.0f / kPi) * cot_tp * cy / fmaxf(sy, 1e-6f));
+}
+__device__ __forceinline__ float hk_E2(float cot_tp, float cy, float sy) {
+  const float v = cot_tp * cy / fmaxf(sy, 1e-6f);
+  return expf((-1.0f / kPi) * v * v);
+}
+__device__ __forceinline__ float hk_eta(float ch, float tan_tp, float cot_tp, float cy, float sy) {
+  const float e1 = hk_E1(cot_tp, cy, sy), e2 = hk_E2(cot_tp, cy, sy);
+  return ch * (cy + sy * tan_tp * e2 / (2.0f - e1));
+}
+__device__ inline float hapke_reflectance(float cosi, float cose, float cosg, float albedo, const dvs_hapke &p) {
+  if (cosi <= 0 || cose <= 0) return 0;
+  cosi = clampf(cosi, 0, 1); cose = clampf(cose, 0, 1); cosg = clampf(cosg, -1, 1);
+  const float sini = sqrtf(fmaxf(0, 1 - cosi * cosi)), sine = sqrtf(fmaxf(0, 1 - cose * cose));
+  const float tg = hk_safe_tan(hk_safe_acos(cosg) * 0.5f);
+  const float psi = hk_safe_acos((cosg - cosi * cose) / (sini * sine + 1e-6f));
+  const float pg = hk_phase(cosg, p.b, p.c);
+  const float BS = hk_shoe(tg, p.hS), BC = hk_cboe(tg, p.hC), K = hk_porosity(p.phi);
+  float mu0, mue, S;
+  const float tp = p.thetap;
+  if (tp <= 1e-8f) {
+    mu0 = cosi; mue = cose; S = 1.0f;
+  } else {
+    const float tan_tp = tanf(tp);
+    const float cot_tp = 1.0f / tan_tp;
+    const float ch = 1.0f / sqrtf(1.0f + kPi * tan_tp * tan_tp);
+    const float sh = sinf(psi * 0.5f), shv = sh * sh;
+    const float cpsi = cosf(psi);
+    const float e1i = hk_E1(cot_tp, cosi, sini), e1e = hk_E1(cot_tp, cose, sine);
+    const float e2i = hk_E2(cot_tp, cosi, sini), e2e = hk_E2(cot_tp, cose, sine);
+    const float eta_i = hk_eta(ch, tan_tp, cot_tp, cosi, sini);
+    const float eta_e = hk_eta(ch, tan_tp, cot_tp, cose, sine);
+    const float fpsi = expf(-2.0f * tanf(psi * 0.5f));
+    if (cosi >= cose) {
+      const float den = 2.0f - e1e - (psi / kPi) * e1i;
+      mu0 = ch * (cosi + sini * tan_tp * (cpsi * e2e + shv * e2i) / den);
+      mue = ch * (cose + sine * tan_tp * (e2e - shv * e2i) / den);
+      S = mue / eta_e * cosi / eta_i * ch / (1.0f - fpsi + fpsi * ch * (cosi / eta_i));
+    } else {
+      const float den = 2.0f - e1i - (psi / kPi) * e1e;
+      mu0 = ch * (cosi + sini * tan_tp * (e2i - shv * e2e) / den);
+      mue = ch * (cose + sine * tan_tp * (cpsi * e2i + shv * e2e) / den);
+      S = mue / eta_e * cosi / eta_i * ch / (1.0f - fpsi + fpsi * ch * (cose / eta_e));
+    }
+  }
+  const float M = hk_H(mu0 / K, albedo) * hk_H(mue / K, albedo) - 1.0f;
+  const float LS = mu0 / (mu0 + mue + 1e-6f);
+  return (albedo / (4.0f * kPi)) * LS * K * (pg * (1.0f + BS * p.B_S0) + M) * (1.0f + BC * p.B_C0) * S;
+}
+__device__ __forceinline__ float hapke_directional(f3 normal, f3 light, f3 view, float albedo, const dvs_hapke &p) {
+  const f3 n = normalize(normal), l = normalize(light), v = normalize(view);
+  return hapke_reflectance(dot(n, l), dot(n, v), dot(l, v), albedo, p);
+}
+__device__ __forceinline__ float material_albedo(float value, float fallback) {
+  return isfinite(value) && value >= 0 && value <= 1 ? value : fallback;
+}
+
+// ---------------------------------------------------------------------------
+// One sub-ray of one pixel sample (sample_subray / the body of sample_pixel,
+// device_optics.cuh:54-94).  SHADE=false traces the primary ray only (depth).
+// ---------------------------------------------------------------------------
+struct SubSample {
+  double radiance;
+  float depth;   // hit distance, -1 on a miss
+  int invalid;   // traversal overflow
+  int shadow;    // a shadow ray was traced
+};
+template <bool SHADE>
+__device__ inline SubSample sample_subray(const DevScene &s, int pixel, const Pose &pose, int sx, int sy,
+                                          int *stack, int stride) {
+  SubSample out{0.0, -1.f, 0, 0};
+  const int ss = s.samples;
+  const float px = (pixel % s.width + (sx + .5f) / ss - .5f - s.cx) / s.fx;
+  const float py = (pixel / s.width + (sy + .5f) / ss - .5f - s.cy) / s.fy;
+  const f3 ray = normalize(rotate_ray(pose, mk(px, -py, -1.f)));
+  const f3 origin = mk(pose.px, pose.py, pose.pz);
+  const Hit hit = trace_nearest(s, origin, ray, stack, stride);
+  if (hit.hit < 0) { out.invalid = 1; return out; }
+  if (!hit.hit) return out;
+  out.depth = hit.t;
+  if (!SHADE) return out;
+  const uint32_t i0 = s.indices[3 * (size_t)hit.tri], i1 = s.indices[3 * (size_t)hit.tri + 1],
+                 i2 = s.indices[3 * (size_t)hit.tri + 2];
+  const float w = 1 - hit.u - hit.v;
+  const f3 n0 = mk(s.normals[3 * (size_t)i0], s.normals[3 * (size_t)i0 + 1], s.normals[3 * (size_t)i0 + 2]);
+  const f3 n1 = mk(s.normals[3 * (size_t)i1], s.normals[3 * (size_t)i1 + 1], s.normals[3 * (size_t)i1 + 2]);
+  const f3 n2 = mk(s.normals[3 * (size_t)i2], s.normals[3 * (size_t)i2 + 1], s.normals[3 * (size_t)i2 + 2]);
+  const f3 normal = normalize(add(add(scale(n0, w), scale(n1, hit.u)), scale(n2, hit.v)));
+  const f3 light = normalize(mk(pose.lx, pose.ly, pose.lz));
+  if (dot(normal, light) <= 0 || dot(normal, neg(ray)) <= 0) return out;
+  const float4 *rec = s.triangles + 3 * (size_t)hit.tri;
+  const f3 so = shadow_origin(rec, hit.u, hit.v, light);
+  out.shadow = 1;
+  if (trace_any(s, so, light, hit.tri, kRayEps, 1e30f, stack, stride)) return out;
+  const float albedo = s.albedos ? material_albedo(s.albedos[i0], s.hapke.w) * w +
+                                       material_albedo(s.albedos[i1], s.hapke.w) * hit.u +
+                                       material_albedo(s.albedos[i2], s.hapke.w) * hit.v
+                                 : s.hapke.w;
+  const double a = 1.0 + (double)(px * px) + (double)(py * py);
+  const double cos4 = 1.0 / (a * a);
+  out.radiance = (double)fmaxf(0.f, hapke_directional(normal, light, neg(ray), albedo, s.hapke) * pose.intensity) * cos4;
+  return out;
+}
+
+}  // namespace dvsdev

@@ -1,0 +1,358 @@
+// gfx950 kernels of the DVS rendering path.
+//
+//  trace_kernel<MODE>  one lane per sub-ray; 256-lane blocks hold 256/ss^2
+//                      pixel samples; per-lane traversal stack in LDS; ordered
+//                      FP64 reduction of each sample's sub-rays in LDS.
+//                      MODE_CELLS   pixel x circuit-cell radiance (sensor),
+//                      MODE_FRAME   frame radiance + depth,
+//                      MODE_DEPTH   event-time depth (primary rays only).
+//  circuit_kernel      one lane per pixel; FP64 receptor integration and
+//                      comparator crossings over a chunk of cells; Philox
+//                      shot noise; crossing times into per-pixel slots.
+//  gather / pose / permute kernels for device event compaction.
+#include <hip/hip_runtime.h>
+
+#include "device_circuit.hpp"
+#include "device_optics.hpp"
+#include "kernels.h"
+
+namespace dvsdev {
+
+// optical_state_at at each cell midpoint (device_optics.cuh:13-26 as called
+// by generate_pixels, src/cuda/event_sensor.cu:41).
+__global__ void cell_pose_kernel(int cells, const double *t0, const double *t1, const int *iv_index,
+                                 const dvs_optical_interval *ivs, Pose *poses) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cells) return;
+  poses[c] = pose_at(ivs[iv_index[c]], (t0[c] + t1[c]) * .5);
+}
+
+__global__ void state_pose_kernel(int n, const dvs_optical_state *states, Pose *poses) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) poses[i] = pose_of(states[i]);
+}
+
+// Interval of an event: the first interval whose end is not before the event
+// (the forward scan of generate_pixels, event_sensor.cu:54), then the pose at
+// the event time.
+__global__ void event_pose_kernel(int events, const double *times, const dvs_optical_interval *ivs,
+                                  int iv_count, Pose *poses) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= events) return;
+  const double t = times[e];
+  int lo = 0, hi = iv_count - 1;
+  while (lo < hi) {
+    const int m = (lo + hi) >> 1;
+    if (t > ivs[m].t1) lo = m + 1; else hi = m;
+  }
+  poses[e] = pose_at(ivs[lo], t);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void trace_kernel(DevScene s, const Pose *poses, int cells, int npix,
+                                                    long long items, const int *item_pixel, double *rad,
+                                                    float *img, float *dep, int *status,
+                                                    unsigned long long *shadow_count) {
+  extern __shared__ double lds_words[];
+  double *red = lds_words;
+  float *dred = reinterpret_cast<float *>(lds_words + 256);
+  int *stack = reinterpret_cast<int *>(dred + 256) + threadIdx.x;
+  const int R = s.samples * s.samples, slots = 256 / R;
+  const int slot = threadIdx.x / R, sub = threadIdx.x - slot * R;
+  const long long q = (long long)blockIdx.x * slots + slot;
+  const bool active = slot < slots && q < items;
+  SubSample r{0.0, -1.f, 0, 0};
+  long long out_index = 0;
+  if (active) {
+    int pixel, pose_index;
+    if (MODE == MODE_CELLS) {
+      pixel = (int)(q / cells);
+      pose_index = (int)(q - (long long)pixel * cells);
+      out_index = (long long)pose_index * npix + pixel;
+    } else if (MODE == MODE_FRAME) {
+      pose_index = (int)(q / npix);
+      pixel = (int)(q - (long long)pose_index * npix);
+      out_index = q;
+    } else {
+      pixel = item_pixel[q];
+      pose_index = (int)q;
+      out_index = q;
+    }
+    const Pose pose = poses[pose_index];
+    r = sample_subray<MODE != MODE_DEPTH>(s, pixel, pose, sub % s.samples, sub / s.samples, stack, 256);
+  }
+  red[threadIdx.x] = r.radiance;
+  dred[threadIdx.x] = r.depth;
+  const unsigned long long shadows = __ballot(r.shadow);
+  const unsigned long long invalid = __ballot(r.invalid);
+  if ((threadIdx.x & 63) == 0) {
+    if (shadows) atomicAdd(shadow_count, (unsigned long long)__popcll(shadows));
+    if (invalid) atomicOr(status, 2);
+  }
+  __syncthreads();
+  if (!active || sub != 0) return;
+  // Sequential sum in sub-ray order j = sy*ss + sx (sample_pixel,
+  // device_optics.cuh:75-93; sample_warps, optical_sampler.cu:19-21).
+  double total = 0, ranges = 0;
+  int hits = 0;
+  for (int j = 0; j < R; ++j) {
+    total += red[threadIdx.x + j];
+    const float d = dred[threadIdx.x + j];
+    if (d >= 0) { ranges += d; ++hits; }
+  }
+  const double mean = total / R;
+  const float depth = hits ? (float)(ranges / hits) : -1.f;
+  if (MODE == MODE_CELLS) {
+    rad[out_index] = mean;
+  } else if (MODE == MODE_FRAME) {
+    if (rad) rad[out_index] = mean;
+    if (img) img[out_index] = (float)mean;
+    if (dep) dep[out_index] = depth;
+  } else {
+    dep[out_index] = depth;
+  }
+}
+
+// Settled receptor and mismatch thresholds (initialize_pixels,
+// src/cuda/event_sensor.cu:18-28).
+__global__ void init_kernel(int npix, const double *radiance, PhotoParams photo, CircuitParams p,
+                            double threshold_std, double threshold_min, uint64_t seed,
+                            dvs_pixel_state *states, int *status) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npix) return;
+  const double r = radiance[i];
+  if (!isfinite(r)) { atomicOr(status, 2); return; }
+  dvs_pixel_state s;
+  s.receptor_log = s.reference_log = log(fmax(1e-30, r * photo.electrons_per_radiance + photo.dark_electrons));
+  s.refractory_until = -1;
+  double zp = 0, zn = 0;
+  if (threshold_std != 0) {
+    NoiseStream rs(seed, (uint64_t)i, kStreamThreshold, 0);
+    zp = rs.normal();
+    zn = rs.normal();
+  }
+  s.threshold_pos = fmax(threshold_min, p.threshold_pos + threshold_std * zp);
+  s.threshold_neg = fmax(threshold_min, p.threshold_neg + threshold_std * zn);
+  s.reset_pending = 0;
+  s.reset_block = 0;
+  s.shot_block = 0;
+  states[i] = s;
+}
+
+// Per-pixel circuit over cells [c_begin, c_begin + c_count) of one process
+// call (the circuit half of generate_pixels, event_sensor.cu:39-49).
+__global__ __launch_bounds__(256) void circuit_kernel(int npix, int c_begin, int c_count, const double *cell_t0,
+                                                      const double *cell_t1, const double *rad,
+                                                      dvs_pixel_state *states, CircuitParams p, PhotoParams photo,
+                                                      uint64_t seed, int max_events, CrossingEvent *slots,
+                                                      int *counts, int *status) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npix) return;
+  int count = counts[i];
+  if (count < 0) return;
+  dvs_pixel_state st = states[i];
+  PixelRegs regs{st.receptor_log, st.reference_log, st.refractory_until, st.threshold_pos,
+                 st.threshold_neg, st.reset_pending};
+  NoiseStream shot(seed, (uint64_t)i, kStreamShot, st.shot_block);
+  NoiseStream reset(seed, (uint64_t)i, kStreamReset, st.reset_block);
+  auto normal = [&]() { return reset.normal(); };
+  CrossingEvent *out = slots + (size_t)i * max_events;
+  for (int c = 0; c < c_count; ++c) {
+    const double t0 = cell_t0[c_begin + c], t1 = cell_t1[c_begin + c];
+    const double radiance = rad[(size_t)c * npix + i];
+    if (!isfinite(radiance)) { atomicOr(status, 2); counts[i] = -1; return; }
+    double rate = radiance * photo.electrons_per_radiance + photo.dark_electrons;
+    if (photo.shot_noise) rate = shot.poisson(rate * (t1 - t0)) / (t1 - t0);
+    const int emitted = advance_pixel(regs, p, rate, t0, t1, out + count, max_events - count, normal);
+    if (emitted < 0) { atomicOr(status, 1); counts[i] = -1; return; }
+    count += emitted;
+  }
+  st.receptor_log = regs.receptor_log;
+  st.reference_log = regs.reference_log;
+  st.refractory_until = regs.refractory_until;
+  st.threshold_pos = regs.threshold_pos;
+  st.threshold_neg = regs.threshold_neg;
+  st.reset_pending = regs.reset_pending;
+  st.shot_block = shot.block;
+  st.reset_block = (uint32_t)reset.block;
+  states[i] = st;
+  counts[i] = count;
+}
+
+// Compaction: pixel i's crossings go to [ends[i] - counts[i], ends[i]).
+__global__ void gather_kernel(int npix, const int *ends, const int *counts, int max_events,
+                              const CrossingEvent *slots, double *time, int *pixel, int *polarity,
+                              unsigned long long *keys, unsigned *order) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npix) return;
+  const int n = counts[i], base = ends[i] - n;
+  for (int j = 0; j < n; ++j) {
+    const CrossingEvent e = slots[(size_t)i * max_events + j];
+    const int k = base + j;
+    time[k] = e.time;
+    pixel[k] = i;
+    polarity[k] = e.polarity;
+    // Order-preserving key of an IEEE double.
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(e.time);
+    keys[k] = (bits >> 63) ? ~bits : (bits | 0x8000000000000000ULL);
+    order[k] = (unsigned)k;
+  }
+}
+
+__global__ void permute_kernel(int events, int width, const unsigned *order, const double *time, const int *pixel,
+                               const int *polarity, const float *depth, dvs_event *out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= events) return;
+  const unsigned j = order[k];
+  dvs_event e;
+  e.timestamp = time[j];
+  e.x = (int16_t)(pixel[j] % width);
+  e.y = (int16_t)(pixel[j] / width);
+  e.polarity = (int8_t)polarity[j];
+  e.depth = depth[j];
+  out[k] = e;
+}
+
+// ---- unit kernels -------------------------------------------------------
+__global__ void hapke_kernel(int64_t n, const float *ci, const float *ce, const float *cg, const float *alb,
+                             dvs_hapke h, float *out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = hapke_reflectance(ci[i], ce[i], cg[i], alb[i], h);
+}
+__global__ void receptor_kernel(int64_t n, const double *u, const double *rate, const double *dt, double charge,
+                                double *out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = receptor_log_at(u[i], rate[i], dt[i], charge);
+}
+__global__ void noise_kernel(uint64_t seed, int64_t pixels, double lambda, int draws, double *poisson,
+                             double *normal) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= pixels) return;
+  NoiseStream shot(seed, (uint64_t)i, kStreamShot, 0), reset(seed, (uint64_t)i, kStreamReset, 0);
+  for (int d = 0; d < draws; ++d) {
+    if (poisson) poisson[i * draws + d] = shot.poisson(lambda);
+    if (normal) normal[i * draws + d] = reset.normal();
+  }
+}
+__global__ __launch_bounds__(256) void rays_kernel(DevScene s, int64_t n, const float *o, const float *d,
+                                                   const int *self, int shadow, int *hit, int *tri, float *t,
+                                                   float *u, float *v) {
+  extern __shared__ double lds_words[];
+  int *stack = reinterpret_cast<int *>(lds_words) + threadIdx.x;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const f3 origin = mk(o[3 * i], o[3 * i + 1], o[3 * i + 2]);
+  const f3 dir = mk(d[3 * i], d[3 * i + 1], d[3 * i + 2]);
+  if (shadow) {
+    hit[i] = trace_any(s, origin, dir, self[i], kRayEps, 1e30f, stack, 256) ? 1 : 0;
+    return;
+  }
+  const Hit h = trace_nearest(s, origin, dir, stack, 256);
+  hit[i] = h.hit; tri[i] = h.tri; t[i] = h.t; u[i] = h.u; v[i] = h.v;
+}
+
+}  // namespace dvsdev
+
+// ---- launchers (host side of this translation unit) ----------------------
+namespace dvsamd {
+using namespace dvsdev;
+
+static size_t trace_lds_bytes(const DevScene &s) {
+  return 256 * (sizeof(double) + sizeof(float)) + (size_t)s.stack_depth * 256 * sizeof(int);
+}
+
+hipError_t launch_cell_poses(int cells, const double *t0, const double *t1, const int *iv_index,
+                             const dvs_optical_interval *ivs, Pose *poses, hipStream_t st) {
+  if (cells <= 0) return hipSuccess;
+  cell_pose_kernel<<<(cells + 255) / 256, 256, 0, st>>>(cells, t0, t1, iv_index, ivs, poses);
+  return hipGetLastError();
+}
+hipError_t launch_state_poses(int n, const dvs_optical_state *states, Pose *poses, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  state_pose_kernel<<<(n + 255) / 256, 256, 0, st>>>(n, states, poses);
+  return hipGetLastError();
+}
+hipError_t launch_event_poses(int events, const double *times, const dvs_optical_interval *ivs, int iv_count,
+                              Pose *poses, hipStream_t st) {
+  if (events <= 0) return hipSuccess;
+  event_pose_kernel<<<(events + 255) / 256, 256, 0, st>>>(events, times, ivs, iv_count, poses);
+  return hipGetLastError();
+}
+hipError_t launch_trace(int mode, const DevScene &s, const Pose *poses, int cells, int npix, long long items,
+                        const int *item_pixel, double *rad, float *img, float *dep, int *status,
+                        unsigned long long *shadow_count, hipStream_t st) {
+  if (items <= 0) return hipSuccess;
+  const int R = s.samples * s.samples, slots = 256 / R;
+  const long long blocks = (items + slots - 1) / slots;
+  if (blocks > 0x7fffffffLL) return hipErrorInvalidConfiguration;
+  const size_t lds = trace_lds_bytes(s);
+  switch (mode) {
+    case MODE_CELLS:
+      trace_kernel<MODE_CELLS><<<(unsigned)blocks, 256, lds, st>>>(s, poses, cells, npix, items, item_pixel, rad,
+                                                                 img, dep, status, shadow_count);
+      break;
+    case MODE_FRAME:
+      trace_kernel<MODE_FRAME><<<(unsigned)blocks, 256, lds, st>>>(s, poses, cells, npix, items, item_pixel, rad,
+                                                                 img, dep, status, shadow_count);
+      break;
+    default:
+      trace_kernel<MODE_DEPTH><<<(unsigned)blocks, 256, lds, st>>>(s, poses, cells, npix, items, item_pixel, rad,
+                                                                 img, dep, status, shadow_count);
+  }
+  return hipGetLastError();
+}
+hipError_t launch_init(int npix, const double *radiance, const PhotoParams &photo, const CircuitParams &p,
+                       double threshold_std, double threshold_min, uint64_t seed, dvs_pixel_state *states,
+                       int *status, hipStream_t st) {
+  init_kernel<<<(npix + 255) / 256, 256, 0, st>>>(npix, radiance, photo, p, threshold_std, threshold_min, seed,
+                                                  states, status);
+  return hipGetLastError();
+}
+hipError_t launch_circuit(int npix, int c_begin, int c_count, const double *t0, const double *t1,
+                          const double *rad, dvs_pixel_state *states, const CircuitParams &p,
+                          const PhotoParams &photo, uint64_t seed, int max_events, CrossingEvent *slots,
+                          int *counts, int *status, hipStream_t st) {
+  circuit_kernel<<<(npix + 255) / 256, 256, 0, st>>>(npix, c_begin, c_count, t0, t1, rad, states, p, photo, seed,
+                                                     max_events, slots, counts, status);
+  return hipGetLastError();
+}
+hipError_t launch_gather(int npix, const int *ends, const int *counts, int max_events, const CrossingEvent *slots,
+                         double *time, int *pixel, int *polarity, unsigned long long *keys, unsigned *order,
+                         hipStream_t st) {
+  gather_kernel<<<(npix + 255) / 256, 256, 0, st>>>(npix, ends, counts, max_events, slots, time, pixel, polarity,
+                                                    keys, order);
+  return hipGetLastError();
+}
+hipError_t launch_permute(int events, int width, const unsigned *order, const double *time, const int *pixel,
+                          const int *polarity, const float *depth, dvs_event *out, hipStream_t st) {
+  if (events <= 0) return hipSuccess;
+  permute_kernel<<<(events + 255) / 256, 256, 0, st>>>(events, width, order, time, pixel, polarity, depth, out);
+  return hipGetLastError();
+}
+hipError_t launch_hapke(int64_t n, const float *ci, const float *ce, const float *cg, const float *alb,
+                        const dvs_hapke &h, float *out, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hapke_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, ci, ce, cg, alb, h, out);
+  return hipGetLastError();
+}
+hipError_t launch_receptor(int64_t n, const double *u, const double *rate, const double *dt, double charge,
+                           double *out, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  receptor_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, u, rate, dt, charge, out);
+  return hipGetLastError();
+}
+hipError_t launch_noise(uint64_t seed, int64_t pixels, double lambda, int draws, double *poisson, double *normal,
+                        hipStream_t st) {
+  if (pixels <= 0) return hipSuccess;
+  noise_kernel<<<(unsigned)((pixels + 255) / 256), 256, 0, st>>>(seed, pixels, lambda, draws, poisson, normal);
+  return hipGetLastError();
+}
+hipError_t launch_rays(const DevScene &s, int64_t n, const float *o, const float *d, const int *self, int shadow,
+                       int *hit, int *tri, float *t, float *u, float *v, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  const size_t lds = (size_t)s.stack_depth * 256 * sizeof(int);
+  rays_kernel<<<(unsigned)((n + 255) / 256), 256, lds, st>>>(s, n, o, d, self, shadow, hit, tri, t, u, v);
+  return hipGetLastError();
+}
+
+}  // namespace dvsamd

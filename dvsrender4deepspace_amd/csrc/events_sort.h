@@ -1,0 +1,13 @@
+// Device scan / stable radix sort wrappers (compiled in events_sort.hip).
+#pragma once
+#include <cstddef>
+#include <hip/hip_runtime.h>
+
+namespace dvsamd {
+hipError_t inclusive_scan_int(void *temp, size_t &temp_bytes, const int *in, int *out, size_t n,
+                              hipStream_t stream);
+// Stable LSD radix sort of (key, value) pairs over bits [begin_bit, end_bit).
+hipError_t sort_pairs_u64(void *temp, size_t &temp_bytes, const unsigned long long *keys_in,
+                          unsigned long long *keys_out, const unsigned *values_in, unsigned *values_out,
+                          size_t n, int begin_bit, int end_bit, hipStream_t stream);
+}  // namespace dvsamd

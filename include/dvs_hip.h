@@ -1,0 +1,208 @@
+/*
+ * dvs_hip.h — C ABI of the MI355X (gfx950) DVS rendering library.
+ *
+ * Plain C types, plain pointers and sizes; no HIP or C++ types cross this
+ * boundary.  Every entry point returns an int status:
+ *   DVS_OK 0, DVS_E_CAPACITY 1 (pixel event capacity exceeded),
+ *   DVS_E_OPTICAL 2 (invalid optical sample / BVH traversal overflow),
+ *   DVS_E_ARGUMENT 3, DVS_E_HIP 4, DVS_E_STATE 5,
+ * and dvs_last_error() returns the message of the calling thread's last
+ * failure, worded as the reference's exceptions.  The C++ layer
+ * (dvs/raycast.hpp in this repository) rethrows them as the reference does.
+ *
+ * Each declaration names the reference interface it replaces
+ * (BugBubbles/DvsRender4DeepSpace, paths relative to its root).
+ */
+#ifndef DVS_HIP_H
+#define DVS_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  DVS_OK = 0,
+  DVS_E_CAPACITY = 1,
+  DVS_E_OPTICAL = 2,
+  DVS_E_ARGUMENT = 3,
+  DVS_E_HIP = 4,
+  DVS_E_STATE = 5
+};
+
+/* BVHNode<float3> (include/dvs/scene_types.hpp:29-35): 40 bytes, the node
+ * format BVHBuilder::build emits and DeviceScene consumes. */
+typedef struct {
+  float min[3], max[3];
+  int32_t left, right, start, count;
+} dvs_bvh_node;
+
+/* HapkeConfig (include/dvs/scene_types.hpp:89-99), same field order. */
+typedef struct {
+  float b, c, hS, hC, thetap, w, B_S0, B_C0, phi;
+} dvs_hapke;
+
+/* CameraConfig (scene_types.hpp:59-62) plus scene.supersample_factor. */
+typedef struct {
+  int32_t width, height;
+  float fx, fy, cx, cy;
+  int32_t samples;
+} dvs_camera;
+
+/* OpticalState (include/dvs/optical_sensor.hpp:15-19); rotation is the
+ * camera-to-world quaternion (x,y,z,w). */
+typedef struct {
+  float position[3], velocity[3], light[3], light_rate[3];
+  float rotation[4];
+  float intensity, intensity_rate;
+} dvs_optical_state;
+
+/* OpticalInterval (optical_sensor.hpp:20). */
+typedef struct {
+  dvs_optical_state start, end;
+  double t0, t1;
+} dvs_optical_interval;
+
+/* PhysicConfig (scene_types.hpp:64-79), same field order and defaults. */
+typedef struct {
+  float threshold_pos, threshold_neg, refractory_time;
+  float readout_bandwidth, readout_latency;
+  float threshold_std, threshold_min;
+  float wavelength_nm, quantum_efficiency, pixel_pitch_um;
+  float fill_factor, f_number, lens_transmission;
+  float radiance_scale;
+  float temperature_k, subthreshold_kappa;
+  float receptor_capacitance_f;
+  float reset_capacitance_f, dark_current_a;
+  float reset_leakage_a, circuit_step;
+  int32_t photon_shot_noise, reset_thermal_noise;
+  int32_t random_seed, max_events_per_step;
+  int32_t readout_queue_capacity;
+} dvs_physic;
+
+/* Event (include/dvs/dvs_base.hpp:9-18): packed, 17 bytes. */
+#pragma pack(push, 1)
+typedef struct {
+  double timestamp;
+  int16_t x, y;
+  int8_t polarity;
+  float depth;
+} dvs_event;
+#pragma pack(pop)
+
+/* PixelState (include/dvs/pixel_model.hpp:11-15) as held on the device,
+ * plus the per-pixel Philox block counters of the shot and reset streams. */
+typedef struct {
+  double receptor_log, reference_log, refractory_until;
+  double threshold_pos, threshold_neg;
+  int32_t reset_pending;
+  uint32_t reset_block;
+  uint64_t shot_block;
+} dvs_pixel_state;
+
+/* Device timings and work counters of one sensor or scene handle. */
+typedef struct {
+  double trace_ms, circuit_ms, events_ms, total_ms;
+  double trace_launches;
+  uint64_t primary_rays, shadow_rays, event_rays;
+  uint64_t pixel_cells, comparator_events, delivered_events;
+} dvs_stats;
+
+typedef struct dvs_scene_s *dvs_scene;
+typedef struct dvs_sensor_s *dvs_sensor;
+
+/* Message of the calling thread's last failure ("" if none). */
+const char *dvs_last_error(void);
+/* Number of visible HIP devices (0 without a GPU); never fails. */
+int dvs_device_count(void);
+
+/* ---- BVH ------------------------------------------------------------
+ * BVHBuilder::build(params, nodes, indices, vertices) (include/bvh/build.h:19-24,
+ * src/bvh/build.cpp:27-76): host SAH build; `indices` (3*nt uint32) is
+ * reordered into leaf order in place; nodes must hold 2*nt entries.
+ * Writes the node count and the root index (last node). */
+int dvs_bvh_build(int max_leaf_size, float sah_node_cost, float sah_primitive_cost,
+                  const float *vertices, int64_t vertex_count, uint32_t *indices,
+                  int64_t triangle_count, dvs_bvh_node *nodes, int64_t *node_count,
+                  int32_t *root);
+/* BVHBuilder::validate (src/bvh/build.cpp:188-242). */
+int dvs_bvh_validate(const dvs_bvh_node *nodes, int64_t node_count, int32_t root);
+
+/* ---- Scene -----------------------------------------------------------
+ * DeviceScene(const SceneBundle&, const Config&, int root)
+ * (include/dvs/device_scene.hpp:5-15, src/cuda/device_scene.cu:15-37):
+ * validates the indexed mesh and BVH and uploads it as the aligned SoA
+ * layout the gfx950 kernels traverse.  albedos may be NULL. */
+int dvs_scene_create(const dvs_bvh_node *nodes, int64_t node_count, int32_t root,
+                     const float *vertices, const float *normals, int64_t vertex_count,
+                     const uint32_t *indices, int64_t triangle_count, const float *albedos,
+                     const dvs_camera *camera, const dvs_hapke *hapke, dvs_scene *out);
+/* ~DeviceScene (device_scene.cu:38-43).  The scene is reference counted:
+ * sensors created on it keep it alive. */
+int dvs_scene_destroy(dvs_scene scene);
+/* Tree depth and SoA sizes of an uploaded scene (diagnostics). */
+int dvs_scene_info(dvs_scene scene, int32_t *max_depth, int64_t *internal_nodes,
+                   int64_t *leaves, int64_t *device_bytes);
+
+/* ---- Frame / depth ---------------------------------------------------
+ * OpticalSampler::evaluate + download (src/cuda/optical_sampler.cu:154-172) as
+ * used by CamRayCaster::render_main (src/cuda/cam_render.cu:16-28): renders
+ * `pose_count` poses; image/depth hold width*height*pose_count floats
+ * (host memory).  radiance (double, may be NULL) receives the unrounded mean. */
+int dvs_frame_render(dvs_scene scene, const dvs_optical_state *poses, int32_t pose_count,
+                     float *image, float *depth, double *radiance);
+/* Same on device buffers (no host copies); for timing. */
+int dvs_frame_render_device(dvs_scene scene, const dvs_optical_state *device_poses,
+                            int32_t pose_count, float *device_image, float *device_depth);
+int dvs_scene_stats(dvs_scene scene, dvs_stats *out);
+
+/* ---- Event sensor ----------------------------------------------------
+ * EventSensor(width, height, PhysicConfig) (src/cuda/event_sensor.cu:244-260).
+ * physic_effects / aer_delay are the reference's compile-time
+ * ENABLE_PHYSIC_EFFECTS / ENABLE_AER_DELAY switches as runtime flags. */
+int dvs_sensor_create(dvs_scene scene, const dvs_physic *physic, int32_t physic_effects,
+                      int32_t aer_delay, dvs_sensor *out);
+int dvs_sensor_destroy(dvs_sensor sensor);
+/* EventSensor::initialize (event_sensor.cu:262-270). */
+int dvs_sensor_initialize(dvs_sensor sensor, const dvs_optical_state *pose, double timestamp);
+/* EventSensor::process_many (event_sensor.cu:274-305): integrates the
+ * contiguous intervals, then runs the readout FIFO; the delivered events are
+ * held until dvs_sensor_take.  *ready_count receives their number. */
+int dvs_sensor_process(dvs_sensor sensor, const dvs_optical_interval *intervals,
+                       int32_t interval_count, int64_t *ready_count);
+/* Copies and clears the held delivered events (time-ordered). */
+int dvs_sensor_take(dvs_sensor sensor, dvs_event *out, int64_t capacity, int64_t *count);
+/* Comparator events of the last process call before the readout FIFO, in
+ * (t,y,x) order (tests and diagnostics). */
+int dvs_sensor_raw_events(dvs_sensor sensor, dvs_event *out, int64_t capacity, int64_t *count);
+/* EventSensor::drain (event_sensor.hpp:71): moves the FIFO backlog to the
+ * held delivered events. */
+int dvs_sensor_drain(dvs_sensor sensor, int64_t *ready_count);
+/* EventSensor::dropped / peak_queue (event_sensor.hpp:72-73). */
+int dvs_sensor_readout_stats(dvs_sensor sensor, int64_t *dropped, int64_t *peak_queue);
+/* Copies the device pixel states (width*height entries). */
+int dvs_sensor_states(dvs_sensor sensor, dvs_pixel_state *out);
+int dvs_sensor_stats(dvs_sensor sensor, dvs_stats *out);
+int dvs_sensor_reset_stats(dvs_sensor sensor);
+
+/* ---- Device kernels exposed for unit parity (no reference counterpart;
+ * they expose the reference's __host__ __device__ functions run on the GPU) */
+/* hapke_reflectance (include/dvs/hapke.hpp:131-151) over n inputs. */
+int dvs_hapke_eval(const float *cosi, const float *cose, const float *cosg, const float *albedo,
+                   int64_t n, const dvs_hapke *hapke, float *out);
+/* receptor_log_at (include/dvs/pixel_model.hpp:18-26) over n inputs. */
+int dvs_receptor_eval(const double *u, const double *rate, const double *dt, double charge,
+                      int64_t n, double *out);
+/* Poisson / normal draws of the device noise streams for (seed, pixel). */
+int dvs_noise_eval(uint64_t seed, int64_t pixels, double lambda, int32_t draws, double *poisson,
+                   double *normal);
+/* Nearest / shadow queries of n rays against a scene (traversal parity). */
+int dvs_trace_rays(dvs_scene scene, const float *origins, const float *directions, int64_t n,
+                   int32_t *hit, int32_t *triangle, float *t, float *u, float *v);
+int dvs_trace_shadow_rays(dvs_scene scene, const float *origins, const float *directions,
+                          const int32_t *self_triangle, int64_t n, int32_t *occluded);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

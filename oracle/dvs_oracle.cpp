@@ -1,0 +1,1066 @@
+// CPU oracle for the DVS rendering hot path — TEST INFRASTRUCTURE ONLY.
+// Restates the reference's device and host code operation for operation; the
+// reference file:line each function follows is cited beside it.  Loaded only
+// by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+// Build: oracle/Makefile (g++ -O2 -fopenmp -ffp-contract=off).
+#include "dvs_oracle.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace orc {
+
+// ---------------------------------------------------------------------------
+// float3 helpers: include/helper_math.hpp:14-77 (same operand order).
+// ---------------------------------------------------------------------------
+struct f3 { float x, y, z; };
+static inline f3 mk(float x, float y, float z) { return f3{x, y, z}; }
+static inline f3 operator+(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline f3 operator-(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline f3 operator-(f3 a) { return mk(-a.x, -a.y, -a.z); }
+static inline f3 operator*(float b, f3 a) { return mk(a.x * b, a.y * b, a.z * b); }
+static inline f3 operator*(f3 a, float b) { return mk(a.x * b, a.y * b, a.z * b); }
+static inline f3 operator/(f3 a, float b) { return mk(a.x / b, a.y / b, a.z / b); }
+static inline f3 operator/(float a, f3 b) { return mk(a / b.x, a / b.y, a / b.z); }
+static inline float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline f3 cross(f3 a, f3 b) {
+  return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static inline f3 normalize(f3 v) {
+  float len = sqrtf(dot(v, v));
+  if (len > 0.0f) return v / len;
+  return mk(0.f, 0.f, 0.f);
+}
+static inline f3 fmax3(f3 a, f3 b) { return mk(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)); }
+static inline f3 fmin3(f3 a, f3 b) { return mk(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)); }
+static inline f3 fmax3(f3 a, float b) { return mk(fmaxf(a.x, b), fmaxf(a.y, b), fmaxf(a.z, b)); }
+static inline float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+static inline f3 ld3(const float *p) { return mk(p[0], p[1], p[2]); }
+
+constexpr float kRayEps = 1e-6f;       // helper_math.hpp:11
+constexpr int kMaxStackSize = 128;     // helper_math.hpp:12
+constexpr float kPi = 3.14159265358979323846f;  // math_types.hpp
+
+// ---------------------------------------------------------------------------
+// BVH build: src/bvh/build.cpp:27-76,126-187 and src/bvh/bining.cpp:9-182.
+// ---------------------------------------------------------------------------
+struct Box { f3 min, max; };
+static inline Box box_empty() {  // common.h:11-16
+  return Box{mk(FLT_MAX, FLT_MAX, FLT_MAX), mk(-FLT_MAX, -FLT_MAX, -FLT_MAX)};
+}
+static inline bool box_is_empty(const Box &b) {  // common.h:18-21
+  return (b.min.x > b.max.x) | (b.min.y > b.max.y) | (b.min.z > b.max.z);
+}
+static inline void box_grow(Box &b, f3 p) {  // common.h:23-31
+  b.min.x = (p.x < b.min.x ? p.x : b.min.x);
+  b.min.y = (p.y < b.min.y ? p.y : b.min.y);
+  b.min.z = (p.z < b.min.z ? p.z : b.min.z);
+  b.max.x = (p.x > b.max.x ? p.x : b.max.x);
+  b.max.y = (p.y > b.max.y ? p.y : b.max.y);
+  b.max.z = (p.z > b.max.z ? p.z : b.max.z);
+}
+static inline void box_grow(Box &b, const Box &o) {  // common.h:33-53
+  if (box_is_empty(o)) return;
+  float xmin = std::min(o.min.x, o.max.x), ymin = std::min(o.min.y, o.max.y),
+        zmin = std::min(o.min.z, o.max.z);
+  float xmax = std::max(o.min.x, o.max.x), ymax = std::max(o.min.y, o.max.y),
+        zmax = std::max(o.min.z, o.max.z);
+  b.min.x = std::min(b.min.x, xmin); b.min.y = std::min(b.min.y, ymin);
+  b.min.z = std::min(b.min.z, zmin);
+  b.max.x = std::max(b.max.x, xmax); b.max.y = std::max(b.max.y, ymax);
+  b.max.z = std::max(b.max.z, zmax);
+}
+static inline f3 box_center(const Box &b) { return (b.min + b.max) * 0.5f; }  // common.h:55-57
+static inline float box_area(const Box &b) {  // common.h:59-62
+  f3 d = b.max - b.min;
+  return std::max(d.x * d.y + d.y * d.z + d.z * d.x, 1e-12f);
+}
+struct Ref { Box bounds; f3 center; int tri; };  // common.h:65-69
+static inline float comp(f3 v, int d) { return d == 0 ? v.x : (d == 1 ? v.y : v.z); }
+
+struct Binning {  // bining.cpp:9-108
+  float leafSAH = 0, splitSAH = 0;
+  int dim = 0, pos = 0, num_bins = 0;
+  f3 scale{};
+  Box cent{};
+  enum { NUM_BINS = 32 };
+  Binning(const Box &range_bounds, int start, int size, const Ref *refs) {
+    Box cb = box_empty();
+    for (int i = 0; i < size; ++i) box_grow(cb, box_center(refs[start + i].bounds));
+    cent = cb;
+    num_bins = (int)std::min((size_t)NUM_BINS, (size_t)(4.0f + 0.05f * size));
+    f3 extent = fmax3(cb.max - cb.min, 1e-12f);
+    scale = (float)num_bins / extent;
+    Box bb[3][NUM_BINS];
+    int bc[3][NUM_BINS];
+    for (int d = 0; d < 3; ++d)
+      for (int b = 0; b < NUM_BINS; ++b) { bb[d][b] = box_empty(); bc[d][b] = 0; }
+    for (int i = 0; i < size; ++i) {
+      const Ref &r = refs[start + i];
+      int bin[3];
+      for (int d = 0; d < 3; ++d) {
+        int idx = (int)((comp(r.center, d) - comp(cb.min, d)) * comp(scale, d));
+        bin[d] = std::clamp(idx, 0, num_bins - 1);
+      }
+      for (int d = 0; d < 3; ++d) { box_grow(bb[d][bin[d]], r.bounds); bc[d][bin[d]]++; }
+    }
+    leafSAH = box_area(range_bounds) * size;
+    float best = FLT_MAX; int bd = 0, bp = 0;
+    for (int d = 0; d < 3; ++d) {
+      Box rb[NUM_BINS]; int rc[NUM_BINS];
+      Box cur = box_empty(); int cnt = 0;
+      for (int b = num_bins - 1; b >= 0; --b) {
+        box_grow(cur, bb[d][b]); cnt += bc[d][b]; rb[b] = cur; rc[b] = cnt;
+      }
+      Box lb = box_empty(); int lc = 0;
+      for (int b = 1; b < num_bins; ++b) {
+        box_grow(lb, bb[d][b - 1]); lc += bc[d][b - 1];
+        float cost = box_area(lb) * lc + box_area(rb[b]) * rc[b];
+        if (cost < best) { best = cost; bd = d; bp = b; }
+      }
+    }
+    dim = bd; pos = bp; splitSAH = best;
+  }
+  // bining.cpp:110-182
+  void split(int start, int size, Ref *refs, int &lstart, int &lsize, int &rstart,
+             int &rsize) const {
+    int end = start + size, l = start, r = end - 1;
+    float split_val = comp(cent.min, dim) +
+                      (float(pos) / num_bins) * (comp(cent.max, dim) - comp(cent.min, dim));
+    while (l <= r) {
+      if (comp(refs[l].center, dim) < split_val) l++;
+      else { std::swap(refs[l], refs[r]); r--; }
+    }
+    int mid = l;
+    lstart = start; lsize = mid - start; rstart = mid; rsize = end - mid;
+    if (lsize == 0 || rsize == 0) {
+      int mid2 = start + (end - start) / 2;
+      lstart = start; lsize = mid2 - start; rstart = mid2; rsize = end - mid2;
+    }
+  }
+};
+
+struct BuildCtx {
+  int max_leaf; float node_cost, prim_cost;
+  std::vector<Ref> refs;
+  std::vector<orc_node> nodes;
+};
+static int make_leaf(BuildCtx &c, int start, int end, const Box &b) {  // build.cpp:176-187
+  orc_node n{};
+  n.min[0] = b.min.x; n.min[1] = b.min.y; n.min[2] = b.min.z;
+  n.max[0] = b.max.x; n.max[1] = b.max.y; n.max[2] = b.max.z;
+  n.left = -1; n.right = -1; n.start = start; n.count = end - start;
+  c.nodes.push_back(n);
+  return (int)c.nodes.size() - 1;
+}
+static int build_node(BuildCtx &c, int start, int end) {  // build.cpp:126-174
+  int size = end - start;
+  if (size <= 0) throw std::runtime_error("BVH build_node empty range");
+  Box rb = box_empty();
+  for (int i = start; i < end; ++i) box_grow(rb, c.refs[i].bounds);
+  if (size <= c.max_leaf) return make_leaf(c, start, end, rb);
+  Binning bin(rb, start, size, c.refs.data());
+  float leaf_cost = c.prim_cost * bin.leafSAH;
+  float split_cost = c.node_cost * box_area(rb) + c.prim_cost * bin.splitSAH;
+  if (leaf_cost <= split_cost) return make_leaf(c, start, end, rb);
+  int ls, lz, rs, rz;
+  bin.split(start, size, c.refs.data(), ls, lz, rs, rz);
+  if (lz <= 0 || rz <= 0) return make_leaf(c, start, end, rb);
+  int li = build_node(c, ls, ls + lz);
+  int ri = build_node(c, rs, rs + rz);
+  orc_node n{};
+  n.min[0] = rb.min.x; n.min[1] = rb.min.y; n.min[2] = rb.min.z;
+  n.max[0] = rb.max.x; n.max[1] = rb.max.y; n.max[2] = rb.max.z;
+  n.left = li; n.right = ri; n.start = 0; n.count = 0;
+  c.nodes.push_back(n);
+  return (int)c.nodes.size() - 1;
+}
+
+// ---------------------------------------------------------------------------
+// Scene + traversal: include/dvs/device_ray.hpp:6-158.
+// ---------------------------------------------------------------------------
+struct Scene {
+  std::vector<orc_node> nodes;
+  int root = 0;
+  std::vector<f3> vertices, normals;
+  std::vector<uint32_t> indices;  // 3 per triangle
+  std::vector<float> albedos;
+  bool has_albedo = false;
+  orc_camera cam{};
+  orc_hapke hapke{};
+};
+
+static inline f3 inverse_direction(f3 d) {  // device_ray.hpp:6-12
+  f3 inv;
+  inv.x = 1.0f / (fabsf(d.x) > 1e-20f ? d.x : copysignf(1e-20f, d.x));
+  inv.y = 1.0f / (fabsf(d.y) > 1e-20f ? d.y : copysignf(1e-20f, d.y));
+  inv.z = 1.0f / (fabsf(d.z) > 1e-20f ? d.z : copysignf(1e-20f, d.z));
+  return inv;
+}
+static inline int two_children(f3 P, f3 idir, float t_min, float t_max, const orc_node &L,
+                               const orc_node &R, float *dist) {  // device_ray.hpp:13-53
+  float t1x0 = (L.min[0] - P.x) * idir.x, t1x1 = (L.max[0] - P.x) * idir.x;
+  float t1y0 = (L.min[1] - P.y) * idir.y, t1y1 = (L.max[1] - P.y) * idir.y;
+  float t1z0 = (L.min[2] - P.z) * idir.z, t1z1 = (L.max[2] - P.z) * idir.z;
+  float t1min = fmaxf(fmaxf(fminf(t1x0, t1x1), fminf(t1y0, t1y1)), fminf(t1z0, t1z1));
+  float t1max = fminf(fminf(fmaxf(t1x0, t1x1), fmaxf(t1y0, t1y1)), fmaxf(t1z0, t1z1));
+  t1min = fmaxf(t_min, t1min); t1max = fminf(t_max, t1max);
+  float t2x0 = (R.min[0] - P.x) * idir.x, t2x1 = (R.max[0] - P.x) * idir.x;
+  float t2y0 = (R.min[1] - P.y) * idir.y, t2y1 = (R.max[1] - P.y) * idir.y;
+  float t2z0 = (R.min[2] - P.z) * idir.z, t2z1 = (R.max[2] - P.z) * idir.z;
+  float t2min = fmaxf(fmaxf(fminf(t2x0, t2x1), fminf(t2y0, t2y1)), fminf(t2z0, t2z1));
+  float t2max = fminf(fminf(fmaxf(t2x0, t2x1), fmaxf(t2y0, t2y1)), fmaxf(t2z0, t2z1));
+  t2min = fmaxf(t_min, t2min); t2max = fminf(t_max, t2max);
+  dist[0] = t1min; dist[1] = t2min;
+  int mask = 0;
+  if (t1max >= t1min) mask |= 1;
+  if (t2max >= t2min) mask |= 2;
+  return mask;
+}
+static inline bool intersect_triangle(f3 ray, f3 o, f3 v0, f3 v1, f3 v2, float *ot, float *ou,
+                                      float *ov) {  // device_ray.hpp:54-80
+  const f3 e1 = v1 - v0, e2 = v2 - v0;
+  const f3 pvec = cross(ray, e2);
+  const float det = dot(e1, pvec);
+  if (fabsf(det) < kRayEps) return false;
+  const float inv_det = 1.0f / det;
+  const f3 tvec = o - v0;
+  const float u = dot(tvec, pvec) * inv_det;
+  if (u < 0.0f || u > 1.0f) return false;
+  const f3 qvec = cross(tvec, e1);
+  const float v = dot(ray, qvec) * inv_det;
+  if (v < 0.0f || (u + v) > 1.0f) return false;
+  const float t = dot(e2, qvec) * inv_det;
+  if (t <= kRayEps) return false;
+  *ot = t; *ou = u; *ov = v;
+  return true;
+}
+static inline void tri_vertices(const Scene &s, int id, f3 &a, f3 &b, f3 &c) {
+  a = s.vertices[s.indices[3 * id]]; b = s.vertices[s.indices[3 * id + 1]];
+  c = s.vertices[s.indices[3 * id + 2]];
+}
+static bool shadow_any_hit(const Scene &s, f3 o, f3 dir, int self, float t_min,
+                           float t_max) {  // device_ray.hpp:81-135
+  f3 idir = inverse_direction(dir);
+  int stack[kMaxStackSize]; int sp = 0;
+  stack[sp++] = s.root;
+  while (sp > 0) {
+    const orc_node &node = s.nodes[stack[--sp]];
+    if (node.count > 0) {
+      for (int i = 0; i < node.count; ++i) {
+        int id = node.start + i;
+        if (id == self) continue;
+        float t, u, v; f3 a, b, c; tri_vertices(s, id, a, b, c);
+        if (intersect_triangle(dir, o, a, b, c, &t, &u, &v))
+          if (t > t_min && t < t_max) return true;
+      }
+      continue;
+    }
+    float dist[2];
+    int mask = two_children(o, idir, t_min, t_max, s.nodes[node.left], s.nodes[node.right], dist);
+    if (mask == 0) continue;
+    if (sp + (mask == 3 ? 2 : 1) > kMaxStackSize) return true;
+    if (mask == 3) {
+      if (dist[0] < dist[1]) { stack[sp++] = node.right; stack[sp++] = node.left; }
+      else { stack[sp++] = node.left; stack[sp++] = node.right; }
+    } else if (mask == 1) stack[sp++] = node.left;
+    else stack[sp++] = node.right;
+  }
+  return false;
+}
+struct Hit { int hit = 0, tri = -1; float t = 0, u = 0, v = 0; };
+static Hit trace_nearest(const Scene &s, f3 origin, f3 ray, int64_t *counts = nullptr) {
+  // device_ray.hpp:136-158
+  Hit hit; float closest = 1e30f;
+  f3 idir = inverse_direction(ray);
+  int stack[kMaxStackSize], sp = 0; stack[sp++] = s.root;
+  while (sp > 0) {
+    const orc_node &node = s.nodes[stack[--sp]];
+    if (node.count > 0) {
+      if (counts) { counts[1]++; counts[2] += node.count; }
+      for (int i = 0; i < node.count; ++i) {
+        int id = node.start + i; float t, u, v; f3 a, b, c; tri_vertices(s, id, a, b, c);
+        if (intersect_triangle(ray, origin, a, b, c, &t, &u, &v) && t < closest) {
+          closest = t; hit.hit = 1; hit.tri = id; hit.t = t; hit.u = u; hit.v = v;
+        }
+      }
+    } else {
+      if (counts) counts[0]++;
+      float dist[2];
+      int mask = two_children(origin, idir, kRayEps, closest, s.nodes[node.left],
+                              s.nodes[node.right], dist);
+      if (!mask) continue;
+      if (sp + (mask == 3 ? 2 : 1) > kMaxStackSize) { hit.hit = -1; return hit; }
+      if (mask == 3) {
+        bool left = dist[0] < dist[1];
+        stack[sp++] = left ? node.right : node.left;
+        stack[sp++] = left ? node.left : node.right;
+      } else stack[sp++] = mask == 1 ? node.left : node.right;
+    }
+  }
+  return hit;
+}
+
+// ---------------------------------------------------------------------------
+// Hapke: include/dvs/hapke.hpp:14-172.
+// ---------------------------------------------------------------------------
+static inline float safe_tan(float x) { return tanf(clampf(x, -kPi / 2.0f + 1e-6f, kPi / 2.0f - 1e-6f)); }
+static inline float safe_acos(float x) { return acosf(clampf(x, -1.0f, 1.0f)); }
+static inline float HG2(float cosg, float b, float c) {  // :22-29
+  const float omb = 1.0f - b * b;
+  const float t1 = ((1.0f + c) * 0.5f) * omb / powf(1.0f - 2.0f * b * cosg + b * b, 1.5f);
+  const float t2 = ((1.0f - c) * 0.5f) * omb / powf(1.0f + 2.0f * b * cosg + b * b, 1.5f);
+  return t1 + t2;
+}
+static inline float B_S(float tg, float hS) {  // :31-34
+  if (hS <= 0.0f) return tg <= 0.0f ? 1.0f : 0.0f;
+  return 1.0f / (1.0f + tg / hS);
+}
+static inline float B_C(float tg, float hC) {  // :36-43
+  if (hC <= 0.0f) return tg <= 0.0f ? 1.0f : 0.0f;
+  const float T = tg / hC;
+  if (T < 1e-8f) return 1.0f;
+  return (1.0f + (1.0f - expf(-T)) / T) / (2.0f * (1.0f + T) * (1.0f + T));
+}
+static inline float Hfunc(float x, float w) {  // :45-51
+  if (x <= 1e-8f) return 1.0f;
+  const float root = sqrtf(fmaxf(0.0f, 1.0f - w));
+  const float r0 = (1.0f - root) / (1.0f + root);
+  const float term = r0 + (1.0f - 2.0f * r0 * x) * 0.5f * logf(1.0f + 1.0f / x);
+  return 1.0f / (1.0f - w * x * term);
+}
+static inline float porosity_K(float phi) {  // :53-57
+  if (phi <= 1e-8f) return 1.0f;
+  const float p = 1.209f * powf(phi, 2.0f / 3.0f);
+  return -logf(1.0f - p) / p;
+}
+static inline float chi(float tp) { const float t = tanf(tp); return 1.0f / sqrtf(1.0f + kPi * t * t); }
+static inline float E1(float tp, float cy, float sy) {
+  const float ss = fmaxf(sy, 1e-6f);
+  return expf((-2.0f / kPi) * (1.0f / tanf(tp)) * cy / ss);
+}
+static inline float E2(float tp, float cy, float sy) {
+  const float ss = fmaxf(sy, 1e-6f);
+  const float v = (1.0f / tanf(tp)) * cy / ss;
+  return expf((-1.0f / kPi) * v * v);
+}
+static inline float eta(float tp, float cy, float sy) {
+  const float ch = chi(tp), e1 = E1(tp, cy, sy), e2 = E2(tp, cy, sy);
+  return ch * (cy + sy * tanf(tp) * e2 / (2.0f - e1));
+}
+static inline float f_psi(float psi) { return expf(-2.0f * tanf(psi * 0.5f)); }
+struct Rough { float mu0 = 0, mue = 0, S = 0; };
+static inline Rough rough_S(float cosi, float cose, float sini, float sine, float psi,
+                            float tp) {  // :86-129
+  if (tp <= 1e-8f) return Rough{cosi, cose, 1.0f};
+  const float ch = chi(tp), ttp = tanf(tp);
+  const float sh = sinf(psi * 0.5f), shv = sh * sh;
+  Rough o;
+  if (cosi >= cose) {
+    o.mu0 = ch * (cosi + sini * ttp * (cosf(psi) * E2(tp, cose, sine) + shv * E2(tp, cosi, sini)) /
+                             (2.0f - E1(tp, cose, sine) - (psi / kPi) * E1(tp, cosi, sini)));
+    o.mue = ch * (cose + sine * ttp * (E2(tp, cose, sine) - shv * E2(tp, cosi, sini)) /
+                             (2.0f - E1(tp, cose, sine) - (psi / kPi) * E1(tp, cosi, sini)));
+    o.S = o.mue / eta(tp, cose, sine) * cosi / eta(tp, cosi, sini) * ch /
+          (1.0f - f_psi(psi) + f_psi(psi) * ch * (cosi / eta(tp, cosi, sini)));
+  } else {
+    o.mu0 = ch * (cosi + sini * ttp * (E2(tp, cosi, sini) - shv * E2(tp, cose, sine)) /
+                             (2.0f - E1(tp, cosi, sini) - (psi / kPi) * E1(tp, cose, sine)));
+    o.mue = ch * (cose + sine * ttp * (cosf(psi) * E2(tp, cosi, sini) + shv * E2(tp, cose, sine)) /
+                             (2.0f - E1(tp, cosi, sini) - (psi / kPi) * E1(tp, cose, sine)));
+    o.S = o.mue / eta(tp, cose, sine) * cosi / eta(tp, cosi, sini) * ch /
+          (1.0f - f_psi(psi) + f_psi(psi) * ch * (cose / eta(tp, cose, sine)));
+  }
+  return o;
+}
+static float hapke_reflectance(float cosi, float cose, float cosg, float albedo,
+                               const orc_hapke &p) {  // :131-151
+  if (cosi <= 0 || cose <= 0) return 0;
+  cosi = clampf(cosi, 0, 1); cose = clampf(cose, 0, 1); cosg = clampf(cosg, -1, 1);
+  const float sini = sqrtf(fmaxf(0, 1 - cosi * cosi)), sine = sqrtf(fmaxf(0, 1 - cose * cose));
+  const float tg = safe_tan(safe_acos(cosg) * 0.5f);
+  const float psi = safe_acos((cosg - cosi * cose) / (sini * sine + 1e-6f));
+  const float pg = HG2(cosg, p.b, p.c);
+  const float BS = B_S(tg, p.hS), BC = B_C(tg, p.hC), K = porosity_K(p.phi);
+  const Rough r = rough_S(cosi, cose, sini, sine, psi, p.thetap);
+  const float M = Hfunc(r.mu0 / K, albedo) * Hfunc(r.mue / K, albedo) - 1.0f;
+  const float LS = r.mu0 / (r.mu0 + r.mue + 1e-6f);
+  return (albedo / (4.0f * kPi)) * LS * K * (pg * (1.0f + BS * p.B_S0) + M) *
+         (1.0f + BC * p.B_C0) * r.S;
+}
+static float hapke_directional(f3 normal, f3 light, f3 view, float albedo,
+                               const orc_hapke &p) {  // :153-157
+  f3 n = normalize(normal), l = normalize(light), v = normalize(view);
+  return hapke_reflectance(dot(n, l), dot(n, v), dot(l, v), albedo, p);
+}
+
+// ---------------------------------------------------------------------------
+// Optics: include/dvs/device_optics.cuh:6-94.
+// ---------------------------------------------------------------------------
+static inline float hermite(float a, float va, float b, float vb, float u, float dt) {  // :6-9
+  float u2 = u * u, u3 = u2 * u;
+  return a + (-2 * u3 + 3 * u2) * (b - a) + (u3 - 2 * u2 + u) * dt * va + (u3 - u2) * dt * vb;
+}
+static void optical_state_at(const orc_interval &iv, double t, orc_state &s) {  // :13-26
+  float dt = (float)(iv.t1 - iv.t0);
+  float u = dt > 0 ? (float)fmin(1.0, fmax(0.0, (t - iv.t0) / dt)) : 0;
+  s = iv.start;
+  for (int k = 0; k < 3; ++k) {
+    s.position[k] = hermite(iv.start.position[k], iv.start.velocity[k], iv.end.position[k],
+                            iv.end.velocity[k], u, dt);
+    s.light[k] = hermite(iv.start.light[k], iv.start.light_rate[k], iv.end.light[k],
+                         iv.end.light_rate[k], u, dt);
+  }
+  s.intensity = fmaxf(0, hermite(iv.start.intensity, iv.start.intensity_rate, iv.end.intensity,
+                                 iv.end.intensity_rate, u, dt));
+  const float *a = iv.start.rotation;
+  float b[4] = {iv.end.rotation[0], iv.end.rotation[1], iv.end.rotation[2], iv.end.rotation[3]};
+  float d = a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
+  if (d < 0) { for (float &x : b) x = -x; d = -d; }
+  float wa = 1 - u, wb = u;
+  if (d < .9995f) {
+    float angle = acosf(fminf(1, d)), inv = 1 / sinf(angle);
+    wa = sinf((1 - u) * angle) * inv; wb = sinf(u * angle) * inv;
+  }
+  float q[4];
+  for (int k = 0; k < 4; ++k) q[k] = wa * a[k] + wb * b[k];
+  // rsqrtf in the reference; restated with IEEE sqrt and division.
+  float norm = 1.0f / sqrtf(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+  for (int k = 0; k < 4; ++k) s.rotation[k] = q[k] * norm;
+}
+static inline f3 rotate_ray(const float *q, f3 v) {  // :27-29
+  f3 qv = mk(q[0], q[1], q[2]);
+  f3 c = 2 * cross(qv, v);
+  return v + q[3] * c + cross(qv, c);
+}
+static inline float material_albedo(float value, float fallback) {  // :31-33
+  return std::isfinite(value) && value >= 0 && value <= 1 ? value : fallback;
+}
+static f3 shadow_origin(const Scene &s, int tri, float u, float v, f3 light) {  // :37-51
+  f3 a = s.vertices[s.indices[3 * tri]];
+  f3 e1 = s.vertices[s.indices[3 * tri + 1]] - a, e2 = s.vertices[s.indices[3 * tri + 2]] - a;
+  f3 point = a + u * e1 + v * e2, normal = normalize(cross(e1, e2));
+  if (dot(normal, light) < 0) normal = -normal;
+  constexpr float epsilon = 5.960464477539063e-8f, gamma = 8 * epsilon / (1 - 8 * epsilon);
+  f3 err = mk(gamma * (fabsf(a.x) + fabsf(u * e1.x) + fabsf(v * e2.x)),
+              gamma * (fabsf(a.y) + fabsf(u * e1.y) + fabsf(v * e2.y)),
+              gamma * (fabsf(a.z) + fabsf(u * e1.z) + fabsf(v * e2.z)));
+  float distance = fabsf(normal.x) * err.x + fabsf(normal.y) * err.y + fabsf(normal.z) * err.z + 1e-6f;
+  f3 offset = normal * distance;
+  point = point + offset;
+  if (offset.x != 0) point.x = nextafterf(point.x, copysignf(INFINITY, offset.x));
+  if (offset.y != 0) point.y = nextafterf(point.y, copysignf(INFINITY, offset.y));
+  if (offset.z != 0) point.z = nextafterf(point.z, copysignf(INFINITY, offset.z));
+  return point;
+}
+struct SubRay { double radiance = 0; float depth = -1; bool valid = true; bool shadow_ray = false; };
+// sample_subray (device_optics.cuh:54-72)
+static SubRay sample_subray(const Scene &s, int pixel, const orc_state &pose, int sx, int sy,
+                            int ss) {
+  SubRay out;
+  float px = (pixel % s.cam.width + (sx + .5f) / ss - .5f - s.cam.cx) / s.cam.fx;
+  float py = (pixel / s.cam.width + (sy + .5f) / ss - .5f - s.cam.cy) / s.cam.fy;
+  f3 ray = normalize(rotate_ray(pose.rotation, mk(px, -py, -1)));
+  Hit hit = trace_nearest(s, ld3(pose.position), ray);
+  if (hit.hit < 0) { out.valid = false; return out; }
+  if (!hit.hit) return out;
+  out.depth = hit.t;
+  const uint32_t *tri = &s.indices[3 * hit.tri];
+  float w = 1 - hit.u - hit.v;
+  f3 normal = normalize(s.normals[tri[0]] * w + s.normals[tri[1]] * hit.u + s.normals[tri[2]] * hit.v);
+  f3 light = normalize(ld3(pose.light));
+  if (dot(normal, light) <= 0 || dot(normal, -ray) <= 0) return out;
+  f3 origin = shadow_origin(s, hit.tri, hit.u, hit.v, light);
+  out.shadow_ray = true;
+  if (shadow_any_hit(s, origin, light, hit.tri, kRayEps, 1e30f)) return out;
+  float albedo = s.has_albedo
+                     ? material_albedo(s.albedos[tri[0]], s.hapke.w) * w +
+                           material_albedo(s.albedos[tri[1]], s.hapke.w) * hit.u +
+                           material_albedo(s.albedos[tri[2]], s.hapke.w) * hit.v
+                     : s.hapke.w;
+  double cos4 = 1.0 / ((1.0 + px * px + py * py) * (1.0 + px * px + py * py));
+  out.radiance = fmaxf(0, hapke_directional(normal, light, -ray, albedo, s.hapke) * pose.intensity) * cos4;
+  return out;
+}
+struct PixelSample { double radiance = 0; float depth = -1; bool valid = true; int shadow_rays = 0; };
+// sample_pixel (device_optics.cuh:73-94).  The per-subray work is the same
+// sequence of operations as sample_subray; accumulation is sequential sy,sx.
+static PixelSample sample_pixel(const Scene &s, int pixel, const orc_state &pose) {
+  PixelSample out; double depth = 0; int valid_depth = 0;
+  int ss = s.cam.samples;
+  for (int sy = 0; sy < ss; ++sy)
+    for (int sx = 0; sx < ss; ++sx) {
+      SubRay r = sample_subray(s, pixel, pose, sx, sy, ss);
+      if (!r.valid) { out.valid = false; return out; }
+      out.shadow_rays += r.shadow_ray;
+      if (r.depth < 0) continue;  // miss
+      depth += r.depth; ++valid_depth;
+      out.radiance += r.radiance;
+    }
+  out.radiance /= ss * ss;
+  out.depth = valid_depth ? (float)(depth / valid_depth) : -1;
+  return out;
+}
+
+// ---------------------------------------------------------------------------
+// Circuit: include/dvs/pixel_model.hpp:18-102.
+// ---------------------------------------------------------------------------
+static inline double receptor_log_at(double u, double rate, double dt, double charge) {
+  if (dt <= 0) return u;
+  rate = fmax(0.0, rate);
+  double x = exp(u);
+  if (rate == 0) return u - log1p(x * dt / charge);
+  double a = exp(-rate * dt / charge), b = -expm1(-rate * dt / charge);
+  return -log(a / x + b / rate);
+}
+static inline double contrast_at(double u, double rate, double dt, double ref,
+                                 const orc_pixel_params &p) {
+  return receptor_log_at(u, rate, dt, p.charge_scale) - ref + p.reference_leak * dt;
+}
+template <class Normal>
+static int advance_pixel(orc_pixel_state &s, const orc_pixel_params &p, double rate, double t0,
+                         double t1, orc_pixel_event *out, int capacity, Normal &&normal) {
+  if (t1 <= t0) return 0;
+  int count = 0; double cursor = t0, u = s.receptor_log;
+  while (cursor < t1) {
+    if (cursor < s.refractory_until) {
+      double end = fmin(t1, s.refractory_until);
+      u = receptor_log_at(u, rate, end - cursor, p.charge_scale);
+      cursor = end; s.reference_log = u;
+      if (cursor < t1 || cursor == s.refractory_until) {
+        if (cursor >= s.refractory_until && s.reset_pending) {
+          if (p.reset_noise_std > 0) s.reference_log += p.reset_noise_std * normal();
+          s.reset_pending = 0;
+        }
+      }
+      if (cursor >= t1) break;
+    } else if (s.reset_pending) {
+      if (p.reset_noise_std > 0) s.reference_log += p.reset_noise_std * normal();
+      s.reset_pending = 0;
+    }
+    double remaining = t1 - cursor, c0 = u - s.reference_log, boundary = remaining;
+    double x = exp(u), target = rate + p.charge_scale * p.reference_leak;
+    if (p.reference_leak > 0 && x > target && target > rate) {
+      double extremum;
+      if (rate > 0) extremum = -p.charge_scale / rate * log((1.0 / target - 1.0 / rate) / (1.0 / x - 1.0 / rate));
+      else extremum = p.charge_scale * (1.0 / target - 1.0 / x);
+      if (extremum > 1e-15 && extremum < boundary) boundary = extremum;
+    }
+    double ub = receptor_log_at(u, rate, boundary, p.charge_scale);
+    double cb = ub - s.reference_log + p.reference_leak * boundary;
+    int polarity = -1; double level = 0;
+    if (c0 >= s.threshold_pos) { polarity = 1; level = s.threshold_pos; }
+    else if (c0 <= -s.threshold_neg) { polarity = 0; level = -s.threshold_neg; }
+    else if (cb >= s.threshold_pos) { polarity = 1; level = s.threshold_pos; }
+    else if (cb <= -s.threshold_neg) { polarity = 0; level = -s.threshold_neg; }
+    if (polarity < 0) {
+      u = ub; s.reference_log -= p.reference_leak * boundary; cursor += boundary;
+      continue;
+    }
+    if (count == capacity) return -1;
+    double lo = 0, hi = boundary;
+    if (c0 >= s.threshold_pos || c0 <= -s.threshold_neg) hi = 0;
+    else if (p.reference_leak == 0) {
+      double target_inverse = exp(-(s.reference_log + level));
+      double initial_inverse = exp(-u);
+      if (rate > 0) {
+        double stationary_inverse = 1.0 / rate;
+        hi = -p.charge_scale / rate * log1p((target_inverse - initial_inverse) / (initial_inverse - stationary_inverse));
+      } else hi = p.charge_scale * (target_inverse - initial_inverse);
+      hi = fmin(boundary, fmax(0.0, hi));
+    } else for (int j = 0; j < 48; ++j) {
+      double mid = (lo + hi) * .5, c = contrast_at(u, rate, mid, s.reference_log, p);
+      if (polarity ? c >= level : c <= level) hi = mid; else lo = mid;
+    }
+    u = receptor_log_at(u, rate, hi, p.charge_scale); cursor += hi;
+    out[count].time = cursor; out[count].polarity = polarity; out[count].pad = 0; ++count;
+    s.reference_log = u; s.refractory_until = cursor + p.refractory_time;
+    s.reset_pending = 1;
+  }
+  s.receptor_log = u;
+  return count;
+}
+
+// Model parameters: src/cuda/event_sensor.cu:13,151-176.
+constexpr double kQ = 1.602176634e-19, kK = 1.380649e-23, kH = 6.62607015e-34, kC = 299792458;
+static void parameters(const orc_physic &c, bool effects, orc_pixel_params &p, orc_photo_params &ph) {
+  p.threshold_pos = c.threshold_pos; p.threshold_neg = c.threshold_neg;
+  double voltage = kK * c.temperature_k / (kQ * c.subthreshold_kappa);
+  p.charge_scale = c.receptor_capacitance_f * voltage / kQ;
+  if (effects) {
+    p.refractory_time = c.refractory_time;
+    p.reference_leak = c.reset_leakage_a / (c.reset_capacitance_f * voltage);
+    p.reset_noise_std = c.reset_thermal_noise ? sqrt(kK * c.temperature_k / c.reset_capacitance_f) / voltage : 0;
+  } else {
+    p.refractory_time = 0; p.reference_leak = 0; p.reset_noise_std = 0;
+  }
+  double pitch = c.pixel_pitch_um * 1e-6;
+  ph.electrons_per_radiance = c.radiance_scale * 3.141592653589793 / (4 * c.f_number * c.f_number) *
+                              c.lens_transmission * pitch * pitch * c.fill_factor *
+                              c.quantum_efficiency * (c.wavelength_nm * 1e-9) / (kH * kC);
+  ph.integration_step = c.circuit_step;
+  if (effects) { ph.dark_electrons = c.dark_current_a / kQ; ph.shot_noise = c.photon_shot_noise; }
+  else { ph.dark_electrons = 0; ph.shot_noise = 0; }
+  ph.pad = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Noise.  The reference draws cuRAND XORWOW (event_sensor.cu:21,26-27,
+// photon_noise.cuh:4-9), a closed library; this build replaces it with a
+// counter-based Philox4x64-10 (Salmon et al., SC'11) keyed by (seed, stream)
+// with counter (block, pixel).  Samplers: Poisson by sequential-search
+// inversion for lambda < 10 (Devroye 1986, X.3), by the transformed rejection
+// PTRS method (Hoermann 1993) up to 1e6, and the reference's rounded normal
+// approximation above 1e6; normals by Box-Muller.
+// ---------------------------------------------------------------------------
+static inline void mulhilo64(uint64_t a, uint64_t b, uint64_t &hi, uint64_t &lo) {
+  unsigned __int128 p = (unsigned __int128)a * b;
+  hi = (uint64_t)(p >> 64); lo = (uint64_t)p;
+}
+static void philox4x64(const uint64_t ctr[4], const uint64_t key[2], uint64_t out[4]) {
+  uint64_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; ++r) {
+    if (r) { k0 += 0x9E3779B97F4A7C15ULL; k1 += 0xBB67AE8584CAA73BULL; }
+    uint64_t hi0, lo0, hi1, lo1;
+    mulhilo64(0xD2E7470EE14C6C93ULL, c0, hi0, lo0);
+    mulhilo64(0xCA5A826395121157ULL, c2, hi1, lo1);
+    uint64_t n0 = hi1 ^ c1 ^ k0, n1 = lo1, n2 = hi0 ^ c3 ^ k1, n3 = lo0;
+    c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+enum { kStreamShot = 0, kStreamThreshold = 1, kStreamReset = 2 };
+constexpr uint64_t kStreamTag = 0x4456535F4E4F4953ULL;  // key word 1 = tag ^ stream
+struct Stream {
+  uint64_t seed, pixel, stream;
+  uint64_t *block;  // next unused block index for this stream
+  uint64_t buf[4]; int used = 4;
+  double uniform() {
+    if (used == 4) {
+      uint64_t ctr[4] = {(*block)++, pixel, 0, 0}, key[2] = {seed, kStreamTag ^ stream};
+      philox4x64(ctr, key, buf); used = 0;
+    }
+    return ((double)(buf[used++] >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  }
+};
+static double normal_draw(Stream &st) {
+  st.used = 4;  // every normal starts a fresh block
+  double u1 = st.uniform(), u2 = st.uniform();
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+static double poisson_draw(Stream &st, double lambda) {
+  if (lambda <= 0) return 0;
+  if (lambda > 1e6) return fmax(0.0, floor(lambda + sqrt(lambda) * normal_draw(st) + .5));
+  st.used = 4;  // every draw starts a fresh block
+  if (lambda < 10) {
+    double u = st.uniform(), p = exp(-lambda), F = p; int k = 0;
+    while (u > F && k < 1000) { ++k; p *= lambda / k; F += p; }
+    return k;
+  }
+  double slam = sqrt(lambda), loglam = log(lambda);
+  double b = 0.931 + 2.53 * slam, a = -0.059 + 0.02483 * b;
+  double invalpha = 1.1239 + 1.1328 / (b - 3.4), vr = 0.9277 - 3.6224 / (b - 2);
+  for (;;) {
+    double U = st.uniform() - 0.5, V = st.uniform();
+    double us = 0.5 - fabs(U);
+    double k = floor((2 * a / us + b) * U + lambda + 0.43);
+    if (us >= 0.07 && V <= vr) return k;
+    if (k < 0 || (us < 0.013 && V > us)) continue;
+    if (log(V) + log(invalpha) - log(a / (us * us) + b) <= -lambda + k * loglam - lgamma(k + 1)) return k;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Readout FIFO: src/event_readout.cpp:6-34.
+// ---------------------------------------------------------------------------
+struct Readout {
+  double bandwidth, latency, available = 0;
+  size_t capacity, dropped = 0, peak = 0;
+  struct Pending { double ready; orc_event e; };
+  std::deque<Pending> pending;
+  Readout(double b, double l, size_t c) : bandwidth(b), latency(l), capacity(c) {}
+  void submit(std::vector<orc_event> events, double now, std::vector<orc_event> &ready) {
+    ready.clear();
+    std::stable_sort(events.begin(), events.end(), [](const orc_event &a, const orc_event &b) {
+      if (a.timestamp != b.timestamp) return a.timestamp < b.timestamp;
+      if (a.y != b.y) return a.y < b.y;
+      return a.x < b.x;
+    });
+    for (const auto &e : events) {
+      while (!pending.empty() && pending.front().ready <= e.timestamp) {
+        ready.push_back(pending.front().e); pending.pop_front();
+      }
+      if (bandwidth <= 0) { ready.push_back(e); continue; }
+      if (pending.size() >= capacity) { ++dropped; continue; }
+      double completed = std::max(available, (double)e.timestamp) + 1.0 / bandwidth;
+      available = completed;
+      orc_event d = e; d.timestamp = completed + latency;
+      pending.push_back({completed + latency, d});
+      peak = std::max(peak, pending.size());
+    }
+    while (!pending.empty() && pending.front().ready <= now) {
+      ready.push_back(pending.front().e); pending.pop_front();
+    }
+  }
+  void drain(std::vector<orc_event> &ready) {
+    ready.clear();
+    while (!pending.empty()) { ready.push_back(pending.front().e); pending.pop_front(); }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Sensor: src/cuda/event_sensor.cu:18-60,244-305.
+// ---------------------------------------------------------------------------
+struct Sensor {
+  int width, height, count, max_events;
+  uint64_t seed;
+  orc_pixel_params p{};
+  orc_photo_params photo{};
+  double threshold_std, threshold_min, last_time = 0;
+  bool initialized = false;
+  std::vector<orc_pixel_state> states;
+  std::vector<uint64_t> shot_block, reset_block;
+  Readout readout;
+  int64_t shadow_rays = 0;
+  Sensor(int w, int h, const orc_physic &c, bool effects, bool aer)
+      : width(w), height(h), count(w * h), max_events(c.max_events_per_step),
+        seed((uint64_t)(uint32_t)c.random_seed), threshold_std(c.threshold_std),
+        threshold_min(c.threshold_min),
+        readout(aer ? c.readout_bandwidth : 0, aer ? c.readout_latency : 0,
+                (size_t)c.readout_queue_capacity) {
+    if (!effects) threshold_std = 0;
+    parameters(c, effects, p, photo);
+    states.assign(count, orc_pixel_state{});
+    shot_block.assign(count, 0); reset_block.assign(count, 0);
+  }
+};
+
+static void set_threads(int threads) {
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#else
+  (void)threads;
+#endif
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+extern "C" {
+
+int orc_bvh_build(int max_leaf, float node_cost, float prim_cost, const float *vertices, int nv,
+                  uint32_t *indices, int nt, orc_node *nodes, int *node_count) {
+  if (nt <= 0) { *node_count = 0; return -1; }
+  BuildCtx c{max_leaf, node_cost, prim_cost, {}, {}};
+  c.refs.reserve(nt);
+  for (int i = 0; i < nt; ++i) {
+    f3 a = ld3(vertices + 3 * (size_t)indices[3 * i]), b = ld3(vertices + 3 * (size_t)indices[3 * i + 1]),
+       d = ld3(vertices + 3 * (size_t)indices[3 * i + 2]);
+    Box box{fmin3(fmin3(a, b), d), fmax3(fmax3(a, b), d)};
+    c.refs.push_back(Ref{box, box_center(box), i});
+  }
+  (void)nv;
+  c.nodes.reserve(2 * (size_t)nt);
+  int root = build_node(c, 0, nt);
+  std::vector<uint32_t> out(3 * (size_t)nt);
+  for (int i = 0; i < nt; ++i)
+    for (int k = 0; k < 3; ++k) out[3 * (size_t)i + k] = indices[3 * (size_t)c.refs[i].tri + k];
+  std::memcpy(indices, out.data(), out.size() * sizeof(uint32_t));
+  std::memcpy(nodes, c.nodes.data(), c.nodes.size() * sizeof(orc_node));
+  *node_count = (int)c.nodes.size();
+  return root;
+}
+
+int orc_bvh_validate(const orc_node *nodes, int n, int root) {  // build.cpp:188-242
+  if (root < 0 || root >= n) return -1;
+  std::vector<char> seen(n, 0);
+  std::vector<int> st{root};
+  while (!st.empty()) {
+    int i = st.back(); st.pop_back();
+    if (i < 0 || i >= n) return -2;
+    if (seen[i]) return -3;
+    seen[i] = 1;
+    if (nodes[i].count > 0) {
+      if (nodes[i].left != -1 || nodes[i].right != -1) return -4;
+      continue;
+    }
+    if (nodes[i].left < 0 || nodes[i].left >= n || nodes[i].right < 0 || nodes[i].right >= n) return -5;
+    st.push_back(nodes[i].right); st.push_back(nodes[i].left);
+  }
+  return 0;
+}
+
+void *orc_scene_create(const orc_node *nodes, int node_count, int root, const float *vertices,
+                       const float *normals, int nv, const uint32_t *indices, int nt,
+                       const float *albedos, orc_camera camera, orc_hapke hapke) {
+  auto *s = new Scene();
+  s->nodes.assign(nodes, nodes + node_count);
+  s->root = root;
+  s->vertices.resize(nv); s->normals.resize(nv);
+  for (int i = 0; i < nv; ++i) { s->vertices[i] = ld3(vertices + 3 * (size_t)i); s->normals[i] = ld3(normals + 3 * (size_t)i); }
+  s->indices.assign(indices, indices + 3 * (size_t)nt);
+  if (albedos) { s->albedos.assign(albedos, albedos + nv); s->has_albedo = true; }
+  s->cam = camera; s->hapke = hapke;
+  return s;
+}
+void orc_scene_destroy(void *s) { delete static_cast<Scene *>(s); }
+
+int orc_trace_nearest(const void *sp, const float o[3], const float d[3], int *tri, float *t,
+                      float *u, float *v) {
+  Hit h = trace_nearest(*static_cast<const Scene *>(sp), ld3(o), ld3(d));
+  *tri = h.tri; *t = h.t; *u = h.u; *v = h.v;
+  return h.hit;
+}
+int orc_trace_nearest_counted(const void *sp, const float o[3], const float d[3], int64_t counts[3]) {
+  return trace_nearest(*static_cast<const Scene *>(sp), ld3(o), ld3(d), counts).hit;
+}
+int orc_trace_shadow(const void *sp, const float o[3], const float d[3], int self, float t_min,
+                     float t_max) {
+  return shadow_any_hit(*static_cast<const Scene *>(sp), ld3(o), ld3(d), self, t_min, t_max);
+}
+void orc_shadow_origin(const void *sp, int tri, float u, float v, const float light[3], float out[3]) {
+  f3 p = shadow_origin(*static_cast<const Scene *>(sp), tri, u, v, ld3(light));
+  out[0] = p.x; out[1] = p.y; out[2] = p.z;
+}
+int orc_sample_pixel(const void *sp, int pixel, const orc_state *pose, double *rad, float *depth) {
+  PixelSample r = sample_pixel(*static_cast<const Scene *>(sp), pixel, *pose);
+  *rad = r.radiance; *depth = r.depth;
+  return r.valid ? 1 : 0;
+}
+void orc_optical_state_at(const orc_interval *iv, double t, orc_state *out) { optical_state_at(*iv, t, *out); }
+
+int orc_render(const void *sp, const orc_state *pose, float *image, float *depth, double *radiance,
+               int threads) {
+  const Scene &s = *static_cast<const Scene *>(sp);
+  set_threads(threads);
+  int n = s.cam.width * s.cam.height, bad = 0, R = s.cam.samples * s.cam.samples;
+#pragma omp parallel for schedule(dynamic, 64) reduction(| : bad)
+  for (int i = 0; i < n; ++i) {
+    // sample_warps (optical_sampler.cu:8-23): ordered sum over j = sy*ss+sx.
+    double total = 0, ranges = 0; int hits = 0;
+    for (int j = 0; j < R; ++j) {
+      SubRay r = sample_subray(s, i, *pose, j % s.cam.samples, j / s.cam.samples, s.cam.samples);
+      if (!r.valid) bad = 1;
+      total += r.radiance;
+      if (r.depth >= 0) { ranges += r.depth; ++hits; }
+    }
+    double rad = total / R;
+    if (radiance) radiance[i] = rad;
+    if (image) image[i] = (float)rad;
+    if (depth) depth[i] = hits ? (float)(ranges / hits) : -1;
+  }
+  return bad ? 2 : 0;
+}
+
+float orc_hapke_reflectance(float ci, float ce, float cg, float a, orc_hapke p) { return hapke_reflectance(ci, ce, cg, a, p); }
+float orc_hapke_directional(const float n[3], const float l[3], const float v[3], float a, orc_hapke p) {
+  return hapke_directional(ld3(n), ld3(l), ld3(v), a, p);
+}
+float orc_invert_hapke_if(float inc, float emi, float phase, float target, orc_hapke p) {  // hapke.hpp:160-172
+  if (!(inc >= 0 && inc < 90 && emi >= 0 && emi < 90 && phase >= 0 && phase <= 180 && target >= 0) || !std::isfinite(target)) return NAN;
+  if (phase > inc + emi + 1e-4f || phase < fabsf(inc - emi) - 1e-4f) return NAN;
+  float ci = cosf(inc * kPi / 180), ce = cosf(emi * kPi / 180), cg = cosf(phase * kPi / 180);
+  float top = kPi * hapke_reflectance(ci, ce, cg, 1, p);
+  if (!std::isfinite(top) || target > top) return NAN;
+  float lo = 0, hi = 1;
+  for (int k = 0; k < 32; ++k) {
+    float w = (lo + hi) * .5f;
+    if (kPi * hapke_reflectance(ci, ce, cg, w, p) < target) lo = w; else hi = w;
+  }
+  return (lo + hi) * .5f;
+}
+
+double orc_receptor_log_at(double u, double rate, double dt, double charge) { return receptor_log_at(u, rate, dt, charge); }
+int orc_advance_pixel(orc_pixel_state *s, const orc_pixel_params *p, double rate, double t0,
+                      double t1, orc_pixel_event *out, int capacity, double fixed_normal) {
+  return advance_pixel(*s, *p, rate, t0, t1, out, capacity, [&] { return fixed_normal; });
+}
+void orc_parameters(const orc_physic *c, int effects, orc_pixel_params *p, orc_photo_params *photo) {
+  parameters(*c, effects != 0, *p, *photo);
+}
+
+void orc_philox4x64(const uint64_t ctr[4], const uint64_t key[2], uint64_t out[4]) { philox4x64(ctr, key, out); }
+void orc_poisson_series(uint64_t seed, uint64_t pixel, double lambda, int n, double *out) {
+  uint64_t block = 0; Stream st{seed, pixel, kStreamShot, &block};
+  for (int i = 0; i < n; ++i) out[i] = poisson_draw(st, lambda);
+}
+void orc_normal_series(uint64_t seed, uint64_t pixel, int n, double *out) {
+  uint64_t block = 0; Stream st{seed, pixel, kStreamReset, &block};
+  for (int i = 0; i < n; ++i) out[i] = normal_draw(st);
+}
+
+void *orc_sensor_create(int w, int h, const orc_physic *c, int effects, int aer) {
+  if (w <= 0 || h <= 0 || w > 32767 || h > 32767 || c->max_events_per_step < 1 || c->max_events_per_step > 64)
+    return nullptr;
+  return new Sensor(w, h, *c, effects != 0, aer != 0);
+}
+void orc_sensor_destroy(void *s) { delete static_cast<Sensor *>(s); }
+
+// initialize_pixels (event_sensor.cu:18-28) + EventSensor::initialize (:262-270).
+int orc_sensor_initialize(void *sp, const void *scp, const orc_state *pose, double timestamp, int threads) {
+  Sensor &s = *static_cast<Sensor *>(sp);
+  const Scene &sc = *static_cast<const Scene *>(scp);
+  if (!std::isfinite(timestamp) || s.initialized) return 3;
+  set_threads(threads);
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(| : bad)
+  for (int i = 0; i < s.count; ++i) {
+    PixelSample o = sample_pixel(sc, i, *pose);
+    if (!o.valid || !std::isfinite(o.radiance)) { bad = 1; continue; }
+    orc_pixel_state st{};
+    st.refractory_until = -1; st.reset_pending = 0;
+    st.receptor_log = st.reference_log = log(fmax(1e-30, o.radiance * s.photo.electrons_per_radiance + s.photo.dark_electrons));
+    double zp = 0, zn = 0;
+    if (s.threshold_std != 0) {
+      uint64_t block = 0; Stream rs{s.seed, (uint64_t)i, kStreamThreshold, &block};
+      zp = normal_draw(rs); zn = normal_draw(rs);
+    }
+    st.threshold_pos = fmax(s.threshold_min, s.p.threshold_pos + s.threshold_std * zp);
+    st.threshold_neg = fmax(s.threshold_min, s.p.threshold_neg + s.threshold_std * zn);
+    s.states[i] = st;
+  }
+  if (bad) return 2;
+  s.last_time = timestamp; s.initialized = true;
+  return 0;
+}
+
+// process_many (event_sensor.cu:274-305) over generate_pixels (:29-60).
+int orc_sensor_process(void *sp, const void *scp, const orc_interval *ivs, int n, orc_event *raw,
+                       int64_t raw_cap, int64_t *raw_count, orc_event *ready, int64_t ready_cap,
+                       int64_t *ready_count, int threads) {
+  Sensor &s = *static_cast<Sensor *>(sp);
+  const Scene &sc = *static_cast<const Scene *>(scp);
+  if (!s.initialized || n <= 0) return 3;
+  double previous = s.last_time, cells = 0;
+  for (int k = 0; k < n; ++k) {
+    const orc_interval &iv = ivs[k];
+    if (!std::isfinite(iv.t0) || !std::isfinite(iv.t1) || iv.t0 != previous || iv.t1 <= iv.t0) return 3;
+    cells += (iv.t1 - iv.t0) / s.photo.integration_step; previous = iv.t1;
+  }
+  if (cells > 1000000) return 3;
+  set_threads(threads);
+  std::vector<std::vector<orc_event>> per_pixel(s.count);
+  int err_capacity = 0, err_invalid = 0; int64_t shadow = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(| : err_capacity, err_invalid) reduction(+ : shadow)
+  for (int i = 0; i < s.count; ++i) {
+    std::vector<orc_pixel_event> local(s.max_events);
+    int count = 0; bool failed = false;
+    Stream shot{s.seed, (uint64_t)i, kStreamShot, &s.shot_block[i]};
+    Stream reset{s.seed, (uint64_t)i, kStreamReset, &s.reset_block[i]};
+    auto normal = [&] { return normal_draw(reset); };
+    for (int k = 0; k < n && !failed; ++k) {
+      const orc_interval &iv = ivs[k];
+      int steps = (int)ceil((iv.t1 - iv.t0) / s.photo.integration_step);
+      double dt = (iv.t1 - iv.t0) / steps;
+      for (int step = 0; step < steps; ++step) {
+        double t0 = iv.t0 + step * dt, t1 = step + 1 == steps ? iv.t1 : iv.t0 + (step + 1) * dt;
+        orc_state pose; optical_state_at(iv, (t0 + t1) * .5, pose);
+        PixelSample o = sample_pixel(sc, i, pose);
+        shadow += o.shadow_rays;
+        if (!o.valid || !std::isfinite(o.radiance)) { err_invalid = 1; failed = true; break; }
+        double rate = o.radiance * s.photo.electrons_per_radiance + s.photo.dark_electrons;
+        if (s.photo.shot_noise) rate = poisson_draw(shot, rate * (t1 - t0)) / (t1 - t0);
+        int emitted = advance_pixel(s.states[i], s.p, rate, t0, t1, local.data() + count, s.max_events - count, normal);
+        if (emitted < 0) { err_capacity = 1; failed = true; break; }
+        count += emitted;
+      }
+    }
+    if (failed) continue;
+    int interval = 0;
+    per_pixel[i].reserve(count);
+    for (int j = 0; j < count; ++j) {
+      while (interval + 1 < n && local[j].time > ivs[interval].t1) ++interval;
+      orc_state pose; optical_state_at(ivs[interval], local[j].time, pose);
+      PixelSample o = sample_pixel(sc, i, pose);
+      if (!o.valid) { err_invalid = 1; break; }
+      orc_event e; e.timestamp = local[j].time; e.x = (int16_t)(i % s.width);
+      e.y = (int16_t)(i / s.width); e.polarity = (int8_t)local[j].polarity; e.depth = o.depth;
+      per_pixel[i].push_back(e);
+    }
+  }
+  s.shadow_rays += shadow;
+  if (err_capacity) return 1;
+  if (err_invalid) return 2;
+  std::vector<orc_event> generated;
+  for (auto &v : per_pixel) generated.insert(generated.end(), v.begin(), v.end());
+  if (raw_count) *raw_count = (int64_t)generated.size();
+  if (raw) std::memcpy(raw, generated.data(), sizeof(orc_event) * std::min<int64_t>(raw_cap, generated.size()));
+  std::vector<orc_event> out;
+  s.readout.submit(std::move(generated), ivs[n - 1].t1, out);
+  s.last_time = ivs[n - 1].t1;
+  if (ready_count) *ready_count = (int64_t)out.size();
+  if (ready) std::memcpy(ready, out.data(), sizeof(orc_event) * std::min<int64_t>(ready_cap, out.size()));
+  return 0;
+}
+int orc_sensor_drain(void *sp, orc_event *ready, int64_t cap, int64_t *count) {
+  Sensor &s = *static_cast<Sensor *>(sp);
+  std::vector<orc_event> out; s.readout.drain(out);
+  *count = (int64_t)out.size();
+  if (ready) std::memcpy(ready, out.data(), sizeof(orc_event) * std::min<int64_t>(cap, out.size()));
+  return 0;
+}
+int64_t orc_sensor_dropped(const void *sp) { return (int64_t)static_cast<const Sensor *>(sp)->readout.dropped; }
+int64_t orc_sensor_peak_queue(const void *sp) { return (int64_t)static_cast<const Sensor *>(sp)->readout.peak; }
+int orc_sensor_states(const void *sp, orc_pixel_state *out) {
+  const Sensor &s = *static_cast<const Sensor *>(sp);
+  std::memcpy(out, s.states.data(), sizeof(orc_pixel_state) * s.states.size());
+  return 0;
+}
+int64_t orc_sensor_shadow_rays(const void *sp) { return static_cast<const Sensor *>(sp)->shadow_rays; }
+
+void *orc_readout_create(double b, double l, int64_t c) { return new Readout(b, l, (size_t)c); }
+void orc_readout_destroy(void *r) { delete static_cast<Readout *>(r); }
+int64_t orc_readout_submit(void *rp, const orc_event *ev, int64_t n, double now, orc_event *ready, int64_t cap) {
+  std::vector<orc_event> in(ev, ev + n), out;
+  static_cast<Readout *>(rp)->submit(std::move(in), now, out);
+  if (ready) std::memcpy(ready, out.data(), sizeof(orc_event) * std::min<int64_t>(cap, out.size()));
+  return (int64_t)out.size();
+}
+int64_t orc_readout_drain(void *rp, orc_event *ready, int64_t cap) {
+  std::vector<orc_event> out; static_cast<Readout *>(rp)->drain(out);
+  if (ready) std::memcpy(ready, out.data(), sizeof(orc_event) * std::min<int64_t>(cap, out.size()));
+  return (int64_t)out.size();
+}
+int64_t orc_readout_dropped(const void *r) { return (int64_t)static_cast<const Readout *>(r)->dropped; }
+int64_t orc_readout_peak(const void *r) { return (int64_t)static_cast<const Readout *>(r)->peak; }
+
+// render_float_to_png_bytes (blenderize.hpp:12-69).
+void orc_render_to_bytes(const float *linear, int width, int height, float dither, uint8_t *out) {
+  auto to_byte = [](float v) -> uint8_t {
+    if (!std::isfinite(v) || v <= 0.0f) return 0;
+    if (v >= 1.0f) return 255;
+    return (uint8_t)(v * 255.0f + 0.5f);
+  };
+  auto srgb = [](float c) { return c < 0.0031308f ? 12.92f * c : 1.055f * std::pow(c, 1.0f / 2.4f) - 0.055f; };
+  auto hash = [](uint32_t x, uint32_t y) {
+    const uint32_t qx = 1103515245u * ((x >> 1u) ^ (y));
+    const uint32_t qy = 1103515245u * ((y >> 1u) ^ (x));
+    const uint32_t n = 1103515245u * ((qx) ^ (qy >> 3u));
+    return float(n) * (1.0f / float(0xffffffffu));
+  };
+  for (int y = 0; y < height; ++y)
+    for (int x = 0; x < width; ++x) {
+      float r = linear[y * width + x];
+      r = std::clamp(r / 100, 0.0f, 1.0f);
+      r = srgb(r);
+      if (dither != 0) {
+        float v = hash(x, y) * 2.0f - 1.0f;
+        v = std::copysign(1.0f, v) * (1.0f - sqrtf(1.0f - fabsf(v)));
+        r += dither * v;
+      }
+      out[y * width + x] = to_byte(r);
+    }
+}
+
+}  // extern "C"
