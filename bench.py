@@ -1,0 +1,173 @@
+#!/usr/bin/env python3
+"""DVS hot-path benchmark on one MI355X.
+
+Workload: the single DAVIS346 request — 346x260 sensor, fx=fy=300, 4x4
+sub-rays per pixel, 1024^2 heightfield (2,093,058 triangles, 200 craters),
+100 fps trajectory, 10 us host horizons of 1 us circuit cells, shot and reset
+noise on (Philox seed), native solver.  One step is one
+EventSensor::process_many call over 64 horizons (the reference driver's native
+batch, apps/main_dvs.cpp:96-98): 640 circuit cells x 89,960 pixels x 16
+sub-rays of ray casting + Hapke shading + shadow rays, the FP64 circuit,
+event-time depth rays, device compaction/sort and the readout FIFO.  Inputs
+(BVH, mesh, pixel state) are resident in HBM before timing starts.
+
+Prints one JSON line: Mrays/s (primary + shadow + event-depth rays traced per
+wall second), events/s, per-step p50/p99 latency, the trace kernel's roofline
+(algorithmic bytes per ray from the product's traversal counters, over the
+HIP-event time of the trace launches) and the CPU oracle timed on the host on
+a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Mrays/s + events/s at DAVIS346; p50/p99 latency, served vs plain, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+PAIR_BYTES, LEAF_BYTES, TRIANGLE_BYTES = 64, 8, 48
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def bytes_per_ray(stats: dict) -> tuple[float, float]:
+    """Algorithmic bytes per primary and per shadow ray of the SoA layout."""
+    def per(prefix):
+        rays = max(stats[f"{prefix}_rays"], 1)
+        total = (stats[f"{prefix}_pairs"] * PAIR_BYTES + stats[f"{prefix}_leaves"] * LEAF_BYTES +
+                 stats[f"{prefix}_triangles"] * TRIANGLE_BYTES)
+        return total / rays
+    return per("primary"), per("shadow")
+
+
+def cpu_baseline(w, mesh, horizons: int) -> dict:
+    """The CPU oracle (restatement of the reference path) on the host cores."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle_ctypes as oracle
+
+    v, n, nodes, root, idx = mesh
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    scene = oracle.Scene(nodes, root, v, n, idx, w.camera, w.hapke)
+    sensor = oracle.Sensor(w.camera.width, w.camera.height, w.physic)
+    sensor.initialize(scene, w.initial_state(), w.start_time, threads)
+    ivs = w.intervals(0, horizons)
+    t0 = time.perf_counter()
+    code, raw, _ = sensor.process(scene, ivs, threads)
+    wall = time.perf_counter() - t0
+    if code:
+        raise RuntimeError(f"oracle failed with status {code}")
+    R = w.camera.samples ** 2
+    cells = int(round((ivs["t1"][-1] - ivs["t0"][0]) / np.float32(w.physic.circuit_step)))
+    rays = w.camera.width * w.camera.height * cells * R + sensor.shadow_rays + len(raw) * R
+    return {"value": round(rays / wall / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"{horizons} horizons ({cells} circuit cells) of the same request from t=0 on the full "
+                      f"346x260 sensor, initialisation excluded; {rays} rays in {wall:.2f} s; "
+                      f"{len(raw)} events ({len(raw) / wall:.1f} events/s)"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="davis346-terrain1024-ss4")
+    ap.add_argument("--horizons", type=int, default=64, help="host horizons per step")
+    ap.add_argument("--cpu-horizons", type=int, default=2, help="horizons in the CPU baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU layout is "
+              "described in DESIGN.md and not launched in this build", file=sys.stderr)
+        return 2
+
+    from dvsrender4deepspace_amd import capi, terrain, workloads
+
+    capi.load_library()
+    if capi.device_count() < 1:
+        raise RuntimeError("no HIP device visible")
+    w = workloads.WORKLOADS[args.workload]
+    t = time.perf_counter()
+    v, n, idx = terrain.mesh(w.terrain)
+    nodes, root, leaf_idx = capi.bvh_build(v, idx, w.leaf_size)
+    log(f"scene: {len(idx)} triangles, {len(nodes)} BVH nodes, built in {time.perf_counter() - t:.1f} s")
+    scene = capi.Scene(nodes, root, v, n, leaf_idx, w.camera, w.hapke)
+    info = scene.info()
+    trav = scene.traversal_stats(w.initial_state())
+    b_primary, b_shadow = bytes_per_ray(trav)
+    log(f"scene on device: {info}; bytes/ray primary {b_primary:.0f}, shadow {b_shadow:.0f}")
+
+    sensor = capi.Sensor(scene, w.physic)
+    sensor.initialize(w.initial_state(), w.start_time)
+    step = 0
+    for _ in range(args.warmup):
+        sensor.process(w.intervals(step, args.horizons))
+        step += args.horizons
+    log(f"warmup done ({args.warmup} steps)")
+    sensor.reset_stats()
+    latencies = []
+    delivered = 0
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        ready = sensor.process(w.intervals(step, args.horizons))
+        latencies.append(time.perf_counter() - t0)
+        delivered += len(ready)
+        step += args.horizons
+        log(f"step {k}: {latencies[-1] * 1e3:.1f} ms, {len(ready)} events delivered")
+    wall = time.perf_counter() - t_start
+    st = sensor.stats()
+    rays = st["primary_rays"] + st["shadow_rays"] + st["event_rays"]
+    mrays = rays / wall / 1e6
+    launches = max(st["trace_launches"], 1)
+    trace_s = st["trace_ms"] / 1e3
+    algorithmic = st["primary_rays"] * b_primary + st["shadow_rays"] * b_shadow
+    achieved = algorithmic / max(trace_s, 1e-30) / 1e9
+    lat = np.array(latencies) * 1e3
+    result = {
+        "metric": METRIC,
+        "value": round(mrays, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (fp64 circuit/timestamps)",
+        "data": "synthetic",
+        "config": {"workload": w.name, "sensor": f"{w.camera.width}x{w.camera.height}",
+                   "supersample": w.camera.samples, "terrain_triangles": int(len(idx)),
+                   "horizons_per_step": args.horizons, "time_resolution_s": w.time_resolution,
+                   "circuit_step_s": w.physic.circuit_step, "noise": "shot+reset on", "solver": "native",
+                   "bvh_leaf_size": w.leaf_size},
+        "events_per_s": round(st["comparator_events"] / wall, 1),
+        "delivered_events_per_s": round(delivered / wall, 1),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
+        "breakdown_ms": {"trace": round(st["trace_ms"], 3), "circuit": round(st["circuit_ms"], 3),
+                         "events": round(st["events_ms"], 3), "wall": round(wall * 1e3, 3)},
+        "rays": {"primary": st["primary_rays"], "shadow": st["shadow_rays"], "event_depth": st["event_rays"]},
+        "roofline": {"bound": "hbm", "kernel": "trace_kernel<MODE_CELLS>", "achieved": round(achieved, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "avg_launch_ms": round(st["trace_ms"] / launches, 3),
+                     "bytes_per_ray": {"primary": round(b_primary, 1), "shadow": round(b_shadow, 1)}},
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle) ...")
+        result["cpu_baseline"] = cpu_baseline(w, (v, n, nodes, root, leaf_idx), args.cpu_horizons)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -166,6 +166,7 @@ def load_library() -> ctypes.CDLL:
         "dvs_frame_render": (ctypes.c_int, [_P, _P, _I32, _P, _P, _P]),
         "dvs_frame_render_device": (ctypes.c_int, [_P, _P, _I32, _P, _P]),
         "dvs_scene_stats": (ctypes.c_int, [_P, ctypes.POINTER(_StatsC)]),
+        "dvs_frame_traversal_stats": (ctypes.c_int, [_P, _P, _P]),
         "dvs_sensor_create": (ctypes.c_int, [_P, ctypes.POINTER(_PhysicC), _I32, _I32, ctypes.POINTER(_P)]),
         "dvs_sensor_destroy": (ctypes.c_int, [_P]),
         "dvs_sensor_initialize": (ctypes.c_int, [_P, _P, ctypes.c_double]),
@@ -298,6 +299,15 @@ class Scene:
         s = _StatsC()
         _check(load_library().dvs_scene_stats(self.handle, ctypes.byref(s)))
         return _stats(s)
+
+    def traversal_stats(self, pose: np.ndarray) -> dict:
+        """Per-frame traversal work (node-pair visits, leaf visits, triangle tests)."""
+        p = np.ascontiguousarray(np.atleast_1d(pose)[:1], dtype=STATE_DTYPE)
+        t = np.zeros(8, np.uint64)
+        _check(load_library().dvs_frame_traversal_stats(self.handle, _ptr(p), _ptr(t)))
+        keys = ("primary_rays", "primary_pairs", "primary_leaves", "primary_triangles",
+                "shadow_rays", "shadow_pairs", "shadow_leaves", "shadow_triangles")
+        return {k: int(v) for k, v in zip(keys, t)}
 
     def trace(self, origins, directions):
         o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)

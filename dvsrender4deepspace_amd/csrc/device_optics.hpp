@@ -162,9 +162,24 @@ __device__ __forceinline__ bool intersect(const float4 *rec, f3 ray, f3 o, float
 
 struct Hit { int hit, tri; float t, u, v; };
 
+// Traversal work counters: the hot path uses NoCount (compiled away); the
+// diagnostic traversal-statistics kernel uses WorkCount.
+struct NoCount {
+  __device__ __forceinline__ void pair() {}
+  __device__ __forceinline__ void leaf() {}
+  __device__ __forceinline__ void triangle() {}
+};
+struct WorkCount {
+  unsigned pairs = 0, leaves = 0, triangles = 0;
+  __device__ __forceinline__ void pair() { ++pairs; }
+  __device__ __forceinline__ void leaf() { ++leaves; }
+  __device__ __forceinline__ void triangle() { ++triangles; }
+};
+
 // Nearest hit (trace_nearest, device_ray.hpp:136-158).  `stack` points at
 // this lane's first LDS slot, consecutive slots `stride` ints apart.
-__device__ inline Hit trace_nearest(const DevScene &s, f3 origin, f3 ray, int *stack, int stride) {
+template <class Count>
+__device__ inline Hit trace_nearest(const DevScene &s, f3 origin, f3 ray, int *stack, int stride, Count &cnt) {
   Hit h{0, -1, 0.f, 0.f, 0.f};
   float closest = 1e30f;
   const f3 idir = inverse_direction(ray);
@@ -172,15 +187,18 @@ __device__ inline Hit trace_nearest(const DevScene &s, f3 origin, f3 ray, int *s
   for (;;) {
     if (ref < 0) {
       const int2 leaf = s.leaves[~ref];
+      cnt.leaf();
       for (int i = 0; i < leaf.y; ++i) {
         const int id = leaf.x + i;
         float t, u, v;
+        cnt.triangle();
         if (intersect(s.triangles + 3 * (size_t)id, ray, origin, t, u, v) && t < closest) {
           closest = t; h.hit = 1; h.tri = id; h.t = t; h.u = u; h.v = v;
         }
       }
     } else {
       float d0, d1; int l, r;
+      cnt.pair();
       const int mask = test_pair(s.pairs + 4 * (size_t)ref, origin, idir, kRayEps, closest, d0, d1, l, r);
       if (mask == 3) {
         if (sp >= s.stack_depth) { h.hit = -1; return h; }
@@ -201,21 +219,25 @@ __device__ inline Hit trace_nearest(const DevScene &s, f3 origin, f3 ray, int *s
 // Any hit toward the light skipping the origin's own triangle
 // (trace_shadow_ray_any_hit, device_ray.hpp:81-135).  Overflow counts as
 // occluded, as in the reference.
+template <class Count>
 __device__ inline bool trace_any(const DevScene &s, f3 origin, f3 dir, int self, float t_min,
-                                 float t_max, int *stack, int stride) {
+                                 float t_max, int *stack, int stride, Count &cnt) {
   const f3 idir = inverse_direction(dir);
   int ref = s.root_ref, sp = 0;
   for (;;) {
     if (ref < 0) {
       const int2 leaf = s.leaves[~ref];
+      cnt.leaf();
       for (int i = 0; i < leaf.y; ++i) {
         const int id = leaf.x + i;
         if (id == self) continue;
         float t, u, v;
+        cnt.triangle();
         if (intersect(s.triangles + 3 * (size_t)id, dir, origin, t, u, v) && t > t_min && t < t_max) return true;
       }
     } else {
       float d0, d1; int l, r;
+      cnt.pair();
       const int mask = test_pair(s.pairs + 4 * (size_t)ref, origin, idir, t_min, t_max, d0, d1, l, r);
       if (mask == 3) {
         if (sp >= s.stack_depth) return true;
@@ -358,16 +380,20 @@ struct SubSample {
   int invalid;   // traversal overflow
   int shadow;    // a shadow ray was traced
 };
-template <bool SHADE>
+template <bool SHADE, class Count = NoCount>
 __device__ inline SubSample sample_subray(const DevScene &s, int pixel, const Pose &pose, int sx, int sy,
-                                          int *stack, int stride) {
+                                          int *stack, int stride, Count *primary = nullptr,
+                                          Count *shadow = nullptr) {
+  Count unused_primary, unused_shadow;
+  Count &cp = primary ? *primary : unused_primary;
+  Count &cs = shadow ? *shadow : unused_shadow;
   SubSample out{0.0, -1.f, 0, 0};
   const int ss = s.samples;
   const float px = (pixel % s.width + (sx + .5f) / ss - .5f - s.cx) / s.fx;
   const float py = (pixel / s.width + (sy + .5f) / ss - .5f - s.cy) / s.fy;
   const f3 ray = normalize(rotate_ray(pose, mk(px, -py, -1.f)));
   const f3 origin = mk(pose.px, pose.py, pose.pz);
-  const Hit hit = trace_nearest(s, origin, ray, stack, stride);
+  const Hit hit = trace_nearest(s, origin, ray, stack, stride, cp);
   if (hit.hit < 0) { out.invalid = 1; return out; }
   if (!hit.hit) return out;
   out.depth = hit.t;
@@ -384,7 +410,7 @@ __device__ inline SubSample sample_subray(const DevScene &s, int pixel, const Po
   const float4 *rec = s.triangles + 3 * (size_t)hit.tri;
   const f3 so = shadow_origin(rec, hit.u, hit.v, light);
   out.shadow = 1;
-  if (trace_any(s, so, light, hit.tri, kRayEps, 1e30f, stack, stride)) return out;
+  if (trace_any(s, so, light, hit.tri, kRayEps, 1e30f, stack, stride, cs)) return out;
   const float albedo = s.albedos ? material_albedo(s.albedos[i0], s.hapke.w) * w +
                                        material_albedo(s.albedos[i1], s.hapke.w) * hit.u +
                                        material_albedo(s.albedos[i2], s.hapke.w) * hit.v

@@ -283,6 +283,25 @@ int dvs_frame_render_device(dvs_scene s, const dvs_optical_state *device_poses, 
   });
 }
 
+int dvs_frame_traversal_stats(dvs_scene s, const dvs_optical_state *pose, uint64_t totals[8]) {
+  return guarded([&] {
+    if (!s || !pose || !totals) fail(DVS_E_ARGUMENT, "invalid traversal statistics request");
+    hipStream_t st = s->stream;
+    DeviceArray<dvs_optical_state> state;
+    state.upload(pose, 1, "upload optical pose", st);
+    s->poses.reserve(1, "optical poses");
+    hip_check(launch_state_poses(1, state.ptr, s->poses.ptr, st), "pose conversion");
+    DeviceArray<unsigned long long> acc;
+    acc.reserve(8, "traversal counters");
+    hip_check(hipMemsetAsync(acc.ptr, 0, 8 * sizeof(unsigned long long), st), "clear traversal counters");
+    hip_check(launch_traversal_stats(s->view, s->poses.ptr, s->view.width * s->view.height, acc.ptr, st),
+              "traversal statistics");
+    hip_check(hipMemcpyAsync(totals, acc.ptr, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st),
+              "traversal statistics");
+    hip_check(hipStreamSynchronize(st), "traversal statistics");
+  });
+}
+
 int dvs_scene_stats(dvs_scene s, dvs_stats *out) {
   return guarded([&] {
     if (!s || !out) fail(DVS_E_ARGUMENT, "null scene");

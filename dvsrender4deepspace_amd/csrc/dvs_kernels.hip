@@ -243,12 +243,40 @@ __global__ __launch_bounds__(256) void rays_kernel(DevScene s, int64_t n, const 
   if (i >= n) return;
   const f3 origin = mk(o[3 * i], o[3 * i + 1], o[3 * i + 2]);
   const f3 dir = mk(d[3 * i], d[3 * i + 1], d[3 * i + 2]);
+  NoCount nc;
   if (shadow) {
-    hit[i] = trace_any(s, origin, dir, self[i], kRayEps, 1e30f, stack, 256) ? 1 : 0;
+    hit[i] = trace_any(s, origin, dir, self[i], kRayEps, 1e30f, stack, 256, nc) ? 1 : 0;
     return;
   }
-  const Hit h = trace_nearest(s, origin, dir, stack, 256);
+  const Hit h = trace_nearest(s, origin, dir, stack, 256, nc);
   hit[i] = h.hit; tri[i] = h.tri; t[i] = h.t; u[i] = h.u; v[i] = h.v;
+}
+
+// Traversal work of every sub-ray of one frame (diagnostic): counts of
+// node-pair visits, leaf visits and triangle tests for primary and shadow
+// rays, summed per wave then atomically.
+__global__ __launch_bounds__(256) void traversal_stats_kernel(DevScene s, const Pose *pose, int npix,
+                                                              unsigned long long *totals) {
+  extern __shared__ double lds_words[];
+  int *stack = reinterpret_cast<int *>(lds_words) + threadIdx.x;
+  const int R = s.samples * s.samples;
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  WorkCount cp, cs;
+  int shadow = 0, primary = 0;
+  if (g < (long long)npix * R) {
+    const int pixel = (int)(g / R), sub = (int)(g - (long long)pixel * R);
+    const SubSample r = sample_subray<true, WorkCount>(s, pixel, *pose, sub % s.samples, sub / s.samples, stack, 256,
+                                                       &cp, &cs);
+    shadow = r.shadow;
+    primary = 1;
+  }
+  unsigned long long v[8] = {(unsigned long long)primary, cp.pairs, cp.leaves, cp.triangles,
+                             (unsigned long long)shadow, cs.pairs, cs.leaves, cs.triangles};
+  for (int k = 0; k < 8; ++k) {
+    unsigned long long x = v[k];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
+    if ((threadIdx.x & 63) == 0 && x) atomicAdd(totals + k, x);
+  }
 }
 
 }  // namespace dvsdev
@@ -345,6 +373,13 @@ hipError_t launch_noise(uint64_t seed, int64_t pixels, double lambda, int draws,
                         hipStream_t st) {
   if (pixels <= 0) return hipSuccess;
   noise_kernel<<<(unsigned)((pixels + 255) / 256), 256, 0, st>>>(seed, pixels, lambda, draws, poisson, normal);
+  return hipGetLastError();
+}
+hipError_t launch_traversal_stats(const DevScene &s, const Pose *pose, int npix, unsigned long long *totals,
+                                  hipStream_t st) {
+  const long long items = (long long)npix * s.samples * s.samples;
+  const size_t lds = (size_t)s.stack_depth * 256 * sizeof(int);
+  traversal_stats_kernel<<<(unsigned)((items + 255) / 256), 256, lds, st>>>(s, pose, npix, totals);
   return hipGetLastError();
 }
 hipError_t launch_rays(const DevScene &s, int64_t n, const float *o, const float *d, const int *self, int shadow,

@@ -42,6 +42,8 @@ hipError_t launch_receptor(int64_t n, const double *u, const double *rate, const
                            double *out, hipStream_t st);
 hipError_t launch_noise(uint64_t seed, int64_t pixels, double lambda, int draws, double *poisson, double *normal,
                         hipStream_t st);
+hipError_t launch_traversal_stats(const DevScene &s, const Pose *pose, int npix, unsigned long long *totals,
+                                  hipStream_t st);
 hipError_t launch_rays(const DevScene &s, int64_t n, const float *o, const float *d, const int *self, int shadow,
                        int *hit, int *tri, float *t, float *u, float *v, hipStream_t st);
 }  // namespace dvsamd

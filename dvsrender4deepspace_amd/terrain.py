@@ -83,9 +83,12 @@ def heightfield(spec: TerrainSpec) -> np.ndarray:
     for _ in range(spec.craters):
         r = rng.uniform(*spec.crater_radius)
         cx, cy = rng.uniform(r, extent - r, size=2) if extent > 2 * r else (extent / 2, extent / 2)
-        d2 = ((x - cx) ** 2 + (y - cy) ** 2) / (r * r)
-        inside = d2 < 1.0
-        h[inside] -= spec.depth_ratio * r * (1.0 - d2[inside])
+        # parabolic bowl of depth depth_ratio * r, applied on its bounding window
+        c0, c1 = max(int(np.floor((cx - r) / spec.spacing)), 0), min(int(np.ceil((cx + r) / spec.spacing)) + 1, n)
+        r0, r1 = max(int(np.floor((cy - r) / spec.spacing)), 0), min(int(np.ceil((cy + r) / spec.spacing)) + 1, n)
+        wx, wy = x[r0:r1, c0:c1], y[r0:r1, c0:c1]
+        d2 = ((wx - cx) ** 2 + (wy - cy) ** 2) / (r * r)
+        h[r0:r1, c0:c1] -= np.where(d2 < 1.0, spec.depth_ratio * r * (1.0 - d2), 0.0)
     return h
 
 

@@ -155,6 +155,11 @@ int dvs_frame_render(dvs_scene scene, const dvs_optical_state *poses, int32_t po
 int dvs_frame_render_device(dvs_scene scene, const dvs_optical_state *device_poses,
                             int32_t pose_count, float *device_image, float *device_depth);
 int dvs_scene_stats(dvs_scene scene, dvs_stats *out);
+/* Traversal work of every sub-ray of one frame, for the roofline's
+ * algorithmic bytes per ray (no reference counterpart): totals[0..7] =
+ * primary rays, their node-pair visits, leaf visits and triangle tests, then
+ * shadow rays, their node-pair visits, leaf visits and triangle tests. */
+int dvs_frame_traversal_stats(dvs_scene scene, const dvs_optical_state *pose, uint64_t totals[8]);
 
 /* ---- Event sensor ----------------------------------------------------
  * EventSensor(width, height, PhysicConfig) (src/cuda/event_sensor.cu:244-260).
