@@ -84,7 +84,18 @@ def main() -> int:
     ap.add_argument("--horizons", type=int, default=64, help="host horizons per step")
     ap.add_argument("--cpu-horizons", type=int, default=2, help="horizons in the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run only a plain 1 GiB device-to-device copy (calibrates rocprofv3 byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": a.numel() * 4, "copy_probe_bytes_written": b.numel() * 4}))
+        return 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU layout is "
               "described in DESIGN.md and not launched in this build", file=sys.stderr)
