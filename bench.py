@@ -115,7 +115,7 @@ def main() -> int:
     info = scene.info()
     trav = scene.traversal_stats(w.initial_state())
     b_primary, b_shadow = bytes_per_ray(trav)
-    log(f"scene on device: {info}; bytes/ray primary {b_primary:.0f}, shadow {b_shadow:.0f}")
+    log(f"scene on device: {info}; bytes/ray primary {b_primary:.0f}, shadow {b_shadow:.0f}; per-frame traversal {trav}")
 
     sensor = capi.Sensor(scene, w.physic)
     sensor.initialize(w.initial_state(), w.start_time)
@@ -161,7 +161,7 @@ def main() -> int:
                    "supersample": w.camera.samples, "terrain_triangles": int(len(idx)),
                    "horizons_per_step": args.horizons, "time_resolution_s": w.time_resolution,
                    "circuit_step_s": w.physic.circuit_step, "noise": "shot+reset on", "solver": "native",
-                   "bvh_leaf_size": w.leaf_size},
+                   "bvh_leaf_size": w.leaf_size, "device_leaf_size": 4},
         "events_per_s": round(st["comparator_events"] / wall, 1),
         "delivered_events_per_s": round(delivered / wall, 1),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},

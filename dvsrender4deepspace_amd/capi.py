@@ -160,6 +160,9 @@ def load_library() -> ctypes.CDLL:
         "dvs_bvh_validate": (ctypes.c_int, [_P, _I64, _I32]),
         "dvs_scene_create": (ctypes.c_int, [_P, _I64, _I32, _P, _P, _I64, _P, _I64, _P, ctypes.POINTER(_CameraC),
                                             ctypes.POINTER(_HapkeC), ctypes.POINTER(_P)]),
+        "dvs_scene_create_ex": (ctypes.c_int, [_P, _I64, _I32, _P, _P, _I64, _P, _I64, _P,
+                                               ctypes.POINTER(_CameraC), ctypes.POINTER(_HapkeC), _I32,
+                                               ctypes.POINTER(_P)]),
         "dvs_scene_destroy": (ctypes.c_int, [_P]),
         "dvs_scene_info": (ctypes.c_int, [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I64), ctypes.POINTER(_I64),
                                           ctypes.POINTER(_I64)]),
@@ -245,7 +248,7 @@ class Scene:
     """DeviceScene + CamRayCaster over one uploaded BVH and mesh."""
 
     def __init__(self, nodes, root, vertices, normals, indices, camera: Camera, hapke: Hapke | None = None,
-                 albedos=None):
+                 albedos=None, device_leaf_size: int | None = None):
         lib = load_library()
         self._keep = [np.ascontiguousarray(nodes, dtype=NODE_DTYPE),
                       np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3),
@@ -258,8 +261,13 @@ class Scene:
         handle = _P()
         cam = _struct(_CameraC, camera)
         hk = _struct(_HapkeC, self.hapke)
-        _check(lib.dvs_scene_create(_ptr(n), len(n), int(root), _ptr(v), _ptr(nr), len(v), _ptr(idx), len(idx),
-                                    _ptr(alb), ctypes.byref(cam), ctypes.byref(hk), ctypes.byref(handle)))
+        if device_leaf_size is None:
+            _check(lib.dvs_scene_create(_ptr(n), len(n), int(root), _ptr(v), _ptr(nr), len(v), _ptr(idx), len(idx),
+                                        _ptr(alb), ctypes.byref(cam), ctypes.byref(hk), ctypes.byref(handle)))
+        else:
+            _check(lib.dvs_scene_create_ex(_ptr(n), len(n), int(root), _ptr(v), _ptr(nr), len(v), _ptr(idx),
+                                           len(idx), _ptr(alb), ctypes.byref(cam), ctypes.byref(hk),
+                                           int(device_leaf_size), ctypes.byref(handle)))
         self.handle = handle
         self._keep = None
 

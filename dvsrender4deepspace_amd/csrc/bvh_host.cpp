@@ -15,6 +15,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <stdexcept>
 #include <vector>
 
@@ -235,64 +236,103 @@ std::string validate_bvh(const dvs_bvh_node *nodes, int64_t count, int32_t root)
 }
 
 SoaBvh relayout(const dvs_bvh_node *nodes, int64_t count, int32_t root, const float *vertices,
-                const uint32_t *indices, int64_t triangle_count) {
+                const uint32_t *indices, int64_t triangle_count, int device_leaf_size) {
   SoaBvh out;
-  std::vector<int32_t> ref(static_cast<size_t>(count), 0);
-  // Pre-order numbering, left child first: internal nodes become pair records,
-  // leaves become (start, count) ranges.  Child refs >= 0 name pair records,
-  // refs < 0 are ~leaf_index.
-  struct Item { int32_t node, depth; };
-  std::vector<Item> pending{{root, 0}};
-  std::vector<int32_t> internal_order;
-  int32_t pairs = 0, leaves = 0;
-  while (!pending.empty()) {
-    Item it = pending.back();
-    pending.pop_back();
-    const dvs_bvh_node &n = nodes[it.node];
-    if (n.count > 0) {
-      ref[it.node] = ~leaves++;
-      out.leaf_ranges.push_back(n.start);
-      out.leaf_ranges.push_back(n.count);
-      if (n.start < 0 || static_cast<int64_t>(n.start) + n.count > triangle_count)
-        throw std::invalid_argument("BVH leaf range outside triangle array");
-      out.max_depth = std::max(out.max_depth, it.depth);
-      continue;
+  // Box inflation of the device-only sub-boxes: far above the float rounding
+  // of the slab test and of the Moller-Trumbore distance (a few ulps of the
+  // coordinates and of t), so a sub-box never hides a triangle the
+  // reference's full leaf scan would accept.
+  float extent = 1.0f;
+  for (int64_t t = 0; t < 3 * triangle_count; ++t)
+    for (int k = 0; k < 3; ++k) extent = std::max(extent, std::fabs(vertices[3 * static_cast<size_t>(indices[t]) + k]));
+  const float margin = 8e-6f * (extent + 1.0f);
+
+  auto new_pair = [&]() {
+    out.pairs.resize(out.pairs.size() + 16, 0.0f);
+    return static_cast<int32_t>(out.pairs.size() / 16 - 1);
+  };
+  auto set_child = [&](int32_t pair, int side, const float lo[3], const float hi[3], int32_t ref) {
+    float *rec = &out.pairs[16 * static_cast<size_t>(pair)];
+    float *box = rec + 6 * side;
+    box[0] = lo[0]; box[1] = lo[1]; box[2] = lo[2]; box[3] = hi[0]; box[4] = hi[1]; box[5] = hi[2];
+    std::memcpy(&rec[12 + side], &ref, 4);
+  };
+  auto corner = [&](int32_t tri, int c) { return vertices + 3 * static_cast<size_t>(indices[3 * static_cast<size_t>(tri) + c]); };
+  auto make_leaf = [&](const std::vector<int32_t> &ids, int32_t lo, int32_t hi, int depth) {
+    const int32_t first = static_cast<int32_t>(out.triangles.size() / 12);
+    for (int32_t id : ids) {
+      const float *a = corner(id, 0), *b = corner(id, 1), *c = corner(id, 2);
+      float rec[12] = {a[0], a[1], a[2], b[0] - a[0], b[1] - a[1], b[2] - a[2], c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+      std::memcpy(&rec[9], &id, 4);
+      std::memcpy(&rec[10], &lo, 4);
+      std::memcpy(&rec[11], &hi, 4);
+      out.triangles.insert(out.triangles.end(), rec, rec + 12);
     }
-    ref[it.node] = pairs++;
-    internal_order.push_back(it.node);
-    pending.push_back({n.right, it.depth + 1});
-    pending.push_back({n.left, it.depth + 1});
-  }
-  out.root_ref = ref[root];
-  out.internal_nodes = pairs;
-  out.leaf_count = leaves;
-  out.pairs.resize(16 * static_cast<size_t>(pairs));
-  for (int32_t p = 0; p < pairs; ++p) {
-    const dvs_bvh_node &n = nodes[internal_order[p]];
-    const dvs_bvh_node &l = nodes[n.left];
-    const dvs_bvh_node &r = nodes[n.right];
-    float *rec = &out.pairs[16 * static_cast<size_t>(p)];
-    // 64-byte record: left box, right box, left ref, right ref, 2 spare words.
-    rec[0] = l.min[0]; rec[1] = l.min[1]; rec[2] = l.min[2]; rec[3] = l.max[0];
-    rec[4] = l.max[1]; rec[5] = l.max[2]; rec[6] = r.min[0]; rec[7] = r.min[1];
-    rec[8] = r.min[2]; rec[9] = r.max[0]; rec[10] = r.max[1]; rec[11] = r.max[2];
-    std::memcpy(&rec[12], &ref[n.left], 4);
-    std::memcpy(&rec[13], &ref[n.right], 4);
-    rec[14] = 0; rec[15] = 0;
-  }
-  // 48-byte triangle records in BVH order: v0, e1 = v1 - v0, e2 = v2 - v0.
-  out.triangles.resize(12 * static_cast<size_t>(triangle_count));
-  for (int64_t t = 0; t < triangle_count; ++t) {
-    const uint32_t *tri = indices + 3 * t;
-    const float *a = vertices + 3 * static_cast<size_t>(tri[0]);
-    const float *b = vertices + 3 * static_cast<size_t>(tri[1]);
-    const float *c = vertices + 3 * static_cast<size_t>(tri[2]);
-    float *rec = &out.triangles[12 * t];
-    rec[0] = a[0]; rec[1] = a[1]; rec[2] = a[2];
-    rec[3] = b[0] - a[0]; rec[4] = b[1] - a[1]; rec[5] = b[2] - a[2];
-    rec[6] = c[0] - a[0]; rec[7] = c[1] - a[1]; rec[8] = c[2] - a[2];
-    rec[9] = 0; rec[10] = 0; rec[11] = 0;
-  }
+    out.leaf_ranges.push_back(first);
+    out.leaf_ranges.push_back(static_cast<int32_t>(ids.size()));
+    out.max_depth = std::max(out.max_depth, depth);
+    return ~static_cast<int32_t>(out.leaf_ranges.size() / 2 - 1);
+  };
+  auto bounds = [&](const int32_t *ids, size_t n, float lo[3], float hi[3]) {
+    for (int k = 0; k < 3; ++k) { lo[k] = FLT_MAX; hi[k] = -FLT_MAX; }
+    for (size_t i = 0; i < n; ++i)
+      for (int c = 0; c < 3; ++c) {
+        const float *p = corner(ids[i], c);
+        for (int k = 0; k < 3; ++k) { lo[k] = std::min(lo[k], p[k]); hi[k] = std::max(hi[k], p[k]); }
+      }
+    for (int k = 0; k < 3; ++k) { lo[k] -= margin; hi[k] += margin; }
+  };
+  // Device-only subtree over one reference leaf: median splits on the longest
+  // centroid axis down to device_leaf_size triangles.
+  std::function<int32_t(std::vector<int32_t> &, int32_t, int32_t, int)> split =
+      [&](std::vector<int32_t> &ids, int32_t lo, int32_t hi, int depth) -> int32_t {
+    if (static_cast<int>(ids.size()) <= device_leaf_size) return make_leaf(ids, lo, hi, depth);
+    float cmin[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, cmax[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    auto centre = [&](int32_t id, int k) { return corner(id, 0)[k] + corner(id, 1)[k] + corner(id, 2)[k]; };
+    for (int32_t id : ids)
+      for (int k = 0; k < 3; ++k) { cmin[k] = std::min(cmin[k], centre(id, k)); cmax[k] = std::max(cmax[k], centre(id, k)); }
+    int axis = 0;
+    for (int k = 1; k < 3; ++k)
+      if (cmax[k] - cmin[k] > cmax[axis] - cmin[axis]) axis = k;
+    std::sort(ids.begin(), ids.end(), [&](int32_t a, int32_t b) {
+      const float ca = centre(a, axis), cb = centre(b, axis);
+      return ca < cb || (ca == cb && a < b);
+    });
+    std::vector<int32_t> left(ids.begin(), ids.begin() + ids.size() / 2), right(ids.begin() + ids.size() / 2, ids.end());
+    const int32_t pair = new_pair();
+    float llo[3], lhi[3], rlo[3], rhi[3];
+    bounds(left.data(), left.size(), llo, lhi);
+    bounds(right.data(), right.size(), rlo, rhi);
+    const int32_t l = split(left, lo, hi, depth + 1);
+    const int32_t r = split(right, lo, hi, depth + 1);
+    set_child(pair, 0, llo, lhi, l);
+    set_child(pair, 1, rlo, rhi, r);
+    return pair;
+  };
+  auto leaf_ref = [&](const dvs_bvh_node &n, int depth) -> int32_t {
+    if (n.start < 0 || static_cast<int64_t>(n.start) + n.count > triangle_count)
+      throw std::invalid_argument("BVH leaf range outside triangle array");
+    std::vector<int32_t> ids(n.count);
+    for (int32_t i = 0; i < n.count; ++i) ids[i] = n.start + i;
+    if (device_leaf_size <= 0 || n.count <= device_leaf_size) return make_leaf(ids, n.start, n.start + n.count, depth);
+    return split(ids, n.start, n.start + n.count, depth);
+  };
+  // Reference tree in pre-order (left first); its internal nodes become pair
+  // records holding both children's reference boxes.
+  std::function<int32_t(int32_t, int)> emit = [&](int32_t node, int depth) -> int32_t {
+    const dvs_bvh_node &n = nodes[node];
+    if (n.count > 0) return leaf_ref(n, depth);
+    const int32_t pair = new_pair();
+    const int32_t l = emit(n.left, depth + 1);
+    const int32_t r = emit(n.right, depth + 1);
+    set_child(pair, 0, nodes[n.left].min, nodes[n.left].max, l);
+    set_child(pair, 1, nodes[n.right].min, nodes[n.right].max, r);
+    return pair;
+  };
+  out.triangles.reserve(12 * static_cast<size_t>(triangle_count));
+  out.root_ref = emit(root, 0);
+  out.internal_nodes = static_cast<int64_t>(out.pairs.size() / 16);
+  out.leaf_count = static_cast<int64_t>(out.leaf_ranges.size() / 2);
   return out;
 }
 

@@ -28,7 +28,14 @@ std::string validate_bvh(const dvs_bvh_node *nodes, int64_t count, int32_t root)
 //              child refs (int bits); a ref >= 0 names a pair record, a ref < 0
 //              is ~leaf_index.
 //  leaf_ranges: (start, count) per leaf.
-//  triangles:  12 floats (48 B) per BVH-ordered triangle: v0, v1-v0, v2-v0.
+//  triangles:  12 floats (48 B) per triangle record: v0, v1-v0, v2-v0, then
+//              (int bits) the reference's BVH-order triangle index and the
+//              [lo, hi) index range of the reference leaf holding it.
+//
+// device_leaf_size > 0 subdivides every reference leaf with more triangles
+// into a device-only subtree (median splits, boxes inflated beyond rounding);
+// the traversal then resolves equal-distance hits inside one reference leaf
+// to the lowest reference index, as the reference's sequential leaf scan does.
 struct SoaBvh {
   std::vector<float> pairs;
   std::vector<int32_t> leaf_ranges;
@@ -39,6 +46,6 @@ struct SoaBvh {
 };
 
 SoaBvh relayout(const dvs_bvh_node *nodes, int64_t count, int32_t root, const float *vertices,
-                const uint32_t *indices, int64_t triangle_count);
+                const uint32_t *indices, int64_t triangle_count, int device_leaf_size);
 
 }  // namespace dvsamd

@@ -140,9 +140,12 @@ __device__ __forceinline__ int test_pair(const float4 *rec, f3 P, f3 idir, float
   left = __float_as_int(r.x); right = __float_as_int(r.y);
   return (t1max >= t1min ? 1 : 0) | (t2max >= t2min ? 2 : 0);
 }
-// Moller-Trumbore on a pre-gathered record (device_ray.hpp:54-80).
-__device__ __forceinline__ bool intersect(const float4 *rec, f3 ray, f3 o, float &ot, float &ou, float &ov) {
+// Moller-Trumbore on a pre-gathered record (device_ray.hpp:54-80).  `meta`
+// receives the record's last float4: (e2.z, reference index, leaf lo, leaf hi).
+__device__ __forceinline__ bool intersect(const float4 *rec, f3 ray, f3 o, float &ot, float &ou, float &ov,
+                                          float4 &meta) {
   const float4 a = rec[0], b = rec[1], c = rec[2];
+  meta = c;
   const f3 v0 = mk(a.x, a.y, a.z), e1 = mk(a.w, b.x, b.y), e2 = mk(b.z, b.w, c.x);
   const f3 pvec = cross(ray, e2);
   const float det = dot(e1, pvec);
@@ -160,7 +163,8 @@ __device__ __forceinline__ bool intersect(const float4 *rec, f3 ray, f3 o, float
   return true;
 }
 
-struct Hit { int hit, tri; float t, u, v; };
+// tri: the reference's BVH-order triangle index; rec: the device record.
+struct Hit { int hit, tri, rec; float t, u, v; };
 
 // Traversal work counters: the hot path uses NoCount (compiled away); the
 // diagnostic traversal-statistics kernel uses WorkCount.
@@ -180,7 +184,7 @@ struct WorkCount {
 // this lane's first LDS slot, consecutive slots `stride` ints apart.
 template <class Count>
 __device__ inline Hit trace_nearest(const DevScene &s, f3 origin, f3 ray, int *stack, int stride, Count &cnt) {
-  Hit h{0, -1, 0.f, 0.f, 0.f};
+  Hit h{0, -1, -1, 0.f, 0.f, 0.f};
   float closest = 1e30f;
   const f3 idir = inverse_direction(ray);
   int ref = s.root_ref, sp = 0;
@@ -189,11 +193,17 @@ __device__ inline Hit trace_nearest(const DevScene &s, f3 origin, f3 ray, int *s
       const int2 leaf = s.leaves[~ref];
       cnt.leaf();
       for (int i = 0; i < leaf.y; ++i) {
-        const int id = leaf.x + i;
+        const int pos = leaf.x + i;
         float t, u, v;
+        float4 meta;
         cnt.triangle();
-        if (intersect(s.triangles + 3 * (size_t)id, ray, origin, t, u, v) && t < closest) {
-          closest = t; h.hit = 1; h.tri = id; h.t = t; h.u = u; h.v = v;
+        if (!intersect(s.triangles + 3 * (size_t)pos, ray, origin, t, u, v, meta)) continue;
+        // Strictly nearer wins; an equal distance wins only against a hit of
+        // the same reference leaf with a higher index (its sequential scan).
+        const int id = __float_as_int(meta.y);
+        if (t < closest || (t == closest && id < h.tri && h.tri >= __float_as_int(meta.z) &&
+                            h.tri < __float_as_int(meta.w))) {
+          closest = t; h.hit = 1; h.tri = id; h.rec = pos; h.t = t; h.u = u; h.v = v;
         }
       }
     } else {
@@ -229,11 +239,12 @@ __device__ inline bool trace_any(const DevScene &s, f3 origin, f3 dir, int self,
       const int2 leaf = s.leaves[~ref];
       cnt.leaf();
       for (int i = 0; i < leaf.y; ++i) {
-        const int id = leaf.x + i;
-        if (id == self) continue;
+        const float4 *rec = s.triangles + 3 * (size_t)(leaf.x + i);
+        if (__float_as_int(rec[2].y) == self) continue;
         float t, u, v;
+        float4 meta;
         cnt.triangle();
-        if (intersect(s.triangles + 3 * (size_t)id, dir, origin, t, u, v) && t > t_min && t < t_max) return true;
+        if (intersect(rec, dir, origin, t, u, v, meta) && t > t_min && t < t_max) return true;
       }
     } else {
       float d0, d1; int l, r;
@@ -407,7 +418,7 @@ __device__ inline SubSample sample_subray(const DevScene &s, int pixel, const Po
   const f3 normal = normalize(add(add(scale(n0, w), scale(n1, hit.u)), scale(n2, hit.v)));
   const f3 light = normalize(mk(pose.lx, pose.ly, pose.lz));
   if (dot(normal, light) <= 0 || dot(normal, neg(ray)) <= 0) return out;
-  const float4 *rec = s.triangles + 3 * (size_t)hit.tri;
+  const float4 *rec = s.triangles + 3 * (size_t)hit.rec;
   const f3 so = shadow_origin(rec, hit.u, hit.v, light);
   out.shadow = 1;
   if (trace_any(s, so, light, hit.tri, kRayEps, 1e30f, stack, stride, cs)) return out;

@@ -190,7 +190,16 @@ int dvs_bvh_validate(const dvs_bvh_node *nodes, int64_t node_count, int32_t root
 int dvs_scene_create(const dvs_bvh_node *nodes, int64_t node_count, int32_t root, const float *vertices,
                      const float *normals, int64_t vertex_count, const uint32_t *indices, int64_t triangle_count,
                      const float *albedos, const dvs_camera *camera, const dvs_hapke *hapke, dvs_scene *out) {
+  return dvs_scene_create_ex(nodes, node_count, root, vertices, normals, vertex_count, indices, triangle_count,
+                             albedos, camera, hapke, DVS_DEFAULT_DEVICE_LEAF_SIZE, out);
+}
+
+int dvs_scene_create_ex(const dvs_bvh_node *nodes, int64_t node_count, int32_t root, const float *vertices,
+                        const float *normals, int64_t vertex_count, const uint32_t *indices, int64_t triangle_count,
+                        const float *albedos, const dvs_camera *camera, const dvs_hapke *hapke,
+                        int32_t device_leaf_size, dvs_scene *out) {
   return guarded([&] {
+    if (device_leaf_size < 0) fail(DVS_E_ARGUMENT, "device leaf size must be >= 0");
     if (!out || !camera || !hapke) fail(DVS_E_ARGUMENT, "invalid scene arguments");
     *out = nullptr;
     // DeviceScene checks (src/cuda/device_scene.cu:16-22).
@@ -203,7 +212,7 @@ int dvs_scene_create(const dvs_bvh_node *nodes, int64_t node_count, int32_t root
       if (indices[i] >= (uint64_t)vertex_count) fail(DVS_E_ARGUMENT, "mesh triangle index outside vertex array");
     if (camera->width <= 0 || camera->height <= 0 || camera->samples < 1 || camera->samples > 16)
       fail(DVS_E_ARGUMENT, "camera grid must be positive with 1..16 sub-rays per axis");
-    SoaBvh soa = relayout(nodes, node_count, root, vertices, indices, triangle_count);
+    SoaBvh soa = relayout(nodes, node_count, root, vertices, indices, triangle_count, device_leaf_size);
     if (soa.max_depth > 64) fail(DVS_E_ARGUMENT, "BVH deeper than the 64-entry device traversal stack");
     std::unique_ptr<dvs_scene_s> s(new dvs_scene_s());
     hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "scene stream");

@@ -137,6 +137,18 @@ int dvs_scene_create(const dvs_bvh_node *nodes, int64_t node_count, int32_t root
                      const float *vertices, const float *normals, int64_t vertex_count,
                      const uint32_t *indices, int64_t triangle_count, const float *albedos,
                      const dvs_camera *camera, const dvs_hapke *hapke, dvs_scene *out);
+/* Same with the device leaf size chosen: every reference leaf larger than
+ * device_leaf_size triangles is subdivided into a device-only subtree
+ * (0 keeps the reference leaves).  Results are those of the reference's leaf
+ * scan: sub-boxes are inflated beyond rounding and equal-distance hits inside
+ * one reference leaf resolve to the lowest reference index.
+ * dvs_scene_create uses DVS_DEFAULT_DEVICE_LEAF_SIZE. */
+#define DVS_DEFAULT_DEVICE_LEAF_SIZE 4
+int dvs_scene_create_ex(const dvs_bvh_node *nodes, int64_t node_count, int32_t root,
+                        const float *vertices, const float *normals, int64_t vertex_count,
+                        const uint32_t *indices, int64_t triangle_count, const float *albedos,
+                        const dvs_camera *camera, const dvs_hapke *hapke, int32_t device_leaf_size,
+                        dvs_scene *out);
 /* ~DeviceScene (device_scene.cu:38-43).  The scene is reference counted:
  * sensors created on it keep it alive. */
 int dvs_scene_destroy(dvs_scene scene);

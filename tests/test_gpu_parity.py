@@ -78,9 +78,14 @@ def test_terrain_frame_parity(oracle, gpu, name):
         _frame_check(oracle, g, o, w.states(np.array([t])))
 
 
-def test_traversal_bit_exact(oracle, gpu):
+@pytest.mark.parametrize("device_leaf", [0, 1, 4, 16])
+def test_traversal_bit_exact(oracle, gpu, device_leaf):
+    """Nearest hits (triangle, t, u, v) and any-hits equal the reference
+    traversal of the leaf-64 tree, with or without device leaf subdivision."""
     w = workloads.PLAIN_SMALL
-    g, o = _pair(w, oracle, gpu)
+    v, n, nodes, root, idx = scenes.workload_mesh(w)
+    g = gpu.Scene(nodes, root, v, n, idx, w.camera, w.hapke, device_leaf_size=device_leaf)
+    o = oracle.Scene(nodes, root, v, n, idx, w.camera, w.hapke)
     rng = np.random.default_rng(7)
     n = 4096
     origins = np.stack([rng.uniform(-80, 80, n), rng.uniform(-80, 80, n), rng.uniform(30, 250, n)], 1).astype(np.float32)
@@ -95,6 +100,18 @@ def test_traversal_bit_exact(oracle, gpu):
             hits += 1
             assert (tri[i], t[i], u[i], v[i]) == (otri, np.float32(ot), np.float32(ou), np.float32(ov))
     assert hits > n // 4
+    # rays aimed at shared mesh vertices: up to six triangles tie at one point
+    verts = scenes.workload_mesh(w)[0]
+    pick = rng.choice(len(verts), 1024, replace=False)
+    org2 = (verts[pick] + np.array([3.0, -2.0, 150.0], np.float32)).astype(np.float32)
+    dir2 = verts[pick] - org2
+    dir2 = (dir2 / np.linalg.norm(dir2, axis=1, keepdims=True)).astype(np.float32)
+    hit2, tri2, t2, u2, v2 = g.trace(org2, dir2)
+    for i in range(len(pick)):
+        oh, otri, ot, ou, ov = o.trace(org2[i], dir2[i])
+        assert hit2[i] == oh
+        if oh == 1:
+            assert (tri2[i], t2[i], u2[i], v2[i]) == (otri, np.float32(ot), np.float32(ou), np.float32(ov))
     # shadow rays toward the sun from just above the surface, skipping no triangle
     light = np.tile(np.array([np.cos(np.pi / 6), 0, np.sin(np.pi / 6)], np.float32), (n, 1))
     lifted = origins.copy()
