@@ -60,7 +60,10 @@ struct NoiseStream {
       : seed(s), pixel(p), stream(st), block(b), used(4) {}
   __device__ double uniform() {  // open interval (0, 1), 53 bits
     if (used == 4) { philox4x64(block++, pixel, 0, 0, seed, kStreamTag ^ stream, buf); used = 0; }
-    return ((double)(buf[used++] >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    // select without a dynamic index so the block stays in registers
+    const uint64_t x = used == 0 ? buf[0] : used == 1 ? buf[1] : used == 2 ? buf[2] : buf[3];
+    ++used;
+    return ((double)(x >> 11) + 0.5) * (1.0 / 9007199254740992.0);
   }
   // Box-Muller, one normal per fresh block.
   __device__ double normal() {

@@ -53,10 +53,12 @@ __global__ __launch_bounds__(256) void trace_kernel(DevScene s, const Pose *pose
                                                     long long items, const int *item_pixel, double *rad,
                                                     float *img, float *dep, int *status,
                                                     unsigned long long *shadow_count) {
+  // LDS: each lane's traversal stack occupies words [k*256 + lane]; once the
+  // lane's sub-ray is done, its first three stack words hold its radiance
+  // (two halves of the double) and depth for the ordered per-sample sum.
   extern __shared__ double lds_words[];
-  double *red = lds_words;
-  float *dred = reinterpret_cast<float *>(lds_words + 256);
-  int *stack = reinterpret_cast<int *>(dred + 256) + threadIdx.x;
+  int *words = reinterpret_cast<int *>(lds_words);
+  int *stack = words + threadIdx.x;
   const int R = s.samples * s.samples, slots = 256 / R;
   const int slot = threadIdx.x / R, sub = threadIdx.x - slot * R;
   const long long q = (long long)blockIdx.x * slots + slot;
@@ -81,8 +83,9 @@ __global__ __launch_bounds__(256) void trace_kernel(DevScene s, const Pose *pose
     const Pose pose = poses[pose_index];
     r = sample_subray<MODE != MODE_DEPTH>(s, pixel, pose, sub % s.samples, sub / s.samples, stack, 256);
   }
-  red[threadIdx.x] = r.radiance;
-  dred[threadIdx.x] = r.depth;
+  stack[0] = __double2loint(r.radiance);
+  stack[256] = __double2hiint(r.radiance);
+  stack[512] = __float_as_int(r.depth);
   const unsigned long long shadows = __ballot(r.shadow);
   const unsigned long long invalid = __ballot(r.invalid);
   if ((threadIdx.x & 63) == 0) {
@@ -96,8 +99,9 @@ __global__ __launch_bounds__(256) void trace_kernel(DevScene s, const Pose *pose
   double total = 0, ranges = 0;
   int hits = 0;
   for (int j = 0; j < R; ++j) {
-    total += red[threadIdx.x + j];
-    const float d = dred[threadIdx.x + j];
+    const int lane = threadIdx.x + j;
+    total += __hiloint2double(words[256 + lane], words[lane]);
+    const float d = __int_as_float(words[512 + lane]);
     if (d >= 0) { ranges += d; ++hits; }
   }
   const double mean = total / R;
@@ -286,7 +290,8 @@ namespace dvsamd {
 using namespace dvsdev;
 
 static size_t trace_lds_bytes(const DevScene &s) {
-  return 256 * (sizeof(double) + sizeof(float)) + (size_t)s.stack_depth * 256 * sizeof(int);
+  // per lane: max(stack depth, 3 words of the radiance/depth hand-off)
+  return (size_t)(s.stack_depth < 3 ? 3 : s.stack_depth) * 256 * sizeof(int);
 }
 
 hipError_t launch_cell_poses(int cells, const double *t0, const double *t1, const int *iv_index,
