@@ -168,7 +168,7 @@ def main() -> int:
         "breakdown_ms": {"trace": round(st["trace_ms"], 3), "circuit": round(st["circuit_ms"], 3),
                          "events": round(st["events_ms"], 3), "wall": round(wall * 1e3, 3)},
         "rays": {"primary": st["primary_rays"], "shadow": st["shadow_rays"], "event_depth": st["event_rays"]},
-        "roofline": {"bound": "hbm", "kernel": "trace_kernel<MODE_CELLS>", "achieved": round(achieved, 2),
+        "roofline": {"bound": "hbm", "kernel": "trace_cells_multi", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "avg_launch_ms": round(st["trace_ms"] / launches, 3),
                      "bytes_per_ray": {"primary": round(b_primary, 1), "shadow": round(b_shadow, 1)}},

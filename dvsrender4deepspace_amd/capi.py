@@ -181,6 +181,20 @@ def load_library() -> ctypes.CDLL:
         "dvs_sensor_states": (ctypes.c_int, [_P, _P]),
         "dvs_sensor_stats": (ctypes.c_int, [_P, ctypes.POINTER(_StatsC)]),
         "dvs_sensor_reset_stats": (ctypes.c_int, [_P]),
+        "dvs_scene_view": (ctypes.c_int, [_P, ctypes.POINTER(_CameraC), _P, ctypes.POINTER(_P)]),
+        "dvs_batch_create": (ctypes.c_int, [_I32, _I32, ctypes.POINTER(_P)]),
+        "dvs_batch_destroy": (ctypes.c_int, [_P]),
+        "dvs_batch_add": (ctypes.c_int, [_P, _P, ctypes.POINTER(_PhysicC), _I32, _I32, _P, ctypes.c_double,
+                                         ctypes.POINTER(_I32)]),
+        "dvs_batch_feed": (ctypes.c_int, [_P, _I32, _P, _I64]),
+        "dvs_batch_step": (ctypes.c_int, [_P, ctypes.POINTER(_I32)]),
+        "dvs_batch_take": (ctypes.c_int, [_P, _I32, _P, _I64, ctypes.POINTER(_I64)]),
+        "dvs_batch_drain": (ctypes.c_int, [_P, _I32, ctypes.POINTER(_I64)]),
+        "dvs_batch_segment_info": (ctypes.c_int, [_P, _I32, ctypes.POINTER(_I64), ctypes.POINTER(_I64),
+                                                  ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
+        "dvs_batch_remove": (ctypes.c_int, [_P, _I32]),
+        "dvs_batch_stats": (ctypes.c_int, [_P, ctypes.POINTER(_StatsC)]),
+        "dvs_batch_reset_stats": (ctypes.c_int, [_P]),
         "dvs_hapke_eval": (ctypes.c_int, [_P, _P, _P, _P, _I64, ctypes.POINTER(_HapkeC), _P]),
         "dvs_receptor_eval": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, _I64, _P]),
         "dvs_noise_eval": (ctypes.c_int, [ctypes.c_uint64, _I64, ctypes.c_double, _I32, _P, _P]),
@@ -308,6 +322,20 @@ class Scene:
         _check(load_library().dvs_scene_stats(self.handle, ctypes.byref(s)))
         return _stats(s)
 
+    def view(self, camera: Camera, hapke: Hapke | None = None) -> "Scene":
+        """Another camera on this scene's resident geometry (shared BVH and mesh)."""
+        out = Scene.__new__(Scene)
+        out.camera = camera
+        out.hapke = hapke or self.hapke
+        out._keep = None
+        out._base = self
+        handle = _P()
+        cam = _struct(_CameraC, camera)
+        hk = _struct(_HapkeC, out.hapke)
+        _check(load_library().dvs_scene_view(self.handle, ctypes.byref(cam), ctypes.byref(hk), ctypes.byref(handle)))
+        out.handle = handle
+        return out
+
     def traversal_stats(self, pose: np.ndarray) -> dict:
         """Per-frame traversal work (node-pair visits, leaf visits, triangle tests)."""
         p = np.ascontiguousarray(np.atleast_1d(pose)[:1], dtype=STATE_DTYPE)
@@ -407,6 +435,75 @@ class Sensor:
 
     def reset_stats(self) -> None:
         _check(load_library().dvs_sensor_reset_stats(self.handle))
+
+
+class Batch:
+    """Multi-segment engine: many sensors advancing together in shared launches."""
+
+    def __init__(self, horizons_per_step: int = 64, use_graph: bool = False):
+        handle = _P()
+        _check(load_library().dvs_batch_create(int(horizons_per_step), int(use_graph), ctypes.byref(handle)))
+        self.handle = handle
+        self.horizons_per_step = horizons_per_step
+        self._scenes = {}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load_library().dvs_batch_destroy(self.handle)
+            self.handle = None
+
+    __del__ = close
+
+    def add(self, scene: Scene, physic: Physic, initial_pose: np.ndarray, t0: float, physic_effects: bool = True,
+            aer_delay: bool = True) -> int:
+        seg = _I32()
+        ph = _struct(_PhysicC, physic)
+        p = np.ascontiguousarray(np.atleast_1d(initial_pose)[:1], dtype=STATE_DTYPE)
+        _check(load_library().dvs_batch_add(self.handle, scene.handle, ctypes.byref(ph), int(physic_effects),
+                                            int(aer_delay), _ptr(p), float(t0), ctypes.byref(seg)))
+        self._scenes[seg.value] = scene
+        return seg.value
+
+    def feed(self, segment: int, intervals: np.ndarray) -> None:
+        iv = np.ascontiguousarray(np.atleast_1d(intervals), dtype=INTERVAL_DTYPE)
+        _check(load_library().dvs_batch_feed(self.handle, int(segment), _ptr(iv), len(iv)))
+
+    def step(self) -> int:
+        pending = _I32()
+        _check(load_library().dvs_batch_step(self.handle, ctypes.byref(pending)))
+        return pending.value
+
+    def info(self, segment: int) -> dict:
+        q, d, e, b = _I64(), _I64(), _I64(), _I64()
+        _check(load_library().dvs_batch_segment_info(self.handle, int(segment), ctypes.byref(q), ctypes.byref(d),
+                                                     ctypes.byref(e), ctypes.byref(b)))
+        return {"queued": q.value, "done": d.value, "events": e.value, "device_bytes": b.value}
+
+    def take(self, segment: int) -> np.ndarray:
+        lib = load_library()
+        n = _I64()
+        code = lib.dvs_batch_take(self.handle, int(segment), None, 0, ctypes.byref(n))
+        if code not in (DVS_OK, DVS_E_ARGUMENT):
+            _check(code)
+        out = np.empty(n.value, EVENT_DTYPE)
+        _check(lib.dvs_batch_take(self.handle, int(segment), _ptr(out), n.value, ctypes.byref(n)))
+        return out[: n.value]
+
+    def drain(self, segment: int) -> np.ndarray:
+        _check(load_library().dvs_batch_drain(self.handle, int(segment), ctypes.byref(_I64())))
+        return self.take(segment)
+
+    def remove(self, segment: int) -> None:
+        _check(load_library().dvs_batch_remove(self.handle, int(segment)))
+        self._scenes.pop(segment, None)
+
+    def stats(self) -> Stats:
+        s = _StatsC()
+        _check(load_library().dvs_batch_stats(self.handle, ctypes.byref(s)))
+        return _stats(s)
+
+    def reset_stats(self) -> None:
+        _check(load_library().dvs_batch_reset_stats(self.handle))
 
 
 def hapke_eval(cosi, cose, cosg, albedo, hapke: Hapke) -> np.ndarray:

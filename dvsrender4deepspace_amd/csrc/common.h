@@ -1,0 +1,93 @@
+// Host-side helpers shared by the C ABI translation units: error reporting in
+// the reference's exception wording, owning device arrays and HIP-event timers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+
+#include "dvs_hip.h"
+
+namespace dvsamd {
+
+// Message of the calling thread's last failure (dvs_last_error).
+std::string &last_error();
+
+struct Failure {
+  int code;
+  std::string message;
+};
+[[noreturn]] inline void fail(int code, std::string message) { throw Failure{code, std::move(message)}; }
+inline void hip_check(hipError_t e, const char *where) {
+  // Message format of CUDA_CHECK (include/dvs/math_types.hpp:6-14).
+  if (e != hipSuccess) fail(DVS_E_HIP, std::string("[HIP Error] in ") + where + " : " + hipGetErrorString(e));
+}
+// Runs f, mapping exceptions to the C ABI status codes.
+template <class F> int guarded(F &&f) {
+  try {
+    f();
+    last_error().clear();
+    return DVS_OK;
+  } catch (const Failure &x) {
+    last_error() = x.message;
+    return x.code;
+  } catch (const std::invalid_argument &x) {
+    last_error() = x.what();
+    return DVS_E_ARGUMENT;
+  } catch (const std::bad_alloc &) {
+    last_error() = "host allocation failed";
+    return DVS_E_STATE;
+  } catch (const std::exception &x) {
+    last_error() = x.what();
+    return DVS_E_STATE;
+  }
+}
+
+// Owning device allocation that grows on demand (contents are not kept).
+template <class T> struct DeviceArray {
+  T *ptr = nullptr;
+  size_t count = 0;
+  DeviceArray() = default;
+  DeviceArray(const DeviceArray &) = delete;
+  DeviceArray &operator=(const DeviceArray &) = delete;
+  ~DeviceArray() { release(); }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    count = 0;
+  }
+  void reserve(size_t n, const char *what) {
+    if (n <= count) return;
+    release();
+    hip_check(hipMalloc(&ptr, std::max<size_t>(n, 1) * sizeof(T)), what);
+    count = n;
+  }
+  void upload(const T *host, size_t n, const char *what, hipStream_t st = nullptr) {
+    reserve(n, what);
+    if (n) hip_check(hipMemcpyAsync(ptr, host, n * sizeof(T), hipMemcpyHostToDevice, st), what);
+  }
+  size_t bytes() const { return count * sizeof(T); }
+};
+
+// Pair of HIP events bracketing work on one stream.
+struct Timer {
+  hipEvent_t start = nullptr, stop = nullptr;
+  Timer() {
+    hip_check(hipEventCreate(&start), "timer event");
+    hip_check(hipEventCreate(&stop), "timer event");
+  }
+  ~Timer() {
+    if (start) (void)hipEventDestroy(start);
+    if (stop) (void)hipEventDestroy(stop);
+  }
+  Timer(const Timer &) = delete;
+  Timer &operator=(const Timer &) = delete;
+  double ms() const {
+    float v = 0;
+    hip_check(hipEventElapsedTime(&v, start, stop), "timer elapsed");
+    return v;
+  }
+};
+
+}  // namespace dvsamd

@@ -6,10 +6,12 @@
 //                      MODE_CELLS   pixel x circuit-cell radiance (sensor),
 //                      MODE_FRAME   frame radiance + depth,
 //                      MODE_DEPTH   event-time depth (primary rays only).
-//  circuit_kernel      one lane per pixel; FP64 receptor integration and
-//                      comparator crossings over a chunk of cells; Philox
-//                      shot noise; crossing times into per-pixel slots.
-//  gather / pose / permute kernels for device event compaction.
+//  *_multi kernels     the engine step over a table of segments (sensors):
+//                      trace_cells_multi   pixel x cell radiance per segment,
+//                      circuit_multi       one lane per pixel: FP64 receptor
+//                                          integration, comparator crossings,
+//                                          Philox shot/reset noise,
+//                      gather/pose/depth/permute for device event compaction.
 #include <hip/hip_runtime.h>
 
 #include "device_circuit.hpp"
@@ -30,22 +32,6 @@ __global__ void cell_pose_kernel(int cells, const double *t0, const double *t1, 
 __global__ void state_pose_kernel(int n, const dvs_optical_state *states, Pose *poses) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) poses[i] = pose_of(states[i]);
-}
-
-// Interval of an event: the first interval whose end is not before the event
-// (the forward scan of generate_pixels, event_sensor.cu:54), then the pose at
-// the event time.
-__global__ void event_pose_kernel(int events, const double *times, const dvs_optical_interval *ivs,
-                                  int iv_count, Pose *poses) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= events) return;
-  const double t = times[e];
-  int lo = 0, hi = iv_count - 1;
-  while (lo < hi) {
-    const int m = (lo + hi) >> 1;
-    if (t > ivs[m].t1) lo = m + 1; else hi = m;
-  }
-  poses[e] = pose_at(ivs[lo], t);
 }
 
 template <int MODE>
@@ -143,32 +129,86 @@ __global__ void init_kernel(int npix, const double *radiance, PhotoParams photo,
   states[i] = s;
 }
 
-// Per-pixel circuit over cells [c_begin, c_begin + c_count) of one process
-// call (the circuit half of generate_pixels, event_sensor.cu:39-49).
-__global__ __launch_bounds__(256) void circuit_kernel(int npix, int c_begin, int c_count, const double *cell_t0,
-                                                      const double *cell_t1, const double *rad,
-                                                      dvs_pixel_state *states, CircuitParams p, PhotoParams photo,
-                                                      uint64_t seed, int max_events, CrossingEvent *slots,
-                                                      int *counts, int *status) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= npix) return;
-  int count = counts[i];
-  if (count < 0) return;
-  dvs_pixel_state st = states[i];
+
+// ---------------------------------------------------------------------------
+// Multi-segment engine step
+// ---------------------------------------------------------------------------
+// Last segment whose first block is <= b (begin[0] == 0).
+template <class T>
+__device__ __forceinline__ int segment_of(const T *begin, int nseg, T b) {
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const int m = (lo + hi + 1) >> 1;
+    if (begin[m] <= b) lo = m; else hi = m - 1;
+  }
+  return lo;
+}
+
+// Radiance of every (pixel, circuit cell) of every segment: the optical half
+// of generate_pixels (src/cuda/event_sensor.cu:39-42).  Items of one block
+// are consecutive cells of one pixel (temporally coherent rays).
+__global__ __launch_bounds__(256) void trace_cells_multi(const SegDev *segs, const long long *block_begin, int nseg,
+                                                         int *status, unsigned long long *shadow_count) {
+  extern __shared__ double lds_words[];
+  int *words = reinterpret_cast<int *>(lds_words);
+  int *stack = words + threadIdx.x;
+  const int si = segment_of<long long>(block_begin, nseg, (long long)blockIdx.x);
+  const SegDev &g = segs[si];
+  const DevScene &s = g.scene;
+  const int R = s.samples * s.samples, slots = 256 / R;
+  const int slot = threadIdx.x / R, sub = threadIdx.x - slot * R;
+  const long long q = ((long long)blockIdx.x - block_begin[si]) * slots + slot;
+  const int cells = g.cells, npix = g.npix;
+  const bool active = slot < slots && q < (long long)npix * cells;
+  SubSample r{0.0, -1.f, 0, 0};
+  int pixel = 0, cell = 0;
+  if (active) {
+    pixel = (int)(q / cells);
+    cell = (int)(q - (long long)pixel * cells);
+    const Pose pose = g.poses[cell];
+    r = sample_subray<true>(s, pixel, pose, sub % s.samples, sub / s.samples, stack, 256);
+  }
+  stack[0] = __double2loint(r.radiance);
+  stack[256] = __double2hiint(r.radiance);
+  const unsigned long long shadows = __ballot(r.shadow);
+  const unsigned long long invalid = __ballot(r.invalid);
+  if ((threadIdx.x & 63) == 0) {
+    if (shadows) atomicAdd(shadow_count, (unsigned long long)__popcll(shadows));
+    if (invalid) atomicOr(status, 2);
+  }
+  __syncthreads();
+  if (!active || sub != 0) return;
+  double total = 0;
+  for (int j = 0; j < R; ++j) total += __hiloint2double(words[256 + threadIdx.x + j], words[threadIdx.x + j]);
+  g.rad[(size_t)cell * npix + pixel] = total / R;
+}
+
+// The circuit half of generate_pixels (event_sensor.cu:39-49): one lane per
+// pixel integrates all cells of this step in order.
+__global__ __launch_bounds__(256) void circuit_multi(const SegDev *segs, const int *block_begin, int nseg,
+                                                     int *status) {
+  const int si = segment_of<int>(block_begin, nseg, (int)blockIdx.x);
+  const SegDev &g = segs[si];
+  const int i = ((int)blockIdx.x - block_begin[si]) * 256 + threadIdx.x;
+  if (i >= g.npix) return;
+  dvs_pixel_state st = g.states[i];
   PixelRegs regs{st.receptor_log, st.reference_log, st.refractory_until, st.threshold_pos,
                  st.threshold_neg, st.reset_pending};
-  NoiseStream shot(seed, (uint64_t)i, kStreamShot, st.shot_block);
-  NoiseStream reset(seed, (uint64_t)i, kStreamReset, st.reset_block);
+  NoiseStream shot(g.seed, (uint64_t)i, kStreamShot, st.shot_block);
+  NoiseStream reset(g.seed, (uint64_t)i, kStreamReset, st.reset_block);
   auto normal = [&]() { return reset.normal(); };
-  CrossingEvent *out = slots + (size_t)i * max_events;
-  for (int c = 0; c < c_count; ++c) {
-    const double t0 = cell_t0[c_begin + c], t1 = cell_t1[c_begin + c];
-    const double radiance = rad[(size_t)c * npix + i];
-    if (!isfinite(radiance)) { atomicOr(status, 2); counts[i] = -1; return; }
+  CrossingEvent *out = g.slots + (size_t)i * g.max_events;
+  const CircuitParams p = g.params;
+  const PhotoParams photo = g.photo;
+  int count = g.counts[i];  // crossings of earlier cell chunks of this step
+  for (int c = 0; c < g.cells; ++c) {
+    const double t0 = g.t0[c], t1 = g.t1[c];
+    const double radiance = g.rad[(size_t)c * g.npix + i];
+    if (!isfinite(radiance)) { atomicOr(status, 2); g.counts[i] = 0; return; }
     double rate = radiance * photo.electrons_per_radiance + photo.dark_electrons;
     if (photo.shot_noise) rate = shot.poisson(rate * (t1 - t0)) / (t1 - t0);
-    const int emitted = advance_pixel(regs, p, rate, t0, t1, out + count, max_events - count, normal);
-    if (emitted < 0) { atomicOr(status, 1); counts[i] = -1; return; }
+    const int emitted = advance_pixel(regs, p, rate, t0, t1, out + count, g.max_events - count, normal);
+    if (emitted < 0) { atomicOr(status, 1); g.counts[i] = 0; return; }
     count += emitted;
   }
   st.receptor_log = regs.receptor_log;
@@ -179,35 +219,92 @@ __global__ __launch_bounds__(256) void circuit_kernel(int npix, int c_begin, int
   st.reset_pending = regs.reset_pending;
   st.shot_block = shot.block;
   st.reset_block = (uint32_t)reset.block;
-  states[i] = st;
-  counts[i] = count;
+  g.states[i] = st;
+  g.counts[i] = count;
 }
 
-// Compaction: pixel i's crossings go to [ends[i] - counts[i], ends[i]).
-__global__ void gather_kernel(int npix, const int *ends, const int *counts, int max_events,
-                              const CrossingEvent *slots, double *time, int *pixel, int *polarity,
-                              unsigned long long *keys, unsigned *order) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= npix) return;
-  const int n = counts[i], base = ends[i] - n;
+// Compaction: global pixel p's crossings go to [ends[p] - count, ends[p]);
+// the global pixel order is segment-major, pixel-major within a segment.
+__global__ void gather_multi(const SegDev *segs, const int *pixel_begin, int nseg, int total_pixels, const int *ends,
+                             double *time, int *seg, int *pixel, int *polarity, unsigned long long *keys,
+                             unsigned *order) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= total_pixels) return;
+  const int si = segment_of<int>(pixel_begin, nseg, p);
+  const SegDev &g = segs[si];
+  const int i = p - pixel_begin[si];
+  const int n = g.counts[i], base = ends[p] - n;
   for (int j = 0; j < n; ++j) {
-    const CrossingEvent e = slots[(size_t)i * max_events + j];
+    const CrossingEvent e = g.slots[(size_t)i * g.max_events + j];
     const int k = base + j;
     time[k] = e.time;
+    seg[k] = si;
     pixel[k] = i;
     polarity[k] = e.polarity;
-    // Order-preserving key of an IEEE double.
+    // order-preserving key of an IEEE double
     const unsigned long long bits = (unsigned long long)__double_as_longlong(e.time);
     keys[k] = (bits >> 63) ? ~bits : (bits | 0x8000000000000000ULL);
     order[k] = (unsigned)k;
   }
 }
 
-__global__ void permute_kernel(int events, int width, const unsigned *order, const double *time, const int *pixel,
-                               const int *polarity, const float *depth, dvs_event *out) {
+// Pose at an event: the first interval of the step whose end is not before
+// the event (the forward scan of generate_pixels, event_sensor.cu:54).
+__global__ void event_poses_multi(const SegDev *segs, int events, const double *time, const int *seg, Pose *poses) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= events) return;
+  const SegDev &g = segs[seg[e]];
+  const double t = time[e];
+  int lo = 0, hi = g.iv_count - 1;
+  while (lo < hi) {
+    const int m = (lo + hi) >> 1;
+    if (t > g.ivs[m].t1) lo = m + 1; else hi = m;
+  }
+  poses[e] = pose_at(g.ivs[lo], t);
+}
+
+// Event-time depth (sample_pixel at the event time, event_sensor.cu:55-58):
+// primary rays only, mean hit range of the sub-rays, -1 if none hit.
+__global__ __launch_bounds__(256) void event_depth_multi(const SegDev *segs, int events, int samples, const int *seg,
+                                                         const int *pixel, const Pose *poses, float *depth,
+                                                         int *status) {
+  extern __shared__ double lds_words[];
+  int *words = reinterpret_cast<int *>(lds_words);
+  int *stack = words + threadIdx.x;
+  const int R = samples * samples, slots = 256 / R;
+  const int slot = threadIdx.x / R, sub = threadIdx.x - slot * R;
+  const long long e = (long long)blockIdx.x * slots + slot;
+  const bool active = slot < slots && e < events;
+  SubSample r{0.0, -1.f, 0, 0};
+  if (active) {
+    const SegDev &g = segs[seg[e]];
+    r = sample_subray<false>(g.scene, pixel[e], poses[e], sub % samples, sub / samples, stack, 256);
+  }
+  stack[0] = __float_as_int(r.depth);
+  if (__ballot(r.invalid) && (threadIdx.x & 63) == 0) atomicOr(status, 2);
+  __syncthreads();
+  if (!active || sub != 0) return;
+  double ranges = 0;
+  int hits = 0;
+  for (int j = 0; j < R; ++j) {
+    const float d = __int_as_float(words[threadIdx.x + j]);
+    if (d >= 0) { ranges += d; ++hits; }
+  }
+  depth[e] = hits ? (float)(ranges / hits) : -1.f;
+}
+
+__global__ void segment_keys_kernel(int events, const unsigned *order, const int *seg, unsigned long long *keys) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < events) keys[k] = (unsigned long long)seg[order[k]];
+}
+
+__global__ void permute_multi(const SegDev *segs, int events, const unsigned *order, const double *time,
+                              const int *seg, const int *pixel, const int *polarity, const float *depth,
+                              dvs_event *out, int *out_seg) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= events) return;
   const unsigned j = order[k];
+  const int width = segs[seg[j]].scene.width;
   dvs_event e;
   e.timestamp = time[j];
   e.x = (int16_t)(pixel[j] % width);
@@ -215,6 +312,12 @@ __global__ void permute_kernel(int events, int width, const unsigned *order, con
   e.polarity = (int8_t)polarity[j];
   e.depth = depth[j];
   out[k] = e;
+  out_seg[k] = seg[j];
+}
+
+__global__ void fill_int_kernel(int *p, int value, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = value;
 }
 
 // ---- unit kernels -------------------------------------------------------
@@ -305,12 +408,6 @@ hipError_t launch_state_poses(int n, const dvs_optical_state *states, Pose *pose
   state_pose_kernel<<<(n + 255) / 256, 256, 0, st>>>(n, states, poses);
   return hipGetLastError();
 }
-hipError_t launch_event_poses(int events, const double *times, const dvs_optical_interval *ivs, int iv_count,
-                              Pose *poses, hipStream_t st) {
-  if (events <= 0) return hipSuccess;
-  event_pose_kernel<<<(events + 255) / 256, 256, 0, st>>>(events, times, ivs, iv_count, poses);
-  return hipGetLastError();
-}
 hipError_t launch_trace(int mode, const DevScene &s, const Pose *poses, int cells, int npix, long long items,
                         const int *item_pixel, double *rad, float *img, float *dep, int *status,
                         unsigned long long *shadow_count, hipStream_t st) {
@@ -341,25 +438,58 @@ hipError_t launch_init(int npix, const double *radiance, const PhotoParams &phot
                                                   states, status);
   return hipGetLastError();
 }
-hipError_t launch_circuit(int npix, int c_begin, int c_count, const double *t0, const double *t1,
-                          const double *rad, dvs_pixel_state *states, const CircuitParams &p,
-                          const PhotoParams &photo, uint64_t seed, int max_events, CrossingEvent *slots,
-                          int *counts, int *status, hipStream_t st) {
-  circuit_kernel<<<(npix + 255) / 256, 256, 0, st>>>(npix, c_begin, c_count, t0, t1, rad, states, p, photo, seed,
-                                                     max_events, slots, counts, status);
+hipError_t launch_trace_cells_multi(const SegDev *segs, const long long *block_begin, int nseg, long long blocks,
+                                    size_t lds, int *status, unsigned long long *shadow_count, hipStream_t st) {
+  if (blocks <= 0) return hipSuccess;
+  if (blocks > 0x7fffffffLL) return hipErrorInvalidConfiguration;
+  trace_cells_multi<<<(unsigned)blocks, 256, lds, st>>>(segs, block_begin, nseg, status, shadow_count);
   return hipGetLastError();
 }
-hipError_t launch_gather(int npix, const int *ends, const int *counts, int max_events, const CrossingEvent *slots,
-                         double *time, int *pixel, int *polarity, unsigned long long *keys, unsigned *order,
-                         hipStream_t st) {
-  gather_kernel<<<(npix + 255) / 256, 256, 0, st>>>(npix, ends, counts, max_events, slots, time, pixel, polarity,
-                                                    keys, order);
+hipError_t launch_circuit_multi(const SegDev *segs, const int *block_begin, int nseg, int blocks, int *status,
+                                hipStream_t st) {
+  if (blocks <= 0) return hipSuccess;
+  circuit_multi<<<blocks, 256, 0, st>>>(segs, block_begin, nseg, status);
   return hipGetLastError();
 }
-hipError_t launch_permute(int events, int width, const unsigned *order, const double *time, const int *pixel,
-                          const int *polarity, const float *depth, dvs_event *out, hipStream_t st) {
+hipError_t launch_gather_multi(const SegDev *segs, const int *pixel_begin, int nseg, int total_pixels,
+                               const int *ends, double *time, int *seg, int *pixel, int *polarity,
+                               unsigned long long *keys, unsigned *order, hipStream_t st) {
+  if (total_pixels <= 0) return hipSuccess;
+  gather_multi<<<(total_pixels + 255) / 256, 256, 0, st>>>(segs, pixel_begin, nseg, total_pixels, ends, time, seg,
+                                                           pixel, polarity, keys, order);
+  return hipGetLastError();
+}
+hipError_t launch_event_poses_multi(const SegDev *segs, int events, const double *time, const int *seg,
+                                    Pose *poses, hipStream_t st) {
   if (events <= 0) return hipSuccess;
-  permute_kernel<<<(events + 255) / 256, 256, 0, st>>>(events, width, order, time, pixel, polarity, depth, out);
+  event_poses_multi<<<(events + 255) / 256, 256, 0, st>>>(segs, events, time, seg, poses);
+  return hipGetLastError();
+}
+hipError_t launch_event_depth_multi(const SegDev *segs, int events, int samples, size_t lds, const int *seg,
+                                    const int *pixel, const Pose *poses, float *depth, int *status, hipStream_t st) {
+  if (events <= 0) return hipSuccess;
+  const int slots = 256 / (samples * samples);
+  event_depth_multi<<<(events + slots - 1) / slots, 256, lds, st>>>(segs, events, samples, seg, pixel, poses, depth,
+                                                                   status);
+  return hipGetLastError();
+}
+hipError_t launch_segment_keys(int events, const unsigned *order, const int *seg, unsigned long long *keys,
+                               hipStream_t st) {
+  if (events <= 0) return hipSuccess;
+  segment_keys_kernel<<<(events + 255) / 256, 256, 0, st>>>(events, order, seg, keys);
+  return hipGetLastError();
+}
+hipError_t launch_permute_multi(const SegDev *segs, int events, const unsigned *order, const double *time,
+                                const int *seg, const int *pixel, const int *polarity, const float *depth,
+                                dvs_event *out, int *out_seg, hipStream_t st) {
+  if (events <= 0) return hipSuccess;
+  permute_multi<<<(events + 255) / 256, 256, 0, st>>>(segs, events, order, time, seg, pixel, polarity, depth, out,
+                                                       out_seg);
+  return hipGetLastError();
+}
+hipError_t launch_fill_int(int *p, int value, int n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  fill_int_kernel<<<(n + 255) / 256, 256, 0, st>>>(p, value, n);
   return hipGetLastError();
 }
 hipError_t launch_hapke(int64_t n, const float *ci, const float *ce, const float *cg, const float *alb,

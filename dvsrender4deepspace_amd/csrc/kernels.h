@@ -7,6 +7,25 @@
 
 namespace dvsdev {
 enum { MODE_CELLS = 0, MODE_FRAME = 1, MODE_DEPTH = 2 };
+
+// One segment (one sensor on one scene view) of an engine step, resident in
+// device memory; every multi-segment kernel finds its segment from blockIdx.
+struct SegDev {
+  DevScene scene;
+  const dvs_optical_interval *ivs;  // this step's intervals of the segment
+  const double *t0, *t1;            // this step's circuit cells of the segment
+  const Pose *poses;                // cell midpoint poses
+  double *rad;                      // [cells][npix] radiance of this step
+  dvs_pixel_state *states;
+  CrossingEvent *slots;             // npix * max_events crossings
+  int *counts;                      // npix crossing counts (segment's slice)
+  CircuitParams params;
+  PhotoParams photo;
+  uint64_t seed;
+  int iv_count, cells, npix, max_events;
+  int pixel_offset;                 // first global pixel index of the segment
+  int pad;
+};
 }
 
 namespace dvsamd {
@@ -15,27 +34,37 @@ using dvsdev::CrossingEvent;
 using dvsdev::DevScene;
 using dvsdev::PhotoParams;
 using dvsdev::Pose;
+using dvsdev::SegDev;
+
+// Multi-segment launches.  block_begin / pixel_begin hold nseg+1 prefix
+// offsets (in blocks) of each segment's work; the grids are their totals.
+hipError_t launch_trace_cells_multi(const SegDev *segs, const long long *block_begin, int nseg, long long blocks,
+                                    size_t lds, int *status, unsigned long long *shadow_count, hipStream_t st);
+hipError_t launch_circuit_multi(const SegDev *segs, const int *block_begin, int nseg, int blocks, int *status,
+                                hipStream_t st);
+hipError_t launch_gather_multi(const SegDev *segs, const int *pixel_begin, int nseg, int total_pixels,
+                               const int *ends, double *time, int *seg, int *pixel, int *polarity,
+                               unsigned long long *keys, unsigned *order, hipStream_t st);
+hipError_t launch_event_poses_multi(const SegDev *segs, int events, const double *time, const int *seg,
+                                    Pose *poses, hipStream_t st);
+hipError_t launch_event_depth_multi(const SegDev *segs, int events, int samples, size_t lds, const int *seg,
+                                    const int *pixel, const Pose *poses, float *depth, int *status, hipStream_t st);
+hipError_t launch_segment_keys(int events, const unsigned *order, const int *seg, unsigned long long *keys,
+                               hipStream_t st);
+hipError_t launch_permute_multi(const SegDev *segs, int events, const unsigned *order, const double *time,
+                                const int *seg, const int *pixel, const int *polarity, const float *depth,
+                                dvs_event *out, int *out_seg, hipStream_t st);
+hipError_t launch_fill_int(int *p, int value, int n, hipStream_t st);
 
 hipError_t launch_cell_poses(int cells, const double *t0, const double *t1, const int *iv_index,
                              const dvs_optical_interval *ivs, Pose *poses, hipStream_t st);
 hipError_t launch_state_poses(int n, const dvs_optical_state *states, Pose *poses, hipStream_t st);
-hipError_t launch_event_poses(int events, const double *times, const dvs_optical_interval *ivs, int iv_count,
-                              Pose *poses, hipStream_t st);
 hipError_t launch_trace(int mode, const DevScene &s, const Pose *poses, int cells, int npix, long long items,
                         const int *item_pixel, double *rad, float *img, float *dep, int *status,
                         unsigned long long *shadow_count, hipStream_t st);
 hipError_t launch_init(int npix, const double *radiance, const PhotoParams &photo, const CircuitParams &p,
                        double threshold_std, double threshold_min, uint64_t seed, dvs_pixel_state *states,
                        int *status, hipStream_t st);
-hipError_t launch_circuit(int npix, int c_begin, int c_count, const double *t0, const double *t1,
-                          const double *rad, dvs_pixel_state *states, const CircuitParams &p,
-                          const PhotoParams &photo, uint64_t seed, int max_events, CrossingEvent *slots,
-                          int *counts, int *status, hipStream_t st);
-hipError_t launch_gather(int npix, const int *ends, const int *counts, int max_events, const CrossingEvent *slots,
-                         double *time, int *pixel, int *polarity, unsigned long long *keys, unsigned *order,
-                         hipStream_t st);
-hipError_t launch_permute(int events, int width, const unsigned *order, const double *time, const int *pixel,
-                          const int *polarity, const float *depth, dvs_event *out, hipStream_t st);
 hipError_t launch_hapke(int64_t n, const float *ci, const float *ce, const float *cg, const float *alb,
                         const dvs_hapke &h, float *out, hipStream_t st);
 hipError_t launch_receptor(int64_t n, const double *u, const double *rate, const double *dt, double charge,

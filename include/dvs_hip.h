@@ -149,6 +149,10 @@ int dvs_scene_create_ex(const dvs_bvh_node *nodes, int64_t node_count, int32_t r
                         const uint32_t *indices, int64_t triangle_count, const float *albedos,
                         const dvs_camera *camera, const dvs_hapke *hapke, int32_t device_leaf_size,
                         dvs_scene *out);
+/* A second camera on the same resident geometry (no reference counterpart:
+ * the reference builds one DeviceScene per camera configuration).  The view
+ * shares the uploaded BVH and mesh; hapke may be NULL to keep the base's. */
+int dvs_scene_view(dvs_scene base, const dvs_camera *camera, const dvs_hapke *hapke, dvs_scene *out);
 /* ~DeviceScene (device_scene.cu:38-43).  The scene is reference counted:
  * sensors created on it keep it alive. */
 int dvs_scene_destroy(dvs_scene scene);
@@ -201,6 +205,31 @@ int dvs_sensor_readout_stats(dvs_sensor sensor, int64_t *dropped, int64_t *peak_
 int dvs_sensor_states(dvs_sensor sensor, dvs_pixel_state *out);
 int dvs_sensor_stats(dvs_sensor sensor, dvs_stats *out);
 int dvs_sensor_reset_stats(dvs_sensor sensor);
+
+/* ---- Multi-segment engine (batched and served forms) -----------------
+ * No reference counterpart: many EventSensors ("segments"), each with its own
+ * scene view, circuit state, Philox key and readout FIFO, advance together.
+ * dvs_batch_step takes up to horizons_per_step queued horizons of every
+ * segment and runs them in shared launches (optionally replayed from a
+ * captured HIP graph while the step shape repeats).  A segment's events are
+ * exactly those of a dvs_sensor fed the same intervals in the same calls. */
+typedef struct dvs_batch_s *dvs_batch;
+int dvs_batch_create(int32_t horizons_per_step, int32_t use_graph, dvs_batch *out);
+int dvs_batch_destroy(dvs_batch batch);
+/* Adds and initialises a segment (EventSensor ctor + initialize). */
+int dvs_batch_add(dvs_batch batch, dvs_scene scene, const dvs_physic *physic, int32_t physic_effects,
+                  int32_t aer_delay, const dvs_optical_state *initial_pose, double t0, int32_t *segment);
+/* Queues contiguous intervals of a segment. */
+int dvs_batch_feed(dvs_batch batch, int32_t segment, const dvs_optical_interval *intervals, int64_t count);
+/* One step; *pending_segments = segments with queued horizons left. */
+int dvs_batch_step(dvs_batch batch, int32_t *pending_segments);
+int dvs_batch_take(dvs_batch batch, int32_t segment, dvs_event *out, int64_t capacity, int64_t *count);
+int dvs_batch_drain(dvs_batch batch, int32_t segment, int64_t *ready_count);
+int dvs_batch_segment_info(dvs_batch batch, int32_t segment, int64_t *queued_horizons, int64_t *done_horizons,
+                           int64_t *events, int64_t *device_bytes);
+int dvs_batch_remove(dvs_batch batch, int32_t segment);
+int dvs_batch_stats(dvs_batch batch, dvs_stats *out);
+int dvs_batch_reset_stats(dvs_batch batch);
 
 /* ---- Device kernels exposed for unit parity (no reference counterpart;
  * they expose the reference's __host__ __device__ functions run on the GPU) */
