@@ -75,8 +75,123 @@ def cpu_baseline(w, mesh, horizons: int) -> dict:
                       f"{len(raw)} events ({len(raw) / wall:.1f} events/s)"}
 
 
+def _latency(reqs) -> dict:
+    lat = np.array([r.latency for r in reqs]) * 1e3
+    return {"p50": round(float(np.percentile(lat, 50)), 2), "p99": round(float(np.percentile(lat, 99)), 2)}
+
+
+def _same_streams(a, b) -> bool:
+    return all(x.event_array().tobytes() == y.event_array().tobytes() for x, y in zip(a, b))
+
+
+def served_mode(args) -> dict:
+    """Served form: Poisson request arrivals over two resident terrains,
+    continuous batching, HBM-footprint admission; against the plain
+    per-request path (scene built and uploaded per request) and the plain path
+    with the scenes kept resident.  Same requests, same seeds."""
+    import dataclasses
+
+    from dvsrender4deepspace_amd import capi, serve, terrain, workloads
+
+    specs = {"terrain512": terrain.TerrainSpec(size=512, amplitude=20.0, craters=100, seed=512),
+             "terrain1024": workloads.DAVIS346.terrain}
+    base = workloads.DAVIS346
+    t = time.perf_counter()
+    meshes = serve.terrain_meshes(specs)
+    log(f"meshes in {time.perf_counter() - t:.1f} s")
+    reqs = serve.request_mix(base, args.requests, args.rate, (args.min_horizons, args.max_horizons), specs,
+                             seed=args.seed, speed=tuple(args.speed))
+    geometries = {k: serve.build_scene(meshes[k], base, base.camera) for k in specs}
+    tune = serve.autotune(geometries, reqs[: min(6, len(reqs))], candidates=((16, False), (32, False), (32, True), (64, False)))
+    best = tune["best"]
+    log(f"autotune: {tune}")
+    served = [dataclasses.replace(r, events=[]) for r in reqs]
+    server = serve.Server(geometries, budget_bytes=args.budget_gb * 1e9,
+                          horizons_per_step=best["horizons_per_step"], use_graph=best["use_graph"])
+    wall_served = server.run(served, realtime=True)
+    log(f"served: {wall_served:.2f} s, {server.steps} steps, peak {server.peak_active} active")
+    resident = [dataclasses.replace(r, events=[]) for r in reqs]
+    wall_resident = serve.run_plain(resident, meshes, realtime=True, scenes_resident=True, geometries=geometries)
+    log(f"plain, scenes resident: {wall_resident:.2f} s")
+    plain = [dataclasses.replace(r, events=[]) for r in reqs]
+    wall_plain = serve.run_plain(plain, meshes, realtime=True)
+    log(f"plain per request: {wall_plain:.2f} s")
+    events = sum(len(r.event_array()) for r in served)
+    noise_off = [i for i, r in enumerate(reqs) if not r.workload.physic.photon_shot_noise]
+    return {
+        "metric": METRIC, "value": round(events / wall_served, 1), "unit": "events/s", "n_gpus": 1,
+        "higher_is_better": True, "data": "synthetic", "mode": "served",
+        "config": {"workload": "served mix", "requests": len(reqs), "arrival_rate_hz": args.rate,
+                   "horizons": [args.min_horizons, args.max_horizons], "resolutions": serve.RESOLUTIONS,
+                   "terrains": {k: v.triangles for k, v in specs.items()}, "admission_budget_gb": args.budget_gb,
+                   "supersample": base.camera.samples, "autotuned": best},
+        "events": events,
+        "served": {"wall_s": round(wall_served, 3), "events_per_s": round(events / wall_served, 1),
+                   "latency_ms": _latency(served), "steps": server.steps, "peak_active": server.peak_active},
+        "plain_per_request": {"wall_s": round(wall_plain, 3), "events_per_s": round(events / wall_plain, 1),
+                              "latency_ms": _latency(plain)},
+        "plain_scenes_resident": {"wall_s": round(wall_resident, 3), "events_per_s": round(events / wall_resident, 1),
+                                  "latency_ms": _latency(resident)},
+        "speedup_vs_plain": round(wall_plain / wall_served, 2),
+        "speedup_vs_resident_plain": round(wall_resident / wall_served, 2),
+        "identical_event_streams": {"all": _same_streams(served, plain) and _same_streams(served, resident),
+                                    "noise_off_requests": len(noise_off)},
+    }
+
+
+def batched_mode(args) -> dict:
+    """Batched form: 64 segments of uneven size over a shared 1024^2 terrain
+    advanced together (graph-replayed steps, autotuned step size), checked
+    against the same segments run one by one."""
+    import dataclasses
+
+    from dvsrender4deepspace_amd import serve, terrain, workloads
+
+    base = workloads.DAVIS346
+    specs = {"terrain1024": base.terrain}
+    meshes = serve.terrain_meshes(specs)
+    reqs = serve.request_mix(base, args.segments, None, (args.min_horizons, args.max_horizons), specs,
+                             seed=args.seed, speed=tuple(args.speed))
+    geometries = {"terrain1024": serve.build_scene(meshes["terrain1024"], base, base.camera)}
+    tune = serve.autotune(geometries, reqs[: min(6, len(reqs))], candidates=((16, False), (32, False), (32, True), (64, False)))
+    best = tune["best"]
+    log(f"autotune: {tune}")
+    batched = [dataclasses.replace(r, events=[]) for r in reqs]
+    server = serve.Server(geometries, horizons_per_step=best["horizons_per_step"], use_graph=best["use_graph"])
+    wall_batched = server.run(batched, realtime=False)
+    st = server.batch.stats()
+    alone = [dataclasses.replace(r, events=[]) for r in reqs]
+    wall_alone = serve.run_plain(alone, meshes, realtime=False, scenes_resident=True, geometries=geometries)
+    events = sum(len(r.event_array()) for r in batched)
+    rays = st["primary_rays"] + st["shadow_rays"] + st["event_rays"]
+    return {
+        "metric": METRIC, "value": round(events / wall_batched, 1), "unit": "events/s", "n_gpus": 1,
+        "higher_is_better": True, "data": "synthetic", "mode": "batched",
+        "config": {"workload": "64 uneven segments", "segments": len(reqs), "resolutions": serve.RESOLUTIONS,
+                   "horizons": [args.min_horizons, args.max_horizons], "terrain_triangles": base.terrain.triangles,
+                   "autotuned": best},
+        "events": events, "batched_wall_s": round(wall_batched, 3), "one_by_one_wall_s": round(wall_alone, 3),
+        "batched_mrays_per_s": round(rays / wall_batched / 1e6, 1),
+        "speedup_vs_one_by_one": round(wall_alone / wall_batched, 2),
+        "identical_event_streams": _same_streams(batched, alone), "steps": server.steps,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--mode", choices=["request", "served", "batched"], default="request",
+                    help="request: the single DAVIS346 request (default bench line); served / batched: the "
+                         "served and batched forms against the plain per-request path")
+    ap.add_argument("--requests", type=int, default=60)
+    ap.add_argument("--segments", type=int, default=64)
+    ap.add_argument("--rate", type=float, default=20.0)
+    ap.add_argument("--min-horizons", type=int, default=50)
+    ap.add_argument("--max-horizons", type=int, default=500)
+    ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--speed", type=float, nargs=2, default=(1000.0, 2000.0),
+                    help="ground speed range (m/s) of the served/batched mix: a low orbital pass, so that "
+                         "windows of a few hundred microseconds move the image by pixels")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
@@ -106,6 +221,9 @@ def main() -> int:
     capi.load_library()
     if capi.device_count() < 1:
         raise RuntimeError("no HIP device visible")
+    if args.mode != "request":
+        print(json.dumps(served_mode(args) if args.mode == "served" else batched_mode(args)), flush=True)
+        return 0
     w = workloads.WORKLOADS[args.workload]
     t = time.perf_counter()
     v, n, idx = terrain.mesh(w.terrain)

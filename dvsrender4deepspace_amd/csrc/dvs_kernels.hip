@@ -414,7 +414,7 @@ hipError_t launch_trace(int mode, const DevScene &s, const Pose *poses, int cell
   if (items <= 0) return hipSuccess;
   const int R = s.samples * s.samples, slots = 256 / R;
   const long long blocks = (items + slots - 1) / slots;
-  if (blocks > 0x7fffffffLL) return hipErrorInvalidConfiguration;
+  if (blocks > kMaxBlocks256) return hipErrorInvalidConfiguration;
   const size_t lds = trace_lds_bytes(s);
   switch (mode) {
     case MODE_CELLS:
@@ -441,7 +441,7 @@ hipError_t launch_init(int npix, const double *radiance, const PhotoParams &phot
 hipError_t launch_trace_cells_multi(const SegDev *segs, const long long *block_begin, int nseg, long long blocks,
                                     size_t lds, int *status, unsigned long long *shadow_count, hipStream_t st) {
   if (blocks <= 0) return hipSuccess;
-  if (blocks > 0x7fffffffLL) return hipErrorInvalidConfiguration;
+  if (blocks > kMaxBlocks256) return hipErrorInvalidConfiguration;
   trace_cells_multi<<<(unsigned)blocks, 256, lds, st>>>(segs, block_begin, nseg, status, shadow_count);
   return hipGetLastError();
 }

@@ -293,7 +293,16 @@ class Engine {
     // within budget: chunk k covers cells [k*C, (k+1)*C) of every segment.
     int max_cells = 0;
     for (auto &w : work) max_cells = std::max(max_cells, w.cells);
-    const int chunk = (int)std::max<size_t>(1, std::min<size_t>(max_cells, radiance_budget_ / (sizeof(double) * pixel_total)));
+    // Two bounds: radiance scratch bytes, and trace blocks per launch
+    // (chunk * sum(npix / pixels_per_block) + one partial block per segment).
+    double block_rate = 0.0;
+    for (auto &w : work) {
+      const int ss = w.seg->scene->view.samples;
+      block_rate += (double)w.seg->npix / (256 / (ss * ss));
+    }
+    const double block_cap = (double)(dvsdev::kMaxBlocks256 - (long long)work.size()) / std::max(block_rate, 1e-30);
+    const int chunk = (int)std::max<double>(
+        1.0, std::min<double>({(double)max_cells, (double)(radiance_budget_ / (sizeof(double) * pixel_total)), block_cap}));
     const int chunks = (max_cells + chunk - 1) / chunk;
     radiance_.reserve((size_t)chunk * pixel_total, "cell radiance");
     size_t lds = 0;

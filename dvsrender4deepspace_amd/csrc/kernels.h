@@ -6,6 +6,10 @@
 #include "device_optics.hpp"
 
 namespace dvsdev {
+
+// A dispatch counts its grid in work-items as a 32-bit value: launches of
+// 256-thread blocks stay below this many blocks.
+constexpr long long kMaxBlocks256 = 0xffffffffLL / 256;
 enum { MODE_CELLS = 0, MODE_FRAME = 1, MODE_DEPTH = 2 };
 
 // One segment (one sensor on one scene view) of an engine step, resident in

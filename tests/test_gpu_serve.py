@@ -1,0 +1,62 @@
+"""GPU: the served form — continuous batching with footprint admission gives
+every request the same result as the plain per-request path."""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+import pytest
+
+from dvsrender4deepspace_amd import capi, serve, terrain, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(count=10, render_fraction=0.2):
+    base = dataclasses.replace(workloads.DAVIS346, camera=dataclasses.replace(workloads.DAVIS346.camera, samples=2))
+    specs = {"a": terrain.TerrainSpec(size=96, craters=4, seed=1), "b": terrain.TerrainSpec(size=128, craters=6, seed=2)}
+    reqs = serve.request_mix(base, count, 200.0, (3, 12), specs, seed=5, render_fraction=render_fraction)
+    # shrink resolutions for a quick test; keep the mix of three sizes
+    small = {64: (32, 24), 173: (40, 30), 346: (48, 36)}
+    for r in reqs:
+        w, h = small[r.camera.width]
+        f = r.camera.fx * w / r.camera.width
+        r.camera = capi.Camera(w, h, f, f, w / 2, h / 2, 2)
+        # fast motion and low thresholds so a few horizons emit events
+        phys = dataclasses.replace(r.workload.physic, threshold_pos=0.03, threshold_neg=0.03, max_events_per_step=64)
+        r.workload = dataclasses.replace(r.workload, speed=4000.0, yaw_rate_deg=300.0, frames=2, fps=100.0,
+                                         altitude=60.0, physic=phys)
+    meshes = serve.terrain_meshes(specs)
+    geometries = {k: serve.build_scene(meshes[k], base, base.camera) for k in specs}
+    return reqs, meshes, geometries
+
+
+def test_served_equals_plain_per_request(gpu):
+    reqs, meshes, geometries = _setup()
+    served = [dataclasses.replace(r, events=[]) for r in reqs]
+    # a budget that admits about two requests at a time forces queueing
+    budget = 2.5 * max(serve.footprint_bytes(r, 4) for r in reqs)
+    server = serve.Server(geometries, budget_bytes=budget, horizons_per_step=4)
+    server.run(served, realtime=False)
+    event_reqs = sum(1 for r in reqs if not r.frames)
+    assert server.peak_used <= budget and server.peak_active < event_reqs and server.steps > 3
+    plain = [dataclasses.replace(r, events=[]) for r in reqs]
+    serve.run_plain(plain, meshes, realtime=False)
+    resident = [dataclasses.replace(r, events=[]) for r in reqs]
+    serve.run_plain(resident, meshes, realtime=False, scenes_resident=True, geometries=geometries)
+    total = 0
+    for a, b, c in zip(served, plain, resident):
+        assert np.isfinite(a.finished) and a.finished >= a.admitted
+        if a.frames:
+            assert np.array_equal(a.images[0], b.images[0]) and np.array_equal(a.images[1], b.images[1])
+            continue
+        ea, eb, ec = a.event_array(), b.event_array(), c.event_array()
+        total += len(ea)
+        assert ea.tobytes() == eb.tobytes() == ec.tobytes(), f"request {a.id} differs served vs plain"
+    assert total > 0
+
+
+def test_autotune_reports_candidates(gpu):
+    reqs, meshes, geometries = _setup(count=4, render_fraction=0.0)
+    tune = serve.autotune(geometries, reqs, candidates=((2, False), (4, True)))
+    assert len(tune["candidates"]) == 2 and tune["best"]["mrays_per_s"] > 0
