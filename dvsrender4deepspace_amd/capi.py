@@ -131,7 +131,25 @@ class _StatsC(ctypes.Structure):
                 ("total_ms", ctypes.c_double), ("trace_launches", ctypes.c_double),
                 ("primary_rays", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64),
                 ("event_rays", ctypes.c_uint64), ("pixel_cells", ctypes.c_uint64),
-                ("comparator_events", ctypes.c_uint64), ("delivered_events", ctypes.c_uint64)]
+                ("comparator_events", ctypes.c_uint64), ("delivered_events", ctypes.c_uint64),
+                ("native_equivalent_rays", ctypes.c_uint64), ("adaptive_windows", ctypes.c_uint64),
+                ("rejected_windows", ctypes.c_uint64)]
+
+
+class _RefineC(ctypes.Structure):
+    _fields_ = [("traced_pixel_samples", ctypes.c_uint64), ("fallback_pixels", ctypes.c_int32),
+                ("maximum_knots", ctypes.c_int32), ("exact_segments", ctypes.c_int32),
+                ("native_cell_queries", ctypes.c_int32)]
+
+
+class _ProbeC(ctypes.Structure):
+    _fields_ = [("maximum_log_voltage_error", ctypes.c_double), ("p99_log_voltage_error", ctypes.c_double),
+                ("mean_log_voltage_error", ctypes.c_double), ("pixels", ctypes.c_int32),
+                ("circuit_cells", ctypes.c_int32)]
+
+
+def _as_dict(c) -> dict:
+    return {name: getattr(c, name) for name, _ in c._fields_}
 
 
 def _struct(cls, obj):
@@ -174,6 +192,11 @@ def load_library() -> ctypes.CDLL:
         "dvs_sensor_destroy": (ctypes.c_int, [_P]),
         "dvs_sensor_initialize": (ctypes.c_int, [_P, _P, ctypes.c_double]),
         "dvs_sensor_process": (ctypes.c_int, [_P, _P, _I32, ctypes.POINTER(_I64)]),
+        "dvs_sensor_process_adaptive": (ctypes.c_int, [_P, _P, _I32, ctypes.c_double, _I32, ctypes.POINTER(_I32),
+                                                       ctypes.POINTER(_I64)]),
+        "dvs_sensor_adaptive_capacity": (ctypes.c_int, [_P, ctypes.POINTER(_I32)]),
+        "dvs_sensor_refinement_stats": (ctypes.c_int, [_P, ctypes.POINTER(_RefineC)]),
+        "dvs_sensor_last_convergence": (ctypes.c_int, [_P, ctypes.POINTER(_ProbeC)]),
         "dvs_sensor_take": (ctypes.c_int, [_P, _P, _I64, ctypes.POINTER(_I64)]),
         "dvs_sensor_raw_events": (ctypes.c_int, [_P, _P, _I64, ctypes.POINTER(_I64)]),
         "dvs_sensor_drain": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),
@@ -193,6 +216,8 @@ def load_library() -> ctypes.CDLL:
         "dvs_batch_segment_info": (ctypes.c_int, [_P, _I32, ctypes.POINTER(_I64), ctypes.POINTER(_I64),
                                                   ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
         "dvs_batch_remove": (ctypes.c_int, [_P, _I32]),
+        "dvs_batch_set_adaptive": (ctypes.c_int, [_P, _I32, ctypes.c_double, _I32, ctypes.c_double]),
+        "dvs_batch_refinement_stats": (ctypes.c_int, [_P, _I32, ctypes.POINTER(_RefineC)]),
         "dvs_batch_stats": (ctypes.c_int, [_P, ctypes.POINTER(_StatsC)]),
         "dvs_batch_reset_stats": (ctypes.c_int, [_P]),
         "dvs_hapke_eval": (ctypes.c_int, [_P, _P, _P, _P, _I64, ctypes.POINTER(_HapkeC), _P]),
@@ -401,6 +426,33 @@ class Sensor:
         _check(load_library().dvs_sensor_process(self.handle, _ptr(iv), len(iv), ctypes.byref(ready)))
         return self._take(ready.value)
 
+    def process_adaptive(self, intervals: np.ndarray, tolerance: float = 1e-2,
+                         check_convergence: bool = False) -> tuple[bool, np.ndarray]:
+        """process_adaptive over the intervals as one window: (accepted,
+        readout-delivered events).  Not accepted: the receptor check rejected
+        the window and nothing changed."""
+        iv = np.ascontiguousarray(np.atleast_1d(intervals), dtype=INTERVAL_DTYPE)
+        ready, ok = _I64(), _I32()
+        _check(load_library().dvs_sensor_process_adaptive(self.handle, _ptr(iv), len(iv), float(tolerance),
+                                                          int(check_convergence), ctypes.byref(ok),
+                                                          ctypes.byref(ready)))
+        return bool(ok.value), self._take(ready.value)
+
+    def adaptive_capacity(self) -> int:
+        c = _I32()
+        _check(load_library().dvs_sensor_adaptive_capacity(self.handle, ctypes.byref(c)))
+        return c.value
+
+    def refinement_stats(self) -> dict:
+        r = _RefineC()
+        _check(load_library().dvs_sensor_refinement_stats(self.handle, ctypes.byref(r)))
+        return _as_dict(r)
+
+    def last_convergence(self) -> dict:
+        r = _ProbeC()
+        _check(load_library().dvs_sensor_last_convergence(self.handle, ctypes.byref(r)))
+        return _as_dict(r)
+
     def raw_events(self) -> np.ndarray:
         """Comparator events of the last process call, (t, y, x) order, before readout."""
         lib = load_library()
@@ -496,6 +548,18 @@ class Batch:
     def remove(self, segment: int) -> None:
         _check(load_library().dvs_batch_remove(self.handle, int(segment)))
         self._scenes.pop(segment, None)
+
+    def set_adaptive(self, segment: int, tolerance: float = 1e-2, horizons: int = 200,
+                     check_period: float = 0.25) -> None:
+        """Adaptive windows for a segment under the driver policy of
+        apps/main_dvs.cpp:96-116 (see dvs_batch_set_adaptive)."""
+        _check(load_library().dvs_batch_set_adaptive(self.handle, int(segment), float(tolerance), int(horizons),
+                                                     float(check_period)))
+
+    def refinement_stats(self, segment: int) -> dict:
+        r = _RefineC()
+        _check(load_library().dvs_batch_refinement_stats(self.handle, int(segment), ctypes.byref(r)))
+        return _as_dict(r)
 
     def stats(self) -> Stats:
         s = _StatsC()

@@ -1,0 +1,136 @@
+// Adaptive optical refinement on gfx950: the OpticalSampler of the reference
+// (include/dvs/optical_sampler.hpp, src/cuda/optical_sampler.cu) and the
+// knot-interpolating circuit of EventSensor::process_adaptive
+// (src/cuda/event_sensor.cu:88-150,306-374), restructured as wavefronts.
+//
+// The reference refines each pixel depth first (one warp per pixel, a
+// 16-entry stack).  Here every unresolved segment of every pixel is one lane
+// of a level-synchronous wavefront: a level decides all pending segments,
+// compacts the resolved ones and the bisected ones by wave ballot, traces the
+// bisected segments' two new probes as one ray batch and forms the next
+// level.  The result equals the depth-first one:
+//  * a segment's decision depends only on its own values and on the number
+//    of left turns on its path (the depth-first stack height when it is
+//    popped), which each entry carries;
+//  * a pixel overflows its knot capacity in the depth-first walk exactly when
+//    its final knot count exceeds the capacity.  The count's running lower
+//    bound (1 + pending segments + knots written) only grows, so a pixel is
+//    dropped as soon as the bound passes the capacity;
+//  * the depth-first knot order is the order of the resolved segments' start
+//    fractions, recovered by sorting (pixel, start) keys.
+#pragma once
+#include "kernels.h"
+
+namespace dvsdev {
+
+enum { ADAPT_ACCEPT = 0, ADAPT_EXACT = 1 };
+enum { LAYOUT_PROBE = 0, LAYOUT_QUERY = 1, LAYOUT_CELL = 2 };
+
+// Window of one segment in an adaptive step.
+struct AdaptDev {
+  double start, end, tolerance;
+  int capacity;          // knot and exact-cell capacity per pixel
+  int accepted;          // 0 once the convergence gate rejected the window
+  int cell_offset;       // first cell of the segment in the step's cell tables
+  int fb_begin;          // first entry of the segment in the fallback pixel list
+  int fb_count;          // fallback pixels of the segment
+  int pad;
+  const double *fb_rad;  // [cell - chunk start][fb_count] native radiance of the fallback pixels
+};
+
+// An unresolved time segment [a, b] of a pixel (fractions of the window) with
+// its probe radiances and the number of left turns on its path.
+struct AdaptEntry {
+  double a, b, left, mid, right;
+  int pixel, zeros;
+};
+// A resolved segment: accepted (two knots) or exact (one negative knot).
+struct AdaptTerm {
+  double a, b, mid, right;
+  int pixel, kind;
+};
+
+// Per-step arrays over the step's global pixels.
+struct AdaptBufs {
+  const double *probes;  // [P][5] initial probe radiance
+  int *bound;            // running lower bound of each pixel's final knot count
+  int *flags;            // bit 0: whole-pixel native tracing (fallback)
+  int *bisects;          // bisections per pixel (two probes each)
+  int *exact_terms;      // exact segments per pixel
+  int *kcount;           // knot count (0 for fallback pixels)
+  int *koff;             // inclusive prefix of kcount
+  double2 *knots;        // {fraction, radiance}; negative fraction: an exact segment ends here
+  int *ecount;           // exact cells per pixel
+  int *eoff;             // inclusive prefix of ecount
+  int *ecell;            // exact cells (segment-local indices), ascending per pixel
+  double *erad;          // their radiance
+  int *fb_index;         // index of the pixel in its segment's fallback list, -1 otherwise
+  const double *cell_frac;  // window fraction of every cell midpoint of the step
+};
+
+struct AdaptCounters {
+  int terms, bisect, pad0, pad1;
+};
+
+// OpticalRefinementStats of one segment (include/dvs/optical_sampler.hpp:16).
+struct AdaptSegStats {
+  unsigned long long traced_pixel_samples;
+  int fallback_pixels, maximum_knots, exact_segments, native_cell_queries;
+};
+
+}  // namespace dvsdev
+
+namespace dvsamd {
+using dvsdev::AdaptBufs;
+using dvsdev::AdaptCounters;
+using dvsdev::AdaptDev;
+using dvsdev::AdaptEntry;
+using dvsdev::AdaptSegStats;
+using dvsdev::AdaptTerm;
+
+hipError_t launch_adapt_probe_poses(int nseg, const SegDev *segs, const AdaptDev *ad, Pose *out, hipStream_t st);
+hipError_t launch_adapt_cell_frac(int cells, const double *t0, const double *t1, const int *cell_begin, int nseg,
+                                  const AdaptDev *ad, double *frac, hipStream_t st);
+// Radiance of `items` pixel samples: LAYOUT_PROBE item q is probe q%5 of
+// global pixel q/5 (pose probe_poses[segment*5 + q%5]); LAYOUT_QUERY item q
+// is item_pixel[q] at poses[q]; LAYOUT_CELL item q is item_pixel[q] at
+// poses[item_pose[q]].
+hipError_t launch_adapt_trace(int layout, const SegDev *segs, const int *pixel_begin, int nseg, int samples,
+                              long long items, const int *item_pixel, const int *item_pose, const Pose *poses,
+                              size_t lds, double *out, int *status, unsigned long long *shadow_count,
+                              hipStream_t st);
+hipError_t launch_adapt_init(int pixels, AdaptBufs b, AdaptEntry *entries, hipStream_t st);
+hipError_t launch_adapt_decide(int n, const AdaptEntry *entries, const SegDev *segs, const AdaptDev *ad,
+                               const int *pixel_begin, int nseg, AdaptBufs b, AdaptTerm *terms,
+                               unsigned long long *keys, AdaptEntry *bisected, int *query_pixel, Pose *query_pose,
+                               AdaptCounters *ctr, hipStream_t st);
+hipError_t launch_adapt_children(int n, const AdaptEntry *bisected, const double *query_rad, AdaptEntry *next,
+                                 hipStream_t st);
+hipError_t launch_adapt_counts(int pixels, const AdaptDev *ad, const int *pixel_begin, int nseg, AdaptBufs b,
+                               hipStream_t st);
+hipError_t launch_iota(unsigned *p, int n, hipStream_t st);
+hipError_t launch_adapt_term_runs(int n, const unsigned long long *sorted_keys, const unsigned *order,
+                                  const AdaptTerm *terms, AdaptBufs b, int *contrib, int *run_begin,
+                                  hipStream_t st);
+hipError_t launch_adapt_write_knots(int n, const unsigned long long *sorted_keys, const unsigned *order,
+                                    const AdaptTerm *terms, AdaptBufs b, const int *contrib, const int *scan,
+                                    const int *run_begin, hipStream_t st);
+hipError_t launch_adapt_exact_count(int pixels, const SegDev *segs, const AdaptDev *ad, const int *pixel_begin,
+                                    int nseg, AdaptBufs b, hipStream_t st);
+hipError_t launch_adapt_exact_list(int pixels, const SegDev *segs, const AdaptDev *ad, const int *pixel_begin,
+                                   int nseg, AdaptBufs b, int *item_pixel, int *item_pose, hipStream_t st);
+hipError_t launch_adapt_fallback_flags(int pixels, const int *flags, int *out, hipStream_t st);
+hipError_t launch_adapt_fallback_list(int pixels, const AdaptDev *ad, const int *pixel_begin, int nseg,
+                                      const int *scan, AdaptBufs b, int *list, hipStream_t st);
+// out[s] = scan[pixel_begin[s + 1] - 1]: inclusive prefix at each segment's end.
+hipError_t launch_adapt_segment_ends(int nseg, const int *pixel_begin, const int *scan, int *out, hipStream_t st);
+hipError_t launch_adapt_segment_stats(int nseg, const int *pixel_begin, AdaptBufs b, AdaptSegStats *out,
+                                      hipStream_t st);
+// Convergence gate: gate g is segment gate_seg[g] with its probe pixels as
+// gate_segs[g].pixel_list and their native cell radiance in gate_segs[g].rad.
+hipError_t launch_adapt_gate(const SegDev *gate_segs, const int *gate_seg, int ngate, const AdaptDev *ad,
+                             AdaptBufs b, double *errors, hipStream_t st);
+hipError_t launch_adapt_integrate(const SegDev *segs, const AdaptDev *ad, const int *block_begin, int nseg,
+                                  int blocks, AdaptBufs b, int c0, int c1, int *status, hipStream_t st);
+
+}  // namespace dvsamd

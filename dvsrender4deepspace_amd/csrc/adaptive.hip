@@ -1,0 +1,579 @@
+// gfx950 kernels of the adaptive optics step (structure and equivalence
+// argument in adaptive.h).
+//
+//  adapt_trace<LAYOUT>   one lane per sub-ray, ordered FP64 mean per pixel
+//                        sample (warp_radiance, optical_sampler.cu:31-38)
+//  adapt_decide          one lane per pending segment: the accept / exact /
+//                        bisect rule of refine_pixels (optical_sampler.cu:
+//                        50-79); resolved and bisected segments compacted by
+//                        wave ballot + one atomic per wave
+//  adapt_children        the two halves of every bisected segment
+//  adapt_counts / term_runs / write_knots
+//                        knot counts, prefix offsets and depth-first knot order
+//  adapt_exact_*         identify_exact_cells (optical_sampler.cu:114-131)
+//  adapt_gate            advance_mean_probes (event_sensor.cu:138-150)
+//  adapt_integrate       integrate_optical_knots and integrate_fallback_tile
+//                        (event_sensor.cu:88-132): one lane per pixel
+#include <hip/hip_runtime.h>
+
+#include "adaptive.h"
+#include "device_circuit.hpp"
+#include "device_optics.hpp"
+
+namespace dvsdev {
+
+// native_pose (optical_sampler.cu:26-30): the pose at t in the first interval
+// whose end is not before t.
+__device__ inline Pose native_pose(const dvs_optical_interval *ivs, int count, double t) {
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int m = (lo + hi) / 2;
+    if (t > ivs[m].t1) lo = m + 1; else hi = m;
+  }
+  return pose_at(ivs[lo], t);
+}
+
+__global__ void adapt_probe_poses(int nseg, const SegDev *segs, const AdaptDev *ad, Pose *out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nseg * 5) return;
+  const int s = k / 5, j = k - s * 5;
+  const double start = ad[s].start, end = ad[s].end;
+  out[k] = native_pose(segs[s].ivs, segs[s].iv_count, start + (end - start) * j * .25);
+}
+
+__global__ void adapt_cell_frac(int cells, const double *t0, const double *t1, const int *cell_begin, int nseg,
+                                const AdaptDev *ad, double *frac) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cells) return;
+  const AdaptDev &a = ad[segment_of<int>(cell_begin, nseg, c)];
+  frac[c] = ((t0[c] + t1[c]) * .5 - a.start) / (a.end - a.start);
+}
+
+template <int LAYOUT>
+__global__ __launch_bounds__(256) void adapt_trace(const SegDev *segs, const int *pixel_begin, int nseg,
+                                                   long long items, const int *item_pixel, const int *item_pose,
+                                                   const Pose *poses, double *out, int *status,
+                                                   unsigned long long *shadow_count) {
+  extern __shared__ double lds_words[];
+  int *words = reinterpret_cast<int *>(lds_words);
+  int *stack = words + threadIdx.x;
+  const int samples = segs[0].scene.samples, R = samples * samples, slots = 256 / R;
+  const int slot = threadIdx.x / R, sub = threadIdx.x - slot * R;
+  const long long q = (long long)blockIdx.x * slots + slot;
+  const bool active = slot < slots && q < items;
+  SubSample r{0.0, -1.f, 0, 0};
+  if (active) {
+    int gp;
+    Pose pose;
+    if (LAYOUT == LAYOUT_PROBE) {
+      gp = (int)(q / 5);
+      const int si = segment_of<int>(pixel_begin, nseg, gp);
+      pose = poses[si * 5 + (int)(q - (long long)gp * 5)];
+    } else if (LAYOUT == LAYOUT_QUERY) {
+      gp = item_pixel[q];
+      pose = poses[q];
+    } else {
+      gp = item_pixel[q];
+      pose = poses[item_pose[q]];
+    }
+    const int si = segment_of<int>(pixel_begin, nseg, gp);
+    r = sample_subray<true>(segs[si].scene, gp - pixel_begin[si], pose, sub % samples, sub / samples, stack, 256);
+  }
+  stack[0] = __double2loint(r.radiance);
+  stack[256] = __double2hiint(r.radiance);
+  const unsigned long long shadows = __ballot(r.shadow);
+  const unsigned long long invalid = __ballot(r.invalid);
+  if ((threadIdx.x & 63) == 0) {
+    if (shadows) atomicAdd(shadow_count, (unsigned long long)__popcll(shadows));
+    if (invalid) atomicOr(status, 2);
+  }
+  __syncthreads();
+  if (!active || sub != 0) return;
+  double total = 0;
+  for (int j = 0; j < R; ++j) total += __hiloint2double(words[256 + threadIdx.x + j], words[threadIdx.x + j]);
+  if (!isfinite(total)) atomicOr(status, 2);
+  out[q] = total / R;
+}
+
+// Two root segments per pixel: [0, 1/2] is popped first with one pending
+// sibling on the depth-first stack, [1/2, 1] with none (refine_pixels, :46-49).
+__global__ void adapt_init(int pixels, AdaptBufs b, AdaptEntry *entries) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= pixels) return;
+  const double *r = b.probes + (size_t)p * 5;
+  entries[2 * p] = AdaptEntry{0.0, .5, r[0], r[1], r[2], p, 1};
+  entries[2 * p + 1] = AdaptEntry{.5, 1.0, r[2], r[3], r[4], p, 0};
+  b.bound[p] = 3;  // knot {0, r0} + two pending segments
+  b.flags[p] = 0;
+  b.bisects[p] = 0;
+  b.exact_terms[p] = 0;
+}
+
+__global__ __launch_bounds__(256) void adapt_decide(int n, const AdaptEntry *entries, const SegDev *segs,
+                                                    const AdaptDev *ad, const int *pixel_begin, int nseg,
+                                                    AdaptBufs b, AdaptTerm *terms, unsigned long long *keys,
+                                                    AdaptEntry *bisected, int *query_pixel, Pose *query_pose,
+                                                    AdaptCounters *ctr) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int decision = -1;  // -1 dropped, ADAPT_ACCEPT, ADAPT_EXACT, 2 bisect
+  AdaptEntry e{};
+  int si = 0;
+  if (k < n) {
+    e = entries[k];
+    si = segment_of<int>(pixel_begin, nseg, e.pixel);
+    const AdaptDev &A = ad[si];
+    if (!(__hip_atomic_load(&b.flags[e.pixel], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 1)) {
+      if (__hip_atomic_load(&b.bound[e.pixel], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > A.capacity) {
+        atomicOr(&b.flags[e.pixel], 1);  // the knot count can only exceed the capacity
+      } else {
+        const SegDev &g = segs[si];
+        const dvs_pixel_state &st = g.states[e.pixel - pixel_begin[si]];
+        const PhotoParams ph = g.photo;
+        const double charge = g.params.charge_scale;
+        const double actual = e.mid * ph.electrons_per_radiance + ph.dark_electrons;
+        const double estimate = (e.left + e.right) * .5 * ph.electrons_per_radiance + ph.dark_electrons;
+        const double residual = fabs(log(fmax(1e-30, actual) / fmax(1e-30, estimate)));
+        const double leaf = (e.b - e.a) * (A.end - A.start);
+        const double verr = fabs(receptor_log_at(st.receptor_log, actual, leaf, charge) -
+                                 receptor_log_at(st.receptor_log, estimate, leaf, charge));
+        const double budget = 1e-3 * fmin(st.threshold_pos, st.threshold_neg);
+        if (verr <= budget && (residual <= A.tolerance || fmax(actual, estimate) * leaf < 1.0))
+          decision = ADAPT_ACCEPT;
+        else if (leaf <= ph.integration_step || e.zeros + 2 > 16)
+          decision = ADAPT_EXACT;
+        else
+          decision = 2;
+      }
+    }
+  }
+  const bool resolved = decision == ADAPT_ACCEPT || decision == ADAPT_EXACT;
+  const unsigned long long tmask = __ballot(resolved);
+  const unsigned long long bmask = __ballot(decision == 2);
+  const int lane = threadIdx.x & 63;
+  int tbase = 0, bbase = 0;
+  if (lane == 0) {
+    if (tmask) tbase = atomicAdd(&ctr->terms, __popcll(tmask));
+    if (bmask) bbase = atomicAdd(&ctr->bisect, __popcll(bmask));
+  }
+  tbase = __shfl(tbase, 0);
+  bbase = __shfl(bbase, 0);
+  const unsigned long long below = (1ull << lane) - 1ull;
+  if (resolved) {
+    const int idx = tbase + __popcll(tmask & below);
+    terms[idx] = AdaptTerm{e.a, e.b, e.mid, e.right, e.pixel, decision};
+    // start fractions are dyadic with at most 21 bits below the point
+    keys[idx] = ((unsigned long long)e.pixel << 24) | (unsigned long long)(e.a * 16777216.0);
+    if (decision == ADAPT_ACCEPT) atomicAdd(&b.bound[e.pixel], 1);
+    else atomicAdd(&b.exact_terms[e.pixel], 1);
+  } else if (decision == 2) {
+    const int idx = bbase + __popcll(bmask & below);
+    bisected[idx] = e;
+    atomicAdd(&b.bound[e.pixel], 1);
+    atomicAdd(&b.bisects[e.pixel], 1);
+    const AdaptDev &A = ad[si];
+    const SegDev &g = segs[si];
+    const double mid = (e.a + e.b) * .5;
+    query_pixel[2 * idx] = e.pixel;
+    query_pixel[2 * idx + 1] = e.pixel;
+    query_pose[2 * idx] = native_pose(g.ivs, g.iv_count, A.start + (A.end - A.start) * (e.a + mid) * .5);
+    query_pose[2 * idx + 1] = native_pose(g.ivs, g.iv_count, A.start + (A.end - A.start) * (mid + e.b) * .5);
+  }
+}
+
+// Halves of bisected segment k: the left half is popped before the right
+// one, with the right half pending on the stack (refine_pixels, :76-78).
+__global__ void adapt_children(int n, const AdaptEntry *bisected, const double *query_rad, AdaptEntry *next) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const AdaptEntry e = bisected[k];
+  const double mid = (e.a + e.b) * .5;
+  next[2 * k] = AdaptEntry{e.a, mid, e.left, query_rad[2 * k], e.mid, e.pixel, e.zeros + 1};
+  next[2 * k + 1] = AdaptEntry{mid, e.b, e.mid, query_rad[2 * k + 1], e.right, e.pixel, e.zeros};
+}
+
+__global__ void adapt_counts(int pixels, const AdaptDev *ad, const int *pixel_begin, int nseg, AdaptBufs b) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= pixels) return;
+  const AdaptDev &A = ad[segment_of<int>(pixel_begin, nseg, p)];
+  int f = b.flags[p];
+  if (b.bound[p] > A.capacity) f |= 1;
+  b.flags[p] = f;
+  b.kcount[p] = (f & 1) ? 0 : b.bound[p];
+}
+
+__global__ void iota_kernel(unsigned *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = (unsigned)i;
+}
+
+// Sorted resolved segments: knots each contributes and each pixel's first one.
+__global__ void adapt_term_runs(int n, const unsigned long long *keys, const unsigned *order, const AdaptTerm *terms,
+                                AdaptBufs b, int *contrib, int *run_begin) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int p = (int)(keys[t] >> 24);
+  contrib[t] = (b.flags[p] & 1) ? 0 : (terms[order[t]].kind == ADAPT_ACCEPT ? 2 : 1);
+  if (t == 0 || (int)(keys[t - 1] >> 24) != p) run_begin[p] = t;
+}
+
+__global__ void adapt_write_knots(int n, const unsigned long long *keys, const unsigned *order,
+                                  const AdaptTerm *terms, AdaptBufs b, const int *contrib, const int *scan,
+                                  const int *run_begin) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int p = (int)(keys[t] >> 24);
+  if (b.flags[p] & 1) return;
+  const int base = b.koff[p] - b.kcount[p];
+  const int rb = run_begin[p];
+  const int pos = base + 1 + (scan[t] - contrib[t]) - (scan[rb] - contrib[rb]);
+  const AdaptTerm T = terms[order[t]];
+  if (T.kind == ADAPT_ACCEPT) {
+    b.knots[pos] = make_double2((T.a + T.b) * .5, T.mid);
+    b.knots[pos + 1] = make_double2(T.b, T.right);
+  } else {
+    b.knots[pos] = make_double2(-T.b, T.right);
+  }
+  if (t == rb) b.knots[base] = make_double2(0.0, b.probes[(size_t)p * 5]);
+}
+
+// Knot bracket of a cell midpoint fraction (integrate_optical_knots, :99).
+__device__ __forceinline__ int knot_bracket(const double2 *v, int size, double fraction, int a) {
+  while (a + 2 < size && fraction > fabs(v[a + 1].x)) ++a;
+  return a;
+}
+
+__global__ void adapt_exact_count(int pixels, const SegDev *segs, const AdaptDev *ad, const int *pixel_begin,
+                                  int nseg, AdaptBufs b) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= pixels) return;
+  b.ecount[p] = 0;
+  if (b.flags[p] & 1) return;
+  const int si = segment_of<int>(pixel_begin, nseg, p);
+  const AdaptDev &A = ad[si];
+  const int cells = segs[si].cells, size = b.kcount[p];
+  const double2 *v = b.knots + (b.koff[p] - size);
+  const double *frac = b.cell_frac + A.cell_offset;
+  int a = 0, used = 0;
+  for (int c = 0; c < cells; ++c) {
+    a = knot_bracket(v, size, frac[c], a);
+    if (v[a + 1].x < 0) {
+      if (used == A.capacity) {  // identify_exact_cells overflow: whole-pixel tracing
+        b.flags[p] |= 1;
+        return;
+      }
+      ++used;
+    }
+  }
+  b.ecount[p] = used;
+}
+
+__global__ void adapt_exact_list(int pixels, const SegDev *segs, const AdaptDev *ad, const int *pixel_begin,
+                                 int nseg, AdaptBufs b, int *item_pixel, int *item_pose) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= pixels || b.ecount[p] == 0) return;
+  const int si = segment_of<int>(pixel_begin, nseg, p);
+  const AdaptDev &A = ad[si];
+  const int cells = segs[si].cells, size = b.kcount[p];
+  const double2 *v = b.knots + (b.koff[p] - size);
+  const double *frac = b.cell_frac + A.cell_offset;
+  int a = 0, at = b.eoff[p] - b.ecount[p];
+  for (int c = 0; c < cells; ++c) {
+    a = knot_bracket(v, size, frac[c], a);
+    if (v[a + 1].x < 0) {
+      b.ecell[at] = c;
+      item_pixel[at] = p;
+      item_pose[at] = A.cell_offset + c;
+      ++at;
+    }
+  }
+}
+
+__global__ void adapt_fallback_flags(int pixels, const int *flags, int *out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < pixels) out[p] = flags[p] & 1;
+}
+
+__global__ void adapt_fallback_list(int pixels, const int *pixel_begin, int nseg, const int *scan, AdaptBufs b,
+                                    int *list) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= pixels) return;
+  if (!(b.flags[p] & 1)) {
+    b.fb_index[p] = -1;
+    return;
+  }
+  const int si = segment_of<int>(pixel_begin, nseg, p);
+  const int begin = pixel_begin[si] ? scan[pixel_begin[si] - 1] : 0;
+  list[scan[p] - 1] = p - pixel_begin[si];
+  b.fb_index[p] = scan[p] - 1 - begin;
+}
+
+__global__ void adapt_segment_ends(int nseg, const int *pixel_begin, const int *scan, int *out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nseg) return;
+  out[s] = pixel_begin[s + 1] > 0 ? scan[pixel_begin[s + 1] - 1] : 0;
+}
+
+// One block per segment: OpticalRefinementStats over its pixels.
+__global__ __launch_bounds__(256) void adapt_segment_stats(const int *pixel_begin, AdaptBufs b, AdaptSegStats *out) {
+  __shared__ unsigned long long q[256];
+  __shared__ int fb[256], mk[256], ex[256], nc[256];
+  const int s = blockIdx.x;
+  unsigned long long queries = 0;
+  int fallback = 0, max_knots = 0, exact = 0, cells = 0;
+  for (int p = pixel_begin[s] + threadIdx.x; p < pixel_begin[s + 1]; p += 256) {
+    queries += 5 + 2ull * (unsigned)b.bisects[p];
+    fallback += b.flags[p] & 1;
+    max_knots = max(max_knots, b.kcount[p]);
+    exact += b.exact_terms[p];
+    cells += b.ecount[p];
+  }
+  q[threadIdx.x] = queries;
+  fb[threadIdx.x] = fallback;
+  mk[threadIdx.x] = max_knots;
+  ex[threadIdx.x] = exact;
+  nc[threadIdx.x] = cells;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      q[threadIdx.x] += q[threadIdx.x + w];
+      fb[threadIdx.x] += fb[threadIdx.x + w];
+      mk[threadIdx.x] = max(mk[threadIdx.x], mk[threadIdx.x + w]);
+      ex[threadIdx.x] += ex[threadIdx.x + w];
+      nc[threadIdx.x] += nc[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[s] = AdaptSegStats{q[0], fb[0], mk[0], ex[0], nc[0]};
+}
+
+// Receptor log voltage of each probe pixel under native and under cached
+// (knot-interpolated) radiance (advance_mean_probes, event_sensor.cu:138-150).
+__global__ void adapt_gate(const SegDev *gate_segs, const int *gate_seg, const AdaptDev *ad, AdaptBufs b,
+                           double *errors) {
+  const int g = blockIdx.x / 4, i = (blockIdx.x & 3) * 256 + threadIdx.x;
+  const SegDev &G = gate_segs[g];
+  if (i >= G.npix) return;
+  const AdaptDev &A = ad[gate_seg[g]];
+  const int local = G.pixel_list[i], gp = G.pixel_offset + local;
+  const double charge = G.params.charge_scale;
+  const PhotoParams ph = G.photo;
+  double ex = G.states[local].receptor_log, ca = ex;
+  const bool knots = !(b.flags[gp] & 1);
+  const int size = b.kcount[gp];
+  const double2 *v = b.knots + (b.koff[gp] - size);
+  const double *frac = b.cell_frac + A.cell_offset;
+  for (int c = 0; c < G.cells; ++c) {
+    const double light = G.rad[(size_t)c * G.npix + i];
+    double cached = light;
+    if (knots) {
+      const double f = frac[c];
+      const int a = knot_bracket(v, size, f, 0);
+      const double w = (f - fabs(v[a].x)) / (fabs(v[a + 1].x) - fabs(v[a].x));
+      if (v[a + 1].x >= 0) cached = v[a].y + (v[a + 1].y - v[a].y) * w;
+    }
+    const double dt = G.t1[c] - G.t0[c];
+    ex = receptor_log_at(ex, light * ph.electrons_per_radiance + ph.dark_electrons, dt, charge);
+    ca = receptor_log_at(ca, fmax(0.0, cached) * ph.electrons_per_radiance + ph.dark_electrons, dt, charge);
+  }
+  errors[g * 1024 + i] = fabs(ex - ca);
+}
+
+// Circuit of cells [c0, c1) of every pixel of an accepted window: knot or
+// exact-cell radiance, or the native radiance of a fallback pixel; shot noise
+// and comparator as in the native circuit (same per-pixel draw order).
+__global__ __launch_bounds__(256) void adapt_integrate(const SegDev *segs, const AdaptDev *ad, const int *block_begin,
+                                                       int nseg, AdaptBufs b, int c0, int c1, int *status) {
+  const int si = segment_of<int>(block_begin, nseg, (int)blockIdx.x);
+  const SegDev &g = segs[si];
+  const AdaptDev &A = ad[si];
+  const int i = ((int)blockIdx.x - block_begin[si]) * 256 + threadIdx.x;
+  if (i >= g.npix || !A.accepted) return;
+  const int gp = g.pixel_offset + i;
+  dvs_pixel_state st = g.states[i];
+  PixelRegs regs{st.receptor_log, st.reference_log, st.refractory_until, st.threshold_pos,
+                 st.threshold_neg, st.reset_pending};
+  NoiseStream shot(g.seed, (uint64_t)i, kStreamShot, st.shot_block);
+  NoiseStream reset(g.seed, (uint64_t)i, kStreamReset, st.reset_block);
+  auto normal = [&]() { return reset.normal(); };
+  CrossingEvent *out = g.slots + (size_t)i * g.max_events;
+  const CircuitParams p = g.params;
+  const PhotoParams photo = g.photo;
+  int count = g.counts[i];
+  const bool fallback = b.flags[gp] & 1;
+  const int fbi = b.fb_index[gp];
+  const int size = b.kcount[gp];
+  const double2 *v = b.knots + (b.koff[gp] - size);
+  const int ec = b.ecount[gp], eo = b.eoff[gp] - ec;
+  const double *frac = b.cell_frac + A.cell_offset;
+  int a = 0, e = 0;
+  if (!fallback)
+    while (e < ec && b.ecell[eo + e] < c0) ++e;
+  const int c_end = min(c1, g.cells);
+  for (int c = c0; c < c_end; ++c) {
+    const double t0 = g.t0[c], t1 = g.t1[c];
+    double light;
+    if (fallback) {
+      light = A.fb_rad[(size_t)(c - c0) * A.fb_count + fbi];
+    } else {
+      const double f = frac[c];
+      a = knot_bracket(v, size, f, a);
+      const double w = (f - fabs(v[a].x)) / (fabs(v[a + 1].x) - fabs(v[a].x));
+      light = v[a].y + (v[a + 1].y - v[a].y) * w;
+      if (e < ec && b.ecell[eo + e] == c) light = b.erad[eo + e++];
+      light = fmax(0.0, light);
+    }
+    if (!isfinite(light)) { atomicOr(status, 2); g.counts[i] = 0; return; }
+    double rate = light * photo.electrons_per_radiance + photo.dark_electrons;
+    if (photo.shot_noise) rate = shot.poisson(rate * (t1 - t0)) / (t1 - t0);
+    const int emitted = advance_pixel(regs, p, rate, t0, t1, out + count, g.max_events - count, normal);
+    if (emitted < 0) { atomicOr(status, 1); g.counts[i] = 0; return; }
+    count += emitted;
+  }
+  st.receptor_log = regs.receptor_log;
+  st.reference_log = regs.reference_log;
+  st.refractory_until = regs.refractory_until;
+  st.threshold_pos = regs.threshold_pos;
+  st.threshold_neg = regs.threshold_neg;
+  st.reset_pending = regs.reset_pending;
+  st.shot_block = shot.block;
+  st.reset_block = (uint32_t)reset.block;
+  g.states[i] = st;
+  g.counts[i] = count;
+}
+
+}  // namespace dvsdev
+
+namespace dvsamd {
+using namespace dvsdev;
+
+namespace {
+inline unsigned grid(long long n, int block = 256) { return (unsigned)((n + block - 1) / block); }
+}  // namespace
+
+hipError_t launch_adapt_probe_poses(int nseg, const SegDev *segs, const AdaptDev *ad, Pose *out, hipStream_t st) {
+  if (nseg <= 0) return hipSuccess;
+  adapt_probe_poses<<<grid(nseg * 5), 256, 0, st>>>(nseg, segs, ad, out);
+  return hipGetLastError();
+}
+hipError_t launch_adapt_cell_frac(int cells, const double *t0, const double *t1, const int *cell_begin, int nseg,
+                                  const AdaptDev *ad, double *frac, hipStream_t st) {
+  if (cells <= 0) return hipSuccess;
+  adapt_cell_frac<<<grid(cells), 256, 0, st>>>(cells, t0, t1, cell_begin, nseg, ad, frac);
+  return hipGetLastError();
+}
+hipError_t launch_adapt_trace(int layout, const SegDev *segs, const int *pixel_begin, int nseg, int samples,
+                              long long items, const int *item_pixel, const int *item_pose, const Pose *poses,
+                              size_t lds, double *out, int *status, unsigned long long *shadow_count,
+                              hipStream_t st) {
+  if (items <= 0) return hipSuccess;
+  const int slots = 256 / (samples * samples);
+  const long long blocks = (items + slots - 1) / slots;
+  if (blocks > kMaxBlocks256) return hipErrorInvalidConfiguration;
+  switch (layout) {
+    case LAYOUT_PROBE:
+      adapt_trace<LAYOUT_PROBE><<<(unsigned)blocks, 256, lds, st>>>(segs, pixel_begin, nseg, items, item_pixel,
+                                                                     item_pose, poses, out, status, shadow_count);
+      break;
+    case LAYOUT_QUERY:
+      adapt_trace<LAYOUT_QUERY><<<(unsigned)blocks, 256, lds, st>>>(segs, pixel_begin, nseg, items, item_pixel,
+                                                                     item_pose, poses, out, status, shadow_count);
+      break;
+    default:
+      adapt_trace<LAYOUT_CELL><<<(unsigned)blocks, 256, lds, st>>>(segs, pixel_begin, nseg, items, item_pixel,
+                                                                    item_pose, poses, out, status, shadow_count);
+  }
+  return hipGetLastError();
+}
+hipError_t launch_adapt_init(int pixels, AdaptBufs b, AdaptEntry *entries, hipStream_t st) {
+  if (pixels <= 0) return hipSuccess;
+  adapt_init<<<grid(pixels), 256, 0, st>>>(pixels, b, entries);
+  return hipGetLastError();
+}
+hipError_t launch_adapt_decide(int n, const AdaptEntry *entries, const SegDev *segs, const AdaptDev *ad,
+                               const int *pixel_begin, int nseg, AdaptBufs b, AdaptTerm *terms,
+                               unsigned long long *keys, AdaptEntry *bisected, int *query_pixel, Pose *query_pose,
+                               AdaptCounters *ctr, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  adapt_decide<<<grid(n), 256, 0, st>>>(n, entries, segs, ad, pixel_begin, nseg, b, terms, keys, bisected,
+                                         query_pixel, query_pose, ctr);
+  return hipGetLastError();
+}
+hipError_t launch_adapt_children(int n, const AdaptEntry *bisected, const double *query_rad, AdaptEntry *next,
+                                 hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  adapt_children<<<grid(n), 256, 0, st>>>(n, bisected, query_rad, next);
+  return hipGetLastError();
+}
+hipError_t launch_adapt_counts(int pixels, const AdaptDev *ad, const int *pixel_begin, int nseg, AdaptBufs b,
+                               hipStream_t st) {
+  if (pixels <= 0) return hipSuccess;
+  adapt_counts<<<grid(pixels), 256, 0, st>>>(pixels, ad, pixel_begin, nseg, b);
+  return hipGetLastError();
+}
+hipError_t launch_iota(unsigned *p, int n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  iota_kernel<<<grid(n), 256, 0, st>>>(p, n);
+  return hipGetLastError();
+}
+hipError_t launch_adapt_term_runs(int n, const unsigned long long *keys, const unsigned *order,
+                                  const AdaptTerm *terms, AdaptBufs b, int *contrib, int *run_begin,
+                                  hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  adapt_term_runs<<<grid(n), 256, 0, st>>>(n, keys, order, terms, b, contrib, run_begin);
+  return hipGetLastError();
+}
+hipError_t launch_adapt_write_knots(int n, const unsigned long long *keys, const unsigned *order,
+                                    const AdaptTerm *terms, AdaptBufs b, const int *contrib, const int *scan,
+                                    const int *run_begin, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  adapt_write_knots<<<grid(n), 256, 0, st>>>(n, keys, order, terms, b, contrib, scan, run_begin);
+  return hipGetLastError();
+}
+hipError_t launch_adapt_exact_count(int pixels, const SegDev *segs, const AdaptDev *ad, const int *pixel_begin,
+                                    int nseg, AdaptBufs b, hipStream_t st) {
+  if (pixels <= 0) return hipSuccess;
+  adapt_exact_count<<<grid(pixels), 256, 0, st>>>(pixels, segs, ad, pixel_begin, nseg, b);
+  return hipGetLastError();
+}
+hipError_t launch_adapt_exact_list(int pixels, const SegDev *segs, const AdaptDev *ad, const int *pixel_begin,
+                                   int nseg, AdaptBufs b, int *item_pixel, int *item_pose, hipStream_t st) {
+  if (pixels <= 0) return hipSuccess;
+  adapt_exact_list<<<grid(pixels), 256, 0, st>>>(pixels, segs, ad, pixel_begin, nseg, b, item_pixel, item_pose);
+  return hipGetLastError();
+}
+hipError_t launch_adapt_fallback_flags(int pixels, const int *flags, int *out, hipStream_t st) {
+  if (pixels <= 0) return hipSuccess;
+  adapt_fallback_flags<<<grid(pixels), 256, 0, st>>>(pixels, flags, out);
+  return hipGetLastError();
+}
+hipError_t launch_adapt_fallback_list(int pixels, const AdaptDev *, const int *pixel_begin, int nseg,
+                                      const int *scan, AdaptBufs b, int *list, hipStream_t st) {
+  if (pixels <= 0) return hipSuccess;
+  adapt_fallback_list<<<grid(pixels), 256, 0, st>>>(pixels, pixel_begin, nseg, scan, b, list);
+  return hipGetLastError();
+}
+hipError_t launch_adapt_segment_ends(int nseg, const int *pixel_begin, const int *scan, int *out, hipStream_t st) {
+  if (nseg <= 0) return hipSuccess;
+  adapt_segment_ends<<<grid(nseg), 256, 0, st>>>(nseg, pixel_begin, scan, out);
+  return hipGetLastError();
+}
+hipError_t launch_adapt_segment_stats(int nseg, const int *pixel_begin, AdaptBufs b, AdaptSegStats *out,
+                                      hipStream_t st) {
+  if (nseg <= 0) return hipSuccess;
+  adapt_segment_stats<<<nseg, 256, 0, st>>>(pixel_begin, b, out);
+  return hipGetLastError();
+}
+hipError_t launch_adapt_gate(const SegDev *gate_segs, const int *gate_seg, int ngate, const AdaptDev *ad,
+                             AdaptBufs b, double *errors, hipStream_t st) {
+  if (ngate <= 0) return hipSuccess;
+  adapt_gate<<<ngate * 4, 256, 0, st>>>(gate_segs, gate_seg, ad, b, errors);
+  return hipGetLastError();
+}
+hipError_t launch_adapt_integrate(const SegDev *segs, const AdaptDev *ad, const int *block_begin, int nseg,
+                                  int blocks, AdaptBufs b, int c0, int c1, int *status, hipStream_t st) {
+  if (blocks <= 0 || c1 <= c0) return hipSuccess;
+  adapt_integrate<<<blocks, 256, 0, st>>>(segs, ad, block_begin, nseg, b, c0, c1, status);
+  return hipGetLastError();
+}
+
+}  // namespace dvsamd

@@ -67,6 +67,17 @@ template <class T> struct DeviceArray {
     reserve(n, what);
     if (n) hip_check(hipMemcpyAsync(ptr, host, n * sizeof(T), hipMemcpyHostToDevice, st), what);
   }
+  // Grows to at least n elements, keeping the first `used` (stream ordered).
+  void grow_keep(size_t n, size_t used, const char *what, hipStream_t st) {
+    if (n <= count) return;
+    T *bigger = nullptr;
+    hip_check(hipMalloc(&bigger, n * sizeof(T)), what);
+    if (used) hip_check(hipMemcpyAsync(bigger, ptr, used * sizeof(T), hipMemcpyDeviceToDevice, st), what);
+    hip_check(hipStreamSynchronize(st), what);
+    release();
+    ptr = bigger;
+    count = n;
+  }
   size_t bytes() const { return count * sizeof(T); }
 };
 

@@ -133,16 +133,6 @@ __global__ void init_kernel(int npix, const double *radiance, PhotoParams photo,
 // ---------------------------------------------------------------------------
 // Multi-segment engine step
 // ---------------------------------------------------------------------------
-// Last segment whose first block is <= b (begin[0] == 0).
-template <class T>
-__device__ __forceinline__ int segment_of(const T *begin, int nseg, T b) {
-  int lo = 0, hi = nseg - 1;
-  while (lo < hi) {
-    const int m = (lo + hi + 1) >> 1;
-    if (begin[m] <= b) lo = m; else hi = m - 1;
-  }
-  return lo;
-}
 
 // Radiance of every (pixel, circuit cell) of every segment: the optical half
 // of generate_pixels (src/cuda/event_sensor.cu:39-42).  Items of one block
@@ -166,7 +156,8 @@ __global__ __launch_bounds__(256) void trace_cells_multi(const SegDev *segs, con
     pixel = (int)(q / cells);
     cell = (int)(q - (long long)pixel * cells);
     const Pose pose = g.poses[cell];
-    r = sample_subray<true>(s, pixel, pose, sub % s.samples, sub / s.samples, stack, 256);
+    const int px = g.pixel_list ? g.pixel_list[pixel] : pixel;
+    r = sample_subray<true>(s, px, pose, sub % s.samples, sub / s.samples, stack, 256);
   }
   stack[0] = __double2loint(r.radiance);
   stack[256] = __double2hiint(r.radiance);

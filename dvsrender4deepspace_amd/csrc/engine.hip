@@ -22,6 +22,7 @@
 #include <memory>
 #include <vector>
 
+#include "adaptive.h"
 #include "common.h"
 #include "dvs_hip.h"
 #include "events_sort.h"
@@ -106,6 +107,16 @@ struct Segment {
   Readout readout;
   std::vector<dvs_event> ready, raw;
   int64_t horizons_done = 0, events_total = 0;
+  // adaptive optics: the host driver policy of apps/main_dvs.cpp:96-116 per
+  // segment (window of `abatch` horizons, halved on a rejected check; a
+  // window of one horizon is then integrated natively; the receptor check
+  // runs at the first window and every check_period seconds after).
+  bool adaptive = false;
+  double tolerance = 1e-2, check_period = .25, next_check = 0;
+  int abatch = 1, capacity = 0;
+  dvs_refinement_stats last_refine{};
+  dvs_receptor_probe last_probe{};
+  bool last_accepted = true;
   ~Segment() { scene_release(scene); }
 };
 
@@ -114,6 +125,8 @@ struct StepSegment {
   Segment *seg;
   std::vector<dvs_optical_interval> ivs;
   int cells = 0, cell_offset = 0, iv_offset = 0, pixel_offset = 0;
+  double tolerance = 0;  // adaptive windows only
+  bool check = false;
 };
 
 struct GraphKey {
@@ -218,45 +231,155 @@ class Engine {
   // One step over every segment with queued intervals; returns the number of
   // segments that still have queued intervals afterwards.
   int step(int only = -1) {
-    std::vector<StepSegment> work;
-    int pixel_total = 0, cell_total = 0, iv_total = 0;
+    struct Group {
+      std::vector<StepSegment> work;
+      int pixels = 0, cells = 0, ivs = 0;
+      void add(StepSegment w) {
+        w.pixel_offset = pixels;
+        w.cell_offset = cells;
+        w.iv_offset = ivs;
+        pixels += w.seg->npix;
+        ivs += (int)w.ivs.size();
+        for (auto &iv : w.ivs) cells += (int)std::ceil((iv.t1 - iv.t0) / w.seg->photo.integration_step);
+        work.push_back(std::move(w));
+      }
+    };
+    Group native, adaptive;
     for (auto &kv : segments_) {
       Segment &s = *kv.second;
       if ((only >= 0 && s.id != only) || s.queue.empty()) continue;
       StepSegment w;
       w.seg = &s;
-      const size_t take = std::min<size_t>(s.queue.size(), (size_t)horizons_);
+      const size_t take = std::min<size_t>(s.queue.size(), (size_t)(s.adaptive ? s.abatch : horizons_));
       w.ivs.assign(s.queue.begin(), s.queue.begin() + take);
-      double cells_estimate = 0;
-      for (auto &iv : w.ivs) cells_estimate += (iv.t1 - iv.t0) / s.photo.integration_step;
-      if (cells_estimate > 1000000) fail(DVS_E_ARGUMENT, "optical horizon too long; reduce batch size");
-      w.pixel_offset = pixel_total;
-      w.cell_offset = cell_total;
-      w.iv_offset = iv_total;
-      pixel_total += s.npix;
-      iv_total += (int)w.ivs.size();
-      for (auto &iv : w.ivs) cell_total += (int)std::ceil((iv.t1 - iv.t0) / s.photo.integration_step);
-      work.push_back(std::move(w));
+      if (s.adaptive) {
+        w.tolerance = s.tolerance;
+        w.check = s.check_period >= 0 && w.ivs.front().t0 >= s.next_check;
+        check_adaptive_window(w);
+        adaptive.add(std::move(w));
+      } else {
+        double cells_estimate = 0;
+        for (auto &iv : w.ivs) cells_estimate += (iv.t1 - iv.t0) / s.photo.integration_step;
+        if (cells_estimate > 1000000) fail(DVS_E_ARGUMENT, "optical horizon too long; reduce batch size");
+        native.add(std::move(w));
+      }
     }
-    if (work.empty()) return 0;
-    run(work, pixel_total, cell_total, iv_total);
-    int pending = 0;
-    for (auto &w : work) {
+    auto consume = [](StepSegment &w) {
       w.seg->queue.erase(w.seg->queue.begin(), w.seg->queue.begin() + w.ivs.size());
       w.seg->horizons_done += (int64_t)w.ivs.size();
+    };
+    if (!native.work.empty()) {
+      run(native.work, native.pixels, native.cells, native.ivs);
+      for (auto &w : native.work) consume(w);
     }
+    if (!adaptive.work.empty()) {
+      std::vector<char> accepted;
+      run_adaptive(adaptive.work, adaptive.pixels, adaptive.cells, adaptive.ivs, accepted);
+      Group retry;
+      for (size_t i = 0; i < adaptive.work.size(); ++i) {
+        StepSegment &w = adaptive.work[i];
+        Segment &s = *w.seg;
+        if (accepted[i]) {
+          consume(w);
+          if (w.check) s.next_check = w.ivs.back().t1 + s.check_period;
+        } else if (s.abatch > 1) {
+          s.abatch = std::max(1, s.abatch / 2);  // retried with the shorter window at the next step
+        } else {
+          StepSegment again;
+          again.seg = &s;
+          again.ivs = w.ivs;
+          again.check = w.check;
+          retry.add(std::move(again));
+        }
+      }
+      if (!retry.work.empty()) {
+        run(retry.work, retry.pixels, retry.cells, retry.ivs);
+        for (auto &w : retry.work) {
+          consume(w);
+          if (w.check) w.seg->next_check = w.ivs.back().t1 + w.seg->check_period;
+        }
+      }
+    }
+    int pending = 0;
     for (auto &kv : segments_) pending += !kv.second->queue.empty();
     return pending;
+  }
+
+  // process_adaptive's argument checks on a window before it is queued.
+  void validate_adaptive(int id, const dvs_optical_interval *ivs, int n, double tolerance) {
+    if (!ivs || n <= 0) fail(DVS_E_ARGUMENT, "empty adaptive optical horizon");
+    StepSegment w;
+    w.seg = &segment(id);
+    w.ivs.assign(ivs, ivs + n);
+    w.tolerance = tolerance;
+    check_adaptive_window(w);
+  }
+
+  int capacity_of(int id) {
+    Segment &s = segment(id);
+    if (!s.capacity) s.capacity = adaptive_capacity(s.npix);
+    return s.capacity;
+  }
+
+  // Switches a segment to adaptive windows of `horizons` horizons.
+  void set_adaptive(int id, double tolerance, int horizons, double check_period) {
+    Segment &s = segment(id);
+    if (!std::isfinite(tolerance) || tolerance <= 0 || horizons < 1 || s.params.refractory_time <= 0)
+      fail(DVS_E_ARGUMENT, "invalid adaptive tolerance or event capacity bound");
+    s.adaptive = true;
+    s.tolerance = tolerance;
+    s.abatch = horizons;
+    s.check_period = check_period;
+    s.next_check = s.last_time;
   }
 
   dvs_stats stats{};
   int horizons() const { return horizons_; }
 
+  // process_adaptive over exactly the queued intervals of one segment, as
+  // one window (event_sensor.cu:306-355); returns whether the convergence
+  // gate accepted it.  The queue is consumed only when accepted.
+  bool step_adaptive_once(int id, double tolerance, bool check) {
+    Segment &s = segment(id);
+    if (s.queue.empty()) fail(DVS_E_ARGUMENT, "empty adaptive optical horizon");
+    StepSegment w;
+    w.seg = &s;
+    w.ivs.assign(s.queue.begin(), s.queue.end());
+    w.tolerance = tolerance;
+    w.check = check;
+    check_adaptive_window(w);
+    int cells = 0;
+    for (auto &iv : w.ivs) cells += (int)std::ceil((iv.t1 - iv.t0) / s.photo.integration_step);
+    std::vector<StepSegment> work;
+    work.push_back(std::move(w));
+    std::vector<char> accepted;
+    run_adaptive(work, s.npix, cells, (int)work[0].ivs.size(), accepted);
+    if (accepted[0]) {
+      s.queue.clear();
+      s.horizons_done += (int64_t)work[0].ivs.size();
+    }
+    return accepted[0] != 0;
+  }
+
  private:
-  void run(std::vector<StepSegment> &work, int pixel_total, int cell_total, int iv_total) {
+  // EventSensor::process_adaptive argument checks (event_sensor.cu:308-312)
+  // and the 1e6-cell horizon bound of upload_intervals (:276-282).
+  void check_adaptive_window(const StepSegment &w) {
+    const Segment &s = *w.seg;
+    const double start = w.ivs.front().t0, end = w.ivs.back().t1;
+    if (!std::isfinite(w.tolerance) || w.tolerance <= 0 || s.params.refractory_time <= 0 ||
+        (end - start) / s.params.refractory_time + 2 > s.max_events)
+      fail(DVS_E_ARGUMENT, "invalid adaptive tolerance or event capacity bound");
+    double cells = 0;
+    for (auto &iv : w.ivs) cells += (iv.t1 - iv.t0) / s.photo.integration_step;
+    if (cells > 1000000) fail(DVS_E_ARGUMENT, "optical horizon too long; reduce batch size");
+  }
+
+  // Cell tables of a step: every segment's intervals and circuit cells
+  // (generate_pixels' cell split, event_sensor.cu:37-40), pixel and circuit
+  // block offsets.  Returns the trace kernels' LDS bytes.
+  size_t prepare(std::vector<StepSegment> &work, int pixel_total, int cell_total, int iv_total) {
     hipStream_t st = stream_;
-    // Host tables: intervals and circuit cells of every segment
-    // (generate_pixels' cell split, event_sensor.cu:37-40).
     std::vector<dvs_optical_interval> ivs;
     std::vector<double> t0s, t1s;
     std::vector<int> civ;
@@ -289,6 +412,52 @@ class Engine {
     counter_.reserve(1, "ray counter");
     hip_check(hipMemsetAsync(status_.ptr, 0, sizeof(int), st), "clear status");
     hip_check(hipMemsetAsync(counter_.ptr, 0, sizeof(unsigned long long), st), "clear ray counter");
+    const int nseg = (int)work.size();
+    std::vector<int> cblocks(nseg + 1), pbegin(nseg + 1);
+    cblocks[0] = 0;
+    pbegin[0] = 0;
+    size_t lds = 0;
+    for (int i = 0; i < nseg; ++i) {
+      cblocks[i + 1] = cblocks[i] + (work[i].seg->npix + 255) / 256;
+      pbegin[i + 1] = pbegin[i] + work[i].seg->npix;
+      const int depth = std::max(3, work[i].seg->scene->view.stack_depth);
+      lds = std::max(lds, (size_t)depth * 256 * sizeof(int));
+    }
+    cblock_.upload(cblocks.data(), cblocks.size(), "segment blocks", st);
+    pbegin_.upload(pbegin.data(), pbegin.size(), "segment pixels", st);
+    circuit_blocks_ = cblocks[nseg];
+    return lds;
+  }
+
+  // Device table entry of segment i over its cells [c0, c1).
+  SegDev table_entry(const StepSegment &w, int c0, int c1, double *rad) {
+    const Segment &s = *w.seg;
+    SegDev d{};
+    d.scene = s.scene->view;
+    d.ivs = intervals_.ptr + w.iv_offset;
+    d.iv_count = (int)w.ivs.size();
+    d.t0 = cell_t0_.ptr + w.cell_offset + c0;
+    d.t1 = cell_t1_.ptr + w.cell_offset + c0;
+    d.poses = poses_.ptr + w.cell_offset + c0;
+    d.cells = c1 - c0;
+    d.rad = rad;
+    d.states = s.states.ptr;
+    d.slots = s.slots.ptr;
+    d.counts = counts_.ptr + w.pixel_offset;
+    d.params = s.params;
+    d.photo = s.photo;
+    d.seed = s.seed;
+    d.npix = s.npix;
+    d.max_events = s.max_events;
+    d.pixel_offset = w.pixel_offset;
+    d.pad = 0;
+    d.pixel_list = nullptr;
+    return d;
+  }
+
+  void run(std::vector<StepSegment> &work, int pixel_total, int cell_total, int iv_total) {
+    hipStream_t st = stream_;
+    const size_t lds = prepare(work, pixel_total, cell_total, iv_total);
     // Cells are traced in chunks so the [cell][pixel] radiance scratch stays
     // within budget: chunk k covers cells [k*C, (k+1)*C) of every segment.
     int max_cells = 0;
@@ -305,62 +474,28 @@ class Engine {
         1.0, std::min<double>({(double)max_cells, (double)(radiance_budget_ / (sizeof(double) * pixel_total)), block_cap}));
     const int chunks = (max_cells + chunk - 1) / chunk;
     radiance_.reserve((size_t)chunk * pixel_total, "cell radiance");
-    size_t lds = 0;
-    for (auto &w : work) {
-      const int depth = std::max(3, w.seg->scene->view.stack_depth);
-      lds = std::max(lds, (size_t)depth * 256 * sizeof(int));
-    }
     // Segment tables of every chunk (device), then the launches.
     const int nseg = (int)work.size();
     std::vector<SegDev> tables((size_t)chunks * nseg);
     std::vector<long long> tblocks((size_t)chunks * (nseg + 1));
-    std::vector<int> cblocks(nseg + 1), pbegin(nseg + 1);
     std::vector<long long> shape{nseg, chunks, chunk};
-    cblocks[0] = 0;
-    pbegin[0] = 0;
-    for (int i = 0; i < nseg; ++i) {
-      cblocks[i + 1] = cblocks[i] + (work[i].seg->npix + 255) / 256;
-      pbegin[i + 1] = pbegin[i] + work[i].seg->npix;
-    }
-    size_t rad_offset = 0;
     for (int k = 0; k < chunks; ++k) {
       long long *tb = &tblocks[(size_t)k * (nseg + 1)];
       tb[0] = 0;
-      rad_offset = 0;
+      size_t rad_offset = 0;
       for (int i = 0; i < nseg; ++i) {
         StepSegment &w = work[i];
-        Segment &s = *w.seg;
         const int c0 = std::min(w.cells, k * chunk), c1 = std::min(w.cells, (k + 1) * chunk);
-        SegDev &d = tables[(size_t)k * nseg + i];
-        d.scene = s.scene->view;
-        d.ivs = intervals_.ptr + w.iv_offset;
-        d.iv_count = (int)w.ivs.size();
-        d.t0 = cell_t0_.ptr + w.cell_offset + c0;
-        d.t1 = cell_t1_.ptr + w.cell_offset + c0;
-        d.poses = poses_.ptr + w.cell_offset + c0;
-        d.cells = c1 - c0;
-        d.rad = radiance_.ptr + rad_offset;
-        rad_offset += (size_t)s.npix * chunk;
-        d.states = s.states.ptr;
-        d.slots = s.slots.ptr;
-        d.counts = counts_.ptr + w.pixel_offset;
-        d.params = s.params;
-        d.photo = s.photo;
-        d.seed = s.seed;
-        d.npix = s.npix;
-        d.max_events = s.max_events;
-        d.pixel_offset = w.pixel_offset;
-        d.pad = 0;
-        const int slots = 256 / (s.scene->view.samples * s.scene->view.samples);
-        tb[i + 1] = tb[i] + ((long long)s.npix * d.cells + slots - 1) / slots;
-        shape.push_back(d.cells);
-        shape.push_back(s.npix);
+        tables[(size_t)k * nseg + i] = table_entry(w, c0, c1, radiance_.ptr + rad_offset);
+        rad_offset += (size_t)w.seg->npix * chunk;
+        const int slots = 256 / (w.seg->scene->view.samples * w.seg->scene->view.samples);
+        tb[i + 1] = tb[i] + ((long long)w.seg->npix * (c1 - c0) + slots - 1) / slots;
+        shape.push_back(c1 - c0);
+        shape.push_back(w.seg->npix);
       }
     }
     segtab_.upload(tables.data(), tables.size(), "segment table", st);
     tblock_.upload(tblocks.data(), tblocks.size(), "segment blocks", st);
-    cblock_.upload(cblocks.data(), cblocks.size(), "segment blocks", st);
-    pbegin_.upload(pbegin.data(), pbegin.size(), "segment pixels", st);
     // A captured graph replays only while every buffer it reads stays put.
     for (const void *ptr : {(const void *)segtab_.ptr, (const void *)tblock_.ptr, (const void *)cblock_.ptr,
                             (const void *)radiance_.ptr, (const void *)counts_.ptr, (const void *)poses_.ptr,
@@ -374,9 +509,10 @@ class Engine {
 
     // Timers bracket every trace and circuit launch (outside graph replay,
     // where only the whole step is timed).
-    Timer step_t, events_t;
+    Timer step_t;
     std::vector<std::unique_ptr<Timer>> trace_t, circuit_t;
     const bool graph = use_graph_;
+    const int cblocks_total = circuit_blocks_;
     hip_check(hipEventRecord(step_t.start, st), "record");
     auto enqueue = [&]() {
       hip_check(launch_fill_int(counts_.ptr, 0, pixel_total, st), "clear events");
@@ -396,22 +532,22 @@ class Engine {
           hip_check(hipEventRecord(trace_t.back()->stop, st), "record");
           hip_check(hipEventRecord(circuit_t.back()->start, st), "record");
         }
-        hip_check(launch_circuit_multi(segtab_.ptr + (size_t)k * nseg, cblock_.ptr, nseg, cblocks[nseg], status_.ptr, st),
+        hip_check(launch_circuit_multi(segtab_.ptr + (size_t)k * nseg, cblock_.ptr, nseg, cblocks_total, status_.ptr, st),
                   "trace and integrate circuits");
         if (!graph) hip_check(hipEventRecord(circuit_t.back()->stop, st), "record");
       }
     };
-    if (use_graph_) {
+    if (graph) {
       GraphKey key{shape};
       if (!graph_exec_ || !(key == graph_key_)) {
         if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
         graph_exec_ = nullptr;
-        hipGraph_t graph = nullptr;
+        hipGraph_t g = nullptr;
         hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "graph capture");
         enqueue();
-        hip_check(hipStreamEndCapture(st, &graph), "graph capture");
-        hip_check(hipGraphInstantiate(&graph_exec_, graph, nullptr, nullptr, 0), "graph instantiate");
-        (void)hipGraphDestroy(graph);
+        hip_check(hipStreamEndCapture(st, &g), "graph capture");
+        hip_check(hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0), "graph instantiate");
+        (void)hipGraphDestroy(g);
         graph_key_ = key;
       }
       hip_check(hipGraphLaunch(graph_exec_, st), "graph launch");
@@ -419,7 +555,32 @@ class Engine {
       enqueue();
     }
     hip_check(hipEventRecord(step_t.stop, st), "record");
-    // Compaction by a device prefix sum over every segment's pixel counts.
+    const std::vector<char> all(nseg, 1);
+    const double events_ms = collect(work, pixel_total, lds, true, all);
+    const double step_ms = step_t.ms();
+    double trace_ms = 0, circuit_ms = 0;
+    for (auto &t : trace_t) trace_ms += t->ms();
+    for (auto &t : circuit_t) circuit_ms += t->ms();
+    if (graph) trace_ms = step_ms;  // graph replay: trace + circuit of the step
+    stats.trace_ms += trace_ms;
+    stats.circuit_ms += circuit_ms;
+    stats.total_ms += step_ms + events_ms;
+    stats.trace_launches += chunks;
+    for (auto &w : work) {
+      const int ss = w.seg->scene->view.samples;
+      stats.primary_rays += (uint64_t)w.seg->npix * w.cells * ss * ss;
+      stats.native_equivalent_rays += (uint64_t)w.seg->npix * w.cells * ss * ss;
+    }
+  }
+
+  // Compaction by a device prefix sum over every segment's pixel counts,
+  // event-time depth (native steps), device sort by (segment, t, y, x) and
+  // each accepted segment's readout FIFO.  Returns the events time (ms).
+  double collect(std::vector<StepSegment> &work, int pixel_total, size_t lds, bool depth,
+                 const std::vector<char> &accepted) {
+    hipStream_t st = stream_;
+    const int nseg = (int)work.size();
+    Timer events_t;
     hip_check(hipEventRecord(events_t.start, st), "record");
     size_t temp_bytes = 0;
     hip_check(inclusive_scan_int(nullptr, temp_bytes, counts_.ptr, ends_.ptr, pixel_total, st), "scan size");
@@ -452,10 +613,15 @@ class Engine {
       hip_check(launch_gather_multi(tab, pbegin_.ptr, nseg, pixel_total, ends_.ptr, ev_time_.ptr, ev_seg_.ptr,
                                     ev_pixel_.ptr, ev_pol_.ptr, keys_.ptr, order_.ptr, st),
                 "event compaction");
-      hip_check(launch_event_poses_multi(tab, total, ev_time_.ptr, ev_seg_.ptr, ev_poses_.ptr, st), "event poses");
-      hip_check(launch_event_depth_multi(tab, total, samples, lds, ev_seg_.ptr, ev_pixel_.ptr, ev_poses_.ptr,
-                                         ev_depth_.ptr, status_.ptr, st),
-                "event depth rays");
+      if (depth) {
+        hip_check(launch_event_poses_multi(tab, total, ev_time_.ptr, ev_seg_.ptr, ev_poses_.ptr, st), "event poses");
+        hip_check(launch_event_depth_multi(tab, total, samples, lds, ev_seg_.ptr, ev_pixel_.ptr, ev_poses_.ptr,
+                                           ev_depth_.ptr, status_.ptr, st),
+                  "event depth rays");
+      } else {
+        // The interpolating path supplies no per-event depth (event_sensor.cu:111-113).
+        hip_check(launch_fill_int(reinterpret_cast<int *>(ev_depth_.ptr), (int)0xBF800000u, total, st), "event depth");
+      }
       // Stable sort by time, then stable by segment: (segment, t, y, x),
       // since compaction order is (segment, y, x).
       temp_bytes = 0;
@@ -495,35 +661,369 @@ class Engine {
     hip_check(hipMemcpyAsync(&shadows, counter_.ptr, sizeof(shadows), hipMemcpyDeviceToHost, st), "ray counter");
     hip_check(hipStreamSynchronize(st), "copy events");
     if (status) fail(DVS_E_OPTICAL, "invalid optical sample or BVH traversal overflow");
-    const double step_ms = step_t.ms(), events_ms = events_t.ms();
-    double trace_ms = 0, circuit_ms = 0;
-    for (auto &t : trace_t) trace_ms += t->ms();
-    for (auto &t : circuit_t) circuit_ms += t->ms();
-    if (graph) trace_ms = step_ms;  // graph replay: trace + circuit of the step
-    stats.trace_ms += trace_ms;
-    stats.circuit_ms += circuit_ms;
+    const double events_ms = events_t.ms();
     stats.events_ms += events_ms;
-    stats.total_ms += step_ms + events_ms;
-    stats.trace_launches += chunks;
     stats.shadow_rays += shadows;
     stats.comparator_events += (uint64_t)total;
-    stats.event_rays += (uint64_t)total * samples * samples;
+    if (depth) stats.event_rays += (uint64_t)total * samples * samples;
     // Per-segment readout on its (t, y, x)-ordered slice.
     size_t at = 0;
     for (int i = 0; i < nseg; ++i) {
       Segment &s = *work[i].seg;
       size_t end = at;
       while (end < (size_t)total && sorted_seg[end] == i) ++end;
+      if (!accepted[i]) continue;  // a rejected window has no events and changes nothing
       s.raw.assign(sorted.begin() + at, sorted.begin() + end);
       s.events_total += (int64_t)(end - at);
       const size_t before = s.ready.size();
       s.readout.submit_sorted(s.raw.data(), s.raw.size(), work[i].ivs.back().t1, s.ready);
       stats.delivered_events += s.ready.size() - before;
-      stats.primary_rays += (uint64_t)s.npix * work[i].cells * samples * samples;
       stats.pixel_cells += (uint64_t)s.npix * work[i].cells;
       s.last_time = work[i].ivs.back().t1;
       at = end;
     }
+    return events_ms;
+  }
+
+  // Knot and exact-cell capacity of OpticalSampler (optical_sampler.cu:141-150):
+  // the largest of 512, 256, 128 whose caches fit the free device memory with
+  // 1 GiB to spare, else 128.
+  static int adaptive_capacity(int npix) {
+    size_t avail = 0, total = 0;
+    hip_check(hipMemGetInfo(&avail, &total), "query optical cache memory budget");
+    for (int candidate : {512, 256, 128}) {
+      const size_t needed = (size_t)npix * candidate * (16 + 32 + sizeof(double) + sizeof(int));
+      if (avail > needed + (size_t(1) << 30)) return candidate;
+    }
+    return 128;
+  }
+
+  // Adaptive windows of every segment of `work` (OpticalSampler::refine,
+  // prepare_exact_cells, sample_cells and EventSensor::process_adaptive).
+  // accepted[i] == 0 when segment i's convergence gate rejected its window;
+  // such a window changes no state.
+  void run_adaptive(std::vector<StepSegment> &work, int pixel_total, int cell_total, int iv_total,
+                    std::vector<char> &accepted) {
+    hipStream_t st = stream_;
+    Timer step_t, refine_t, circuit_t;
+    hip_check(hipEventRecord(step_t.start, st), "record");
+    const size_t lds = prepare(work, pixel_total, cell_total, iv_total);
+    const int nseg = (int)work.size();
+    const int P = pixel_total;
+    const int samples = work[0].seg->scene->view.samples, R = samples * samples, slots = 256 / R;
+    accepted.assign(nseg, 1);
+    std::vector<SegDev> tables(nseg);
+    std::vector<AdaptDev> ad(nseg);
+    std::vector<int> cell_begin(nseg + 1);
+    for (int i = 0; i < nseg; ++i) {
+      StepSegment &w = work[i];
+      Segment &s = *w.seg;
+      if (!s.capacity) s.capacity = adaptive_capacity(s.npix);
+      tables[i] = table_entry(w, 0, w.cells, nullptr);
+      AdaptDev &a = ad[i];
+      a = AdaptDev{};
+      a.start = w.ivs.front().t0;
+      a.end = w.ivs.back().t1;
+      a.tolerance = w.tolerance;
+      a.capacity = s.capacity;
+      a.accepted = 1;
+      a.cell_offset = w.cell_offset;
+      cell_begin[i] = w.cell_offset;
+    }
+    cell_begin[nseg] = cell_total;
+    segtab_.upload(tables.data(), tables.size(), "segment table", st);
+    adapt_.upload(ad.data(), ad.size(), "adaptive windows", st);
+    cellbeg_.upload(cell_begin.data(), cell_begin.size(), "segment cells", st);
+    hip_check(launch_fill_int(counts_.ptr, 0, P, st), "clear events");
+    hip_check(launch_cell_poses(cell_total, cell_t0_.ptr, cell_t1_.ptr, cell_iv_.ptr, intervals_.ptr, poses_.ptr, st),
+              "cell poses");
+    cell_frac_.reserve(cell_total, "cell fractions");
+    hip_check(launch_adapt_cell_frac(cell_total, cell_t0_.ptr, cell_t1_.ptr, cellbeg_.ptr, nseg, adapt_.ptr,
+                                     cell_frac_.ptr, st),
+              "cell fractions");
+    hip_check(hipEventRecord(refine_t.start, st), "record");
+    // Five initial probes per pixel.
+    probe_poses_.reserve((size_t)nseg * 5, "probe poses");
+    hip_check(launch_adapt_probe_poses(nseg, segtab_.ptr, adapt_.ptr, probe_poses_.ptr, st), "probe poses");
+    probes_.reserve((size_t)P * 5, "optical probes");
+    hip_check(launch_adapt_trace(dvsdev::LAYOUT_PROBE, segtab_.ptr, pbegin_.ptr, nseg, samples, (long long)P * 5,
+                                 nullptr, nullptr, probe_poses_.ptr, lds, probes_.ptr, status_.ptr, counter_.ptr, st),
+              "refine pixel optical launch");
+    for (auto *a : {&bound_, &flags_, &bisects_, &exact_terms_, &kcount_, &koff_, &ecount_, &eoff_, &fb_index_,
+                    &fb_scan_, &run_begin_})
+      a->reserve(P, "optical refinement state");
+    AdaptBufs b{};
+    b.probes = probes_.ptr;
+    b.bound = bound_.ptr;
+    b.flags = flags_.ptr;
+    b.bisects = bisects_.ptr;
+    b.exact_terms = exact_terms_.ptr;
+    b.kcount = kcount_.ptr;
+    b.koff = koff_.ptr;
+    b.ecount = ecount_.ptr;
+    b.eoff = eoff_.ptr;
+    b.fb_index = fb_index_.ptr;
+    b.cell_frac = cell_frac_.ptr;
+    ent_[0].reserve((size_t)2 * P, "refinement segments");
+    hip_check(launch_adapt_init(P, b, ent_[0].ptr, st), "refine pixel optical launch");
+    ctr_.reserve(1, "refinement counters");
+    hip_check(hipMemsetAsync(ctr_.ptr, 0, sizeof(AdaptCounters), st), "clear refinement counters");
+    // Level-synchronous refinement.
+    long long n = 2LL * P;
+    int cur = 0, terms_total = 0, levels = 0;
+    AdaptCounters host_ctr{};
+    while (n > 0) {
+      if (++levels > 64) fail(DVS_E_STATE, "optical refinement did not terminate");
+      terms_.grow_keep((size_t)terms_total + n, terms_total, "resolved segments", st);
+      tkeys_.grow_keep((size_t)terms_total + n, terms_total, "resolved segment keys", st);
+      bis_.reserve(n, "bisected segments");
+      qpix_.reserve(2 * n, "optical queries");
+      qpose_.reserve(2 * n, "optical queries");
+      hip_check(hipMemsetAsync(&ctr_.ptr->bisect, 0, sizeof(int), st), "clear refinement counters");
+      hip_check(launch_adapt_decide((int)n, ent_[cur].ptr, segtab_.ptr, adapt_.ptr, pbegin_.ptr, nseg, b, terms_.ptr,
+                                    tkeys_.ptr, bis_.ptr, qpix_.ptr, qpose_.ptr, ctr_.ptr, st),
+                "refine pixel optical launch");
+      hip_check(hipMemcpyAsync(&host_ctr, ctr_.ptr, sizeof(host_ctr), hipMemcpyDeviceToHost, st), "refinement count");
+      hip_check(hipStreamSynchronize(st), "refinement count");
+      terms_total = host_ctr.terms;
+      const long long nb = host_ctr.bisect;
+      if (nb == 0) break;
+      qrad_.reserve(2 * nb, "optical queries");
+      hip_check(launch_adapt_trace(dvsdev::LAYOUT_QUERY, segtab_.ptr, pbegin_.ptr, nseg, samples, 2 * nb, qpix_.ptr,
+                                   nullptr, qpose_.ptr, lds, qrad_.ptr, status_.ptr, counter_.ptr, st),
+                "refine pixel optical launch");
+      ent_[1 - cur].reserve(2 * nb, "refinement segments");
+      hip_check(launch_adapt_children((int)nb, bis_.ptr, qrad_.ptr, ent_[1 - cur].ptr, st), "refine pixel optical launch");
+      cur = 1 - cur;
+      n = 2 * nb;
+    }
+    // Knot counts, offsets and depth-first knot order.
+    hip_check(launch_adapt_counts(P, adapt_.ptr, pbegin_.ptr, nseg, b, st), "pixel optical knots");
+    size_t temp_bytes = 0;
+    hip_check(inclusive_scan_int(nullptr, temp_bytes, kcount_.ptr, koff_.ptr, P, st), "scan size");
+    temp_.reserve(temp_bytes, "scan scratch");
+    hip_check(inclusive_scan_int(temp_.ptr, temp_bytes, kcount_.ptr, koff_.ptr, P, st), "knot offsets");
+    int knots_total = 0;
+    hip_check(hipMemcpyAsync(&knots_total, koff_.ptr + P - 1, sizeof(int), hipMemcpyDeviceToHost, st), "knot count");
+    hip_check(hipStreamSynchronize(st), "knot count");
+    knots_.reserve(std::max(knots_total, 1), "pixel optical knots");
+    b.knots = knots_.ptr;
+    if (terms_total > 0) {
+      torder_.reserve(terms_total, "knot order");
+      torder2_.reserve(terms_total, "knot order");
+      tkeys2_.reserve(terms_total, "knot order");
+      contrib_.reserve(terms_total, "knot order");
+      cscan_.reserve(terms_total, "knot order");
+      hip_check(launch_iota(torder_.ptr, terms_total, st), "knot order");
+      int bits = 1;
+      while ((1LL << bits) < (long long)P) ++bits;
+      temp_bytes = 0;
+      hip_check(sort_pairs_u64(nullptr, temp_bytes, tkeys_.ptr, tkeys2_.ptr, torder_.ptr, torder2_.ptr, terms_total,
+                               0, 24 + bits, st),
+                "sort size");
+      temp_.reserve(temp_bytes, "sort scratch");
+      hip_check(sort_pairs_u64(temp_.ptr, temp_bytes, tkeys_.ptr, tkeys2_.ptr, torder_.ptr, torder2_.ptr, terms_total,
+                               0, 24 + bits, st),
+                "knot order");
+      hip_check(launch_adapt_term_runs(terms_total, tkeys2_.ptr, torder2_.ptr, terms_.ptr, b, contrib_.ptr,
+                                       run_begin_.ptr, st),
+                "knot order");
+      temp_bytes = 0;
+      hip_check(inclusive_scan_int(nullptr, temp_bytes, contrib_.ptr, cscan_.ptr, terms_total, st), "scan size");
+      temp_.reserve(temp_bytes, "scan scratch");
+      hip_check(inclusive_scan_int(temp_.ptr, temp_bytes, contrib_.ptr, cscan_.ptr, terms_total, st), "knot order");
+      hip_check(launch_adapt_write_knots(terms_total, tkeys2_.ptr, torder2_.ptr, terms_.ptr, b, contrib_.ptr,
+                                         cscan_.ptr, run_begin_.ptr, st),
+                "pixel optical knots");
+    }
+    // Local exact cells (identify_exact_cells / sample_exact_queries).
+    hip_check(launch_adapt_exact_count(P, segtab_.ptr, adapt_.ptr, pbegin_.ptr, nseg, b, st), "identify local exact optical cells");
+    temp_bytes = 0;
+    hip_check(inclusive_scan_int(nullptr, temp_bytes, ecount_.ptr, eoff_.ptr, P, st), "scan size");
+    temp_.reserve(temp_bytes, "scan scratch");
+    hip_check(inclusive_scan_int(temp_.ptr, temp_bytes, ecount_.ptr, eoff_.ptr, P, st), "local exact query count");
+    int exact_total = 0;
+    hip_check(hipMemcpyAsync(&exact_total, eoff_.ptr + P - 1, sizeof(int), hipMemcpyDeviceToHost, st), "local exact query count");
+    hip_check(hipStreamSynchronize(st), "local exact query count");
+    ecell_.reserve(std::max(exact_total, 1), "local exact optical cells");
+    erad_.reserve(std::max(exact_total, 1), "local exact radiance");
+    eitem_pix_.reserve(std::max(exact_total, 1), "compact exact ray queue");
+    eitem_pose_.reserve(std::max(exact_total, 1), "compact exact ray queue");
+    b.ecell = ecell_.ptr;
+    b.erad = erad_.ptr;
+    if (exact_total > 0) {
+      hip_check(launch_adapt_exact_list(P, segtab_.ptr, adapt_.ptr, pbegin_.ptr, nseg, b, eitem_pix_.ptr,
+                                        eitem_pose_.ptr, st),
+                "identify local exact optical cells");
+      hip_check(launch_adapt_trace(dvsdev::LAYOUT_CELL, segtab_.ptr, pbegin_.ptr, nseg, samples, exact_total,
+                                   eitem_pix_.ptr, eitem_pose_.ptr, poses_.ptr, lds, erad_.ptr, status_.ptr,
+                                   counter_.ptr, st),
+                "trace local exact ray cells");
+    }
+    // Whole-pixel fallback list (compact_fallback, event_sensor.cu:116-118).
+    hip_check(launch_adapt_fallback_flags(P, flags_.ptr, fb_scan_.ptr, st), "compact exact fallback pixels");
+    temp_bytes = 0;
+    hip_check(inclusive_scan_int(nullptr, temp_bytes, fb_scan_.ptr, fb_scan_.ptr, P, st), "scan size");
+    temp_.reserve(temp_bytes, "scan scratch");
+    hip_check(inclusive_scan_int(temp_.ptr, temp_bytes, fb_scan_.ptr, fb_scan_.ptr, P, st), "compact exact fallback pixels");
+    fb_ends_.reserve(nseg, "fallback counts");
+    hip_check(launch_adapt_segment_ends(nseg, pbegin_.ptr, fb_scan_.ptr, fb_ends_.ptr, st), "fallback counts");
+    std::vector<int> fb_ends(nseg);
+    hip_check(hipMemcpyAsync(fb_ends.data(), fb_ends_.ptr, sizeof(int) * nseg, hipMemcpyDeviceToHost, st), "fallback counts");
+    hip_check(hipStreamSynchronize(st), "fallback counts");
+    const int fb_total = nseg ? fb_ends[nseg - 1] : 0;
+    fb_list_.reserve(std::max(fb_total, 1), "exact fallback pixel IDs");
+    hip_check(launch_adapt_fallback_list(P, adapt_.ptr, pbegin_.ptr, nseg, fb_scan_.ptr, b, fb_list_.ptr, st),
+              "compact exact fallback pixels");
+    // Refinement statistics of every segment.
+    seg_stats_.reserve(nseg, "refinement statistics");
+    hip_check(launch_adapt_segment_stats(nseg, pbegin_.ptr, b, seg_stats_.ptr, st), "refinement statistics");
+    std::vector<AdaptSegStats> seg_stats(nseg);
+    hip_check(hipMemcpyAsync(seg_stats.data(), seg_stats_.ptr, sizeof(AdaptSegStats) * nseg, hipMemcpyDeviceToHost, st),
+              "refinement statistics");
+    // Convergence gate (verify_receptor, event_sensor.cu:357-374) of the
+    // segments that check this window.
+    std::vector<int> gate_seg;
+    for (int i = 0; i < nseg; ++i)
+      if (work[i].check) gate_seg.push_back(i);
+    const int ngate = (int)gate_seg.size();
+    uint64_t gate_samples = 0;
+    if (ngate > 0) {
+      std::vector<int> ids((size_t)ngate * 1024);
+      std::vector<SegDev> gtab(ngate);
+      std::vector<long long> gblocks(ngate + 1);
+      size_t rad_total = 0;
+      for (int g = 0; g < ngate; ++g) {
+        const StepSegment &w = work[gate_seg[g]];
+        const int np = std::min(1024, w.seg->npix);
+        for (int i = 0; i < np; ++i) ids[(size_t)g * 1024 + i] = (int)(((uint64_t)i * 2654435761ULL) % (uint64_t)w.seg->npix);
+        rad_total += (size_t)np * w.cells;
+      }
+      gate_ids_.upload(ids.data(), ids.size(), "mean receptor probe IDs", st);
+      gate_rad_.reserve(std::max<size_t>(rad_total, 1), "mean receptor probe radiance");
+      size_t off = 0;
+      gblocks[0] = 0;
+      for (int g = 0; g < ngate; ++g) {
+        const StepSegment &w = work[gate_seg[g]];
+        const int np = std::min(1024, w.seg->npix);
+        gtab[g] = table_entry(w, 0, w.cells, gate_rad_.ptr + off);
+        gtab[g].npix = np;
+        gtab[g].pixel_list = gate_ids_.ptr + (size_t)g * 1024;
+        off += (size_t)np * w.cells;
+        gblocks[g + 1] = gblocks[g] + ((long long)np * w.cells + slots - 1) / slots;
+        gate_samples += (uint64_t)np * w.cells;
+      }
+      gate_tab_.upload(gtab.data(), gtab.size(), "mean receptor probes", st);
+      gate_blocks_.upload(gblocks.data(), gblocks.size(), "mean receptor probes", st);
+      gate_idx_.upload(gate_seg.data(), gate_seg.size(), "mean receptor probes", st);
+      hip_check(launch_trace_cells_multi(gate_tab_.ptr, gate_blocks_.ptr, ngate, gblocks[ngate], lds, status_.ptr,
+                                         counter_.ptr, st),
+                "mean receptor probe optics");
+      gate_err_.reserve((size_t)ngate * 1024, "mean receptor probe errors");
+      hip_check(launch_adapt_gate(gate_tab_.ptr, gate_idx_.ptr, ngate, adapt_.ptr, b, gate_err_.ptr, st),
+                "mean receptor probe integration");
+      std::vector<double> errors((size_t)ngate * 1024);
+      hip_check(hipMemcpyAsync(errors.data(), gate_err_.ptr, sizeof(double) * errors.size(), hipMemcpyDeviceToHost, st),
+                "download mean receptor convergence");
+      hip_check(hipStreamSynchronize(st), "download mean receptor convergence");
+      bool changed = false;
+      for (int g = 0; g < ngate; ++g) {
+        StepSegment &w = work[gate_seg[g]];
+        Segment &s = *w.seg;
+        const int np = std::min(1024, s.npix);
+        std::vector<double> e(errors.begin() + (size_t)g * 1024, errors.begin() + (size_t)g * 1024 + np);
+        dvs_receptor_probe r{};
+        r.pixels = np;
+        r.circuit_cells = w.cells;
+        for (double v : e) r.mean_log_voltage_error += v;
+        std::sort(e.begin(), e.end());
+        r.maximum_log_voltage_error = e.back();
+        r.p99_log_voltage_error = e[(size_t)(np * .99)];
+        r.mean_log_voltage_error /= np;
+        s.last_probe = r;
+        const double threshold = std::min(s.params.threshold_pos, s.params.threshold_neg);
+        if (r.maximum_log_voltage_error > .02 * threshold || r.p99_log_voltage_error > .005 * threshold) {
+          accepted[gate_seg[g]] = 0;
+          ad[gate_seg[g]].accepted = 0;
+          changed = true;
+        }
+      }
+      if (changed) adapt_.upload(ad.data(), ad.size(), "adaptive windows", st);
+    }
+    hip_check(hipEventRecord(refine_t.stop, st), "record");
+    // Integration, with the fallback pixels' native radiance traced in cell
+    // chunks (sample_cells / integrate_fallback_tile).
+    hip_check(hipEventRecord(circuit_t.start, st), "record");
+    int max_cells = 0;
+    for (auto &w : work) max_cells = std::max(max_cells, w.cells);
+    int chunk = max_cells;
+    if (fb_total > 0)
+      chunk = (int)std::max<size_t>(1, std::min<size_t>(max_cells, radiance_budget_ / (sizeof(double) * fb_total)));
+    uint64_t fallback_samples = 0;
+    std::vector<long long> fblocks(nseg + 1);
+    std::vector<SegDev> ftab(nseg);
+    if (fb_total > 0) {
+      radiance_.reserve((size_t)chunk * fb_total, "fallback optical tile");
+      size_t off = 0;
+      for (int i = 0; i < nseg; ++i) {
+        const int begin = i ? fb_ends[i - 1] : 0, count = fb_ends[i] - begin;
+        ad[i].fb_begin = begin;
+        ad[i].fb_count = count;
+        ad[i].fb_rad = radiance_.ptr + off;
+        off += (size_t)count * chunk;
+        if (accepted[i]) fallback_samples += (uint64_t)count * work[i].cells;
+      }
+      adapt_.upload(ad.data(), ad.size(), "adaptive windows", st);
+    }
+    for (int c0 = 0; c0 < max_cells; c0 += chunk) {
+      const int c1 = std::min(max_cells, c0 + chunk);
+      if (fb_total > 0) {
+        fblocks[0] = 0;
+        for (int i = 0; i < nseg; ++i) {
+          StepSegment &w = work[i];
+          const int a0 = std::min(w.cells, c0), a1 = std::min(w.cells, c1);
+          ftab[i] = table_entry(w, a0, a1, const_cast<double *>(ad[i].fb_rad));
+          ftab[i].npix = accepted[i] ? ad[i].fb_count : 0;
+          ftab[i].pixel_list = fb_list_.ptr + ad[i].fb_begin;
+          fblocks[i + 1] = fblocks[i] + ((long long)ftab[i].npix * (a1 - a0) + slots - 1) / slots;
+        }
+        fb_tab_.upload(ftab.data(), ftab.size(), "fallback optical tile", st);
+        fb_blocks_.upload(fblocks.data(), fblocks.size(), "fallback optical tile", st);
+        if (fblocks[nseg] > 0)
+          hip_check(launch_trace_cells_multi(fb_tab_.ptr, fb_blocks_.ptr, nseg, fblocks[nseg], lds, status_.ptr,
+                                             counter_.ptr, st),
+                    "parallel exact cell optics launch");
+      }
+      hip_check(launch_adapt_integrate(segtab_.ptr, adapt_.ptr, cblock_.ptr, nseg, circuit_blocks_, b, c0, c1,
+                                       status_.ptr, st),
+                "adaptive circuit launch");
+      if (fb_total > 0) hip_check(hipStreamSynchronize(st), "exact fallback circuit tile");  // tables are reused
+    }
+    hip_check(hipEventRecord(circuit_t.stop, st), "record");
+    hip_check(hipEventRecord(step_t.stop, st), "record");
+    const double events_ms = collect(work, pixel_total, lds, false, accepted);
+    // Statistics.
+    const double refine_ms = refine_t.ms(), circuit_ms = circuit_t.ms(), step_ms = step_t.ms();
+    stats.trace_ms += refine_ms;
+    stats.circuit_ms += circuit_ms;
+    stats.total_ms += step_ms + events_ms;
+    uint64_t samples_traced = gate_samples + fallback_samples;
+    for (int i = 0; i < nseg; ++i) {
+      Segment &s = *work[i].seg;
+      const AdaptSegStats &g = seg_stats[i];
+      samples_traced += g.traced_pixel_samples + (uint64_t)g.native_cell_queries;
+      s.last_refine = dvs_refinement_stats{g.traced_pixel_samples, g.fallback_pixels, g.maximum_knots,
+                                           g.exact_segments, g.native_cell_queries};
+      s.last_accepted = accepted[i] != 0;
+      if (accepted[i]) {
+        stats.native_equivalent_rays += (uint64_t)s.npix * work[i].cells * R;
+        stats.adaptive_windows += 1;
+      } else {
+        stats.rejected_windows += 1;
+      }
+    }
+    stats.primary_rays += samples_traced * R;
+    stats.trace_launches += 1;
   }
 
   int horizons_;
@@ -545,6 +1045,23 @@ class Engine {
   DeviceArray<unsigned char> temp_;
   hipGraphExec_t graph_exec_ = nullptr;
   GraphKey graph_key_;
+  int circuit_blocks_ = 0;
+  // adaptive step state
+  DeviceArray<AdaptDev> adapt_;
+  DeviceArray<int> cellbeg_, bound_, flags_, bisects_, exact_terms_, kcount_, koff_, ecount_, eoff_, fb_index_,
+      fb_scan_, run_begin_, qpix_, contrib_, cscan_, ecell_, eitem_pix_, eitem_pose_, fb_ends_, fb_list_, gate_ids_,
+      gate_idx_;
+  DeviceArray<double> cell_frac_, probes_, qrad_, erad_, gate_rad_, gate_err_;
+  DeviceArray<Pose> probe_poses_, qpose_;
+  DeviceArray<AdaptEntry> ent_[2], bis_;
+  DeviceArray<AdaptCounters> ctr_;
+  DeviceArray<AdaptTerm> terms_;
+  DeviceArray<unsigned long long> tkeys_, tkeys2_;
+  DeviceArray<unsigned> torder_, torder2_;
+  DeviceArray<double2> knots_;
+  DeviceArray<AdaptSegStats> seg_stats_;
+  DeviceArray<SegDev> gate_tab_, fb_tab_;
+  DeviceArray<long long> gate_blocks_, fb_blocks_;
  public:
   size_t segment_bytes(int id) {
     Segment &s = segment(id);
@@ -673,6 +1190,52 @@ int dvs_sensor_reset_stats(dvs_sensor s) {
   });
 }
 
+int dvs_sensor_process_adaptive(dvs_sensor s, const dvs_optical_interval *ivs, int32_t n, double tolerance,
+                                int32_t check_convergence, int32_t *accepted, int64_t *ready_count) {
+  return guarded([&] {
+    if (!s) fail(DVS_E_ARGUMENT, "null sensor");
+    if (accepted) *accepted = 0;
+    s->engine->validate_adaptive(s->id, ivs, n, tolerance);
+    Segment &seg = s->engine->segment(s->id);
+    seg.queue.clear();
+    seg.fed_until = seg.last_time;
+    bool ok = false;
+    try {
+      s->engine->feed(s->id, ivs, n);
+      ok = s->engine->step_adaptive_once(s->id, tolerance, check_convergence != 0);
+    } catch (...) {
+      seg.queue.clear();
+      seg.fed_until = seg.last_time;
+      throw;
+    }
+    seg.queue.clear();
+    seg.fed_until = seg.last_time;
+    if (accepted) *accepted = ok ? 1 : 0;
+    if (ready_count) *ready_count = (int64_t)seg.ready.size();
+  });
+}
+
+int dvs_sensor_adaptive_capacity(dvs_sensor s, int32_t *capacity) {
+  return guarded([&] {
+    if (!s || !capacity) fail(DVS_E_ARGUMENT, "null sensor");
+    *capacity = s->engine->capacity_of(s->id);
+  });
+}
+
+int dvs_sensor_refinement_stats(dvs_sensor s, dvs_refinement_stats *out) {
+  return guarded([&] {
+    if (!s || !out) fail(DVS_E_ARGUMENT, "null sensor");
+    *out = s->engine->segment(s->id).last_refine;
+  });
+}
+
+int dvs_sensor_last_convergence(dvs_sensor s, dvs_receptor_probe *out) {
+  return guarded([&] {
+    if (!s || !out) fail(DVS_E_ARGUMENT, "null sensor");
+    *out = s->engine->segment(s->id).last_probe;
+  });
+}
+
 // ---- Multi-segment batch engine -------------------------------------------
 int dvs_batch_create(int32_t horizons_per_step, int32_t use_graph, dvs_batch *out) {
   return guarded([&] {
@@ -751,6 +1314,20 @@ int dvs_batch_remove(dvs_batch b, int32_t segment) {
     if (!b) fail(DVS_E_ARGUMENT, "null batch");
     b->engine->segment(segment);
     b->engine->remove(segment);
+  });
+}
+
+int dvs_batch_set_adaptive(dvs_batch b, int32_t segment, double tolerance, int32_t horizons, double check_period) {
+  return guarded([&] {
+    if (!b) fail(DVS_E_ARGUMENT, "null batch");
+    b->engine->set_adaptive(segment, tolerance, horizons, check_period);
+  });
+}
+
+int dvs_batch_refinement_stats(dvs_batch b, int32_t segment, dvs_refinement_stats *out) {
+  return guarded([&] {
+    if (!b || !out) fail(DVS_E_ARGUMENT, "null batch");
+    *out = b->engine->segment(segment).last_refine;
   });
 }
 

@@ -29,7 +29,19 @@ struct SegDev {
   int iv_count, cells, npix, max_events;
   int pixel_offset;                 // first global pixel index of the segment
   int pad;
+  const int *pixel_list;            // when set, item pixel k traces sensor pixel pixel_list[k]
 };
+
+// Last segment whose first block (or pixel) is <= b (begin[0] == 0).
+template <class T>
+__device__ __forceinline__ int segment_of(const T *begin, int nseg, T b) {
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const int m = (lo + hi + 1) >> 1;
+    if (begin[m] <= b) lo = m; else hi = m - 1;
+  }
+  return lo;
+}
 }
 
 namespace dvsamd {

@@ -106,7 +106,24 @@ typedef struct {
   double trace_launches;
   uint64_t primary_rays, shadow_rays, event_rays;
   uint64_t pixel_cells, comparator_events, delivered_events;
+  /* rays the native path would trace for the same accepted windows, and the
+   * adaptive windows accepted / rejected by the convergence gate */
+  uint64_t native_equivalent_rays, adaptive_windows, rejected_windows;
 } dvs_stats;
+
+/* OpticalRefinementStats (include/dvs/optical_sampler.hpp:16) of the last
+ * adaptive window. */
+typedef struct {
+  uint64_t traced_pixel_samples;
+  int32_t fallback_pixels, maximum_knots, exact_segments, native_cell_queries;
+} dvs_refinement_stats;
+
+/* ReceptorProbeResult (include/dvs/event_sensor.hpp:39) of the last checked
+ * adaptive window. */
+typedef struct {
+  double maximum_log_voltage_error, p99_log_voltage_error, mean_log_voltage_error;
+  int32_t pixels, circuit_cells;
+} dvs_receptor_probe;
 
 typedef struct dvs_scene_s *dvs_scene;
 typedef struct dvs_sensor_s *dvs_sensor;
@@ -191,6 +208,24 @@ int dvs_sensor_initialize(dvs_sensor sensor, const dvs_optical_state *pose, doub
  * held until dvs_sensor_take.  *ready_count receives their number. */
 int dvs_sensor_process(dvs_sensor sensor, const dvs_optical_interval *intervals,
                        int32_t interval_count, int64_t *ready_count);
+/* EventSensor::process_adaptive (event_sensor.cu:306-355) through
+ * DvsRayCaster::advance_adaptive (include/dvs/raycast.hpp:40-43): the
+ * intervals form one window; per-pixel optical refinement
+ * (OpticalSampler::refine, src/cuda/optical_sampler.cu:39-84) to the log
+ * tolerance, local exact cells and whole-pixel fallback, then the circuit.
+ * With check_convergence the receptor check (verify_receptor, :357-374)
+ * runs first; *accepted = 0 means it rejected the window and nothing
+ * changed.  Events of this path carry depth -1. */
+int dvs_sensor_process_adaptive(dvs_sensor sensor, const dvs_optical_interval *intervals,
+                                int32_t interval_count, double tolerance, int32_t check_convergence,
+                                int32_t *accepted, int64_t *ready_count);
+/* DvsRayCaster::adaptive_cache_capacity (raycast.hpp:36-39): knot and exact
+ * cell capacity per pixel (optical_sampler.cu:141-150). */
+int dvs_sensor_adaptive_capacity(dvs_sensor sensor, int32_t *capacity);
+/* DvsRayCaster::refinement_statistics (raycast.hpp:55). */
+int dvs_sensor_refinement_stats(dvs_sensor sensor, dvs_refinement_stats *out);
+/* EventSensor::last_convergence (include/dvs/event_sensor.hpp:54). */
+int dvs_sensor_last_convergence(dvs_sensor sensor, dvs_receptor_probe *out);
 /* Copies and clears the held delivered events (time-ordered). */
 int dvs_sensor_take(dvs_sensor sensor, dvs_event *out, int64_t capacity, int64_t *count);
 /* Comparator events of the last process call before the readout FIFO, in
@@ -228,6 +263,15 @@ int dvs_batch_drain(dvs_batch batch, int32_t segment, int64_t *ready_count);
 int dvs_batch_segment_info(dvs_batch batch, int32_t segment, int64_t *queued_horizons, int64_t *done_horizons,
                            int64_t *events, int64_t *device_bytes);
 int dvs_batch_remove(dvs_batch batch, int32_t segment);
+/* Adaptive windows for a segment, with the host driver policy of
+ * apps/main_dvs.cpp:96-116 applied per segment: windows of `horizons`
+ * horizons (process_adaptive at `tolerance`), the receptor check at the
+ * first window and then once every check_period seconds (never when
+ * negative); a rejected window is retried at half the length, and a rejected
+ * one-horizon window is integrated natively. */
+int dvs_batch_set_adaptive(dvs_batch batch, int32_t segment, double tolerance, int32_t horizons,
+                           double check_period);
+int dvs_batch_refinement_stats(dvs_batch batch, int32_t segment, dvs_refinement_stats *out);
 int dvs_batch_stats(dvs_batch batch, dvs_stats *out);
 int dvs_batch_reset_stats(dvs_batch batch);
 

@@ -1003,6 +1003,271 @@ int orc_sensor_process(void *sp, const void *scp, const orc_interval *ivs, int n
   if (ready) std::memcpy(ready, out.data(), sizeof(orc_event) * std::min<int64_t>(ready_cap, out.size()));
   return 0;
 }
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Adaptive optics: OpticalSampler::refine / prepare_exact_cells /
+// sample_cells (src/cuda/optical_sampler.cu:25-84,114-138,176-264) and
+// EventSensor::process_adaptive / verify_receptor
+// (src/cuda/event_sensor.cu:88-150,306-374), one pixel at a time.
+// ---------------------------------------------------------------------------
+namespace orc {
+struct Knot { double fraction, radiance; };
+struct ExactCell { double start, end; int interval, index; };
+
+// native_pose (optical_sampler.cu:26-30).
+static void native_pose(const orc_interval *ivs, int count, double t, orc_state &out) {
+  int lo = 0, hi = count - 1;
+  while (lo < hi) { int m = (lo + hi) / 2; if (t > ivs[m].t1) lo = m + 1; else hi = m; }
+  optical_state_at(ivs[lo], t, out);
+}
+
+// refine_pixels (optical_sampler.cu:39-84) for one pixel: returns the knot
+// count or -1 (fallback), adding to the statistics.
+static int refine_pixel(const Scene &sc, int pixel, const orc_interval *ivs, int n, double start, double end,
+                        const orc_photo_params &photo, double tolerance, const orc_pixel_state &state,
+                        double charge, int capacity, Knot *knots, uint64_t &queries, int &exact_segments,
+                        bool &invalid) {
+  auto radiance = [&](double t) {
+    orc_state pose; native_pose(ivs, n, t, pose);
+    PixelSample o = sample_pixel(sc, pixel, pose);
+    if (!o.valid || !std::isfinite(o.radiance)) invalid = true;
+    ++queries;
+    return o.radiance;
+  };
+  double initial[5];
+  for (int j = 0; j < 5; ++j) initial[j] = radiance(start + (end - start) * j * .25);
+  struct Entry { double a, b, left, middle, right; };
+  Entry stack[16];
+  stack[0] = {.5, 1, initial[2], initial[3], initial[4]};
+  stack[1] = {0, .5, initial[0], initial[1], initial[2]};
+  knots[0] = {0, initial[0]};
+  int top = 2, written = 1;
+  while (top) {
+    const Entry e = stack[--top];
+    const double actual = e.middle * photo.electrons_per_radiance + photo.dark_electrons;
+    const double estimate = (e.left + e.right) * .5 * photo.electrons_per_radiance + photo.dark_electrons;
+    const double residual = fabs(log(fmax(1e-30, actual) / fmax(1e-30, estimate)));
+    const double leaf = (e.b - e.a) * (end - start);
+    const double verr = fabs(receptor_log_at(state.receptor_log, actual, leaf, charge) -
+                             receptor_log_at(state.receptor_log, estimate, leaf, charge));
+    const double budget = 1e-3 * fmin(state.threshold_pos, state.threshold_neg);
+    if (verr <= budget && (residual <= tolerance || fmax(actual, estimate) * leaf < 1.0)) {
+      if (written + 2 > capacity) return -1;
+      knots[written] = {(e.a + e.b) * .5, e.middle};
+      knots[written + 1] = {e.b, e.right};
+      written += 2;
+    } else if (leaf <= photo.integration_step || top + 2 > 16) {
+      if (written >= capacity) return -1;
+      knots[written++] = {-e.b, e.right};
+      ++exact_segments;
+    } else {
+      const double mid = (e.a + e.b) * .5;
+      const double left_mid = radiance(start + (end - start) * (e.a + mid) * .5);
+      const double right_mid = radiance(start + (end - start) * (mid + e.b) * .5);
+      stack[top] = {mid, e.b, e.middle, right_mid, e.right};
+      stack[top + 1] = {e.a, mid, e.left, left_mid, e.middle};
+      top += 2;
+    }
+  }
+  return written;
+}
+
+// Knot bracket of a cell midpoint fraction (the shared walk of
+// integrate_optical_knots / identify_exact_cells / advance_mean_probes).
+static inline int knot_bracket(const Knot *v, int size, double fraction, int a) {
+  while (a + 2 < size && fraction > fabs(v[a + 1].fraction)) ++a;
+  return a;
+}
+
+template <class F>
+static void for_each_cell(const orc_interval *ivs, int n, double step_len, F &&f) {
+  int cell = 0;
+  for (int k = 0; k < n; ++k) {
+    const orc_interval &iv = ivs[k];
+    const int steps = (int)ceil((iv.t1 - iv.t0) / step_len);
+    const double dt = (iv.t1 - iv.t0) / steps;
+    for (int step = 0; step < steps; ++step, ++cell)
+      f(k, cell, iv.t0 + step * dt, step + 1 == steps ? iv.t1 : iv.t0 + (step + 1) * dt);
+  }
+}
+}  // namespace orc
+
+extern "C" {
+
+int orc_sensor_process_adaptive(void *sp, const void *scp, const orc_interval *ivs, int n, double tolerance,
+                                int check, int capacity, int *accepted, orc_refinement_stats *stats_out,
+                                orc_receptor_probe *probe_out, orc_event *raw, int64_t raw_cap,
+                                int64_t *raw_count, orc_event *ready, int64_t ready_cap, int64_t *ready_count,
+                                int threads) {
+  Sensor &s = *static_cast<Sensor *>(sp);
+  const Scene &sc = *static_cast<const Scene *>(scp);
+  if (accepted) *accepted = 0;
+  if (n <= 0 || capacity < 3) return 3;
+  const double start = ivs[0].t0, end = ivs[n - 1].t1;
+  if (!std::isfinite(tolerance) || tolerance <= 0 || s.p.refractory_time <= 0 ||
+      (end - start) / s.p.refractory_time + 2 > s.max_events)
+    return 3;
+  if (!s.initialized) return 3;
+  double previous = s.last_time, cells_total = 0;
+  for (int k = 0; k < n; ++k) {
+    const orc_interval &iv = ivs[k];
+    if (!std::isfinite(iv.t0) || !std::isfinite(iv.t1) || iv.t0 != previous || iv.t1 <= iv.t0) return 3;
+    cells_total += (iv.t1 - iv.t0) / s.photo.integration_step; previous = iv.t1;
+  }
+  if (cells_total > 1000000) return 3;
+  set_threads(threads);
+  const double step_len = s.photo.integration_step, charge = s.p.charge_scale;
+  std::vector<Knot> knots((size_t)s.count * capacity);
+  std::vector<int> kcount(s.count);
+  std::vector<std::vector<ExactCell>> exact(s.count);
+  std::vector<std::vector<double>> exact_rad(s.count);
+  uint64_t queries = 0; int exact_segments = 0, fallback = 0, max_knots = 0, native_queries = 0, invalid = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : queries, exact_segments, fallback, native_queries) \
+    reduction(max : max_knots) reduction(| : invalid)
+  for (int i = 0; i < s.count; ++i) {
+    bool bad = false;
+    Knot *v = knots.data() + (size_t)i * capacity;
+    int size = refine_pixel(sc, i, ivs, n, start, end, s.photo, tolerance, s.states[i], charge, capacity, v,
+                            queries, exact_segments, bad);
+    if (bad) invalid = 1;
+    if (size < 0) { kcount[i] = -1; ++fallback; continue; }
+    max_knots = std::max(max_knots, size);
+    // identify_exact_cells (optical_sampler.cu:114-131).
+    int a = 0; bool overflow = false;
+    for_each_cell(ivs, n, step_len, [&](int k, int cell, double left, double right) {
+      if (overflow) return;
+      const double fraction = ((left + right) * .5 - start) / (end - start);
+      a = knot_bracket(v, size, fraction, a);
+      if (v[a + 1].fraction < 0) {
+        if ((int)exact[i].size() == capacity) { overflow = true; return; }
+        exact[i].push_back({left, right, k, cell});
+      }
+    });
+    if (overflow) { kcount[i] = -1; ++fallback; exact[i].clear(); continue; }
+    kcount[i] = size;
+    native_queries += (int)exact[i].size();
+    // sample_exact_queries (optical_sampler.cu:132-138).
+    for (const ExactCell &c : exact[i]) {
+      orc_state pose; optical_state_at(ivs[c.interval], (c.start + c.end) * .5, pose);
+      PixelSample o = sample_pixel(sc, i, pose);
+      if (!o.valid || !std::isfinite(o.radiance)) invalid = 1;
+      exact_rad[i].push_back(o.radiance);
+    }
+  }
+  if (invalid) return 2;
+  if (stats_out) {
+    stats_out->traced_pixel_samples = queries; stats_out->fallback_pixels = fallback;
+    stats_out->maximum_knots = max_knots; stats_out->exact_segments = exact_segments;
+    stats_out->native_cell_queries = native_queries;
+  }
+  // Convergence gate: capture_mean_probes / advance_mean_probes /
+  // verify_receptor (event_sensor.cu:134-150,318-326,357-374).
+  if (check) {
+    const int np = std::min(1024, s.count);
+    std::vector<double> errors(np);
+    int cells_seen = 0, bad = 0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(| : bad)
+    for (int i = 0; i < np; ++i) {
+      const int pixel = (int)(((uint64_t)i * 2654435761ULL) % (uint64_t)s.count);
+      double ex = s.states[pixel].receptor_log, ca = ex;
+      const Knot *v = knots.data() + (size_t)pixel * capacity;
+      const int size = kcount[pixel];
+      for_each_cell(ivs, n, step_len, [&](int k, int, double left, double right) {
+        orc_state pose; optical_state_at(ivs[k], (left + right) * .5, pose);
+        PixelSample o = sample_pixel(sc, pixel, pose);
+        if (!o.valid || !std::isfinite(o.radiance)) bad = 1;
+        double light = o.radiance, cached = light;
+        if (size >= 0) {
+          const double fraction = ((left + right) * .5 - start) / (end - start);
+          const int a = knot_bracket(v, size, fraction, 0);
+          const double w = (fraction - fabs(v[a].fraction)) / (fabs(v[a + 1].fraction) - fabs(v[a].fraction));
+          if (v[a + 1].fraction >= 0) cached = v[a].radiance + (v[a + 1].radiance - v[a].radiance) * w;
+        }
+        ex = receptor_log_at(ex, light * s.photo.electrons_per_radiance + s.photo.dark_electrons, right - left, charge);
+        ca = receptor_log_at(ca, fmax(0.0, cached) * s.photo.electrons_per_radiance + s.photo.dark_electrons,
+                             right - left, charge);
+      });
+      errors[i] = fabs(ex - ca);
+    }
+    if (bad) return 2;
+    for_each_cell(ivs, n, step_len, [&](int, int, double, double) { ++cells_seen; });
+    orc_receptor_probe r{};
+    r.pixels = np; r.circuit_cells = cells_seen;
+    for (double e : errors) r.mean_log_voltage_error += e;
+    std::sort(errors.begin(), errors.end());
+    r.maximum_log_voltage_error = errors.back();
+    r.p99_log_voltage_error = errors[(size_t)(np * .99)];
+    r.mean_log_voltage_error /= np;
+    if (probe_out) *probe_out = r;
+    const double threshold = std::min(s.p.threshold_pos, s.p.threshold_neg);
+    if (r.maximum_log_voltage_error > .02 * threshold || r.p99_log_voltage_error > .005 * threshold) {
+      if (raw_count) *raw_count = 0;
+      if (ready_count) *ready_count = 0;
+      return 0;
+    }
+  }
+  // integrate_optical_knots and the exact fallback tiles
+  // (event_sensor.cu:88-132,331-353).
+  std::vector<std::vector<orc_event>> per_pixel(s.count);
+  int err_capacity = 0, err_invalid = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(| : err_capacity, err_invalid)
+  for (int i = 0; i < s.count; ++i) {
+    std::vector<orc_pixel_event> local(s.max_events);
+    int count = 0, tile_start = 0; bool failed = false;
+    Stream shot{s.seed, (uint64_t)i, kStreamShot, &s.shot_block[i]};
+    Stream reset{s.seed, (uint64_t)i, kStreamReset, &s.reset_block[i]};
+    auto normal = [&] { return normal_draw(reset); };
+    const Knot *v = knots.data() + (size_t)i * capacity;
+    const int size = kcount[i];
+    int a = 0; size_t e = 0;
+    for_each_cell(ivs, n, step_len, [&](int k, int cell, double t0, double t1) {
+      if (failed) return;
+      double light;
+      if (size < 0) {
+        // fallback tiles of 64 cells, each with its own event capacity
+        if (cell % 64 == 0) tile_start = count;
+        orc_state pose; optical_state_at(ivs[k], (t0 + t1) * .5, pose);
+        PixelSample o = sample_pixel(sc, i, pose);
+        if (!o.valid || !std::isfinite(o.radiance)) { err_invalid = 1; failed = true; return; }
+        light = o.radiance;
+      } else {
+        const double fraction = ((t0 + t1) * .5 - start) / (end - start);
+        a = knot_bracket(v, size, fraction, a);
+        const double w = (fraction - fabs(v[a].fraction)) / (fabs(v[a + 1].fraction) - fabs(v[a].fraction));
+        light = v[a].radiance + (v[a + 1].radiance - v[a].radiance) * w;
+        if (e < exact[i].size() && exact[i][e].index == cell) light = exact_rad[i][e++];
+        light = fmax(0.0, light);
+      }
+      double rate = light * s.photo.electrons_per_radiance + s.photo.dark_electrons;
+      if (s.photo.shot_noise) rate = poisson_draw(shot, rate * (t1 - t0)) / (t1 - t0);
+      const int cap = size < 0 ? s.max_events - (count - tile_start) : s.max_events - count;
+      if ((int)local.size() < count + cap) local.resize(count + cap);
+      const int emitted = advance_pixel(s.states[i], s.p, rate, t0, t1, local.data() + count, cap, normal);
+      if (emitted < 0) { err_capacity = 1; failed = true; return; }
+      count += emitted;
+    });
+    if (failed) continue;
+    for (int j = 0; j < count; ++j) {
+      orc_event ev; ev.timestamp = local[j].time; ev.x = (int16_t)(i % s.width); ev.y = (int16_t)(i / s.width);
+      ev.polarity = (int8_t)local[j].polarity; ev.depth = -1;
+      per_pixel[i].push_back(ev);
+    }
+  }
+  if (err_capacity) return 1;
+  if (err_invalid) return 2;
+  std::vector<orc_event> generated;
+  for (auto &pv : per_pixel) generated.insert(generated.end(), pv.begin(), pv.end());
+  if (raw_count) *raw_count = (int64_t)generated.size();
+  if (raw) std::memcpy(raw, generated.data(), sizeof(orc_event) * std::min<int64_t>(raw_cap, generated.size()));
+  std::vector<orc_event> out;
+  s.readout.submit(std::move(generated), end, out);
+  s.last_time = end;
+  if (accepted) *accepted = 1;
+  if (ready_count) *ready_count = (int64_t)out.size();
+  if (ready) std::memcpy(ready, out.data(), sizeof(orc_event) * std::min<int64_t>(ready_cap, out.size()));
+  return 0;
+}
 int orc_sensor_drain(void *sp, orc_event *ready, int64_t cap, int64_t *count) {
   Sensor &s = *static_cast<Sensor *>(sp);
   std::vector<orc_event> out; s.readout.drain(out);

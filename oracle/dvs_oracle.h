@@ -195,6 +195,27 @@ int orc_sensor_process(void *sensor, const void *scene, const orc_interval *inte
                        int n, orc_event *raw, int64_t raw_capacity, int64_t *raw_count,
                        orc_event *ready, int64_t ready_capacity, int64_t *ready_count,
                        int threads);
+/* OpticalRefinementStats (include/dvs/optical_sampler.hpp:16). */
+typedef struct {
+  uint64_t traced_pixel_samples;
+  int32_t fallback_pixels, maximum_knots, exact_segments, native_cell_queries;
+} orc_refinement_stats;
+/* ReceptorProbeResult (include/dvs/event_sensor.hpp:39). */
+typedef struct {
+  double maximum_log_voltage_error, p99_log_voltage_error, mean_log_voltage_error;
+  int32_t pixels, circuit_cells;
+} orc_receptor_probe;
+/* process_adaptive (src/cuda/event_sensor.cu:306-355) with OpticalSampler
+ * refine / prepare_exact_cells / sample_cells (src/cuda/optical_sampler.cu)
+ * at knot and exact-cell capacity `capacity`.  *accepted is 0 when the
+ * convergence gate rejected the window (no state changed).  Events carry
+ * depth -1.  Returns as orc_sensor_process. */
+int orc_sensor_process_adaptive(void *sensor, const void *scene, const orc_interval *intervals,
+                                int n, double tolerance, int check_convergence, int capacity,
+                                int *accepted, orc_refinement_stats *stats,
+                                orc_receptor_probe *probe, orc_event *raw, int64_t raw_capacity,
+                                int64_t *raw_count, orc_event *ready, int64_t ready_capacity,
+                                int64_t *ready_count, int threads);
 int orc_sensor_drain(void *sensor, orc_event *ready, int64_t capacity, int64_t *count);
 int64_t orc_sensor_dropped(const void *sensor);
 int64_t orc_sensor_peak_queue(const void *sensor);

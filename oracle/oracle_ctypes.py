@@ -29,6 +29,18 @@ PIXEL_STATE = np.dtype([("receptor_log", "<f8"), ("reference_log", "<f8"), ("ref
 PIXEL_EVENT = np.dtype([("time", "<f8"), ("polarity", "<i4"), ("pad", "<i4")])
 
 
+class RefinementStats(ctypes.Structure):
+    _fields_ = [("traced_pixel_samples", ctypes.c_uint64), ("fallback_pixels", ctypes.c_int32),
+                ("maximum_knots", ctypes.c_int32), ("exact_segments", ctypes.c_int32),
+                ("native_cell_queries", ctypes.c_int32)]
+
+
+class ReceptorProbe(ctypes.Structure):
+    _fields_ = [("maximum_log_voltage_error", ctypes.c_double), ("p99_log_voltage_error", ctypes.c_double),
+                ("mean_log_voltage_error", ctypes.c_double), ("pixels", ctypes.c_int32),
+                ("circuit_cells", ctypes.c_int32)]
+
+
 class Hapke(ctypes.Structure):
     _fields_ = [(n, ctypes.c_float) for n in ("b", "c", "hS", "hC", "thetap", "w", "B_S0", "B_C0", "phi")]
 
@@ -103,6 +115,8 @@ def lib() -> ctypes.CDLL:
             "orc_sensor_destroy": (None, [_P]),
             "orc_sensor_initialize": (I, [_P, _P, _P, D, I]),
             "orc_sensor_process": (I, [_P, _P, _P, I, _P, I64, ctypes.POINTER(I64), _P, I64, ctypes.POINTER(I64), I]),
+            "orc_sensor_process_adaptive": (I, [_P, _P, _P, I, ctypes.c_double, I, I, ctypes.POINTER(I), _P, _P,
+                                                _P, I64, ctypes.POINTER(I64), _P, I64, ctypes.POINTER(I64), I]),
             "orc_sensor_drain": (I, [_P, _P, I64, ctypes.POINTER(I64)]),
             "orc_sensor_dropped": (I64, [_P]),
             "orc_sensor_peak_queue": (I64, [_P]),
@@ -236,6 +250,22 @@ class Sensor:
         code = lib().orc_sensor_process(self.handle, scene.handle, ptr(iv), len(iv), ptr(raw), cap, ctypes.byref(nr),
                                         ptr(ready), cap, ctypes.byref(nd), threads)
         return code, raw[: nr.value].copy(), ready[: nd.value].copy()
+
+    def process_adaptive(self, scene: Scene, intervals, tolerance=1e-2, check=False, capacity=512, threads=0,
+                         events_capacity=None):
+        """process_adaptive: returns (code, accepted, raw, ready, refinement stats dict, probe dict)."""
+        iv = np.ascontiguousarray(np.atleast_1d(intervals), INTERVAL)
+        cap = events_capacity or self.width * self.height * 64
+        raw = np.empty(cap, EVENT)
+        ready = np.empty(cap, EVENT)
+        nr, nd, acc = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int()
+        st, pr = RefinementStats(), ReceptorProbe()
+        code = lib().orc_sensor_process_adaptive(self.handle, scene.handle, ptr(iv), len(iv), float(tolerance),
+                                                 int(check), int(capacity), ctypes.byref(acc), ctypes.byref(st),
+                                                 ctypes.byref(pr), ptr(raw), cap, ctypes.byref(nr), ptr(ready), cap,
+                                                 ctypes.byref(nd), threads)
+        as_dict = lambda c: {name: getattr(c, name) for name, _ in c._fields_}
+        return code, bool(acc.value), raw[: nr.value].copy(), ready[: nd.value].copy(), as_dict(st), as_dict(pr)
 
     def drain(self):
         cap = self.width * self.height * 64 + 1024
