@@ -35,7 +35,7 @@ struct AdaptDev {
   int fb_begin;          // first entry of the segment in the fallback pixel list
   int fb_count;          // fallback pixels of the segment
   int pad;
-  const double *fb_rad;  // [cell - chunk start][fb_count] native radiance of the fallback pixels
+  const double *fb_rad;  // [cell - chunk start][fb_count] photocurrent of the fallback pixels (native trace)
 };
 
 // An unresolved time segment [a, b] of a pixel (fractions of the window) with
@@ -66,6 +66,7 @@ struct AdaptBufs {
   double *erad;          // their radiance
   int *fb_index;         // index of the pixel in its segment's fallback list, -1 otherwise
   const double *cell_frac;  // window fraction of every cell midpoint of the step
+  double *rates;         // [cell - chunk start][pixel] photocurrent of knot pixels
 };
 
 struct AdaptCounters {
@@ -130,7 +131,9 @@ hipError_t launch_adapt_segment_stats(int nseg, const int *pixel_begin, AdaptBuf
 // gate_segs[g].pixel_list and their native cell radiance in gate_segs[g].rad.
 hipError_t launch_adapt_gate(const SegDev *gate_segs, const int *gate_seg, int ngate, const AdaptDev *ad,
                              AdaptBufs b, double *errors, hipStream_t st);
+hipError_t launch_adapt_rates(const SegDev *segs, const AdaptDev *ad, const int *pixel_begin, int nseg, int pixels,
+                              AdaptBufs b, int c0, int c1, int *status, hipStream_t st);
 hipError_t launch_adapt_integrate(const SegDev *segs, const AdaptDev *ad, const int *block_begin, int nseg,
-                                  int blocks, AdaptBufs b, int c0, int c1, int *status, hipStream_t st);
+                                  int blocks, int pixels, AdaptBufs b, int c0, int c1, int *status, hipStream_t st);
 
 }  // namespace dvsamd

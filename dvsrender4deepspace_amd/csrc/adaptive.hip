@@ -12,8 +12,11 @@
 //                        knot counts, prefix offsets and depth-first knot order
 //  adapt_exact_*         identify_exact_cells (optical_sampler.cu:114-131)
 //  adapt_gate            advance_mean_probes (event_sensor.cu:138-150)
+//  adapt_rates           photocurrent of every (cell, pixel) from knots or
+//                        exact cells, with the cell's shot-noise draw
 //  adapt_integrate       integrate_optical_knots and integrate_fallback_tile
-//                        (event_sensor.cu:88-132): one lane per pixel
+//                        (event_sensor.cu:88-132): one lane per pixel, the
+//                        serial receptor chain only
 #include <hip/hip_runtime.h>
 
 #include "adaptive.h"
@@ -378,11 +381,60 @@ __global__ void adapt_gate(const SegDev *gate_segs, const int *gate_seg, const A
   errors[g * 1024 + i] = fabs(ex - ca);
 }
 
-// Circuit of cells [c0, c1) of every pixel of an accepted window: knot or
-// exact-cell radiance, or the native radiance of a fallback pixel; shot noise
-// and comparator as in the native circuit (same per-pixel draw order).
+// Knot bracket by bisection: the index the walk above stops at, i.e. the
+// first a with fraction <= |v[a+1].x|, at most size - 2 (|x| is ascending).
+__device__ __forceinline__ int knot_bracket_search(const double2 *v, int size, double fraction) {
+  int lo = 1, hi = size - 1;  // first j in [1, size-1] with fraction <= |v[j].x|, else size-1
+  while (lo < hi) {
+    const int m = (lo + hi) >> 1;
+    if (fraction > fabs(v[m].x)) lo = m + 1; else hi = m;
+  }
+  return lo - 1;
+}
+
+// Photocurrent of every (cell, pixel) of cells [c0, c1) for pixels that
+// interpolate knots (integrate_optical_knots, event_sensor.cu:97-105): knot
+// or exact-cell radiance, then the cell's shot-noise draw.  One thread per
+// (pixel, cell): x over pixels, y over the chunk's cells.
+__global__ __launch_bounds__(256) void adapt_rates(const SegDev *segs, const AdaptDev *ad, const int *pixel_begin,
+                                                   int nseg, int pixels, AdaptBufs b, int c0, int *status) {
+  const int gp = blockIdx.x * 256 + threadIdx.x;
+  const int c = c0 + (int)blockIdx.y;
+  if (gp >= pixels) return;
+  const int si = segment_of<int>(pixel_begin, nseg, gp);
+  const SegDev &g = segs[si];
+  const AdaptDev &A = ad[si];
+  if (c >= g.cells || !A.accepted || (b.flags[gp] & 1)) return;
+  const int i = gp - pixel_begin[si];
+  const int size = b.kcount[gp];
+  const double2 *v = b.knots + (b.koff[gp] - size);
+  const double f = b.cell_frac[A.cell_offset + c];
+  const int a = knot_bracket_search(v, size, f);
+  const double w = (f - fabs(v[a].x)) / (fabs(v[a + 1].x) - fabs(v[a].x));
+  double light = v[a].y + (v[a + 1].y - v[a].y) * w;
+  const int ec = b.ecount[gp];
+  if (ec) {  // exact cell of this pixel?
+    const int *cells = b.ecell + (b.eoff[gp] - ec);
+    int lo = 0, hi = ec;
+    while (lo < hi) {
+      const int m = (lo + hi) >> 1;
+      if (cells[m] < c) lo = m + 1; else hi = m;
+    }
+    if (lo < ec && cells[lo] == c) light = b.erad[b.eoff[gp] - ec + lo];
+  }
+  light = fmax(0.0, light);
+  if (!isfinite(light)) atomicOr(status, 2);
+  const dvs_pixel_state &st = g.states[i];
+  b.rates[(size_t)blockIdx.y * pixels + gp] =
+      cell_rate(light, g.photo, g.t1[c] - g.t0[c], g.seed, (uint64_t)i, st.shot_block + (uint64_t)(c - c0));
+}
+
+// Circuit of cells [c0, c1) of every pixel of an accepted window: the
+// photocurrent of a knot pixel from adapt_rates, of a fallback pixel from
+// its native trace; comparator and reset noise as in the native circuit.
 __global__ __launch_bounds__(256) void adapt_integrate(const SegDev *segs, const AdaptDev *ad, const int *block_begin,
-                                                       int nseg, AdaptBufs b, int c0, int c1, int *status) {
+                                                       int nseg, int pixels, AdaptBufs b, int c0, int c1,
+                                                       int *status) {
   const int si = segment_of<int>(block_begin, nseg, (int)blockIdx.x);
   const SegDev &g = segs[si];
   const AdaptDev &A = ad[si];
@@ -392,40 +444,19 @@ __global__ __launch_bounds__(256) void adapt_integrate(const SegDev *segs, const
   dvs_pixel_state st = g.states[i];
   PixelRegs regs{st.receptor_log, st.reference_log, st.refractory_until, st.threshold_pos,
                  st.threshold_neg, st.reset_pending};
-  NoiseStream shot(g.seed, (uint64_t)i, kStreamShot, st.shot_block);
   NoiseStream reset(g.seed, (uint64_t)i, kStreamReset, st.reset_block);
   auto normal = [&]() { return reset.normal(); };
   CrossingEvent *out = g.slots + (size_t)i * g.max_events;
   const CircuitParams p = g.params;
-  const PhotoParams photo = g.photo;
   int count = g.counts[i];
   const bool fallback = b.flags[gp] & 1;
-  const int fbi = b.fb_index[gp];
-  const int size = b.kcount[gp];
-  const double2 *v = b.knots + (b.koff[gp] - size);
-  const int ec = b.ecount[gp], eo = b.eoff[gp] - ec;
-  const double *frac = b.cell_frac + A.cell_offset;
-  int a = 0, e = 0;
-  if (!fallback)
-    while (e < ec && b.ecell[eo + e] < c0) ++e;
+  const double *rates = fallback ? A.fb_rad + b.fb_index[gp] : b.rates + gp;
+  const size_t stride = fallback ? (size_t)A.fb_count : (size_t)pixels;
   const int c_end = min(c1, g.cells);
   for (int c = c0; c < c_end; ++c) {
-    const double t0 = g.t0[c], t1 = g.t1[c];
-    double light;
-    if (fallback) {
-      light = A.fb_rad[(size_t)(c - c0) * A.fb_count + fbi];
-    } else {
-      const double f = frac[c];
-      a = knot_bracket(v, size, f, a);
-      const double w = (f - fabs(v[a].x)) / (fabs(v[a + 1].x) - fabs(v[a].x));
-      light = v[a].y + (v[a + 1].y - v[a].y) * w;
-      if (e < ec && b.ecell[eo + e] == c) light = b.erad[eo + e++];
-      light = fmax(0.0, light);
-    }
-    if (!isfinite(light)) { atomicOr(status, 2); g.counts[i] = 0; return; }
-    double rate = light * photo.electrons_per_radiance + photo.dark_electrons;
-    if (photo.shot_noise) rate = shot.poisson(rate * (t1 - t0)) / (t1 - t0);
-    const int emitted = advance_pixel(regs, p, rate, t0, t1, out + count, g.max_events - count, normal);
+    const double rate = rates[(size_t)(c - c0) * stride];
+    if (!isfinite(rate)) { atomicOr(status, 2); g.counts[i] = 0; return; }
+    const int emitted = advance_pixel(regs, p, rate, g.t0[c], g.t1[c], out + count, g.max_events - count, normal);
     if (emitted < 0) { atomicOr(status, 1); g.counts[i] = 0; return; }
     count += emitted;
   }
@@ -435,7 +466,7 @@ __global__ __launch_bounds__(256) void adapt_integrate(const SegDev *segs, const
   st.threshold_pos = regs.threshold_pos;
   st.threshold_neg = regs.threshold_neg;
   st.reset_pending = regs.reset_pending;
-  st.shot_block = shot.block;
+  st.shot_block += (uint64_t)max(0, c_end - c0);
   st.reset_block = (uint32_t)reset.block;
   g.states[i] = st;
   g.counts[i] = count;
@@ -569,10 +600,18 @@ hipError_t launch_adapt_gate(const SegDev *gate_segs, const int *gate_seg, int n
   adapt_gate<<<ngate * 4, 256, 0, st>>>(gate_segs, gate_seg, ad, b, errors);
   return hipGetLastError();
 }
+hipError_t launch_adapt_rates(const SegDev *segs, const AdaptDev *ad, const int *pixel_begin, int nseg, int pixels,
+                              AdaptBufs b, int c0, int c1, int *status, hipStream_t st) {
+  if (pixels <= 0 || c1 <= c0) return hipSuccess;
+  if (c1 - c0 > 65535) return hipErrorInvalidConfiguration;
+  adapt_rates<<<dim3(grid(pixels), (unsigned)(c1 - c0)), 256, 0, st>>>(segs, ad, pixel_begin, nseg, pixels, b, c0,
+                                                                      status);
+  return hipGetLastError();
+}
 hipError_t launch_adapt_integrate(const SegDev *segs, const AdaptDev *ad, const int *block_begin, int nseg,
-                                  int blocks, AdaptBufs b, int c0, int c1, int *status, hipStream_t st) {
+                                  int blocks, int pixels, AdaptBufs b, int c0, int c1, int *status, hipStream_t st) {
   if (blocks <= 0 || c1 <= c0) return hipSuccess;
-  adapt_integrate<<<blocks, 256, 0, st>>>(segs, ad, block_begin, nseg, b, c0, c1, status);
+  adapt_integrate<<<blocks, 256, 0, st>>>(segs, ad, block_begin, nseg, pixels, b, c0, c1, status);
   return hipGetLastError();
 }
 

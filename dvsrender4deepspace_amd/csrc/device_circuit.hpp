@@ -32,10 +32,12 @@ __device__ __forceinline__ double receptor_log_at(double u, double rate, double 
 
 // ---------------------------------------------------------------------------
 // Noise.  Philox4x64-10 (Salmon, Moraes, Dror, Shaw, SC'11) keyed by
-// (seed, tag ^ stream), counter (block, pixel, 0, 0).  Each draw starts a
-// fresh block, so a pixel's stream depends only on how many draws it has
-// made: batching, launch shape and the presence of other requests cannot
-// change it.
+// (seed, tag ^ stream), counter (block, pixel, word, 0).  Each draw starts a
+// fresh block.  Shot noise is keyed per circuit cell: the draw of a pixel's
+// n-th cell since initialisation uses word = n and blocks 0, 1, ... of its
+// own, so it can be drawn by any thread at any time.  Reset noise (word 0)
+// advances a per-pixel block counter, as its draws occur at crossings.
+// Batching, launch shape and other requests cannot change either stream.
 // ---------------------------------------------------------------------------
 enum { kStreamShot = 0, kStreamThreshold = 1, kStreamReset = 2 };
 constexpr uint64_t kStreamTag = 0x4456535F4E4F4953ULL;
@@ -53,13 +55,13 @@ __device__ __forceinline__ void philox4x64(uint64_t c0, uint64_t c1, uint64_t c2
   out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 struct NoiseStream {
-  uint64_t seed, pixel, stream, block;
+  uint64_t seed, pixel, stream, block, word;
   uint64_t buf[4];
   int used;
-  __device__ NoiseStream(uint64_t s, uint64_t p, uint64_t st, uint64_t b)
-      : seed(s), pixel(p), stream(st), block(b), used(4) {}
+  __device__ NoiseStream(uint64_t s, uint64_t p, uint64_t st, uint64_t b, uint64_t w = 0)
+      : seed(s), pixel(p), stream(st), block(b), word(w), used(4) {}
   __device__ double uniform() {  // open interval (0, 1), 53 bits
-    if (used == 4) { philox4x64(block++, pixel, 0, 0, seed, kStreamTag ^ stream, buf); used = 0; }
+    if (used == 4) { philox4x64(block++, pixel, word, 0, seed, kStreamTag ^ stream, buf); used = 0; }
     // select without a dynamic index so the block stays in registers
     const uint64_t x = used == 0 ? buf[0] : used == 1 ? buf[1] : used == 2 ? buf[2] : buf[3];
     ++used;
@@ -98,6 +100,19 @@ struct NoiseStream {
     }
   }
 };
+
+// Photocurrent of one circuit cell (generate_pixels, event_sensor.cu:42-43):
+// radiance -> electrons/s, with shot noise rate = Poisson(rate * dt) / dt
+// (poisson_electrons, photon_noise.cuh:4-9) from the cell's own counter.
+__device__ inline double cell_rate(double light, const PhotoParams &ph, double dt, uint64_t seed, uint64_t pixel,
+                                   uint64_t cell_serial) {
+  double rate = light * ph.electrons_per_radiance + ph.dark_electrons;
+  if (ph.shot_noise) {
+    NoiseStream shot(seed, pixel, kStreamShot, 0, cell_serial);
+    rate = shot.poisson(rate * dt) / dt;
+  }
+  return rate;
+}
 
 // ---------------------------------------------------------------------------
 // advance_pixel (include/dvs/pixel_model.hpp:35-102) in FP64: refractory hold,

@@ -8,6 +8,8 @@
 //                      MODE_DEPTH   event-time depth (primary rays only).
 //  *_multi kernels     the engine step over a table of segments (sensors):
 //                      trace_cells_multi   pixel x cell radiance per segment,
+//                      cell_rates_multi    radiance -> photocurrent with the
+//                                          cell's Philox shot-noise draw,
 //                      circuit_multi       one lane per pixel: FP64 receptor
 //                                          integration, comparator crossings,
 //                                          Philox shot/reset noise,
@@ -174,6 +176,27 @@ __global__ __launch_bounds__(256) void trace_cells_multi(const SegDev *segs, con
   g.rad[(size_t)cell * npix + pixel] = total / R;
 }
 
+// Radiance -> photocurrent in place for every (item, cell) of a chunk table:
+// x over the items (pixels, or pixel-list entries) of all segments, y over
+// the chunk's cells.  The shot-noise draw of a cell uses the pixel's cell
+// serial (cells integrated so far, advanced by the circuit after each chunk)
+// plus the cell's index in the chunk.
+__global__ __launch_bounds__(256) void cell_rates_multi(const SegDev *segs, const int *item_begin, int nseg,
+                                                        int items, int *status) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  const int c = (int)blockIdx.y;
+  if (k >= items) return;
+  const int si = segment_of<int>(item_begin, nseg, k);
+  const SegDev &g = segs[si];
+  const int item = k - item_begin[si];
+  if (c >= g.cells || item >= g.npix) return;
+  const int px = g.pixel_list ? g.pixel_list[item] : item;
+  double *cell = g.rad + (size_t)c * g.npix + item;
+  const double light = *cell;
+  if (!isfinite(light)) atomicOr(status, 2);
+  *cell = cell_rate(light, g.photo, g.t1[c] - g.t0[c], g.seed, (uint64_t)px, g.states[px].shot_block + (uint64_t)c);
+}
+
 // The circuit half of generate_pixels (event_sensor.cu:39-49): one lane per
 // pixel integrates all cells of this step in order.
 __global__ __launch_bounds__(256) void circuit_multi(const SegDev *segs, const int *block_begin, int nseg,
@@ -185,19 +208,15 @@ __global__ __launch_bounds__(256) void circuit_multi(const SegDev *segs, const i
   dvs_pixel_state st = g.states[i];
   PixelRegs regs{st.receptor_log, st.reference_log, st.refractory_until, st.threshold_pos,
                  st.threshold_neg, st.reset_pending};
-  NoiseStream shot(g.seed, (uint64_t)i, kStreamShot, st.shot_block);
   NoiseStream reset(g.seed, (uint64_t)i, kStreamReset, st.reset_block);
   auto normal = [&]() { return reset.normal(); };
   CrossingEvent *out = g.slots + (size_t)i * g.max_events;
   const CircuitParams p = g.params;
-  const PhotoParams photo = g.photo;
   int count = g.counts[i];  // crossings of earlier cell chunks of this step
   for (int c = 0; c < g.cells; ++c) {
     const double t0 = g.t0[c], t1 = g.t1[c];
-    const double radiance = g.rad[(size_t)c * g.npix + i];
-    if (!isfinite(radiance)) { atomicOr(status, 2); g.counts[i] = 0; return; }
-    double rate = radiance * photo.electrons_per_radiance + photo.dark_electrons;
-    if (photo.shot_noise) rate = shot.poisson(rate * (t1 - t0)) / (t1 - t0);
+    const double rate = g.rad[(size_t)c * g.npix + i];  // photocurrent from the trace
+    if (!isfinite(rate)) { atomicOr(status, 2); g.counts[i] = 0; return; }
     const int emitted = advance_pixel(regs, p, rate, t0, t1, out + count, g.max_events - count, normal);
     if (emitted < 0) { atomicOr(status, 1); g.counts[i] = 0; return; }
     count += emitted;
@@ -208,7 +227,7 @@ __global__ __launch_bounds__(256) void circuit_multi(const SegDev *segs, const i
   st.threshold_pos = regs.threshold_pos;
   st.threshold_neg = regs.threshold_neg;
   st.reset_pending = regs.reset_pending;
-  st.shot_block = shot.block;
+  st.shot_block += (uint64_t)g.cells;
   st.reset_block = (uint32_t)reset.block;
   g.states[i] = st;
   g.counts[i] = count;
@@ -434,6 +453,14 @@ hipError_t launch_trace_cells_multi(const SegDev *segs, const long long *block_b
   if (blocks <= 0) return hipSuccess;
   if (blocks > kMaxBlocks256) return hipErrorInvalidConfiguration;
   trace_cells_multi<<<(unsigned)blocks, 256, lds, st>>>(segs, block_begin, nseg, status, shadow_count);
+  return hipGetLastError();
+}
+hipError_t launch_cell_rates_multi(const SegDev *segs, const int *item_begin, int nseg, int items, int cells,
+                                   int *status, hipStream_t st) {
+  if (items <= 0 || cells <= 0) return hipSuccess;
+  if (cells > 65535) return hipErrorInvalidConfiguration;
+  cell_rates_multi<<<dim3((unsigned)((items + 255) / 256), (unsigned)cells), 256, 0, st>>>(segs, item_begin, nseg,
+                                                                                          items, status);
   return hipGetLastError();
 }
 hipError_t launch_circuit_multi(const SegDev *segs, const int *block_begin, int nseg, int blocks, int *status,

@@ -471,7 +471,8 @@ class Engine {
     }
     const double block_cap = (double)(dvsdev::kMaxBlocks256 - (long long)work.size()) / std::max(block_rate, 1e-30);
     const int chunk = (int)std::max<double>(
-        1.0, std::min<double>({(double)max_cells, (double)(radiance_budget_ / (sizeof(double) * pixel_total)), block_cap}));
+        1.0, std::min<double>({(double)max_cells, (double)(radiance_budget_ / (sizeof(double) * pixel_total)), block_cap,
+                               65535.0}));
     const int chunks = (max_cells + chunk - 1) / chunk;
     radiance_.reserve((size_t)chunk * pixel_total, "cell radiance");
     // Segment tables of every chunk (device), then the launches.
@@ -532,6 +533,9 @@ class Engine {
           hip_check(hipEventRecord(trace_t.back()->stop, st), "record");
           hip_check(hipEventRecord(circuit_t.back()->start, st), "record");
         }
+        hip_check(launch_cell_rates_multi(segtab_.ptr + (size_t)k * nseg, pbegin_.ptr, nseg, pixel_total, chunk,
+                                          status_.ptr, st),
+                  "trace and integrate circuits");
         hip_check(launch_circuit_multi(segtab_.ptr + (size_t)k * nseg, cblock_.ptr, nseg, cblocks_total, status_.ptr, st),
                   "trace and integrate circuits");
         if (!graph) hip_check(hipEventRecord(circuit_t.back()->stop, st), "record");
@@ -956,14 +960,21 @@ class Engine {
     hip_check(hipEventRecord(circuit_t.start, st), "record");
     int max_cells = 0;
     for (auto &w : work) max_cells = std::max(max_cells, w.cells);
-    int chunk = max_cells;
-    if (fb_total > 0)
-      chunk = (int)std::max<size_t>(1, std::min<size_t>(max_cells, radiance_budget_ / (sizeof(double) * fb_total)));
+    // Photocurrent of every pixel for a chunk of cells: knot pixels in
+    // rates_ [cell][P], fallback pixels in radiance_ [cell][fallback list].
+    const int chunk = (int)std::max<size_t>(
+        1, std::min<size_t>({(size_t)max_cells, (size_t)65535,
+                             radiance_budget_ / (sizeof(double) * ((size_t)P + (size_t)fb_total))}));
+    rates_.reserve((size_t)chunk * P, "adaptive photocurrent");
+    b.rates = rates_.ptr;
     uint64_t fallback_samples = 0;
     std::vector<long long> fblocks(nseg + 1);
     std::vector<SegDev> ftab(nseg);
     if (fb_total > 0) {
       radiance_.reserve((size_t)chunk * fb_total, "fallback optical tile");
+      std::vector<int> fb_items(nseg + 1, 0);
+      for (int i = 0; i < nseg; ++i) fb_items[i + 1] = fb_ends[i];
+      fb_items_.upload(fb_items.data(), fb_items.size(), "fallback optical tile", st);
       size_t off = 0;
       for (int i = 0; i < nseg; ++i) {
         const int begin = i ? fb_ends[i - 1] : 0, count = fb_ends[i] - begin;
@@ -989,12 +1000,17 @@ class Engine {
         }
         fb_tab_.upload(ftab.data(), ftab.size(), "fallback optical tile", st);
         fb_blocks_.upload(fblocks.data(), fblocks.size(), "fallback optical tile", st);
-        if (fblocks[nseg] > 0)
+        if (fblocks[nseg] > 0) {
           hip_check(launch_trace_cells_multi(fb_tab_.ptr, fb_blocks_.ptr, nseg, fblocks[nseg], lds, status_.ptr,
                                              counter_.ptr, st),
                     "parallel exact cell optics launch");
+          hip_check(launch_cell_rates_multi(fb_tab_.ptr, fb_items_.ptr, nseg, fb_total, c1 - c0, status_.ptr, st),
+                    "parallel exact cell optics launch");
+        }
       }
-      hip_check(launch_adapt_integrate(segtab_.ptr, adapt_.ptr, cblock_.ptr, nseg, circuit_blocks_, b, c0, c1,
+      hip_check(launch_adapt_rates(segtab_.ptr, adapt_.ptr, pbegin_.ptr, nseg, P, b, c0, c1, status_.ptr, st),
+                "adaptive photocurrent launch");
+      hip_check(launch_adapt_integrate(segtab_.ptr, adapt_.ptr, cblock_.ptr, nseg, circuit_blocks_, P, b, c0, c1,
                                        status_.ptr, st),
                 "adaptive circuit launch");
       if (fb_total > 0) hip_check(hipStreamSynchronize(st), "exact fallback circuit tile");  // tables are reused
@@ -1050,8 +1066,8 @@ class Engine {
   DeviceArray<AdaptDev> adapt_;
   DeviceArray<int> cellbeg_, bound_, flags_, bisects_, exact_terms_, kcount_, koff_, ecount_, eoff_, fb_index_,
       fb_scan_, run_begin_, qpix_, contrib_, cscan_, ecell_, eitem_pix_, eitem_pose_, fb_ends_, fb_list_, gate_ids_,
-      gate_idx_;
-  DeviceArray<double> cell_frac_, probes_, qrad_, erad_, gate_rad_, gate_err_;
+      gate_idx_, fb_items_;
+  DeviceArray<double> cell_frac_, probes_, qrad_, erad_, gate_rad_, gate_err_, rates_;
   DeviceArray<Pose> probe_poses_, qpose_;
   DeviceArray<AdaptEntry> ent_[2], bis_;
   DeviceArray<AdaptCounters> ctr_;

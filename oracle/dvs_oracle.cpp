@@ -641,13 +641,17 @@ static void philox4x64(const uint64_t ctr[4], const uint64_t key[2], uint64_t ou
 }
 enum { kStreamShot = 0, kStreamThreshold = 1, kStreamReset = 2 };
 constexpr uint64_t kStreamTag = 0x4456535F4E4F4953ULL;  // key word 1 = tag ^ stream
+// Counter (block, pixel, word, 0).  Shot noise: one stream per circuit cell,
+// word = the pixel's cell serial since initialisation, blocks from 0; reset
+// and threshold noise: word 0 and a per-pixel block counter.
 struct Stream {
   uint64_t seed, pixel, stream;
   uint64_t *block;  // next unused block index for this stream
+  uint64_t word = 0;
   uint64_t buf[4]; int used = 4;
   double uniform() {
     if (used == 4) {
-      uint64_t ctr[4] = {(*block)++, pixel, 0, 0}, key[2] = {seed, kStreamTag ^ stream};
+      uint64_t ctr[4] = {(*block)++, pixel, word, 0}, key[2] = {seed, kStreamTag ^ stream};
       philox4x64(ctr, key, buf); used = 0;
     }
     return ((double)(buf[used++] >> 11) + 0.5) * (1.0 / 9007199254740992.0);
@@ -678,6 +682,17 @@ static double poisson_draw(Stream &st, double lambda) {
     if (k < 0 || (us < 0.013 && V > us)) continue;
     if (log(V) + log(invalpha) - log(a / (us * us) + b) <= -lambda + k * loglam - lgamma(k + 1)) return k;
   }
+}
+// Photocurrent of one circuit cell with its shot-noise draw (cell serial n).
+static double cell_rate(double light, const orc_photo_params &ph, double dt, uint64_t seed, uint64_t pixel,
+                        uint64_t serial) {
+  double rate = light * ph.electrons_per_radiance + ph.dark_electrons;
+  if (ph.shot_noise) {
+    uint64_t block = 0;
+    Stream shot{seed, pixel, kStreamShot, &block, serial};
+    rate = poisson_draw(shot, rate * dt) / dt;
+  }
+  return rate;
 }
 
 // ---------------------------------------------------------------------------
@@ -956,7 +971,6 @@ int orc_sensor_process(void *sp, const void *scp, const orc_interval *ivs, int n
   for (int i = 0; i < s.count; ++i) {
     std::vector<orc_pixel_event> local(s.max_events);
     int count = 0; bool failed = false;
-    Stream shot{s.seed, (uint64_t)i, kStreamShot, &s.shot_block[i]};
     Stream reset{s.seed, (uint64_t)i, kStreamReset, &s.reset_block[i]};
     auto normal = [&] { return normal_draw(reset); };
     for (int k = 0; k < n && !failed; ++k) {
@@ -969,8 +983,7 @@ int orc_sensor_process(void *sp, const void *scp, const orc_interval *ivs, int n
         PixelSample o = sample_pixel(sc, i, pose);
         shadow += o.shadow_rays;
         if (!o.valid || !std::isfinite(o.radiance)) { err_invalid = 1; failed = true; break; }
-        double rate = o.radiance * s.photo.electrons_per_radiance + s.photo.dark_electrons;
-        if (s.photo.shot_noise) rate = poisson_draw(shot, rate * (t1 - t0)) / (t1 - t0);
+        const double rate = cell_rate(o.radiance, s.photo, t1 - t0, s.seed, (uint64_t)i, s.shot_block[i]++);
         int emitted = advance_pixel(s.states[i], s.p, rate, t0, t1, local.data() + count, s.max_events - count, normal);
         if (emitted < 0) { err_capacity = 1; failed = true; break; }
         count += emitted;
@@ -1215,7 +1228,6 @@ int orc_sensor_process_adaptive(void *sp, const void *scp, const orc_interval *i
   for (int i = 0; i < s.count; ++i) {
     std::vector<orc_pixel_event> local(s.max_events);
     int count = 0, tile_start = 0; bool failed = false;
-    Stream shot{s.seed, (uint64_t)i, kStreamShot, &s.shot_block[i]};
     Stream reset{s.seed, (uint64_t)i, kStreamReset, &s.reset_block[i]};
     auto normal = [&] { return normal_draw(reset); };
     const Knot *v = knots.data() + (size_t)i * capacity;
@@ -1239,8 +1251,7 @@ int orc_sensor_process_adaptive(void *sp, const void *scp, const orc_interval *i
         if (e < exact[i].size() && exact[i][e].index == cell) light = exact_rad[i][e++];
         light = fmax(0.0, light);
       }
-      double rate = light * s.photo.electrons_per_radiance + s.photo.dark_electrons;
-      if (s.photo.shot_noise) rate = poisson_draw(shot, rate * (t1 - t0)) / (t1 - t0);
+      const double rate = cell_rate(light, s.photo, t1 - t0, s.seed, (uint64_t)i, s.shot_block[i]++);
       const int cap = size < 0 ? s.max_events - (count - tile_start) : s.max_events - count;
       if ((int)local.size() < count + cap) local.resize(count + cap);
       const int emitted = advance_pixel(s.states[i], s.p, rate, t0, t1, local.data() + count, cap, normal);
