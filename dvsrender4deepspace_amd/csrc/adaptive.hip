@@ -394,39 +394,45 @@ __device__ __forceinline__ int knot_bracket_search(const double2 *v, int size, d
 
 // Photocurrent of every (cell, pixel) of cells [c0, c1) for pixels that
 // interpolate knots (integrate_optical_knots, event_sensor.cu:97-105): knot
-// or exact-cell radiance, then the cell's shot-noise draw.  One thread per
-// (pixel, cell): x over pixels, y over the chunk's cells.
+// or exact-cell radiance, then the cell's shot-noise draw.  A thread takes
+// one pixel and kRateCells consecutive cells (x over pixels, y over cell
+// runs): one bisection finds the first cell's knot bracket and exact-cell
+// position, then both walk forward as in the reference, so each knot is read
+// once per run while the rate stores stay coalesced across pixels.
+constexpr int kRateCells = 32;
 __global__ __launch_bounds__(256) void adapt_rates(const SegDev *segs, const AdaptDev *ad, const int *pixel_begin,
-                                                   int nseg, int pixels, AdaptBufs b, int c0, int *status) {
+                                                   int nseg, int pixels, AdaptBufs b, int c0, int c1, int *status) {
   const int gp = blockIdx.x * 256 + threadIdx.x;
-  const int c = c0 + (int)blockIdx.y;
   if (gp >= pixels) return;
   const int si = segment_of<int>(pixel_begin, nseg, gp);
   const SegDev &g = segs[si];
   const AdaptDev &A = ad[si];
-  if (c >= g.cells || !A.accepted || (b.flags[gp] & 1)) return;
+  const int cb = c0 + (int)blockIdx.y * kRateCells, ce = min(min(c1, g.cells), cb + kRateCells);
+  if (cb >= ce || !A.accepted || (b.flags[gp] & 1)) return;
   const int i = gp - pixel_begin[si];
   const int size = b.kcount[gp];
   const double2 *v = b.knots + (b.koff[gp] - size);
-  const double f = b.cell_frac[A.cell_offset + c];
-  const int a = knot_bracket_search(v, size, f);
-  const double w = (f - fabs(v[a].x)) / (fabs(v[a + 1].x) - fabs(v[a].x));
-  double light = v[a].y + (v[a + 1].y - v[a].y) * w;
-  const int ec = b.ecount[gp];
-  if (ec) {  // exact cell of this pixel?
-    const int *cells = b.ecell + (b.eoff[gp] - ec);
-    int lo = 0, hi = ec;
-    while (lo < hi) {
-      const int m = (lo + hi) >> 1;
-      if (cells[m] < c) lo = m + 1; else hi = m;
-    }
-    if (lo < ec && cells[lo] == c) light = b.erad[b.eoff[gp] - ec + lo];
+  const double *frac = b.cell_frac + A.cell_offset;
+  int a = knot_bracket_search(v, size, frac[cb]);
+  const int ec = b.ecount[gp], eo = b.eoff[gp] - ec;
+  int e = 0, hi = ec;  // first exact cell >= cb
+  while (e < hi) {
+    const int m = (e + hi) >> 1;
+    if (b.ecell[eo + m] < cb) e = m + 1; else hi = m;
   }
-  light = fmax(0.0, light);
-  if (!isfinite(light)) atomicOr(status, 2);
-  const dvs_pixel_state &st = g.states[i];
-  b.rates[(size_t)blockIdx.y * pixels + gp] =
-      cell_rate(light, g.photo, g.t1[c] - g.t0[c], g.seed, (uint64_t)i, st.shot_block + (uint64_t)(c - c0));
+  const uint64_t serial0 = g.states[i].shot_block;
+  double *out = b.rates + gp;
+  for (int c = cb; c < ce; ++c) {
+    const double f = frac[c];
+    a = knot_bracket(v, size, f, a);
+    const double w = (f - fabs(v[a].x)) / (fabs(v[a + 1].x) - fabs(v[a].x));
+    double light = v[a].y + (v[a + 1].y - v[a].y) * w;
+    if (e < ec && b.ecell[eo + e] == c) light = b.erad[eo + e++];
+    light = fmax(0.0, light);
+    if (!isfinite(light)) atomicOr(status, 2);
+    out[(size_t)(c - c0) * pixels] =
+        cell_rate(light, g.photo, g.t1[c] - g.t0[c], g.seed, (uint64_t)i, serial0 + (uint64_t)(c - c0));
+  }
 }
 
 // Circuit of cells [c0, c1) of every pixel of an accepted window: the
@@ -603,9 +609,8 @@ hipError_t launch_adapt_gate(const SegDev *gate_segs, const int *gate_seg, int n
 hipError_t launch_adapt_rates(const SegDev *segs, const AdaptDev *ad, const int *pixel_begin, int nseg, int pixels,
                               AdaptBufs b, int c0, int c1, int *status, hipStream_t st) {
   if (pixels <= 0 || c1 <= c0) return hipSuccess;
-  if (c1 - c0 > 65535) return hipErrorInvalidConfiguration;
-  adapt_rates<<<dim3(grid(pixels), (unsigned)(c1 - c0)), 256, 0, st>>>(segs, ad, pixel_begin, nseg, pixels, b, c0,
-                                                                      status);
+  const unsigned runs = (unsigned)((c1 - c0 + kRateCells - 1) / kRateCells);
+  adapt_rates<<<dim3(grid(pixels), runs), 256, 0, st>>>(segs, ad, pixel_begin, nseg, pixels, b, c0, c1, status);
   return hipGetLastError();
 }
 hipError_t launch_adapt_integrate(const SegDev *segs, const AdaptDev *ad, const int *block_begin, int nseg,
