@@ -133,7 +133,8 @@ class _StatsC(ctypes.Structure):
                 ("event_rays", ctypes.c_uint64), ("pixel_cells", ctypes.c_uint64),
                 ("comparator_events", ctypes.c_uint64), ("delivered_events", ctypes.c_uint64),
                 ("native_equivalent_rays", ctypes.c_uint64), ("adaptive_windows", ctypes.c_uint64),
-                ("rejected_windows", ctypes.c_uint64)]
+                ("rejected_windows", ctypes.c_uint64), ("rates_ms", ctypes.c_double),
+                ("circuit_launches", ctypes.c_double)]
 
 
 class _RefineC(ctypes.Structure):

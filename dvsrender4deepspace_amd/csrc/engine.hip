@@ -570,6 +570,7 @@ class Engine {
     stats.circuit_ms += circuit_ms;
     stats.total_ms += step_ms + events_ms;
     stats.trace_launches += chunks;
+    stats.circuit_launches += chunks;
     for (auto &w : work) {
       const int ss = w.seg->scene->view.samples;
       stats.primary_rays += (uint64_t)w.seg->npix * w.cells * ss * ss;
@@ -958,6 +959,7 @@ class Engine {
     // Integration, with the fallback pixels' native radiance traced in cell
     // chunks (sample_cells / integrate_fallback_tile).
     hip_check(hipEventRecord(circuit_t.start, st), "record");
+    std::vector<std::unique_ptr<Timer>> integrate_t;
     int max_cells = 0;
     for (auto &w : work) max_cells = std::max(max_cells, w.cells);
     // Photocurrent of every pixel for a chunk of cells: knot pixels in
@@ -1010,18 +1012,25 @@ class Engine {
       }
       hip_check(launch_adapt_rates(segtab_.ptr, adapt_.ptr, pbegin_.ptr, nseg, P, b, c0, c1, status_.ptr, st),
                 "adaptive photocurrent launch");
+      integrate_t.emplace_back(new Timer());
+      hip_check(hipEventRecord(integrate_t.back()->start, st), "record");
       hip_check(launch_adapt_integrate(segtab_.ptr, adapt_.ptr, cblock_.ptr, nseg, circuit_blocks_, P, b, c0, c1,
                                        status_.ptr, st),
                 "adaptive circuit launch");
+      hip_check(hipEventRecord(integrate_t.back()->stop, st), "record");
       if (fb_total > 0) hip_check(hipStreamSynchronize(st), "exact fallback circuit tile");  // tables are reused
     }
     hip_check(hipEventRecord(circuit_t.stop, st), "record");
     hip_check(hipEventRecord(step_t.stop, st), "record");
     const double events_ms = collect(work, pixel_total, lds, false, accepted);
     // Statistics.
-    const double refine_ms = refine_t.ms(), circuit_ms = circuit_t.ms(), step_ms = step_t.ms();
+    const double refine_ms = refine_t.ms(), photo_circuit_ms = circuit_t.ms(), step_ms = step_t.ms();
+    double circuit_ms = 0;
+    for (auto &t : integrate_t) circuit_ms += t->ms();
     stats.trace_ms += refine_ms;
     stats.circuit_ms += circuit_ms;
+    stats.rates_ms += photo_circuit_ms - circuit_ms;
+    stats.circuit_launches += (double)integrate_t.size();
     stats.total_ms += step_ms + events_ms;
     uint64_t samples_traced = gate_samples + fallback_samples;
     for (int i = 0; i < nseg; ++i) {

@@ -91,7 +91,9 @@ typedef struct {
 #pragma pack(pop)
 
 /* PixelState (include/dvs/pixel_model.hpp:11-15) as held on the device,
- * plus the per-pixel Philox block counters of the shot and reset streams. */
+ * plus the noise counters: reset_block is the reset stream's next Philox
+ * block, shot_block the number of circuit cells integrated so far (the
+ * serial that keys each cell's shot-noise draw). */
 typedef struct {
   double receptor_log, reference_log, refractory_until;
   double threshold_pos, threshold_neg;
@@ -109,6 +111,9 @@ typedef struct {
   /* rays the native path would trace for the same accepted windows, and the
    * adaptive windows accepted / rejected by the convergence gate */
   uint64_t native_equivalent_rays, adaptive_windows, rejected_windows;
+  /* adaptive steps: photocurrent kernels (knot interpolation, fallback
+   * tiles, shot noise) apart from the serial circuit in circuit_ms */
+  double rates_ms, circuit_launches;
 } dvs_stats;
 
 /* OpticalRefinementStats (include/dvs/optical_sampler.hpp:16) of the last
