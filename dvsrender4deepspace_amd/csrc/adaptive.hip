@@ -255,6 +255,9 @@ __global__ void adapt_exact_count(int pixels, const SegDev *segs, const AdaptDev
   const AdaptDev &A = ad[si];
   const int cells = segs[si].cells, size = b.kcount[p];
   const double2 *v = b.knots + (b.koff[p] - size);
+  bool any = false;  // only pixels with an exact segment have exact cells
+  for (int k = 1; k < size && !any; ++k) any = v[k].x < 0;
+  if (!any) return;
   const double *frac = b.cell_frac + A.cell_offset;
   int a = 0, used = 0;
   for (int c = 0; c < cells; ++c) {
@@ -459,8 +462,13 @@ __global__ __launch_bounds__(256) void adapt_integrate(const SegDev *segs, const
   const double *rates = fallback ? A.fb_rad + b.fb_index[gp] : b.rates + gp;
   const size_t stride = fallback ? (size_t)A.fb_count : (size_t)pixels;
   const int c_end = min(c1, g.cells);
+  // The next cell's photocurrent is loaded before this cell is integrated, so
+  // its HBM latency overlaps the FP64 chain (one or two waves per SIMD cannot
+  // hide it, and the crossing stores keep the compiler from hoisting it).
+  double next = c0 < c_end ? rates[0] : 0.0;
   for (int c = c0; c < c_end; ++c) {
-    const double rate = rates[(size_t)(c - c0) * stride];
+    const double rate = next;
+    if (c + 1 < c_end) next = rates[(size_t)(c + 1 - c0) * stride];
     if (!isfinite(rate)) { atomicOr(status, 2); g.counts[i] = 0; return; }
     const int emitted = advance_pixel(regs, p, rate, g.t0[c], g.t1[c], out + count, g.max_events - count, normal);
     if (emitted < 0) { atomicOr(status, 1); g.counts[i] = 0; return; }

@@ -213,9 +213,11 @@ __global__ __launch_bounds__(256) void circuit_multi(const SegDev *segs, const i
   CrossingEvent *out = g.slots + (size_t)i * g.max_events;
   const CircuitParams p = g.params;
   int count = g.counts[i];  // crossings of earlier cell chunks of this step
+  double next = g.cells > 0 ? g.rad[i] : 0.0;  // photocurrent of the next cell, loaded one cell ahead
   for (int c = 0; c < g.cells; ++c) {
     const double t0 = g.t0[c], t1 = g.t1[c];
-    const double rate = g.rad[(size_t)c * g.npix + i];  // photocurrent from the trace
+    const double rate = next;
+    if (c + 1 < g.cells) next = g.rad[(size_t)(c + 1) * g.npix + i];
     if (!isfinite(rate)) { atomicOr(status, 2); g.counts[i] = 0; return; }
     const int emitted = advance_pixel(regs, p, rate, t0, t1, out + count, g.max_events - count, normal);
     if (emitted < 0) { atomicOr(status, 1); g.counts[i] = 0; return; }
