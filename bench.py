@@ -98,6 +98,17 @@ def _same_streams(a, b) -> bool:
     return all(x.event_array().tobytes() == y.event_array().tobytes() for x, y in zip(a, b))
 
 
+def _tune(serve, geometries, reqs, adaptive: bool) -> dict:
+    """Native steps: the autotuned (horizons per step, graph replay).  Adaptive
+    segments step by the driver's windows, so there is nothing to tune."""
+    if adaptive:
+        return {"horizons_per_step": 64, "use_graph": False, "note": "adaptive windows of the driver policy"}
+    tune = serve.autotune(geometries, reqs[: min(6, len(reqs))],
+                          candidates=((16, False), (32, False), (32, True), (64, False)))
+    log(f"autotune: {tune}")
+    return tune["best"]
+
+
 def served_mode(args) -> dict:
     """Served form: Poisson request arrivals over two resident terrains,
     continuous batching, HBM-footprint admission; against the plain
@@ -113,12 +124,11 @@ def served_mode(args) -> dict:
     t = time.perf_counter()
     meshes = serve.terrain_meshes(specs)
     log(f"meshes in {time.perf_counter() - t:.1f} s")
+    adaptive = args.solver == "adaptive"
     reqs = serve.request_mix(base, args.requests, args.rate, (args.min_horizons, args.max_horizons), specs,
-                             seed=args.seed, speed=tuple(args.speed))
+                             seed=args.seed, speed=tuple(args.speed), adaptive=adaptive)
     geometries = {k: serve.build_scene(meshes[k], base, base.camera) for k in specs}
-    tune = serve.autotune(geometries, reqs[: min(6, len(reqs))], candidates=((16, False), (32, False), (32, True), (64, False)))
-    best = tune["best"]
-    log(f"autotune: {tune}")
+    best = _tune(serve, geometries, reqs, adaptive)
     served = [dataclasses.replace(r, events=[]) for r in reqs]
     server = serve.Server(geometries, budget_bytes=args.budget_gb * 1e9,
                           horizons_per_step=best["horizons_per_step"], use_graph=best["use_graph"])
@@ -138,7 +148,7 @@ def served_mode(args) -> dict:
         "config": {"workload": "served mix", "requests": len(reqs), "arrival_rate_hz": args.rate,
                    "horizons": [args.min_horizons, args.max_horizons], "resolutions": serve.RESOLUTIONS,
                    "terrains": {k: v.triangles for k, v in specs.items()}, "admission_budget_gb": args.budget_gb,
-                   "supersample": base.camera.samples, "autotuned": best},
+                   "supersample": base.camera.samples, "solver": args.solver, "autotuned": best},
         "events": events,
         "served": {"wall_s": round(wall_served, 3), "events_per_s": round(events / wall_served, 1),
                    "latency_ms": _latency(served), "steps": server.steps, "peak_active": server.peak_active},
@@ -164,12 +174,11 @@ def batched_mode(args) -> dict:
     base = workloads.DAVIS346
     specs = {"terrain1024": base.terrain}
     meshes = serve.terrain_meshes(specs)
+    adaptive = args.solver == "adaptive"
     reqs = serve.request_mix(base, args.segments, None, (args.min_horizons, args.max_horizons), specs,
-                             seed=args.seed, speed=tuple(args.speed))
+                             seed=args.seed, speed=tuple(args.speed), adaptive=adaptive)
     geometries = {"terrain1024": serve.build_scene(meshes["terrain1024"], base, base.camera)}
-    tune = serve.autotune(geometries, reqs[: min(6, len(reqs))], candidates=((16, False), (32, False), (32, True), (64, False)))
-    best = tune["best"]
-    log(f"autotune: {tune}")
+    best = _tune(serve, geometries, reqs, adaptive)
     batched = [dataclasses.replace(r, events=[]) for r in reqs]
     server = serve.Server(geometries, horizons_per_step=best["horizons_per_step"], use_graph=best["use_graph"])
     wall_batched = server.run(batched, realtime=False)
@@ -183,7 +192,7 @@ def batched_mode(args) -> dict:
         "higher_is_better": True, "data": "synthetic", "mode": "batched",
         "config": {"workload": "64 uneven segments", "segments": len(reqs), "resolutions": serve.RESOLUTIONS,
                    "horizons": [args.min_horizons, args.max_horizons], "terrain_triangles": base.terrain.triangles,
-                   "autotuned": best},
+                   "solver": args.solver, "autotuned": best},
         "events": events, "batched_wall_s": round(wall_batched, 3), "one_by_one_wall_s": round(wall_alone, 3),
         "batched_mrays_per_s": round(rays / wall_batched / 1e6, 1),
         "speedup_vs_one_by_one": round(wall_alone / wall_batched, 2),
@@ -199,13 +208,14 @@ def main() -> int:
     ap.add_argument("--requests", type=int, default=60)
     ap.add_argument("--segments", type=int, default=64)
     ap.add_argument("--rate", type=float, default=20.0)
-    ap.add_argument("--min-horizons", type=int, default=50)
-    ap.add_argument("--max-horizons", type=int, default=500)
+    ap.add_argument("--min-horizons", type=int, default=2000)
+    ap.add_argument("--max-horizons", type=int, default=20000)
+    ap.add_argument("--solver", choices=["adaptive", "native"], default="adaptive",
+                    help="event solver of the served/batched requests")
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--seed", type=int, default=2026)
-    ap.add_argument("--speed", type=float, nargs=2, default=(1000.0, 2000.0),
-                    help="ground speed range (m/s) of the served/batched mix: a low orbital pass, so that "
-                         "windows of a few hundred microseconds move the image by pixels")
+    ap.add_argument("--speed", type=float, nargs=2, default=(100.0, 400.0),
+                    help="ground speed range (m/s) of the served/batched mix")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40, help="timed driver windows (request mode)")
     ap.add_argument("--warmup", type=int, default=2)

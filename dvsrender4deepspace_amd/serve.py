@@ -10,10 +10,13 @@ view of its terrain.  Render (frame) jobs run through the same loop.
 
 ``run_plain`` is the plain per-request path of the reference's driver
 (apps/main_dvs.cpp:60-129): per request the scene is built (SAH BVH) and
-uploaded, a sensor is created and initialised, horizons go through
-process_many calls of the reference's native batch size and the readout is
-drained.  ``scenes_resident=True`` keeps the scenes uploaded between requests
-(isolating the launch-sharing gain from the setup gain).
+uploaded, a sensor is created and initialised, the horizons go through the
+driver loop (``driver.drive``: native calls of 64 horizons, or adaptive
+windows with the receptor check) and the readout is drained.
+``scenes_resident=True`` keeps the scenes uploaded between requests
+(isolating the launch-sharing gain from the setup gain).  Served requests
+of the adaptive solver run under the same window policy inside the engine,
+so both paths give every request the same events.
 
 ``autotune`` picks the engine's horizons per step and graph replay by timing
 short probes of the actual segment mix.
@@ -27,10 +30,8 @@ from collections import deque
 
 import numpy as np
 
-from . import capi, terrain
+from . import capi, driver, terrain
 from .workloads import Workload
-
-NATIVE_BATCH = 64  # horizons per process_many call (apps/main_dvs.cpp:96-98)
 
 
 @dataclasses.dataclass
@@ -45,6 +46,7 @@ class Request:
     workload: Workload
     horizons: int = 0
     frames: int = 0
+    adaptive: bool = False
     # filled by the runners
     events: list = dataclasses.field(default_factory=list)
     images: object = None
@@ -67,9 +69,11 @@ def footprint_bytes(req: Request, horizons_per_step: int) -> int:
     if req.frames:
         return npix * req.frames * (4 + 4 + 8) + (1 << 20)
     w = req.workload
-    cells = int(np.ceil(horizons_per_step * w.time_resolution / w.physic.circuit_step))
+    step = driver.batch_size(w.physic, w.time_resolution, True) if req.adaptive else horizons_per_step
+    cells = int(np.ceil(step * w.time_resolution / w.physic.circuit_step))
     slots = npix * w.physic.max_events_per_step
-    return npix * (56 + 8 + cells * 8) + slots * (16 + 64) + (1 << 20)
+    knots = 64 * 16 if req.adaptive else 0  # typical refinement knots per pixel
+    return npix * (56 + 8 + cells * 8 + knots) + slots * (16 + 64) + (1 << 20)
 
 
 class Server:
@@ -116,6 +120,9 @@ class Server:
                 continue
             w = req.workload
             seg = self.batch.add(view, w.physic, w.initial_state(), w.start_time)
+            if req.adaptive:
+                self.batch.set_adaptive(seg, driver.DEFAULT_TOLERANCE,
+                                        driver.batch_size(w.physic, w.time_resolution, True), driver.CHECK_PERIOD)
             self.batch.feed(seg, w.intervals(0, req.horizons))
             self.active[seg] = (req, need)
             self.used += need
@@ -193,9 +200,8 @@ def run_plain(requests: list, meshes: dict, realtime: bool = True, scenes_reside
         else:
             sensor = capi.Sensor(scene, w.physic)
             sensor.initialize(w.initial_state(), w.start_time)
-            for k in range(0, req.horizons, NATIVE_BATCH):
-                req.events.append(sensor.process(w.intervals(k, min(NATIVE_BATCH, req.horizons - k))))
-            req.events.append(sensor.drain())
+            batch = driver.batch_size(w.physic, w.time_resolution, req.adaptive)
+            req.events.extend(driver.drive(sensor, w.intervals(0, req.horizons), adaptive=req.adaptive, batch=batch))
             sensor.close()
         scene.close()
         req.finished = clock()
@@ -230,11 +236,13 @@ RESOLUTIONS = ((64, 48), (173, 130), (346, 260))
 
 
 def request_mix(base: Workload, count: int, rate_hz: float | None, horizons: tuple, terrains: dict, seed: int,
-                noise_fraction: float = 0.5, render_fraction: float = 0.0, speed: tuple = (20.0, 80.0)) -> list:
+                noise_fraction: float = 0.5, render_fraction: float = 0.0, speed: tuple = (20.0, 80.0),
+                adaptive: bool = False) -> list:
     """Seeded request mix: resolution from RESOLUTIONS (fx scaled with width),
     length U(horizons), terrain uniform over `terrains`, per-request seed and
     noise flag, ground speed U(speed) and yaw rate U(-5, 5) deg/s seeded;
-    Poisson arrivals at rate_hz (all at t=0 when None)."""
+    Poisson arrivals at rate_hz (all at t=0 when None); event jobs use the
+    adaptive solver when `adaptive`."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
@@ -263,7 +271,8 @@ def request_mix(base: Workload, count: int, rate_hz: float | None, horizons: tup
                                 frames=n_f if render else 2, fps=fps,
                                 speed=float(rng.uniform(*speed)), yaw_rate_deg=float(rng.uniform(-5, 5)),
                                 altitude=float(min(base.altitude, 0.8 * half * f / wpx * 2)))
-        out.append(Request(id=i, arrival=t, terrain=key, camera=cam, workload=w, horizons=n_h, frames=n_f))
+        out.append(Request(id=i, arrival=t, terrain=key, camera=cam, workload=w, horizons=n_h, frames=n_f,
+                           adaptive=adaptive and not render))
     return out
 
 

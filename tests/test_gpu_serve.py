@@ -12,10 +12,11 @@ from dvsrender4deepspace_amd import capi, serve, terrain, workloads
 pytestmark = pytest.mark.gpu
 
 
-def _setup(count=10, render_fraction=0.2):
+def _setup(count=10, render_fraction=0.2, adaptive=False, horizons=(3, 12)):
     base = dataclasses.replace(workloads.DAVIS346, camera=dataclasses.replace(workloads.DAVIS346.camera, samples=2))
     specs = {"a": terrain.TerrainSpec(size=96, craters=4, seed=1), "b": terrain.TerrainSpec(size=128, craters=6, seed=2)}
-    reqs = serve.request_mix(base, count, 200.0, (3, 12), specs, seed=5, render_fraction=render_fraction)
+    reqs = serve.request_mix(base, count, 200.0, horizons, specs, seed=5, render_fraction=render_fraction,
+                             adaptive=adaptive)
     # shrink resolutions for a quick test; keep the mix of three sizes
     small = {64: (32, 24), 173: (40, 30), 346: (48, 36)}
     for r in reqs:
@@ -31,8 +32,11 @@ def _setup(count=10, render_fraction=0.2):
     return reqs, meshes, geometries
 
 
-def test_served_equals_plain_per_request(gpu):
-    reqs, meshes, geometries = _setup()
+@pytest.mark.parametrize("adaptive", [False, True])
+def test_served_equals_plain_per_request(gpu, adaptive):
+    """Every request's events are the same served (continuous batching, the
+    engine's per-segment solver policy) as run alone through the driver loop."""
+    reqs, meshes, geometries = _setup(adaptive=adaptive, horizons=(40, 260) if adaptive else (3, 12))
     served = [dataclasses.replace(r, events=[]) for r in reqs]
     # a budget that admits about two requests at a time forces queueing
     budget = 2.5 * max(serve.footprint_bytes(r, 4) for r in reqs)
