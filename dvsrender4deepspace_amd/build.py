@@ -34,6 +34,10 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
 HIP_SOURCES = ["dvs_kernels.hip", "adaptive.hip", "dvs_capi.hip", "engine.hip", "events_sort.hip"]
 HOST_SOURCES = ["bvh_host.cpp"]
+# Host I/O library (event writers, configuration): HDF5 1.10 C library of the
+# image's conda prefix, searched after the system headers.
+IO_LIBRARY = LIBDIR / "libdvs_io.so"
+HDF5_PREFIX = Path("/opt/conda")
 
 
 def _hipcc() -> str:
@@ -73,6 +77,19 @@ def build_library(jobs: int = 4) -> Path:
     return LIBRARY
 
 
+def build_io() -> Path:
+    """Compiles libdvs_io.so (event writers and configuration, host C++)."""
+    LIBDIR.mkdir(exist_ok=True)
+    if not (HDF5_PREFIX / "include" / "hdf5.h").exists():
+        raise RuntimeError(f"HDF5 headers not found under {HDF5_PREFIX}; events.h5 cannot be built")
+    tmp = IO_LIBRARY.with_suffix(".so.tmp")
+    _run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17", f"-I{ROOT / 'include'}",
+          f"-idirafter{HDF5_PREFIX / 'include'}", str(CSRC / "io_host.cpp"), "-o", str(tmp),
+          f"-L{HDF5_PREFIX / 'lib'}", "-lhdf5", f"-Wl,-rpath,{HDF5_PREFIX / 'lib'}"])
+    os.replace(tmp, IO_LIBRARY)
+    return IO_LIBRARY
+
+
 def build_oracle() -> Path:
     """Builds the CPU oracle (test infrastructure) with its Makefile."""
     _run(["make", "-s", "-C", str(ROOT / "oracle")])
@@ -81,5 +98,6 @@ def build_oracle() -> Path:
 
 if __name__ == "__main__":
     print(build_library(), file=sys.stderr)
+    print(build_io(), file=sys.stderr)
     if "--with-oracle" in sys.argv:
         print(build_oracle(), file=sys.stderr)

@@ -235,11 +235,11 @@ def load_library() -> ctypes.CDLL:
     return lib
 
 
-def exported_symbols() -> list[str]:
-    """Names of every C entry point declared in include/dvs_hip.h."""
-    header = (Path(__file__).resolve().parent.parent / "include" / "dvs_hip.h").read_text()
+def exported_symbols(header: str = "dvs_hip.h") -> list[str]:
+    """Names of every C entry point declared in include/<header>."""
+    text = (Path(__file__).resolve().parent.parent / "include" / header).read_text()
     import re
-    return sorted(set(re.findall(r"\b(dvs_[a-z0-9_]+)\s*\(", header)))
+    return sorted(set(re.findall(r"\b(dvs_[a-z0-9_]+)\s*\(", text)))
 
 
 def _check(code: int) -> None:

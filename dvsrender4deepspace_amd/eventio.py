@@ -1,0 +1,169 @@
+"""Event stream files and the YAML configuration of bin/dvs, over
+libdvs_io.so (include/dvs_io.h).
+
+EventWriter writes events.bin (13-byte records) or events.h5 (x, y, t in
+microseconds, p and ms_to_idx), as BinaryEventStreamWriter and
+Hdf5EventStreamWriter do (include/dvs/dvs_writer.hpp).  load_config reads a
+configuration file with Config's grammar and checks (src/scene_types.cpp).
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+from pathlib import Path
+
+import numpy as np
+
+from . import capi
+
+LIBRARY_PATH = Path(__file__).resolve().parent / "lib" / "libdvs_io.so"
+BIN_RECORD_DTYPE = np.dtype([("t", "<f4"), ("x", "<i2"), ("y", "<i2"), ("p", "i1"), ("depth", "<f4")])
+assert BIN_RECORD_DTYPE.itemsize == 13
+FORMATS = {"bin": 0, "hdf5": 1, "h5": 1, "binary": 0}
+
+_lib = None
+_P, _I32, _I64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+
+
+class _SceneC(ctypes.Structure):
+    _fields_ = [("frame_count", ctypes.c_int32), ("fps", ctypes.c_float), ("imu_fps", ctypes.c_float),
+                ("cam_fps", ctypes.c_float), ("time_resolution", ctypes.c_float),
+                ("blur_resolution", ctypes.c_float), ("supersample_factor", ctypes.c_int32),
+                ("translation", ctypes.c_float * 3), ("rotation", ctypes.c_float * 3)]
+
+
+class _ConfigC(ctypes.Structure):
+    _fields_ = [("camera", capi._CameraC), ("physic", capi._PhysicC), ("scene", _SceneC), ("hapke", capi._HapkeC)]
+
+
+@dataclasses.dataclass(frozen=True)
+class SceneConfig:
+    """SceneConfig (include/dvs/scene_types.hpp:80-86)."""
+    frame_count: int
+    fps: float
+    imu_fps: float
+    cam_fps: float
+    time_resolution: float
+    blur_resolution: float
+    supersample_factor: int
+    translation: tuple
+    rotation: tuple
+
+
+@dataclasses.dataclass(frozen=True)
+class Config:
+    camera: capi.Camera
+    physic: capi.Physic
+    scene: SceneConfig
+    hapke: capi.Hapke
+
+
+class IoError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+
+
+class ConfigError(ValueError):
+    pass
+
+
+def load_library() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIBRARY_PATH.exists():
+        raise RuntimeError(f"I/O library not built: {LIBRARY_PATH} (run __graft_entry__.build())")
+    lib = ctypes.CDLL(str(LIBRARY_PATH))
+    sig = {
+        "dvs_io_last_error": (ctypes.c_char_p, []),
+        "dvs_writer_open": (ctypes.c_int, [ctypes.c_char_p, _I32, ctypes.POINTER(_P)]),
+        "dvs_writer_append": (ctypes.c_int, [_P, _P, _I64]),
+        "dvs_writer_close": (ctypes.c_int, [_P, ctypes.c_double]),
+        "dvs_writer_count": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),
+        "dvs_writer_destroy": (ctypes.c_int, [_P]),
+        "dvs_events_h5_read": (ctypes.c_int, [ctypes.c_char_p, _P, _P, _P, _P, _I64, ctypes.POINTER(_I64)]),
+        "dvs_events_h5_ms_to_idx": (ctypes.c_int, [ctypes.c_char_p, _P, _I64, ctypes.POINTER(_I64)]),
+        "dvs_config_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_ConfigC)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(code: int) -> None:
+    if code == capi.DVS_OK:
+        return
+    message = load_library().dvs_io_last_error().decode()
+    if code == capi.DVS_E_ARGUMENT:
+        raise ConfigError(message)
+    raise IoError(code, message)
+
+
+class EventWriter:
+    """EventStreamWriter: append(events) in time order, close(end_time)."""
+
+    def __init__(self, path, fmt: str = "bin"):
+        if fmt not in FORMATS:
+            raise IoError(capi.DVS_E_STATE, f"unsupported event format: {fmt}")
+        handle = _P()
+        _check(load_library().dvs_writer_open(str(path).encode(), FORMATS[fmt], ctypes.byref(handle)))
+        self.handle = handle
+        self.path = Path(path)
+
+    def append(self, events: np.ndarray) -> None:
+        ev = np.ascontiguousarray(np.atleast_1d(events), dtype=capi.EVENT_DTYPE)
+        _check(load_library().dvs_writer_append(self.handle, ev.ctypes.data, len(ev)))
+
+    def close(self, end_time_seconds: float) -> None:
+        _check(load_library().dvs_writer_close(self.handle, float(end_time_seconds)))
+
+    @property
+    def count(self) -> int:
+        n = _I64()
+        _check(load_library().dvs_writer_count(self.handle, ctypes.byref(n)))
+        return n.value
+
+    def release(self):
+        if getattr(self, "handle", None):
+            load_library().dvs_writer_destroy(self.handle)
+            self.handle = None
+
+    __del__ = release
+
+
+def read_bin(path) -> np.ndarray:
+    return np.fromfile(path, dtype=BIN_RECORD_DTYPE)
+
+
+def read_h5(path) -> dict:
+    lib = load_library()
+    n = _I64()
+    _check(lib.dvs_events_h5_read(str(path).encode(), None, None, None, None, 0, ctypes.byref(n)))
+    out = {"x": np.empty(n.value, np.int16), "y": np.empty(n.value, np.int16), "t": np.empty(n.value, np.int64),
+           "p": np.empty(n.value, np.int8)}
+    _check(lib.dvs_events_h5_read(str(path).encode(), out["x"].ctypes.data, out["y"].ctypes.data,
+                                  out["t"].ctypes.data, out["p"].ctypes.data, n.value, ctypes.byref(n)))
+    m = _I64()
+    _check(lib.dvs_events_h5_ms_to_idx(str(path).encode(), None, 0, ctypes.byref(m)))
+    idx = np.empty(m.value, np.int64)
+    _check(lib.dvs_events_h5_ms_to_idx(str(path).encode(), idx.ctypes.data, m.value, ctypes.byref(m)))
+    out["ms_to_idx"] = idx
+    return out
+
+
+def load_config(path) -> Config:
+    """Config(path): raises ConfigError with the reference's message."""
+    c = _ConfigC()
+    _check(load_library().dvs_config_load(str(path).encode(), ctypes.byref(c)))
+    fields = lambda s: {name: getattr(s, name) for name, _ in s._fields_}
+    cam = fields(c.camera)
+    phys = fields(c.physic)
+    scene = fields(c.scene)
+    scene["translation"] = tuple(c.scene.translation)
+    scene["rotation"] = tuple(c.scene.rotation)
+    return Config(camera=capi.Camera(**cam), physic=capi.Physic(**phys), scene=SceneConfig(**scene),
+                  hapke=capi.Hapke(**fields(c.hapke)))

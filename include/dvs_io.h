@@ -1,0 +1,71 @@
+/*
+ * Host I/O of the DVS path: the event stream writers of bin/dvs and the YAML
+ * configuration.  C ABI of libdvs_io.so (host code, HDF5 1.10 C library);
+ * the status codes and thread-local message follow dvs_hip.h.
+ */
+#ifndef DVS_IO_H
+#define DVS_IO_H
+#include <stdint.h>
+
+#include "dvs_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Message of the calling thread's last I/O failure ("" if none). */
+const char *dvs_io_last_error(void);
+
+/* ---- Event stream writers (include/dvs/dvs_writer.hpp) -----------------
+ * DVS_EVENTS_BIN:  BinaryEventStreamWriter (:22-58), packed 13-byte records
+ *                  {float t, int16 x, int16 y, int8 p, float depth}.
+ * DVS_EVENTS_HDF5: Hdf5EventStreamWriter (:60-264), extendable datasets x, y
+ *                  (int16), t (int64 microseconds, llround(t * 1e6)), p (int8)
+ *                  in chunks of 65,536, and ms_to_idx (int64) written at close:
+ *                  entry m is the index of the first event at or after m ms.
+ *                  Appends must be globally time ordered (each batch is
+ *                  stably sorted by time first). */
+enum { DVS_EVENTS_BIN = 0, DVS_EVENTS_HDF5 = 1 };
+typedef struct dvs_writer_s *dvs_writer;
+int dvs_writer_open(const char *path, int32_t format, dvs_writer *out);
+int dvs_writer_append(dvs_writer writer, const dvs_event *events, int64_t count);
+/* close(end_time_seconds) (:127-143); closing twice is a no-op. */
+int dvs_writer_close(dvs_writer writer, double end_time_seconds);
+int dvs_writer_count(dvs_writer writer, int64_t *count);
+int dvs_writer_destroy(dvs_writer writer);
+
+/* Reads an events.h5 file back: x, y, t (microseconds), p into caller
+ * buffers of `capacity` entries (any may be NULL); *count = events. */
+int dvs_events_h5_read(const char *path, int16_t *x, int16_t *y, int64_t *t, int8_t *p, int64_t capacity,
+                       int64_t *count);
+/* Reads the ms_to_idx dataset of an events.h5 file. */
+int dvs_events_h5_ms_to_idx(const char *path, int64_t *out, int64_t capacity, int64_t *count);
+
+/* ---- Configuration (Config, include/dvs/scene_types.hpp:64-109,
+ * src/scene_types.cpp) -------------------------------------------------- */
+/* SceneConfig (scene_types.hpp:80-86). */
+typedef struct {
+  int32_t frame_count;
+  float fps, imu_fps, cam_fps, time_resolution, blur_resolution;
+  int32_t supersample_factor;
+  float translation[3], rotation[3];
+} dvs_scene_config;
+
+typedef struct {
+  dvs_camera camera; /* samples = scene.supersample_factor */
+  dvs_physic physic;
+  dvs_scene_config scene;
+  dvs_hapke hapke;
+} dvs_config;
+
+/* Config::Config(path, err): two-level "section: / key: value" YAML,
+ * sections intrinsic, physic, scene (required) and hapke; required keys,
+ * optional physic keys over the PhysicConfig defaults, and the reference's
+ * range checks.  Failures return DVS_E_ARGUMENT with the reference's
+ * message. */
+int dvs_config_load(const char *path, dvs_config *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -22,6 +22,15 @@ def test_library_exports_every_declared_symbol():
     assert not missing, f"declared in include/dvs_hip.h but not exported: {missing}"
 
 
+def test_io_library_exports_every_declared_symbol():
+    from dvsrender4deepspace_amd import eventio
+    lib = eventio.load_library()
+    names = capi.exported_symbols("dvs_io.h")
+    assert len(names) >= 9
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, f"declared in include/dvs_io.h but not exported: {missing}"
+
+
 def test_layouts_match_oracle(oracle):
     assert capi.NODE_DTYPE == oracle.NODE
     assert capi.STATE_DTYPE == oracle.STATE
