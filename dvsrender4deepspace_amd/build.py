@@ -90,6 +90,23 @@ def build_io() -> Path:
     return IO_LIBRARY
 
 
+BINDIR = PACKAGE / "bin"
+API_SELFTEST = BINDIR / "dvs_api_selftest"
+
+
+def build_api_selftest() -> Path:
+    """Links the C++ API self-test (include/dvs_render_amd.hpp) against both
+    libraries; run by the tests."""
+    BINDIR.mkdir(exist_ok=True)
+    rocm_lib = Path("/opt/rocm/lib")
+    tmp = API_SELFTEST.with_suffix(".tmp")
+    _run(["g++", "-O2", "-std=c++17", f"-I{ROOT / 'include'}", str(CSRC / "api_selftest.cpp"), "-o", str(tmp),
+          f"-L{LIBDIR}", "-ldvs_hip", "-ldvs_io", f"-Wl,-rpath,$ORIGIN/../lib",
+          f"-Wl,-rpath-link,{rocm_lib}", f"-Wl,-rpath-link,{HDF5_PREFIX / 'lib'}", f"-Wl,-rpath,{rocm_lib}"])
+    os.replace(tmp, API_SELFTEST)
+    return API_SELFTEST
+
+
 def build_oracle() -> Path:
     """Builds the CPU oracle (test infrastructure) with its Makefile."""
     _run(["make", "-s", "-C", str(ROOT / "oracle")])
@@ -99,5 +116,6 @@ def build_oracle() -> Path:
 if __name__ == "__main__":
     print(build_library(), file=sys.stderr)
     print(build_io(), file=sys.stderr)
+    print(build_api_selftest(), file=sys.stderr)
     if "--with-oracle" in sys.argv:
         print(build_oracle(), file=sys.stderr)

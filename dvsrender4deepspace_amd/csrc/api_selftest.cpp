@@ -1,0 +1,168 @@
+// Exercises the C++ API (include/dvs_render_amd.hpp) the way the
+// reference's callers use it.
+//   dvs_api_selftest --cpu <config.yaml> <out_dir>
+//       Config, BVHBuilder, writers and pose conversion (no GPU).
+//   dvs_api_selftest --gpu <out_dir>
+//       DeviceScene, CamRayCaster and DvsRayCaster on the plane fixture of
+//       tests/test_sensor.cu:66-80 (48x36 grid): frame depth, native and
+//       adaptive events written to events.bin / events.h5 for comparison.
+// Prints one line per check and exits non-zero on the first failure.
+#include <cstdio>
+#include <iostream>
+#include <stdexcept>
+#include <string>
+
+#include "dvs_render_amd.hpp"
+
+namespace {
+void expect(bool ok, const char *what) {
+  if (!ok) throw std::runtime_error(std::string("check failed: ") + what);
+  std::cout << "ok " << what << "\n";
+}
+
+dvs::SceneBundle<dvs::float3, dvs::uint3> plane() {
+  dvs::SceneBundle<dvs::float3, dvs::uint3> b;
+  b.vertices = {{-100, -100, 0}, {100, -100, 0}, {100, 100, 0}, {-100, 100, 0}};
+  b.normals.assign(4, dvs::float3{0, 0, 1});
+  b.bvh_triangles = {{0, 1, 2}, {0, 2, 3}};
+  return b;
+}
+
+int cpu(const std::string &config_path, const std::string &out) {
+  std::string err;
+  dvs::Config config(config_path, err);
+  expect(err.empty() && config.camera.width == 346 && config.scene.supersample_factor == 16, "Config loads");
+  std::string bad;
+  dvs::Config missing(out + "/no-such-file.yaml", bad);
+  expect(bad.rfind("Cannot open file", 0) == 0, "Config reports a missing file");
+  auto b = plane();
+  dvs::BVHBuildParams params;
+  const int root = dvs::BVHBuilder::build(params, b.bvh_nodes, b.bvh_triangles, b.vertices);
+  expect(root == (int)b.bvh_nodes.size() - 1 && b.bvh_nodes[root].count == 2, "BVHBuilder builds one leaf");
+  dvs::BVHBuilder::validate(b.bvh_nodes, root);
+  bool threw = false;
+  try {
+    dvs::BVHBuilder::validate(b.bvh_nodes, 7);
+  } catch (const std::invalid_argument &) {
+    threw = true;
+  } catch (const std::runtime_error &) {
+    threw = true;
+  }
+  expect(threw, "BVHBuilder::validate rejects a bad root");
+  float q[4];
+  const float identity[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  dvs::CamRayCaster::quaternion_of(identity, q);
+  expect(q[0] == 0 && q[1] == 0 && q[2] == 0 && q[3] == 1, "identity matrix gives the unit quaternion");
+  const float half_turn_z[9] = {-1, 0, 0, 0, -1, 0, 0, 0, 1};  // trace -1: largest-diagonal branch
+  dvs::CamRayCaster::quaternion_of(half_turn_z, q);
+  expect(std::fabs(std::fabs(q[2]) - 1) < 1e-6f && std::fabs(q[3]) < 1e-6f, "half turn about z");
+  {
+    auto w = dvs::make_event_writer("bin", out);
+    w->append({dvs::Event{.001234, 2, 3, 1, 10}, dvs::Event{.0021, 4, 5, 0, 20}});
+    w->close(.003);
+    expect(w->event_count() == 2, "binary writer counts events");
+  }
+  {
+    auto w = dvs::make_event_writer("hdf5", out);
+    w->append({dvs::Event{.001234, 2, 3, 1, 10}, dvs::Event{.0021, 4, 5, 0, 20}});
+    bool disorder = false;
+    try {
+      w->append({dvs::Event{.001, 1, 1, 1, 0}});
+    } catch (const std::runtime_error &) {
+      disorder = true;
+    }
+    expect(disorder, "HDF5 writer rejects out-of-order batches");
+    w->close(.003);
+  }
+  threw = false;
+  try {
+    dvs::make_event_writer("tiff", out);
+  } catch (const std::runtime_error &) {
+    threw = true;
+  }
+  expect(threw, "make_event_writer rejects unknown formats");
+  return 0;
+}
+
+int gpu(const std::string &out) {
+  auto bundle = plane();
+  dvs::BVHBuildParams params;
+  const int root = dvs::BVHBuilder::build(params, bundle.bvh_nodes, bundle.bvh_triangles, bundle.vertices);
+  std::string err;
+  // The fixture's optics and circuit (48x36 grid of the DAVIS field of view,
+  // one sub-ray; thresholds 0.28, 64 events per pixel, no FIFO delay).
+  dvs::Config config(std::string(), err);
+  const double scale = 48 / 346.0;  // intrinsics scaled in double, rounded once
+  config.camera = {48, 36, (float)(226.3802 * scale), (float)(226.1500 * scale), (float)(173.6471 * scale),
+                   (float)(133.7327 * scale)};
+  config.scene.supersample_factor = 1;
+  config.hapke = dvs::HapkeConfig{};
+  config.physic.threshold_pos = config.physic.threshold_neg = .28f;
+  config.physic.threshold_std = 0;
+  config.physic.photon_shot_noise = 0;
+  config.physic.max_events_per_step = 64;
+  config.physic.readout_bandwidth = 0;
+  config.physic.readout_latency = 0;
+  config.physic.random_seed = 42;
+  auto scene = std::make_shared<dvs::DeviceScene>(bundle, config, root);
+  dvs::CamRayCaster cam(scene);
+  const float identity[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  std::vector<float> image, depth;
+  cam.render_main(dvs::make_float3(0, 0, 10), identity, dvs::make_float3(0, 0, 1), 1000, image, depth);
+  const int centre = 18 * 48 + 24;
+  expect(std::fabs(depth[centre] - 10) < 0.05f && image[centre] > 0, "nadir frame: depth 10 m at the centre");
+
+  dvs::CameraTrajectoryState s;
+  s.position.v[2] = 10;
+  s.light_intensity = 1000;
+  dvs::DvsRayCaster native(scene, config), adaptive(scene, config);
+  native.initialize(s, 0.0);
+  adaptive.initialize(s, 0.0);
+  const dvs::OpticalState pose = dvs::to_optical_state(s);
+  dvs::OpticalState start = pose, end = pose;
+  start.intensity_rate = 8e6f;
+  end.intensity_rate = -8e6f;
+  std::vector<dvs::Event> ev_native, ev_adaptive, tail;
+  native.advance_many({{pose, pose, 0, .002}}, ev_native);
+  expect(ev_native.empty(), "static scene: no startup burst");
+  native.advance_many({{start, end, .002, .004}}, ev_native);
+  expect(!ev_native.empty(), "Hermite pulse emits events");
+  bool ok = adaptive.advance_adaptive({{pose, pose, 0, .002}}, 1e-4, ev_adaptive, true);
+  expect(ok && ev_adaptive.empty(), "adaptive static window accepted by the receptor check");
+  ok = adaptive.advance_adaptive({{start, end, .002, .004}}, 1e-4, ev_adaptive);
+  expect(ok && !ev_adaptive.empty(), "adaptive pulse window emits events");
+  const auto stats = adaptive.refinement_statistics();
+  expect(stats.traced_pixel_samples >= 5u * 48 * 36 && adaptive.adaptive_cache_capacity() >= 128,
+         "refinement statistics");
+  native.drain(tail);
+  expect(native.dropped_events() == 0, "no FIFO drops");
+  bool threw = false;
+  try {
+    native.advance_many({{pose, pose, .01, .011}}, tail);  // not contiguous with .004
+  } catch (const std::invalid_argument &e) {
+    threw = std::string(e.what()).find("contiguous") != std::string::npos;
+  }
+  expect(threw, "non-contiguous intervals throw std::invalid_argument");
+  dvs::BinaryEventStreamWriter bin(out + "/native.bin");
+  bin.append(ev_native);
+  bin.close(.004);
+  dvs::Hdf5EventStreamWriter h5(out + "/adaptive.h5");
+  h5.append(ev_adaptive);
+  h5.close(.004);
+  std::cout << "events native " << ev_native.size() << " adaptive " << ev_adaptive.size() << "\n";
+  return 0;
+}
+}  // namespace
+
+int main(int argc, char **argv) {
+  try {
+    const std::string mode = argc > 1 ? argv[1] : "";
+    if (mode == "--cpu" && argc == 4) return cpu(argv[2], argv[3]);
+    if (mode == "--gpu" && argc == 3) return gpu(argv[2]);
+    std::cerr << "usage: dvs_api_selftest --cpu <config.yaml> <out_dir> | --gpu <out_dir>\n";
+    return 2;
+  } catch (const std::exception &e) {
+    std::cerr << "FAIL: " << e.what() << "\n";
+    return 1;
+  }
+}

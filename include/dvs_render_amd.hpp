@@ -1,0 +1,455 @@
+// C++17 API of the DVS path on MI355X: the classes of the reference's
+// include/dvs_render.hpp (DeviceScene, DvsRayCaster, CamRayCaster,
+// BVHBuilder, Config, the event stream writers) with the same names,
+// signatures and error behaviour, implemented over the C ABIs of
+// libdvs_hip.so (dvs_hip.h) and libdvs_io.so (dvs_io.h).  Types are re-homed
+// off CUDA's vector types and Eigen: float3/float4/uint3 here are plain
+// structs of the same layout, and any trajectory state with Eigen-style
+// accessors (x(), y(), z(), w()) converts through to_optical_state.
+//
+// Errors: an invalid argument throws std::invalid_argument, every other
+// failure std::runtime_error, with the reference's message text.
+#pragma once
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "dvs_hip.h"
+#include "dvs_io.h"
+
+namespace dvs {
+
+struct float3 { float x = 0, y = 0, z = 0; };
+struct float4 { float x = 0, y = 0, z = 0, w = 0; };
+struct uint3 { uint32_t x = 0, y = 0, z = 0; };
+inline float3 make_float3(float x, float y, float z) { return {x, y, z}; }
+inline float4 make_float4(float x, float y, float z, float w) { return {x, y, z, w}; }
+
+using Event = dvs_event;  // packed 17 bytes (include/dvs/dvs_base.hpp:9-18)
+
+// OpticalState / OpticalInterval (include/dvs/optical_sensor.hpp:15-20).
+struct OpticalState {
+  float3 position, velocity, light, light_rate;
+  float4 rotation;
+  float intensity = 0, intensity_rate = 0;
+};
+struct OpticalInterval {
+  OpticalState start, end;
+  double t0 = 0, t1 = 0;
+};
+static_assert(sizeof(OpticalState) == sizeof(dvs_optical_state), "OpticalState layout");
+static_assert(sizeof(OpticalInterval) == sizeof(dvs_optical_interval), "OpticalInterval layout");
+
+// BVHNode<float3> (include/dvs/scene_types.hpp:29-35): 40 bytes.
+template <class F3> struct AABB { F3 min, max; };
+template <class F3> struct BVHNode {
+  AABB<F3> box;
+  int left = -1, right = -1, start = 0, count = 0;
+};
+static_assert(sizeof(BVHNode<float3>) == sizeof(dvs_bvh_node), "BVHNode layout");
+
+// Configuration (include/dvs/scene_types.hpp:59-109), PhysicConfig and
+// HapkeConfig with the reference's defaults.
+struct CameraConfig {
+  int width = 0, height = 0;
+  float fx = 0, fy = 0, cx = 0, cy = 0;
+};
+struct PhysicConfig {
+  float threshold_pos = .3f, threshold_neg = .3f, refractory_time = .0002f;
+  float readout_bandwidth = 5e6f, readout_latency = 0;
+  float threshold_std = 0.03f, threshold_min = 0.01f;
+  float wavelength_nm = 550, quantum_efficiency = .3f, pixel_pitch_um = 15;
+  float fill_factor = .2f, f_number = 2, lens_transmission = .8f;
+  float radiance_scale = .001f;
+  float temperature_k = 300, subthreshold_kappa = .8f;
+  float receptor_capacitance_f = 1e-15f;
+  float reset_capacitance_f = 20e-15f, dark_current_a = 1e-18f;
+  float reset_leakage_a = 0, circuit_step = 1e-6f;
+  int photon_shot_noise = 1, reset_thermal_noise = 0;
+  int random_seed = 114514, max_events_per_step = 16;
+  int readout_queue_capacity = 1000000;
+};
+static_assert(sizeof(PhysicConfig) == sizeof(dvs_physic), "PhysicConfig layout");
+struct SceneConfig {
+  int frame_count = 0;
+  float fps = 0, imu_fps = 0, cam_fps = 0;
+  float time_resolution = 0, blur_resolution = 0;
+  int supersample_factor = 1;
+  std::vector<float> translation{0, 0, 0}, rotation{0, 0, 0};
+};
+struct HapkeConfig {
+  float b = 0.24f, c = 0.305f, hS = 0.08f, hC = 1.0f;
+  float thetap = 26.0f * 3.14159265358979323846f / 180.0f;
+  float w = 0.324f, B_S0 = 2.42f, B_C0 = 0.0f, phi = 0.3f;
+};
+static_assert(sizeof(HapkeConfig) == sizeof(dvs_hapke), "HapkeConfig layout");
+
+namespace detail {
+[[noreturn]] inline void raise(int code, const char *message) {
+  if (code == DVS_E_ARGUMENT) throw std::invalid_argument(message);
+  throw std::runtime_error(message);
+}
+inline void check(int code) {
+  if (code != DVS_OK) raise(code, dvs_last_error());
+}
+inline void check_io(int code) {
+  if (code != DVS_OK) raise(code, dvs_io_last_error());
+}
+inline dvs_physic to_c(const PhysicConfig &p) {
+  dvs_physic c;
+  static_assert(sizeof c == sizeof p, "layout");
+  std::memcpy(&c, &p, sizeof c);
+  return c;
+}
+inline dvs_hapke to_c(const HapkeConfig &h) {
+  dvs_hapke c;
+  std::memcpy(&c, &h, sizeof c);
+  return c;
+}
+}  // namespace detail
+
+class Config {
+ public:
+  CameraConfig camera;
+  PhysicConfig physic;
+  SceneConfig scene;
+  HapkeConfig hapke;
+  // Config(path, err) (src/scene_types.cpp): on a malformed file err holds
+  // the message; an unparsable optional physic value throws
+  // std::invalid_argument, as in the reference.
+  Config(const std::string &config_file, std::string &err) {
+    dvs_config c{};
+    const int code = dvs_config_load(config_file.c_str(), &c);
+    if (code != DVS_OK) {
+      err = dvs_io_last_error();
+      if (err.rfind("invalid physic parameter", 0) == 0) throw std::invalid_argument(err);
+      return;
+    }
+    camera = {c.camera.width, c.camera.height, c.camera.fx, c.camera.fy, c.camera.cx, c.camera.cy};
+    std::memcpy(&physic, &c.physic, sizeof physic);
+    std::memcpy(&hapke, &c.hapke, sizeof hapke);
+    scene.frame_count = c.scene.frame_count;
+    scene.fps = c.scene.fps;
+    scene.imu_fps = c.scene.imu_fps;
+    scene.cam_fps = c.scene.cam_fps;
+    scene.time_resolution = c.scene.time_resolution;
+    scene.blur_resolution = c.scene.blur_resolution;
+    scene.supersample_factor = c.scene.supersample_factor;
+    scene.translation.assign(c.scene.translation, c.scene.translation + 3);
+    scene.rotation.assign(c.scene.rotation, c.scene.rotation + 3);
+  }
+};
+
+template <class F3, class U3> struct SceneBundle {
+  std::vector<BVHNode<F3>> bvh_nodes;
+  std::vector<F3> vertices, normals;
+  std::vector<U3> bvh_triangles;
+  std::vector<float> albedos;
+  bool use_albedo = false;
+};
+
+// BVHBuilder (include/bvh/build.h:8-40).
+struct BVHBuildParams {
+  int max_leaf_size = 8;
+  float sah_node_cost = 1.0f;
+  float sah_primitive_cost = 1.0f;
+};
+class BVHBuilder {
+ public:
+  // Indexed build: nodes are replaced, indices reordered to leaf order;
+  // returns the root (the last node).
+  static int build(const BVHBuildParams &params, std::vector<BVHNode<float3>> &out_nodes,
+                   std::vector<uint3> &indices, std::vector<float3> &vertices) {
+    out_nodes.assign(std::max<size_t>(2 * indices.size(), 1), BVHNode<float3>{});
+    int64_t count = 0;
+    int32_t root = -1;
+    detail::check(dvs_bvh_build(params.max_leaf_size, params.sah_node_cost, params.sah_primitive_cost,
+                                reinterpret_cast<const float *>(vertices.data()), (int64_t)vertices.size(),
+                                reinterpret_cast<uint32_t *>(indices.data()), (int64_t)indices.size(),
+                                reinterpret_cast<dvs_bvh_node *>(out_nodes.data()), &count, &root));
+    out_nodes.resize((size_t)count);
+    return root;
+  }
+  static void validate(const std::vector<BVHNode<float3>> &nodes, int root) {
+    detail::check(dvs_bvh_validate(reinterpret_cast<const dvs_bvh_node *>(nodes.data()), (int64_t)nodes.size(),
+                                   root));
+  }
+};
+
+// DeviceScene (include/dvs/device_scene.hpp:5-15): one owner of the
+// uploaded BVH and mesh, shared by the DVS and frame casters.
+class DeviceScene {
+ public:
+  DeviceScene(const SceneBundle<float3, uint3> &mesh, const Config &config, int root) {
+    dvs_camera cam{config.camera.width, config.camera.height, config.camera.fx, config.camera.fy,
+                   config.camera.cx,    config.camera.cy,     config.scene.supersample_factor};
+    const dvs_hapke hk = detail::to_c(config.hapke);
+    if (mesh.use_albedo && mesh.albedos.size() != mesh.vertices.size())
+      throw std::invalid_argument("invalid indexed mesh/BVH/material dimensions");
+    detail::check(dvs_scene_create(reinterpret_cast<const dvs_bvh_node *>(mesh.bvh_nodes.data()),
+                                   (int64_t)mesh.bvh_nodes.size(), root,
+                                   reinterpret_cast<const float *>(mesh.vertices.data()),
+                                   reinterpret_cast<const float *>(mesh.normals.data()),
+                                   (int64_t)mesh.vertices.size(),
+                                   reinterpret_cast<const uint32_t *>(mesh.bvh_triangles.data()),
+                                   (int64_t)mesh.bvh_triangles.size(),
+                                   mesh.use_albedo ? mesh.albedos.data() : nullptr, &cam, &hk, &scene_));
+    width_ = cam.width;
+    height_ = cam.height;
+  }
+  ~DeviceScene() { dvs_scene_destroy(scene_); }
+  DeviceScene(const DeviceScene &) = delete;
+  DeviceScene &operator=(const DeviceScene &) = delete;
+  dvs_scene view() const { return scene_; }
+  int width() const { return width_; }
+  int height() const { return height_; }
+
+ private:
+  dvs_scene scene_ = nullptr;
+  int width_ = 0, height_ = 0;
+};
+
+// CameraTrajectoryState with Eigen-style accessors (include/dvs/trajectory.hpp:59-70).
+struct Vec3 {
+  float v[3] = {0, 0, 0};
+  float x() const { return v[0]; }
+  float y() const { return v[1]; }
+  float z() const { return v[2]; }
+};
+struct Quaternion {
+  float q[4] = {0, 0, 0, 1};  // x, y, z, w
+  float x() const { return q[0]; }
+  float y() const { return q[1]; }
+  float z() const { return q[2]; }
+  float w() const { return q[3]; }
+};
+struct CameraTrajectoryState {
+  float timestamp = 0;
+  Vec3 position;
+  Quaternion quaternion;
+  Vec3 linear_velocity, angular_velocity, light_direction{{0, 0, 1}}, light_direction_rate;
+  float light_intensity = 0, light_intensity_rate = 0;
+};
+
+// to_optical_state (include/dvs/raycast.hpp:17-22) for any state type with
+// these members and Eigen-style accessors.
+template <class State>
+OpticalState to_optical_state(const State &s) {
+  OpticalState p{};
+  p.position = {s.position.x(), s.position.y(), s.position.z()};
+  p.velocity = {s.linear_velocity.x(), s.linear_velocity.y(), s.linear_velocity.z()};
+  p.light = {s.light_direction.x(), s.light_direction.y(), s.light_direction.z()};
+  p.light_rate = {s.light_direction_rate.x(), s.light_direction_rate.y(), s.light_direction_rate.z()};
+  p.intensity = s.light_intensity;
+  p.intensity_rate = s.light_intensity_rate;
+  p.rotation = {s.quaternion.x(), s.quaternion.y(), s.quaternion.z(), s.quaternion.w()};
+  return p;
+}
+
+// OpticalRefinementStats / ReceptorProbeResult of the adaptive solver.
+using OpticalRefinementStats = dvs_refinement_stats;
+using ReceptorProbeResult = dvs_receptor_probe;
+
+// DvsRayCaster (include/dvs/raycast.hpp:23-62).
+class DvsRayCaster {
+ public:
+  DvsRayCaster(const SceneBundle<float3, uint3> &scene, const Config &config, int root_index)
+      : DvsRayCaster(std::make_shared<DeviceScene>(scene, config, root_index), config) {}
+  DvsRayCaster(std::shared_ptr<DeviceScene> scene, const Config &config) : scene_(std::move(scene)) {
+    const dvs_physic p = detail::to_c(config.physic);
+    detail::check(dvs_sensor_create(scene_->view(), &p, 1, 1, &sensor_));
+  }
+  ~DvsRayCaster() { dvs_sensor_destroy(sensor_); }
+  DvsRayCaster(const DvsRayCaster &) = delete;
+  DvsRayCaster &operator=(const DvsRayCaster &) = delete;
+
+  template <class State>
+  void initialize(const State &state, double timestamp) {
+    const OpticalState o = to_optical_state(state);
+    detail::check(dvs_sensor_initialize(sensor_, reinterpret_cast<const dvs_optical_state *>(&o), timestamp));
+  }
+  template <class State>
+  void advance(const State &start, const State &end, double t0, double t1, std::vector<Event> &events) {
+    advance_many({OpticalInterval{to_optical_state(start), to_optical_state(end), t0, t1}}, events);
+  }
+  // events receives the readout-delivered events of this call.
+  void advance_many(const std::vector<OpticalInterval> &intervals, std::vector<Event> &events) {
+    int64_t ready = 0;
+    detail::check(dvs_sensor_process(sensor_, reinterpret_cast<const dvs_optical_interval *>(intervals.data()),
+                                     (int32_t)intervals.size(), &ready));
+    take(events, ready);
+  }
+  int adaptive_cache_capacity() {
+    int32_t c = 0;
+    detail::check(dvs_sensor_adaptive_capacity(sensor_, &c));
+    return c;
+  }
+  // false: the receptor check rejected the window; nothing changed and
+  // events is empty.
+  bool advance_adaptive(const std::vector<OpticalInterval> &intervals, double tolerance, std::vector<Event> &events,
+                        bool check_convergence = false) {
+    int32_t accepted = 0;
+    int64_t ready = 0;
+    detail::check(dvs_sensor_process_adaptive(sensor_,
+                                              reinterpret_cast<const dvs_optical_interval *>(intervals.data()),
+                                              (int32_t)intervals.size(), tolerance, check_convergence ? 1 : 0,
+                                              &accepted, &ready));
+    take(events, ready);
+    return accepted != 0;
+  }
+  ReceptorProbeResult last_convergence() const {
+    ReceptorProbeResult r{};
+    detail::check(dvs_sensor_last_convergence(sensor_, &r));
+    return r;
+  }
+  OpticalRefinementStats refinement_statistics() const {
+    OpticalRefinementStats r{};
+    detail::check(dvs_sensor_refinement_stats(sensor_, &r));
+    return r;
+  }
+  void drain(std::vector<Event> &events) {
+    int64_t ready = 0;
+    detail::check(dvs_sensor_drain(sensor_, &ready));
+    take(events, ready);
+  }
+  std::size_t dropped_events() const {
+    int64_t dropped = 0, peak = 0;
+    detail::check(dvs_sensor_readout_stats(sensor_, &dropped, &peak));
+    return (std::size_t)dropped;
+  }
+  std::size_t peak_queue() const {
+    int64_t dropped = 0, peak = 0;
+    detail::check(dvs_sensor_readout_stats(sensor_, &dropped, &peak));
+    return (std::size_t)peak;
+  }
+
+ private:
+  void take(std::vector<Event> &events, int64_t ready) {
+    events.resize((size_t)ready);
+    int64_t got = 0;
+    detail::check(dvs_sensor_take(sensor_, events.data(), ready, &got));
+    events.resize((size_t)got);
+  }
+  std::shared_ptr<DeviceScene> scene_;
+  dvs_sensor sensor_ = nullptr;
+};
+
+// CamRayCaster (include/dvs/raycast.hpp:64-86).
+class CamRayCaster {
+ public:
+  CamRayCaster(const SceneBundle<float3, uint3> &scene, const Config &config, const int root_index)
+      : CamRayCaster(std::make_shared<DeviceScene>(scene, config, root_index)) {}
+  explicit CamRayCaster(std::shared_ptr<DeviceScene> scene) : scene_(std::move(scene)) {}
+
+  // camera_world: camera-to-world rotation, 3x3 column major.  The pose's
+  // quaternion is the matrix's, by the trace / largest-diagonal method
+  // (Shepperd 1978), normalised.
+  void render_main(const float3 &camera_origin, const float *camera_world, const float3 &light_dir,
+                   const float light_intensity, std::vector<float> &light_frame, std::vector<float> &depth_map) {
+    dvs_optical_state pose{};
+    pose.position[0] = camera_origin.x;
+    pose.position[1] = camera_origin.y;
+    pose.position[2] = camera_origin.z;
+    pose.light[0] = light_dir.x;
+    pose.light[1] = light_dir.y;
+    pose.light[2] = light_dir.z;
+    pose.intensity = light_intensity;
+    quaternion_of(camera_world, pose.rotation);
+    const size_t n = (size_t)scene_->width() * scene_->height();
+    light_frame.resize(n);
+    depth_map.resize(n);
+    detail::check(dvs_frame_render(scene_->view(), &pose, 1, light_frame.data(), depth_map.data(), nullptr));
+  }
+  void render_main(const float3 &camera_origin, const float *camera_world, const float3 &light_dir,
+                   const float light_intensity, std::vector<float> &light_frame) {
+    std::vector<float> depth;
+    render_main(camera_origin, camera_world, light_dir, light_intensity, light_frame, depth);
+  }
+
+  static void quaternion_of(const float *m, float q[4]) {  // m column major: m[c*3+r]
+    auto at = [m](int r, int c) { return m[c * 3 + r]; };
+    float x, y, z, w;
+    const float trace = at(0, 0) + at(1, 1) + at(2, 2);
+    if (trace > 0) {
+      float t = std::sqrt(trace + 1.0f);
+      w = 0.5f * t;
+      t = 0.5f / t;
+      x = (at(2, 1) - at(1, 2)) * t;
+      y = (at(0, 2) - at(2, 0)) * t;
+      z = (at(1, 0) - at(0, 1)) * t;
+    } else {
+      int i = 0;
+      if (at(1, 1) > at(0, 0)) i = 1;
+      if (at(2, 2) > at(i, i)) i = 2;
+      const int j = (i + 1) % 3, k = (j + 1) % 3;
+      float v[3];
+      float t = std::sqrt(at(i, i) - at(j, j) - at(k, k) + 1.0f);
+      v[i] = 0.5f * t;
+      t = 0.5f / t;
+      w = (at(k, j) - at(j, k)) * t;
+      v[j] = (at(j, i) + at(i, j)) * t;
+      v[k] = (at(k, i) + at(i, k)) * t;
+      x = v[0];
+      y = v[1];
+      z = v[2];
+    }
+    const float norm = std::sqrt(x * x + y * y + z * z + w * w);
+    q[0] = x / norm;
+    q[1] = y / norm;
+    q[2] = z / norm;
+    q[3] = w / norm;
+  }
+
+ private:
+  std::shared_ptr<DeviceScene> scene_;
+};
+
+// EventStreamWriter / BinaryEventStreamWriter / Hdf5EventStreamWriter
+// (include/dvs/dvs_writer.hpp) and make_event_writer (apps/main_dvs.cpp:23-34).
+class EventStreamWriter {
+ public:
+  EventStreamWriter(std::string output_path, int32_t format) : path_(std::move(output_path)) {
+    detail::check_io(dvs_writer_open(path_.c_str(), format, &writer_));
+  }
+  virtual ~EventStreamWriter() { dvs_writer_destroy(writer_); }
+  EventStreamWriter(const EventStreamWriter &) = delete;
+  EventStreamWriter &operator=(const EventStreamWriter &) = delete;
+  void append(std::vector<Event> events) {
+    detail::check_io(dvs_writer_append(writer_, events.data(), (int64_t)events.size()));
+  }
+  void close(double end_time_seconds) { detail::check_io(dvs_writer_close(writer_, end_time_seconds)); }
+  const std::string &path() const { return path_; }
+  std::size_t event_count() const {
+    int64_t n = 0;
+    detail::check_io(dvs_writer_count(writer_, &n));
+    return (std::size_t)n;
+  }
+
+ private:
+  std::string path_;
+  dvs_writer writer_ = nullptr;
+};
+class BinaryEventStreamWriter final : public EventStreamWriter {
+ public:
+  explicit BinaryEventStreamWriter(std::string output_path)
+      : EventStreamWriter(std::move(output_path), DVS_EVENTS_BIN) {}
+};
+class Hdf5EventStreamWriter final : public EventStreamWriter {
+ public:
+  explicit Hdf5EventStreamWriter(std::string output_path)
+      : EventStreamWriter(std::move(output_path), DVS_EVENTS_HDF5) {}
+};
+inline std::unique_ptr<EventStreamWriter> make_event_writer(const std::string &event_format,
+                                                            const std::string &output_dir) {
+  if (event_format == "hdf5") return std::make_unique<Hdf5EventStreamWriter>(output_dir + "/events.h5");
+  if (event_format == "bin") return std::make_unique<BinaryEventStreamWriter>(output_dir + "/events.bin");
+  throw std::runtime_error("unsupported event format: " + event_format);
+}
+
+}  // namespace dvs
