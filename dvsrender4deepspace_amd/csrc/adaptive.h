@@ -29,6 +29,7 @@ enum { LAYOUT_PROBE = 0, LAYOUT_QUERY = 1, LAYOUT_CELL = 2 };
 // Window of one segment in an adaptive step.
 struct AdaptDev {
   double start, end, tolerance;
+  unsigned long long cell_base;  // circuit cells the segment integrated before this window
   int capacity;          // knot and exact-cell capacity per pixel
   int accepted;          // 0 once the convergence gate rejected the window
   int cell_offset;       // first cell of the segment in the step's cell tables

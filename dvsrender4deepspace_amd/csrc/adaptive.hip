@@ -423,7 +423,9 @@ __global__ __launch_bounds__(256) void adapt_rates(const SegDev *segs, const Ada
     const int m = (e + hi) >> 1;
     if (b.ecell[eo + m] < cb) e = m + 1; else hi = m;
   }
-  const uint64_t serial0 = g.states[i].shot_block;
+  // cell serials from the segment's count, not the pixel state: the circuit
+  // of the previous chunk may be updating the state while this runs
+  const uint64_t serial0 = A.cell_base;
   double *out = b.rates + gp;
   for (int c = cb; c < ce; ++c) {
     const double f = frac[c];
@@ -434,7 +436,7 @@ __global__ __launch_bounds__(256) void adapt_rates(const SegDev *segs, const Ada
     light = fmax(0.0, light);
     if (!isfinite(light)) atomicOr(status, 2);
     out[(size_t)(c - c0) * pixels] =
-        cell_rate(light, g.photo, g.t1[c] - g.t0[c], g.seed, (uint64_t)i, serial0 + (uint64_t)(c - c0));
+        cell_rate(light, g.photo, g.t1[c] - g.t0[c], g.seed, (uint64_t)i, serial0 + (uint64_t)c);
   }
 }
 

@@ -111,6 +111,7 @@ struct Segment {
   // segment (window of `abatch` horizons, halved on a rejected check; a
   // window of one horizon is then integrated natively; the receptor check
   // runs at the first window and every check_period seconds after).
+  uint64_t cells_done = 0;  // circuit cells integrated since initialisation (every pixel alike)
   bool adaptive = false;
   double tolerance = 1e-2, check_period = .25, next_check = 0;
   int abatch = 1, capacity = 0;
@@ -140,10 +141,18 @@ class Engine {
  public:
   explicit Engine(int horizons_per_step, bool use_graph) : horizons_(horizons_per_step), use_graph_(use_graph) {
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "engine stream");
+    // second stream (default priority) for the photocurrent of the next cell
+    // chunk while the serial circuit integrates the current one
+    hip_check(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking), "engine stream");
+    for (hipEvent_t *e : {&ev_start_, &ev_rates_[0], &ev_rates_[1], &ev_int_[0], &ev_int_[1]})
+      hip_check(hipEventCreateWithFlags(e, hipEventDisableTiming), "engine events");
   }
   ~Engine() {
     segments_.clear();
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    for (hipEvent_t e : {ev_start_, ev_rates_[0], ev_rates_[1], ev_int_[0], ev_int_[1]})
+      if (e) (void)hipEventDestroy(e);
+    if (side_) (void)hipStreamDestroy(side_);
     if (stream_) (void)hipStreamDestroy(stream_);
   }
 
@@ -684,6 +693,7 @@ class Engine {
       s.readout.submit_sorted(s.raw.data(), s.raw.size(), work[i].ivs.back().t1, s.ready);
       stats.delivered_events += s.ready.size() - before;
       stats.pixel_cells += (uint64_t)s.npix * work[i].cells;
+      s.cells_done += (uint64_t)work[i].cells;
       s.last_time = work[i].ivs.back().t1;
       at = end;
     }
@@ -730,6 +740,7 @@ class Engine {
       a.start = w.ivs.front().t0;
       a.end = w.ivs.back().t1;
       a.tolerance = w.tolerance;
+      a.cell_base = s.cells_done;
       a.capacity = s.capacity;
       a.accepted = 1;
       a.cell_offset = w.cell_offset;
@@ -964,10 +975,16 @@ class Engine {
     for (auto &w : work) max_cells = std::max(max_cells, w.cells);
     // Photocurrent of every pixel for a chunk of cells: knot pixels in
     // rates_ [cell][P], fallback pixels in radiance_ [cell][fallback list].
-    const int chunk = (int)std::max<size_t>(
-        1, std::min<size_t>({(size_t)max_cells, (size_t)65535,
-                             radiance_budget_ / (sizeof(double) * ((size_t)P + (size_t)fb_total))}));
-    rates_.reserve((size_t)chunk * P, "adaptive photocurrent");
+    // Without fallback pixels the photocurrent of chunk k+1 is computed on
+    // the side stream while the circuit integrates chunk k (two rate slots).
+    const bool pipelined = fb_total == 0;
+    const size_t slots_needed = pipelined ? 2 : 1;
+    size_t chunk_cap = std::min<size_t>((size_t)max_cells, 65535);
+    chunk_cap = std::min<size_t>(chunk_cap, radiance_budget_ / (sizeof(double) * slots_needed *
+                                                                  ((size_t)P + (size_t)fb_total)));
+    if (pipelined && max_cells >= 256) chunk_cap = std::min<size_t>(chunk_cap, (size_t)(max_cells + 3) / 4);
+    const int chunk = (int)std::max<size_t>(1, chunk_cap);
+    rates_.reserve((size_t)chunk * P * slots_needed, "adaptive photocurrent");
     b.rates = rates_.ptr;
     uint64_t fallback_samples = 0;
     std::vector<long long> fblocks(nseg + 1);
@@ -988,7 +1005,29 @@ class Engine {
       }
       adapt_.upload(ad.data(), ad.size(), "adaptive windows", st);
     }
-    for (int c0 = 0; c0 < max_cells; c0 += chunk) {
+    if (pipelined) {
+      hip_check(hipEventRecord(ev_start_, st), "record");
+      int k = 0;
+      for (int c0 = 0; c0 < max_cells; c0 += chunk, ++k) {
+        const int c1 = std::min(max_cells, c0 + chunk), slot = k & 1;
+        AdaptBufs bk = b;
+        bk.rates = rates_.ptr + (size_t)slot * chunk * P;
+        if (k == 0) hip_check(hipStreamWaitEvent(side_, ev_start_, 0), "stream wait");
+        if (k >= 2) hip_check(hipStreamWaitEvent(side_, ev_int_[slot], 0), "stream wait");  // slot free again
+        hip_check(launch_adapt_rates(segtab_.ptr, adapt_.ptr, pbegin_.ptr, nseg, P, bk, c0, c1, status_.ptr, side_),
+                  "adaptive photocurrent launch");
+        hip_check(hipEventRecord(ev_rates_[slot], side_), "record");
+        hip_check(hipStreamWaitEvent(st, ev_rates_[slot], 0), "stream wait");
+        integrate_t.emplace_back(new Timer());
+        hip_check(hipEventRecord(integrate_t.back()->start, st), "record");
+        hip_check(launch_adapt_integrate(segtab_.ptr, adapt_.ptr, cblock_.ptr, nseg, circuit_blocks_, P, bk, c0, c1,
+                                         status_.ptr, st),
+                  "adaptive circuit launch");
+        hip_check(hipEventRecord(integrate_t.back()->stop, st), "record");
+        hip_check(hipEventRecord(ev_int_[slot], st), "record");
+      }
+    }
+    for (int c0 = 0; !pipelined && c0 < max_cells; c0 += chunk) {
       const int c1 = std::min(max_cells, c0 + chunk);
       if (fb_total > 0) {
         fblocks[0] = 0;
@@ -1053,7 +1092,8 @@ class Engine {
 
   int horizons_;
   bool use_graph_;
-  hipStream_t stream_ = nullptr;
+  hipStream_t stream_ = nullptr, side_ = nullptr;
+  hipEvent_t ev_start_ = nullptr, ev_rates_[2] = {nullptr, nullptr}, ev_int_[2] = {nullptr, nullptr};
   std::map<int, std::unique_ptr<Segment>> segments_;
   int next_id_ = 0;
   size_t radiance_budget_ = size_t(4) << 30;
