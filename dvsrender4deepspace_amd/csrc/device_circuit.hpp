@@ -129,31 +129,49 @@ __device__ inline int advance_pixel(PixelRegs &s, const CircuitParams &p, double
   if (t1 <= t0) return 0;
   int count = 0;
   double cursor = t0, u = s.receptor_log;
+  // Each pass advances one piece: the rest of a refractory hold, or the
+  // free-running stretch up to the cell end (or the leakage extremum).  The
+  // receptor solution is evaluated at one place per pass so that lanes of a
+  // wave in different phases share it; the operations on each lane are those
+  // of the reference's sequence (a hold ending inside a cell continues with
+  // the free-running stretch in the next pass).
   while (cursor < t1) {
-    if (cursor < s.refractory_until) {
-      const double end = fmin(t1, s.refractory_until);
-      u = receptor_log_at(u, rate, end - cursor, p.charge_scale);
+    const bool hold = cursor < s.refractory_until;
+    if (!hold && s.reset_pending) {
+      if (p.reset_noise_std > 0) s.reference_log += p.reset_noise_std * normal();
+      s.reset_pending = 0;
+    }
+    double end = 0, c0 = 0, boundary = 0, dt;
+    if (hold) {
+      end = fmin(t1, s.refractory_until);
+      dt = end - cursor;
+    } else {
+      const double remaining = t1 - cursor;
+      c0 = u - s.reference_log;
+      boundary = remaining;
+      if (p.reference_leak > 0) {
+        const double x = exp(u), target = rate + p.charge_scale * p.reference_leak;
+        if (x > target && target > rate) {
+          double extremum;
+          if (rate > 0) extremum = -p.charge_scale / rate * log((1.0 / target - 1.0 / rate) / (1.0 / x - 1.0 / rate));
+          else extremum = p.charge_scale * (1.0 / target - 1.0 / x);
+          if (extremum > 1e-15 && extremum < boundary) boundary = extremum;
+        }
+      }
+      dt = boundary;
+    }
+    const double un = receptor_log_at(u, rate, dt, p.charge_scale);
+    if (hold) {
+      u = un;
       cursor = end;
       s.reference_log = u;
       if ((cursor < t1 || cursor == s.refractory_until) && cursor >= s.refractory_until && s.reset_pending) {
         if (p.reset_noise_std > 0) s.reference_log += p.reset_noise_std * normal();
         s.reset_pending = 0;
       }
-      if (cursor >= t1) break;
-    } else if (s.reset_pending) {
-      if (p.reset_noise_std > 0) s.reference_log += p.reset_noise_std * normal();
-      s.reset_pending = 0;
+      continue;
     }
-    const double remaining = t1 - cursor, c0 = u - s.reference_log;
-    double boundary = remaining;
-    const double x = exp(u), target = rate + p.charge_scale * p.reference_leak;
-    if (p.reference_leak > 0 && x > target && target > rate) {
-      double extremum;
-      if (rate > 0) extremum = -p.charge_scale / rate * log((1.0 / target - 1.0 / rate) / (1.0 / x - 1.0 / rate));
-      else extremum = p.charge_scale * (1.0 / target - 1.0 / x);
-      if (extremum > 1e-15 && extremum < boundary) boundary = extremum;
-    }
-    const double ub = receptor_log_at(u, rate, boundary, p.charge_scale);
+    const double ub = un;
     const double cb = ub - s.reference_log + p.reference_leak * boundary;
     int polarity = -1;
     double level = 0;
