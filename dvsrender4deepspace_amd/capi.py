@@ -33,9 +33,9 @@ EVENT_DTYPE = np.dtype([("timestamp", "<f8"), ("x", "<i2"), ("y", "<i2"), ("pola
                         ("depth", "<f4")])
 PIXEL_STATE_DTYPE = np.dtype([("receptor_log", "<f8"), ("reference_log", "<f8"), ("refractory_until", "<f8"),
                               ("threshold_pos", "<f8"), ("threshold_neg", "<f8"), ("reset_pending", "<i4"),
-                              ("reset_block", "<u4"), ("shot_block", "<u8")])
+                              ("reset_block", "<u4"), ("shot_block", "<u8"), ("receptor_inverse", "<f8")])
 assert NODE_DTYPE.itemsize == 40 and STATE_DTYPE.itemsize == 72 and INTERVAL_DTYPE.itemsize == 160
-assert EVENT_DTYPE.itemsize == 17 and PIXEL_STATE_DTYPE.itemsize == 56
+assert EVENT_DTYPE.itemsize == 17 and PIXEL_STATE_DTYPE.itemsize == 64
 
 
 class DvsError(RuntimeError):

@@ -453,37 +453,14 @@ __global__ __launch_bounds__(256) void adapt_integrate(const SegDev *segs, const
   if (i >= g.npix || !A.accepted) return;
   const int gp = g.pixel_offset + i;
   dvs_pixel_state st = g.states[i];
-  PixelRegs regs{st.receptor_log, st.reference_log, st.refractory_until, st.threshold_pos,
-                 st.threshold_neg, st.reset_pending};
-  NoiseStream reset(g.seed, (uint64_t)i, kStreamReset, st.reset_block);
-  auto normal = [&]() { return reset.normal(); };
-  CrossingEvent *out = g.slots + (size_t)i * g.max_events;
-  const CircuitParams p = g.params;
-  int count = g.counts[i];
   const bool fallback = b.flags[gp] & 1;
   const double *rates = fallback ? A.fb_rad + b.fb_index[gp] : b.rates + gp;
   const size_t stride = fallback ? (size_t)A.fb_count : (size_t)pixels;
   const int c_end = min(c1, g.cells);
-  // The next cell's photocurrent is loaded before this cell is integrated, so
-  // its HBM latency overlaps the FP64 chain (one or two waves per SIMD cannot
-  // hide it, and the crossing stores keep the compiler from hoisting it).
-  double next = c0 < c_end ? rates[0] : 0.0;
-  for (int c = c0; c < c_end; ++c) {
-    const double rate = next;
-    if (c + 1 < c_end) next = rates[(size_t)(c + 1 - c0) * stride];
-    if (!isfinite(rate)) { atomicOr(status, 2); g.counts[i] = 0; return; }
-    const int emitted = advance_pixel(regs, p, rate, g.t0[c], g.t1[c], out + count, g.max_events - count, normal);
-    if (emitted < 0) { atomicOr(status, 1); g.counts[i] = 0; return; }
-    count += emitted;
-  }
-  st.receptor_log = regs.receptor_log;
-  st.reference_log = regs.reference_log;
-  st.refractory_until = regs.refractory_until;
-  st.threshold_pos = regs.threshold_pos;
-  st.threshold_neg = regs.threshold_neg;
-  st.reset_pending = regs.reset_pending;
+  const int count = circuit_cells(st, g.params, g.seed, (uint64_t)i, rates, stride, g.t0, g.t1, c0, c_end,
+                                  g.slots + (size_t)i * g.max_events, g.counts[i], g.max_events);
+  if (count < 0) { atomicOr(status, count == -1 ? 1 : 2); g.counts[i] = 0; return; }
   st.shot_block += (uint64_t)max(0, c_end - c0);
-  st.reset_block = (uint32_t)reset.block;
   g.states[i] = st;
   g.counts[i] = count;
 }

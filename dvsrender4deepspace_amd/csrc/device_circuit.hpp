@@ -21,13 +21,18 @@ struct PhotoParams {
 struct CrossingEvent { double time; int polarity; int pad; };
 
 // Exact constant-rate receptor solution (receptor_log_at, pixel_model.hpp:18-26).
-__device__ __forceinline__ double receptor_log_at(double u, double rate, double dt, double charge) {
+// receptor_log_from takes x = exp(u) from the caller, which evaluates it once
+// for all pieces that start at u.
+__device__ __forceinline__ double receptor_log_from(double u, double x, double rate, double dt, double charge) {
   if (dt <= 0) return u;
   rate = fmax(0.0, rate);
-  const double x = exp(u);
   if (rate == 0) return u - log1p(x * dt / charge);
   const double a = exp(-rate * dt / charge), b = -expm1(-rate * dt / charge);
   return -log(a / x + b / rate);
+}
+__device__ __forceinline__ double receptor_log_at(double u, double rate, double dt, double charge) {
+  if (dt <= 0) return u;
+  return receptor_log_from(u, exp(u), rate, dt, charge);
 }
 
 // ---------------------------------------------------------------------------
@@ -141,6 +146,7 @@ __device__ inline int advance_pixel(PixelRegs &s, const CircuitParams &p, double
       if (p.reset_noise_std > 0) s.reference_log += p.reset_noise_std * normal();
       s.reset_pending = 0;
     }
+    const double x = exp(u);
     double end = 0, c0 = 0, boundary = 0, dt;
     if (hold) {
       end = fmin(t1, s.refractory_until);
@@ -150,7 +156,7 @@ __device__ inline int advance_pixel(PixelRegs &s, const CircuitParams &p, double
       c0 = u - s.reference_log;
       boundary = remaining;
       if (p.reference_leak > 0) {
-        const double x = exp(u), target = rate + p.charge_scale * p.reference_leak;
+        const double target = rate + p.charge_scale * p.reference_leak;
         if (x > target && target > rate) {
           double extremum;
           if (rate > 0) extremum = -p.charge_scale / rate * log((1.0 / target - 1.0 / rate) / (1.0 / x - 1.0 / rate));
@@ -160,7 +166,9 @@ __device__ inline int advance_pixel(PixelRegs &s, const CircuitParams &p, double
       }
       dt = boundary;
     }
-    const double un = receptor_log_at(u, rate, dt, p.charge_scale);
+    // dt > 0 here: a hold ends after the cursor and a free piece is the rest
+    // of the cell or an extremum above 1e-15
+    const double un = receptor_log_from(u, x, rate, dt, p.charge_scale);
     if (hold) {
       u = un;
       cursor = end;
@@ -202,11 +210,11 @@ __device__ inline int advance_pixel(PixelRegs &s, const CircuitParams &p, double
     } else {
       for (int j = 0; j < 48; ++j) {
         const double mid = (lo + hi) * .5;
-        const double c = receptor_log_at(u, rate, mid, p.charge_scale) - s.reference_log + p.reference_leak * mid;
+        const double c = receptor_log_from(u, x, rate, mid, p.charge_scale) - s.reference_log + p.reference_leak * mid;
         if (polarity ? c >= level : c <= level) hi = mid; else lo = mid;
       }
     }
-    u = receptor_log_at(u, rate, hi, p.charge_scale);
+    u = receptor_log_from(u, x, rate, hi, p.charge_scale);
     cursor += hi;
     out[count].time = cursor;
     out[count].polarity = polarity;
@@ -216,6 +224,168 @@ __device__ inline int advance_pixel(PixelRegs &s, const CircuitParams &p, double
     s.reset_pending = 1;
   }
   s.receptor_log = u;
+  return count;
+}
+
+// ---------------------------------------------------------------------------
+// The same circuit without reference leakage, integrated in the inverse
+// receptor variable y = exp(-u).  Over a piece of length dt at photocurrent r
+// the constant-rate solution above is affine in y:
+//   y' = exp(-r dt / C) y + (-expm1(-r dt / C)) / r     (r > 0)
+//   y' = y + dt / C                                      (r = 0)
+// so a cell costs one such map instead of exp(u), log and log1p, and the
+// comparator tests u - ref >= theta_pos and u - ref <= -theta_neg become
+// y <= exp(-(ref + theta_pos)) and y >= exp(-(ref - theta_neg)).  The analytic
+// crossing root of the reference is already written in these inverse
+// voltages.  y moves from cell to cell by whole-cell maps and is kept in the
+// pixel state, so the receptor trajectory does not depend on how steps,
+// chunks or crossings split the time line; pieces inside a cell (hold ends,
+// crossings) start from partial maps.  The results are the reference's up to
+// rounding (the reference's own u-space sequence rounds differently).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double inverse_map(double y, double rate, double dt, double charge) {
+  if (dt <= 0) return y;
+  if (rate == 0) return y + dt / charge;
+  const double e = -rate * dt / charge;
+  return exp(e) * y + -expm1(e) / rate;
+}
+
+struct InverseRegs {
+  double y, reference_log, refractory_until, threshold_pos, threshold_neg;
+  double y_pos, y_neg;  // exp(-(reference_log + threshold_pos)), exp(-(reference_log - threshold_neg))
+  int reset_pending;
+  __device__ void set_reference(double ref) {
+    reference_log = ref;
+    y_pos = exp(-(ref + threshold_pos));
+    y_neg = exp(-(ref - threshold_neg));
+  }
+};
+
+// One cell [t0, t1].  A free-running cell whose start and end stay inside
+// (y_pos, y_neg), or a cell inside a refractory hold, only moves y; other
+// cells run the reference's pieces.  Within a hold the reference tracks the
+// reference voltage to the receptor; that value is only read when the hold
+// ends, so whole held cells leave it to the caller.
+template <class Normal>
+__device__ inline int advance_cell_inverse(InverseRegs &s, const CircuitParams &p, double rate, double t0,
+                                           double t1, CrossingEvent *out, int capacity, Normal &&normal) {
+  if (t1 <= t0) return 0;
+  rate = fmax(0.0, rate);
+  const double y0 = s.y, yb = inverse_map(y0, rate, t1 - t0, p.charge_scale);
+  s.y = yb;
+  if (t0 >= s.refractory_until && !s.reset_pending) {
+    if (y0 > s.y_pos && y0 < s.y_neg && yb > s.y_pos && yb < s.y_neg) return 0;
+  } else if (s.refractory_until > t1) {
+    return 0;
+  }
+  int count = 0;
+  double cursor = t0, y = y0;
+  while (cursor < t1) {
+    const bool hold = cursor < s.refractory_until;
+    if (!hold && s.reset_pending) {
+      if (p.reset_noise_std > 0) s.set_reference(s.reference_log + p.reset_noise_std * normal());
+      s.reset_pending = 0;
+    }
+    if (hold) {
+      const double end = fmin(t1, s.refractory_until);
+      y = end == t1 ? yb : inverse_map(y, rate, end - cursor, p.charge_scale);
+      cursor = end;
+      double ref = -log(y);
+      if ((cursor < t1 || cursor == s.refractory_until) && cursor >= s.refractory_until && s.reset_pending) {
+        if (p.reset_noise_std > 0) ref += p.reset_noise_std * normal();
+        s.reset_pending = 0;
+      }
+      s.set_reference(ref);
+      continue;
+    }
+    // free piece [cursor, t1]: comparator at both ends, as the reference
+    int polarity = -1;
+    double target = 0;
+    bool at_start = true;
+    if (y <= s.y_pos) { polarity = 1; target = s.y_pos; }
+    else if (y >= s.y_neg) { polarity = 0; target = s.y_neg; }
+    else {
+      at_start = false;
+      if (yb <= s.y_pos) { polarity = 1; target = s.y_pos; }
+      else if (yb >= s.y_neg) { polarity = 0; target = s.y_neg; }
+    }
+    if (polarity < 0) break;
+    if (count == capacity) return -1;
+    double hi = 0;
+    if (!at_start) {
+      if (rate > 0) hi = -p.charge_scale / rate * log1p((target - y) / (y - 1.0 / rate));
+      else hi = p.charge_scale * (target - y);
+      hi = fmin(t1 - cursor, fmax(0.0, hi));
+    }
+    y = inverse_map(y, rate, hi, p.charge_scale);
+    cursor += hi;
+    out[count].time = cursor;
+    out[count].polarity = polarity;
+    ++count;
+    s.set_reference(-log(y));
+    s.refractory_until = cursor + p.refractory_time;
+    s.reset_pending = 1;
+  }
+  return count;
+}
+
+// The circuit of one pixel over cells [c_begin, c_end): photocurrent of cell c
+// at rates[(c - c_begin) * stride] (loaded one cell ahead, so its latency
+// overlaps the FP64 work of the cell before), windows [t0[c], t1[c]],
+// crossings appended at out[count...] up to `capacity` in all.  Without
+// leakage the inverse form above runs and the state carries y; with it the
+// reference's sequence runs on u.  Returns the new count, -1 past capacity,
+// -2 on a non-finite photocurrent.
+__device__ inline int circuit_cells(dvs_pixel_state &st, const CircuitParams &p, uint64_t seed, uint64_t pixel,
+                                    const double *rates, size_t stride, const double *t0, const double *t1,
+                                    int c_begin, int c_end, CrossingEvent *out, int count, int capacity) {
+  NoiseStream reset(seed, pixel, kStreamReset, st.reset_block);
+  auto normal = [&]() { return reset.normal(); };
+  double next = c_begin < c_end ? rates[0] : 0.0;
+  if (p.reference_leak > 0) {
+    PixelRegs s{st.receptor_log, st.reference_log, st.refractory_until, st.threshold_pos, st.threshold_neg,
+                st.reset_pending};
+    for (int c = c_begin; c < c_end; ++c) {
+      const double rate = next;
+      if (c + 1 < c_end) next = rates[(size_t)(c + 1 - c_begin) * stride];
+      if (!isfinite(rate)) return -2;
+      const int emitted = advance_pixel(s, p, rate, t0[c], t1[c], out + count, capacity - count, normal);
+      if (emitted < 0) return -1;
+      count += emitted;
+    }
+    st.receptor_log = s.receptor_log;
+    st.receptor_inverse = exp(-s.receptor_log);
+    st.reference_log = s.reference_log;
+    st.refractory_until = s.refractory_until;
+    st.reset_pending = s.reset_pending;
+  } else {
+    InverseRegs s;
+    s.y = st.receptor_inverse;
+    s.refractory_until = st.refractory_until;
+    s.threshold_pos = st.threshold_pos;
+    s.threshold_neg = st.threshold_neg;
+    s.reset_pending = st.reset_pending;
+    s.set_reference(st.reference_log);
+    double last = -INFINITY;
+    for (int c = c_begin; c < c_end; ++c) {
+      const double rate = next;
+      if (c + 1 < c_end) next = rates[(size_t)(c + 1 - c_begin) * stride];
+      if (!isfinite(rate)) return -2;
+      const int emitted = advance_cell_inverse(s, p, rate, t0[c], t1[c], out + count, capacity - count, normal);
+      if (emitted < 0) return -1;
+      count += emitted;
+      last = t1[c];
+    }
+    if (c_end > c_begin) {
+      st.receptor_inverse = s.y;
+      st.receptor_log = -log(s.y);
+      // inside a hold the reference voltage follows the receptor
+      st.reference_log = s.refractory_until > last ? st.receptor_log : s.reference_log;
+      st.refractory_until = s.refractory_until;
+      st.reset_pending = s.reset_pending;
+    }
+  }
+  st.reset_block = (uint32_t)reset.block;
   return count;
 }
 

@@ -116,6 +116,7 @@ __global__ void init_kernel(int npix, const double *radiance, PhotoParams photo,
   if (!isfinite(r)) { atomicOr(status, 2); return; }
   dvs_pixel_state s;
   s.receptor_log = s.reference_log = log(fmax(1e-30, r * photo.electrons_per_radiance + photo.dark_electrons));
+  s.receptor_inverse = exp(-s.receptor_log);
   s.refractory_until = -1;
   double zp = 0, zn = 0;
   if (threshold_std != 0) {
@@ -206,31 +207,10 @@ __global__ __launch_bounds__(256) void circuit_multi(const SegDev *segs, const i
   const int i = ((int)blockIdx.x - block_begin[si]) * 256 + threadIdx.x;
   if (i >= g.npix) return;
   dvs_pixel_state st = g.states[i];
-  PixelRegs regs{st.receptor_log, st.reference_log, st.refractory_until, st.threshold_pos,
-                 st.threshold_neg, st.reset_pending};
-  NoiseStream reset(g.seed, (uint64_t)i, kStreamReset, st.reset_block);
-  auto normal = [&]() { return reset.normal(); };
-  CrossingEvent *out = g.slots + (size_t)i * g.max_events;
-  const CircuitParams p = g.params;
-  int count = g.counts[i];  // crossings of earlier cell chunks of this step
-  double next = g.cells > 0 ? g.rad[i] : 0.0;  // photocurrent of the next cell, loaded one cell ahead
-  for (int c = 0; c < g.cells; ++c) {
-    const double t0 = g.t0[c], t1 = g.t1[c];
-    const double rate = next;
-    if (c + 1 < g.cells) next = g.rad[(size_t)(c + 1) * g.npix + i];
-    if (!isfinite(rate)) { atomicOr(status, 2); g.counts[i] = 0; return; }
-    const int emitted = advance_pixel(regs, p, rate, t0, t1, out + count, g.max_events - count, normal);
-    if (emitted < 0) { atomicOr(status, 1); g.counts[i] = 0; return; }
-    count += emitted;
-  }
-  st.receptor_log = regs.receptor_log;
-  st.reference_log = regs.reference_log;
-  st.refractory_until = regs.refractory_until;
-  st.threshold_pos = regs.threshold_pos;
-  st.threshold_neg = regs.threshold_neg;
-  st.reset_pending = regs.reset_pending;
+  const int count = circuit_cells(st, g.params, g.seed, (uint64_t)i, g.rad + i, (size_t)g.npix, g.t0, g.t1, 0,
+                                  g.cells, g.slots + (size_t)i * g.max_events, g.counts[i], g.max_events);
+  if (count < 0) { atomicOr(status, count == -1 ? 1 : 2); g.counts[i] = 0; return; }
   st.shot_block += (uint64_t)g.cells;
-  st.reset_block = (uint32_t)reset.block;
   g.states[i] = st;
   g.counts[i] = count;
 }

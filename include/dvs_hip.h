@@ -93,13 +93,17 @@ typedef struct {
 /* PixelState (include/dvs/pixel_model.hpp:11-15) as held on the device,
  * plus the noise counters: reset_block is the reset stream's next Philox
  * block, shot_block the number of circuit cells integrated so far (the
- * serial that keys each cell's shot-noise draw). */
+ * serial that keys each cell's shot-noise draw).  receptor_inverse is
+ * exp(-receptor_log): without reference leakage the circuit integrates this
+ * variable and carries it exactly from step to step (receptor_log is then
+ * derived from it); with leakage it is derived from receptor_log. */
 typedef struct {
   double receptor_log, reference_log, refractory_until;
   double threshold_pos, threshold_neg;
   int32_t reset_pending;
   uint32_t reset_block;
   uint64_t shot_block;
+  double receptor_inverse;
 } dvs_pixel_state;
 
 /* Device timings and work counters of one sensor or scene handle. */
