@@ -132,8 +132,12 @@ hipError_t launch_adapt_segment_stats(int nseg, const int *pixel_begin, AdaptBuf
 // gate_segs[g].pixel_list and their native cell radiance in gate_segs[g].rad.
 hipError_t launch_adapt_gate(const SegDev *gate_segs, const int *gate_seg, int ngate, const AdaptDev *ad,
                              AdaptBufs b, double *errors, hipStream_t st);
+// Photocurrent of cells [c0, c1) of the knot pixels; `rare` holds
+// adapt_rates_flags(pixels, c1 - c0) per-wave flags of the second phase.
+size_t adapt_rates_flags(int pixels, int cells);
 hipError_t launch_adapt_rates(const SegDev *segs, const AdaptDev *ad, const int *pixel_begin, int nseg, int pixels,
-                              AdaptBufs b, int c0, int c1, int *status, hipStream_t st);
+                              AdaptBufs b, int c0, int c1, int *status, int *rare, size_t rare_capacity,
+                              hipStream_t st);
 hipError_t launch_adapt_integrate(const SegDev *segs, const AdaptDev *ad, const int *block_begin, int nseg,
                                   int blocks, int pixels, AdaptBufs b, int c0, int c1, int *status, hipStream_t st);
 

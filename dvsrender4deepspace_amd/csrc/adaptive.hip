@@ -403,15 +403,16 @@ __device__ __forceinline__ int knot_bracket_search(const double2 *v, int size, d
 // position, then both walk forward as in the reference, so each knot is read
 // once per run while the rate stores stay coalesced across pixels.
 constexpr int kRateCells = 32;
-__global__ __launch_bounds__(256) void adapt_rates(const SegDev *segs, const AdaptDev *ad, const int *pixel_begin,
-                                                   int nseg, int pixels, AdaptBufs b, int c0, int c1, int *status) {
+// Returns whether any cell of the run was left to adapt_rates_rare.
+__device__ __forceinline__ bool adapt_rates_run(const SegDev *segs, const AdaptDev *ad, const int *pixel_begin,
+                                                int nseg, int pixels, const AdaptBufs &b, int c0, int c1, int *status) {
   const int gp = blockIdx.x * 256 + threadIdx.x;
-  if (gp >= pixels) return;
+  if (gp >= pixels) return false;
   const int si = segment_of<int>(pixel_begin, nseg, gp);
   const SegDev &g = segs[si];
   const AdaptDev &A = ad[si];
   const int cb = c0 + (int)blockIdx.y * kRateCells, ce = min(min(c1, g.cells), cb + kRateCells);
-  if (cb >= ce || !A.accepted || (b.flags[gp] & 1)) return;
+  if (cb >= ce || !A.accepted || (b.flags[gp] & 1)) return false;
   const int i = gp - pixel_begin[si];
   const int size = b.kcount[gp];
   const double2 *v = b.knots + (b.koff[gp] - size);
@@ -427,6 +428,8 @@ __global__ __launch_bounds__(256) void adapt_rates(const SegDev *segs, const Ada
   // of the previous chunk may be updating the state while this runs
   const uint64_t serial0 = A.cell_base;
   double *out = b.rates + gp;
+  uint64_t quad[4] = {0, 0, 0, 0}, quad_index = ~0ull;
+  bool marked = false;
   for (int c = cb; c < ce; ++c) {
     const double f = frac[c];
     a = knot_bracket(v, size, f, a);
@@ -435,8 +438,46 @@ __global__ __launch_bounds__(256) void adapt_rates(const SegDev *segs, const Ada
     if (e < ec && b.ecell[eo + e] == c) light = b.erad[eo + e++];
     light = fmax(0.0, light);
     if (!isfinite(light)) atomicOr(status, 2);
-    out[(size_t)(c - c0) * pixels] =
-        cell_rate(light, g.photo, g.t1[c] - g.t0[c], g.seed, (uint64_t)i, serial0 + (uint64_t)c);
+    const uint64_t serial = serial0 + (uint64_t)c;
+    if (g.photo.shot_noise && serial >> 2 != quad_index) {
+      quad_index = serial >> 2;
+      shot_quad(g.seed, (uint64_t)i, quad_index, quad);
+    }
+    const double rate = cell_rate_first(light, g.photo, g.t1[c] - g.t0[c], serial, quad);
+    out[(size_t)(c - c0) * pixels] = rate;
+    marked |= rate < 0;
+  }
+  return marked;
+}
+__device__ __forceinline__ size_t rate_wave() {
+  return ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 4 + (threadIdx.x >> 6);
+}
+__global__ __launch_bounds__(256) void adapt_rates(const SegDev *segs, const AdaptDev *ad, const int *pixel_begin,
+                                                   int nseg, int pixels, AdaptBufs b, int c0, int c1, int *status,
+                                                   int *rare) {
+  const bool marked = adapt_rates_run(segs, ad, pixel_begin, nseg, pixels, b, c0, c1, status);
+  // one flag per wave: does any lane leave a draw to adapt_rates_rare
+  const unsigned long long any = __ballot(marked);
+  if ((threadIdx.x & 63) == 0) rare[rate_wave()] = any != 0;
+}
+// The draws adapt_rates left to the cells' own streams (lambda >= 10), in
+// the waves that flagged any.
+__global__ __launch_bounds__(256) void adapt_rates_rare(const SegDev *segs, const AdaptDev *ad,
+                                                        const int *pixel_begin, int nseg, int pixels, AdaptBufs b,
+                                                        int c0, int c1, const int *rare) {
+  if (!rare[rate_wave()]) return;
+  const int gp = blockIdx.x * 256 + threadIdx.x;
+  if (gp >= pixels) return;
+  const int si = segment_of<int>(pixel_begin, nseg, gp);
+  const SegDev &g = segs[si];
+  const AdaptDev &A = ad[si];
+  const int cb = c0 + (int)blockIdx.y * kRateCells, ce = min(min(c1, g.cells), cb + kRateCells);
+  if (cb >= ce || !A.accepted || (b.flags[gp] & 1)) return;
+  const int i = gp - pixel_begin[si];
+  double *out = b.rates + gp;
+  for (int c = cb; c < ce; ++c) {
+    double &r = out[(size_t)(c - c0) * pixels];
+    if (r < 0) r = cell_rate_rest(r, g.t1[c] - g.t0[c], g.seed, (uint64_t)i, A.cell_base + (uint64_t)c);
   }
 }
 
@@ -593,11 +634,17 @@ hipError_t launch_adapt_gate(const SegDev *gate_segs, const int *gate_seg, int n
   adapt_gate<<<ngate * 4, 256, 0, st>>>(gate_segs, gate_seg, ad, b, errors);
   return hipGetLastError();
 }
+size_t adapt_rates_flags(int pixels, int cells) {
+  return pixels <= 0 || cells <= 0 ? 0 : (size_t)grid(pixels) * 4 * (size_t)((cells + kRateCells - 1) / kRateCells);
+}
 hipError_t launch_adapt_rates(const SegDev *segs, const AdaptDev *ad, const int *pixel_begin, int nseg, int pixels,
-                              AdaptBufs b, int c0, int c1, int *status, hipStream_t st) {
+                              AdaptBufs b, int c0, int c1, int *status, int *rare, size_t rare_capacity,
+                              hipStream_t st) {
   if (pixels <= 0 || c1 <= c0) return hipSuccess;
-  const unsigned runs = (unsigned)((c1 - c0 + kRateCells - 1) / kRateCells);
-  adapt_rates<<<dim3(grid(pixels), runs), 256, 0, st>>>(segs, ad, pixel_begin, nseg, pixels, b, c0, c1, status);
+  if (adapt_rates_flags(pixels, c1 - c0) > rare_capacity) return hipErrorInvalidConfiguration;
+  const dim3 g(grid(pixels), (unsigned)((c1 - c0 + kRateCells - 1) / kRateCells));
+  adapt_rates<<<g, 256, 0, st>>>(segs, ad, pixel_begin, nseg, pixels, b, c0, c1, status, rare);
+  adapt_rates_rare<<<g, 256, 0, st>>>(segs, ad, pixel_begin, nseg, pixels, b, c0, c1, rare);
   return hipGetLastError();
 }
 hipError_t launch_adapt_integrate(const SegDev *segs, const AdaptDev *ad, const int *block_begin, int nseg,

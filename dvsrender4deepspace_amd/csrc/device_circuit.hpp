@@ -40,11 +40,14 @@ __device__ __forceinline__ double receptor_log_at(double u, double rate, double 
 // (seed, tag ^ stream), counter (block, pixel, word, 0).  Each draw starts a
 // fresh block.  Shot noise is keyed per circuit cell: the draw of a pixel's
 // n-th cell since initialisation uses word = n and blocks 0, 1, ... of its
-// own, so it can be drawn by any thread at any time.  Reset noise (word 0)
-// advances a per-pixel block counter, as its draws occur at crossings.
-// Batching, launch shape and other requests cannot change either stream.
+// own, so it can be drawn by any thread at any time.  The inversion branch
+// (lambda < 10, one uniform) instead takes word n mod 4 of block n / 4 of the
+// pixel's shared shot stream (word 0), so four consecutive cells share one
+// Philox block.  Reset noise (word 0) advances a per-pixel block counter, as
+// its draws occur at crossings.  Batching, launch shape and other requests
+// cannot change any of these streams.
 // ---------------------------------------------------------------------------
-enum { kStreamShot = 0, kStreamThreshold = 1, kStreamReset = 2 };
+enum { kStreamShot = 0, kStreamThreshold = 1, kStreamReset = 2, kStreamShotShared = 3 };
 constexpr uint64_t kStreamTag = 0x4456535F4E4F4953ULL;
 
 __device__ __forceinline__ void philox4x64(uint64_t c0, uint64_t c1, uint64_t c2, uint64_t c3, uint64_t k0,
@@ -59,6 +62,25 @@ __device__ __forceinline__ void philox4x64(uint64_t c0, uint64_t c1, uint64_t c2
   }
   out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
+__device__ __forceinline__ double uniform_of(uint64_t x) {  // open interval (0, 1), 53 bits
+  return ((double)(x >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+// The shared shot-noise block of cells 4 quad .. 4 quad + 3.
+__device__ __forceinline__ void shot_quad(uint64_t seed, uint64_t pixel, uint64_t quad, uint64_t out[4]) {
+  philox4x64(quad, pixel, 0, 0, seed, kStreamTag ^ kStreamShotShared, out);
+}
+__device__ __forceinline__ uint64_t quad_word(const uint64_t w[4], uint64_t serial) {
+  const int j = (int)(serial & 3);  // selected without a dynamic index so the block stays in registers
+  return j == 0 ? w[0] : j == 1 ? w[1] : j == 2 ? w[2] : w[3];
+}
+// Sequential-search inversion of Poisson(lambda), 0 < lambda < 10, from one
+// uniform u (Devroye 1986, ch. X.3).
+__device__ __forceinline__ double poisson_inversion(double lambda, double u) {
+  double p = exp(-lambda), F = p;
+  int k = 0;
+  while (u > F && k < 1000) { ++k; p *= lambda / k; F += p; }
+  return k;
+}
 struct NoiseStream {
   uint64_t seed, pixel, stream, block, word;
   uint64_t buf[4];
@@ -70,7 +92,7 @@ struct NoiseStream {
     // select without a dynamic index so the block stays in registers
     const uint64_t x = used == 0 ? buf[0] : used == 1 ? buf[1] : used == 2 ? buf[2] : buf[3];
     ++used;
-    return ((double)(x >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    return uniform_of(x);
   }
   // Box-Muller, one normal per fresh block.
   __device__ double normal() {
@@ -81,17 +103,13 @@ struct NoiseStream {
   // Poisson(lambda): sequential-search inversion below 10 (Devroye 1986,
   // ch. X.3), PTRS transformed rejection (Hoermann 1993) up to 1e6, and the
   // reference's rounded normal approximation above (photon_noise.cuh:4-9).
+  // A NaN lambda gives NaN (the rejection loop would never accept it).
   __device__ double poisson(double lambda) {
+    if (lambda != lambda) return lambda;
     if (lambda <= 0) return 0;
     if (lambda > 1e6) return fmax(0.0, floor(lambda + sqrt(lambda) * normal() + .5));
     used = 4;
-    if (lambda < 10) {
-      const double u = uniform();
-      double p = exp(-lambda), F = p;
-      int k = 0;
-      while (u > F && k < 1000) { ++k; p *= lambda / k; F += p; }
-      return k;
-    }
+    if (lambda < 10) return poisson_inversion(lambda, uniform());
     const double slam = sqrt(lambda), loglam = log(lambda);
     const double b = 0.931 + 2.53 * slam, a = -0.059 + 0.02483 * b;
     const double invalpha = 1.1239 + 1.1328 / (b - 3.4), vr = 0.9277 - 3.6224 / (b - 2);
@@ -106,17 +124,33 @@ struct NoiseStream {
   }
 };
 
+// The shot-noise draws outside the inversion branch, from the cell's own
+// stream.  They are rare and need far more registers than the rest of the
+// photocurrent (lgamma, Box-Muller), so they run in kernels of their own.
+__device__ inline double shot_stream_draw(uint64_t seed, uint64_t pixel, uint64_t cell_serial, double lambda) {
+  NoiseStream shot(seed, pixel, kStreamShot, 0, cell_serial);
+  return shot.poisson(lambda);
+}
+
 // Photocurrent of one circuit cell (generate_pixels, event_sensor.cu:42-43):
 // radiance -> electrons/s, with shot noise rate = Poisson(rate * dt) / dt
-// (poisson_electrons, photon_noise.cuh:4-9) from the cell's own counter.
-__device__ inline double cell_rate(double light, const PhotoParams &ph, double dt, uint64_t seed, uint64_t pixel,
-                                   uint64_t cell_serial) {
-  double rate = light * ph.electrons_per_radiance + ph.dark_electrons;
-  if (ph.shot_noise) {
-    NoiseStream shot(seed, pixel, kStreamShot, 0, cell_serial);
-    rate = shot.poisson(rate * dt) / dt;
-  }
-  return rate;
+// (poisson_electrons, photon_noise.cuh:4-9) keyed by the cell serial, in two
+// phases.  cell_rate_first decides every cell whose draw is the inversion
+// branch (or needs no draw) and returns -lambda (< 0) for a cell whose draw
+// needs the cell's own stream (lambda >= 10); cell_rate_rest finishes it.
+// `quad` is shot_quad(seed, pixel, cell_serial / 4) (unused without noise).
+__device__ __forceinline__ double cell_rate_first(double light, const PhotoParams &ph, double dt, uint64_t cell_serial,
+                                                  const uint64_t quad[4]) {
+  const double rate = light * ph.electrons_per_radiance + ph.dark_electrons;
+  if (!ph.shot_noise) return rate;
+  const double lambda = rate * dt;
+  if (lambda != lambda) return lambda;  // NaN photocurrent: the circuit reports it
+  if (lambda <= 0) return 0.0 / dt;
+  if (lambda < 10) return poisson_inversion(lambda, uniform_of(quad_word(quad, cell_serial))) / dt;
+  return -lambda;
+}
+__device__ inline double cell_rate_rest(double marked, double dt, uint64_t seed, uint64_t pixel, uint64_t cell_serial) {
+  return shot_stream_draw(seed, pixel, cell_serial, -marked) / dt;
 }
 
 // ---------------------------------------------------------------------------

@@ -183,7 +183,36 @@ __global__ __launch_bounds__(256) void trace_cells_multi(const SegDev *segs, con
 // serial (cells integrated so far, advanced by the circuit after each chunk)
 // plus the cell's index in the chunk.
 __global__ __launch_bounds__(256) void cell_rates_multi(const SegDev *segs, const int *item_begin, int nseg,
-                                                        int items, int *status) {
+                                                        int items, int *status, int *rare) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  const int c = (int)blockIdx.y;
+  bool marked = false;
+  if (k < items) {
+    const int si = segment_of<int>(item_begin, nseg, k);
+    const SegDev &g = segs[si];
+    const int item = k - item_begin[si];
+    if (c < g.cells && item < g.npix) {
+      const int px = g.pixel_list ? g.pixel_list[item] : item;
+      double *cell = g.rad + (size_t)c * g.npix + item;
+      const double light = *cell;
+      if (!isfinite(light)) atomicOr(status, 2);
+      const uint64_t serial = g.states[px].shot_block + (uint64_t)c;
+      uint64_t quad[4] = {0, 0, 0, 0};
+      if (g.photo.shot_noise) shot_quad(g.seed, (uint64_t)px, serial >> 2, quad);
+      const double rate = cell_rate_first(light, g.photo, g.t1[c] - g.t0[c], serial, quad);
+      *cell = rate;
+      marked = rate < 0;
+    }
+  }
+  // one flag per wave: does any lane leave a draw to cell_rates_rare
+  const unsigned long long any = __ballot(marked);
+  if ((threadIdx.x & 63) == 0) rare[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 4 + (threadIdx.x >> 6)] = any != 0;
+}
+
+// Finishes the cells cell_rates_multi marked, in the waves that flagged any.
+__global__ __launch_bounds__(256) void cell_rates_rare(const SegDev *segs, const int *item_begin, int nseg, int items,
+                                                       const int *rare) {
+  if (!rare[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 4 + (threadIdx.x >> 6)]) return;
   const int k = blockIdx.x * 256 + threadIdx.x;
   const int c = (int)blockIdx.y;
   if (k >= items) return;
@@ -193,9 +222,8 @@ __global__ __launch_bounds__(256) void cell_rates_multi(const SegDev *segs, cons
   if (c >= g.cells || item >= g.npix) return;
   const int px = g.pixel_list ? g.pixel_list[item] : item;
   double *cell = g.rad + (size_t)c * g.npix + item;
-  const double light = *cell;
-  if (!isfinite(light)) atomicOr(status, 2);
-  *cell = cell_rate(light, g.photo, g.t1[c] - g.t0[c], g.seed, (uint64_t)px, g.states[px].shot_block + (uint64_t)c);
+  const double v = *cell;
+  if (v < 0) *cell = cell_rate_rest(v, g.t1[c] - g.t0[c], g.seed, (uint64_t)px, g.states[px].shot_block + (uint64_t)c);
 }
 
 // The circuit half of generate_pixels (event_sensor.cu:39-49): one lane per
@@ -437,12 +465,16 @@ hipError_t launch_trace_cells_multi(const SegDev *segs, const long long *block_b
   trace_cells_multi<<<(unsigned)blocks, 256, lds, st>>>(segs, block_begin, nseg, status, shadow_count);
   return hipGetLastError();
 }
+size_t cell_rates_flags(int items, int cells) {
+  return items <= 0 || cells <= 0 ? 0 : (size_t)((items + 255) / 256) * 4 * (size_t)cells;
+}
 hipError_t launch_cell_rates_multi(const SegDev *segs, const int *item_begin, int nseg, int items, int cells,
-                                   int *status, hipStream_t st) {
+                                   int *status, int *rare, size_t rare_capacity, hipStream_t st) {
   if (items <= 0 || cells <= 0) return hipSuccess;
-  if (cells > 65535) return hipErrorInvalidConfiguration;
-  cell_rates_multi<<<dim3((unsigned)((items + 255) / 256), (unsigned)cells), 256, 0, st>>>(segs, item_begin, nseg,
-                                                                                          items, status);
+  if (cells > 65535 || cell_rates_flags(items, cells) > rare_capacity) return hipErrorInvalidConfiguration;
+  const dim3 grid((unsigned)((items + 255) / 256), (unsigned)cells);
+  cell_rates_multi<<<grid, 256, 0, st>>>(segs, item_begin, nseg, items, status, rare);
+  cell_rates_rare<<<grid, 256, 0, st>>>(segs, item_begin, nseg, items, rare);
   return hipGetLastError();
 }
 hipError_t launch_circuit_multi(const SegDev *segs, const int *block_begin, int nseg, int blocks, int *status,

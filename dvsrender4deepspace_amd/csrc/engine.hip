@@ -506,11 +506,14 @@ class Engine {
     }
     segtab_.upload(tables.data(), tables.size(), "segment table", st);
     tblock_.upload(tblocks.data(), tblocks.size(), "segment blocks", st);
+    const size_t rare_native = cell_rates_flags(pixel_total, chunk);
+    rare_native_.reserve(rare_native, "photocurrent flags");
     // A captured graph replays only while every buffer it reads stays put.
     for (const void *ptr : {(const void *)segtab_.ptr, (const void *)tblock_.ptr, (const void *)cblock_.ptr,
                             (const void *)radiance_.ptr, (const void *)counts_.ptr, (const void *)poses_.ptr,
                             (const void *)intervals_.ptr, (const void *)cell_t0_.ptr, (const void *)cell_t1_.ptr,
-                            (const void *)cell_iv_.ptr, (const void *)status_.ptr, (const void *)counter_.ptr})
+                            (const void *)cell_iv_.ptr, (const void *)status_.ptr, (const void *)counter_.ptr,
+                            (const void *)rare_native_.ptr})
       shape.push_back((long long)(uintptr_t)ptr);
     shape.push_back((long long)lds);
     shape.push_back(pixel_total);
@@ -543,7 +546,7 @@ class Engine {
           hip_check(hipEventRecord(circuit_t.back()->start, st), "record");
         }
         hip_check(launch_cell_rates_multi(segtab_.ptr + (size_t)k * nseg, pbegin_.ptr, nseg, pixel_total, chunk,
-                                          status_.ptr, st),
+                                          status_.ptr, rare_native_.ptr, rare_native, st),
                   "trace and integrate circuits");
         hip_check(launch_circuit_multi(segtab_.ptr + (size_t)k * nseg, cblock_.ptr, nseg, cblocks_total, status_.ptr, st),
                   "trace and integrate circuits");
@@ -986,6 +989,10 @@ class Engine {
     const int chunk = (int)std::max<size_t>(1, chunk_cap);
     rates_.reserve((size_t)chunk * P * slots_needed, "adaptive photocurrent");
     b.rates = rates_.ptr;
+    // one flag buffer per stream that runs photocurrent kernels
+    const size_t rare_knots = adapt_rates_flags(P, chunk), rare_fallback = cell_rates_flags(fb_total, chunk);
+    rare_adapt_.reserve(rare_knots, "photocurrent flags");
+    rare_native_.reserve(rare_fallback, "photocurrent flags");
     uint64_t fallback_samples = 0;
     std::vector<long long> fblocks(nseg + 1);
     std::vector<SegDev> ftab(nseg);
@@ -1014,7 +1021,8 @@ class Engine {
         bk.rates = rates_.ptr + (size_t)slot * chunk * P;
         if (k == 0) hip_check(hipStreamWaitEvent(side_, ev_start_, 0), "stream wait");
         if (k >= 2) hip_check(hipStreamWaitEvent(side_, ev_int_[slot], 0), "stream wait");  // slot free again
-        hip_check(launch_adapt_rates(segtab_.ptr, adapt_.ptr, pbegin_.ptr, nseg, P, bk, c0, c1, status_.ptr, side_),
+        hip_check(launch_adapt_rates(segtab_.ptr, adapt_.ptr, pbegin_.ptr, nseg, P, bk, c0, c1, status_.ptr,
+                                     rare_adapt_.ptr, rare_knots, side_),
                   "adaptive photocurrent launch");
         hip_check(hipEventRecord(ev_rates_[slot], side_), "record");
         hip_check(hipStreamWaitEvent(st, ev_rates_[slot], 0), "stream wait");
@@ -1045,11 +1053,13 @@ class Engine {
           hip_check(launch_trace_cells_multi(fb_tab_.ptr, fb_blocks_.ptr, nseg, fblocks[nseg], lds, status_.ptr,
                                              counter_.ptr, st),
                     "parallel exact cell optics launch");
-          hip_check(launch_cell_rates_multi(fb_tab_.ptr, fb_items_.ptr, nseg, fb_total, c1 - c0, status_.ptr, st),
+          hip_check(launch_cell_rates_multi(fb_tab_.ptr, fb_items_.ptr, nseg, fb_total, c1 - c0, status_.ptr,
+                                            rare_native_.ptr, rare_fallback, st),
                     "parallel exact cell optics launch");
         }
       }
-      hip_check(launch_adapt_rates(segtab_.ptr, adapt_.ptr, pbegin_.ptr, nseg, P, b, c0, c1, status_.ptr, st),
+      hip_check(launch_adapt_rates(segtab_.ptr, adapt_.ptr, pbegin_.ptr, nseg, P, b, c0, c1, status_.ptr,
+                                   rare_adapt_.ptr, rare_knots, st),
                 "adaptive photocurrent launch");
       integrate_t.emplace_back(new Timer());
       hip_check(hipEventRecord(integrate_t.back()->start, st), "record");
@@ -1117,6 +1127,7 @@ class Engine {
       fb_scan_, run_begin_, qpix_, contrib_, cscan_, ecell_, eitem_pix_, eitem_pose_, fb_ends_, fb_list_, gate_ids_,
       gate_idx_, fb_items_;
   DeviceArray<double> cell_frac_, probes_, qrad_, erad_, gate_rad_, gate_err_, rates_;
+  DeviceArray<int> rare_native_, rare_adapt_;  // per-wave flags of the photocurrent's second phase
   DeviceArray<Pose> probe_poses_, qpose_;
   DeviceArray<AdaptEntry> ent_[2], bis_;
   DeviceArray<AdaptCounters> ctr_;

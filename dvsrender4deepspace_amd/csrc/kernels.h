@@ -57,9 +57,11 @@ using dvsdev::SegDev;
 hipError_t launch_trace_cells_multi(const SegDev *segs, const long long *block_begin, int nseg, long long blocks,
                                     size_t lds, int *status, unsigned long long *shadow_count, hipStream_t st);
 // Radiance -> photocurrent in place over `items` table items (prefix
-// item_begin) and up to `cells` cells (at most 65535).
+// item_begin) and up to `cells` cells (at most 65535).  `rare` holds
+// cell_rates_flags(items, cells) per-wave flags of the second phase.
+size_t cell_rates_flags(int items, int cells);
 hipError_t launch_cell_rates_multi(const SegDev *segs, const int *item_begin, int nseg, int items, int cells,
-                                   int *status, hipStream_t st);
+                                   int *status, int *rare, size_t rare_capacity, hipStream_t st);
 hipError_t launch_circuit_multi(const SegDev *segs, const int *block_begin, int nseg, int blocks, int *status,
                                 hipStream_t st);
 hipError_t launch_gather_multi(const SegDev *segs, const int *pixel_begin, int nseg, int total_pixels,

@@ -639,11 +639,13 @@ static void philox4x64(const uint64_t ctr[4], const uint64_t key[2], uint64_t ou
   }
   out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
-enum { kStreamShot = 0, kStreamThreshold = 1, kStreamReset = 2 };
+enum { kStreamShot = 0, kStreamThreshold = 1, kStreamReset = 2, kStreamShotShared = 3 };
 constexpr uint64_t kStreamTag = 0x4456535F4E4F4953ULL;  // key word 1 = tag ^ stream
 // Counter (block, pixel, word, 0).  Shot noise: one stream per circuit cell,
-// word = the pixel's cell serial since initialisation, blocks from 0; reset
-// and threshold noise: word 0 and a per-pixel block counter.
+// word = the pixel's cell serial n since initialisation, blocks from 0,
+// except that the inversion branch (lambda < 10) takes its one uniform from
+// word n mod 4 of block n / 4 of the shared shot stream (word 0); reset and
+// threshold noise: word 0 and a per-pixel block counter.
 struct Stream {
   uint64_t seed, pixel, stream;
   uint64_t *block;  // next unused block index for this stream
@@ -662,12 +664,13 @@ static double normal_draw(Stream &st) {
   double u1 = st.uniform(), u2 = st.uniform();
   return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
 }
-static double poisson_draw(Stream &st, double lambda) {
+static double poisson_draw(Stream &st, double lambda, double first = -1) {
+  if (lambda != lambda) return lambda;  // NaN in, NaN out (rejection would never accept)
   if (lambda <= 0) return 0;
   if (lambda > 1e6) return fmax(0.0, floor(lambda + sqrt(lambda) * normal_draw(st) + .5));
   st.used = 4;  // every draw starts a fresh block
   if (lambda < 10) {
-    double u = st.uniform(), p = exp(-lambda), F = p; int k = 0;
+    double u = first > 0 ? first : st.uniform(), p = exp(-lambda), F = p; int k = 0;
     while (u > F && k < 1000) { ++k; p *= lambda / k; F += p; }
     return k;
   }
@@ -690,7 +693,10 @@ static double cell_rate(double light, const orc_photo_params &ph, double dt, uin
   if (ph.shot_noise) {
     uint64_t block = 0;
     Stream shot{seed, pixel, kStreamShot, &block, serial};
-    rate = poisson_draw(shot, rate * dt) / dt;
+    uint64_t ctr[4] = {serial >> 2, pixel, 0, 0}, key[2] = {seed, kStreamTag ^ kStreamShotShared}, quad[4];
+    philox4x64(ctr, key, quad);
+    const double first = ((double)(quad[serial & 3] >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    rate = poisson_draw(shot, rate * dt, first) / dt;
   }
   return rate;
 }

@@ -282,6 +282,27 @@ def test_batching_invariance(gpu, shot):
         assert np.array_equal(all_[f], one[f])
 
 
+@pytest.mark.parametrize("radiance_scale,threshold,refractory", [(1e-3, .03, 0.0), (5.0, .15, 20e-6),
+                                                                  (1e6, .15, 20e-6)])
+def test_shot_noise_branches_match_oracle(oracle, gpu, radiance_scale, threshold, refractory):
+    """Noise-on events against the oracle with each Poisson branch dominant:
+    inversion from the shared shot blocks (about 0.01 electrons per cell), the
+    cells' own streams with PTRS (tens of electrons) and the rounded normal
+    (millions), the last two in the photocurrent's second phase."""
+    w = scenes.fast_pass(workloads.small_variant(workloads.DAVIS346, size=128, width=48, height=36, craters=8))
+    phys = dataclasses.replace(w.physic, threshold_pos=threshold, threshold_neg=threshold, refractory_time=refractory,
+                               max_events_per_step=64, radiance_scale=radiance_scale)
+    w = dataclasses.replace(w, physic=phys)
+    g, o = _pair(w, oracle, gpu)
+    sensor = gpu.Sensor(g, phys)
+    sensor.initialize(w.initial_state(), w.start_time)
+    raw, ready = _run_sensor(sensor, w, 12, 6)
+    _, oraw, oready = _oracle_sensor(oracle, o, w, 12, 6)
+    assert len(oraw) > 20
+    _assert_events_match(raw, oraw)
+    _assert_events_match(ready, oready)
+
+
 def test_noise_on_reproducible_per_seed(gpu):
     w = scenes.fast_pass(workloads.small_variant(workloads.DAVIS346, size=128, width=48, height=36, craters=8))
     v, n, nodes, root, idx = scenes.workload_mesh(w)
