@@ -363,30 +363,57 @@ __device__ inline int advance_cell_inverse(InverseRegs &s, const CircuitParams &
   return count;
 }
 
+// Visits cells [c_begin, c_end) with their photocurrent rates[(c - c_begin) *
+// stride], loading the next kCellPrefetch cells while the current ones are
+// integrated: a free-running cell is far shorter than an HBM round trip, so
+// one wave needs that many loads in flight.  step(rate, c) returns < 0 to stop.
+constexpr int kCellPrefetch = 8;
+template <class Step>
+__device__ __forceinline__ int for_each_cell(const double *rates, size_t stride, int c_begin, int c_end, Step &&step) {
+  double cur[kCellPrefetch], nxt[kCellPrefetch];
+#pragma unroll
+  for (int j = 0; j < kCellPrefetch; ++j) cur[j] = c_begin + j < c_end ? rates[(size_t)j * stride] : 0.0;
+  for (int cb = c_begin; cb < c_end; cb += kCellPrefetch) {
+#pragma unroll
+    for (int j = 0; j < kCellPrefetch; ++j) {
+      const int c = cb + kCellPrefetch + j;
+      nxt[j] = c < c_end ? rates[(size_t)(c - c_begin) * stride] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < kCellPrefetch; ++j) {
+      if (cb + j < c_end) {
+        const int r = step(cur[j], cb + j);
+        if (r < 0) return r;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kCellPrefetch; ++j) cur[j] = nxt[j];
+  }
+  return 0;
+}
+
 // The circuit of one pixel over cells [c_begin, c_end): photocurrent of cell c
-// at rates[(c - c_begin) * stride] (loaded one cell ahead, so its latency
-// overlaps the FP64 work of the cell before), windows [t0[c], t1[c]],
-// crossings appended at out[count...] up to `capacity` in all.  Without
-// leakage the inverse form above runs and the state carries y; with it the
-// reference's sequence runs on u.  Returns the new count, -1 past capacity,
-// -2 on a non-finite photocurrent.
+// at rates[(c - c_begin) * stride], windows [t0[c], t1[c]], crossings appended
+// at out[count...] up to `capacity` in all.  Without leakage the inverse form
+// above runs and the state carries y; with it the reference's sequence runs
+// on u.  Returns the new count, -1 past capacity, -2 on a non-finite
+// photocurrent.
 __device__ inline int circuit_cells(dvs_pixel_state &st, const CircuitParams &p, uint64_t seed, uint64_t pixel,
                                     const double *rates, size_t stride, const double *t0, const double *t1,
                                     int c_begin, int c_end, CrossingEvent *out, int count, int capacity) {
   NoiseStream reset(seed, pixel, kStreamReset, st.reset_block);
   auto normal = [&]() { return reset.normal(); };
-  double next = c_begin < c_end ? rates[0] : 0.0;
   if (p.reference_leak > 0) {
     PixelRegs s{st.receptor_log, st.reference_log, st.refractory_until, st.threshold_pos, st.threshold_neg,
                 st.reset_pending};
-    for (int c = c_begin; c < c_end; ++c) {
-      const double rate = next;
-      if (c + 1 < c_end) next = rates[(size_t)(c + 1 - c_begin) * stride];
+    const int r = for_each_cell(rates, stride, c_begin, c_end, [&](double rate, int c) {
       if (!isfinite(rate)) return -2;
       const int emitted = advance_pixel(s, p, rate, t0[c], t1[c], out + count, capacity - count, normal);
       if (emitted < 0) return -1;
       count += emitted;
-    }
+      return 0;
+    });
+    if (r < 0) return r;
     st.receptor_log = s.receptor_log;
     st.receptor_inverse = exp(-s.receptor_log);
     st.reference_log = s.reference_log;
@@ -400,21 +427,19 @@ __device__ inline int circuit_cells(dvs_pixel_state &st, const CircuitParams &p,
     s.threshold_neg = st.threshold_neg;
     s.reset_pending = st.reset_pending;
     s.set_reference(st.reference_log);
-    double last = -INFINITY;
-    for (int c = c_begin; c < c_end; ++c) {
-      const double rate = next;
-      if (c + 1 < c_end) next = rates[(size_t)(c + 1 - c_begin) * stride];
+    const int r = for_each_cell(rates, stride, c_begin, c_end, [&](double rate, int c) {
       if (!isfinite(rate)) return -2;
       const int emitted = advance_cell_inverse(s, p, rate, t0[c], t1[c], out + count, capacity - count, normal);
       if (emitted < 0) return -1;
       count += emitted;
-      last = t1[c];
-    }
+      return 0;
+    });
+    if (r < 0) return r;
     if (c_end > c_begin) {
       st.receptor_inverse = s.y;
       st.receptor_log = -log(s.y);
       // inside a hold the reference voltage follows the receptor
-      st.reference_log = s.refractory_until > last ? st.receptor_log : s.reference_log;
+      st.reference_log = s.refractory_until > t1[c_end - 1] ? st.receptor_log : s.reference_log;
       st.refractory_until = s.refractory_until;
       st.reset_pending = s.reset_pending;
     }
