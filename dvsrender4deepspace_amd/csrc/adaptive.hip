@@ -491,15 +491,20 @@ __global__ __launch_bounds__(256) void adapt_integrate(const SegDev *segs, const
   const SegDev &g = segs[si];
   const AdaptDev &A = ad[si];
   const int i = ((int)blockIdx.x - block_begin[si]) * 256 + threadIdx.x;
-  if (i >= g.npix || !A.accepted) return;
-  const int gp = g.pixel_offset + i;
-  dvs_pixel_state st = g.states[i];
+  if (!A.accepted) return;  // the whole block: blocks do not straddle segments
+  // lanes past the segment's pixels stay in the cell loop (they hold cells of
+  // the wave's shared tiles) without a pixel of their own
+  const bool own = i < g.npix;
+  const int gp = g.pixel_offset + (own ? i : 0);
+  dvs_pixel_state st = own ? g.states[i] : dvs_pixel_state{};
   const bool fallback = b.flags[gp] & 1;
   const double *rates = fallback ? A.fb_rad + b.fb_index[gp] : b.rates + gp;
   const size_t stride = fallback ? (size_t)A.fb_count : (size_t)pixels;
   const int c_end = min(c1, g.cells);
-  const int count = circuit_cells(st, g.params, g.seed, (uint64_t)i, rates, stride, g.t0, g.t1, c0, c_end,
-                                  g.slots + (size_t)i * g.max_events, g.counts[i], g.max_events);
+  const int count = circuit_cells(st, g.params, g.seed, (uint64_t)i, own, rates, stride, g.t0, g.t1, g.maps, c0,
+                                  c_end, g.slots + (size_t)(own ? i : 0) * g.max_events, own ? g.counts[i] : 0,
+                                  g.max_events);
+  if (!own) return;
   if (count < 0) { atomicOr(status, count == -1 ? 1 : 2); g.counts[i] = 0; return; }
   st.shot_block += (uint64_t)max(0, c_end - c0);
   g.states[i] = st;

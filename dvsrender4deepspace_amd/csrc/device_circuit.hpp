@@ -277,11 +277,57 @@ __device__ inline int advance_pixel(PixelRegs &s, const CircuitParams &p, double
 // crossings) start from partial maps.  The results are the reference's up to
 // rounding (the reference's own u-space sequence rounds differently).
 // ---------------------------------------------------------------------------
+struct InverseFactors {
+  double a, q;
+};
+__device__ __forceinline__ InverseFactors inverse_factors(double rate, double dt, double charge) {
+  const double e = -rate * dt / charge;
+  return InverseFactors{exp(e), -expm1(e) / rate};
+}
 __device__ __forceinline__ double inverse_map(double y, double rate, double dt, double charge) {
   if (dt <= 0) return y;
   if (rate == 0) return y + dt / charge;
-  const double e = -rate * dt / charge;
-  return exp(e) * y + -expm1(e) / rate;
+  const InverseFactors f = inverse_factors(rate, dt, charge);
+  return f.a * y + f.q;
+}
+
+// One circuit cell as the inverse circuit reads it: its window and its
+// whole-cell maps for the photocurrents shot noise makes common there, zero
+// (y + d0) and k = 1, 2 electrons (rate k / dt, the value the photocurrent
+// kernels write).  They depend on the cell alone, so one small kernel
+// evaluates them for every pixel, with the operations of inverse_map; a pixel
+// whose photocurrent is none of these evaluates its own.
+constexpr int kMappedElectrons = 2;
+struct CellMaps {
+  double t0, t1, d0;
+  double rate[kMappedElectrons], a[kMappedElectrons], q[kMappedElectrons];
+};
+constexpr int kCellWords = 3 + 3 * kMappedElectrons;  // doubles per CellMaps
+static_assert(sizeof(CellMaps) == kCellWords * sizeof(double), "CellMaps is read as words");
+__device__ __forceinline__ CellMaps cell_maps(double t0, double t1, double charge) {
+  CellMaps m;
+  const double dt = t1 - t0;
+  m.t0 = t0;
+  m.t1 = t1;
+  m.d0 = dt / charge;
+#pragma unroll
+  for (int k = 0; k < kMappedElectrons; ++k) {
+    m.rate[k] = (double)(k + 1) / dt;
+    const InverseFactors f = inverse_factors(m.rate[k], dt, charge);
+    m.a[k] = f.a;
+    m.q[k] = f.q;
+  }
+  return m;
+}
+// inverse_map over the whole cell, dt = t1 - t0 > 0, rate >= 0.
+__device__ __forceinline__ double inverse_map_cell(double y, double rate, double dt, double charge,
+                                                   const CellMaps &m) {
+  if (rate == 0) return y + m.d0;
+#pragma unroll
+  for (int k = 0; k < kMappedElectrons; ++k)
+    if (rate == m.rate[k]) return m.a[k] * y + m.q[k];
+  const InverseFactors f = inverse_factors(rate, dt, charge);
+  return f.a * y + f.q;
 }
 
 struct InverseRegs {
@@ -300,24 +346,17 @@ struct InverseRegs {
 // cells run the reference's pieces.  Within a hold the reference tracks the
 // reference voltage to the receptor; that value is only read when the hold
 // ends, so whole held cells leave it to the caller.
-template <class Normal>
-__device__ inline int advance_cell_inverse(InverseRegs &s, const CircuitParams &p, double rate, double t0,
-                                           double t1, CrossingEvent *out, int capacity, Normal &&normal) {
-  if (t1 <= t0) return 0;
-  rate = fmax(0.0, rate);
-  const double y0 = s.y, yb = inverse_map(y0, rate, t1 - t0, p.charge_scale);
-  s.y = yb;
-  if (t0 >= s.refractory_until && !s.reset_pending) {
-    if (y0 > s.y_pos && y0 < s.y_neg && yb > s.y_pos && yb < s.y_neg) return 0;
-  } else if (s.refractory_until > t1) {
-    return 0;
-  }
+typedef __attribute__((address_space(1))) CrossingEvent *GlobalEvents;
+// The reference's pieces of a cell with a crossing, a hold end or a pending
+// reset; y0 and yb are y at t0 and t1.
+__device__ __forceinline__ int cell_pieces(InverseRegs &s, const CircuitParams &p, double rate, double t0, double t1,
+                                           double y0, double yb, GlobalEvents out, int capacity, NoiseStream &reset) {
   int count = 0;
   double cursor = t0, y = y0;
   while (cursor < t1) {
     const bool hold = cursor < s.refractory_until;
     if (!hold && s.reset_pending) {
-      if (p.reset_noise_std > 0) s.set_reference(s.reference_log + p.reset_noise_std * normal());
+      if (p.reset_noise_std > 0) s.set_reference(s.reference_log + p.reset_noise_std * reset.normal());
       s.reset_pending = 0;
     }
     if (hold) {
@@ -326,7 +365,7 @@ __device__ inline int advance_cell_inverse(InverseRegs &s, const CircuitParams &
       cursor = end;
       double ref = -log(y);
       if ((cursor < t1 || cursor == s.refractory_until) && cursor >= s.refractory_until && s.reset_pending) {
-        if (p.reset_noise_std > 0) ref += p.reset_noise_std * normal();
+        if (p.reset_noise_std > 0) ref += p.reset_noise_std * reset.normal();
         s.reset_pending = 0;
       }
       s.set_reference(ref);
@@ -363,87 +402,156 @@ __device__ inline int advance_cell_inverse(InverseRegs &s, const CircuitParams &
   return count;
 }
 
-// Visits cells [c_begin, c_end) with their photocurrent rates[(c - c_begin) *
-// stride], loading the next kCellPrefetch cells while the current ones are
-// integrated: a free-running cell is far shorter than an HBM round trip, so
-// one wave needs that many loads in flight.  step(rate, c) returns < 0 to stop.
-constexpr int kCellPrefetch = 8;
-template <class Step>
-__device__ __forceinline__ int for_each_cell(const double *rates, size_t stride, int c_begin, int c_end, Step &&step) {
-  double cur[kCellPrefetch], nxt[kCellPrefetch];
-#pragma unroll
-  for (int j = 0; j < kCellPrefetch; ++j) cur[j] = c_begin + j < c_end ? rates[(size_t)j * stride] : 0.0;
-  for (int cb = c_begin; cb < c_end; cb += kCellPrefetch) {
-#pragma unroll
-    for (int j = 0; j < kCellPrefetch; ++j) {
-      const int c = cb + kCellPrefetch + j;
-      nxt[j] = c < c_end ? rates[(size_t)(c - c_begin) * stride] : 0.0;
-    }
-#pragma unroll
-    for (int j = 0; j < kCellPrefetch; ++j) {
-      if (cb + j < c_end) {
-        const int r = step(cur[j], cb + j);
-        if (r < 0) return r;
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < kCellPrefetch; ++j) cur[j] = nxt[j];
+__device__ __forceinline__ int advance_cell_inverse(InverseRegs &s, const CircuitParams &p, double rate,
+                                                   const CellMaps &m, GlobalEvents out, int capacity,
+                                                   NoiseStream &reset) {
+  const double t0 = m.t0, t1 = m.t1;
+  if (t1 <= t0) return 0;
+  rate = fmax(0.0, rate);
+  const double y0 = s.y, yb = inverse_map_cell(y0, rate, t1 - t0, p.charge_scale, m);
+  s.y = yb;
+  if (t0 >= s.refractory_until && !s.reset_pending) {
+    if (y0 > s.y_pos && y0 < s.y_neg && yb > s.y_pos && yb < s.y_neg) return 0;
+  } else if (s.refractory_until > t1) {
+    return 0;
   }
-  return 0;
+  return cell_pieces(s, p, rate, t0, t1, y0, yb, out, capacity, reset);
 }
 
+// Global-memory views of the circuit's arrays.  Through generic pointers the
+// compiler emits flat loads, whose waits also drain every prefetched load.
+typedef const __attribute__((address_space(1))) double *GlobalDoubles;
+
+// The value of x held by lane `lane` (wave-uniform) of the wave.
+__device__ __forceinline__ double from_lane(double x, int lane) {
+  const long long b = __double_as_longlong(x);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane), lo = __builtin_amdgcn_readlane((int)b, lane);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Photocurrent of cells [c_begin, c_end) at rates[(c - c_begin) * stride],
+// loaded kCellPrefetch cells ahead of the circuit: a free-running cell is far
+// shorter than an HBM round trip, so one wave needs that many loads in flight.
+// The CellMaps of those cells are the same for every lane: the wave loads
+// their kCellPrefetch * kCellWords doubles as two coalesced words per lane
+// and broadcasts them with readlane.  Every load is issued unconditionally
+// (from a clamped, valid address) one block ahead, so the compiler's waits
+// before a block cover only the loads of the block before.
+constexpr int kCellPrefetch = 8;
+static_assert(kCellPrefetch * kCellWords <= 128, "a block's maps fit two words per lane");
+
 // The circuit of one pixel over cells [c_begin, c_end): photocurrent of cell c
-// at rates[(c - c_begin) * stride], windows [t0[c], t1[c]], crossings appended
-// at out[count...] up to `capacity` in all.  Without leakage the inverse form
-// above runs and the state carries y; with it the reference's sequence runs
-// on u.  Returns the new count, -1 past capacity, -2 on a non-finite
-// photocurrent.
+// at rates[(c - c_begin) * stride], windows [t0[c], t1[c]] and their
+// cell_maps at maps[c], crossings appended at out[count...] up to `capacity`
+// in all.  Without leakage the inverse form above runs and the state carries
+// y; with it the reference's sequence runs on u.  Returns the new count, -1
+// past capacity, -2 on a non-finite photocurrent.  Every lane of the wave
+// calls it with the same cells (inactive lanes, past the segment's pixels,
+// with active = false and a valid rates pointer): the lanes share the maps'
+// loads.
 __device__ inline int circuit_cells(dvs_pixel_state &st, const CircuitParams &p, uint64_t seed, uint64_t pixel,
-                                    const double *rates, size_t stride, const double *t0, const double *t1,
-                                    int c_begin, int c_end, CrossingEvent *out, int count, int capacity) {
+                                    bool active, const double *rates, size_t stride, const double *t0,
+                                    const double *t1, const CellMaps *maps, int c_begin, int c_end,
+                                    CrossingEvent *out_events, int count, int capacity) {
+  const GlobalDoubles R = (GlobalDoubles)rates, T0 = (GlobalDoubles)t0, T1 = (GlobalDoubles)t1;
+  const GlobalEvents out = (GlobalEvents)out_events;
   NoiseStream reset(seed, pixel, kStreamReset, st.reset_block);
-  auto normal = [&]() { return reset.normal(); };
+  if (c_end <= c_begin) return count;
   if (p.reference_leak > 0) {
+    if (!active) return 0;
+    auto normal = [&]() { return reset.normal(); };
     PixelRegs s{st.receptor_log, st.reference_log, st.refractory_until, st.threshold_pos, st.threshold_neg,
                 st.reset_pending};
-    const int r = for_each_cell(rates, stride, c_begin, c_end, [&](double rate, int c) {
+    double next = R[0];
+    for (int c = c_begin; c < c_end; ++c) {
+      const double rate = next;
+      if (c + 1 < c_end) next = R[(size_t)(c + 1 - c_begin) * stride];
       if (!isfinite(rate)) return -2;
-      const int emitted = advance_pixel(s, p, rate, t0[c], t1[c], out + count, capacity - count, normal);
+      const int emitted = advance_pixel(s, p, rate, T0[c], T1[c], out_events + count, capacity - count, normal);
       if (emitted < 0) return -1;
       count += emitted;
-      return 0;
-    });
-    if (r < 0) return r;
+    }
     st.receptor_log = s.receptor_log;
     st.receptor_inverse = exp(-s.receptor_log);
     st.reference_log = s.reference_log;
     st.refractory_until = s.refractory_until;
     st.reset_pending = s.reset_pending;
-  } else {
-    InverseRegs s;
-    s.y = st.receptor_inverse;
-    s.refractory_until = st.refractory_until;
-    s.threshold_pos = st.threshold_pos;
-    s.threshold_neg = st.threshold_neg;
-    s.reset_pending = st.reset_pending;
-    s.set_reference(st.reference_log);
-    const int r = for_each_cell(rates, stride, c_begin, c_end, [&](double rate, int c) {
-      if (!isfinite(rate)) return -2;
-      const int emitted = advance_cell_inverse(s, p, rate, t0[c], t1[c], out + count, capacity - count, normal);
-      if (emitted < 0) return -1;
-      count += emitted;
-      return 0;
-    });
-    if (r < 0) return r;
-    if (c_end > c_begin) {
-      st.receptor_inverse = s.y;
-      st.receptor_log = -log(s.y);
-      // inside a hold the reference voltage follows the receptor
-      st.reference_log = s.refractory_until > t1[c_end - 1] ? st.receptor_log : s.reference_log;
-      st.refractory_until = s.refractory_until;
-      st.reset_pending = s.reset_pending;
-    }
+    st.reset_block = (uint32_t)reset.block;
+    return count;
   }
+  const CircuitParams pr = p;  // in registers: stores to the crossings may alias p
+  InverseRegs s;
+  s.y = st.receptor_inverse;
+  s.refractory_until = st.refractory_until;
+  s.threshold_pos = st.threshold_pos;
+  s.threshold_neg = st.threshold_neg;
+  s.reset_pending = st.reset_pending;
+  s.set_reference(st.reference_log);
+  int code = 0;
+  const int lane = (int)(threadIdx.x & 63);
+  const GlobalDoubles W = (GlobalDoubles)maps;  // CellMaps as words
+  const long long last_word = (long long)c_end * kCellWords - 1;
+  auto word = [&](int cb, int k) {
+    const long long w = (long long)cb * kCellWords + k;
+    return W[w < last_word ? w : last_word];
+  };
+  auto rate_of = [&](int c) { return R[active && c < c_end ? (size_t)(c - c_begin) * stride : 0]; };
+  double cur[kCellPrefetch], nxt[kCellPrefetch];
+#pragma unroll
+  for (int j = 0; j < kCellPrefetch; ++j) cur[j] = rate_of(c_begin + j);
+  double w0 = word(c_begin, lane), w1 = word(c_begin, 64 + lane);
+  for (int cb = c_begin; cb < c_end; cb += kCellPrefetch) {
+#pragma unroll
+    for (int j = 0; j < kCellPrefetch; ++j) nxt[j] = rate_of(cb + kCellPrefetch + j);
+    const double n0 = word(cb + kCellPrefetch, lane), n1 = word(cb + kCellPrefetch, 64 + lane);
+#pragma clang loop unroll(full)
+    for (int j = 0; j < kCellPrefetch; ++j) {
+      if (cb + j < c_end) {
+        double f[kCellWords];
+#pragma unroll
+        for (int k = 0; k < kCellWords; ++k) {
+          const int w = j * kCellWords + k;
+          f[k] = from_lane(w < 64 ? w0 : w1, w & 63);
+        }
+        CellMaps m;
+        m.t0 = f[0];
+        m.t1 = f[1];
+        m.d0 = f[2];
+#pragma unroll
+        for (int k = 0; k < kMappedElectrons; ++k) {
+          m.rate[k] = f[3 + k];
+          m.a[k] = f[3 + kMappedElectrons + k];
+          m.q[k] = f[3 + 2 * kMappedElectrons + k];
+        }
+        if (active) {
+          const double rate = cur[j];
+          if (!isfinite(rate)) {
+            code = -2;
+            active = false;
+          } else {
+            const int emitted = advance_cell_inverse(s, pr, rate, m, out + count, capacity - count, reset);
+            if (emitted < 0) {
+              code = -1;
+              active = false;
+            } else {
+              count += emitted;
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kCellPrefetch; ++j) cur[j] = nxt[j];
+    w0 = n0;
+    w1 = n1;
+  }
+  if (code < 0) return code;
+  st.receptor_inverse = s.y;
+  st.receptor_log = -log(s.y);
+  // inside a hold the reference voltage follows the receptor
+  st.reference_log = s.refractory_until > T1[c_end - 1] ? st.receptor_log : s.reference_log;
+  st.refractory_until = s.refractory_until;
+  st.reset_pending = s.reset_pending;
   st.reset_block = (uint32_t)reset.block;
   return count;
 }

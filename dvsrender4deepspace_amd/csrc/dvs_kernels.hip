@@ -233,14 +233,25 @@ __global__ __launch_bounds__(256) void circuit_multi(const SegDev *segs, const i
   const int si = segment_of<int>(block_begin, nseg, (int)blockIdx.x);
   const SegDev &g = segs[si];
   const int i = ((int)blockIdx.x - block_begin[si]) * 256 + threadIdx.x;
-  if (i >= g.npix) return;
-  dvs_pixel_state st = g.states[i];
-  const int count = circuit_cells(st, g.params, g.seed, (uint64_t)i, g.rad + i, (size_t)g.npix, g.t0, g.t1, 0,
-                                  g.cells, g.slots + (size_t)i * g.max_events, g.counts[i], g.max_events);
+  // lanes past the segment's pixels stay in the cell loop (they hold cells of
+  // the wave's shared tiles) without a pixel of their own
+  const bool own = i < g.npix;
+  dvs_pixel_state st = own ? g.states[i] : dvs_pixel_state{};
+  const int count = circuit_cells(st, g.params, g.seed, (uint64_t)i, own, g.rad + (own ? i : 0), (size_t)g.npix,
+                                  g.t0, g.t1, g.maps, 0, g.cells, g.slots + (size_t)(own ? i : 0) * g.max_events,
+                                  own ? g.counts[i] : 0, g.max_events);
+  if (!own) return;
   if (count < 0) { atomicOr(status, count == -1 ? 1 : 2); g.counts[i] = 0; return; }
   st.shot_block += (uint64_t)g.cells;
   g.states[i] = st;
   g.counts[i] = count;
+}
+
+__global__ void cell_maps_kernel(int cells, const double *t0, const double *t1, const int *cell_begin, int nseg,
+                                 const double *charge, CellMaps *out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cells) return;
+  out[c] = cell_maps(t0[c], t1[c], charge[segment_of<int>(cell_begin, nseg, c)]);
 }
 
 // Compaction: global pixel p's crossings go to [ends[p] - count, ends[p]);
@@ -463,6 +474,12 @@ hipError_t launch_trace_cells_multi(const SegDev *segs, const long long *block_b
   if (blocks <= 0) return hipSuccess;
   if (blocks > kMaxBlocks256) return hipErrorInvalidConfiguration;
   trace_cells_multi<<<(unsigned)blocks, 256, lds, st>>>(segs, block_begin, nseg, status, shadow_count);
+  return hipGetLastError();
+}
+hipError_t launch_cell_maps(int cells, const double *t0, const double *t1, const int *cell_begin, int nseg,
+                            const double *charge, CellMaps *out, hipStream_t st) {
+  if (cells <= 0) return hipSuccess;
+  cell_maps_kernel<<<(unsigned)((cells + 255) / 256), 256, 0, st>>>(cells, t0, t1, cell_begin, nseg, charge, out);
   return hipGetLastError();
 }
 size_t cell_rates_flags(int items, int cells) {

@@ -435,6 +435,19 @@ class Engine {
     cblock_.upload(cblocks.data(), cblocks.size(), "segment blocks", st);
     pbegin_.upload(pbegin.data(), pbegin.size(), "segment pixels", st);
     circuit_blocks_ = cblocks[nseg];
+    // whole-cell receptor maps of every cell, for each segment's charge scale
+    std::vector<int> cell_begin(nseg + 1);
+    std::vector<double> charge(nseg);
+    for (int i = 0; i < nseg; ++i) {
+      cell_begin[i] = work[i].cell_offset;
+      charge[i] = work[i].seg->params.charge_scale;
+    }
+    cell_begin[nseg] = cell_total;
+    cellbeg_.upload(cell_begin.data(), cell_begin.size(), "segment cells", st);
+    charge_.upload(charge.data(), charge.size(), "segment charge", st);
+    maps_.reserve(cell_total, "cell maps");
+    hip_check(launch_cell_maps(cell_total, cell_t0_.ptr, cell_t1_.ptr, cellbeg_.ptr, nseg, charge_.ptr, maps_.ptr, st),
+              "cell maps");
     return lds;
   }
 
@@ -447,6 +460,7 @@ class Engine {
     d.iv_count = (int)w.ivs.size();
     d.t0 = cell_t0_.ptr + w.cell_offset + c0;
     d.t1 = cell_t1_.ptr + w.cell_offset + c0;
+    d.maps = maps_.ptr + w.cell_offset + c0;
     d.poses = poses_.ptr + w.cell_offset + c0;
     d.cells = c1 - c0;
     d.rad = rad;
@@ -732,7 +746,6 @@ class Engine {
     accepted.assign(nseg, 1);
     std::vector<SegDev> tables(nseg);
     std::vector<AdaptDev> ad(nseg);
-    std::vector<int> cell_begin(nseg + 1);
     for (int i = 0; i < nseg; ++i) {
       StepSegment &w = work[i];
       Segment &s = *w.seg;
@@ -747,12 +760,9 @@ class Engine {
       a.capacity = s.capacity;
       a.accepted = 1;
       a.cell_offset = w.cell_offset;
-      cell_begin[i] = w.cell_offset;
     }
-    cell_begin[nseg] = cell_total;
     segtab_.upload(tables.data(), tables.size(), "segment table", st);
     adapt_.upload(ad.data(), ad.size(), "adaptive windows", st);
-    cellbeg_.upload(cell_begin.data(), cell_begin.size(), "segment cells", st);
     hip_check(launch_fill_int(counts_.ptr, 0, P, st), "clear events");
     hip_check(launch_cell_poses(cell_total, cell_t0_.ptr, cell_t1_.ptr, cell_iv_.ptr, intervals_.ptr, poses_.ptr, st),
               "cell poses");
@@ -1111,7 +1121,8 @@ class Engine {
   DeviceArray<long long> tblock_;
   DeviceArray<int> cblock_, pbegin_, counts_, ends_, status_, cell_iv_, ev_seg_, ev_pixel_, ev_pol_, events_seg_;
   DeviceArray<dvs_optical_interval> intervals_;
-  DeviceArray<double> cell_t0_, cell_t1_, radiance_, ev_time_;
+  DeviceArray<double> cell_t0_, cell_t1_, radiance_, ev_time_, charge_;
+  DeviceArray<CellMaps> maps_;
   DeviceArray<Pose> poses_, ev_poses_;
   DeviceArray<float> ev_depth_;
   DeviceArray<unsigned long long> keys_, keys2_, counter_;

@@ -18,6 +18,7 @@ struct SegDev {
   DevScene scene;
   const dvs_optical_interval *ivs;  // this step's intervals of the segment
   const double *t0, *t1;            // this step's circuit cells of the segment
+  const CellMaps *maps;             // their whole-cell receptor maps
   const Pose *poses;                // cell midpoint poses
   double *rad;                      // [cells][npix] radiance of this step
   dvs_pixel_state *states;
@@ -45,6 +46,7 @@ __device__ __forceinline__ int segment_of(const T *begin, int nseg, T b) {
 }
 
 namespace dvsamd {
+using dvsdev::CellMaps;
 using dvsdev::CircuitParams;
 using dvsdev::CrossingEvent;
 using dvsdev::DevScene;
@@ -56,6 +58,10 @@ using dvsdev::SegDev;
 // offsets (in blocks) of each segment's work; the grids are their totals.
 hipError_t launch_trace_cells_multi(const SegDev *segs, const long long *block_begin, int nseg, long long blocks,
                                     size_t lds, int *status, unsigned long long *shadow_count, hipStream_t st);
+// cell_maps of every cell of a step; cell_begin holds nseg+1 prefix offsets
+// of the segments' cells and charge their charge scales.
+hipError_t launch_cell_maps(int cells, const double *t0, const double *t1, const int *cell_begin, int nseg,
+                            const double *charge, CellMaps *out, hipStream_t st);
 // Radiance -> photocurrent in place over `items` table items (prefix
 // item_begin) and up to `cells` cells (at most 65535).  `rare` holds
 // cell_rates_flags(items, cells) per-wave flags of the second phase.
