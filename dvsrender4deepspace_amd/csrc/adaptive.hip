@@ -492,6 +492,10 @@ __global__ __launch_bounds__(256) void adapt_integrate(const SegDev *segs, const
   const AdaptDev &A = ad[si];
   const int i = ((int)blockIdx.x - block_begin[si]) * 256 + threadIdx.x;
   if (!A.accepted) return;  // the whole block: blocks do not straddle segments
+  // The serial circuit is latency bound at one or two waves per SIMD while
+  // the next chunk's photocurrent kernel fills the SIMDs beside it: its waves
+  // issue first.
+  __builtin_amdgcn_s_setprio(3);
   // lanes past the segment's pixels stay in the cell loop (they hold cells of
   // the wave's shared tiles) without a pixel of their own
   const bool own = i < g.npix;

@@ -292,40 +292,46 @@ __device__ __forceinline__ double inverse_map(double y, double rate, double dt, 
 }
 
 // One circuit cell as the inverse circuit reads it: its window and its
-// whole-cell maps for the photocurrents shot noise makes common there, zero
-// (y + d0) and k = 1, 2 electrons (rate k / dt, the value the photocurrent
-// kernels write).  They depend on the cell alone, so one small kernel
-// evaluates them for every pixel, with the operations of inverse_map; a pixel
-// whose photocurrent is none of these evaluates its own.
+// whole-cell maps y' = a y + q for the photocurrents shot noise makes common
+// there: map[0] = {1, dt / C} for zero (1 y = y exactly, so this is
+// inverse_map's y + dt / C) and map[k] for k = 1, 2 electrons (rate[k - 1] =
+// k / dt, the value the photocurrent kernels write).  They depend on the cell
+// alone, so one small kernel evaluates them for every pixel, with the
+// operations of inverse_map; a pixel whose photocurrent is none of these
+// evaluates its own.  A cell without length gets a NaN map[0], which no
+// range test passes.
 constexpr int kMappedElectrons = 2;
 struct CellMaps {
-  double t0, t1, d0;
-  double rate[kMappedElectrons], a[kMappedElectrons], q[kMappedElectrons];
+  double t0, t1;
+  double rate[kMappedElectrons];
+  double map[kMappedElectrons + 1][2];
 };
-constexpr int kCellWords = 3 + 3 * kMappedElectrons;  // doubles per CellMaps
+constexpr int kCellWords = 4 + 2 * (kMappedElectrons + 1);  // doubles per CellMaps
 static_assert(sizeof(CellMaps) == kCellWords * sizeof(double), "CellMaps is read as words");
+static_assert(kCellWords % 2 == 0, "CellMaps entries stay 16-byte aligned");
 __device__ __forceinline__ CellMaps cell_maps(double t0, double t1, double charge) {
   CellMaps m;
   const double dt = t1 - t0;
   m.t0 = t0;
   m.t1 = t1;
-  m.d0 = dt / charge;
+  m.map[0][0] = dt > 0 ? 1.0 : __builtin_nan("");
+  m.map[0][1] = dt / charge;
 #pragma unroll
   for (int k = 0; k < kMappedElectrons; ++k) {
     m.rate[k] = (double)(k + 1) / dt;
     const InverseFactors f = inverse_factors(m.rate[k], dt, charge);
-    m.a[k] = f.a;
-    m.q[k] = f.q;
+    m.map[k + 1][0] = f.a;
+    m.map[k + 1][1] = f.q;
   }
   return m;
 }
 // inverse_map over the whole cell, dt = t1 - t0 > 0, rate >= 0.
 __device__ __forceinline__ double inverse_map_cell(double y, double rate, double dt, double charge,
                                                    const CellMaps &m) {
-  if (rate == 0) return y + m.d0;
+  if (rate == 0) return y + m.map[0][1];
 #pragma unroll
   for (int k = 0; k < kMappedElectrons; ++k)
-    if (rate == m.rate[k]) return m.a[k] * y + m.q[k];
+    if (rate == m.rate[k]) return m.map[k + 1][0] * y + m.map[k + 1][1];
   const InverseFactors f = inverse_factors(rate, dt, charge);
   return f.a * y + f.q;
 }
@@ -433,22 +439,29 @@ __device__ __forceinline__ double from_lane(double x, int lane) {
 // loaded kCellPrefetch cells ahead of the circuit: a free-running cell is far
 // shorter than an HBM round trip, so one wave needs that many loads in flight.
 // The CellMaps of those cells are the same for every lane: the wave loads
-// their kCellPrefetch * kCellWords doubles as two coalesced words per lane
-// and broadcasts them with readlane.  Every load is issued unconditionally
-// (from a clamped, valid address) one block ahead, so the compiler's waits
+// their kCellPrefetch * kCellWords doubles as two coalesced words per lane,
+// one block ahead, and stages them in LDS for the block.  Every global load
+// is issued unconditionally (from a clamped, valid address), so the waits
 // before a block cover only the loads of the block before.
 constexpr int kCellPrefetch = 8;
-static_assert(kCellPrefetch * kCellWords <= 128, "a block's maps fit two words per lane");
+constexpr int kBlockWords = kCellPrefetch * kCellWords;
+static_assert(kBlockWords <= 128, "a block's maps fit two words per lane");
 
 // The circuit of one pixel over cells [c_begin, c_end): photocurrent of cell c
-// at rates[(c - c_begin) * stride], windows [t0[c], t1[c]] and their
-// cell_maps at maps[c], crossings appended at out[count...] up to `capacity`
-// in all.  Without leakage the inverse form above runs and the state carries
-// y; with it the reference's sequence runs on u.  Returns the new count, -1
-// past capacity, -2 on a non-finite photocurrent.  Every lane of the wave
-// calls it with the same cells (inactive lanes, past the segment's pixels,
-// with active = false and a valid rates pointer): the lanes share the maps'
-// loads.
+// at rates[(c - c_begin) * stride], windows and maps at maps[c] (cell_maps),
+// crossings appended at out[count...] up to `capacity` in all.  Without
+// leakage the inverse form above runs and the state carries y; with it the
+// reference's sequence runs on u (t0, t1 hold the windows).  Returns the new
+// count, -1 past capacity, -2 on a non-finite photocurrent.  Every lane of a
+// 256-thread block calls it with the same cells (inactive lanes, past the
+// segment's pixels, with active = false and a valid rates pointer): the lanes
+// share the maps' loads.
+//
+// A block of kCellPrefetch cells first runs in bulk: without branches, every
+// cell is its tabulated map and the comparator's range test.  A lane whose
+// block has no crossing, no hold end, no pending reset and tabulated
+// photocurrents only (nearly all of them) keeps that result, which is what
+// the cell-by-cell path gives; the others run the block again cell by cell.
 __device__ inline int circuit_cells(dvs_pixel_state &st, const CircuitParams &p, uint64_t seed, uint64_t pixel,
                                     bool active, const double *rates, size_t stride, const double *t0,
                                     const double *t1, const CellMaps *maps, int c_begin, int c_end,
@@ -479,6 +492,8 @@ __device__ inline int circuit_cells(dvs_pixel_state &st, const CircuitParams &p,
     st.reset_block = (uint32_t)reset.block;
     return count;
   }
+  __shared__ __attribute__((aligned(16))) double block_words[4][2 * 64];  // one block's CellMaps per wave
+  double *L = block_words[threadIdx.x >> 6];
   const CircuitParams pr = p;  // in registers: stores to the crossings may alias p
   InverseRegs s;
   s.y = st.receptor_inverse;
@@ -504,26 +519,50 @@ __device__ inline int circuit_cells(dvs_pixel_state &st, const CircuitParams &p,
 #pragma unroll
     for (int j = 0; j < kCellPrefetch; ++j) nxt[j] = rate_of(cb + kCellPrefetch + j);
     const double n0 = word(cb + kCellPrefetch, lane), n1 = word(cb + kCellPrefetch, 64 + lane);
-#pragma clang loop unroll(full)
+    // stage this block's maps; LDS operations of a wave complete in order
+    __builtin_amdgcn_wave_barrier();
+    L[lane] = w0;
+    L[64 + lane] = w1;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int n = min(kCellPrefetch, c_end - cb);
+    // bulk pass, in phases so that each phase's LDS reads are in flight
+    // together: the tabulated photocurrents, the chosen maps, the y chain
+    const double y0 = s.y;
+    double y = y0;
+    bool ok = active, in_range = y0 > s.y_pos && y0 < s.y_neg;
+    double2 r[kCellPrefetch], aq[kCellPrefetch];
+#pragma unroll
+    for (int j = 0; j < kCellPrefetch; ++j) r[j] = *reinterpret_cast<const double2 *>(L + j * kCellWords + 2);
+    int k[kCellPrefetch];
+#pragma unroll
     for (int j = 0; j < kCellPrefetch; ++j) {
-      if (cb + j < c_end) {
-        double f[kCellWords];
+      const double rate = cur[j];
+      const bool one = rate == r[j].x, two = rate == r[j].y;
+      k[j] = one ? 1 : two ? 2 : 0;
+      ok = ok && (j >= n || one || two || rate == 0);
+    }
 #pragma unroll
-        for (int k = 0; k < kCellWords; ++k) {
-          const int w = j * kCellWords + k;
-          f[k] = from_lane(w < 64 ? w0 : w1, w & 63);
-        }
-        CellMaps m;
-        m.t0 = f[0];
-        m.t1 = f[1];
-        m.d0 = f[2];
+    for (int j = 0; j < kCellPrefetch; ++j) aq[j] = *reinterpret_cast<const double2 *>(L + j * kCellWords + 4 + 2 * k[j]);
 #pragma unroll
-        for (int k = 0; k < kMappedElectrons; ++k) {
-          m.rate[k] = f[3 + k];
-          m.a[k] = f[3 + kMappedElectrons + k];
-          m.q[k] = f[3 + 2 * kMappedElectrons + k];
-        }
-        if (active) {
+    for (int j = 0; j < kCellPrefetch; ++j) {
+      if (j < n) {
+        y = aq[j].x * y + aq[j].y;
+        in_range = in_range && y > s.y_pos && y < s.y_neg;
+      }
+    }
+    const double first_t0 = L[0], last_t1 = L[(n - 1) * kCellWords + 1];
+    const bool free = first_t0 >= s.refractory_until && !s.reset_pending;
+    ok = ok && y == y && (free ? in_range : s.refractory_until > last_t1);
+    if (ok) {
+      s.y = y;
+    } else if (active) {
+      // cell by cell from the block's start
+#pragma clang loop unroll(full)
+      for (int j = 0; j < kCellPrefetch; ++j) {
+        if (j < n && active) {
+          const CellMaps m = *reinterpret_cast<const CellMaps *>(L + j * kCellWords);
           const double rate = cur[j];
           if (!isfinite(rate)) {
             code = -2;
