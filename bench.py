@@ -317,9 +317,10 @@ def request_mode(args) -> dict:
     # dominant kernel: the serial circuit (adapt_integrate), HIP-event timed
     launches = max(st["circuit_launches"], 1)
     circuit_s = st["circuit_ms"] / 1e3
+    state_bytes = capi.PIXEL_STATE_DTYPE.itemsize
     bytes_integrate = (npix * cells * 8            # photocurrent in, one double per pixel cell
-                       + st["circuit_launches"] * npix * (2 * 56 + 2 * 4)  # pixel state and counts in/out
-                       + st["comparator_events"] * 16)                      # crossings out
+                       + st["circuit_launches"] * npix * (2 * state_bytes + 2 * 4)  # pixel state and counts in/out
+                       + st["comparator_events"] * 16)                               # crossings out
     achieved = bytes_integrate / max(circuit_s, 1e-30) / 1e9
     result = {
         "metric": METRIC,
@@ -356,8 +357,9 @@ def request_mode(args) -> dict:
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": None, "avg_launch_ms": round(st["circuit_ms"] / launches, 3),
                      "bytes_per_launch": int(bytes_integrate / launches),
-                     "note": "one lane per pixel, cells in series (FP64 receptor chain): issue-bound at "
-                             f"{npix / 64:.0f} waves; photocurrent in is 8 B per pixel cell"},
+                     "note": "one lane per pixel, cells in series (FP64 inverse receptor): latency bound at "
+                             f"{npix / 64:.0f} waves beside the next chunk's photocurrent kernel; photocurrent "
+                             "in is 8 B per pixel cell"},
     }
     sensor.close()
 

@@ -132,8 +132,9 @@ class Config {
       return;
     }
     camera = {c.camera.width, c.camera.height, c.camera.fx, c.camera.fy, c.camera.cx, c.camera.cy};
-    std::memcpy(&physic, &c.physic, sizeof physic);
-    std::memcpy(&hapke, &c.hapke, sizeof hapke);
+    // same fields in the same order (see the static_asserts on their sizes)
+    std::memcpy(static_cast<void *>(&physic), &c.physic, sizeof physic);
+    std::memcpy(static_cast<void *>(&hapke), &c.hapke, sizeof hapke);
     scene.frame_count = c.scene.frame_count;
     scene.fps = c.scene.fps;
     scene.imu_fps = c.scene.imu_fps;
@@ -251,6 +252,8 @@ OpticalState to_optical_state(const State &s) {
   p.rotation = {s.quaternion.x(), s.quaternion.y(), s.quaternion.z(), s.quaternion.w()};
   return p;
 }
+// An optical state is already one (callers that build intervals directly).
+inline OpticalState to_optical_state(const OpticalState &s) { return s; }
 
 // OpticalRefinementStats / ReceptorProbeResult of the adaptive solver.
 using OpticalRefinementStats = dvs_refinement_stats;
