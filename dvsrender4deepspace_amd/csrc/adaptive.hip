@@ -484,7 +484,7 @@ __global__ __launch_bounds__(256) void adapt_rates_rare(const SegDev *segs, cons
 // Circuit of cells [c0, c1) of every pixel of an accepted window: the
 // photocurrent of a knot pixel from adapt_rates, of a fallback pixel from
 // its native trace; comparator and reset noise as in the native circuit.
-__global__ __launch_bounds__(256) void adapt_integrate(const SegDev *segs, const AdaptDev *ad, const int *block_begin,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void adapt_integrate(const SegDev *segs, const AdaptDev *ad, const int *block_begin,
                                                        int nseg, int pixels, AdaptBufs b, int c0, int c1,
                                                        int *status) {
   const int si = segment_of<int>(block_begin, nseg, (int)blockIdx.x);

@@ -506,23 +506,25 @@ __device__ inline int circuit_cells(dvs_pixel_state &st, const CircuitParams &p,
   const int lane = (int)(threadIdx.x & 63);
   const GlobalDoubles W = (GlobalDoubles)maps;  // CellMaps as words
   const long long last_word = (long long)c_end * kCellWords - 1;
-  auto word = [&](int cb, int k) {
-    const long long w = (long long)cb * kCellWords + k;
-    return W[w < last_word ? w : last_word];
-  };
-  auto rate_of = [&](int c) { return R[active && c < c_end ? (size_t)(c - c_begin) * stride : 0]; };
-  double cur[kCellPrefetch], nxt[kCellPrefetch];
+  // Clamped, branch-free addresses: every lane (inactive ones hold a valid
+  // rates pointer) loads in every block, so the compiler can count the loads
+  // in flight exactly.
+  auto word = [&](int cb, int k) { return W[min((long long)cb * kCellWords + k, last_word)]; };
+  auto rate_of = [&](int c) { return R[(size_t)(min(c, c_end - 1) - c_begin) * stride]; };
+  // One block of cells from cb: integrates it with the photocurrents `use`
+  // and map words u0/u1 (loaded one block earlier) while loading the next
+  // block's into `fill` and f0/f1.  The loop below alternates two register
+  // sets, so no value moves between them: a move would wait for the load.
+  auto block = [&](const double (&use)[kCellPrefetch], double u0, double u1, double (&fill)[kCellPrefetch],
+                   double &f0, double &f1, int cb) __attribute__((always_inline)) {
 #pragma unroll
-  for (int j = 0; j < kCellPrefetch; ++j) cur[j] = rate_of(c_begin + j);
-  double w0 = word(c_begin, lane), w1 = word(c_begin, 64 + lane);
-  for (int cb = c_begin; cb < c_end; cb += kCellPrefetch) {
-#pragma unroll
-    for (int j = 0; j < kCellPrefetch; ++j) nxt[j] = rate_of(cb + kCellPrefetch + j);
-    const double n0 = word(cb + kCellPrefetch, lane), n1 = word(cb + kCellPrefetch, 64 + lane);
+    for (int j = 0; j < kCellPrefetch; ++j) fill[j] = rate_of(cb + kCellPrefetch + j);
+    f0 = word(cb + kCellPrefetch, lane);
+    f1 = word(cb + kCellPrefetch, 64 + lane);
     // stage this block's maps; LDS operations of a wave complete in order
     __builtin_amdgcn_wave_barrier();
-    L[lane] = w0;
-    L[64 + lane] = w1;
+    L[lane] = u0;
+    L[64 + lane] = u1;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -538,7 +540,7 @@ __device__ inline int circuit_cells(dvs_pixel_state &st, const CircuitParams &p,
     int k[kCellPrefetch];
 #pragma unroll
     for (int j = 0; j < kCellPrefetch; ++j) {
-      const double rate = cur[j];
+      const double rate = use[j];
       const bool one = rate == r[j].x, two = rate == r[j].y;
       k[j] = one ? 1 : two ? 2 : 0;
       ok = ok && (j >= n || one || two || rate == 0);
@@ -562,8 +564,18 @@ __device__ inline int circuit_cells(dvs_pixel_state &st, const CircuitParams &p,
 #pragma clang loop unroll(full)
       for (int j = 0; j < kCellPrefetch; ++j) {
         if (j < n && active) {
-          const CellMaps m = *reinterpret_cast<const CellMaps *>(L + j * kCellWords);
-          const double rate = cur[j];
+          CellMaps m;
+          const double *w = L + j * kCellWords;
+          m.t0 = w[0];
+          m.t1 = w[1];
+#pragma unroll
+          for (int e = 0; e < kMappedElectrons; ++e) m.rate[e] = w[2 + e];
+#pragma unroll
+          for (int e = 0; e <= kMappedElectrons; ++e) {
+            m.map[e][0] = w[2 + kMappedElectrons + 2 * e];
+            m.map[e][1] = w[3 + kMappedElectrons + 2 * e];
+          }
+          const double rate = use[j];
           if (!isfinite(rate)) {
             code = -2;
             active = false;
@@ -579,10 +591,14 @@ __device__ inline int circuit_cells(dvs_pixel_state &st, const CircuitParams &p,
         }
       }
     }
+  };
+  double ra[kCellPrefetch], rb[kCellPrefetch];
 #pragma unroll
-    for (int j = 0; j < kCellPrefetch; ++j) cur[j] = nxt[j];
-    w0 = n0;
-    w1 = n1;
+  for (int j = 0; j < kCellPrefetch; ++j) ra[j] = rate_of(c_begin + j);
+  double a0 = word(c_begin, lane), a1 = word(c_begin, 64 + lane), b0 = 0, b1 = 0;
+  for (int cb = c_begin; cb < c_end; cb += 2 * kCellPrefetch) {
+    block(ra, a0, a1, rb, b0, b1, cb);
+    if (cb + kCellPrefetch < c_end) block(rb, b0, b1, ra, a0, a1, cb + kCellPrefetch);
   }
   if (code < 0) return code;
   st.receptor_inverse = s.y;

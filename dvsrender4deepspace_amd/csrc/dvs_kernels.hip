@@ -228,7 +228,7 @@ __global__ __launch_bounds__(256) void cell_rates_rare(const SegDev *segs, const
 
 // The circuit half of generate_pixels (event_sensor.cu:39-49): one lane per
 // pixel integrates all cells of this step in order.
-__global__ __launch_bounds__(256) void circuit_multi(const SegDev *segs, const int *block_begin, int nseg,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void circuit_multi(const SegDev *segs, const int *block_begin, int nseg,
                                                      int *status) {
   const int si = segment_of<int>(block_begin, nseg, (int)blockIdx.x);
   const SegDev &g = segs[si];
