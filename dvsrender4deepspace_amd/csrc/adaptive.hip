@@ -428,7 +428,7 @@ __device__ __forceinline__ bool adapt_rates_run(const SegDev *segs, const AdaptD
   // of the previous chunk may be updating the state while this runs
   const uint64_t serial0 = A.cell_base;
   double *out = b.rates + gp;
-  uint64_t quad[4] = {0, 0, 0, 0}, quad_index = ~0ull;
+  uint64_t shared[4] = {0, 0, 0, 0}, shared_index = ~0ull;
   bool marked = false;
   for (int c = cb; c < ce; ++c) {
     const double f = frac[c];
@@ -439,11 +439,11 @@ __device__ __forceinline__ bool adapt_rates_run(const SegDev *segs, const AdaptD
     light = fmax(0.0, light);
     if (!isfinite(light)) atomicOr(status, 2);
     const uint64_t serial = serial0 + (uint64_t)c;
-    if (g.photo.shot_noise && serial >> 2 != quad_index) {
-      quad_index = serial >> 2;
-      shot_quad(g.seed, (uint64_t)i, quad_index, quad);
+    if (g.photo.shot_noise && serial / kShotCellsPerBlock != shared_index) {
+      shared_index = serial / kShotCellsPerBlock;
+      shot_block(g.seed, (uint64_t)i, shared_index, shared);
     }
-    const double rate = cell_rate_first(light, g.photo, g.t1[c] - g.t0[c], serial, quad);
+    const double rate = cell_rate_first(light, g.photo, g.t1[c] - g.t0[c], serial, shared);
     out[(size_t)(c - c0) * pixels] = rate;
     marked |= rate < 0;
   }

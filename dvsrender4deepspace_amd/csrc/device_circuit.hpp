@@ -41,9 +41,11 @@ __device__ __forceinline__ double receptor_log_at(double u, double rate, double 
 // fresh block.  Shot noise is keyed per circuit cell: the draw of a pixel's
 // n-th cell since initialisation uses word = n and blocks 0, 1, ... of its
 // own, so it can be drawn by any thread at any time.  The inversion branch
-// (lambda < 10, one uniform) instead takes word n mod 4 of block n / 4 of the
-// pixel's shared shot stream (word 0), so four consecutive cells share one
-// Philox block.  Reset noise (word 0) advances a per-pixel block counter, as
+// (lambda < 10, one uniform) instead takes a 32-bit uniform: half n mod 2 of
+// word (n / 2) mod 4 of block n / 8 of the pixel's shared shot stream (word
+// 0), so eight consecutive cells share one Philox block.  (The inversion
+// resolves no probability near 2^-32 below lambda = 10.)  Reset noise (word
+// 0) advances a per-pixel block counter, as
 // its draws occur at crossings.  Batching, launch shape and other requests
 // cannot change any of these streams.
 // ---------------------------------------------------------------------------
@@ -65,13 +67,18 @@ __device__ __forceinline__ void philox4x64(uint64_t c0, uint64_t c1, uint64_t c2
 __device__ __forceinline__ double uniform_of(uint64_t x) {  // open interval (0, 1), 53 bits
   return ((double)(x >> 11) + 0.5) * (1.0 / 9007199254740992.0);
 }
-// The shared shot-noise block of cells 4 quad .. 4 quad + 3.
-__device__ __forceinline__ void shot_quad(uint64_t seed, uint64_t pixel, uint64_t quad, uint64_t out[4]) {
-  philox4x64(quad, pixel, 0, 0, seed, kStreamTag ^ kStreamShotShared, out);
+// The shared shot-noise block of cells 8 block .. 8 block + 7
+// (block = cell serial / kShotCellsPerBlock).
+constexpr int kShotCellsPerBlock = 8;
+__device__ __forceinline__ void shot_block(uint64_t seed, uint64_t pixel, uint64_t block, uint64_t out[4]) {
+  philox4x64(block, pixel, 0, 0, seed, kStreamTag ^ kStreamShotShared, out);
 }
-__device__ __forceinline__ uint64_t quad_word(const uint64_t w[4], uint64_t serial) {
-  const int j = (int)(serial & 3);  // selected without a dynamic index so the block stays in registers
-  return j == 0 ? w[0] : j == 1 ? w[1] : j == 2 ? w[2] : w[3];
+// The inversion uniform of cell `serial` in its shared block w.
+__device__ __forceinline__ double shot_uniform(const uint64_t w[4], uint64_t serial) {
+  const int j = (int)((serial >> 1) & 3);  // selected without a dynamic index so the block stays in registers
+  const uint64_t x = j == 0 ? w[0] : j == 1 ? w[1] : j == 2 ? w[2] : w[3];
+  const uint32_t h = (serial & 1) ? (uint32_t)(x >> 32) : (uint32_t)x;
+  return ((double)h + 0.5) * (1.0 / 4294967296.0);
 }
 // Sequential-search inversion of Poisson(lambda), 0 < lambda < 10, from one
 // uniform u (Devroye 1986, ch. X.3).
@@ -138,15 +145,16 @@ __device__ inline double shot_stream_draw(uint64_t seed, uint64_t pixel, uint64_
 // phases.  cell_rate_first decides every cell whose draw is the inversion
 // branch (or needs no draw) and returns -lambda (< 0) for a cell whose draw
 // needs the cell's own stream (lambda >= 10); cell_rate_rest finishes it.
-// `quad` is shot_quad(seed, pixel, cell_serial / 4) (unused without noise).
+// `shared` is shot_block(seed, pixel, cell_serial / kShotCellsPerBlock)
+// (unused without noise).
 __device__ __forceinline__ double cell_rate_first(double light, const PhotoParams &ph, double dt, uint64_t cell_serial,
-                                                  const uint64_t quad[4]) {
+                                                  const uint64_t shared[4]) {
   const double rate = light * ph.electrons_per_radiance + ph.dark_electrons;
   if (!ph.shot_noise) return rate;
   const double lambda = rate * dt;
   if (lambda != lambda) return lambda;  // NaN photocurrent: the circuit reports it
   if (lambda <= 0) return 0.0 / dt;
-  if (lambda < 10) return poisson_inversion(lambda, uniform_of(quad_word(quad, cell_serial))) / dt;
+  if (lambda < 10) return poisson_inversion(lambda, shot_uniform(shared, cell_serial)) / dt;
   return -lambda;
 }
 __device__ inline double cell_rate_rest(double marked, double dt, uint64_t seed, uint64_t pixel, uint64_t cell_serial) {

@@ -643,9 +643,9 @@ enum { kStreamShot = 0, kStreamThreshold = 1, kStreamReset = 2, kStreamShotShare
 constexpr uint64_t kStreamTag = 0x4456535F4E4F4953ULL;  // key word 1 = tag ^ stream
 // Counter (block, pixel, word, 0).  Shot noise: one stream per circuit cell,
 // word = the pixel's cell serial n since initialisation, blocks from 0,
-// except that the inversion branch (lambda < 10) takes its one uniform from
-// word n mod 4 of block n / 4 of the shared shot stream (word 0); reset and
-// threshold noise: word 0 and a per-pixel block counter.
+// except that the inversion branch (lambda < 10) takes one 32-bit uniform:
+// half n mod 2 of word (n / 2) mod 4 of block n / 8 of the shared shot stream
+// (word 0); reset and threshold noise: word 0 and a per-pixel block counter.
 struct Stream {
   uint64_t seed, pixel, stream;
   uint64_t *block;  // next unused block index for this stream
@@ -693,9 +693,11 @@ static double cell_rate(double light, const orc_photo_params &ph, double dt, uin
   if (ph.shot_noise) {
     uint64_t block = 0;
     Stream shot{seed, pixel, kStreamShot, &block, serial};
-    uint64_t ctr[4] = {serial >> 2, pixel, 0, 0}, key[2] = {seed, kStreamTag ^ kStreamShotShared}, quad[4];
-    philox4x64(ctr, key, quad);
-    const double first = ((double)(quad[serial & 3] >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    uint64_t ctr[4] = {serial >> 3, pixel, 0, 0}, key[2] = {seed, kStreamTag ^ kStreamShotShared}, words[4];
+    philox4x64(ctr, key, words);
+    const uint64_t w = words[(serial >> 1) & 3];
+    const uint32_t half = (serial & 1) ? (uint32_t)(w >> 32) : (uint32_t)w;
+    const double first = ((double)half + 0.5) * (1.0 / 4294967296.0);
     rate = poisson_draw(shot, rate * dt, first) / dt;
   }
   return rate;
@@ -915,6 +917,10 @@ void orc_philox4x64(const uint64_t ctr[4], const uint64_t key[2], uint64_t out[4
 void orc_poisson_series(uint64_t seed, uint64_t pixel, double lambda, int n, double *out) {
   uint64_t block = 0; Stream st{seed, pixel, kStreamShot, &block};
   for (int i = 0; i < n; ++i) out[i] = poisson_draw(st, lambda);
+}
+void orc_shot_counts(uint64_t seed, uint64_t pixel, double lambda, uint64_t serial0, int n, double *out) {
+  const orc_photo_params ph{1.0, 0.0, 1.0, 1, 0};  // rate = light, dt = 1: the draw is the electron count
+  for (int i = 0; i < n; ++i) out[i] = cell_rate(lambda, ph, 1.0, seed, pixel, serial0 + (uint64_t)i);
 }
 void orc_normal_series(uint64_t seed, uint64_t pixel, int n, double *out) {
   uint64_t block = 0; Stream st{seed, pixel, kStreamReset, &block};

@@ -179,6 +179,9 @@ void orc_parameters(const orc_physic *c, int physic_effects, orc_pixel_params *p
 void orc_philox4x64(const uint64_t ctr[4], const uint64_t key[2], uint64_t out[4]);
 /* Draw `n` Poisson(lambda) variates for (seed, pixel) from block counter 0. */
 void orc_poisson_series(uint64_t seed, uint64_t pixel, double lambda, int n, double *out);
+/* Shot-noise electron counts of cells serial0 .. serial0 + n - 1 of one
+ * pixel at mean lambda, as the photocurrent draws them. */
+void orc_shot_counts(uint64_t seed, uint64_t pixel, double lambda, uint64_t serial0, int n, double *out);
 void orc_normal_series(uint64_t seed, uint64_t pixel, int n, double *out);
 
 /* ---- Sensor (src/cuda/event_sensor.cu) ---- */

@@ -111,6 +111,7 @@ def lib() -> ctypes.CDLL:
             "orc_philox4x64": (None, [_P, _P, _P]),
             "orc_poisson_series": (None, [ctypes.c_uint64, ctypes.c_uint64, D, I, _P]),
             "orc_normal_series": (None, [ctypes.c_uint64, ctypes.c_uint64, I, _P]),
+            "orc_shot_counts": (None, [ctypes.c_uint64, ctypes.c_uint64, D, ctypes.c_uint64, I, _P]),
             "orc_sensor_create": (_P, [_P, I, ctypes.POINTER(Physic), I, I]),
             "orc_sensor_destroy": (None, [_P]),
             "orc_sensor_initialize": (I, [_P, _P, _P, D, I]),
@@ -317,6 +318,12 @@ def philox4x64(ctr, key):
 def poisson_series(seed, pixel, lam, n):
     out = np.empty(n, np.float64)
     lib().orc_poisson_series(seed, pixel, lam, n, ptr(out))
+    return out
+
+
+def shot_counts(seed, pixel, lam, serial0, n):
+    out = np.empty(n, np.float64)
+    lib().orc_shot_counts(seed, pixel, lam, serial0, n, ptr(out))
     return out
 
 

@@ -225,6 +225,34 @@ def test_poisson_distribution_chi_square(oracle):
         assert chi2 < dof + 6 * math.sqrt(2 * max(dof, 1)) + 10
 
 
+@pytest.mark.parametrize("lam", [0.06, 3.0, 25.0])
+def test_cell_shot_counts_are_poisson(oracle, lam):
+    """Per-cell shot draws as the photocurrent keys them (consecutive cell
+    serials; below lambda = 10 eight cells share a Philox block, one 32-bit
+    half word each): Poisson moments, a chi-square test, and no correlation
+    between neighbouring cells, which share blocks."""
+    x = np.concatenate([oracle.shot_counts(11, p, lam, 1000 * p + 3, 1024) for p in range(64)])
+    n = len(x)
+    assert np.all(x == np.floor(x)) and x.min() >= 0
+    assert abs(x.mean() - lam) < 6 * math.sqrt(lam / n)
+    assert abs(x.var() - lam) < 6 * lam * math.sqrt(2 / n) + 6 * math.sqrt(lam / n)
+    kmax = int(lam + 6 * math.sqrt(lam) + 5)
+    obs = np.bincount(np.minimum(x.astype(int), kmax), minlength=kmax + 1)
+    pmf = np.array([math.exp(-lam + k * math.log(lam) - math.lgamma(k + 1)) for k in range(kmax + 1)])
+    pmf[-1] += 1 - pmf.sum()
+    exp = pmf * n
+    keep = exp > 5
+    chi2 = np.sum((obs[keep] - exp[keep]) ** 2 / exp[keep])
+    dof = keep.sum() - 1
+    assert chi2 < dof + 6 * math.sqrt(2 * max(dof, 1)) + 10
+    for lag in (1, 2, 8):
+        a = x.reshape(64, 1024)
+        r = np.corrcoef(a[:, :-lag].ravel(), a[:, lag:].ravel())[0, 1]
+        assert abs(r) < 6 / math.sqrt(n)
+    # the same serial gives the same count; other serials of a block differ
+    assert np.array_equal(oracle.shot_counts(11, 5, lam, 5003, 16), oracle.shot_counts(11, 5, lam, 5003, 16))
+
+
 def test_normal_moments(oracle):
     z = np.concatenate([oracle.normal_series(5, i, 256) for i in range(256)])
     assert abs(z.mean()) < 6 / math.sqrt(len(z)) and abs(z.var() - 1) < 0.03
