@@ -40,11 +40,18 @@ struct AdaptDev {
 };
 
 // An unresolved time segment [a, b] of a pixel (fractions of the window) with
-// its probe radiances and the number of left turns on its path.
+// its probe radiances and the number of left turns on its path.  lspec and
+// rspec are the radiances at the midpoints of its two halves when they were
+// traced ahead, with the queries of the bisection that made the segment
+// (negative otherwise): if the segment bisects, its halves need no trace.
 struct AdaptEntry {
   double a, b, left, mid, right;
+  double lspec, rspec;
   int pixel, zeros;
 };
+// Queries per traced bisection: the midpoints of the two halves and,
+// traced ahead, the midpoints of the halves' halves.
+constexpr int kBisectQueries = 6;
 // A resolved segment: accepted (two knots) or exact (one negative knot).
 struct AdaptTerm {
   double a, b, mid, right;
@@ -71,7 +78,7 @@ struct AdaptBufs {
 };
 
 struct AdaptCounters {
-  int terms, bisect, pad0, pad1;
+  int terms, bisect, direct, pad;  // resolved, traced and untraced bisections
 };
 
 // OpticalRefinementStats of one segment (include/dvs/optical_sampler.hpp:16).
@@ -102,10 +109,15 @@ hipError_t launch_adapt_trace(int layout, const SegDev *segs, const int *pixel_b
                               size_t lds, double *out, int *status, unsigned long long *shadow_count,
                               hipStream_t st);
 hipError_t launch_adapt_init(int pixels, AdaptBufs b, AdaptEntry *entries, hipStream_t st);
+// One level: resolved segments to terms; a bisected segment whose halves'
+// midpoints were traced ahead writes its halves to `direct` (ctr->direct
+// pairs), any other to `bisected` with kBisectQueries queries each
+// (ctr->bisect).
 hipError_t launch_adapt_decide(int n, const AdaptEntry *entries, const SegDev *segs, const AdaptDev *ad,
                                const int *pixel_begin, int nseg, AdaptBufs b, AdaptTerm *terms,
                                unsigned long long *keys, AdaptEntry *bisected, int *query_pixel, Pose *query_pose,
-                               AdaptCounters *ctr, hipStream_t st);
+                               AdaptEntry *direct, AdaptCounters *ctr, hipStream_t st);
+// The halves of the traced bisections, with their halves' midpoints.
 hipError_t launch_adapt_children(int n, const AdaptEntry *bisected, const double *query_rad, AdaptEntry *next,
                                  hipStream_t st);
 hipError_t launch_adapt_counts(int pixels, const AdaptDev *ad, const int *pixel_begin, int nseg, AdaptBufs b,

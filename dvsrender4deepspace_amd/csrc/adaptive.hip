@@ -104,8 +104,8 @@ __global__ void adapt_init(int pixels, AdaptBufs b, AdaptEntry *entries) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= pixels) return;
   const double *r = b.probes + (size_t)p * 5;
-  entries[2 * p] = AdaptEntry{0.0, .5, r[0], r[1], r[2], p, 1};
-  entries[2 * p + 1] = AdaptEntry{.5, 1.0, r[2], r[3], r[4], p, 0};
+  entries[2 * p] = AdaptEntry{0.0, .5, r[0], r[1], r[2], -1.0, -1.0, p, 1};
+  entries[2 * p + 1] = AdaptEntry{.5, 1.0, r[2], r[3], r[4], -1.0, -1.0, p, 0};
   b.bound[p] = 3;  // knot {0, r0} + two pending segments
   b.flags[p] = 0;
   b.bisects[p] = 0;
@@ -116,7 +116,7 @@ __global__ __launch_bounds__(256) void adapt_decide(int n, const AdaptEntry *ent
                                                     const AdaptDev *ad, const int *pixel_begin, int nseg,
                                                     AdaptBufs b, AdaptTerm *terms, unsigned long long *keys,
                                                     AdaptEntry *bisected, int *query_pixel, Pose *query_pose,
-                                                    AdaptCounters *ctr) {
+                                                    AdaptEntry *direct, AdaptCounters *ctr) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   int decision = -1;  // -1 dropped, ADAPT_ACCEPT, ADAPT_EXACT, 2 bisect
   AdaptEntry e{};
@@ -150,16 +150,20 @@ __global__ __launch_bounds__(256) void adapt_decide(int n, const AdaptEntry *ent
     }
   }
   const bool resolved = decision == ADAPT_ACCEPT || decision == ADAPT_EXACT;
+  const bool ahead = e.lspec >= 0;  // the halves' midpoints are already traced
   const unsigned long long tmask = __ballot(resolved);
-  const unsigned long long bmask = __ballot(decision == 2);
+  const unsigned long long bmask = __ballot(decision == 2 && !ahead);
+  const unsigned long long dmask = __ballot(decision == 2 && ahead);
   const int lane = threadIdx.x & 63;
-  int tbase = 0, bbase = 0;
+  int tbase = 0, bbase = 0, dbase = 0;
   if (lane == 0) {
     if (tmask) tbase = atomicAdd(&ctr->terms, __popcll(tmask));
     if (bmask) bbase = atomicAdd(&ctr->bisect, __popcll(bmask));
+    if (dmask) dbase = atomicAdd(&ctr->direct, __popcll(dmask));
   }
   tbase = __shfl(tbase, 0);
   bbase = __shfl(bbase, 0);
+  dbase = __shfl(dbase, 0);
   const unsigned long long below = (1ull << lane) - 1ull;
   if (resolved) {
     const int idx = tbase + __popcll(tmask & below);
@@ -169,17 +173,29 @@ __global__ __launch_bounds__(256) void adapt_decide(int n, const AdaptEntry *ent
     if (decision == ADAPT_ACCEPT) atomicAdd(&b.bound[e.pixel], 1);
     else atomicAdd(&b.exact_terms[e.pixel], 1);
   } else if (decision == 2) {
-    const int idx = bbase + __popcll(bmask & below);
-    bisected[idx] = e;
     atomicAdd(&b.bound[e.pixel], 1);
     atomicAdd(&b.bisects[e.pixel], 1);
-    const AdaptDev &A = ad[si];
-    const SegDev &g = segs[si];
     const double mid = (e.a + e.b) * .5;
-    query_pixel[2 * idx] = e.pixel;
-    query_pixel[2 * idx + 1] = e.pixel;
-    query_pose[2 * idx] = native_pose(g.ivs, g.iv_count, A.start + (A.end - A.start) * (e.a + mid) * .5);
-    query_pose[2 * idx + 1] = native_pose(g.ivs, g.iv_count, A.start + (A.end - A.start) * (mid + e.b) * .5);
+    if (ahead) {
+      const int idx = dbase + __popcll(dmask & below);
+      direct[2 * idx] = AdaptEntry{e.a, mid, e.left, e.lspec, e.mid, -1.0, -1.0, e.pixel, e.zeros + 1};
+      direct[2 * idx + 1] = AdaptEntry{mid, e.b, e.mid, e.rspec, e.right, -1.0, -1.0, e.pixel, e.zeros};
+    } else {
+      const int idx = bbase + __popcll(bmask & below);
+      bisected[idx] = e;
+      const AdaptDev &A = ad[si];
+      const SegDev &g = segs[si];
+      // the halves' midpoints, then the midpoints of their halves, with the
+      // operations a half's own bisection would use
+      const double l_mid = (e.a + mid) * .5, r_mid = (mid + e.b) * .5;
+      const double at[kBisectQueries] = {l_mid, r_mid, (e.a + l_mid) * .5, (l_mid + mid) * .5, (mid + r_mid) * .5,
+                                         (r_mid + e.b) * .5};
+#pragma unroll
+      for (int j = 0; j < kBisectQueries; ++j) {
+        query_pixel[kBisectQueries * idx + j] = e.pixel;
+        query_pose[kBisectQueries * idx + j] = native_pose(g.ivs, g.iv_count, A.start + (A.end - A.start) * at[j]);
+      }
+    }
   }
 }
 
@@ -190,8 +206,9 @@ __global__ void adapt_children(int n, const AdaptEntry *bisected, const double *
   if (k >= n) return;
   const AdaptEntry e = bisected[k];
   const double mid = (e.a + e.b) * .5;
-  next[2 * k] = AdaptEntry{e.a, mid, e.left, query_rad[2 * k], e.mid, e.pixel, e.zeros + 1};
-  next[2 * k + 1] = AdaptEntry{mid, e.b, e.mid, query_rad[2 * k + 1], e.right, e.pixel, e.zeros};
+  const double *q = query_rad + (size_t)kBisectQueries * k;
+  next[2 * k] = AdaptEntry{e.a, mid, e.left, q[0], e.mid, q[2], q[3], e.pixel, e.zeros + 1};
+  next[2 * k + 1] = AdaptEntry{mid, e.b, e.mid, q[1], e.right, q[4], q[5], e.pixel, e.zeros};
 }
 
 __global__ void adapt_counts(int pixels, const AdaptDev *ad, const int *pixel_begin, int nseg, AdaptBufs b) {
@@ -566,10 +583,10 @@ hipError_t launch_adapt_init(int pixels, AdaptBufs b, AdaptEntry *entries, hipSt
 hipError_t launch_adapt_decide(int n, const AdaptEntry *entries, const SegDev *segs, const AdaptDev *ad,
                                const int *pixel_begin, int nseg, AdaptBufs b, AdaptTerm *terms,
                                unsigned long long *keys, AdaptEntry *bisected, int *query_pixel, Pose *query_pose,
-                               AdaptCounters *ctr, hipStream_t st) {
+                               AdaptEntry *direct, AdaptCounters *ctr, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   adapt_decide<<<grid(n), 256, 0, st>>>(n, entries, segs, ad, pixel_begin, nseg, b, terms, keys, bisected,
-                                         query_pixel, query_pose, ctr);
+                                         query_pixel, query_pose, direct, ctr);
   return hipGetLastError();
 }
 hipError_t launch_adapt_children(int n, const AdaptEntry *bisected, const double *query_rad, AdaptEntry *next,

@@ -797,7 +797,10 @@ class Engine {
     hip_check(launch_adapt_init(P, b, ent_[0].ptr, st), "refine pixel optical launch");
     ctr_.reserve(1, "refinement counters");
     hip_check(hipMemsetAsync(ctr_.ptr, 0, sizeof(AdaptCounters), st), "clear refinement counters");
-    // Level-synchronous refinement.
+    // Level-synchronous refinement.  A traced bisection also traces the
+    // midpoints of its halves' halves, so the next level's bisections need
+    // no trace: the levels alternate between one ray launch and none, and
+    // every probe value is the one a bisection would trace itself.
     long long n = 2LL * P;
     int cur = 0, terms_total = 0, levels = 0;
     AdaptCounters host_ctr{};
@@ -806,25 +809,29 @@ class Engine {
       terms_.grow_keep((size_t)terms_total + n, terms_total, "resolved segments", st);
       tkeys_.grow_keep((size_t)terms_total + n, terms_total, "resolved segment keys", st);
       bis_.reserve(n, "bisected segments");
-      qpix_.reserve(2 * n, "optical queries");
-      qpose_.reserve(2 * n, "optical queries");
-      hip_check(hipMemsetAsync(&ctr_.ptr->bisect, 0, sizeof(int), st), "clear refinement counters");
+      qpix_.reserve((size_t)dvsdev::kBisectQueries * n, "optical queries");
+      qpose_.reserve((size_t)dvsdev::kBisectQueries * n, "optical queries");
+      ent_[1 - cur].reserve(2 * n, "refinement segments");
+      hip_check(hipMemsetAsync(&ctr_.ptr->bisect, 0, 2 * sizeof(int), st), "clear refinement counters");
       hip_check(launch_adapt_decide((int)n, ent_[cur].ptr, segtab_.ptr, adapt_.ptr, pbegin_.ptr, nseg, b, terms_.ptr,
-                                    tkeys_.ptr, bis_.ptr, qpix_.ptr, qpose_.ptr, ctr_.ptr, st),
+                                    tkeys_.ptr, bis_.ptr, qpix_.ptr, qpose_.ptr, ent_[1 - cur].ptr, ctr_.ptr, st),
                 "refine pixel optical launch");
       hip_check(hipMemcpyAsync(&host_ctr, ctr_.ptr, sizeof(host_ctr), hipMemcpyDeviceToHost, st), "refinement count");
       hip_check(hipStreamSynchronize(st), "refinement count");
       terms_total = host_ctr.terms;
-      const long long nb = host_ctr.bisect;
-      if (nb == 0) break;
-      qrad_.reserve(2 * nb, "optical queries");
-      hip_check(launch_adapt_trace(dvsdev::LAYOUT_QUERY, segtab_.ptr, pbegin_.ptr, nseg, samples, 2 * nb, qpix_.ptr,
-                                   nullptr, qpose_.ptr, lds, qrad_.ptr, status_.ptr, counter_.ptr, st),
-                "refine pixel optical launch");
-      ent_[1 - cur].reserve(2 * nb, "refinement segments");
-      hip_check(launch_adapt_children((int)nb, bis_.ptr, qrad_.ptr, ent_[1 - cur].ptr, st), "refine pixel optical launch");
+      const long long nb = host_ctr.bisect, nd = host_ctr.direct;
+      if (nb + nd == 0) break;
+      if (nb > 0) {
+        const long long queries = (long long)dvsdev::kBisectQueries * nb;
+        qrad_.reserve(queries, "optical queries");
+        hip_check(launch_adapt_trace(dvsdev::LAYOUT_QUERY, segtab_.ptr, pbegin_.ptr, nseg, samples, queries,
+                                     qpix_.ptr, nullptr, qpose_.ptr, lds, qrad_.ptr, status_.ptr, counter_.ptr, st),
+                  "refine pixel optical launch");
+        hip_check(launch_adapt_children((int)nb, bis_.ptr, qrad_.ptr, ent_[1 - cur].ptr + 2 * nd, st),
+                  "refine pixel optical launch");
+      }
       cur = 1 - cur;
-      n = 2 * nb;
+      n = 2 * (nb + nd);
     }
     // Knot counts, offsets and depth-first knot order.
     hip_check(launch_adapt_counts(P, adapt_.ptr, pbegin_.ptr, nseg, b, st), "pixel optical knots");
