@@ -34,8 +34,17 @@ EVENT_DTYPE = np.dtype([("timestamp", "<f8"), ("x", "<i2"), ("y", "<i2"), ("pola
 PIXEL_STATE_DTYPE = np.dtype([("receptor_log", "<f8"), ("reference_log", "<f8"), ("refractory_until", "<f8"),
                               ("threshold_pos", "<f8"), ("threshold_neg", "<f8"), ("reset_pending", "<i4"),
                               ("reset_block", "<u4"), ("shot_block", "<u8"), ("receptor_inverse", "<f8")])
+FRAME_DTYPE = np.dtype([("frame_index", "<i4"), ("timestamp_seconds", "<f8"), ("camera_origin", "<f4", (3,)),
+                        ("camera_world", "<f4", (9,)), ("light_dir", "<f4", (3,)), ("light_intensity", "<f4")],
+                       align=True)
+TRAJECTORY_STATE_DTYPE = np.dtype([("timestamp", "<f4"), ("position", "<f4", (3,)), ("orientation", "<f4", (9,)),
+                                   ("quaternion", "<f4", (4,)), ("linear_velocity", "<f4", (3,)),
+                                   ("angular_velocity", "<f4", (3,)), ("light_direction", "<f4", (3,)),
+                                   ("light_direction_rate", "<f4", (3,)), ("light_intensity", "<f4"),
+                                   ("light_intensity_rate", "<f4")])
 assert NODE_DTYPE.itemsize == 40 and STATE_DTYPE.itemsize == 72 and INTERVAL_DTYPE.itemsize == 160
 assert EVENT_DTYPE.itemsize == 17 and PIXEL_STATE_DTYPE.itemsize == 64
+assert FRAME_DTYPE.itemsize == 80 and TRAJECTORY_STATE_DTYPE.itemsize == 124
 
 
 class DvsError(RuntimeError):
@@ -226,6 +235,12 @@ def load_library() -> ctypes.CDLL:
         "dvs_noise_eval": (ctypes.c_int, [ctypes.c_uint64, _I64, ctypes.c_double, _I32, _P, _P]),
         "dvs_trace_rays": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P, _P, _P, _P]),
         "dvs_trace_shadow_rays": (ctypes.c_int, [_P, _P, _P, _P, _I64, _P]),
+        "dvs_trajectory_create": (ctypes.c_int, [_P, _I32, ctypes.c_float, _P, _P, ctypes.POINTER(_P)]),
+        "dvs_trajectory_destroy": (ctypes.c_int, [_P]),
+        "dvs_trajectory_range": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                                ctypes.POINTER(_I32)]),
+        "dvs_trajectory_evaluate": (ctypes.c_int, [_P, _P, _I64, _P]),
+        "dvs_trajectory_intervals": (ctypes.c_int, [_P, _P, _I64, _P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -488,6 +503,67 @@ class Sensor:
 
     def reset_stats(self) -> None:
         _check(load_library().dvs_sensor_reset_stats(self.handle))
+
+
+class Trajectory:
+    """CameraTrajectory (src/trajectory.cpp:604-790) over FRAME_DTYPE records:
+    mount translation and ZYX Euler rotation in the camera frame, as
+    SceneConfig's translation/rotation.  Invalid frames raise InvalidArgument
+    with the reference's std::invalid_argument messages."""
+
+    def __init__(self, frames: np.ndarray, fps: float, translation=(0.0, 0.0, 0.0), rotation=(0.0, 0.0, 0.0)):
+        lib = load_library()
+        f = np.ascontiguousarray(np.atleast_1d(frames), dtype=FRAME_DTYPE)
+        tr = np.ascontiguousarray(translation, dtype=np.float32)
+        ro = np.ascontiguousarray(rotation, dtype=np.float32)
+        if tr.shape != (3,) or ro.shape != (3,):
+            raise InvalidArgument(DVS_E_ARGUMENT,
+                                  "trajectory requires positive fps and three-component mount transforms")
+        handle = _P()
+        _check(lib.dvs_trajectory_create(_ptr(f), len(f), float(fps), _ptr(tr), _ptr(ro), ctypes.byref(handle)))
+        self.handle = handle
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load_library().dvs_trajectory_destroy(self.handle)
+            self.handle = None
+
+    __del__ = close
+
+    def range(self) -> tuple[float, float, int]:
+        """(start_time(), end_time(), size())."""
+        a, b, n = ctypes.c_float(), ctypes.c_float(), _I32()
+        _check(load_library().dvs_trajectory_range(self.handle, ctypes.byref(a), ctypes.byref(b), ctypes.byref(n)))
+        return a.value, b.value, n.value
+
+    def evaluate(self, times) -> np.ndarray:
+        """evaluate(float t) at each time, on the host."""
+        t = np.ascontiguousarray(np.atleast_1d(times), dtype=np.float32)
+        out = np.empty(len(t), TRAJECTORY_STATE_DTYPE)
+        _check(load_library().dvs_trajectory_evaluate(self.handle, _ptr(t), len(t), _ptr(out)))
+        return out
+
+    def intervals(self, times) -> np.ndarray:
+        """The len(times) - 1 optical intervals between consecutive FP64 times,
+        each endpoint state evaluated once at float(t), on the device."""
+        t = np.ascontiguousarray(np.atleast_1d(times), dtype=np.float64)
+        out = np.empty(max(len(t) - 1, 0), INTERVAL_DTYPE)
+        _check(load_library().dvs_trajectory_intervals(self.handle, _ptr(t), len(t), _ptr(out)))
+        return out
+
+
+def optical_states(states: np.ndarray) -> np.ndarray:
+    """to_optical_state (include/dvs/raycast.hpp:17-22) of trajectory states."""
+    s = np.atleast_1d(states)
+    out = np.zeros(len(s), STATE_DTYPE)
+    out["position"] = s["position"]
+    out["velocity"] = s["linear_velocity"]
+    out["light"] = s["light_direction"]
+    out["light_rate"] = s["light_direction_rate"]
+    out["rotation"] = s["quaternion"]
+    out["intensity"] = s["light_intensity"]
+    out["intensity_rate"] = s["light_intensity_rate"]
+    return out
 
 
 class Batch:

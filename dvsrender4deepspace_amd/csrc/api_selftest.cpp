@@ -1,11 +1,13 @@
 // Exercises the C++ API (include/dvs_render_amd.hpp) the way the
 // reference's callers use it.
 //   dvs_api_selftest --cpu <config.yaml> <out_dir>
-//       Config, BVHBuilder, writers and pose conversion (no GPU).
+//       Config, BVHBuilder, writers, pose conversion and CameraTrajectory
+//       (no GPU).
 //   dvs_api_selftest --gpu <out_dir>
 //       DeviceScene, CamRayCaster and DvsRayCaster on the plane fixture of
 //       tests/test_sensor.cu:66-80 (48x36 grid): frame depth, native and
-//       adaptive events written to events.bin / events.h5 for comparison.
+//       adaptive events written to events.bin / events.h5 for comparison;
+//       trajectory intervals built on the device driving a DvsRayCaster.
 // Prints one line per check and exits non-zero on the first failure.
 #include <cstdio>
 #include <iostream>
@@ -81,6 +83,40 @@ int cpu(const std::string &config_path, const std::string &out) {
     threw = true;
   }
   expect(threw, "make_event_writer rejects unknown formats");
+  {
+    // tests/test_render.cu:115-130: local mount, world light, seconds
+    dvs::FrameData<dvs::Vec3, dvs::Mat3> frame;
+    frame.camera_origin = dvs::Vec3(10, 20, 30);
+    frame.camera_world(0, 0) = 0, frame.camera_world(1, 0) = 1;  // rotation by pi/2 about z
+    frame.camera_world(0, 1) = -1, frame.camera_world(1, 1) = 0;
+    frame.light_dir = dvs::Vec3(0, 0, 1);
+    frame.light_intensity = 1000;
+    frame.timestamp_seconds = .125;
+    auto last = frame;
+    last.timestamp_seconds = .5;
+    dvs::SceneConfig sc;
+    sc.fps = 100;
+    sc.translation = {1, 0, 0};
+    sc.rotation = {3.14159265f / 2, 0, 0};
+    dvs::CameraTrajectory trajectory({frame, last}, sc);
+    const auto pose = trajectory.evaluate(.125f);
+    const float dp = std::fabs(pose.position.x() - 10) + std::fabs(pose.position.y() - 21) +
+                     std::fabs(pose.position.z() - 30);
+    // Rz(pi/2) Rx(pi/2): columns (0,1,0), (0,0,1), (1,0,0)
+    const float expected[9] = {0, 1, 0, 0, 0, 1, 1, 0, 0};
+    float dr = 0;
+    for (int k = 0; k < 9; ++k) dr += std::fabs(pose.orientation.m[k] - expected[k]);
+    expect(dp < 1e-5f && dr < 1e-5f && pose.light_direction.z() == 1 && trajectory.start_time() == .125f &&
+               trajectory.end_time() == .5f && trajectory.size() == 2,
+           "CameraTrajectory mount, light and seconds");
+    threw = false;
+    try {
+      dvs::CameraTrajectory duplicate({frame, frame}, sc);
+    } catch (const std::invalid_argument &e) {
+      threw = std::string(e.what()) == "trajectory timestamps must be strictly increasing";
+    }
+    expect(threw, "CameraTrajectory rejects duplicate timestamps");
+  }
   return 0;
 }
 
@@ -150,6 +186,35 @@ int gpu(const std::string &out) {
   h5.append(ev_adaptive);
   h5.close(.004);
   std::cout << "events native " << ev_native.size() << " adaptive " << ev_adaptive.size() << "\n";
+  {
+    // bin/dvs's loop over a trajectory: device-built intervals at 10 us
+    // horizons, advanced in one call, equal to the host evaluation's.
+    dvs::FrameData<dvs::Vec3, dvs::Mat3> a, b;
+    a.camera_origin = dvs::Vec3(0, 0, 10);
+    b.camera_origin = dvs::Vec3(.5f, 0, 10);
+    a.light_dir = b.light_dir = dvs::Vec3(0, 0, 1);
+    a.light_intensity = b.light_intensity = 1000;
+    b.frame_index = 1;
+    dvs::SceneConfig sc;
+    sc.fps = 100;
+    dvs::CameraTrajectory trajectory({a, b}, sc);
+    std::vector<double> times;
+    for (int k = 0; k <= 200; ++k) times.push_back(std::min<double>(trajectory.end_time(), k * (double)1e-5f));
+    const auto ivs = trajectory.intervals(times);
+    bool same = ivs.size() == 200;
+    for (size_t k = 0; same && k < ivs.size(); k += 37) {
+      const auto h = dvs::to_optical_state(trajectory.evaluate((float)times[k]));
+      same = std::fabs(h.position.x - ivs[k].start.position.x) < 1e-5f && ivs[k].t0 == times[k] &&
+             std::fabs(h.velocity.x - ivs[k].start.velocity.x) < 1e-4f;
+    }
+    expect(same, "device trajectory intervals equal the host evaluation");
+    dvs::DvsRayCaster moving(scene, config);
+    moving.initialize(trajectory.evaluate(0), 0.0);
+    std::vector<dvs::Event> ev;
+    moving.advance_many(ivs, ev);
+    moving.drain(ev);
+    expect(moving.dropped_events() == 0, "trajectory window advances");
+  }
   return 0;
 }
 }  // namespace

@@ -284,6 +284,54 @@ int dvs_batch_refinement_stats(dvs_batch batch, int32_t segment, dvs_refinement_
 int dvs_batch_stats(dvs_batch batch, dvs_stats *out);
 int dvs_batch_reset_stats(dvs_batch batch);
 
+/* ---- Trajectory (CameraTrajectory, include/dvs/trajectory.hpp:81-213,
+ * src/trajectory.cpp) ----------------------------------------------------
+ * FrameData<Vec3, Mat3> (include/dvs/scene_types.hpp:43-50): camera_world
+ * is the camera-to-world rotation, column major (m[3 * col + row]); a NaN
+ * timestamp means frame_index / fps (frame_time, scene_types.hpp:53-56). */
+typedef struct {
+  int32_t frame_index;
+  double timestamp_seconds;
+  float camera_origin[3];
+  float camera_world[9];
+  float light_dir[3];
+  float light_intensity;
+} dvs_frame;
+
+/* CameraTrajectoryState (trajectory.hpp:44-60): orientation column major,
+ * quaternion (x, y, z, w). */
+typedef struct {
+  float timestamp;
+  float position[3];
+  float orientation[9];
+  float quaternion[4];
+  float linear_velocity[3], angular_velocity[3];
+  float light_direction[3], light_direction_rate[3];
+  float light_intensity, light_intensity_rate;
+} dvs_trajectory_state;
+
+typedef struct dvs_trajectory_s *dvs_trajectory;
+
+/* CameraTrajectory(frames, scene_config) / build (trajectory.cpp:604-716):
+ * auto-clamped Bezier keys of the mounted camera position, continuous
+ * quaternions of its orientation (mount translation and ZYX Euler rotation
+ * in the camera frame), the unit light direction and intensity.  Invalid
+ * frames, fps or mount, and non-increasing times are DVS_E_ARGUMENT with the
+ * reference's messages.  Host only: the keys go to the device on first use. */
+int dvs_trajectory_create(const dvs_frame *frames, int32_t count, float fps, const float translation[3],
+                          const float rotation[3], dvs_trajectory *out);
+int dvs_trajectory_destroy(dvs_trajectory trajectory);
+/* start_time(), end_time(), size(). */
+int dvs_trajectory_range(dvs_trajectory trajectory, float *start_time, float *end_time, int32_t *size);
+/* evaluate(t) (trajectory.cpp:719-787) for n times, on the host. */
+int dvs_trajectory_evaluate(dvs_trajectory trajectory, const float *times, int64_t n, dvs_trajectory_state *out);
+/* The n - 1 intervals between consecutive times (n >= 2) as bin/dvs builds
+ * them (apps/main_dvs.cpp:103-107): {to_optical_state(evaluate((float)t_i)),
+ * to_optical_state(evaluate((float)t_{i+1})), t_i, t_{i+1}}, every state
+ * evaluated once, on the device. */
+int dvs_trajectory_intervals(dvs_trajectory trajectory, const double *times, int64_t n,
+                             dvs_optical_interval *out);
+
 /* ---- Device kernels exposed for unit parity (no reference counterpart;
  * they expose the reference's __host__ __device__ functions run on the GPU) */
 /* hapke_reflectance (include/dvs/hapke.hpp:131-151) over n inputs. */

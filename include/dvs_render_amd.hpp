@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -216,12 +217,27 @@ class DeviceScene {
   int width_ = 0, height_ = 0;
 };
 
-// CameraTrajectoryState with Eigen-style accessors (include/dvs/trajectory.hpp:59-70).
+// Vec3 / Mat3 / Quaternionf stand-ins with Eigen-style accessors, and
+// CameraTrajectoryState (include/dvs/trajectory.hpp:44-60).
 struct Vec3 {
   float v[3] = {0, 0, 0};
+  Vec3() = default;
+  Vec3(float x, float y, float z) : v{x, y, z} {}
   float x() const { return v[0]; }
   float y() const { return v[1]; }
   float z() const { return v[2]; }
+  float operator()(int i) const { return v[i]; }
+  float *data() { return v; }
+  const float *data() const { return v; }
+};
+// Column major, (row, col) access as Eigen's Matrix3f.
+struct Mat3 {
+  float m[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  static Mat3 Identity() { return Mat3{}; }
+  float operator()(int r, int c) const { return m[3 * c + r]; }
+  float &operator()(int r, int c) { return m[3 * c + r]; }
+  float *data() { return m; }
+  const float *data() const { return m; }
 };
 struct Quaternion {
   float q[4] = {0, 0, 0, 1};  // x, y, z, w
@@ -233,9 +249,21 @@ struct Quaternion {
 struct CameraTrajectoryState {
   float timestamp = 0;
   Vec3 position;
+  Mat3 orientation;
   Quaternion quaternion;
-  Vec3 linear_velocity, angular_velocity, light_direction{{0, 0, 1}}, light_direction_rate;
+  Vec3 linear_velocity, angular_velocity, light_direction{0, 0, 1}, light_direction_rate;
   float light_intensity = 0, light_intensity_rate = 0;
+};
+
+// FrameData (include/dvs/scene_types.hpp:43-50): a NaN timestamp means
+// frame_index / fps.
+template <class V, class M> struct FrameData {
+  int frame_index = 0;
+  double timestamp_seconds = std::numeric_limits<double>::quiet_NaN();
+  V camera_origin;
+  M camera_world;
+  V light_dir;
+  float light_intensity = 0.0f;
 };
 
 // to_optical_state (include/dvs/raycast.hpp:17-22) for any state type with
@@ -254,6 +282,89 @@ OpticalState to_optical_state(const State &s) {
 }
 // An optical state is already one (callers that build intervals directly).
 inline OpticalState to_optical_state(const OpticalState &s) { return s; }
+
+// CameraTrajectory (include/dvs/trajectory.hpp:81-116, src/trajectory.cpp):
+// the keys live in a dvs_trajectory; single evaluations run on the host,
+// intervals() evaluates every endpoint of a window on the device.  Copies
+// share the keys.
+class CameraTrajectory {
+ public:
+  CameraTrajectory() = default;
+  CameraTrajectory(const std::vector<FrameData<Vec3, Mat3>> &frames, SceneConfig &scene_config) {
+    build(frames, scene_config.fps, scene_config.translation, scene_config.rotation);
+  }
+  void build(const std::vector<FrameData<Vec3, Mat3>> &frames, float fps, std::vector<float> &translation,
+             std::vector<float> &rotation) {
+    if (translation.size() != 3 || rotation.size() != 3)
+      throw std::invalid_argument("trajectory requires positive fps and three-component mount transforms");
+    std::vector<dvs_frame> c(frames.size());
+    for (size_t i = 0; i < frames.size(); ++i) {
+      const auto &f = frames[i];
+      c[i].frame_index = f.frame_index;
+      c[i].timestamp_seconds = f.timestamp_seconds;
+      std::memcpy(c[i].camera_origin, f.camera_origin.data(), sizeof(c[i].camera_origin));
+      std::memcpy(c[i].camera_world, f.camera_world.data(), sizeof(c[i].camera_world));
+      std::memcpy(c[i].light_dir, f.light_dir.data(), sizeof(c[i].light_dir));
+      c[i].light_intensity = f.light_intensity;
+    }
+    dvs_trajectory h = nullptr;
+    detail::check(dvs_trajectory_create(c.data(), (int32_t)c.size(), fps, translation.data(), rotation.data(), &h));
+    handle_.reset(h, [](dvs_trajectory p) { dvs_trajectory_destroy(p); });
+  }
+  bool valid() const { return size() > 0; }
+  std::size_t size() const { return range().size; }
+  float start_time() const { return range().start; }
+  float end_time() const { return range().end; }
+  CameraTrajectoryState evaluate(float t) const {
+    CameraTrajectoryState out;
+    out.timestamp = t;
+    if (!handle_) return out;
+    dvs_trajectory_state s;
+    detail::check(dvs_trajectory_evaluate(handle_.get(), &t, 1, &s));
+    out.timestamp = s.timestamp;
+    out.position = Vec3(s.position[0], s.position[1], s.position[2]);
+    std::memcpy(out.orientation.m, s.orientation, sizeof(s.orientation));
+    std::memcpy(out.quaternion.q, s.quaternion, sizeof(s.quaternion));
+    out.linear_velocity = Vec3(s.linear_velocity[0], s.linear_velocity[1], s.linear_velocity[2]);
+    out.angular_velocity = Vec3(s.angular_velocity[0], s.angular_velocity[1], s.angular_velocity[2]);
+    out.light_direction = Vec3(s.light_direction[0], s.light_direction[1], s.light_direction[2]);
+    out.light_direction_rate =
+        Vec3(s.light_direction_rate[0], s.light_direction_rate[1], s.light_direction_rate[2]);
+    out.light_intensity = s.light_intensity;
+    out.light_intensity_rate = s.light_intensity_rate;
+    return out;
+  }
+  Vec3 position(float t) const { return evaluate(t).position; }
+  Mat3 orientation(float t) const { return evaluate(t).orientation; }
+  Quaternion quaternion(float t) const { return evaluate(t).quaternion; }
+  Vec3 linear_velocity(float t) const { return evaluate(t).linear_velocity; }
+  Vec3 angular_velocity(float t) const { return evaluate(t).angular_velocity; }
+  Vec3 light_direction(float t) const { return evaluate(t).light_direction; }
+  Vec3 light_direction_rate(float t) const { return evaluate(t).light_direction_rate; }
+  float light_intensity(float t) const { return evaluate(t).light_intensity; }
+  float light_intensity_rate(float t) const { return evaluate(t).light_intensity_rate; }
+  // The intervals bin/dvs builds between consecutive host horizons
+  // (apps/main_dvs.cpp:103-107), each endpoint evaluated once, on the device.
+  std::vector<OpticalInterval> intervals(const std::vector<double> &times) const {
+    if (!handle_) throw std::invalid_argument("trajectory has not been built");
+    std::vector<OpticalInterval> out(times.size() > 1 ? times.size() - 1 : 0);
+    detail::check(dvs_trajectory_intervals(handle_.get(), times.data(), (int64_t)times.size(),
+                                           reinterpret_cast<dvs_optical_interval *>(out.data())));
+    return out;
+  }
+
+ private:
+  struct Range {
+    float start = 0, end = 0;
+    int32_t size = 0;
+  };
+  Range range() const {
+    Range r;
+    if (handle_) detail::check(dvs_trajectory_range(handle_.get(), &r.start, &r.end, &r.size));
+    return r;
+  }
+  std::shared_ptr<dvs_trajectory_s> handle_;
+};
 
 // OpticalRefinementStats / ReceptorProbeResult of the adaptive solver.
 using OpticalRefinementStats = dvs_refinement_stats;

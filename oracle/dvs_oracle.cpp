@@ -1352,3 +1352,348 @@ void orc_render_to_bytes(const float *linear, int width, int height, float dithe
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Camera trajectory: src/trajectory.cpp (CameraTrajectory), float arithmetic.
+// Eigen 3.4 (quaternion <-> matrix, AngleAxis, norms) is not in the
+// reference tree; its published formulas are restated: Shepperd's method for
+// Quaternion(Matrix3), the Rodrigues matrix for AngleAxis, the Hamilton
+// product and the quaternion's rotation matrix.
+// ---------------------------------------------------------------------------
+namespace orc {
+namespace traj {
+
+struct Key { float time, value, lt, lv, rt, rv; };
+
+struct Scalar {
+  std::vector<Key> k;
+  // bezier / bezier_derivative (trajectory.cpp:114-128)
+  static float bz(float p0, float p1, float p2, float p3, float u) {
+    const float a = p0 + u * (p1 - p0), b = p1 + u * (p2 - p1), c = p2 + u * (p3 - p2);
+    const float d = a + u * (b - a), e = b + u * (c - b);
+    return d + u * (e - d);
+  }
+  static float dbz(float p0, float p1, float p2, float p3, float u) {
+    const float v = 1.0f - u;
+    return 3.0f * v * v * (p1 - p0) + 6.0f * v * u * (p2 - p1) + 3.0f * u * u * (p3 - p2);
+  }
+  // compute_auto_clamped_tangents (trajectory.cpp:26-82) and
+  // build_auto_clamped (:130-193)
+  void build(const std::vector<float> &t, const std::vector<float> &y) {
+    const size_t n = t.size();
+    k.assign(n, Key{});
+    for (size_t i = 0; i < n; ++i) k[i] = Key{t[i], y[i], t[i], y[i], t[i], y[i]};
+    if (n < 2) return;
+    std::vector<float> m(n, 0.0f), h(n - 1), d(n - 1);
+    for (size_t i = 0; i + 1 < n; ++i) {
+      h[i] = t[i + 1] - t[i];
+      d[i] = std::fabs(h[i]) > 1e-12f ? (y[i + 1] - y[i]) / h[i] : 0.0f;
+    }
+    if (n == 2) {
+      m[0] = m[1] = d[0];
+    } else {
+      m[0] = d[0];
+      m[n - 1] = d[n - 2];
+      for (size_t i = 1; i + 1 < n; ++i) {
+        if (d[i - 1] == 0.0f || d[i] == 0.0f || d[i - 1] * d[i] <= 0.0f) continue;
+        const float w1 = 2.0f * h[i] + h[i - 1], w2 = h[i] + 2.0f * h[i - 1];
+        const float den = w1 / d[i - 1] + w2 / d[i];
+        m[i] = std::fabs(den) > 1e-12f ? (w1 + w2) / den : 0.0f;
+      }
+    }
+    for (size_t i = 0; i + 1 < n; ++i) {
+      const float span = t[i + 1] - t[i];
+      k[i].rt = t[i] + span / 3.0f;
+      k[i].rv = y[i] + m[i] * span / 3.0f;
+      k[i + 1].lt = t[i + 1] - span / 3.0f;
+      k[i + 1].lv = y[i + 1] - m[i + 1] * span / 3.0f;
+    }
+  }
+  // find_segment (trajectory.cpp:195-215): upper_bound of t, minus one
+  int segment(float t) const {
+    const int n = (int)k.size();
+    if (t <= k.front().time) return 0;
+    if (t >= k.back().time) return n - 2;
+    int idx = (int)(std::upper_bound(k.begin(), k.end(), t, [](float x, const Key &q) { return x < q.time; }) -
+                    k.begin()) - 1;
+    return std::max(0, std::min(idx, n - 2));
+  }
+  // solve_bezier_time (trajectory.cpp:217-267)
+  static float solve(const Key &a, const Key &b, float t) {
+    const float x0 = a.time, x1 = a.rt, x2 = b.lt, x3 = b.time;
+    if (x3 - x0 <= 1e-12f) return 0.0f;
+    float u = std::min(std::max((t - x0) / (x3 - x0), 0.0f), 1.0f);
+    for (int it = 0; it < 8; ++it) {
+      const float x = bz(x0, x1, x2, x3, u), dx = dbz(x0, x1, x2, x3, u);
+      if (std::fabs(dx) < 1e-10f) break;
+      const float nu = std::min(std::max(u - (x - t) / dx, 0.0f), 1.0f);
+      if (std::fabs(nu - u) < 1e-7f) return nu;
+      u = nu;
+    }
+    float lo = 0.0f, hi = 1.0f;
+    for (int it = 0; it < 24; ++it) {
+      const float mid = 0.5f * (lo + hi);
+      (bz(x0, x1, x2, x3, mid) < t ? lo : hi) = mid;
+    }
+    return 0.5f * (lo + hi);
+  }
+  // evaluate (trajectory.cpp:269-295)
+  float value(float t) const {
+    if (k.empty()) return 0.0f;
+    if (k.size() == 1 || t <= k.front().time) return k.front().value;
+    if (t >= k.back().time) return k.back().value;
+    const int s = segment(t);
+    const float u = solve(k[s], k[s + 1], t);
+    return bz(k[s].value, k[s].rv, k[s + 1].lv, k[s + 1].value, u);
+  }
+  // derivative (trajectory.cpp:297-328): segment of the clamped time, curve
+  // parameter of the unclamped one
+  float rate(float t) const {
+    if (k.size() < 2) return 0.0f;
+    float tc = t;
+    if (tc < k.front().time + 1e-7f) tc = k.front().time + 1e-7f;
+    if (tc > k.back().time - 1e-7f) tc = k.back().time - 1e-7f;
+    const int s = segment(tc);
+    const float u = solve(k[s], k[s + 1], t);
+    const float dx = dbz(k[s].time, k[s].rt, k[s + 1].lt, k[s + 1].time, u);
+    const float dy = dbz(k[s].value, k[s].rv, k[s + 1].lv, k[s + 1].value, u);
+    return std::fabs(dx) < 1e-12f ? 0.0f : dy / dx;
+  }
+};
+
+// Quaternion stored (w, x, y, z) here.
+struct Q { float w, x, y, z; };
+static Q qmul(const Q &a, const Q &b) {
+  return Q{a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z, a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y,
+           a.w * b.y + a.y * b.w + a.z * b.x - a.x * b.z, a.w * b.z + a.z * b.w + a.x * b.y - a.y * b.x};
+}
+static float qnorm(const Q &q) { return std::sqrt(q.w * q.w + q.x * q.x + q.y * q.y + q.z * q.z); }
+static Q qunit(Q q) {
+  const float n = qnorm(q);
+  if (n > 0) { q.w /= n; q.x /= n; q.y /= n; q.z /= n; }
+  return q;
+}
+static float qdot(const Q &a, const Q &b) { return a.w * b.w + a.x * b.x + a.y * b.y + a.z * b.z; }
+static Q qneg(const Q &q) { return Q{-q.w, -q.x, -q.y, -q.z}; }
+
+// Row-major 3x3 here: R[r][c].
+struct M { float a[3][3]; };
+static M mmul(const M &p, const M &q) {
+  M r{};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) r.a[i][j] = p.a[i][0] * q.a[0][j] + p.a[i][1] * q.a[1][j] + p.a[i][2] * q.a[2][j];
+  return r;
+}
+// AngleAxis(angle, unit e).toRotationMatrix(): (1-c) e e^T + c I + s [e]x
+static M angle_axis(float angle, int axis) {
+  const float e[3] = {axis == 0 ? 1.0f : 0.0f, axis == 1 ? 1.0f : 0.0f, axis == 2 ? 1.0f : 0.0f};
+  const float s = std::sin(angle), c = std::cos(angle);
+  const float cross[3][3] = {{0, -e[2], e[1]}, {e[2], 0, -e[0]}, {-e[1], e[0], 0}};
+  M r{};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) r.a[i][j] = (1.0f - c) * e[i] * e[j] + (i == j ? c : 0.0f) + s * cross[i][j];
+  return r;
+}
+// Quaternion(R) by Shepperd's method (Eigen quaternion_assign_impl<3x3>).
+static Q from_matrix(const M &R) {
+  const float tr = R.a[0][0] + R.a[1][1] + R.a[2][2];
+  Q q;
+  if (tr > 0) {
+    const float s = std::sqrt(tr + 1.0f);
+    const float f = 0.5f / s;
+    q = Q{0.5f * s, (R.a[2][1] - R.a[1][2]) * f, (R.a[0][2] - R.a[2][0]) * f, (R.a[1][0] - R.a[0][1]) * f};
+  } else {
+    int i = R.a[1][1] > R.a[0][0] ? 1 : 0;
+    if (R.a[2][2] > R.a[i][i]) i = 2;
+    const int j = (i + 1) % 3, k = (j + 1) % 3;
+    const float s = std::sqrt(R.a[i][i] - R.a[j][j] - R.a[k][k] + 1.0f);
+    const float f = 0.5f / s;
+    float v[3];
+    v[i] = 0.5f * s;
+    v[j] = (R.a[j][i] + R.a[i][j]) * f;
+    v[k] = (R.a[k][i] + R.a[i][k]) * f;
+    q = Q{(R.a[k][j] - R.a[j][k]) * f, v[0], v[1], v[2]};
+  }
+  return q;
+}
+// Quaternion::toRotationMatrix
+static M to_matrix(const Q &q) {
+  M r{};
+  const float xx = q.x * q.x, yy = q.y * q.y, zz = q.z * q.z;
+  const float xy = q.x * q.y, xz = q.x * q.z, yz = q.y * q.z, wx = q.w * q.x, wy = q.w * q.y, wz = q.w * q.z;
+  r.a[0][0] = 1 - (2 * yy + 2 * zz); r.a[0][1] = 2 * xy - 2 * wz;       r.a[0][2] = 2 * xz + 2 * wy;
+  r.a[1][0] = 2 * xy + 2 * wz;       r.a[1][1] = 1 - (2 * xx + 2 * zz); r.a[1][2] = 2 * yz - 2 * wx;
+  r.a[2][0] = 2 * xz - 2 * wy;       r.a[2][1] = 2 * yz + 2 * wx;       r.a[2][2] = 1 - (2 * xx + 2 * yy);
+  return r;
+}
+// quat_exp / quat_log (trajectory.cpp:346-387)
+static Q qexp(const float v[3]) {
+  const float th = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  if (th < 1e-8f) return qunit(Q{1.0f, 0.5f * v[0], 0.5f * v[1], 0.5f * v[2]});
+  const float s = std::sin(0.5f * th);
+  return qunit(Q{std::cos(0.5f * th), s * (v[0] / th), s * (v[1] / th), s * (v[2] / th)});
+}
+static void qlog(Q q, float out[3]) {
+  q = qunit(q);
+  if (q.w < 0.0f) q = qneg(q);
+  const float sh = std::sqrt(q.x * q.x + q.y * q.y + q.z * q.z);
+  if (sh < 1e-8f) { out[0] = 2.0f * q.x; out[1] = 2.0f * q.y; out[2] = 2.0f * q.z; return; }
+  const float f = 2.0f * std::atan2(sh, q.w) / sh;
+  out[0] = q.x * f; out[1] = q.y * f; out[2] = q.z * f;
+}
+
+struct Trajectory {
+  Scalar px, py, pz, lx, ly, lz, intensity, phase;
+  std::vector<float> times;
+  std::vector<Q> rot;
+  size_t size() const { return px.k.size(); }
+};
+
+// CameraTrajectory::build (trajectory.cpp:609-731), messages included.
+static void build(Trajectory &T, const orc_frame *f, int n, float fps, const float tr[3], const float ro[3]) {
+  if (!std::isfinite(fps) || fps <= 0)
+    throw std::invalid_argument("trajectory requires positive fps and three-component mount transforms");
+  const M mount = mmul(mmul(angle_axis(ro[2], 2), angle_axis(ro[1], 1)), angle_axis(ro[0], 0));
+  if (n == 0) return;
+  std::vector<float> t(n), x(n), y(n), z(n), lx(n), ly(n), lz(n), li(n), ph(n);
+  std::vector<Q> q(n);
+  for (int i = 0; i < n; ++i) {
+    const double secs = std::isnan(f[i].timestamp_seconds) ? f[i].frame_index / (double)fps : f[i].timestamp_seconds;
+    t[i] = (float)secs;
+    if (!std::isfinite(t[i])) throw std::invalid_argument("non-finite trajectory timestamp");
+  }
+  for (int i = 1; i < n; ++i)
+    if (!(t[i] > t[i - 1])) throw std::invalid_argument("trajectory timestamps must be strictly increasing");
+  for (int i = 0; i < n; ++i) {
+    M W;  // camera_world is column major
+    bool finite = true;
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        W.a[r][c] = f[i].camera_world[3 * c + r];
+        finite = finite && std::isfinite(W.a[r][c]);
+      }
+    for (int c = 0; c < 3; ++c)
+      finite = finite && std::isfinite(f[i].camera_origin[c]) && std::isfinite(f[i].light_dir[c]);
+    const float *L = f[i].light_dir;
+    const float ln = std::sqrt(L[0] * L[0] + L[1] * L[1] + L[2] * L[2]);
+    float frob = 0;
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        const float g = W.a[0][r] * W.a[0][c] + W.a[1][r] * W.a[1][c] + W.a[2][r] * W.a[2][c] - (r == c);
+        frob += g * g;
+      }
+    const float det = W.a[0][0] * (W.a[1][1] * W.a[2][2] - W.a[2][1] * W.a[1][2]) -
+                      W.a[1][0] * (W.a[0][1] * W.a[2][2] - W.a[2][1] * W.a[0][2]) +
+                      W.a[2][0] * (W.a[0][1] * W.a[1][2] - W.a[1][1] * W.a[0][2]);
+    if (!finite || ln < 1e-12f || !std::isfinite(f[i].light_intensity) || f[i].light_intensity < 0 ||
+        std::sqrt(frob) > 2e-3f || det <= 0)
+      throw std::invalid_argument("trajectory requires finite rigid poses and valid world lighting");
+    const float *o = f[i].camera_origin;
+    x[i] = o[0] + (W.a[0][0] * tr[0] + W.a[0][1] * tr[1] + W.a[0][2] * tr[2]);
+    y[i] = o[1] + (W.a[1][0] * tr[0] + W.a[1][1] * tr[1] + W.a[1][2] * tr[2]);
+    z[i] = o[2] + (W.a[2][0] * tr[0] + W.a[2][1] * tr[1] + W.a[2][2] * tr[2]);
+    Q qi = qunit(from_matrix(mmul(W, mount)));  // matrix_to_quaternion_continuous (:580-592)
+    if (i > 0 && qdot(qi, q[i - 1]) < 0.0f) qi = qneg(qi);
+    q[i] = qunit(qi);
+    lx[i] = L[0] / ln; ly[i] = L[1] / ln; lz[i] = L[2] / ln;
+    li[i] = f[i].light_intensity;
+    ph[i] = (float)i;
+  }
+  T.px.build(t, x); T.py.build(t, y); T.pz.build(t, z);
+  // RotationBezierCurve::build (:389-436)
+  T.times = t;
+  T.rot.resize(n);
+  for (int i = 0; i < n; ++i) {
+    T.rot[i] = qunit(q[i]);
+    if (i > 0 && qdot(T.rot[i], T.rot[i - 1]) < 0.0f) T.rot[i] = qneg(T.rot[i]);
+  }
+  T.phase.build(t, ph);
+  T.lx.build(t, lx); T.ly.build(t, ly); T.lz.build(t, lz);
+  T.intensity.build(t, li);
+}
+
+// CameraTrajectory::evaluate (trajectory.cpp:733-790) with
+// RotationBezierCurve::evaluate (:477-564).
+static void evaluate(const Trajectory &T, float t, orc_trajectory_state *s) {
+  std::memset(s, 0, sizeof(*s));
+  s->timestamp = t;
+  s->orientation[0] = s->orientation[4] = s->orientation[8] = 1;
+  s->quaternion[3] = 1;
+  s->light_direction[2] = 1;
+  if (T.size() == 0) return;
+  const float tc = std::min(std::max(t, T.px.k.front().time), T.px.k.back().time);
+  s->timestamp = tc;
+  const Scalar *P[3] = {&T.px, &T.py, &T.pz};
+  for (int c = 0; c < 3; ++c) {
+    s->position[c] = P[c]->value(tc);
+    s->linear_velocity[c] = P[c]->rate(tc);
+  }
+  const int n = (int)T.rot.size();
+  if (n >= 2) {
+    const float phase = T.phase.value(tc), pdot = T.phase.rate(tc);
+    const int seg = std::max(0, std::min((int)std::floor(phase), n - 2));
+    const float alpha = std::min(std::max(phase - (float)seg, 0.0f), 1.0f);
+    const Q &q0 = T.rot[seg], &q1 = T.rot[seg + 1];
+    Q rel = qunit(qmul(Q{q0.w, -q0.x, -q0.y, -q0.z}, q1));
+    if (rel.w < 0.0f) rel = qneg(rel);
+    float rho[3];
+    qlog(rel, rho);
+    const float arho[3] = {alpha * rho[0], alpha * rho[1], alpha * rho[2]};
+    const Q q = qunit(qmul(q0, qexp(arho)));
+    const M R = to_matrix(q);
+    const float wb[3] = {pdot * rho[0], pdot * rho[1], pdot * rho[2]};
+    for (int r = 0; r < 3; ++r) {
+      for (int c = 0; c < 3; ++c) s->orientation[3 * c + r] = R.a[r][c];
+      s->angular_velocity[r] = R.a[r][0] * wb[0] + R.a[r][1] * wb[1] + R.a[r][2] * wb[2];
+    }
+    s->quaternion[0] = q.x; s->quaternion[1] = q.y; s->quaternion[2] = q.z; s->quaternion[3] = q.w;
+  }
+  const float L[3] = {T.lx.value(tc), T.ly.value(tc), T.lz.value(tc)};
+  const float Ld[3] = {T.lx.rate(tc), T.ly.rate(tc), T.lz.rate(tc)};
+  const float ln = std::sqrt(L[0] * L[0] + L[1] * L[1] + L[2] * L[2]);
+  if (ln > 1e-12f) {
+    // normalized_vector_derivative (:594-607)
+    const float vd = L[0] * Ld[0] + L[1] * Ld[1] + L[2] * Ld[2];
+    for (int c = 0; c < 3; ++c) {
+      s->light_direction[c] = L[c] / ln;
+      s->light_direction_rate[c] = Ld[c] / ln - L[c] * (vd / (ln * ln * ln));
+    }
+  }
+  s->light_intensity = T.intensity.value(tc);
+  s->light_intensity_rate = T.intensity.rate(tc);
+}
+
+}  // namespace traj
+}  // namespace orc
+
+extern "C" {
+
+void *orc_trajectory_create(const orc_frame *frames, int count, float fps, const float translation[3],
+                            const float rotation[3], char *error, int error_len) {
+  try {
+    auto T = std::make_unique<orc::traj::Trajectory>();
+    orc::traj::build(*T, frames, count, fps, translation, rotation);
+    if (error && error_len > 0) error[0] = 0;
+    return T.release();
+  } catch (const std::exception &e) {
+    if (error && error_len > 0) {
+      std::strncpy(error, e.what(), error_len - 1);
+      error[error_len - 1] = 0;
+    }
+    return nullptr;
+  }
+}
+void orc_trajectory_destroy(void *t) { delete static_cast<orc::traj::Trajectory *>(t); }
+void orc_trajectory_range(const void *tp, float *start, float *end, int *size) {
+  const auto &T = *static_cast<const orc::traj::Trajectory *>(tp);
+  const bool any = T.size() > 0;
+  *start = any ? T.px.k.front().time : 0.0f;
+  *end = any ? T.px.k.back().time : 0.0f;
+  *size = (int)T.size();
+}
+void orc_trajectory_evaluate(const void *tp, const float *times, int64_t n, orc_trajectory_state *out) {
+  const auto &T = *static_cast<const orc::traj::Trajectory *>(tp);
+  for (int64_t i = 0; i < n; ++i) orc::traj::evaluate(T, times[i], out + i);
+}
+
+}  // extern "C"

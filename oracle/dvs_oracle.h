@@ -21,6 +21,12 @@
  *                           (tests/test_sensor.cu:176-177); serial FIFO.
  *   philox4x64-10           numpy.random.Philox outputs and the Random123
  *                           published answer for the zero counter/key.
+ *   trajectory              constant-trajectory exactness
+ *                           (tests/test_sensor.cu:32-36), local mount and
+ *                           seconds (tests/test_render.cu:115-130), linear
+ *                           and monotone data, Bezier/Fritsch-Carlson closed
+ *                           forms on 2-3 keys, constant-rate rotation about
+ *                           one axis, quaternion <-> matrix round trips.
  *   poisson / normal        moments (tests/test_sensor.cu:57-65).  The
  *                           reference draws from cuRAND (closed library), so
  *                           noise-on event streams are "parity unpinned" at
@@ -233,6 +239,35 @@ int64_t orc_readout_submit(void *r, const orc_event *events, int64_t n, double n
 int64_t orc_readout_drain(void *r, orc_event *ready, int64_t capacity);
 int64_t orc_readout_dropped(const void *r);
 int64_t orc_readout_peak(const void *r);
+
+/* ---- Camera trajectory (src/trajectory.cpp) ---- */
+/* FrameData<Vec3, Mat3> (include/dvs/scene_types.hpp:43-50); camera_world
+ * column major; NaN timestamp -> frame_index / fps. */
+typedef struct {
+  int32_t frame_index;
+  double timestamp_seconds;
+  float camera_origin[3];
+  float camera_world[9];
+  float light_dir[3];
+  float light_intensity;
+} orc_frame;
+/* CameraTrajectoryState (include/dvs/trajectory.hpp:44-60); orientation
+ * column major, quaternion (x, y, z, w). */
+typedef struct {
+  float timestamp;
+  float position[3];
+  float orientation[9];
+  float quaternion[4];
+  float linear_velocity[3], angular_velocity[3];
+  float light_direction[3], light_direction_rate[3];
+  float light_intensity, light_intensity_rate;
+} orc_trajectory_state;
+/* NULL with the reference's std::invalid_argument message in `error`. */
+void *orc_trajectory_create(const orc_frame *frames, int count, float fps, const float translation[3],
+                            const float rotation[3], char *error, int error_len);
+void orc_trajectory_destroy(void *t);
+void orc_trajectory_range(const void *t, float *start, float *end, int *size);
+void orc_trajectory_evaluate(const void *t, const float *times, int64_t n, orc_trajectory_state *out);
 
 /* ---- Image encode (include/dvs/blenderize.hpp) ---- */
 void orc_render_to_bytes(const float *linear, int width, int height, float dither,

@@ -27,6 +27,13 @@ PIXEL_STATE = np.dtype([("receptor_log", "<f8"), ("reference_log", "<f8"), ("ref
                         ("threshold_pos", "<f8"), ("threshold_neg", "<f8"), ("reset_pending", "<i4"),
                         ("pad", "<u4")])
 PIXEL_EVENT = np.dtype([("time", "<f8"), ("polarity", "<i4"), ("pad", "<i4")])
+FRAME = np.dtype([("frame_index", "<i4"), ("timestamp_seconds", "<f8"), ("camera_origin", "<f4", (3,)),
+                  ("camera_world", "<f4", (9,)), ("light_dir", "<f4", (3,)), ("light_intensity", "<f4")], align=True)
+TRAJECTORY_STATE = np.dtype([("timestamp", "<f4"), ("position", "<f4", (3,)), ("orientation", "<f4", (9,)),
+                             ("quaternion", "<f4", (4,)), ("linear_velocity", "<f4", (3,)),
+                             ("angular_velocity", "<f4", (3,)), ("light_direction", "<f4", (3,)),
+                             ("light_direction_rate", "<f4", (3,)), ("light_intensity", "<f4"),
+                             ("light_intensity_rate", "<f4")])
 
 
 class RefinementStats(ctypes.Structure):
@@ -130,6 +137,10 @@ def lib() -> ctypes.CDLL:
             "orc_readout_dropped": (I64, [_P]),
             "orc_readout_peak": (I64, [_P]),
             "orc_render_to_bytes": (None, [_P, I, I, F, _P]),
+            "orc_trajectory_create": (_P, [_P, I, F, _P, _P, ctypes.c_char_p, I]),
+            "orc_trajectory_destroy": (None, [_P]),
+            "orc_trajectory_range": (None, [_P, ctypes.POINTER(F), ctypes.POINTER(F), ctypes.POINTER(I)]),
+            "orc_trajectory_evaluate": (None, [_P, _P, I64, _P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -384,3 +395,34 @@ def render_to_bytes(linear, dither=0.0):
     out = np.empty((h, w), np.uint8)
     lib().orc_render_to_bytes(ptr(a), w, h, dither, ptr(out))
     return out
+
+
+class Trajectory:
+    """CameraTrajectory (src/trajectory.cpp) over FRAME records; raises
+    ValueError with the reference's message where it throws
+    std::invalid_argument."""
+
+    def __init__(self, frames, fps, translation=(0.0, 0.0, 0.0), rotation=(0.0, 0.0, 0.0)):
+        frames = np.ascontiguousarray(frames, FRAME)
+        tr = np.ascontiguousarray(translation, np.float32)
+        ro = np.ascontiguousarray(rotation, np.float32)
+        err = ctypes.create_string_buffer(256)
+        self._h = lib().orc_trajectory_create(ptr(frames), len(frames), fps, ptr(tr), ptr(ro), err, 256)
+        if not self._h:
+            raise ValueError(err.value.decode())
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_trajectory_destroy(self._h)
+            self._h = None
+
+    def range(self):
+        a, b, n = ctypes.c_float(), ctypes.c_float(), ctypes.c_int()
+        lib().orc_trajectory_range(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(n))
+        return a.value, b.value, n.value
+
+    def evaluate(self, times):
+        t = np.ascontiguousarray(np.atleast_1d(times), np.float32)
+        out = np.zeros(len(t), TRAJECTORY_STATE)
+        lib().orc_trajectory_evaluate(self._h, ptr(t), len(t), ptr(out))
+        return out
