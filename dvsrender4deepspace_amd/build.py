@@ -78,14 +78,15 @@ def build_library(jobs: int = 4) -> Path:
 
 
 def build_io() -> Path:
-    """Compiles libdvs_io.so (event writers and configuration, host C++)."""
+    """Compiles libdvs_io.so (event writers, configuration, scene and
+    trajectory loaders, frame images; host C++)."""
     LIBDIR.mkdir(exist_ok=True)
     if not (HDF5_PREFIX / "include" / "hdf5.h").exists():
         raise RuntimeError(f"HDF5 headers not found under {HDF5_PREFIX}; events.h5 cannot be built")
     tmp = IO_LIBRARY.with_suffix(".so.tmp")
     _run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17", f"-I{ROOT / 'include'}",
           f"-idirafter{HDF5_PREFIX / 'include'}", str(CSRC / "io_host.cpp"), "-o", str(tmp),
-          f"-L{HDF5_PREFIX / 'lib'}", "-lhdf5", f"-Wl,-rpath,{HDF5_PREFIX / 'lib'}"])
+          f"-L{HDF5_PREFIX / 'lib'}", "-lhdf5", "-lz", f"-Wl,-rpath,{HDF5_PREFIX / 'lib'}"])
     os.replace(tmp, IO_LIBRARY)
     return IO_LIBRARY
 
@@ -94,17 +95,32 @@ BINDIR = PACKAGE / "bin"
 API_SELFTEST = BINDIR / "dvs_api_selftest"
 
 
+# Programs on the C++ API: the self-test and the bin/dvs and bin/cam front
+# ends (apps/main_dvs.cpp, apps/main_cam.cpp).
+PROGRAMS = {"dvs_api_selftest": "api_selftest.cpp", "dvs": "main_dvs.cpp", "cam": "main_cam.cpp"}
+
+
+def _link_program(name: str) -> Path:
+    BINDIR.mkdir(exist_ok=True)
+    rocm_lib = Path("/opt/rocm/lib")
+    exe = BINDIR / name
+    tmp = exe.with_suffix(".tmp")
+    _run(["g++", "-O2", "-std=c++17", f"-I{ROOT / 'include'}", f"-I{CSRC}", str(CSRC / PROGRAMS[name]), "-o",
+          str(tmp), f"-L{LIBDIR}", "-ldvs_hip", "-ldvs_io", f"-Wl,-rpath,$ORIGIN/../lib",
+          f"-Wl,-rpath-link,{rocm_lib}", f"-Wl,-rpath-link,{HDF5_PREFIX / 'lib'}", f"-Wl,-rpath,{rocm_lib}"])
+    os.replace(tmp, exe)
+    return exe
+
+
 def build_api_selftest() -> Path:
     """Links the C++ API self-test (include/dvs_render_amd.hpp) against both
     libraries; run by the tests."""
-    BINDIR.mkdir(exist_ok=True)
-    rocm_lib = Path("/opt/rocm/lib")
-    tmp = API_SELFTEST.with_suffix(".tmp")
-    _run(["g++", "-O2", "-std=c++17", f"-I{ROOT / 'include'}", str(CSRC / "api_selftest.cpp"), "-o", str(tmp),
-          f"-L{LIBDIR}", "-ldvs_hip", "-ldvs_io", f"-Wl,-rpath,$ORIGIN/../lib",
-          f"-Wl,-rpath-link,{rocm_lib}", f"-Wl,-rpath-link,{HDF5_PREFIX / 'lib'}", f"-Wl,-rpath,{rocm_lib}"])
-    os.replace(tmp, API_SELFTEST)
-    return API_SELFTEST
+    return _link_program("dvs_api_selftest")
+
+
+def build_apps() -> list[Path]:
+    """Links bin/dvs and bin/cam."""
+    return [_link_program("dvs"), _link_program("cam")]
 
 
 def build_oracle() -> Path:
@@ -117,5 +133,6 @@ if __name__ == "__main__":
     print(build_library(), file=sys.stderr)
     print(build_io(), file=sys.stderr)
     print(build_api_selftest(), file=sys.stderr)
+    print(build_apps(), file=sys.stderr)
     if "--with-oracle" in sys.argv:
         print(build_oracle(), file=sys.stderr)

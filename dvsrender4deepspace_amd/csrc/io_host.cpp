@@ -1,12 +1,15 @@
-// Host I/O of the DVS path (libdvs_io.so): event stream writers and the YAML
-// configuration, behind the C ABI of include/dvs_io.h.
+// Host I/O of the DVS path (libdvs_io.so): event stream writers, the YAML
+// configuration, the scene bundle and trajectory loaders and the frame image
+// files, behind the C ABI of include/dvs_io.h.
 //
 // Formats follow include/dvs/dvs_writer.hpp (13-byte binary records; HDF5
 // x/y/t/p datasets with microsecond times and a millisecond index) and the
 // configuration grammar and checks of src/scene_types.cpp.
 #include <hdf5.h>
+#include <zlib.h>
 
 #include <algorithm>
+#include <cctype>
 #include <charconv>
 #include <cmath>
 #include <cstdio>
@@ -568,6 +571,283 @@ int dvs_config_load(const char *path, dvs_config *out) {
   return guarded([&] {
     if (!path || !out) fail(DVS_E_ARGUMENT, "invalid configuration arguments");
     load_config(path, *out);
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Scene bundle and trajectory files (src/scene_data.cpp:19-283), frame
+// encode (include/dvs/blenderize.hpp:12-69) and the image files of bin/cam.
+// ---------------------------------------------------------------------------
+namespace {
+
+// read_binary_f32 / read_binary_u32 (scene_data.cpp:32-90): whole file as
+// 4-byte words; the messages name the path.
+template <class T>
+std::vector<T> read_words(const std::string &path, const char *kind, bool with_data) {
+  std::ifstream in(path, std::ios::binary);
+  if (!in) fail(DVS_E_ARGUMENT, "failed to open " + path);
+  in.seekg(0, std::ios::end);
+  const std::streamoff size = in.tellg();
+  in.seekg(0, std::ios::beg);
+  if (size < 0 || size % (std::streamoff)sizeof(T) != 0)
+    fail(DVS_E_ARGUMENT, std::string("invalid ") + kind + " binary size for " + path);
+  std::vector<T> words;
+  if (!with_data) {
+    words.resize((size_t)(size / (std::streamoff)sizeof(T)));  // sizes only; contents unread
+    return words;
+  }
+  words.resize((size_t)(size / (std::streamoff)sizeof(T)));
+  in.read(reinterpret_cast<char *>(words.data()), size);
+  if (!in) fail(DVS_E_ARGUMENT, "failed to read " + path);
+  return words;
+}
+
+std::string join(const char *dir, const char *name) {
+  std::string d(dir);
+  if (!d.empty() && d.back() != '/') d += '/';
+  return d + name;
+}
+
+// Column-major rotation matrix of a quaternion given (w, x, y, z), not
+// normalised first (Eigen's toRotationMatrix on the text file's values).
+void quaternion_matrix(float w, float x, float y, float z, float m[9]) {
+  const float x2 = x + x, y2 = y + y, z2 = z + z;
+  const float wx = x2 * w, wy = y2 * w, wz = z2 * w, xx = x2 * x, xy = y2 * x, xz = z2 * x;
+  const float yy = y2 * y, yz = z2 * y, zz = z2 * z;
+  m[0] = 1 - (yy + zz); m[3] = xy - wz;       m[6] = xz + wy;
+  m[1] = xy + wz;       m[4] = 1 - (xx + zz); m[7] = yz - wx;
+  m[2] = xz - wy;       m[5] = yz + wx;       m[8] = 1 - (xx + yy);
+}
+
+std::string trimmed(const std::string &s) {
+  size_t b = 0, e = s.size();
+  while (b < e && std::isspace((unsigned char)s[b])) ++b;
+  while (e > b && std::isspace((unsigned char)s[e - 1])) --e;
+  return s.substr(b, e - b);
+}
+
+// load_frames_from_text (scene_data.cpp:238-283): one frame per non-blank
+// line "frame_idx x y z qx qy qz qw lx ly lz L"; the timestamp stays NaN
+// (frame_index / fps).
+std::vector<dvs_frame> frames_from_text(const std::string &path) {
+  std::ifstream in(path);
+  if (!in) {
+    std::ifstream probe(path, std::ios::binary);
+    fail(DVS_E_ARGUMENT, (probe ? "failed to open " : "file does not exist: ") + path);
+  }
+  std::vector<dvs_frame> frames;
+  std::string line;
+  while (std::getline(in, line)) {
+    line = trimmed(line);
+    if (line.empty()) continue;
+    std::istringstream fields(line);
+    dvs_frame f{};
+    f.timestamp_seconds = std::numeric_limits<double>::quiet_NaN();
+    float q[4];  // x, y, z, w
+    if (!(fields >> f.frame_index >> f.camera_origin[0] >> f.camera_origin[1] >> f.camera_origin[2] >> q[0] >>
+          q[1] >> q[2] >> q[3] >> f.light_dir[0] >> f.light_dir[1] >> f.light_dir[2] >> f.light_intensity))
+      fail(DVS_E_ARGUMENT, "failed to parse camera orientation in line: " + line);
+    quaternion_matrix(q[3], q[0], q[1], q[2], f.camera_world);
+    frames.push_back(f);
+  }
+  return frames;
+}
+
+// load_frames_from_binary (scene_data.cpp:165-198): 24 floats per frame,
+// {t, origin[3], camera_world as a row-major 4x4, light_dir[3], intensity}.
+std::vector<dvs_frame> frames_from_binary(const char *dir, bool with_data) {
+  const auto raw = read_words<float>(join(dir, "frames.bin"), "f32", with_data);
+  constexpr size_t kStride = 24;
+  if (raw.size() % kStride != 0) fail(DVS_E_ARGUMENT, "frames.bin has invalid stride");
+  std::vector<dvs_frame> frames(raw.size() / kStride);
+  if (!with_data) return frames;
+  for (size_t i = 0; i < frames.size(); ++i) {
+    const float *r = raw.data() + kStride * i;
+    dvs_frame &f = frames[i];
+    f.frame_index = 0;
+    f.timestamp_seconds = r[0];
+    for (int c = 0; c < 3; ++c) f.camera_origin[c] = r[1 + c];
+    for (int row = 0; row < 3; ++row)
+      for (int col = 0; col < 3; ++col) f.camera_world[3 * col + row] = r[4 + 4 * row + col];
+    for (int c = 0; c < 3; ++c) f.light_dir[c] = r[20 + c];
+    f.light_intensity = r[23];
+  }
+  return frames;
+}
+
+// linear_to_srgb, display_exposure, hash dither and to_byte
+// (blenderize.hpp:12-69).
+uint8_t encode_pixel(float linear, int x, int y, float dither) {
+  float v = std::min(std::max(linear / 100, 0.0f), 1.0f);
+  v = v < 0.0031308f ? 12.92f * v : 1.055f * std::pow(v, 1.0f / 2.4f) - 0.055f;
+  if (dither != 0.0f) {
+    const uint32_t ux = (uint32_t)x, uy = (uint32_t)y;
+    const uint32_t hx = 1103515245u * ((ux >> 1u) ^ uy), hy = 1103515245u * ((uy >> 1u) ^ ux);
+    float h = float(1103515245u * (hx ^ (hy >> 3u))) * (1.0f / float(0xffffffffu));
+    h = h * 2.0f - 1.0f;
+    v += dither * (std::copysign(1.0f, h) * (1.0f - std::sqrt(1.0f - std::fabs(h))));
+  }
+  if (!std::isfinite(v) || v <= 0.0f) return 0;
+  if (v >= 1.0f) return 255;
+  return (uint8_t)(v * 255.0f + 0.5f);
+}
+
+void put_be32(std::string &s, uint32_t v) {
+  for (int k = 3; k >= 0; --k) s.push_back((char)((v >> (8 * k)) & 0xff));
+}
+void png_chunk(std::ofstream &out, const char *type, const std::string &data) {
+  std::string buf;
+  put_be32(buf, (uint32_t)data.size());
+  buf.append(type, 4);
+  buf += data;
+  const uLong crc = crc32(crc32(0L, Z_NULL, 0), reinterpret_cast<const Bytef *>(buf.data() + 4),
+                          (uInt)(buf.size() - 4));
+  put_be32(buf, (uint32_t)crc);
+  out.write(buf.data(), (std::streamsize)buf.size());
+}
+
+// 8-bit greyscale PNG (filter 0 on every row, zlib level 6).
+void write_png(const std::string &path, const uint8_t *gray, int width, int height) {
+  std::string raw;
+  raw.reserve((size_t)(width + 1) * height);
+  for (int y = 0; y < height; ++y) {
+    raw.push_back(0);
+    raw.append(reinterpret_cast<const char *>(gray) + (size_t)y * width, (size_t)width);
+  }
+  uLongf packed_size = compressBound((uLong)raw.size());
+  std::string packed(packed_size, '\0');
+  if (compress2(reinterpret_cast<Bytef *>(&packed[0]), &packed_size, reinterpret_cast<const Bytef *>(raw.data()),
+                (uLong)raw.size(), 6) != Z_OK)
+    fail(DVS_E_STATE, "PNG compression failed for " + path);
+  packed.resize(packed_size);
+  std::ofstream out(path, std::ios::binary);
+  if (!out) fail(DVS_E_STATE, "failed to open " + path);
+  out.write("\x89PNG\r\n\x1a\n", 8);
+  std::string ihdr;
+  put_be32(ihdr, (uint32_t)width);
+  put_be32(ihdr, (uint32_t)height);
+  ihdr += std::string("\x08\x00\x00\x00\x00", 5);  // depth 8, greyscale, deflate, filter 0, no interlace
+  png_chunk(out, "IHDR", ihdr);
+  png_chunk(out, "IDAT", packed);
+  png_chunk(out, "IEND", std::string());
+  if (!out) fail(DVS_E_STATE, "failed to write " + path);
+}
+
+// Single-strip, uncompressed little-endian TIFF of float32 samples.
+void write_tiff_f32(const std::string &path, const float *data, int width, int height) {
+  const uint32_t pixels = (uint32_t)width * (uint32_t)height, bytes = pixels * 4;
+  struct Entry { uint16_t tag, type; uint32_t count, value; };
+  const Entry entries[] = {
+      {256, 4, 1, (uint32_t)width},   // ImageWidth
+      {257, 4, 1, (uint32_t)height},  // ImageLength
+      {258, 3, 1, 32},                // BitsPerSample
+      {259, 3, 1, 1},                 // Compression: none
+      {262, 3, 1, 1},                 // Photometric: black is zero
+      {273, 4, 1, 8},                 // StripOffsets: data right after the header
+      {277, 3, 1, 1},                 // SamplesPerPixel
+      {278, 4, 1, (uint32_t)height},  // RowsPerStrip
+      {279, 4, 1, bytes},             // StripByteCounts
+      {284, 3, 1, 1},                 // PlanarConfiguration: chunky
+      {339, 3, 1, 3},                 // SampleFormat: IEEE float
+  };
+  std::ofstream out(path, std::ios::binary);
+  if (!out) fail(DVS_E_STATE, "failed to open " + path);
+  auto u16 = [&](uint16_t v) { out.write(reinterpret_cast<const char *>(&v), 2); };
+  auto u32 = [&](uint32_t v) { out.write(reinterpret_cast<const char *>(&v), 4); };
+  out.write("II", 2);
+  u16(42);
+  u32(8 + bytes);  // first IFD after the pixels
+  out.write(reinterpret_cast<const char *>(data), bytes);
+  const uint16_t n = (uint16_t)(sizeof(entries) / sizeof(entries[0]));
+  u16(n);
+  for (const Entry &e : entries) {
+    u16(e.tag);
+    u16(e.type);
+    u32(e.count);
+    if (e.type == 3) {  // SHORT values sit left-justified in the 4-byte field
+      u16((uint16_t)e.value);
+      u16(0);
+    } else {
+      u32(e.value);
+    }
+  }
+  u32(0);
+  if (!out) fail(DVS_E_STATE, "failed to write " + path);
+}
+
+}  // namespace
+
+extern "C" {
+
+int dvs_io_mesh_load(const char *dir, float *vertices, float *normals, int64_t vertex_capacity, uint32_t *indices,
+                     int64_t triangle_capacity, int64_t *vertex_count, int64_t *triangle_count) {
+  return guarded([&] {
+    if (!dir || !vertex_count || !triangle_count) fail(DVS_E_ARGUMENT, "invalid mesh arguments");
+    const bool with_data = vertices && normals && indices;
+    const auto v = read_words<float>(join(dir, "mesh_vertices.bin"), "f32", with_data);
+    const auto n = read_words<float>(join(dir, "mesh_normals.bin"), "f32", with_data);
+    const auto t = read_words<uint32_t>(join(dir, "mesh_indices.bin"), "u32", with_data);
+    if (v.size() % 3 || n.size() % 3 || t.size() % 3) fail(DVS_E_ARGUMENT, "mesh binaries must be Nx3");
+    if (v.empty() || t.empty() || v.size() != n.size())
+      fail(DVS_E_ARGUMENT, "mesh requires nonempty vertices/triangles and matching normals");
+    *vertex_count = (int64_t)(v.size() / 3);
+    *triangle_count = (int64_t)(t.size() / 3);
+    if (!with_data) return;
+    for (uint32_t i : t)
+      if (i >= v.size() / 3) fail(DVS_E_ARGUMENT, "mesh index outside vertex array");
+    for (float x : v)
+      if (!std::isfinite(x)) fail(DVS_E_ARGUMENT, "nonfinite mesh vertex");
+    for (float x : n)
+      if (!std::isfinite(x)) fail(DVS_E_ARGUMENT, "nonfinite mesh normal");
+    if (vertex_capacity < *vertex_count || triangle_capacity < *triangle_count)
+      fail(DVS_E_ARGUMENT, "mesh buffers are smaller than the mesh");
+    std::memcpy(vertices, v.data(), v.size() * sizeof(float));
+    std::memcpy(normals, n.data(), n.size() * sizeof(float));
+    std::memcpy(indices, t.data(), t.size() * sizeof(uint32_t));
+  });
+}
+
+int dvs_io_frames_binary(const char *dir, dvs_frame *out, int64_t capacity, int64_t *count) {
+  return guarded([&] {
+    if (!dir || !count) fail(DVS_E_ARGUMENT, "invalid frame arguments");
+    const auto frames = frames_from_binary(dir, out != nullptr);
+    *count = (int64_t)frames.size();
+    if (out) std::memcpy(out, frames.data(), (size_t)std::min<int64_t>(capacity, *count) * sizeof(dvs_frame));
+  });
+}
+
+int dvs_io_frames_text(const char *path, dvs_frame *out, int64_t capacity, int64_t *count) {
+  return guarded([&] {
+    if (!path || !count) fail(DVS_E_ARGUMENT, "invalid frame arguments");
+    const auto frames = frames_from_text(path);
+    *count = (int64_t)frames.size();
+    if (out) std::memcpy(out, frames.data(), (size_t)std::min<int64_t>(capacity, *count) * sizeof(dvs_frame));
+  });
+}
+
+int dvs_io_frame_bytes(const float *linear, int32_t width, int32_t height, float dither, uint8_t *out) {
+  return guarded([&] {
+    if (width < 0 || height < 0 || ((int64_t)width * height > 0 && (!linear || !out)))
+      fail(DVS_E_ARGUMENT, "invalid frame encode arguments");
+    for (int y = 0; y < height; ++y)
+      for (int x = 0; x < width; ++x)
+        out[(size_t)y * width + x] = encode_pixel(linear[(size_t)y * width + x], x, y, dither);
+  });
+}
+
+int dvs_io_write_png(const char *path, const uint8_t *gray, int32_t width, int32_t height) {
+  return guarded([&] {
+    if (!path || !gray || width <= 0 || height <= 0) fail(DVS_E_ARGUMENT, "invalid PNG arguments");
+    write_png(path, gray, width, height);
+  });
+}
+
+int dvs_io_write_tiff(const char *path, const float *data, int32_t width, int32_t height) {
+  return guarded([&] {
+    if (!path || !data || width <= 0 || height <= 0) fail(DVS_E_ARGUMENT, "invalid TIFF arguments");
+    write_tiff_f32(path, data, width, height);
   });
 }
 

@@ -1,4 +1,5 @@
-"""Event stream files and the YAML configuration of bin/dvs, over
+"""Event stream files, the YAML configuration, the scene bundle and
+trajectory loaders and the frame images of bin/dvs and bin/cam, over
 libdvs_io.so (include/dvs_io.h).
 
 EventWriter writes events.bin (13-byte records) or events.h5 (x, y, t in
@@ -85,6 +86,13 @@ def load_library() -> ctypes.CDLL:
         "dvs_events_h5_read": (ctypes.c_int, [ctypes.c_char_p, _P, _P, _P, _P, _I64, ctypes.POINTER(_I64)]),
         "dvs_events_h5_ms_to_idx": (ctypes.c_int, [ctypes.c_char_p, _P, _I64, ctypes.POINTER(_I64)]),
         "dvs_config_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_ConfigC)]),
+        "dvs_io_mesh_load": (ctypes.c_int, [ctypes.c_char_p, _P, _P, _I64, _P, _I64, ctypes.POINTER(_I64),
+                                            ctypes.POINTER(_I64)]),
+        "dvs_io_frames_binary": (ctypes.c_int, [ctypes.c_char_p, _P, _I64, ctypes.POINTER(_I64)]),
+        "dvs_io_frames_text": (ctypes.c_int, [ctypes.c_char_p, _P, _I64, ctypes.POINTER(_I64)]),
+        "dvs_io_frame_bytes": (ctypes.c_int, [_P, _I32, _I32, ctypes.c_float, _P]),
+        "dvs_io_write_png": (ctypes.c_int, [ctypes.c_char_p, _P, _I32, _I32]),
+        "dvs_io_write_tiff": (ctypes.c_int, [ctypes.c_char_p, _P, _I32, _I32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -167,3 +175,58 @@ def load_config(path) -> Config:
     scene["rotation"] = tuple(c.scene.rotation)
     return Config(camera=capi.Camera(**cam), physic=capi.Physic(**phys), scene=SceneConfig(**scene),
                   hapke=capi.Hapke(**fields(c.hapke)))
+
+
+def load_mesh(directory) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """load_mesh_data (src/scene_data.cpp:93-163): (vertices (N,3) f32,
+    normals (N,3) f32, indices (M,3) u32); raises ConfigError with the
+    reference's message."""
+    lib = load_library()
+    d = str(directory).encode()
+    nv, nt = _I64(), _I64()
+    _check(lib.dvs_io_mesh_load(d, None, None, 0, None, 0, ctypes.byref(nv), ctypes.byref(nt)))
+    v = np.empty((nv.value, 3), np.float32)
+    n = np.empty((nv.value, 3), np.float32)
+    t = np.empty((nt.value, 3), np.uint32)
+    _check(lib.dvs_io_mesh_load(d, v.ctypes.data, n.ctypes.data, nv.value, t.ctypes.data, nt.value,
+                                ctypes.byref(nv), ctypes.byref(nt)))
+    return v, n, t
+
+
+def _frames(fn, path) -> np.ndarray:
+    p = str(path).encode()
+    count = _I64()
+    _check(fn(p, None, 0, ctypes.byref(count)))
+    out = np.empty(count.value, capi.FRAME_DTYPE)
+    _check(fn(p, out.ctypes.data, count.value, ctypes.byref(count)))
+    return out
+
+
+def load_frames_binary(directory) -> np.ndarray:
+    """load_frames_from_binary (src/scene_data.cpp:165-198): FRAME_DTYPE records."""
+    return _frames(load_library().dvs_io_frames_binary, directory)
+
+
+def load_frames_text(path) -> np.ndarray:
+    """load_frames_from_text (src/scene_data.cpp:238-283): FRAME_DTYPE records."""
+    return _frames(load_library().dvs_io_frames_text, path)
+
+
+def frame_bytes(linear: np.ndarray, dither: float = 0.0) -> np.ndarray:
+    """render_float_to_png_bytes (include/dvs/blenderize.hpp:50-69) of an
+    (H, W) radiance frame."""
+    img = np.ascontiguousarray(linear, dtype=np.float32)
+    out = np.empty(img.shape, np.uint8)
+    _check(load_library().dvs_io_frame_bytes(img.ctypes.data, img.shape[1], img.shape[0], float(dither),
+                                             out.ctypes.data))
+    return out
+
+
+def write_png(path, gray: np.ndarray) -> None:
+    g = np.ascontiguousarray(gray, dtype=np.uint8)
+    _check(load_library().dvs_io_write_png(str(path).encode(), g.ctypes.data, g.shape[1], g.shape[0]))
+
+
+def write_tiff(path, data: np.ndarray) -> None:
+    d = np.ascontiguousarray(data, dtype=np.float32)
+    _check(load_library().dvs_io_write_tiff(str(path).encode(), d.ctypes.data, d.shape[1], d.shape[0]))

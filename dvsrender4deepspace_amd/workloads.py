@@ -169,3 +169,64 @@ def small_variant(w: Workload, size: int = 64, width: int = 32, height: int = 24
                       cam.samples if samples is None else samples),
         terrain=dataclasses.replace(w.terrain, size=size, craters=craters),
         altitude=min(w.altitude, 0.9 * (size - 1) * w.terrain.spacing * cam.fx / max(cam.width, 1)))
+
+
+def quaternion_matrices(q: np.ndarray) -> np.ndarray:
+    """Row-major rotation matrices (..., 3, 3) of unit quaternions (x, y, z, w)."""
+    q = np.asarray(q, np.float64)
+    x, y, z, w = q[..., 0], q[..., 1], q[..., 2], q[..., 3]
+    return np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)], -1),
+        np.stack([2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)], -1),
+        np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], -1)], -2)
+
+
+def write_bundle(w: Workload, directory, frames: int | None = None) -> None:
+    """A scene bundle in the reference's format (src/scene_data.cpp:93-236):
+    mesh_vertices.bin / mesh_normals.bin (float32 N x 3), mesh_indices.bin
+    (uint32 M x 3) of the workload's terrain, and frames.bin with 24 floats
+    per frame {t, origin[3], camera_world as a row-major 4x4, light_dir[3],
+    intensity} sampled from the workload's flight at its fps."""
+    from pathlib import Path
+
+    from . import terrain as terrain_mod
+
+    d = Path(directory)
+    d.mkdir(parents=True, exist_ok=True)
+    v, n, idx = terrain_mod.mesh(w.terrain)
+    np.ascontiguousarray(v, np.float32).tofile(d / "mesh_vertices.bin")
+    np.ascontiguousarray(n, np.float32).tofile(d / "mesh_normals.bin")
+    np.ascontiguousarray(idx, np.uint32).tofile(d / "mesh_indices.bin")
+    count = w.frames if frames is None else frames
+    t = (np.arange(count) / w.fps).astype(np.float32)
+    s = w.states(t)
+    rec = np.zeros((count, 24), np.float32)
+    rec[:, 0] = t
+    rec[:, 1:4] = s["position"]
+    m = np.zeros((count, 4, 4))
+    m[:, :3, :3] = quaternion_matrices(s["rotation"])
+    m[:, :3, 3] = s["position"]
+    m[:, 3, 3] = 1
+    rec[:, 4:20] = m.reshape(count, 16)
+    rec[:, 20:23] = s["light"]
+    rec[:, 23] = s["intensity"]
+    rec.tofile(d / "frames.bin")
+
+
+def write_config(w: Workload, path, frame_count: int | None = None, imu_fps: float = 200.0, cam_fps: float = 30.0,
+                 physic: Physic | None = None) -> None:
+    """The workload as a configuration file (Config's grammar,
+    src/scene_types.cpp): intrinsics, every physic field, scene timing at
+    the workload's fps and time resolution, and its Hapke parameters."""
+    from pathlib import Path
+
+    p = physic or w.physic
+    cam, h = w.camera, w.hapke
+    lines = ["intrinsic:"] + [f"  {k}: {getattr(cam, k)!r}" for k in ("width", "height", "fx", "fy", "cx", "cy")]
+    lines += ["", "physic:"] + [f"  {f.name}: {getattr(p, f.name)!r}" for f in dataclasses.fields(p)]
+    lines += ["", "scene:", f"  frame_count: {w.frames if frame_count is None else frame_count}",
+              f"  fps: {w.fps!r}", f"  imu_fps: {imu_fps!r}", f"  cam_fps: {cam_fps!r}",
+              f"  time_resolution: {w.time_resolution!r}", f"  supersample_factor: {cam.samples}",
+              "  blur_resolution: 1e-3"]
+    lines += ["", "hapke:"] + [f"  {f.name}: {getattr(h, f.name)!r}" for f in dataclasses.fields(h)]
+    Path(path).write_text("\n".join(lines) + "\n")

@@ -1,6 +1,7 @@
 /*
- * Host I/O of the DVS path: the event stream writers of bin/dvs and the YAML
- * configuration.  C ABI of libdvs_io.so (host code, HDF5 1.10 C library);
+ * Host I/O of the DVS path: the event stream writers of bin/dvs, the YAML
+ * configuration, the scene bundle and trajectory loaders, and the frame
+ * images of bin/cam.  C ABI of libdvs_io.so (host code, HDF5 1.10 C library);
  * the status codes and thread-local message follow dvs_hip.h.
  */
 #ifndef DVS_IO_H
@@ -64,6 +65,34 @@ typedef struct {
  * range checks.  Failures return DVS_E_ARGUMENT with the reference's
  * message. */
 int dvs_config_load(const char *path, dvs_config *out);
+
+/* ---- Scene bundle and trajectory files (src/scene_data.cpp) ------------
+ * Every loader fails with DVS_E_ARGUMENT and the reference's message
+ * ("failed to open <path>", "invalid f32 binary size for <path>", ...). */
+/* load_mesh_data (scene_data.cpp:93-163): <dir>/mesh_vertices.bin and
+ * mesh_normals.bin (float32, N x 3) and mesh_indices.bin (uint32, M x 3).
+ * With any buffer NULL only the counts are returned (from the file sizes);
+ * otherwise the buffers hold vertex_capacity vertices and normals and
+ * triangle_capacity triangles, and indices and finiteness are checked. */
+int dvs_io_mesh_load(const char *dir, float *vertices, float *normals, int64_t vertex_capacity, uint32_t *indices,
+                     int64_t triangle_capacity, int64_t *vertex_count, int64_t *triangle_count);
+/* load_frames_from_binary (scene_data.cpp:165-198): <dir>/frames.bin, 24
+ * floats per frame {t, origin[3], camera_world row-major 4x4, light_dir[3],
+ * intensity}; timestamp_seconds = t.  out == NULL returns the count only. */
+int dvs_io_frames_binary(const char *dir, dvs_frame *out, int64_t capacity, int64_t *count);
+/* load_frames_from_text (scene_data.cpp:238-283): non-blank lines
+ * "frame_idx x y z qx qy qz qw lx ly lz L"; camera_world is the rotation
+ * matrix of the quaternion as given (not normalised); timestamps NaN. */
+int dvs_io_frames_text(const char *path, dvs_frame *out, int64_t capacity, int64_t *count);
+
+/* ---- Frame images of bin/cam (apps/main_cam.cpp:79-96) ----------------- */
+/* render_float_to_png_bytes (include/dvs/blenderize.hpp:50-69): exposure
+ * x/100 clamped to [0,1], sRGB, optional hash dither, to_byte. */
+int dvs_io_frame_bytes(const float *linear, int32_t width, int32_t height, float dither, uint8_t *out);
+/* 8-bit greyscale PNG (images/<t>.png). */
+int dvs_io_write_png(const char *path, const uint8_t *gray, int32_t width, int32_t height);
+/* Uncompressed float32 TIFF (depth/<t>.tif: mean hit range, -1 on a miss). */
+int dvs_io_write_tiff(const char *path, const float *data, int32_t width, int32_t height);
 
 #ifdef __cplusplus
 }

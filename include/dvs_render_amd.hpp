@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <iostream>
 #include <limits>
 #include <memory>
 #include <stdexcept>
@@ -365,6 +366,83 @@ class CameraTrajectory {
   }
   std::shared_ptr<dvs_trajectory_s> handle_;
 };
+
+// Scene bundle and trajectory files (src/scene_data.cpp:93-283) over
+// libdvs_io.so: false with the reference's message in *err on failure.
+template <class F3, class U3> struct MeshData {
+  std::vector<F3> vertices, normals;
+  std::vector<U3> indices;
+};
+namespace detail {
+inline bool io_ok(int code, std::string *err) {
+  if (code == DVS_OK) return true;
+  if (err) *err = dvs_io_last_error();
+  return false;
+}
+inline FrameData<Vec3, Mat3> from_c(const dvs_frame &c) {
+  FrameData<Vec3, Mat3> f;
+  f.frame_index = c.frame_index;
+  f.timestamp_seconds = c.timestamp_seconds;
+  f.camera_origin = Vec3(c.camera_origin[0], c.camera_origin[1], c.camera_origin[2]);
+  std::memcpy(f.camera_world.m, c.camera_world, sizeof(c.camera_world));
+  f.light_dir = Vec3(c.light_dir[0], c.light_dir[1], c.light_dir[2]);
+  f.light_intensity = c.light_intensity;
+  return f;
+}
+template <class Load>
+bool load_frames(Load load, const std::string &path, std::vector<FrameData<Vec3, Mat3>> &out, std::string *err) {
+  int64_t n = 0;
+  if (!io_ok(load(path.c_str(), nullptr, 0, &n), err)) return false;
+  std::vector<dvs_frame> c((size_t)n);
+  if (!io_ok(load(path.c_str(), c.data(), n, &n), err)) return false;
+  for (const dvs_frame &f : c) out.push_back(from_c(f));
+  return true;
+}
+}  // namespace detail
+
+inline bool load_mesh_data(const std::string &dir, MeshData<float3, uint3> &out, std::string *err) {
+  int64_t nv = 0, nt = 0;
+  if (!detail::io_ok(dvs_io_mesh_load(dir.c_str(), nullptr, nullptr, 0, nullptr, 0, &nv, &nt), err)) return false;
+  out.vertices.resize((size_t)nv);
+  out.normals.resize((size_t)nv);
+  out.indices.resize((size_t)nt);
+  return detail::io_ok(dvs_io_mesh_load(dir.c_str(), &out.vertices[0].x, &out.normals[0].x, nv, &out.indices[0].x,
+                                        nt, &nv, &nt),
+                       err);
+}
+inline bool load_frames_from_binary(const std::string &dir, std::vector<FrameData<Vec3, Mat3>> &out,
+                                    std::string *err) {
+  out.clear();
+  return detail::load_frames(dvs_io_frames_binary, dir, out, err);
+}
+inline bool load_frames_from_text(const std::string &path, std::vector<FrameData<Vec3, Mat3>> &out,
+                                  std::string *err) {
+  return detail::load_frames(dvs_io_frames_text, path, out, err);
+}
+// load_scene_bundle(bundle_dir) (scene_data.cpp:200-236): mesh and frames.bin.
+inline bool load_scene_bundle(const std::string &bundle_dir, SceneBundle<float3, uint3> &out_bundle,
+                              std::vector<FrameData<Vec3, Mat3>> &frames, std::string *err) {
+  MeshData<float3, uint3> mesh;
+  if (!load_mesh_data(bundle_dir, mesh, err)) return false;
+  std::cout << "Loaded mesh with " << mesh.vertices.size() << " vertices, " << mesh.normals.size()
+            << " normals, and " << mesh.indices.size() << " triangles" << std::endl;
+  if (!load_frames_from_binary(bundle_dir, frames, err)) return false;
+  if (frames.empty()) {
+    if (err) *err = "frames.bin is empty";
+    return false;
+  }
+  out_bundle.bvh_triangles = std::move(mesh.indices);
+  out_bundle.vertices = std::move(mesh.vertices);
+  out_bundle.normals = std::move(mesh.normals);
+  out_bundle.bvh_nodes.clear();
+  return true;
+}
+// render_float_to_png_bytes (include/dvs/blenderize.hpp:50-69).
+inline void render_float_to_png_bytes(const float *linear, int width, int height, float dither,
+                                      std::vector<uint8_t> &out_bytes) {
+  out_bytes.resize((size_t)width * height);
+  detail::check_io(dvs_io_frame_bytes(linear, width, height, dither, out_bytes.data()));
+}
 
 // OpticalRefinementStats / ReceptorProbeResult of the adaptive solver.
 using OpticalRefinementStats = dvs_refinement_stats;
