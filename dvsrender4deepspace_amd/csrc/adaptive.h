@@ -97,7 +97,8 @@ using dvsdev::AdaptEntry;
 using dvsdev::AdaptSegStats;
 using dvsdev::AdaptTerm;
 
-hipError_t launch_adapt_probe_poses(int nseg, const SegDev *segs, const AdaptDev *ad, Pose *out, hipStream_t st);
+hipError_t launch_adapt_probe_poses(int nseg, const SegDev *segs, const AdaptDev *ad, Pose *out,
+                                    const Pose *prev_end, Pose *cur_end, int *reuse, hipStream_t st);
 hipError_t launch_adapt_cell_frac(int cells, const double *t0, const double *t1, const int *cell_begin, int nseg,
                                   const AdaptDev *ad, double *frac, hipStream_t st);
 // Radiance of `items` pixel samples: LAYOUT_PROBE item q is probe q%5 of
@@ -107,7 +108,7 @@ hipError_t launch_adapt_cell_frac(int cells, const double *t0, const double *t1,
 hipError_t launch_adapt_trace(int layout, const SegDev *segs, const int *pixel_begin, int nseg, int samples,
                               long long items, const int *item_pixel, const int *item_pose, const Pose *poses,
                               size_t lds, double *out, int *status, unsigned long long *shadow_count,
-                              hipStream_t st);
+                              hipStream_t st, const int *reuse = nullptr, const double *prev_probes = nullptr);
 hipError_t launch_adapt_init(int pixels, AdaptBufs b, AdaptEntry *entries, hipStream_t st);
 // One level: resolved segments to terms; a bisected segment whose halves'
 // midpoints were traced ahead writes its halves to `direct` (ctr->direct

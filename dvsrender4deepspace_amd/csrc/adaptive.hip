@@ -36,12 +36,27 @@ __device__ inline Pose native_pose(const dvs_optical_interval *ivs, int count, d
   return pose_at(ivs[lo], t);
 }
 
-__global__ void adapt_probe_poses(int nseg, const SegDev *segs, const AdaptDev *ad, Pose *out) {
+// The five probe poses of every segment.  The end pose is kept (cur_end);
+// the start probe of a segment reuses the previous window's end radiance
+// when its pose equals the kept end pose bit for bit (prev_end, given only
+// while the segment layout is unchanged): the trace is a function of the
+// pose, the scene and the pixel, so the radiance is the one a trace gives.
+__device__ __forceinline__ bool same_pose(const Pose &a, const Pose &b) {
+  const unsigned *x = reinterpret_cast<const unsigned *>(&a), *y = reinterpret_cast<const unsigned *>(&b);
+  bool same = true;
+  for (int w = 0; w < 11; ++w) same &= x[w] == y[w];  // every field but the padding
+  return same;
+}
+__global__ void adapt_probe_poses(int nseg, const SegDev *segs, const AdaptDev *ad, Pose *out, const Pose *prev_end,
+                                  Pose *cur_end, int *reuse) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nseg * 5) return;
   const int s = k / 5, j = k - s * 5;
   const double start = ad[s].start, end = ad[s].end;
-  out[k] = native_pose(segs[s].ivs, segs[s].iv_count, start + (end - start) * j * .25);
+  const Pose pose = native_pose(segs[s].ivs, segs[s].iv_count, start + (end - start) * j * .25);
+  out[k] = pose;
+  if (j == 4) cur_end[s] = pose;
+  if (j == 0) reuse[s] = prev_end != nullptr && same_pose(pose, prev_end[s]);
 }
 
 __global__ void adapt_cell_frac(int cells, const double *t0, const double *t1, const int *cell_begin, int nseg,
@@ -52,11 +67,15 @@ __global__ void adapt_cell_frac(int cells, const double *t0, const double *t1, c
   frac[c] = ((t0[c] + t1[c]) * .5 - a.start) / (a.end - a.start);
 }
 
+// LAYOUT_PROBE items run probe-major (item q is probe q / P of pixel q % P,
+// written to out[pixel * 5 + probe]), so that reused start probes fill whole
+// blocks that trace nothing.
 template <int LAYOUT>
 __global__ __launch_bounds__(256) void adapt_trace(const SegDev *segs, const int *pixel_begin, int nseg,
                                                    long long items, const int *item_pixel, const int *item_pose,
                                                    const Pose *poses, double *out, int *status,
-                                                   unsigned long long *shadow_count) {
+                                                   unsigned long long *shadow_count, const int *reuse,
+                                                   const double *prev_probes) {
   extern __shared__ double lds_words[];
   int *words = reinterpret_cast<int *>(lds_words);
   int *stack = words + threadIdx.x;
@@ -65,13 +84,19 @@ __global__ __launch_bounds__(256) void adapt_trace(const SegDev *segs, const int
   const long long q = (long long)blockIdx.x * slots + slot;
   const bool active = slot < slots && q < items;
   SubSample r{0.0, -1.f, 0, 0};
+  long long at = q;  // output index
+  bool reused = false;
   if (active) {
     int gp;
     Pose pose;
     if (LAYOUT == LAYOUT_PROBE) {
-      gp = (int)(q / 5);
+      const long long P = items / 5;
+      const int j = (int)(q / P);
+      gp = (int)(q - (long long)j * P);
       const int si = segment_of<int>(pixel_begin, nseg, gp);
-      pose = poses[si * 5 + (int)(q - (long long)gp * 5)];
+      pose = poses[si * 5 + j];
+      at = (long long)gp * 5 + j;
+      reused = j == 0 && reuse[si];
     } else if (LAYOUT == LAYOUT_QUERY) {
       gp = item_pixel[q];
       pose = poses[q];
@@ -80,7 +105,8 @@ __global__ __launch_bounds__(256) void adapt_trace(const SegDev *segs, const int
       pose = poses[item_pose[q]];
     }
     const int si = segment_of<int>(pixel_begin, nseg, gp);
-    r = sample_subray<true>(segs[si].scene, gp - pixel_begin[si], pose, sub % samples, sub / samples, stack, 256);
+    if (!reused)
+      r = sample_subray<true>(segs[si].scene, gp - pixel_begin[si], pose, sub % samples, sub / samples, stack, 256);
   }
   stack[0] = __double2loint(r.radiance);
   stack[256] = __double2hiint(r.radiance);
@@ -92,10 +118,16 @@ __global__ __launch_bounds__(256) void adapt_trace(const SegDev *segs, const int
   }
   __syncthreads();
   if (!active || sub != 0) return;
+  if (reused) {
+    const double kept = prev_probes[at + 4];  // the previous window's end probe of this pixel
+    if (!isfinite(kept)) atomicOr(status, 2);
+    out[at] = kept;
+    return;
+  }
   double total = 0;
   for (int j = 0; j < R; ++j) total += __hiloint2double(words[256 + threadIdx.x + j], words[threadIdx.x + j]);
   if (!isfinite(total)) atomicOr(status, 2);
-  out[q] = total / R;
+  out[at] = total / R;
 }
 
 // Two root segments per pixel: [0, 1/2] is popped first with one pending
@@ -541,9 +573,10 @@ namespace {
 inline unsigned grid(long long n, int block = 256) { return (unsigned)((n + block - 1) / block); }
 }  // namespace
 
-hipError_t launch_adapt_probe_poses(int nseg, const SegDev *segs, const AdaptDev *ad, Pose *out, hipStream_t st) {
+hipError_t launch_adapt_probe_poses(int nseg, const SegDev *segs, const AdaptDev *ad, Pose *out,
+                                    const Pose *prev_end, Pose *cur_end, int *reuse, hipStream_t st) {
   if (nseg <= 0) return hipSuccess;
-  adapt_probe_poses<<<grid(nseg * 5), 256, 0, st>>>(nseg, segs, ad, out);
+  adapt_probe_poses<<<grid(nseg * 5), 256, 0, st>>>(nseg, segs, ad, out, prev_end, cur_end, reuse);
   return hipGetLastError();
 }
 hipError_t launch_adapt_cell_frac(int cells, const double *t0, const double *t1, const int *cell_begin, int nseg,
@@ -555,7 +588,8 @@ hipError_t launch_adapt_cell_frac(int cells, const double *t0, const double *t1,
 hipError_t launch_adapt_trace(int layout, const SegDev *segs, const int *pixel_begin, int nseg, int samples,
                               long long items, const int *item_pixel, const int *item_pose, const Pose *poses,
                               size_t lds, double *out, int *status, unsigned long long *shadow_count,
-                              hipStream_t st) {
+                              hipStream_t st, const int *reuse, const double *prev_probes) {
+  if (layout == LAYOUT_PROBE && (items % 5 != 0 || !reuse)) return hipErrorInvalidValue;
   if (items <= 0) return hipSuccess;
   const int slots = 256 / (samples * samples);
   const long long blocks = (items + slots - 1) / slots;
@@ -563,15 +597,18 @@ hipError_t launch_adapt_trace(int layout, const SegDev *segs, const int *pixel_b
   switch (layout) {
     case LAYOUT_PROBE:
       adapt_trace<LAYOUT_PROBE><<<(unsigned)blocks, 256, lds, st>>>(segs, pixel_begin, nseg, items, item_pixel,
-                                                                     item_pose, poses, out, status, shadow_count);
+                                                                     item_pose, poses, out, status, shadow_count,
+                                                                     reuse, prev_probes);
       break;
     case LAYOUT_QUERY:
       adapt_trace<LAYOUT_QUERY><<<(unsigned)blocks, 256, lds, st>>>(segs, pixel_begin, nseg, items, item_pixel,
-                                                                     item_pose, poses, out, status, shadow_count);
+                                                                     item_pose, poses, out, status, shadow_count,
+                                                                     reuse, prev_probes);
       break;
     default:
       adapt_trace<LAYOUT_CELL><<<(unsigned)blocks, 256, lds, st>>>(segs, pixel_begin, nseg, items, item_pixel,
-                                                                    item_pose, poses, out, status, shadow_count);
+                                                                    item_pose, poses, out, status, shadow_count,
+                                                                     reuse, prev_probes);
   }
   return hipGetLastError();
 }

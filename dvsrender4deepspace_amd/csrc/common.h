@@ -44,6 +44,18 @@ template <class F> int guarded(F &&f) {
   }
 }
 
+// Waits for `st` by polling it.  A blocking synchronize can put the host
+// thread to sleep and wake it tens of microseconds after the stream drains;
+// between the small read-backs and launches of one step that wake-up is
+// most of the host's share of the step.
+inline void stream_wait(hipStream_t st, const char *what) {
+  for (;;) {
+    const hipError_t e = hipStreamQuery(st);
+    if (e == hipSuccess) return;
+    if (e != hipErrorNotReady) hip_check(e, what);
+  }
+}
+
 // Owning device allocation that grows on demand (contents are not kept).
 template <class T> struct DeviceArray {
   T *ptr = nullptr;

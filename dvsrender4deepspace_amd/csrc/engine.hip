@@ -621,7 +621,7 @@ class Engine {
     unsigned long long shadows = 0;
     hip_check(hipMemcpyAsync(&status, status_.ptr, sizeof(int), hipMemcpyDeviceToHost, st), "copy status");
     hip_check(hipMemcpyAsync(&total, ends_.ptr + pixel_total - 1, sizeof(int), hipMemcpyDeviceToHost, st), "copy count");
-    hip_check(hipStreamSynchronize(st), "copy count");
+    stream_wait(st, "copy count");
     if (status & 1) fail(DVS_E_CAPACITY, "pixel event capacity exceeded; reduce numerical window");
     if (status) fail(DVS_E_OPTICAL, "invalid optical sample or BVH traversal overflow");
     std::vector<dvs_event> sorted(total);
@@ -690,7 +690,7 @@ class Engine {
     hip_check(hipEventRecord(events_t.stop, st), "record");
     hip_check(hipMemcpyAsync(&status, status_.ptr, sizeof(int), hipMemcpyDeviceToHost, st), "copy status");
     hip_check(hipMemcpyAsync(&shadows, counter_.ptr, sizeof(shadows), hipMemcpyDeviceToHost, st), "ray counter");
-    hip_check(hipStreamSynchronize(st), "copy events");
+    stream_wait(st, "copy events");
     if (status) fail(DVS_E_OPTICAL, "invalid optical sample or BVH traversal overflow");
     const double events_ms = events_t.ms();
     stats.events_ms += events_ms;
@@ -771,18 +771,36 @@ class Engine {
                                      cell_frac_.ptr, st),
               "cell fractions");
     hip_check(hipEventRecord(refine_t.start, st), "record");
-    // Five initial probes per pixel.
+    // Five initial probes per pixel.  The probes alternate between two
+    // buffers across windows: a start probe whose pose equals the previous
+    // window's end pose takes that probe's radiance, provided the segments
+    // and their pixels are laid out as in the previous adaptive step.
+    std::vector<long long> layout;
+    for (auto &w : work) {
+      layout.push_back(w.seg->id);
+      layout.push_back(w.seg->npix);
+    }
+    const int pprev = probe_cur_, pcur = 1 - pprev;
+    const bool reuse_layout = layout == probe_layout_;
     probe_poses_.reserve((size_t)nseg * 5, "probe poses");
-    hip_check(launch_adapt_probe_poses(nseg, segtab_.ptr, adapt_.ptr, probe_poses_.ptr, st), "probe poses");
-    probes_.reserve((size_t)P * 5, "optical probes");
+    end_pose_[pcur].reserve(nseg, "probe poses");
+    probe_reuse_.reserve(nseg, "probe poses");
+    hip_check(launch_adapt_probe_poses(nseg, segtab_.ptr, adapt_.ptr, probe_poses_.ptr,
+                                       reuse_layout ? end_pose_[pprev].ptr : nullptr, end_pose_[pcur].ptr,
+                                       probe_reuse_.ptr, st),
+              "probe poses");
+    probes_[pcur].reserve((size_t)P * 5, "optical probes");
     hip_check(launch_adapt_trace(dvsdev::LAYOUT_PROBE, segtab_.ptr, pbegin_.ptr, nseg, samples, (long long)P * 5,
-                                 nullptr, nullptr, probe_poses_.ptr, lds, probes_.ptr, status_.ptr, counter_.ptr, st),
+                                 nullptr, nullptr, probe_poses_.ptr, lds, probes_[pcur].ptr, status_.ptr, counter_.ptr,
+                                 st, probe_reuse_.ptr, probes_[pprev].ptr),
               "refine pixel optical launch");
+    probe_layout_ = std::move(layout);
+    probe_cur_ = pcur;
     for (auto *a : {&bound_, &flags_, &bisects_, &exact_terms_, &kcount_, &koff_, &ecount_, &eoff_, &fb_index_,
                     &fb_scan_, &run_begin_})
       a->reserve(P, "optical refinement state");
     AdaptBufs b{};
-    b.probes = probes_.ptr;
+    b.probes = probes_[pcur].ptr;
     b.bound = bound_.ptr;
     b.flags = flags_.ptr;
     b.bisects = bisects_.ptr;
@@ -817,7 +835,7 @@ class Engine {
                                     tkeys_.ptr, bis_.ptr, qpix_.ptr, qpose_.ptr, ent_[1 - cur].ptr, ctr_.ptr, st),
                 "refine pixel optical launch");
       hip_check(hipMemcpyAsync(&host_ctr, ctr_.ptr, sizeof(host_ctr), hipMemcpyDeviceToHost, st), "refinement count");
-      hip_check(hipStreamSynchronize(st), "refinement count");
+      stream_wait(st, "refinement count");
       terms_total = host_ctr.terms;
       const long long nb = host_ctr.bisect, nd = host_ctr.direct;
       if (nb + nd == 0) break;
@@ -841,7 +859,7 @@ class Engine {
     hip_check(inclusive_scan_int(temp_.ptr, temp_bytes, kcount_.ptr, koff_.ptr, P, st), "knot offsets");
     int knots_total = 0;
     hip_check(hipMemcpyAsync(&knots_total, koff_.ptr + P - 1, sizeof(int), hipMemcpyDeviceToHost, st), "knot count");
-    hip_check(hipStreamSynchronize(st), "knot count");
+    stream_wait(st, "knot count");
     knots_.reserve(std::max(knots_total, 1), "pixel optical knots");
     b.knots = knots_.ptr;
     if (terms_total > 0) {
@@ -880,7 +898,7 @@ class Engine {
     hip_check(inclusive_scan_int(temp_.ptr, temp_bytes, ecount_.ptr, eoff_.ptr, P, st), "local exact query count");
     int exact_total = 0;
     hip_check(hipMemcpyAsync(&exact_total, eoff_.ptr + P - 1, sizeof(int), hipMemcpyDeviceToHost, st), "local exact query count");
-    hip_check(hipStreamSynchronize(st), "local exact query count");
+    stream_wait(st, "local exact query count");
     ecell_.reserve(std::max(exact_total, 1), "local exact optical cells");
     erad_.reserve(std::max(exact_total, 1), "local exact radiance");
     eitem_pix_.reserve(std::max(exact_total, 1), "compact exact ray queue");
@@ -906,7 +924,7 @@ class Engine {
     hip_check(launch_adapt_segment_ends(nseg, pbegin_.ptr, fb_scan_.ptr, fb_ends_.ptr, st), "fallback counts");
     std::vector<int> fb_ends(nseg);
     hip_check(hipMemcpyAsync(fb_ends.data(), fb_ends_.ptr, sizeof(int) * nseg, hipMemcpyDeviceToHost, st), "fallback counts");
-    hip_check(hipStreamSynchronize(st), "fallback counts");
+    stream_wait(st, "fallback counts");
     const int fb_total = nseg ? fb_ends[nseg - 1] : 0;
     fb_list_.reserve(std::max(fb_total, 1), "exact fallback pixel IDs");
     hip_check(launch_adapt_fallback_list(P, adapt_.ptr, pbegin_.ptr, nseg, fb_scan_.ptr, b, fb_list_.ptr, st),
@@ -961,7 +979,7 @@ class Engine {
       std::vector<double> errors((size_t)ngate * 1024);
       hip_check(hipMemcpyAsync(errors.data(), gate_err_.ptr, sizeof(double) * errors.size(), hipMemcpyDeviceToHost, st),
                 "download mean receptor convergence");
-      hip_check(hipStreamSynchronize(st), "download mean receptor convergence");
+      stream_wait(st, "download mean receptor convergence");
       bool changed = false;
       for (int g = 0; g < ngate; ++g) {
         StepSegment &w = work[gate_seg[g]];
@@ -1084,7 +1102,7 @@ class Engine {
                                        status_.ptr, st),
                 "adaptive circuit launch");
       hip_check(hipEventRecord(integrate_t.back()->stop, st), "record");
-      if (fb_total > 0) hip_check(hipStreamSynchronize(st), "exact fallback circuit tile");  // tables are reused
+      if (fb_total > 0) stream_wait(st, "exact fallback circuit tile");  // tables are reused
     }
     hip_check(hipEventRecord(circuit_t.stop, st), "record");
     hip_check(hipEventRecord(step_t.stop, st), "record");
@@ -1144,7 +1162,14 @@ class Engine {
   DeviceArray<int> cellbeg_, bound_, flags_, bisects_, exact_terms_, kcount_, koff_, ecount_, eoff_, fb_index_,
       fb_scan_, run_begin_, qpix_, contrib_, cscan_, ecell_, eitem_pix_, eitem_pose_, fb_ends_, fb_list_, gate_ids_,
       gate_idx_, fb_items_;
-  DeviceArray<double> cell_frac_, probes_, qrad_, erad_, gate_rad_, gate_err_, rates_;
+  DeviceArray<double> cell_frac_, qrad_, erad_, gate_rad_, gate_err_, rates_;
+  // probe radiance and end poses of the last two adaptive windows, and the
+  // segment layout (id, pixels) of the last one
+  DeviceArray<double> probes_[2];
+  DeviceArray<Pose> end_pose_[2];
+  DeviceArray<int> probe_reuse_;
+  std::vector<long long> probe_layout_;
+  int probe_cur_ = 0;
   DeviceArray<int> rare_native_, rare_adapt_;  // per-wave flags of the photocurrent's second phase
   DeviceArray<Pose> probe_poses_, qpose_;
   DeviceArray<AdaptEntry> ent_[2], bis_;
