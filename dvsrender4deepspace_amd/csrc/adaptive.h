@@ -77,9 +77,11 @@ struct AdaptBufs {
   double *rates;         // [cell - chunk start][pixel] photocurrent of knot pixels
 };
 
-struct AdaptCounters {
-  int terms, bisect, direct, pad;  // resolved, traced and untraced bisections
-};
+// Refinement counters of one window: the resolved segments so far, then
+// the traced and untraced bisections of each level (level L at 2 + 2L), so
+// that no level needs its counters cleared on its own.
+constexpr int kMaxRefineLevels = 64;
+constexpr int kRefineCounterWords = 2 + 2 * kMaxRefineLevels;
 
 // OpticalRefinementStats of one segment (include/dvs/optical_sampler.hpp:16).
 struct AdaptSegStats {
@@ -91,7 +93,8 @@ struct AdaptSegStats {
 
 namespace dvsamd {
 using dvsdev::AdaptBufs;
-using dvsdev::AdaptCounters;
+using dvsdev::kMaxRefineLevels;
+using dvsdev::kRefineCounterWords;
 using dvsdev::AdaptDev;
 using dvsdev::AdaptEntry;
 using dvsdev::AdaptSegStats;
@@ -117,7 +120,7 @@ hipError_t launch_adapt_init(int pixels, AdaptBufs b, AdaptEntry *entries, hipSt
 hipError_t launch_adapt_decide(int n, const AdaptEntry *entries, const SegDev *segs, const AdaptDev *ad,
                                const int *pixel_begin, int nseg, AdaptBufs b, AdaptTerm *terms,
                                unsigned long long *keys, AdaptEntry *bisected, int *query_pixel, Pose *query_pose,
-                               AdaptEntry *direct, AdaptCounters *ctr, hipStream_t st);
+                               AdaptEntry *direct, int *terms_ctr, int *level_ctr, hipStream_t st);
 // The halves of the traced bisections, with their halves' midpoints.
 hipError_t launch_adapt_children(int n, const AdaptEntry *bisected, const double *query_rad, AdaptEntry *next,
                                  hipStream_t st);

@@ -148,7 +148,7 @@ __global__ __launch_bounds__(256) void adapt_decide(int n, const AdaptEntry *ent
                                                     const AdaptDev *ad, const int *pixel_begin, int nseg,
                                                     AdaptBufs b, AdaptTerm *terms, unsigned long long *keys,
                                                     AdaptEntry *bisected, int *query_pixel, Pose *query_pose,
-                                                    AdaptEntry *direct, AdaptCounters *ctr) {
+                                                    AdaptEntry *direct, int *terms_ctr, int *level_ctr) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   int decision = -1;  // -1 dropped, ADAPT_ACCEPT, ADAPT_EXACT, 2 bisect
   AdaptEntry e{};
@@ -189,9 +189,9 @@ __global__ __launch_bounds__(256) void adapt_decide(int n, const AdaptEntry *ent
   const int lane = threadIdx.x & 63;
   int tbase = 0, bbase = 0, dbase = 0;
   if (lane == 0) {
-    if (tmask) tbase = atomicAdd(&ctr->terms, __popcll(tmask));
-    if (bmask) bbase = atomicAdd(&ctr->bisect, __popcll(bmask));
-    if (dmask) dbase = atomicAdd(&ctr->direct, __popcll(dmask));
+    if (tmask) tbase = atomicAdd(terms_ctr, __popcll(tmask));
+    if (bmask) bbase = atomicAdd(&level_ctr[0], __popcll(bmask));
+    if (dmask) dbase = atomicAdd(&level_ctr[1], __popcll(dmask));
   }
   tbase = __shfl(tbase, 0);
   bbase = __shfl(bbase, 0);
@@ -369,13 +369,16 @@ __global__ void adapt_segment_ends(int nseg, const int *pixel_begin, const int *
 }
 
 // One block per segment: OpticalRefinementStats over its pixels.
+// Per-segment refinement statistics: kStatBlocks blocks per segment each
+// reduce a strided share of its pixels and add it to out (cleared first).
+constexpr int kStatBlocks = 64;
 __global__ __launch_bounds__(256) void adapt_segment_stats(const int *pixel_begin, AdaptBufs b, AdaptSegStats *out) {
   __shared__ unsigned long long q[256];
   __shared__ int fb[256], mk[256], ex[256], nc[256];
-  const int s = blockIdx.x;
+  const int s = blockIdx.y;
   unsigned long long queries = 0;
   int fallback = 0, max_knots = 0, exact = 0, cells = 0;
-  for (int p = pixel_begin[s] + threadIdx.x; p < pixel_begin[s + 1]; p += 256) {
+  for (int p = pixel_begin[s] + (int)blockIdx.x * 256 + threadIdx.x; p < pixel_begin[s + 1]; p += kStatBlocks * 256) {
     queries += 5 + 2ull * (unsigned)b.bisects[p];
     fallback += b.flags[p] & 1;
     max_knots = max(max_knots, b.kcount[p]);
@@ -398,7 +401,14 @@ __global__ __launch_bounds__(256) void adapt_segment_stats(const int *pixel_begi
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[s] = AdaptSegStats{q[0], fb[0], mk[0], ex[0], nc[0]};
+  if (threadIdx.x == 0 && (q[0] | fb[0] | mk[0] | ex[0] | nc[0])) {
+    AdaptSegStats &o = out[s];
+    atomicAdd(&o.traced_pixel_samples, q[0]);
+    atomicAdd(&o.fallback_pixels, fb[0]);
+    atomicMax(&o.maximum_knots, mk[0]);
+    atomicAdd(&o.exact_segments, ex[0]);
+    atomicAdd(&o.native_cell_queries, nc[0]);
+  }
 }
 
 // Receptor log voltage of each probe pixel under native and under cached
@@ -620,10 +630,10 @@ hipError_t launch_adapt_init(int pixels, AdaptBufs b, AdaptEntry *entries, hipSt
 hipError_t launch_adapt_decide(int n, const AdaptEntry *entries, const SegDev *segs, const AdaptDev *ad,
                                const int *pixel_begin, int nseg, AdaptBufs b, AdaptTerm *terms,
                                unsigned long long *keys, AdaptEntry *bisected, int *query_pixel, Pose *query_pose,
-                               AdaptEntry *direct, AdaptCounters *ctr, hipStream_t st) {
+                               AdaptEntry *direct, int *terms_ctr, int *level_ctr, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   adapt_decide<<<grid(n), 256, 0, st>>>(n, entries, segs, ad, pixel_begin, nseg, b, terms, keys, bisected,
-                                         query_pixel, query_pose, direct, ctr);
+                                         query_pixel, query_pose, direct, terms_ctr, level_ctr);
   return hipGetLastError();
 }
 hipError_t launch_adapt_children(int n, const AdaptEntry *bisected, const double *query_rad, AdaptEntry *next,
@@ -688,7 +698,10 @@ hipError_t launch_adapt_segment_ends(int nseg, const int *pixel_begin, const int
 hipError_t launch_adapt_segment_stats(int nseg, const int *pixel_begin, AdaptBufs b, AdaptSegStats *out,
                                       hipStream_t st) {
   if (nseg <= 0) return hipSuccess;
-  adapt_segment_stats<<<nseg, 256, 0, st>>>(pixel_begin, b, out);
+  static_assert(sizeof(AdaptSegStats) % sizeof(int) == 0, "statistics are cleared word by word");
+  const hipError_t e = launch_fill_int(reinterpret_cast<int *>(out), 0, nseg * (int)(sizeof(AdaptSegStats) / 4), st);
+  if (e != hipSuccess) return e;
+  adapt_segment_stats<<<dim3(kStatBlocks, nseg), 256, 0, st>>>(pixel_begin, b, out);
   return hipGetLastError();
 }
 hipError_t launch_adapt_gate(const SegDev *gate_segs, const int *gate_seg, int ngate, const AdaptDev *ad,

@@ -4,8 +4,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <stdexcept>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "dvs_hip.h"
 
@@ -56,6 +59,57 @@ inline void stream_wait(hipStream_t st, const char *what) {
   }
 }
 
+// Page-locked staging for the small host-to-device copies of one engine
+// step.  A copy from pageable memory blocks the host until the stream has
+// drained up to it; from page-locked memory it is queued and returns.  The
+// staging is a bump allocator: regions stay untouched until reset(), which
+// the engine calls when a step starts (the previous step's copies have
+// completed by then, its results having been read back).
+class Staging {
+ public:
+  Staging() = default;
+  Staging(const Staging &) = delete;
+  Staging &operator=(const Staging &) = delete;
+  ~Staging() {
+    for (auto &b : blocks_) (void)hipHostFree(b.first);
+  }
+  void reset() {
+    if (blocks_.size() > 1) {  // one block of the combined size from now on
+      size_t total = 0;
+      for (auto &b : blocks_) {
+        total += b.second;
+        (void)hipHostFree(b.first);
+      }
+      blocks_.clear();
+      grow(total);
+    }
+    used_ = 0;
+  }
+  // Copies n elements to staging and queues their copy to dst on st.
+  template <class T> void upload(T *dst, const T *src, size_t n, const char *what, hipStream_t st) {
+    if (!n) return;
+    const size_t bytes = n * sizeof(T), at = (used_ + 255) & ~size_t(255);
+    if (blocks_.empty() || at + bytes > blocks_.back().second) {
+      grow(std::max(bytes, blocks_.empty() ? size_t(1) << 20 : 2 * blocks_.back().second));
+      return upload(dst, src, n, what, st);
+    }
+    char *p = blocks_.back().first + at;
+    std::memcpy(p, src, bytes);
+    used_ = at + bytes;
+    hip_check(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, st), what);
+  }
+
+ private:
+  void grow(size_t bytes) {
+    void *p = nullptr;
+    hip_check(hipHostMalloc(&p, bytes, hipHostMallocDefault), "page-locked staging");
+    blocks_.emplace_back(static_cast<char *>(p), bytes);
+    used_ = 0;
+  }
+  std::vector<std::pair<char *, size_t>> blocks_;
+  size_t used_ = 0;
+};
+
 // Owning device allocation that grows on demand (contents are not kept).
 template <class T> struct DeviceArray {
   T *ptr = nullptr;
@@ -78,6 +132,10 @@ template <class T> struct DeviceArray {
   void upload(const T *host, size_t n, const char *what, hipStream_t st = nullptr) {
     reserve(n, what);
     if (n) hip_check(hipMemcpyAsync(ptr, host, n * sizeof(T), hipMemcpyHostToDevice, st), what);
+  }
+  void upload(const T *host, size_t n, const char *what, hipStream_t st, Staging &staging) {
+    reserve(n, what);
+    staging.upload(ptr, host, n, what, st);
   }
   // Grows to at least n elements, keeping the first `used` (stream ordered).
   void grow_keep(size_t n, size_t used, const char *what, hipStream_t st) {

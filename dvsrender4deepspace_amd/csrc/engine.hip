@@ -389,6 +389,8 @@ class Engine {
   // block offsets.  Returns the trace kernels' LDS bytes.
   size_t prepare(std::vector<StepSegment> &work, int pixel_total, int cell_total, int iv_total) {
     hipStream_t st = stream_;
+    stream_wait(st, "step start");  // the previous step's staged copies are done
+    staging_.reset();
     std::vector<dvs_optical_interval> ivs;
     std::vector<double> t0s, t1s;
     std::vector<int> civ;
@@ -410,17 +412,17 @@ class Engine {
       }
       w.cells = (int)(t0s.size()) - w.cell_offset;
     }
-    intervals_.upload(ivs.data(), ivs.size(), "trajectory interval upload", st);
-    cell_t0_.upload(t0s.data(), t0s.size(), "cell times", st);
-    cell_t1_.upload(t1s.data(), t1s.size(), "cell times", st);
-    cell_iv_.upload(civ.data(), civ.size(), "cell intervals", st);
+    intervals_.upload(ivs.data(), ivs.size(), "trajectory interval upload", st, staging_);
+    cell_t0_.upload(t0s.data(), t0s.size(), "cell times", st, staging_);
+    cell_t1_.upload(t1s.data(), t1s.size(), "cell times", st, staging_);
+    cell_iv_.upload(civ.data(), civ.size(), "cell intervals", st, staging_);
     poses_.reserve(cell_total, "cell poses");
     counts_.reserve(pixel_total, "event counts");
     ends_.reserve(pixel_total, "event offsets");
     status_.reserve(1, "error flag");
     counter_.reserve(1, "ray counter");
-    hip_check(hipMemsetAsync(status_.ptr, 0, sizeof(int), st), "clear status");
-    hip_check(hipMemsetAsync(counter_.ptr, 0, sizeof(unsigned long long), st), "clear ray counter");
+    hip_check(launch_fill_int(status_.ptr, 0, 1, st), "clear status");
+    hip_check(launch_fill_int(reinterpret_cast<int *>(counter_.ptr), 0, 2, st), "clear ray counter");
     const int nseg = (int)work.size();
     std::vector<int> cblocks(nseg + 1), pbegin(nseg + 1);
     cblocks[0] = 0;
@@ -432,8 +434,8 @@ class Engine {
       const int depth = std::max(3, work[i].seg->scene->view.stack_depth);
       lds = std::max(lds, (size_t)depth * 256 * sizeof(int));
     }
-    cblock_.upload(cblocks.data(), cblocks.size(), "segment blocks", st);
-    pbegin_.upload(pbegin.data(), pbegin.size(), "segment pixels", st);
+    cblock_.upload(cblocks.data(), cblocks.size(), "segment blocks", st, staging_);
+    pbegin_.upload(pbegin.data(), pbegin.size(), "segment pixels", st, staging_);
     circuit_blocks_ = cblocks[nseg];
     // whole-cell receptor maps of every cell, for each segment's charge scale
     std::vector<int> cell_begin(nseg + 1);
@@ -443,8 +445,8 @@ class Engine {
       charge[i] = work[i].seg->params.charge_scale;
     }
     cell_begin[nseg] = cell_total;
-    cellbeg_.upload(cell_begin.data(), cell_begin.size(), "segment cells", st);
-    charge_.upload(charge.data(), charge.size(), "segment charge", st);
+    cellbeg_.upload(cell_begin.data(), cell_begin.size(), "segment cells", st, staging_);
+    charge_.upload(charge.data(), charge.size(), "segment charge", st, staging_);
     maps_.reserve(cell_total, "cell maps");
     hip_check(launch_cell_maps(cell_total, cell_t0_.ptr, cell_t1_.ptr, cellbeg_.ptr, nseg, charge_.ptr, maps_.ptr, st),
               "cell maps");
@@ -518,8 +520,8 @@ class Engine {
         shape.push_back(w.seg->npix);
       }
     }
-    segtab_.upload(tables.data(), tables.size(), "segment table", st);
-    tblock_.upload(tblocks.data(), tblocks.size(), "segment blocks", st);
+    segtab_.upload(tables.data(), tables.size(), "segment table", st, staging_);
+    tblock_.upload(tblocks.data(), tblocks.size(), "segment blocks", st, staging_);
     const size_t rare_native = cell_rates_flags(pixel_total, chunk);
     rare_native_.reserve(rare_native, "photocurrent flags");
     // A captured graph replays only while every buffer it reads stays put.
@@ -761,8 +763,8 @@ class Engine {
       a.accepted = 1;
       a.cell_offset = w.cell_offset;
     }
-    segtab_.upload(tables.data(), tables.size(), "segment table", st);
-    adapt_.upload(ad.data(), ad.size(), "adaptive windows", st);
+    segtab_.upload(tables.data(), tables.size(), "segment table", st, staging_);
+    adapt_.upload(ad.data(), ad.size(), "adaptive windows", st, staging_);
     hip_check(launch_fill_int(counts_.ptr, 0, P, st), "clear events");
     hip_check(launch_cell_poses(cell_total, cell_t0_.ptr, cell_t1_.ptr, cell_iv_.ptr, intervals_.ptr, poses_.ptr, st),
               "cell poses");
@@ -813,31 +815,33 @@ class Engine {
     b.cell_frac = cell_frac_.ptr;
     ent_[0].reserve((size_t)2 * P, "refinement segments");
     hip_check(launch_adapt_init(P, b, ent_[0].ptr, st), "refine pixel optical launch");
-    ctr_.reserve(1, "refinement counters");
-    hip_check(hipMemsetAsync(ctr_.ptr, 0, sizeof(AdaptCounters), st), "clear refinement counters");
+    ctr_.reserve(kRefineCounterWords, "refinement counters");
+    hip_check(launch_fill_int(ctr_.ptr, 0, kRefineCounterWords, st), "clear refinement counters");
     // Level-synchronous refinement.  A traced bisection also traces the
     // midpoints of its halves' halves, so the next level's bisections need
     // no trace: the levels alternate between one ray launch and none, and
     // every probe value is the one a bisection would trace itself.
     long long n = 2LL * P;
     int cur = 0, terms_total = 0, levels = 0;
-    AdaptCounters host_ctr{};
+    int host_ctr[kRefineCounterWords];
     while (n > 0) {
-      if (++levels > 64) fail(DVS_E_STATE, "optical refinement did not terminate");
+      if (++levels > kMaxRefineLevels) fail(DVS_E_STATE, "optical refinement did not terminate");
+      int *level_ctr = ctr_.ptr + 2 * levels;  // words 2 + 2 (levels - 1) and the next
       terms_.grow_keep((size_t)terms_total + n, terms_total, "resolved segments", st);
       tkeys_.grow_keep((size_t)terms_total + n, terms_total, "resolved segment keys", st);
       bis_.reserve(n, "bisected segments");
       qpix_.reserve((size_t)dvsdev::kBisectQueries * n, "optical queries");
       qpose_.reserve((size_t)dvsdev::kBisectQueries * n, "optical queries");
       ent_[1 - cur].reserve(2 * n, "refinement segments");
-      hip_check(hipMemsetAsync(&ctr_.ptr->bisect, 0, 2 * sizeof(int), st), "clear refinement counters");
       hip_check(launch_adapt_decide((int)n, ent_[cur].ptr, segtab_.ptr, adapt_.ptr, pbegin_.ptr, nseg, b, terms_.ptr,
-                                    tkeys_.ptr, bis_.ptr, qpix_.ptr, qpose_.ptr, ent_[1 - cur].ptr, ctr_.ptr, st),
+                                    tkeys_.ptr, bis_.ptr, qpix_.ptr, qpose_.ptr, ent_[1 - cur].ptr, ctr_.ptr,
+                                    level_ctr, st),
                 "refine pixel optical launch");
-      hip_check(hipMemcpyAsync(&host_ctr, ctr_.ptr, sizeof(host_ctr), hipMemcpyDeviceToHost, st), "refinement count");
+      const size_t words = 2 + 2 * (size_t)levels;
+      hip_check(hipMemcpyAsync(host_ctr, ctr_.ptr, sizeof(int) * words, hipMemcpyDeviceToHost, st), "refinement count");
       stream_wait(st, "refinement count");
-      terms_total = host_ctr.terms;
-      const long long nb = host_ctr.bisect, nd = host_ctr.direct;
+      terms_total = host_ctr[0];
+      const long long nb = host_ctr[words - 2], nd = host_ctr[words - 1];
       if (nb + nd == 0) break;
       if (nb > 0) {
         const long long queries = (long long)dvsdev::kBisectQueries * nb;
@@ -953,7 +957,7 @@ class Engine {
         for (int i = 0; i < np; ++i) ids[(size_t)g * 1024 + i] = (int)(((uint64_t)i * 2654435761ULL) % (uint64_t)w.seg->npix);
         rad_total += (size_t)np * w.cells;
       }
-      gate_ids_.upload(ids.data(), ids.size(), "mean receptor probe IDs", st);
+      gate_ids_.upload(ids.data(), ids.size(), "mean receptor probe IDs", st, staging_);
       gate_rad_.reserve(std::max<size_t>(rad_total, 1), "mean receptor probe radiance");
       size_t off = 0;
       gblocks[0] = 0;
@@ -967,9 +971,9 @@ class Engine {
         gblocks[g + 1] = gblocks[g] + ((long long)np * w.cells + slots - 1) / slots;
         gate_samples += (uint64_t)np * w.cells;
       }
-      gate_tab_.upload(gtab.data(), gtab.size(), "mean receptor probes", st);
-      gate_blocks_.upload(gblocks.data(), gblocks.size(), "mean receptor probes", st);
-      gate_idx_.upload(gate_seg.data(), gate_seg.size(), "mean receptor probes", st);
+      gate_tab_.upload(gtab.data(), gtab.size(), "mean receptor probes", st, staging_);
+      gate_blocks_.upload(gblocks.data(), gblocks.size(), "mean receptor probes", st, staging_);
+      gate_idx_.upload(gate_seg.data(), gate_seg.size(), "mean receptor probes", st, staging_);
       hip_check(launch_trace_cells_multi(gate_tab_.ptr, gate_blocks_.ptr, ngate, gblocks[ngate], lds, status_.ptr,
                                          counter_.ptr, st),
                 "mean receptor probe optics");
@@ -1002,7 +1006,7 @@ class Engine {
           changed = true;
         }
       }
-      if (changed) adapt_.upload(ad.data(), ad.size(), "adaptive windows", st);
+      if (changed) adapt_.upload(ad.data(), ad.size(), "adaptive windows", st, staging_);
     }
     hip_check(hipEventRecord(refine_t.stop, st), "record");
     // Integration, with the fallback pixels' native radiance traced in cell
@@ -1035,7 +1039,7 @@ class Engine {
       radiance_.reserve((size_t)chunk * fb_total, "fallback optical tile");
       std::vector<int> fb_items(nseg + 1, 0);
       for (int i = 0; i < nseg; ++i) fb_items[i + 1] = fb_ends[i];
-      fb_items_.upload(fb_items.data(), fb_items.size(), "fallback optical tile", st);
+      fb_items_.upload(fb_items.data(), fb_items.size(), "fallback optical tile", st, staging_);
       size_t off = 0;
       for (int i = 0; i < nseg; ++i) {
         const int begin = i ? fb_ends[i - 1] : 0, count = fb_ends[i] - begin;
@@ -1045,7 +1049,7 @@ class Engine {
         off += (size_t)count * chunk;
         if (accepted[i]) fallback_samples += (uint64_t)count * work[i].cells;
       }
-      adapt_.upload(ad.data(), ad.size(), "adaptive windows", st);
+      adapt_.upload(ad.data(), ad.size(), "adaptive windows", st, staging_);
     }
     if (pipelined) {
       hip_check(hipEventRecord(ev_start_, st), "record");
@@ -1082,8 +1086,8 @@ class Engine {
           ftab[i].pixel_list = fb_list_.ptr + ad[i].fb_begin;
           fblocks[i + 1] = fblocks[i] + ((long long)ftab[i].npix * (a1 - a0) + slots - 1) / slots;
         }
-        fb_tab_.upload(ftab.data(), ftab.size(), "fallback optical tile", st);
-        fb_blocks_.upload(fblocks.data(), fblocks.size(), "fallback optical tile", st);
+        fb_tab_.upload(ftab.data(), ftab.size(), "fallback optical tile", st, staging_);
+        fb_blocks_.upload(fblocks.data(), fblocks.size(), "fallback optical tile", st, staging_);
         if (fblocks[nseg] > 0) {
           hip_check(launch_trace_cells_multi(fb_tab_.ptr, fb_blocks_.ptr, nseg, fblocks[nseg], lds, status_.ptr,
                                              counter_.ptr, st),
@@ -1173,7 +1177,8 @@ class Engine {
   DeviceArray<int> rare_native_, rare_adapt_;  // per-wave flags of the photocurrent's second phase
   DeviceArray<Pose> probe_poses_, qpose_;
   DeviceArray<AdaptEntry> ent_[2], bis_;
-  DeviceArray<AdaptCounters> ctr_;
+  DeviceArray<int> ctr_;  // refinement counters (kRefineCounterWords)
+  Staging staging_;       // page-locked source of the step's uploads
   DeviceArray<AdaptTerm> terms_;
   DeviceArray<unsigned long long> tkeys_, tkeys2_;
   DeviceArray<unsigned> torder_, torder2_;
