@@ -126,13 +126,11 @@ hipError_t launch_adapt_children(int n, const AdaptEntry *bisected, const double
                                  hipStream_t st);
 hipError_t launch_adapt_counts(int pixels, const AdaptDev *ad, const int *pixel_begin, int nseg, AdaptBufs b,
                                hipStream_t st);
-hipError_t launch_iota(unsigned *p, int n, hipStream_t st);
-hipError_t launch_adapt_term_runs(int n, const unsigned long long *sorted_keys, const unsigned *order,
-                                  const AdaptTerm *terms, AdaptBufs b, int *contrib, int *run_begin,
-                                  hipStream_t st);
-hipError_t launch_adapt_write_knots(int n, const unsigned long long *sorted_keys, const unsigned *order,
-                                    const AdaptTerm *terms, AdaptBufs b, const int *contrib, const int *scan,
-                                    const int *run_begin, hipStream_t st);
+// Knots of every knot pixel in time order from the resolved segments
+// (terms, keys as written by adapt_decide); fill holds P zeroed counters and
+// bucket at least as many words as knots.
+hipError_t launch_adapt_knots(int n, const unsigned long long *keys, const AdaptTerm *terms, AdaptBufs b, int *fill,
+                              unsigned *bucket, hipStream_t st);
 hipError_t launch_adapt_exact_count(int pixels, const SegDev *segs, const AdaptDev *ad, const int *pixel_begin,
                                     int nseg, AdaptBufs b, hipStream_t st);
 hipError_t launch_adapt_exact_list(int pixels, const SegDev *segs, const AdaptDev *ad, const int *pixel_begin,

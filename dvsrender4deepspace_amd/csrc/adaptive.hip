@@ -253,39 +253,47 @@ __global__ void adapt_counts(int pixels, const AdaptDev *ad, const int *pixel_be
   b.kcount[p] = (f & 1) ? 0 : b.bound[p];
 }
 
-__global__ void iota_kernel(unsigned *p, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = (unsigned)i;
-}
-
-// Sorted resolved segments: knots each contributes and each pixel's first one.
-__global__ void adapt_term_runs(int n, const unsigned long long *keys, const unsigned *order, const AdaptTerm *terms,
-                                AdaptBufs b, int *contrib, int *run_begin) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const int p = (int)(keys[t] >> 24);
-  contrib[t] = (b.flags[p] & 1) ? 0 : (terms[order[t]].kind == ADAPT_ACCEPT ? 2 : 1);
-  if (t == 0 || (int)(keys[t - 1] >> 24) != p) run_begin[p] = t;
-}
-
-__global__ void adapt_write_knots(int n, const unsigned long long *keys, const unsigned *order,
-                                  const AdaptTerm *terms, AdaptBufs b, const int *contrib, const int *scan,
-                                  const int *run_begin) {
+// Knots in time order without a sort.  A knot pixel's resolved segments
+// partition its window, and segment t's knots start at
+//   base + 1 + (knots of the pixel's segments that start before t),
+// base being the pixel's first knot (the window start).  Each segment first
+// drops a word {start fraction, knots} into a bucket inside its pixel's knot
+// range (a pixel has fewer segments than knots), then counts the bucket.
+// Start fractions are dyadic with at most 21 bits below the point, so the
+// 24-bit key of the segment key word is exact.
+__global__ void adapt_term_bucket(int n, const unsigned long long *keys, const AdaptTerm *terms, AdaptBufs b,
+                                  int *fill, unsigned *bucket) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int p = (int)(keys[t] >> 24);
   if (b.flags[p] & 1) return;
   const int base = b.koff[p] - b.kcount[p];
-  const int rb = run_begin[p];
-  const int pos = base + 1 + (scan[t] - contrib[t]) - (scan[rb] - contrib[rb]);
-  const AdaptTerm T = terms[order[t]];
+  const unsigned knots = terms[t].kind == ADAPT_ACCEPT ? 2u : 1u;
+  bucket[base + atomicAdd(&fill[p], 1)] = ((unsigned)(keys[t] & 0xFFFFFFu) << 2) | knots;
+}
+
+__global__ void adapt_place_knots(int n, const unsigned long long *keys, const AdaptTerm *terms, AdaptBufs b,
+                                  const int *fill, const unsigned *bucket) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int p = (int)(keys[t] >> 24);
+  if (b.flags[p] & 1) return;
+  const int base = b.koff[p] - b.kcount[p], m = fill[p];
+  const unsigned mine = (unsigned)(keys[t] & 0xFFFFFFu);
+  int before = 0;
+  for (int i = 0; i < m; ++i) {
+    const unsigned w = bucket[base + i];
+    before += (w >> 2) < mine ? (int)(w & 3u) : 0;
+  }
+  const int pos = base + 1 + before;
+  const AdaptTerm T = terms[t];
   if (T.kind == ADAPT_ACCEPT) {
     b.knots[pos] = make_double2((T.a + T.b) * .5, T.mid);
     b.knots[pos + 1] = make_double2(T.b, T.right);
   } else {
     b.knots[pos] = make_double2(-T.b, T.right);
   }
-  if (t == rb) b.knots[base] = make_double2(0.0, b.probes[(size_t)p * 5]);
+  if (mine == 0) b.knots[base] = make_double2(0.0, b.probes[(size_t)p * 5]);
 }
 
 // Knot bracket of a cell midpoint fraction (integrate_optical_knots, :99).
@@ -648,23 +656,11 @@ hipError_t launch_adapt_counts(int pixels, const AdaptDev *ad, const int *pixel_
   adapt_counts<<<grid(pixels), 256, 0, st>>>(pixels, ad, pixel_begin, nseg, b);
   return hipGetLastError();
 }
-hipError_t launch_iota(unsigned *p, int n, hipStream_t st) {
+hipError_t launch_adapt_knots(int n, const unsigned long long *keys, const AdaptTerm *terms, AdaptBufs b, int *fill,
+                              unsigned *bucket, hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  iota_kernel<<<grid(n), 256, 0, st>>>(p, n);
-  return hipGetLastError();
-}
-hipError_t launch_adapt_term_runs(int n, const unsigned long long *keys, const unsigned *order,
-                                  const AdaptTerm *terms, AdaptBufs b, int *contrib, int *run_begin,
-                                  hipStream_t st) {
-  if (n <= 0) return hipSuccess;
-  adapt_term_runs<<<grid(n), 256, 0, st>>>(n, keys, order, terms, b, contrib, run_begin);
-  return hipGetLastError();
-}
-hipError_t launch_adapt_write_knots(int n, const unsigned long long *keys, const unsigned *order,
-                                    const AdaptTerm *terms, AdaptBufs b, const int *contrib, const int *scan,
-                                    const int *run_begin, hipStream_t st) {
-  if (n <= 0) return hipSuccess;
-  adapt_write_knots<<<grid(n), 256, 0, st>>>(n, keys, order, terms, b, contrib, scan, run_begin);
+  adapt_term_bucket<<<grid(n), 256, 0, st>>>(n, keys, terms, b, fill, bucket);
+  adapt_place_knots<<<grid(n), 256, 0, st>>>(n, keys, terms, b, fill, bucket);
   return hipGetLastError();
 }
 hipError_t launch_adapt_exact_count(int pixels, const SegDev *segs, const AdaptDev *ad, const int *pixel_begin,

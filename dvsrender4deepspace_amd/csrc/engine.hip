@@ -799,7 +799,7 @@ class Engine {
     probe_layout_ = std::move(layout);
     probe_cur_ = pcur;
     for (auto *a : {&bound_, &flags_, &bisects_, &exact_terms_, &kcount_, &koff_, &ecount_, &eoff_, &fb_index_,
-                    &fb_scan_, &run_begin_})
+                    &fb_scan_})
       a->reserve(P, "optical refinement state");
     AdaptBufs b{};
     b.probes = probes_[pcur].ptr;
@@ -867,31 +867,10 @@ class Engine {
     knots_.reserve(std::max(knots_total, 1), "pixel optical knots");
     b.knots = knots_.ptr;
     if (terms_total > 0) {
-      torder_.reserve(terms_total, "knot order");
-      torder2_.reserve(terms_total, "knot order");
-      tkeys2_.reserve(terms_total, "knot order");
-      contrib_.reserve(terms_total, "knot order");
-      cscan_.reserve(terms_total, "knot order");
-      hip_check(launch_iota(torder_.ptr, terms_total, st), "knot order");
-      int bits = 1;
-      while ((1LL << bits) < (long long)P) ++bits;
-      temp_bytes = 0;
-      hip_check(sort_pairs_u64(nullptr, temp_bytes, tkeys_.ptr, tkeys2_.ptr, torder_.ptr, torder2_.ptr, terms_total,
-                               0, 24 + bits, st),
-                "sort size");
-      temp_.reserve(temp_bytes, "sort scratch");
-      hip_check(sort_pairs_u64(temp_.ptr, temp_bytes, tkeys_.ptr, tkeys2_.ptr, torder_.ptr, torder2_.ptr, terms_total,
-                               0, 24 + bits, st),
-                "knot order");
-      hip_check(launch_adapt_term_runs(terms_total, tkeys2_.ptr, torder2_.ptr, terms_.ptr, b, contrib_.ptr,
-                                       run_begin_.ptr, st),
-                "knot order");
-      temp_bytes = 0;
-      hip_check(inclusive_scan_int(nullptr, temp_bytes, contrib_.ptr, cscan_.ptr, terms_total, st), "scan size");
-      temp_.reserve(temp_bytes, "scan scratch");
-      hip_check(inclusive_scan_int(temp_.ptr, temp_bytes, contrib_.ptr, cscan_.ptr, terms_total, st), "knot order");
-      hip_check(launch_adapt_write_knots(terms_total, tkeys2_.ptr, torder2_.ptr, terms_.ptr, b, contrib_.ptr,
-                                         cscan_.ptr, run_begin_.ptr, st),
+      kfill_.reserve(P, "knot order");
+      kbucket_.reserve(std::max(knots_total, 1), "knot order");
+      hip_check(launch_fill_int(kfill_.ptr, 0, P, st), "knot order");
+      hip_check(launch_adapt_knots(terms_total, tkeys_.ptr, terms_.ptr, b, kfill_.ptr, kbucket_.ptr, st),
                 "pixel optical knots");
     }
     // Local exact cells (identify_exact_cells / sample_exact_queries).
@@ -1164,8 +1143,8 @@ class Engine {
   // adaptive step state
   DeviceArray<AdaptDev> adapt_;
   DeviceArray<int> cellbeg_, bound_, flags_, bisects_, exact_terms_, kcount_, koff_, ecount_, eoff_, fb_index_,
-      fb_scan_, run_begin_, qpix_, contrib_, cscan_, ecell_, eitem_pix_, eitem_pose_, fb_ends_, fb_list_, gate_ids_,
-      gate_idx_, fb_items_;
+      fb_scan_, qpix_, kfill_, ecell_, eitem_pix_, eitem_pose_, fb_ends_, fb_list_, gate_ids_, gate_idx_, fb_items_;
+  DeviceArray<unsigned> kbucket_;  // per-pixel segment words of the knot order
   DeviceArray<double> cell_frac_, qrad_, erad_, gate_rad_, gate_err_, rates_;
   // probe radiance and end poses of the last two adaptive windows, and the
   // segment layout (id, pixels) of the last one
@@ -1180,8 +1159,7 @@ class Engine {
   DeviceArray<int> ctr_;  // refinement counters (kRefineCounterWords)
   Staging staging_;       // page-locked source of the step's uploads
   DeviceArray<AdaptTerm> terms_;
-  DeviceArray<unsigned long long> tkeys_, tkeys2_;
-  DeviceArray<unsigned> torder_, torder2_;
+  DeviceArray<unsigned long long> tkeys_;
   DeviceArray<double2> knots_;
   DeviceArray<AdaptSegStats> seg_stats_;
   DeviceArray<SegDev> gate_tab_, fb_tab_;
