@@ -129,6 +129,12 @@ template <class T> struct DeviceArray {
     hip_check(hipMalloc(&ptr, std::max<size_t>(n, 1) * sizeof(T)), what);
     count = n;
   }
+  // reserve() with half again as much room when it has to grow, for sizes
+  // that drift from call to call (a free and an allocation cost a
+  // device-wide synchronisation each).
+  void reserve_ahead(size_t n, const char *what) {
+    if (n > count) reserve(n + n / 2, what);
+  }
   void upload(const T *host, size_t n, const char *what, hipStream_t st = nullptr) {
     reserve(n, what);
     if (n) hip_check(hipMemcpyAsync(ptr, host, n * sizeof(T), hipMemcpyHostToDevice, st), what);
@@ -149,6 +155,27 @@ template <class T> struct DeviceArray {
     count = n;
   }
   size_t bytes() const { return count * sizeof(T); }
+};
+
+// Page-locked host buffer for read-backs, grown with room to spare.
+template <class T> struct PinnedArray {
+  T *ptr = nullptr;
+  size_t count = 0;
+  PinnedArray() = default;
+  PinnedArray(const PinnedArray &) = delete;
+  PinnedArray &operator=(const PinnedArray &) = delete;
+  ~PinnedArray() {
+    if (ptr) (void)hipHostFree(ptr);
+  }
+  void reserve(size_t n, const char *what) {
+    if (n <= count) return;
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    count = 0;
+    const size_t want = std::max<size_t>(n + n / 2, 1);
+    hip_check(hipHostMalloc(reinterpret_cast<void **>(&ptr), want * sizeof(T), hipHostMallocDefault), what);
+    count = want;
+  }
 };
 
 // Pair of HIP events bracketing work on one stream.

@@ -621,27 +621,33 @@ class Engine {
     hip_check(inclusive_scan_int(temp_.ptr, temp_bytes, counts_.ptr, ends_.ptr, pixel_total, st), "event scan");
     int status = 0, total = 0;
     unsigned long long shadows = 0;
-    hip_check(hipMemcpyAsync(&status, status_.ptr, sizeof(int), hipMemcpyDeviceToHost, st), "copy status");
-    hip_check(hipMemcpyAsync(&total, ends_.ptr + pixel_total - 1, sizeof(int), hipMemcpyDeviceToHost, st), "copy count");
+    host_words_.reserve(4, "status");
+    hip_check(hipMemcpyAsync(host_words_.ptr, status_.ptr, sizeof(int), hipMemcpyDeviceToHost, st), "copy status");
+    hip_check(hipMemcpyAsync(host_words_.ptr + 1, ends_.ptr + pixel_total - 1, sizeof(int), hipMemcpyDeviceToHost, st),
+              "copy count");
     stream_wait(st, "copy count");
+    status = host_words_.ptr[0];
+    total = host_words_.ptr[1];
     if (status & 1) fail(DVS_E_CAPACITY, "pixel event capacity exceeded; reduce numerical window");
     if (status) fail(DVS_E_OPTICAL, "invalid optical sample or BVH traversal overflow");
-    std::vector<dvs_event> sorted(total);
-    std::vector<int> sorted_seg(total);
+    // read-backs land in page-locked memory: events, their segments (with
+    // more than one segment), the status word and the shadow ray count
+    host_events_.reserve(std::max(total, 1), "events");
+    if (nseg > 1) host_event_seg_.reserve(std::max(total, 1), "event segments");
     const int samples = work[0].seg->scene->view.samples;
     if (total > 0) {
-      ev_time_.reserve(total, "event times");
-      ev_seg_.reserve(total, "event segments");
-      ev_pixel_.reserve(total, "event pixels");
-      ev_pol_.reserve(total, "event polarity");
-      ev_depth_.reserve(total, "event depth");
-      ev_poses_.reserve(total, "event poses");
-      keys_.reserve(total, "event keys");
-      keys2_.reserve(total, "event keys");
-      order_.reserve(total, "event order");
-      order2_.reserve(total, "event order");
-      events_out_.reserve(total, "events");
-      events_seg_.reserve(total, "event segments");
+      ev_time_.reserve_ahead(total, "event times");
+      ev_seg_.reserve_ahead(total, "event segments");
+      ev_pixel_.reserve_ahead(total, "event pixels");
+      ev_pol_.reserve_ahead(total, "event polarity");
+      ev_depth_.reserve_ahead(total, "event depth");
+      ev_poses_.reserve_ahead(total, "event poses");
+      keys_.reserve_ahead(total, "event keys");
+      keys2_.reserve_ahead(total, "event keys");
+      order_.reserve_ahead(total, "event order");
+      order2_.reserve_ahead(total, "event order");
+      events_out_.reserve_ahead(total, "events");
+      events_seg_.reserve_ahead(total, "event segments");
       const SegDev *tab = segtab_.ptr;  // chunk 0 table: ivs, scene, counts, slots are chunk independent
       hip_check(launch_gather_multi(tab, pbegin_.ptr, nseg, pixel_total, ends_.ptr, ev_time_.ptr, ev_seg_.ptr,
                                     ev_pixel_.ptr, ev_pol_.ptr, keys_.ptr, order_.ptr, st),
@@ -684,16 +690,24 @@ class Engine {
       hip_check(launch_permute_multi(tab, total, final_order, ev_time_.ptr, ev_seg_.ptr, ev_pixel_.ptr, ev_pol_.ptr,
                                      ev_depth_.ptr, events_out_.ptr, events_seg_.ptr, st),
                 "event records");
-      hip_check(hipMemcpyAsync(sorted.data(), events_out_.ptr, sizeof(dvs_event) * total, hipMemcpyDeviceToHost, st),
+      hip_check(hipMemcpyAsync(host_events_.ptr, events_out_.ptr, sizeof(dvs_event) * total, hipMemcpyDeviceToHost,
+                               st),
                 "copy events");
-      hip_check(hipMemcpyAsync(sorted_seg.data(), events_seg_.ptr, sizeof(int) * total, hipMemcpyDeviceToHost, st),
-                "copy events");
+      if (nseg > 1)
+        hip_check(hipMemcpyAsync(host_event_seg_.ptr, events_seg_.ptr, sizeof(int) * total, hipMemcpyDeviceToHost,
+                                 st),
+                  "copy events");
     }
     hip_check(hipEventRecord(events_t.stop, st), "record");
-    hip_check(hipMemcpyAsync(&status, status_.ptr, sizeof(int), hipMemcpyDeviceToHost, st), "copy status");
-    hip_check(hipMemcpyAsync(&shadows, counter_.ptr, sizeof(shadows), hipMemcpyDeviceToHost, st), "ray counter");
+    hip_check(hipMemcpyAsync(host_words_.ptr, status_.ptr, sizeof(int), hipMemcpyDeviceToHost, st), "copy status");
+    hip_check(hipMemcpyAsync(host_words_.ptr + 2, counter_.ptr, sizeof(shadows), hipMemcpyDeviceToHost, st),
+              "ray counter");
     stream_wait(st, "copy events");
+    status = host_words_.ptr[0];
+    std::memcpy(&shadows, host_words_.ptr + 2, sizeof(shadows));
     if (status) fail(DVS_E_OPTICAL, "invalid optical sample or BVH traversal overflow");
+    const dvs_event *sorted = host_events_.ptr;
+    const int *sorted_seg = host_event_seg_.ptr;
     const double events_ms = events_t.ms();
     stats.events_ms += events_ms;
     stats.shadow_rays += shadows;
@@ -704,9 +718,10 @@ class Engine {
     for (int i = 0; i < nseg; ++i) {
       Segment &s = *work[i].seg;
       size_t end = at;
+      if (nseg == 1) end = (size_t)total;
       while (end < (size_t)total && sorted_seg[end] == i) ++end;
       if (!accepted[i]) continue;  // a rejected window has no events and changes nothing
-      s.raw.assign(sorted.begin() + at, sorted.begin() + end);
+      s.raw.assign(sorted + at, sorted + end);
       s.events_total += (int64_t)(end - at);
       const size_t before = s.ready.size();
       s.readout.submit_sorted(s.raw.data(), s.raw.size(), work[i].ivs.back().t1, s.ready);
@@ -1160,6 +1175,8 @@ class Engine {
   Staging staging_;       // page-locked source of the step's uploads
   DeviceArray<AdaptTerm> terms_;
   DeviceArray<unsigned long long> tkeys_;
+  PinnedArray<dvs_event> host_events_;
+  PinnedArray<int> host_event_seg_, host_words_;
   DeviceArray<double2> knots_;
   DeviceArray<AdaptSegStats> seg_stats_;
   DeviceArray<SegDev> gate_tab_, fb_tab_;
