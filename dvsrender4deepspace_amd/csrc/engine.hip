@@ -838,7 +838,8 @@ class Engine {
     // every probe value is the one a bisection would trace itself.
     long long n = 2LL * P;
     int cur = 0, terms_total = 0, levels = 0;
-    int host_ctr[kRefineCounterWords];
+    host_small_.reserve(kRefineCounterWords + 2, "refinement counters");
+    int *host_ctr = host_small_.ptr;  // page-locked: the copy is queued, then the stream is polled
     while (n > 0) {
       if (++levels > kMaxRefineLevels) fail(DVS_E_STATE, "optical refinement did not terminate");
       int *level_ctr = ctr_.ptr + 2 * levels;  // words 2 + 2 (levels - 1) and the next
@@ -876,9 +877,11 @@ class Engine {
     hip_check(inclusive_scan_int(nullptr, temp_bytes, kcount_.ptr, koff_.ptr, P, st), "scan size");
     temp_.reserve(temp_bytes, "scan scratch");
     hip_check(inclusive_scan_int(temp_.ptr, temp_bytes, kcount_.ptr, koff_.ptr, P, st), "knot offsets");
-    int knots_total = 0;
-    hip_check(hipMemcpyAsync(&knots_total, koff_.ptr + P - 1, sizeof(int), hipMemcpyDeviceToHost, st), "knot count");
+    hip_check(hipMemcpyAsync(host_ctr + kRefineCounterWords, koff_.ptr + P - 1, sizeof(int), hipMemcpyDeviceToHost,
+                             st),
+              "knot count");
     stream_wait(st, "knot count");
+    const int knots_total = host_ctr[kRefineCounterWords];
     knots_.reserve(std::max(knots_total, 1), "pixel optical knots");
     b.knots = knots_.ptr;
     if (terms_total > 0) {
@@ -894,9 +897,11 @@ class Engine {
     hip_check(inclusive_scan_int(nullptr, temp_bytes, ecount_.ptr, eoff_.ptr, P, st), "scan size");
     temp_.reserve(temp_bytes, "scan scratch");
     hip_check(inclusive_scan_int(temp_.ptr, temp_bytes, ecount_.ptr, eoff_.ptr, P, st), "local exact query count");
-    int exact_total = 0;
-    hip_check(hipMemcpyAsync(&exact_total, eoff_.ptr + P - 1, sizeof(int), hipMemcpyDeviceToHost, st), "local exact query count");
+    hip_check(hipMemcpyAsync(host_ctr + kRefineCounterWords + 1, eoff_.ptr + P - 1, sizeof(int),
+                             hipMemcpyDeviceToHost, st),
+              "local exact query count");
     stream_wait(st, "local exact query count");
+    const int exact_total = host_ctr[kRefineCounterWords + 1];
     ecell_.reserve(std::max(exact_total, 1), "local exact optical cells");
     erad_.reserve(std::max(exact_total, 1), "local exact radiance");
     eitem_pix_.reserve(std::max(exact_total, 1), "compact exact ray queue");
@@ -920,9 +925,11 @@ class Engine {
     hip_check(inclusive_scan_int(temp_.ptr, temp_bytes, fb_scan_.ptr, fb_scan_.ptr, P, st), "compact exact fallback pixels");
     fb_ends_.reserve(nseg, "fallback counts");
     hip_check(launch_adapt_segment_ends(nseg, pbegin_.ptr, fb_scan_.ptr, fb_ends_.ptr, st), "fallback counts");
-    std::vector<int> fb_ends(nseg);
-    hip_check(hipMemcpyAsync(fb_ends.data(), fb_ends_.ptr, sizeof(int) * nseg, hipMemcpyDeviceToHost, st), "fallback counts");
+    host_fb_ends_.reserve(nseg, "fallback counts");
+    hip_check(hipMemcpyAsync(host_fb_ends_.ptr, fb_ends_.ptr, sizeof(int) * nseg, hipMemcpyDeviceToHost, st),
+              "fallback counts");
     stream_wait(st, "fallback counts");
+    const std::vector<int> fb_ends(host_fb_ends_.ptr, host_fb_ends_.ptr + nseg);
     const int fb_total = nseg ? fb_ends[nseg - 1] : 0;
     fb_list_.reserve(std::max(fb_total, 1), "exact fallback pixel IDs");
     hip_check(launch_adapt_fallback_list(P, adapt_.ptr, pbegin_.ptr, nseg, fb_scan_.ptr, b, fb_list_.ptr, st),
@@ -930,8 +937,9 @@ class Engine {
     // Refinement statistics of every segment.
     seg_stats_.reserve(nseg, "refinement statistics");
     hip_check(launch_adapt_segment_stats(nseg, pbegin_.ptr, b, seg_stats_.ptr, st), "refinement statistics");
-    std::vector<AdaptSegStats> seg_stats(nseg);
-    hip_check(hipMemcpyAsync(seg_stats.data(), seg_stats_.ptr, sizeof(AdaptSegStats) * nseg, hipMemcpyDeviceToHost, st),
+    host_seg_stats_.reserve(nseg, "refinement statistics");
+    const AdaptSegStats *seg_stats = host_seg_stats_.ptr;  // read after the step's final wait
+    hip_check(hipMemcpyAsync(host_seg_stats_.ptr, seg_stats_.ptr, sizeof(AdaptSegStats) * nseg, hipMemcpyDeviceToHost, st),
               "refinement statistics");
     // Convergence gate (verify_receptor, event_sensor.cu:357-374) of the
     // segments that check this window.
@@ -1176,7 +1184,8 @@ class Engine {
   DeviceArray<AdaptTerm> terms_;
   DeviceArray<unsigned long long> tkeys_;
   PinnedArray<dvs_event> host_events_;
-  PinnedArray<int> host_event_seg_, host_words_;
+  PinnedArray<int> host_event_seg_, host_words_, host_small_, host_fb_ends_;
+  PinnedArray<AdaptSegStats> host_seg_stats_;
   DeviceArray<double2> knots_;
   DeviceArray<AdaptSegStats> seg_stats_;
   DeviceArray<SegDev> gate_tab_, fb_tab_;
