@@ -10,10 +10,12 @@ Workload: the single DAVIS346 request.
   windows of 200 horizons, the receptor check at the first window and every
   0.25 s.
 
-One step is one driver call (EventSensor::process_adaptive over a window).
-It covers the refinement probes and bisection, local exact cells, the
-photocurrent with shot noise, the FP64 circuit of 2,000 cells x 89,960
-pixels, device compaction/sort and the readout FIFO.  Warm-up windows run on
+One step is one driver call as bin/dvs makes it: the window's trajectory
+states (CameraTrajectory over the flight's 100 fps frames, evaluated on the
+device) and EventSensor::process_adaptive over the window.  It covers the
+refinement probes and bisection, local exact cells, the photocurrent with
+shot noise, the FP64 circuit of 2,000 cells x 89,960 pixels, device
+compaction/sort and the readout FIFO.  Warm-up windows run on
 a separate sensor; the timed sensor starts at t=0.  Inputs (BVH, mesh, pixel
 state) are resident in HBM before timing starts.
 
@@ -59,10 +61,10 @@ def bytes_per_ray(stats: dict) -> tuple[float, float]:
     return per("primary"), per("shadow")
 
 
-def cpu_baseline(w, mesh, horizons: int) -> dict:
+def cpu_baseline(w, mesh, horizons: int, ivs) -> dict:
     """The CPU oracle (restatement of the reference path) on the host cores:
-    the request's first adaptive window, checked by the receptor gate as the
-    driver checks it."""
+    the request's first adaptive window (intervals `ivs`), checked by the
+    receptor gate as the driver checks it."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle_ctypes as oracle
 
@@ -70,8 +72,7 @@ def cpu_baseline(w, mesh, horizons: int) -> dict:
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     scene = oracle.Scene(nodes, root, v, n, idx, w.camera, w.hapke)
     sensor = oracle.Sensor(w.camera.width, w.camera.height, w.physic)
-    sensor.initialize(scene, w.initial_state(), w.start_time, threads)
-    ivs = w.intervals(0, horizons)
+    sensor.initialize(scene, ivs["start"][:1], w.start_time, threads)
     t0 = time.perf_counter()
     code, ok, raw, _, st, _ = sensor.process_adaptive(scene, ivs, tolerance=1e-2, check=True, capacity=512,
                                                       threads=threads)
@@ -264,22 +265,28 @@ def request_mode(args) -> dict:
     log(f"scene: {len(idx)} triangles, {len(nodes)} BVH nodes, built in {time.perf_counter() - t:.1f} s")
     scene = capi.Scene(nodes, root, v, n, leaf_idx, w.camera, w.hapke)
     info = scene.info()
-    trav = scene.traversal_stats(w.initial_state())
+    trav = scene.traversal_stats(workloads.horizon_intervals(w, workloads.trajectory(w), w.start_time, 0, 1)
+                                 ["start"][:1])
     b_primary, b_shadow = bytes_per_ray(trav)
     log(f"scene on device: {info}; bytes/ray primary {b_primary:.0f}, shadow {b_shadow:.0f}")
     npix = w.camera.width * w.camera.height
     R = w.camera.samples ** 2
 
-    # --- adaptive optics: one step = one driver call (window of `batch` horizons)
+    # --- adaptive optics: one step = one driver call (window of `batch`
+    # horizons) flown along the flight's CameraTrajectory (frames at the
+    # workload's fps), the window's states evaluated on the device
     batch = driver.batch_size(w.physic, w.time_resolution, adaptive=True)
+    traj = workloads.trajectory(w)
+    first = workloads.horizon_intervals(w, traj, w.start_time, 0, 1)
+    initial = first["start"][:1]
 
     def windows(sensor, count, timed):
         step, next_check, hz, lat, ready_n, rejected = 0, w.start_time, batch, [], 0, 0
-        done = 0
+        done, t_prev = 0, w.start_time
         while done < count:
-            ivs = w.intervals(step, hz)
-            check = float(ivs["t0"][0]) >= next_check
             t0 = time.perf_counter()
+            ivs = workloads.horizon_intervals(w, traj, t_prev, step, hz)
+            check = float(ivs["t0"][0]) >= next_check
             ok, ready = sensor.process_adaptive(ivs, driver.DEFAULT_TOLERANCE, check)
             if not ok:
                 rejected += 1
@@ -292,18 +299,19 @@ def request_mode(args) -> dict:
             lat.append(time.perf_counter() - t0)
             ready_n += len(ready)
             step += len(ivs)
+            t_prev = float(ivs["t1"][-1])
             done += 1
             if timed and done % 10 == 0:
                 log(f"window {done}: {lat[-1] * 1e3:.1f} ms, {len(ready)} events delivered")
         return step, lat, ready_n, rejected
 
     warm = capi.Sensor(scene, w.physic)
-    warm.initialize(w.initial_state(), w.start_time)
+    warm.initialize(initial, w.start_time)
     windows(warm, max(args.warmup, 1), False)
     warm.close()
     log(f"warmup done ({args.warmup} windows on a separate sensor)")
     sensor = capi.Sensor(scene, w.physic)
-    sensor.initialize(w.initial_state(), w.start_time)
+    sensor.initialize(initial, w.start_time)
     sensor.reset_stats()
     t_start = time.perf_counter()
     horizons, latencies, delivered, rejected = windows(sensor, args.steps, True)
@@ -365,13 +373,15 @@ def request_mode(args) -> dict:
 
     # --- native solver on the same request: the cell-trace roofline
     nat = capi.Sensor(scene, w.physic)
-    nat.initialize(w.initial_state(), w.start_time)
-    nat.process(w.intervals(0, args.horizons))
+    nat.initialize(initial, w.start_time)
+    ivs = workloads.horizon_intervals(w, traj, w.start_time, 0, args.horizons)
+    nat.process(ivs)
     nat.reset_stats()
     t0 = time.perf_counter()
     nsteps = 2
     for k in range(nsteps):
-        nat.process(w.intervals(args.horizons * (k + 1), args.horizons))
+        ivs = workloads.horizon_intervals(w, traj, float(ivs["t1"][-1]), args.horizons * (k + 1), args.horizons)
+        nat.process(ivs)
     nwall = time.perf_counter() - t0
     ns = nat.stats()
     trace_s = ns["trace_ms"] / 1e3
@@ -391,7 +401,8 @@ def request_mode(args) -> dict:
     nat.close()
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
-        result["cpu_baseline"] = cpu_baseline(w, (v, n, nodes, root, leaf_idx), batch)
+        result["cpu_baseline"] = cpu_baseline(w, (v, n, nodes, root, leaf_idx), batch,
+                                              workloads.horizon_intervals(w, traj, w.start_time, 0, batch))
     return result
 
 

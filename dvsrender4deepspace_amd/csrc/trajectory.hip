@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstring>
 #include <memory>
 #include <vector>
 
@@ -14,11 +15,14 @@
 
 namespace dvstraj {
 
-// Optical states at `n` times (to_optical_state of evaluate((float)t)).
-__global__ void optical_states_kernel(View v, const double *times, int n, dvs_optical_state *out) {
+// The n - 1 intervals between consecutive times: each time's optical state
+// (to_optical_state of evaluate((float)t)) is evaluated once and written as
+// the end of the interval before it and the start of the one after it.
+__global__ void intervals_kernel(View v, const double *times, int n, dvs_optical_interval *out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const Sample s = evaluate(v, (float)times[i], false);
+  const double t = times[i];
+  const Sample s = evaluate(v, (float)t, false);
   dvs_optical_state o;
   for (int c = 0; c < 3; ++c) {
     o.position[c] = s.position[c];
@@ -32,7 +36,14 @@ __global__ void optical_states_kernel(View v, const double *times, int n, dvs_op
   o.rotation[3] = s.rotation.w;
   o.intensity = s.intensity;
   o.intensity_rate = s.intensity_rate;
-  out[i] = o;
+  if (i + 1 < n) {
+    out[i].start = o;
+    out[i].t0 = t;
+  }
+  if (i > 0) {
+    out[i - 1].end = o;
+    out[i - 1].t1 = t;
+  }
 }
 
 }  // namespace dvstraj
@@ -154,7 +165,13 @@ struct Trajectory {
   DeviceArray<Key> device_keys;
   DeviceArray<float> device_quats;
   DeviceArray<double> device_times;
-  DeviceArray<dvs_optical_state> device_states;
+  DeviceArray<dvs_optical_interval> device_intervals;
+  PinnedArray<double> host_times;
+  PinnedArray<dvs_optical_interval> host_intervals;
+  hipStream_t stream = nullptr;
+  ~Trajectory() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
   dvstraj::View host_view() const { return dvstraj::View{keys.data(), quats.data(), count}; }
 };
 
@@ -320,24 +337,38 @@ int dvs_trajectory_intervals(dvs_trajectory trajectory, const double *times, int
     Trajectory &tr = get(trajectory);
     if (n < 2 || !times || !out) throw std::invalid_argument("invalid trajectory arguments");
     if (n > (int64_t)INT32_MAX) throw std::invalid_argument("too many trajectory times");
-    std::vector<dvs_optical_state> states((size_t)n);
     if (tr.count == 0) {
+      std::vector<dvs_optical_state> states((size_t)n);
       for (int64_t i = 0; i < n; ++i) states[i] = optical_of(default_state((float)times[i]));
-    } else {
-      if (!tr.device_keys.ptr) {
-        tr.device_keys.upload(tr.keys.data(), tr.keys.size(), "trajectory keys");
-        tr.device_quats.upload(tr.quats.data(), tr.quats.size(), "trajectory keys");
-      }
-      tr.device_times.upload(times, (size_t)n, "trajectory times");
-      tr.device_states.reserve((size_t)n, "trajectory states");
-      const dvstraj::View v{tr.device_keys.ptr, tr.device_quats.ptr, tr.count};
-      dvstraj::optical_states_kernel<<<(unsigned)((n + 255) / 256), 256>>>(v, tr.device_times.ptr, (int)n,
-                                                                            tr.device_states.ptr);
-      hip_check(hipGetLastError(), "trajectory launch");
-      hip_check(hipMemcpy(states.data(), tr.device_states.ptr, sizeof(dvs_optical_state) * n, hipMemcpyDeviceToHost),
-                "trajectory states");
+      for (int64_t i = 0; i + 1 < n; ++i)
+        out[i] = dvs_optical_interval{states[i], states[i + 1], times[i], times[i + 1]};
+      return;
     }
-    for (int64_t i = 0; i + 1 < n; ++i) out[i] = dvs_optical_interval{states[i], states[i + 1], times[i], times[i + 1]};
+    if (!tr.stream) hip_check(hipStreamCreateWithFlags(&tr.stream, hipStreamNonBlocking), "trajectory stream");
+    hipStream_t st = tr.stream;
+    if (!tr.device_keys.ptr) {
+      tr.device_keys.upload(tr.keys.data(), tr.keys.size(), "trajectory keys", st);
+      tr.device_quats.upload(tr.quats.data(), tr.quats.size(), "trajectory keys", st);
+    }
+    // times in and intervals out through page-locked buffers
+    stream_wait(st, "trajectory times");  // the previous call's copies are done
+    tr.host_times.reserve((size_t)n, "trajectory times");
+    tr.host_intervals.reserve((size_t)n - 1, "trajectory intervals");
+    std::memcpy(tr.host_times.ptr, times, sizeof(double) * (size_t)n);
+    tr.device_times.reserve((size_t)n, "trajectory times");
+    tr.device_intervals.reserve((size_t)n - 1, "trajectory intervals");
+    hip_check(hipMemcpyAsync(tr.device_times.ptr, tr.host_times.ptr, sizeof(double) * (size_t)n,
+                             hipMemcpyHostToDevice, st),
+              "trajectory times");
+    const dvstraj::View v{tr.device_keys.ptr, tr.device_quats.ptr, tr.count};
+    dvstraj::intervals_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(v, tr.device_times.ptr, (int)n,
+                                                                          tr.device_intervals.ptr);
+    hip_check(hipGetLastError(), "trajectory launch");
+    hip_check(hipMemcpyAsync(tr.host_intervals.ptr, tr.device_intervals.ptr,
+                             sizeof(dvs_optical_interval) * (size_t)(n - 1), hipMemcpyDeviceToHost, st),
+              "trajectory intervals");
+    stream_wait(st, "trajectory intervals");
+    std::memcpy(out, tr.host_intervals.ptr, sizeof(dvs_optical_interval) * (size_t)(n - 1));
   });
 }
 

@@ -18,7 +18,7 @@ import math
 
 import numpy as np
 
-from .capi import INTERVAL_DTYPE, STATE_DTYPE, Camera, Hapke, Physic
+from .capi import FRAME_DTYPE, INTERVAL_DTYPE, STATE_DTYPE, Camera, Hapke, Physic
 from .terrain import TerrainSpec
 
 # Hapke parameters of the workloads: w=0.3, B0=1.0, h=0.06, theta-bar=20 deg,
@@ -181,6 +181,39 @@ def quaternion_matrices(q: np.ndarray) -> np.ndarray:
         np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], -1)], -2)
 
 
+def flight_frames(w: Workload, count: int | None = None) -> np.ndarray:
+    """Trajectory frames of the workload's flight at its fps (FRAME_DTYPE):
+    frame k at t = k / fps with the flight's position, orientation (the
+    rotation matrix of its quaternion), light and intensity."""
+    n = w.frames if count is None else count
+    t = (np.arange(n) / w.fps).astype(np.float32)
+    s = w.states(t)
+    f = np.zeros(n, FRAME_DTYPE)
+    f["frame_index"] = np.arange(n)
+    f["timestamp_seconds"] = t.astype(np.float64)
+    f["camera_origin"] = s["position"]
+    f["camera_world"] = np.transpose(quaternion_matrices(s["rotation"]), (0, 2, 1)).reshape(n, 9)  # column major
+    f["light_dir"] = s["light"]
+    f["light_intensity"] = s["intensity"]
+    return f
+
+
+def trajectory(w: Workload) -> "Trajectory":
+    """CameraTrajectory over the workload's frames (src/trajectory.cpp)."""
+    from .capi import Trajectory
+    return Trajectory(flight_frames(w), w.fps)
+
+
+def horizon_intervals(w: Workload, traj, start_time: float, first_step: int, count: int) -> np.ndarray:
+    """The OpticalIntervals of host horizons first_step+1 .. first_step+count
+    flown along `traj`, from `start_time` (the end of the previous horizon):
+    bin/dvs's window, each state evaluated once on the device."""
+    ends = w.horizon_times(first_step, count)
+    times = np.concatenate([[start_time], ends])
+    keep = np.concatenate([[True], np.diff(times) > 0])  # a horizon that does not advance is dropped
+    return traj.intervals(times[keep])
+
+
 def write_bundle(w: Workload, directory, frames: int | None = None) -> None:
     """A scene bundle in the reference's format (src/scene_data.cpp:93-236):
     mesh_vertices.bin / mesh_normals.bin (float32 N x 3), mesh_indices.bin
@@ -197,19 +230,18 @@ def write_bundle(w: Workload, directory, frames: int | None = None) -> None:
     np.ascontiguousarray(v, np.float32).tofile(d / "mesh_vertices.bin")
     np.ascontiguousarray(n, np.float32).tofile(d / "mesh_normals.bin")
     np.ascontiguousarray(idx, np.uint32).tofile(d / "mesh_indices.bin")
-    count = w.frames if frames is None else frames
-    t = (np.arange(count) / w.fps).astype(np.float32)
-    s = w.states(t)
+    f = flight_frames(w, frames)
+    count = len(f)
     rec = np.zeros((count, 24), np.float32)
-    rec[:, 0] = t
-    rec[:, 1:4] = s["position"]
+    rec[:, 0] = f["timestamp_seconds"]
+    rec[:, 1:4] = f["camera_origin"]
     m = np.zeros((count, 4, 4))
-    m[:, :3, :3] = quaternion_matrices(s["rotation"])
-    m[:, :3, 3] = s["position"]
+    m[:, :3, :3] = np.transpose(f["camera_world"].reshape(count, 3, 3), (0, 2, 1))
+    m[:, :3, 3] = f["camera_origin"]
     m[:, 3, 3] = 1
     rec[:, 4:20] = m.reshape(count, 16)
-    rec[:, 20:23] = s["light"]
-    rec[:, 23] = s["intensity"]
+    rec[:, 20:23] = f["light_dir"]
+    rec[:, 23] = f["light_intensity"]
     rec.tofile(d / "frames.bin")
 
 
