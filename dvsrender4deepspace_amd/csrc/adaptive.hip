@@ -494,6 +494,10 @@ __device__ __forceinline__ bool adapt_rates_run(const SegDev *segs, const AdaptD
   // cell serials from the segment's count, not the pixel state: the circuit
   // of the previous chunk may be updating the state while this runs
   const uint64_t serial0 = A.cell_base;
+  // the segment's fields in registers (the stores below could alias them)
+  const PhotoParams ph = g.photo;
+  const uint64_t seed = g.seed;
+  const CellMaps *maps = g.maps;
   double *out = b.rates + gp;
   uint64_t shared[4] = {0, 0, 0, 0}, shared_index = ~0ull;
   bool marked = false;
@@ -506,11 +510,11 @@ __device__ __forceinline__ bool adapt_rates_run(const SegDev *segs, const AdaptD
     light = fmax(0.0, light);
     if (!isfinite(light)) atomicOr(status, 2);
     const uint64_t serial = serial0 + (uint64_t)c;
-    if (g.photo.shot_noise && serial / kShotCellsPerBlock != shared_index) {
+    if (ph.shot_noise && serial / kShotCellsPerBlock != shared_index) {
       shared_index = serial / kShotCellsPerBlock;
-      shot_block(g.seed, (uint64_t)i, shared_index, shared);
+      shot_block(seed, (uint64_t)i, shared_index, shared);
     }
-    const double rate = cell_rate_first(light, g.photo, g.t1[c] - g.t0[c], serial, shared);
+    const double rate = cell_rate_first(light, ph, maps + c, serial, shared);
     out[(size_t)(c - c0) * pixels] = rate;
     marked |= rate < 0;
   }

@@ -146,17 +146,7 @@ __device__ inline double shot_stream_draw(uint64_t seed, uint64_t pixel, uint64_
 // branch (or needs no draw) and returns -lambda (< 0) for a cell whose draw
 // needs the cell's own stream (lambda >= 10); cell_rate_rest finishes it.
 // `shared` is shot_block(seed, pixel, cell_serial / kShotCellsPerBlock)
-// (unused without noise).
-__device__ __forceinline__ double cell_rate_first(double light, const PhotoParams &ph, double dt, uint64_t cell_serial,
-                                                  const uint64_t shared[4]) {
-  const double rate = light * ph.electrons_per_radiance + ph.dark_electrons;
-  if (!ph.shot_noise) return rate;
-  const double lambda = rate * dt;
-  if (lambda != lambda) return lambda;  // NaN photocurrent: the circuit reports it
-  if (lambda <= 0) return 0.0 / dt;
-  if (lambda < 10) return poisson_inversion(lambda, shot_uniform(shared, cell_serial)) / dt;
-  return -lambda;
-}
+// (unused without noise).  cell_rate_first follows the cell maps below.
 __device__ inline double cell_rate_rest(double marked, double dt, uint64_t seed, uint64_t pixel, uint64_t cell_serial) {
   return shot_stream_draw(seed, pixel, cell_serial, -marked) / dt;
 }
@@ -332,6 +322,26 @@ __device__ __forceinline__ CellMaps cell_maps(double t0, double t1, double charg
     m.map[k + 1][1] = f.q;
   }
   return m;
+}
+// First phase of a cell's photocurrent (see cell_rate_rest) from its
+// CellMaps entry: dt = t1 - t0, and the counts one and two divided by dt
+// taken from the entry, which holds the same quotients.
+__device__ __forceinline__ double cell_rate_first(double light, const PhotoParams &ph, const CellMaps *m,
+                                                  uint64_t cell_serial, const uint64_t shared[4]) {
+  const double dt = m->t1 - m->t0;
+  const double rate = light * ph.electrons_per_radiance + ph.dark_electrons;
+  if (!ph.shot_noise) return rate;
+  const double lambda = rate * dt;
+  if (lambda != lambda) return lambda;  // NaN photocurrent: the circuit reports it
+  if (lambda <= 0) return 0.0 / dt;
+  if (lambda < 10) {
+    const double k = poisson_inversion(lambda, shot_uniform(shared, cell_serial));
+    if (k == 0 && dt > 0) return 0.0;  // 0 / dt
+    if (k == 1) return m->rate[0];
+    if (k == 2) return m->rate[1];
+    return k / dt;
+  }
+  return -lambda;
 }
 // inverse_map over the whole cell, dt = t1 - t0 > 0, rate >= 0.
 __device__ __forceinline__ double inverse_map_cell(double y, double rate, double dt, double charge,

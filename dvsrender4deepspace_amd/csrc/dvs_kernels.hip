@@ -199,7 +199,7 @@ __global__ __launch_bounds__(256) void cell_rates_multi(const SegDev *segs, cons
       const uint64_t serial = g.states[px].shot_block + (uint64_t)c;
       uint64_t shared[4] = {0, 0, 0, 0};
       if (g.photo.shot_noise) shot_block(g.seed, (uint64_t)px, serial / kShotCellsPerBlock, shared);
-      const double rate = cell_rate_first(light, g.photo, g.t1[c] - g.t0[c], serial, shared);
+      const double rate = cell_rate_first(light, g.photo, g.maps + c, serial, shared);
       *cell = rate;
       marked = rate < 0;
     }
