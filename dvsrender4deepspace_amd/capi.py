@@ -233,6 +233,7 @@ def load_library() -> ctypes.CDLL:
         "dvs_hapke_eval": (ctypes.c_int, [_P, _P, _P, _P, _I64, ctypes.POINTER(_HapkeC), _P]),
         "dvs_receptor_eval": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, _I64, _P]),
         "dvs_noise_eval": (ctypes.c_int, [ctypes.c_uint64, _I64, ctypes.c_double, _I32, _P, _P]),
+        "dvs_poisson_eval": (ctypes.c_int, [_P, _P, _I64, _P]),
         "dvs_trace_rays": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P, _P, _P, _P]),
         "dvs_trace_shadow_rays": (ctypes.c_int, [_P, _P, _P, _P, _I64, _P]),
         "dvs_trajectory_create": (ctypes.c_int, [_P, _I32, ctypes.c_float, _P, _P, ctypes.POINTER(_P)]),
@@ -661,6 +662,15 @@ def receptor_eval(u, rate, dt, charge: float) -> np.ndarray:
     a, b, c = (np.ascontiguousarray(x) for x in (a, b, c))
     out = np.empty(a.shape, np.float64)
     _check(load_library().dvs_receptor_eval(_ptr(a), _ptr(b), _ptr(c), float(charge), a.size, _ptr(out)))
+    return out
+
+
+def poisson_eval(lam, u) -> np.ndarray:
+    """The device's Poisson inversion count of each uniform u at lambda."""
+    lam = np.ascontiguousarray(lam, np.float64)
+    u = np.ascontiguousarray(u, np.float64)
+    out = np.empty(len(lam), np.float64)
+    _check(load_library().dvs_poisson_eval(_ptr(lam), _ptr(u), len(lam), _ptr(out)))
     return out
 
 

@@ -270,6 +270,18 @@ int dvs_receptor_eval(const double *u, const double *rate, const double *dt, dou
   });
 }
 
+int dvs_poisson_eval(const double *lambda, const double *u, int64_t n, double *out) {
+  return guarded([&] {
+    if (!lambda || !u || !out || n < 0) fail(DVS_E_ARGUMENT, "invalid poisson arguments");
+    DeviceArray<double> a, b, o;
+    a.upload(lambda, n, "poisson input");
+    b.upload(u, n, "poisson input");
+    o.reserve(n, "poisson output");
+    hip_check(launch_poisson(n, a.ptr, b.ptr, o.ptr, nullptr), "poisson kernel");
+    hip_check(hipMemcpy(out, o.ptr, n * sizeof(double), hipMemcpyDeviceToHost), "poisson output");
+  });
+}
+
 int dvs_noise_eval(uint64_t seed, int64_t pixels, double lambda, int32_t draws, double *poisson, double *normal) {
   return guarded([&] {
     if (pixels < 0 || draws < 0) fail(DVS_E_ARGUMENT, "invalid noise arguments");

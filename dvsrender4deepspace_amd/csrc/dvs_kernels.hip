@@ -362,6 +362,10 @@ __global__ void receptor_kernel(int64_t n, const double *u, const double *rate, 
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = receptor_log_at(u[i], rate[i], dt[i], charge);
 }
+__global__ void poisson_kernel(int64_t n, const double *lambda, const double *u, double *out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = poisson_inversion(lambda[i], u[i]);
+}
 __global__ void noise_kernel(uint64_t seed, int64_t pixels, double lambda, int draws, double *poisson,
                              double *normal) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -551,6 +555,11 @@ hipError_t launch_receptor(int64_t n, const double *u, const double *rate, const
                            double *out, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   receptor_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, u, rate, dt, charge, out);
+  return hipGetLastError();
+}
+hipError_t launch_poisson(int64_t n, const double *lambda, const double *u, double *out, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  poisson_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, lambda, u, out);
   return hipGetLastError();
 }
 hipError_t launch_noise(uint64_t seed, int64_t pixels, double lambda, int draws, double *poisson, double *normal,

@@ -340,6 +340,9 @@ int dvs_hapke_eval(const float *cosi, const float *cose, const float *cosg, cons
 /* receptor_log_at (include/dvs/pixel_model.hpp:18-26) over n inputs. */
 int dvs_receptor_eval(const double *u, const double *rate, const double *dt, double charge,
                       int64_t n, double *out);
+/* The device's sequential-search Poisson inversion of uniform u[i] at
+ * lambda[i] (0 < lambda < 10), the shot-noise draw of a circuit cell. */
+int dvs_poisson_eval(const double *lambda, const double *u, int64_t n, double *out);
 /* Poisson / normal draws of the device noise streams for (seed, pixel). */
 int dvs_noise_eval(uint64_t seed, int64_t pixels, double lambda, int32_t draws, double *poisson,
                    double *normal);

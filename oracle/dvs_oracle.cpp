@@ -914,6 +914,13 @@ void orc_parameters(const orc_physic *c, int effects, orc_pixel_params *p, orc_p
 }
 
 void orc_philox4x64(const uint64_t ctr[4], const uint64_t key[2], uint64_t out[4]) { philox4x64(ctr, key, out); }
+// Sequential-search inversion (the lambda < 10 branch of poisson_draw).
+double orc_poisson_inversion(double lambda, double u) {
+  double p = exp(-lambda), F = p;
+  int k = 0;
+  while (u > F && k < 1000) { ++k; p *= lambda / k; F += p; }
+  return k;
+}
 void orc_poisson_series(uint64_t seed, uint64_t pixel, double lambda, int n, double *out) {
   uint64_t block = 0; Stream st{seed, pixel, kStreamShot, &block};
   for (int i = 0; i < n; ++i) out[i] = poisson_draw(st, lambda);

@@ -188,6 +188,8 @@ void orc_poisson_series(uint64_t seed, uint64_t pixel, double lambda, int n, dou
 /* Shot-noise electron counts of cells serial0 .. serial0 + n - 1 of one
  * pixel at mean lambda, as the photocurrent draws them. */
 void orc_shot_counts(uint64_t seed, uint64_t pixel, double lambda, uint64_t serial0, int n, double *out);
+/* The Poisson inversion of one uniform (0 < lambda < 10). */
+double orc_poisson_inversion(double lambda, double u);
 void orc_normal_series(uint64_t seed, uint64_t pixel, int n, double *out);
 
 /* ---- Sensor (src/cuda/event_sensor.cu) ---- */

@@ -119,6 +119,7 @@ def lib() -> ctypes.CDLL:
             "orc_poisson_series": (None, [ctypes.c_uint64, ctypes.c_uint64, D, I, _P]),
             "orc_normal_series": (None, [ctypes.c_uint64, ctypes.c_uint64, I, _P]),
             "orc_shot_counts": (None, [ctypes.c_uint64, ctypes.c_uint64, D, ctypes.c_uint64, I, _P]),
+            "orc_poisson_inversion": (D, [D, D]),
             "orc_sensor_create": (_P, [_P, I, ctypes.POINTER(Physic), I, I]),
             "orc_sensor_destroy": (None, [_P]),
             "orc_sensor_initialize": (I, [_P, _P, _P, D, I]),
@@ -336,6 +337,11 @@ def shot_counts(seed, pixel, lam, serial0, n):
     out = np.empty(n, np.float64)
     lib().orc_shot_counts(seed, pixel, lam, serial0, n, ptr(out))
     return out
+
+
+def poisson_inversion(lam, u):
+    f = lib().orc_poisson_inversion
+    return np.array([f(float(a), float(b)) for a, b in zip(np.atleast_1d(lam), np.atleast_1d(u))])
 
 
 def normal_series(seed, pixel, n):

@@ -318,3 +318,37 @@ def test_noise_on_reproducible_per_seed(gpu):
     assert len(runs[0]) > 0
     assert len(runs[0]) == len(runs[1]) and runs[0].tobytes() == runs[1].tobytes()
     assert runs[0].tobytes() != runs[2].tobytes()
+
+
+def test_poisson_inversion_at_thresholds(gpu, oracle):
+    """The device's sequential-search Poisson inversion equals the oracle's
+    at uniforms 1e-12 .. 1e-3 either side of each of its first six
+    cumulative thresholds and at random ones, for lambda from 1e-4 to 10
+    (offsets stay above the last-place differences of the two exp)."""
+    import math
+    rng = np.random.default_rng(8)
+    lams = np.concatenate([np.geomspace(1e-4, 0.999, 400), np.linspace(1.0, 9.9, 20)])
+    lam_list, u_list = [], []
+    for lam in lams:
+        p = math.exp(-lam)
+        F = p
+        thresholds = [F]
+        for k in range(1, 6):
+            p *= lam / k
+            F += p
+            thresholds.append(F)
+        for F in thresholds:
+            for d in (1e-12, 1e-9, 1e-6, 1.9e-5, 2.1e-5, 1e-4, 1e-3):
+                for u in (F - d, F + d):
+                    if 0 < u < 1:
+                        lam_list.append(lam)
+                        u_list.append(u)
+        for u in rng.random(40):
+            lam_list.append(lam)
+            u_list.append(u)
+    lam_a, u_a = np.array(lam_list), np.array(u_list)
+    print(f"poisson inversion: {len(lam_a)} draws", flush=True)
+    dev = gpu.poisson_eval(lam_a, u_a)
+    ref = oracle.poisson_inversion(lam_a, u_a)
+    bad = np.nonzero(dev != ref)[0]
+    assert len(bad) == 0, [(lam_a[i], u_a[i], dev[i], ref[i]) for i in bad[:5]]
