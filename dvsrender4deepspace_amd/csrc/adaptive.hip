@@ -501,11 +501,17 @@ __device__ __forceinline__ bool adapt_rates_run(const SegDev *segs, const AdaptD
   double *out = b.rates + gp;
   uint64_t shared[4] = {0, 0, 0, 0}, shared_index = ~0ull;
   bool marked = false;
+  // the knot pair around the cell stays in registers (knot_bracket's walk)
+  double2 va = v[a], vb = v[a + 1];
   for (int c = cb; c < ce; ++c) {
     const double f = frac[c];
-    a = knot_bracket(v, size, f, a);
-    const double w = (f - fabs(v[a].x)) / (fabs(v[a + 1].x) - fabs(v[a].x));
-    double light = v[a].y + (v[a + 1].y - v[a].y) * w;
+    while (a + 2 < size && f > fabs(vb.x)) {
+      ++a;
+      va = vb;
+      vb = v[a + 1];
+    }
+    const double w = (f - fabs(va.x)) / (fabs(vb.x) - fabs(va.x));
+    double light = va.y + (vb.y - va.y) * w;
     if (e < ec && b.ecell[eo + e] == c) light = b.erad[eo + e++];
     light = fmax(0.0, light);
     if (!isfinite(light)) atomicOr(status, 2);
