@@ -501,10 +501,20 @@ __device__ __forceinline__ bool adapt_rates_run(const SegDev *segs, const AdaptD
   double *out = b.rates + gp;
   uint64_t shared[4] = {0, 0, 0, 0}, shared_index = ~0ull;
   bool marked = false;
-  // the knot pair around the cell stays in registers (knot_bracket's walk)
+  // the knot pair around the cell stays in registers (knot_bracket's walk),
+  // and the next cell's fraction and map words are loaded a cell ahead
   double2 va = v[a], vb = v[a + 1];
+  // (entries are 16-byte aligned: {t0, t1} and {rate[0], rate[1]})
+  double f_next = frac[cb];
+  double2 t_next = reinterpret_cast<const double2 *>(maps + cb)[0];
+  double2 r_next = reinterpret_cast<const double2 *>(maps + cb)[1];
   for (int c = cb; c < ce; ++c) {
-    const double f = frac[c];
+    const double f = f_next;
+    const double2 t = t_next, r = r_next;
+    const int cn = min(c + 1, ce - 1);
+    f_next = frac[cn];
+    t_next = reinterpret_cast<const double2 *>(maps + cn)[0];
+    r_next = reinterpret_cast<const double2 *>(maps + cn)[1];
     while (a + 2 < size && f > fabs(vb.x)) {
       ++a;
       va = vb;
@@ -520,7 +530,7 @@ __device__ __forceinline__ bool adapt_rates_run(const SegDev *segs, const AdaptD
       shared_index = serial / kShotCellsPerBlock;
       shot_block(seed, (uint64_t)i, shared_index, shared);
     }
-    const double rate = cell_rate_first(light, ph, maps + c, serial, shared);
+    const double rate = cell_rate_first(light, ph, t.y - t.x, r.x, r.y, serial, shared);
     out[(size_t)(c - c0) * pixels] = rate;
     marked |= rate < 0;
   }

@@ -323,12 +323,11 @@ __device__ __forceinline__ CellMaps cell_maps(double t0, double t1, double charg
   }
   return m;
 }
-// First phase of a cell's photocurrent (see cell_rate_rest) from its
-// CellMaps entry: dt = t1 - t0, and the counts one and two divided by dt
-// taken from the entry, which holds the same quotients.
-__device__ __forceinline__ double cell_rate_first(double light, const PhotoParams &ph, const CellMaps *m,
-                                                  uint64_t cell_serial, const uint64_t shared[4]) {
-  const double dt = m->t1 - m->t0;
+// First phase of a cell's photocurrent (see cell_rate_rest) from the words
+// of its CellMaps entry: dt = t1 - t0, and the counts one and two divided
+// by dt (the entry's rate[0] and rate[1], the same quotients).
+__device__ __forceinline__ double cell_rate_first(double light, const PhotoParams &ph, double dt, double one_per_dt,
+                                                  double two_per_dt, uint64_t cell_serial, const uint64_t shared[4]) {
   const double rate = light * ph.electrons_per_radiance + ph.dark_electrons;
   if (!ph.shot_noise) return rate;
   const double lambda = rate * dt;
@@ -337,8 +336,8 @@ __device__ __forceinline__ double cell_rate_first(double light, const PhotoParam
   if (lambda < 10) {
     const double k = poisson_inversion(lambda, shot_uniform(shared, cell_serial));
     if (k == 0 && dt > 0) return 0.0;  // 0 / dt
-    if (k == 1) return m->rate[0];
-    if (k == 2) return m->rate[1];
+    if (k == 1) return one_per_dt;
+    if (k == 2) return two_per_dt;
     return k / dt;
   }
   return -lambda;

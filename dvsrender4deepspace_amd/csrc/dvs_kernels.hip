@@ -199,7 +199,8 @@ __global__ __launch_bounds__(256) void cell_rates_multi(const SegDev *segs, cons
       const uint64_t serial = g.states[px].shot_block + (uint64_t)c;
       uint64_t shared[4] = {0, 0, 0, 0};
       if (g.photo.shot_noise) shot_block(g.seed, (uint64_t)px, serial / kShotCellsPerBlock, shared);
-      const double rate = cell_rate_first(light, g.photo, g.maps + c, serial, shared);
+      const CellMaps &m = g.maps[c];
+      const double rate = cell_rate_first(light, g.photo, m.t1 - m.t0, m.rate[0], m.rate[1], serial, shared);
       *cell = rate;
       marked = rate < 0;
     }
