@@ -120,10 +120,22 @@ __device__ __forceinline__ f3 inverse_direction(f3 d) {  // device_ray.hpp:6-12
   inv.z = 1.0f / (fabsf(d.z) > 1e-20f ? d.z : copysignf(1e-20f, d.z));
   return inv;
 }
+// Scene arrays in the global address space and the traversal stack in LDS:
+// generic (flat) accesses would wait on both memory counters.
+typedef const __attribute__((address_space(1))) float *GlobalFloats;
+typedef const __attribute__((address_space(1))) int *GlobalInts;
+typedef __attribute__((address_space(3))) int *LdsInts;
+// float4 i of a record (16-byte aligned), generic or global.
+__device__ __forceinline__ float4 record4(const float4 *rec, int i) { return rec[i]; }
+__device__ __forceinline__ float4 record4(GlobalFloats rec, int i) {
+  return make_float4(rec[4 * i], rec[4 * i + 1], rec[4 * i + 2], rec[4 * i + 3]);
+}
+
 // Slab test of both children of one pair record (device_ray.hpp:13-53).
-__device__ __forceinline__ int test_pair(const float4 *rec, f3 P, f3 idir, float t_min, float t_max,
+template <class Rec>
+__device__ __forceinline__ int test_pair(Rec rec, f3 P, f3 idir, float t_min, float t_max,
                                          float &d0, float &d1, int &left, int &right) {
-  const float4 a = rec[0], b = rec[1], c = rec[2], r = rec[3];
+  const float4 a = record4(rec, 0), b = record4(rec, 1), c = record4(rec, 2), r = record4(rec, 3);
   const float t1x0 = (a.x - P.x) * idir.x, t1x1 = (a.w - P.x) * idir.x;
   const float t1y0 = (a.y - P.y) * idir.y, t1y1 = (b.x - P.y) * idir.y;
   const float t1z0 = (a.z - P.z) * idir.z, t1z1 = (b.y - P.z) * idir.z;
@@ -142,9 +154,9 @@ __device__ __forceinline__ int test_pair(const float4 *rec, f3 P, f3 idir, float
 }
 // Moller-Trumbore on a pre-gathered record (device_ray.hpp:54-80).  `meta`
 // receives the record's last float4: (e2.z, reference index, leaf lo, leaf hi).
-__device__ __forceinline__ bool intersect(const float4 *rec, f3 ray, f3 o, float &ot, float &ou, float &ov,
-                                          float4 &meta) {
-  const float4 a = rec[0], b = rec[1], c = rec[2];
+template <class Rec>
+__device__ __forceinline__ bool intersect(Rec rec, f3 ray, f3 o, float &ot, float &ou, float &ov, float4 &meta) {
+  const float4 a = record4(rec, 0), b = record4(rec, 1), c = record4(rec, 2);
   meta = c;
   const f3 v0 = mk(a.x, a.y, a.z), e1 = mk(a.w, b.x, b.y), e2 = mk(b.z, b.w, c.x);
   const f3 pvec = cross(ray, e2);
@@ -187,17 +199,20 @@ __device__ inline Hit trace_nearest(const DevScene &s, f3 origin, f3 ray, int *s
   Hit h{0, -1, -1, 0.f, 0.f, 0.f};
   float closest = 1e30f;
   const f3 idir = inverse_direction(ray);
+  const GlobalFloats pairs = (GlobalFloats)s.pairs, tris = (GlobalFloats)s.triangles;
+  const GlobalInts leaves = (GlobalInts)s.leaves;
+  const LdsInts lds = (LdsInts)stack;
   int ref = s.root_ref, sp = 0;
   for (;;) {
     if (ref < 0) {
-      const int2 leaf = s.leaves[~ref];
+      const int2 leaf = make_int2(leaves[2 * ~ref], leaves[2 * ~ref + 1]);
       cnt.leaf();
       for (int i = 0; i < leaf.y; ++i) {
         const int pos = leaf.x + i;
         float t, u, v;
         float4 meta;
         cnt.triangle();
-        if (!intersect(s.triangles + 3 * (size_t)pos, ray, origin, t, u, v, meta)) continue;
+        if (!intersect(tris + 12 * (size_t)pos, ray, origin, t, u, v, meta)) continue;
         // Strictly nearer wins; an equal distance wins only against a hit of
         // the same reference leaf with a higher index (its sequential scan).
         const int id = __float_as_int(meta.y);
@@ -209,11 +224,11 @@ __device__ inline Hit trace_nearest(const DevScene &s, f3 origin, f3 ray, int *s
     } else {
       float d0, d1; int l, r;
       cnt.pair();
-      const int mask = test_pair(s.pairs + 4 * (size_t)ref, origin, idir, kRayEps, closest, d0, d1, l, r);
+      const int mask = test_pair(pairs + 16 * (size_t)ref, origin, idir, kRayEps, closest, d0, d1, l, r);
       if (mask == 3) {
         if (sp >= s.stack_depth) { h.hit = -1; return h; }
         const bool left_first = d0 < d1;
-        stack[sp * stride] = left_first ? r : l;
+        lds[sp * stride] = left_first ? r : l;
         ++sp;
         ref = left_first ? l : r;
         continue;
@@ -222,7 +237,7 @@ __device__ inline Hit trace_nearest(const DevScene &s, f3 origin, f3 ray, int *s
     }
     if (sp == 0) break;
     --sp;
-    ref = stack[sp * stride];
+    ref = lds[sp * stride];
   }
   return h;
 }
@@ -233,14 +248,17 @@ template <class Count>
 __device__ inline bool trace_any(const DevScene &s, f3 origin, f3 dir, int self, float t_min,
                                  float t_max, int *stack, int stride, Count &cnt) {
   const f3 idir = inverse_direction(dir);
+  const GlobalFloats pairs = (GlobalFloats)s.pairs, tris = (GlobalFloats)s.triangles;
+  const GlobalInts leaves = (GlobalInts)s.leaves;
+  const LdsInts lds = (LdsInts)stack;
   int ref = s.root_ref, sp = 0;
   for (;;) {
     if (ref < 0) {
-      const int2 leaf = s.leaves[~ref];
+      const int2 leaf = make_int2(leaves[2 * ~ref], leaves[2 * ~ref + 1]);
       cnt.leaf();
       for (int i = 0; i < leaf.y; ++i) {
-        const float4 *rec = s.triangles + 3 * (size_t)(leaf.x + i);
-        if (__float_as_int(rec[2].y) == self) continue;
+        const GlobalFloats rec = tris + 12 * (size_t)(leaf.x + i);
+        if (__float_as_int(rec[9]) == self) continue;  // record word 9: the reference triangle index
         float t, u, v;
         float4 meta;
         cnt.triangle();
@@ -249,11 +267,11 @@ __device__ inline bool trace_any(const DevScene &s, f3 origin, f3 dir, int self,
     } else {
       float d0, d1; int l, r;
       cnt.pair();
-      const int mask = test_pair(s.pairs + 4 * (size_t)ref, origin, idir, t_min, t_max, d0, d1, l, r);
+      const int mask = test_pair(pairs + 16 * (size_t)ref, origin, idir, t_min, t_max, d0, d1, l, r);
       if (mask == 3) {
         if (sp >= s.stack_depth) return true;
         const bool left_first = d0 < d1;
-        stack[sp * stride] = left_first ? r : l;
+        lds[sp * stride] = left_first ? r : l;
         ++sp;
         ref = left_first ? l : r;
         continue;
@@ -262,7 +280,7 @@ __device__ inline bool trace_any(const DevScene &s, f3 origin, f3 dir, int self,
     }
     if (sp == 0) break;
     --sp;
-    ref = stack[sp * stride];
+    ref = lds[sp * stride];
   }
   return false;
 }
