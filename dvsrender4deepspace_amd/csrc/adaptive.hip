@@ -108,8 +108,9 @@ __global__ __launch_bounds__(256) void adapt_trace(const SegDev *segs, const int
     if (!reused)
       r = sample_subray<true>(segs[si].scene, gp - pixel_begin[si], pose, sub % samples, sub / samples, stack, 256);
   }
-  stack[0] = __double2loint(r.radiance);
-  stack[256] = __double2hiint(r.radiance);
+  const LdsInts lw = (LdsInts)words;  // the lanes' radiances, low then high words
+  lw[threadIdx.x] = __double2loint(r.radiance);
+  lw[256 + threadIdx.x] = __double2hiint(r.radiance);
   const unsigned long long shadows = __ballot(r.shadow);
   const unsigned long long invalid = __ballot(r.invalid);
   if ((threadIdx.x & 63) == 0) {
@@ -125,7 +126,7 @@ __global__ __launch_bounds__(256) void adapt_trace(const SegDev *segs, const int
     return;
   }
   double total = 0;
-  for (int j = 0; j < R; ++j) total += __hiloint2double(words[256 + threadIdx.x + j], words[threadIdx.x + j]);
+  for (int j = 0; j < R; ++j) total += __hiloint2double(lw[256 + threadIdx.x + j], lw[threadIdx.x + j]);
   if (!isfinite(total)) atomicOr(status, 2);
   out[at] = total / R;
 }

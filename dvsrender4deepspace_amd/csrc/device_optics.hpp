@@ -286,8 +286,9 @@ __device__ inline bool trace_any(const DevScene &s, f3 origin, f3 dir, int self,
 }
 
 // shadow_origin (device_optics.cuh:37-51)
-__device__ inline f3 shadow_origin(const float4 *rec, float u, float v, f3 light) {
-  const float4 ra = rec[0], rb = rec[1], rc = rec[2];
+template <class Rec>
+__device__ inline f3 shadow_origin(Rec rec, float u, float v, f3 light) {
+  const float4 ra = record4(rec, 0), rb = record4(rec, 1), rc = record4(rec, 2);
   const f3 a = mk(ra.x, ra.y, ra.z), e1 = mk(ra.w, rb.x, rb.y), e2 = mk(rb.z, rb.w, rc.x);
   f3 point = add(add(a, scale(e1, u)), scale(e2, v));
   f3 normal = normalize(cross(e1, e2));
@@ -427,17 +428,18 @@ __device__ inline SubSample sample_subray(const DevScene &s, int pixel, const Po
   if (!hit.hit) return out;
   out.depth = hit.t;
   if (!SHADE) return out;
-  const uint32_t i0 = s.indices[3 * (size_t)hit.tri], i1 = s.indices[3 * (size_t)hit.tri + 1],
-                 i2 = s.indices[3 * (size_t)hit.tri + 2];
+  const GlobalInts indices = (GlobalInts)s.indices;
+  const GlobalFloats normals = (GlobalFloats)s.normals;
+  const uint32_t i0 = (uint32_t)indices[3 * (size_t)hit.tri], i1 = (uint32_t)indices[3 * (size_t)hit.tri + 1],
+                 i2 = (uint32_t)indices[3 * (size_t)hit.tri + 2];
   const float w = 1 - hit.u - hit.v;
-  const f3 n0 = mk(s.normals[3 * (size_t)i0], s.normals[3 * (size_t)i0 + 1], s.normals[3 * (size_t)i0 + 2]);
-  const f3 n1 = mk(s.normals[3 * (size_t)i1], s.normals[3 * (size_t)i1 + 1], s.normals[3 * (size_t)i1 + 2]);
-  const f3 n2 = mk(s.normals[3 * (size_t)i2], s.normals[3 * (size_t)i2 + 1], s.normals[3 * (size_t)i2 + 2]);
+  const f3 n0 = mk(normals[3 * (size_t)i0], normals[3 * (size_t)i0 + 1], normals[3 * (size_t)i0 + 2]);
+  const f3 n1 = mk(normals[3 * (size_t)i1], normals[3 * (size_t)i1 + 1], normals[3 * (size_t)i1 + 2]);
+  const f3 n2 = mk(normals[3 * (size_t)i2], normals[3 * (size_t)i2 + 1], normals[3 * (size_t)i2 + 2]);
   const f3 normal = normalize(add(add(scale(n0, w), scale(n1, hit.u)), scale(n2, hit.v)));
   const f3 light = normalize(mk(pose.lx, pose.ly, pose.lz));
   if (dot(normal, light) <= 0 || dot(normal, neg(ray)) <= 0) return out;
-  const float4 *rec = s.triangles + 3 * (size_t)hit.rec;
-  const f3 so = shadow_origin(rec, hit.u, hit.v, light);
+  const f3 so = shadow_origin((GlobalFloats)s.triangles + 12 * (size_t)hit.rec, hit.u, hit.v, light);
   out.shadow = 1;
   if (trace_any(s, so, light, hit.tri, kRayEps, 1e30f, stack, stride, cs)) return out;
   const float albedo = s.albedos ? material_albedo(s.albedos[i0], s.hapke.w) * w +

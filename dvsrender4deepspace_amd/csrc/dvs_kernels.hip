@@ -162,8 +162,9 @@ __global__ __launch_bounds__(256) void trace_cells_multi(const SegDev *segs, con
     const int px = g.pixel_list ? g.pixel_list[pixel] : pixel;
     r = sample_subray<true>(s, px, pose, sub % s.samples, sub / s.samples, stack, 256);
   }
-  stack[0] = __double2loint(r.radiance);
-  stack[256] = __double2hiint(r.radiance);
+  const LdsInts lw = (LdsInts)words;  // the lanes' radiances, low then high words
+  lw[threadIdx.x] = __double2loint(r.radiance);
+  lw[256 + threadIdx.x] = __double2hiint(r.radiance);
   const unsigned long long shadows = __ballot(r.shadow);
   const unsigned long long invalid = __ballot(r.invalid);
   if ((threadIdx.x & 63) == 0) {
@@ -173,7 +174,7 @@ __global__ __launch_bounds__(256) void trace_cells_multi(const SegDev *segs, con
   __syncthreads();
   if (!active || sub != 0) return;
   double total = 0;
-  for (int j = 0; j < R; ++j) total += __hiloint2double(words[256 + threadIdx.x + j], words[threadIdx.x + j]);
+  for (int j = 0; j < R; ++j) total += __hiloint2double(lw[256 + threadIdx.x + j], lw[threadIdx.x + j]);
   g.rad[(size_t)cell * npix + pixel] = total / R;
 }
 
