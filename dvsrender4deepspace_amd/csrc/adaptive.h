@@ -7,8 +7,9 @@
 // 16-entry stack).  Here every unresolved segment of every pixel is one lane
 // of a level-synchronous wavefront: a level decides all pending segments,
 // compacts the resolved ones and the bisected ones by wave ballot, traces the
-// bisected segments' two new probes as one ray batch and forms the next
-// level.  The result equals the depth-first one:
+// bisected segments' new probes as one ray batch (with the midpoints of the
+// halves' halves, so that the next level's bisections need no trace) and
+// forms the next level.  The result equals the depth-first one:
 //  * a segment's decision depends only on its own values and on the number
 //    of left turns on its path (the depth-first stack height when it is
 //    popped), which each entry carries;
@@ -17,7 +18,10 @@
 //    bound (1 + pending segments + knots written) only grows, so a pixel is
 //    dropped as soon as the bound passes the capacity;
 //  * the depth-first knot order is the order of the resolved segments' start
-//    fractions, recovered by sorting (pixel, start) keys.
+//    fractions.  A pixel's resolved segments partition its window, so a
+//    segment's first knot follows the knots of the pixel's segments that
+//    start before it: each segment is placed by counting those in a
+//    per-pixel bucket, without a sort.
 #pragma once
 #include "kernels.h"
 
