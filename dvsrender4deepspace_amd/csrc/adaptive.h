@@ -81,11 +81,16 @@ struct AdaptBufs {
   double *rates;         // [cell - chunk start][pixel] photocurrent of knot pixels
 };
 
-// Refinement counters of one window: the resolved segments so far, then
-// the traced and untraced bisections of each level (level L at 2 + 2L), so
-// that no level needs its counters cleared on its own.
+// Refinement counters of one window, cleared once per window.  Four head
+// words, then the traced and untraced bisections of each level (level L >= 1
+// at kRefineHeadWords + 2 (L - 1)).
+//   head[0] resolved segments so far
+//   head[1] exact segments so far
+//   head[2] pixels whose knot-count bound has passed their capacity
+//   head[3] unused
 constexpr int kMaxRefineLevels = 64;
-constexpr int kRefineCounterWords = 2 + 2 * kMaxRefineLevels;
+constexpr int kRefineHeadWords = 4;
+constexpr int kRefineCounterWords = kRefineHeadWords + 2 * kMaxRefineLevels;
 
 // OpticalRefinementStats of one segment (include/dvs/optical_sampler.hpp:16).
 struct AdaptSegStats {
@@ -99,6 +104,7 @@ namespace dvsamd {
 using dvsdev::AdaptBufs;
 using dvsdev::kMaxRefineLevels;
 using dvsdev::kRefineCounterWords;
+using dvsdev::kRefineHeadWords;
 using dvsdev::AdaptDev;
 using dvsdev::AdaptEntry;
 using dvsdev::AdaptSegStats;
@@ -118,13 +124,15 @@ hipError_t launch_adapt_trace(int layout, const SegDev *segs, const int *pixel_b
                               hipStream_t st, const int *reuse = nullptr, const double *prev_probes = nullptr);
 hipError_t launch_adapt_init(int pixels, AdaptBufs b, AdaptEntry *entries, hipStream_t st);
 // One level: resolved segments to terms; a bisected segment whose halves'
-// midpoints were traced ahead writes its halves to `direct` (ctr->direct
+// midpoints were traced ahead writes its halves to `direct` (level_ctr[1]
 // pairs), any other to `bisected` with kBisectQueries queries each
-// (ctr->bisect).
-hipError_t launch_adapt_decide(int n, const AdaptEntry *entries, const SegDev *segs, const AdaptDev *ad,
-                               const int *pixel_begin, int nseg, AdaptBufs b, AdaptTerm *terms,
+// (level_ctr[0]).  The level has n entries, or, with prev_level (the
+// previous level's two counters), 2 (prev_level[0] + prev_level[1]) entries
+// of which n is an upper bound.  head: the four head counter words.
+hipError_t launch_adapt_decide(int n, const int *prev_level, const AdaptEntry *entries, const SegDev *segs,
+                               const AdaptDev *ad, const int *pixel_begin, int nseg, AdaptBufs b, AdaptTerm *terms,
                                unsigned long long *keys, AdaptEntry *bisected, int *query_pixel, Pose *query_pose,
-                               AdaptEntry *direct, int *terms_ctr, int *level_ctr, hipStream_t st);
+                               AdaptEntry *direct, int *head, int *level_ctr, hipStream_t st);
 // The halves of the traced bisections, with their halves' midpoints.
 hipError_t launch_adapt_children(int n, const AdaptEntry *bisected, const double *query_rad, AdaptEntry *next,
                                  hipStream_t st);

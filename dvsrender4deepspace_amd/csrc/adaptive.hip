@@ -8,7 +8,7 @@
 //                        50-79); resolved and bisected segments compacted by
 //                        wave ballot + one atomic per wave
 //  adapt_children        the two halves of every bisected segment
-//  adapt_counts / term_runs / write_knots
+//  adapt_counts / term_bucket / place_knots
 //                        knot counts, prefix offsets and depth-first knot order
 //  adapt_exact_*         identify_exact_cells (optical_sampler.cu:114-131)
 //  adapt_gate            advance_mean_probes (event_sensor.cu:138-150)
@@ -145,12 +145,13 @@ __global__ void adapt_init(int pixels, AdaptBufs b, AdaptEntry *entries) {
   b.exact_terms[p] = 0;
 }
 
-__global__ __launch_bounds__(256) void adapt_decide(int n, const AdaptEntry *entries, const SegDev *segs,
-                                                    const AdaptDev *ad, const int *pixel_begin, int nseg,
-                                                    AdaptBufs b, AdaptTerm *terms, unsigned long long *keys,
-                                                    AdaptEntry *bisected, int *query_pixel, Pose *query_pose,
-                                                    AdaptEntry *direct, int *terms_ctr, int *level_ctr) {
+__global__ __launch_bounds__(256) void adapt_decide(int bound, const int *prev_level, const AdaptEntry *entries,
+                                                    const SegDev *segs, const AdaptDev *ad, const int *pixel_begin,
+                                                    int nseg, AdaptBufs b, AdaptTerm *terms,
+                                                    unsigned long long *keys, AdaptEntry *bisected, int *query_pixel,
+                                                    Pose *query_pose, AdaptEntry *direct, int *head, int *level_ctr) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = prev_level ? min(bound, 2 * (prev_level[0] + prev_level[1])) : bound;
   int decision = -1;  // -1 dropped, ADAPT_ACCEPT, ADAPT_EXACT, 2 bisect
   AdaptEntry e{};
   int si = 0;
@@ -185,12 +186,14 @@ __global__ __launch_bounds__(256) void adapt_decide(int n, const AdaptEntry *ent
   const bool resolved = decision == ADAPT_ACCEPT || decision == ADAPT_EXACT;
   const bool ahead = e.lspec >= 0;  // the halves' midpoints are already traced
   const unsigned long long tmask = __ballot(resolved);
+  const unsigned long long emask = __ballot(decision == ADAPT_EXACT);
   const unsigned long long bmask = __ballot(decision == 2 && !ahead);
   const unsigned long long dmask = __ballot(decision == 2 && ahead);
   const int lane = threadIdx.x & 63;
   int tbase = 0, bbase = 0, dbase = 0;
   if (lane == 0) {
-    if (tmask) tbase = atomicAdd(terms_ctr, __popcll(tmask));
+    if (tmask) tbase = atomicAdd(&head[0], __popcll(tmask));
+    if (emask) atomicAdd(&head[1], __popcll(emask));
     if (bmask) bbase = atomicAdd(&level_ctr[0], __popcll(bmask));
     if (dmask) dbase = atomicAdd(&level_ctr[1], __popcll(dmask));
   }
@@ -203,10 +206,14 @@ __global__ __launch_bounds__(256) void adapt_decide(int n, const AdaptEntry *ent
     terms[idx] = AdaptTerm{e.a, e.b, e.mid, e.right, e.pixel, decision};
     // start fractions are dyadic with at most 21 bits below the point
     keys[idx] = ((unsigned long long)e.pixel << 24) | (unsigned long long)(e.a * 16777216.0);
-    if (decision == ADAPT_ACCEPT) atomicAdd(&b.bound[e.pixel], 1);
-    else atomicAdd(&b.exact_terms[e.pixel], 1);
+    // the bound grows by one from 3: it passes the capacity exactly once
+    if (decision == ADAPT_ACCEPT) {
+      if (atomicAdd(&b.bound[e.pixel], 1) == ad[si].capacity) atomicAdd(&head[2], 1);
+    } else {
+      atomicAdd(&b.exact_terms[e.pixel], 1);
+    }
   } else if (decision == 2) {
-    atomicAdd(&b.bound[e.pixel], 1);
+    if (atomicAdd(&b.bound[e.pixel], 1) == ad[si].capacity) atomicAdd(&head[2], 1);
     atomicAdd(&b.bisects[e.pixel], 1);
     const double mid = (e.a + e.b) * .5;
     if (ahead) {
@@ -656,13 +663,13 @@ hipError_t launch_adapt_init(int pixels, AdaptBufs b, AdaptEntry *entries, hipSt
   adapt_init<<<grid(pixels), 256, 0, st>>>(pixels, b, entries);
   return hipGetLastError();
 }
-hipError_t launch_adapt_decide(int n, const AdaptEntry *entries, const SegDev *segs, const AdaptDev *ad,
-                               const int *pixel_begin, int nseg, AdaptBufs b, AdaptTerm *terms,
+hipError_t launch_adapt_decide(int n, const int *prev_level, const AdaptEntry *entries, const SegDev *segs,
+                               const AdaptDev *ad, const int *pixel_begin, int nseg, AdaptBufs b, AdaptTerm *terms,
                                unsigned long long *keys, AdaptEntry *bisected, int *query_pixel, Pose *query_pose,
-                               AdaptEntry *direct, int *terms_ctr, int *level_ctr, hipStream_t st) {
+                               AdaptEntry *direct, int *head, int *level_ctr, hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  adapt_decide<<<grid(n), 256, 0, st>>>(n, entries, segs, ad, pixel_begin, nseg, b, terms, keys, bisected,
-                                         query_pixel, query_pose, direct, terms_ctr, level_ctr);
+  adapt_decide<<<grid(n), 256, 0, st>>>(n, prev_level, entries, segs, ad, pixel_begin, nseg, b, terms, keys,
+                                         bisected, query_pixel, query_pose, direct, head, level_ctr);
   return hipGetLastError();
 }
 hipError_t launch_adapt_children(int n, const AdaptEntry *bisected, const double *query_rad, AdaptEntry *next,

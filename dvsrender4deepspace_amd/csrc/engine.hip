@@ -836,27 +836,49 @@ class Engine {
     // midpoints of its halves' halves, so the next level's bisections need
     // no trace: the levels alternate between one ray launch and none, and
     // every probe value is the one a bisection would trace itself.
-    long long n = 2LL * P;
-    int cur = 0, terms_total = 0, levels = 0;
+    //
+    // The host reads the counters after each level but one kind: when every
+    // bisection of a level was traced, each entry of the next level carries
+    // its halves' midpoints (radiance is never negative), so that level
+    // traces nothing.  It is followed at once by the level after it, which
+    // takes its entry count from the device; the host checks the claim at
+    // the next read.
+    long long n = 2LL * P;  // entries of the level, or an upper bound on them when !counted
+    bool counted = true, untraced = false;
+    long long terms_bound = 0;  // resolved segments written so far, at most
+    int cur = 0, terms_total = 0, levels = 0, exact_terms = 0, over_capacity = 0;
     host_small_.reserve(kRefineCounterWords + 2, "refinement counters");
     int *host_ctr = host_small_.ptr;  // page-locked: the copy is queued, then the stream is polled
     while (n > 0) {
       if (++levels > kMaxRefineLevels) fail(DVS_E_STATE, "optical refinement did not terminate");
-      int *level_ctr = ctr_.ptr + 2 * levels;  // words 2 + 2 (levels - 1) and the next
-      terms_.grow_keep((size_t)terms_total + n, terms_total, "resolved segments", st);
-      tkeys_.grow_keep((size_t)terms_total + n, terms_total, "resolved segment keys", st);
+      int *level_ctr = ctr_.ptr + kRefineHeadWords + 2 * (levels - 1);
+      terms_.grow_keep((size_t)(terms_bound + n), (size_t)terms_bound, "resolved segments", st);
+      tkeys_.grow_keep((size_t)(terms_bound + n), (size_t)terms_bound, "resolved segment keys", st);
       bis_.reserve(n, "bisected segments");
       qpix_.reserve((size_t)dvsdev::kBisectQueries * n, "optical queries");
       qpose_.reserve((size_t)dvsdev::kBisectQueries * n, "optical queries");
       ent_[1 - cur].reserve(2 * n, "refinement segments");
-      hip_check(launch_adapt_decide((int)n, ent_[cur].ptr, segtab_.ptr, adapt_.ptr, pbegin_.ptr, nseg, b, terms_.ptr,
-                                    tkeys_.ptr, bis_.ptr, qpix_.ptr, qpose_.ptr, ent_[1 - cur].ptr, ctr_.ptr,
-                                    level_ctr, st),
+      hip_check(launch_adapt_decide((int)n, counted ? nullptr : level_ctr - 2, ent_[cur].ptr, segtab_.ptr,
+                                    adapt_.ptr, pbegin_.ptr, nseg, b, terms_.ptr, tkeys_.ptr, bis_.ptr, qpix_.ptr,
+                                    qpose_.ptr, ent_[1 - cur].ptr, ctr_.ptr, level_ctr, st),
                 "refine pixel optical launch");
-      const size_t words = 2 + 2 * (size_t)levels;
+      terms_bound += n;
+      if (untraced && counted) {
+        cur = 1 - cur;
+        n = 2 * n;
+        counted = untraced = false;
+        continue;
+      }
+      const size_t words = kRefineHeadWords + 2 * (size_t)levels;
       hip_check(hipMemcpyAsync(host_ctr, ctr_.ptr, sizeof(int) * words, hipMemcpyDeviceToHost, st), "refinement count");
       stream_wait(st, "refinement count");
       terms_total = host_ctr[0];
+      terms_bound = terms_total;
+      exact_terms = host_ctr[1];
+      over_capacity = host_ctr[2];
+      // a traced bisection at the level read without a wait: one of the
+      // level's midpoint radiances was not a number
+      if (!counted && host_ctr[words - 4] != 0) fail(DVS_E_OPTICAL, "invalid optical sample or BVH traversal overflow");
       const long long nb = host_ctr[words - 2], nd = host_ctr[words - 1];
       if (nb + nd == 0) break;
       if (nb > 0) {
@@ -868,6 +890,8 @@ class Engine {
         hip_check(launch_adapt_children((int)nb, bis_.ptr, qrad_.ptr, ent_[1 - cur].ptr + 2 * nd, st),
                   "refine pixel optical launch");
       }
+      untraced = nd == 0;  // every entry of the next level is a traced bisection's half
+      counted = true;
       cur = 1 - cur;
       n = 2 * (nb + nd);
     }
@@ -877,31 +901,41 @@ class Engine {
     hip_check(inclusive_scan_int(nullptr, temp_bytes, kcount_.ptr, koff_.ptr, P, st), "scan size");
     temp_.reserve(temp_bytes, "scan scratch");
     hip_check(inclusive_scan_int(temp_.ptr, temp_bytes, kcount_.ptr, koff_.ptr, P, st), "knot offsets");
-    hip_check(hipMemcpyAsync(host_ctr + kRefineCounterWords, koff_.ptr + P - 1, sizeof(int), hipMemcpyDeviceToHost,
-                             st),
-              "knot count");
-    stream_wait(st, "knot count");
-    const int knots_total = host_ctr[kRefineCounterWords];
-    knots_.reserve(std::max(knots_total, 1), "pixel optical knots");
+    // A knot pixel has 1 + 2 (accepted) + (exact) knots: P + 2 terms_total
+    // bounds the total without reading it back.
+    const size_t knots_bound = (size_t)P + 2 * (size_t)terms_total;
+    knots_.reserve(knots_bound, "pixel optical knots");
     b.knots = knots_.ptr;
     if (terms_total > 0) {
       kfill_.reserve(P, "knot order");
-      kbucket_.reserve(std::max(knots_total, 1), "knot order");
+      kbucket_.reserve(knots_bound, "knot order");
       hip_check(launch_fill_int(kfill_.ptr, 0, P, st), "knot order");
       hip_check(launch_adapt_knots(terms_total, tkeys_.ptr, terms_.ptr, b, kfill_.ptr, kbucket_.ptr, st),
                 "pixel optical knots");
     }
+    // Without exact segments there are no exact cells, and without them and
+    // without a pixel past its capacity no pixel falls back to native
+    // tracing: the exact-cell and fallback lists, and their read-backs, are
+    // skipped.
+    const bool any_exact = exact_terms > 0, any_fallback = any_exact || over_capacity > 0;
     // Local exact cells (identify_exact_cells / sample_exact_queries).
-    hip_check(launch_adapt_exact_count(P, segtab_.ptr, adapt_.ptr, pbegin_.ptr, nseg, b, st), "identify local exact optical cells");
-    temp_bytes = 0;
-    hip_check(inclusive_scan_int(nullptr, temp_bytes, ecount_.ptr, eoff_.ptr, P, st), "scan size");
-    temp_.reserve(temp_bytes, "scan scratch");
-    hip_check(inclusive_scan_int(temp_.ptr, temp_bytes, ecount_.ptr, eoff_.ptr, P, st), "local exact query count");
-    hip_check(hipMemcpyAsync(host_ctr + kRefineCounterWords + 1, eoff_.ptr + P - 1, sizeof(int),
-                             hipMemcpyDeviceToHost, st),
-              "local exact query count");
-    stream_wait(st, "local exact query count");
-    const int exact_total = host_ctr[kRefineCounterWords + 1];
+    int exact_total = 0;
+    if (any_exact) {
+      hip_check(launch_adapt_exact_count(P, segtab_.ptr, adapt_.ptr, pbegin_.ptr, nseg, b, st),
+                "identify local exact optical cells");
+      temp_bytes = 0;
+      hip_check(inclusive_scan_int(nullptr, temp_bytes, ecount_.ptr, eoff_.ptr, P, st), "scan size");
+      temp_.reserve(temp_bytes, "scan scratch");
+      hip_check(inclusive_scan_int(temp_.ptr, temp_bytes, ecount_.ptr, eoff_.ptr, P, st), "local exact query count");
+      hip_check(hipMemcpyAsync(host_ctr + kRefineCounterWords, eoff_.ptr + P - 1, sizeof(int), hipMemcpyDeviceToHost,
+                               st),
+                "local exact query count");
+      stream_wait(st, "local exact query count");
+      exact_total = host_ctr[kRefineCounterWords];
+    } else {
+      hip_check(launch_fill_int(ecount_.ptr, 0, P, st), "identify local exact optical cells");
+      hip_check(launch_fill_int(eoff_.ptr, 0, P, st), "identify local exact optical cells");
+    }
     ecell_.reserve(std::max(exact_total, 1), "local exact optical cells");
     erad_.reserve(std::max(exact_total, 1), "local exact radiance");
     eitem_pix_.reserve(std::max(exact_total, 1), "compact exact ray queue");
@@ -918,22 +952,28 @@ class Engine {
                 "trace local exact ray cells");
     }
     // Whole-pixel fallback list (compact_fallback, event_sensor.cu:116-118).
-    hip_check(launch_adapt_fallback_flags(P, flags_.ptr, fb_scan_.ptr, st), "compact exact fallback pixels");
-    temp_bytes = 0;
-    hip_check(inclusive_scan_int(nullptr, temp_bytes, fb_scan_.ptr, fb_scan_.ptr, P, st), "scan size");
-    temp_.reserve(temp_bytes, "scan scratch");
-    hip_check(inclusive_scan_int(temp_.ptr, temp_bytes, fb_scan_.ptr, fb_scan_.ptr, P, st), "compact exact fallback pixels");
-    fb_ends_.reserve(nseg, "fallback counts");
-    hip_check(launch_adapt_segment_ends(nseg, pbegin_.ptr, fb_scan_.ptr, fb_ends_.ptr, st), "fallback counts");
-    host_fb_ends_.reserve(nseg, "fallback counts");
-    hip_check(hipMemcpyAsync(host_fb_ends_.ptr, fb_ends_.ptr, sizeof(int) * nseg, hipMemcpyDeviceToHost, st),
-              "fallback counts");
-    stream_wait(st, "fallback counts");
-    const std::vector<int> fb_ends(host_fb_ends_.ptr, host_fb_ends_.ptr + nseg);
+    // fb_index is read for fallback pixels only.
+    std::vector<int> fb_ends(nseg, 0);
+    if (any_fallback) {
+      hip_check(launch_adapt_fallback_flags(P, flags_.ptr, fb_scan_.ptr, st), "compact exact fallback pixels");
+      temp_bytes = 0;
+      hip_check(inclusive_scan_int(nullptr, temp_bytes, fb_scan_.ptr, fb_scan_.ptr, P, st), "scan size");
+      temp_.reserve(temp_bytes, "scan scratch");
+      hip_check(inclusive_scan_int(temp_.ptr, temp_bytes, fb_scan_.ptr, fb_scan_.ptr, P, st),
+                "compact exact fallback pixels");
+      fb_ends_.reserve(nseg, "fallback counts");
+      hip_check(launch_adapt_segment_ends(nseg, pbegin_.ptr, fb_scan_.ptr, fb_ends_.ptr, st), "fallback counts");
+      host_fb_ends_.reserve(nseg, "fallback counts");
+      hip_check(hipMemcpyAsync(host_fb_ends_.ptr, fb_ends_.ptr, sizeof(int) * nseg, hipMemcpyDeviceToHost, st),
+                "fallback counts");
+      stream_wait(st, "fallback counts");
+      fb_ends.assign(host_fb_ends_.ptr, host_fb_ends_.ptr + nseg);
+    }
     const int fb_total = nseg ? fb_ends[nseg - 1] : 0;
     fb_list_.reserve(std::max(fb_total, 1), "exact fallback pixel IDs");
-    hip_check(launch_adapt_fallback_list(P, adapt_.ptr, pbegin_.ptr, nseg, fb_scan_.ptr, b, fb_list_.ptr, st),
-              "compact exact fallback pixels");
+    if (any_fallback)
+      hip_check(launch_adapt_fallback_list(P, adapt_.ptr, pbegin_.ptr, nseg, fb_scan_.ptr, b, fb_list_.ptr, st),
+                "compact exact fallback pixels");
     // Refinement statistics of every segment.
     seg_stats_.reserve(nseg, "refinement statistics");
     hip_check(launch_adapt_segment_stats(nseg, pbegin_.ptr, b, seg_stats_.ptr, st), "refinement statistics");
