@@ -59,54 +59,110 @@ inline void stream_wait(hipStream_t st, const char *what) {
   }
 }
 
+// Staged copies of one scatter launch: copy k moves bytes[k] from byte
+// offset src[k] of a staging mirror to dst[k].
+constexpr int kScatterCopies = 16;
+struct ScatterArgs {
+  unsigned long long src[kScatterCopies];
+  char *dst[kScatterCopies];
+  unsigned bytes[kScatterCopies];
+  int count;
+};
+// Queues the copies of `a` from device memory `mirror` on st (one block per
+// copy).
+hipError_t launch_staged_scatter(const char *mirror, const ScatterArgs &a, hipStream_t st);
+
 // Page-locked staging for the small host-to-device copies of one engine
 // step.  A copy from pageable memory blocks the host until the stream has
-// drained up to it; from page-locked memory it is queued and returns.  The
-// staging is a bump allocator: regions stay untouched until reset(), which
-// the engine calls when a step starts (the previous step's copies have
+// drained up to it; from page-locked memory it is queued and returns, but
+// each queued copy still costs the host ten to twenty microseconds.  So
+// upload() only records a copy: flush() moves every recorded region with one
+// host-to-device copy into a device mirror of the staging block and one
+// scatter kernel per kScatterCopies regions.  A kernel that reads an
+// uploaded array must be queued after the flush.
+//
+// The staging is a bump allocator: regions stay untouched until reset(),
+// which the engine calls when a step starts (the previous step's copies have
 // completed by then, its results having been read back).
 class Staging {
  public:
   Staging() = default;
   Staging(const Staging &) = delete;
   Staging &operator=(const Staging &) = delete;
-  ~Staging() {
-    for (auto &b : blocks_) (void)hipHostFree(b.first);
-  }
+  ~Staging() { free_blocks(); }
   void reset() {
     if (blocks_.size() > 1) {  // one block of the combined size from now on
       size_t total = 0;
-      for (auto &b : blocks_) {
-        total += b.second;
-        (void)hipHostFree(b.first);
-      }
-      blocks_.clear();
+      for (auto &b : blocks_) total += b.bytes;
+      free_blocks();
       grow(total);
     }
     used_ = 0;
+    pending_.clear();
   }
-  // Copies n elements to staging and queues their copy to dst on st.
+  // Copies n elements to staging and records their copy to dst.
   template <class T> void upload(T *dst, const T *src, size_t n, const char *what, hipStream_t st) {
     if (!n) return;
     const size_t bytes = n * sizeof(T), at = (used_ + 255) & ~size_t(255);
-    if (blocks_.empty() || at + bytes > blocks_.back().second) {
-      grow(std::max(bytes, blocks_.empty() ? size_t(1) << 20 : 2 * blocks_.back().second));
+    if (bytes > 0xFFFFFFFFu) throw std::runtime_error(std::string(what) + ": staged copy too large");
+    if (blocks_.empty() || at + bytes > blocks_.back().bytes) {
+      flush(st);  // the recorded regions live in the current block
+      grow(std::max(bytes, blocks_.empty() ? size_t(1) << 20 : 2 * blocks_.back().bytes));
       return upload(dst, src, n, what, st);
     }
-    char *p = blocks_.back().first + at;
-    std::memcpy(p, src, bytes);
+    std::memcpy(blocks_.back().host + at, src, bytes);
     used_ = at + bytes;
-    hip_check(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, st), what);
+    pending_.push_back(Region{at, reinterpret_cast<char *>(dst), bytes});
+  }
+  // Queues every recorded copy on st.
+  void flush(hipStream_t st) {
+    if (pending_.empty()) return;
+    const Block &b = blocks_.back();
+    const size_t lo = pending_.front().at, hi = pending_.back().at + pending_.back().bytes;
+    hip_check(hipMemcpyAsync(b.device + lo, b.host + lo, hi - lo, hipMemcpyHostToDevice, st), "staged upload");
+    for (size_t k = 0; k < pending_.size(); k += kScatterCopies) {
+      ScatterArgs a{};
+      a.count = (int)std::min<size_t>(kScatterCopies, pending_.size() - k);
+      for (int j = 0; j < a.count; ++j) {
+        a.src[j] = pending_[k + j].at;
+        a.dst[j] = pending_[k + j].dst;
+        a.bytes[j] = (unsigned)pending_[k + j].bytes;
+      }
+      hip_check(launch_staged_scatter(b.device, a, st), "staged upload");
+    }
+    pending_.clear();
   }
 
  private:
+  struct Block {
+    char *host, *device;
+    size_t bytes;
+  };
+  struct Region {
+    size_t at;
+    char *dst;
+    size_t bytes;
+  };
   void grow(size_t bytes) {
-    void *p = nullptr;
-    hip_check(hipHostMalloc(&p, bytes, hipHostMallocDefault), "page-locked staging");
-    blocks_.emplace_back(static_cast<char *>(p), bytes);
+    Block b{nullptr, nullptr, bytes};
+    hip_check(hipHostMalloc(reinterpret_cast<void **>(&b.host), bytes, hipHostMallocDefault), "page-locked staging");
+    const hipError_t e = hipMalloc(reinterpret_cast<void **>(&b.device), bytes);
+    if (e != hipSuccess) {
+      (void)hipHostFree(b.host);
+      hip_check(e, "staging mirror");
+    }
+    blocks_.push_back(b);
     used_ = 0;
   }
-  std::vector<std::pair<char *, size_t>> blocks_;
+  void free_blocks() {
+    for (auto &b : blocks_) {
+      (void)hipHostFree(b.host);
+      (void)hipFree(b.device);
+    }
+    blocks_.clear();
+  }
+  std::vector<Block> blocks_;
+  std::vector<Region> pending_;
   size_t used_ = 0;
 };
 
@@ -129,11 +185,11 @@ template <class T> struct DeviceArray {
     hip_check(hipMalloc(&ptr, std::max<size_t>(n, 1) * sizeof(T)), what);
     count = n;
   }
-  // reserve() with half again as much room when it has to grow, for sizes
-  // that drift from call to call (a free and an allocation cost a
-  // device-wide synchronisation each).
+  // reserve() with room to spare when it has to grow (at least half again
+  // as much as asked and twice the old size), for sizes that drift from call
+  // to call: a free and an allocation cost a device-wide synchronisation each.
   void reserve_ahead(size_t n, const char *what) {
-    if (n > count) reserve(n + n / 2, what);
+    if (n > count) reserve(std::max(n + n / 2, 2 * count), what);
   }
   void upload(const T *host, size_t n, const char *what, hipStream_t st = nullptr) {
     reserve(n, what);
@@ -143,9 +199,11 @@ template <class T> struct DeviceArray {
     reserve(n, what);
     staging.upload(ptr, host, n, what, st);
   }
-  // Grows to at least n elements, keeping the first `used` (stream ordered).
+  // Grows to at least n elements (with the room of reserve_ahead), keeping
+  // the first `used` (stream ordered).
   void grow_keep(size_t n, size_t used, const char *what, hipStream_t st) {
     if (n <= count) return;
+    n = std::max(n + n / 2, 2 * count);
     T *bigger = nullptr;
     hip_check(hipMalloc(&bigger, n * sizeof(T)), what);
     if (used) hip_check(hipMemcpyAsync(bigger, ptr, used * sizeof(T), hipMemcpyDeviceToDevice, st), what);
@@ -172,7 +230,7 @@ template <class T> struct PinnedArray {
     if (ptr) (void)hipHostFree(ptr);
     ptr = nullptr;
     count = 0;
-    const size_t want = std::max<size_t>(n + n / 2, 1);
+    const size_t want = std::max<size_t>(std::max(n + n / 2, 2 * count), 1);
     hip_check(hipHostMalloc(reinterpret_cast<void **>(&ptr), want * sizeof(T), hipHostMallocDefault), what);
     count = want;
   }

@@ -16,6 +16,7 @@
 //                      gather/pose/depth/permute for device event compaction.
 #include <hip/hip_runtime.h>
 
+#include "common.h"
 #include "device_circuit.hpp"
 #include "device_optics.hpp"
 #include "kernels.h"
@@ -353,6 +354,27 @@ __global__ void fill_int_kernel(int *p, int value, int n) {
   if (i < n) p[i] = value;
 }
 
+// Block k moves staged copy k: 16-byte words when source, destination and
+// size allow, else 4-byte words, else bytes.
+__global__ __launch_bounds__(256) void staged_scatter_kernel(const char *mirror, dvsamd::ScatterArgs a) {
+  const int k = blockIdx.x;
+  const char *src = mirror + a.src[k];
+  char *dst = a.dst[k];
+  const unsigned bytes = a.bytes[k];
+  const uintptr_t align = (uintptr_t)src | (uintptr_t)dst | (uintptr_t)bytes;
+  if (align % 16 == 0) {
+    const uint4 *s = reinterpret_cast<const uint4 *>(src);
+    uint4 *d = reinterpret_cast<uint4 *>(dst);
+    for (unsigned i = threadIdx.x; i < bytes / 16; i += blockDim.x) d[i] = s[i];
+  } else if (align % 4 == 0) {
+    const unsigned *s = reinterpret_cast<const unsigned *>(src);
+    unsigned *d = reinterpret_cast<unsigned *>(dst);
+    for (unsigned i = threadIdx.x; i < bytes / 4; i += blockDim.x) d[i] = s[i];
+  } else {
+    for (unsigned i = threadIdx.x; i < bytes; i += blockDim.x) dst[i] = src[i];
+  }
+}
+
 // ---- unit kernels -------------------------------------------------------
 __global__ void hapke_kernel(int64_t n, const float *ci, const float *ce, const float *cg, const float *alb,
                              dvs_hapke h, float *out) {
@@ -545,6 +567,12 @@ hipError_t launch_permute_multi(const SegDev *segs, int events, const unsigned *
 hipError_t launch_fill_int(int *p, int value, int n, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   fill_int_kernel<<<(n + 255) / 256, 256, 0, st>>>(p, value, n);
+  return hipGetLastError();
+}
+hipError_t launch_staged_scatter(const char *mirror, const ScatterArgs &a, hipStream_t st) {
+  if (a.count <= 0) return hipSuccess;
+  if (a.count > kScatterCopies) return hipErrorInvalidValue;
+  staged_scatter_kernel<<<a.count, 256, 0, st>>>(mirror, a);
   return hipGetLastError();
 }
 hipError_t launch_hapke(int64_t n, const float *ci, const float *ce, const float *cg, const float *alb,

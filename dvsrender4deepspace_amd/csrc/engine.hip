@@ -447,6 +447,7 @@ class Engine {
     cell_begin[nseg] = cell_total;
     cellbeg_.upload(cell_begin.data(), cell_begin.size(), "segment cells", st, staging_);
     charge_.upload(charge.data(), charge.size(), "segment charge", st, staging_);
+    staging_.flush(st);
     maps_.reserve(cell_total, "cell maps");
     hip_check(launch_cell_maps(cell_total, cell_t0_.ptr, cell_t1_.ptr, cellbeg_.ptr, nseg, charge_.ptr, maps_.ptr, st),
               "cell maps");
@@ -522,6 +523,7 @@ class Engine {
     }
     segtab_.upload(tables.data(), tables.size(), "segment table", st, staging_);
     tblock_.upload(tblocks.data(), tblocks.size(), "segment blocks", st, staging_);
+    staging_.flush(st);
     const size_t rare_native = cell_rates_flags(pixel_total, chunk);
     rare_native_.reserve(rare_native, "photocurrent flags");
     // A captured graph replays only while every buffer it reads stays put.
@@ -617,7 +619,7 @@ class Engine {
     hip_check(hipEventRecord(events_t.start, st), "record");
     size_t temp_bytes = 0;
     hip_check(inclusive_scan_int(nullptr, temp_bytes, counts_.ptr, ends_.ptr, pixel_total, st), "scan size");
-    temp_.reserve(temp_bytes, "scan scratch");
+    temp_.reserve_ahead(temp_bytes, "scan scratch");
     hip_check(inclusive_scan_int(temp_.ptr, temp_bytes, counts_.ptr, ends_.ptr, pixel_total, st), "event scan");
     int status = 0, total = 0;
     unsigned long long shadows = 0;
@@ -632,22 +634,26 @@ class Engine {
     if (status) fail(DVS_E_OPTICAL, "invalid optical sample or BVH traversal overflow");
     // read-backs land in page-locked memory: events, their segments (with
     // more than one segment), the status word and the shadow ray count
-    host_events_.reserve(std::max(total, 1), "events");
-    if (nseg > 1) host_event_seg_.reserve(std::max(total, 1), "event segments");
+    // Event buffers start with room for one event per pixel: the count
+    // grows over the first steps after initialisation (the sensor starts at
+    // equilibrium), and every growth is a reallocation.
+    const size_t event_room = std::max<size_t>((size_t)total, (size_t)pixel_total);
+    host_events_.reserve(event_room, "events");
+    if (nseg > 1) host_event_seg_.reserve(event_room, "event segments");
     const int samples = work[0].seg->scene->view.samples;
     if (total > 0) {
-      ev_time_.reserve_ahead(total, "event times");
-      ev_seg_.reserve_ahead(total, "event segments");
-      ev_pixel_.reserve_ahead(total, "event pixels");
-      ev_pol_.reserve_ahead(total, "event polarity");
-      ev_depth_.reserve_ahead(total, "event depth");
-      ev_poses_.reserve_ahead(total, "event poses");
-      keys_.reserve_ahead(total, "event keys");
-      keys2_.reserve_ahead(total, "event keys");
-      order_.reserve_ahead(total, "event order");
-      order2_.reserve_ahead(total, "event order");
-      events_out_.reserve_ahead(total, "events");
-      events_seg_.reserve_ahead(total, "event segments");
+      ev_time_.reserve_ahead(event_room, "event times");
+      ev_seg_.reserve_ahead(event_room, "event segments");
+      ev_pixel_.reserve_ahead(event_room, "event pixels");
+      ev_pol_.reserve_ahead(event_room, "event polarity");
+      ev_depth_.reserve_ahead(event_room, "event depth");
+      ev_poses_.reserve_ahead(event_room, "event poses");
+      keys_.reserve_ahead(event_room, "event keys");
+      keys2_.reserve_ahead(event_room, "event keys");
+      order_.reserve_ahead(event_room, "event order");
+      order2_.reserve_ahead(event_room, "event order");
+      events_out_.reserve_ahead(event_room, "events");
+      events_seg_.reserve_ahead(event_room, "event segments");
       const SegDev *tab = segtab_.ptr;  // chunk 0 table: ivs, scene, counts, slots are chunk independent
       hip_check(launch_gather_multi(tab, pbegin_.ptr, nseg, pixel_total, ends_.ptr, ev_time_.ptr, ev_seg_.ptr,
                                     ev_pixel_.ptr, ev_pol_.ptr, keys_.ptr, order_.ptr, st),
@@ -673,7 +679,7 @@ class Engine {
                                st),
                 "sort size");
       need = std::max(need, temp_bytes);
-      temp_.reserve(need, "sort scratch");
+      temp_.reserve_ahead(need, "sort scratch");
       temp_bytes = need;
       hip_check(sort_pairs_u64(temp_.ptr, temp_bytes, keys_.ptr, keys2_.ptr, order_.ptr, order2_.ptr, total, 0, 64, st),
                 "event sort");
@@ -780,6 +786,7 @@ class Engine {
     }
     segtab_.upload(tables.data(), tables.size(), "segment table", st, staging_);
     adapt_.upload(ad.data(), ad.size(), "adaptive windows", st, staging_);
+    staging_.flush(st);
     hip_check(launch_fill_int(counts_.ptr, 0, P, st), "clear events");
     hip_check(launch_cell_poses(cell_total, cell_t0_.ptr, cell_t1_.ptr, cell_iv_.ptr, intervals_.ptr, poses_.ptr, st),
               "cell poses");
@@ -883,7 +890,7 @@ class Engine {
       if (nb + nd == 0) break;
       if (nb > 0) {
         const long long queries = (long long)dvsdev::kBisectQueries * nb;
-        qrad_.reserve(queries, "optical queries");
+        qrad_.reserve_ahead(queries, "optical queries");
         hip_check(launch_adapt_trace(dvsdev::LAYOUT_QUERY, segtab_.ptr, pbegin_.ptr, nseg, samples, queries,
                                      qpix_.ptr, nullptr, qpose_.ptr, lds, qrad_.ptr, status_.ptr, counter_.ptr, st),
                   "refine pixel optical launch");
@@ -899,16 +906,16 @@ class Engine {
     hip_check(launch_adapt_counts(P, adapt_.ptr, pbegin_.ptr, nseg, b, st), "pixel optical knots");
     size_t temp_bytes = 0;
     hip_check(inclusive_scan_int(nullptr, temp_bytes, kcount_.ptr, koff_.ptr, P, st), "scan size");
-    temp_.reserve(temp_bytes, "scan scratch");
+    temp_.reserve_ahead(temp_bytes, "scan scratch");
     hip_check(inclusive_scan_int(temp_.ptr, temp_bytes, kcount_.ptr, koff_.ptr, P, st), "knot offsets");
     // A knot pixel has 1 + 2 (accepted) + (exact) knots: P + 2 terms_total
     // bounds the total without reading it back.
     const size_t knots_bound = (size_t)P + 2 * (size_t)terms_total;
-    knots_.reserve(knots_bound, "pixel optical knots");
+    knots_.reserve_ahead(knots_bound, "pixel optical knots");
     b.knots = knots_.ptr;
     if (terms_total > 0) {
       kfill_.reserve(P, "knot order");
-      kbucket_.reserve(knots_bound, "knot order");
+      kbucket_.reserve_ahead(knots_bound, "knot order");
       hip_check(launch_fill_int(kfill_.ptr, 0, P, st), "knot order");
       hip_check(launch_adapt_knots(terms_total, tkeys_.ptr, terms_.ptr, b, kfill_.ptr, kbucket_.ptr, st),
                 "pixel optical knots");
@@ -925,7 +932,7 @@ class Engine {
                 "identify local exact optical cells");
       temp_bytes = 0;
       hip_check(inclusive_scan_int(nullptr, temp_bytes, ecount_.ptr, eoff_.ptr, P, st), "scan size");
-      temp_.reserve(temp_bytes, "scan scratch");
+      temp_.reserve_ahead(temp_bytes, "scan scratch");
       hip_check(inclusive_scan_int(temp_.ptr, temp_bytes, ecount_.ptr, eoff_.ptr, P, st), "local exact query count");
       hip_check(hipMemcpyAsync(host_ctr + kRefineCounterWords, eoff_.ptr + P - 1, sizeof(int), hipMemcpyDeviceToHost,
                                st),
@@ -958,7 +965,7 @@ class Engine {
       hip_check(launch_adapt_fallback_flags(P, flags_.ptr, fb_scan_.ptr, st), "compact exact fallback pixels");
       temp_bytes = 0;
       hip_check(inclusive_scan_int(nullptr, temp_bytes, fb_scan_.ptr, fb_scan_.ptr, P, st), "scan size");
-      temp_.reserve(temp_bytes, "scan scratch");
+      temp_.reserve_ahead(temp_bytes, "scan scratch");
       hip_check(inclusive_scan_int(temp_.ptr, temp_bytes, fb_scan_.ptr, fb_scan_.ptr, P, st),
                 "compact exact fallback pixels");
       fb_ends_.reserve(nseg, "fallback counts");
@@ -1016,6 +1023,7 @@ class Engine {
       gate_tab_.upload(gtab.data(), gtab.size(), "mean receptor probes", st, staging_);
       gate_blocks_.upload(gblocks.data(), gblocks.size(), "mean receptor probes", st, staging_);
       gate_idx_.upload(gate_seg.data(), gate_seg.size(), "mean receptor probes", st, staging_);
+      staging_.flush(st);
       hip_check(launch_trace_cells_multi(gate_tab_.ptr, gate_blocks_.ptr, ngate, gblocks[ngate], lds, status_.ptr,
                                          counter_.ptr, st),
                 "mean receptor probe optics");
@@ -1048,7 +1056,10 @@ class Engine {
           changed = true;
         }
       }
-      if (changed) adapt_.upload(ad.data(), ad.size(), "adaptive windows", st, staging_);
+      if (changed) {
+        adapt_.upload(ad.data(), ad.size(), "adaptive windows", st, staging_);
+        staging_.flush(st);
+      }
     }
     hip_check(hipEventRecord(refine_t.stop, st), "record");
     // Integration, with the fallback pixels' native radiance traced in cell
@@ -1092,6 +1103,7 @@ class Engine {
         if (accepted[i]) fallback_samples += (uint64_t)count * work[i].cells;
       }
       adapt_.upload(ad.data(), ad.size(), "adaptive windows", st, staging_);
+      staging_.flush(st);
     }
     if (pipelined) {
       hip_check(hipEventRecord(ev_start_, st), "record");
@@ -1130,6 +1142,7 @@ class Engine {
         }
         fb_tab_.upload(ftab.data(), ftab.size(), "fallback optical tile", st, staging_);
         fb_blocks_.upload(fblocks.data(), fblocks.size(), "fallback optical tile", st, staging_);
+        staging_.flush(st);
         if (fblocks[nseg] > 0) {
           hip_check(launch_trace_cells_multi(fb_tab_.ptr, fb_blocks_.ptr, nseg, fblocks[nseg], lds, status_.ptr,
                                              counter_.ptr, st),

@@ -77,18 +77,41 @@ __host__ __device__ inline float solve_time(const Key &k0, const Key &k1, float 
   return 0.5f * (lo + hi);
 }
 
-__host__ __device__ inline float curve_value(const Key *k, int count, float t) {
-  if (count == 1 || t <= k[0].time) return k[0].value;
-  if (t >= k[count - 1].time) return k[count - 1].value;
+// Every curve of a trajectory has its keys at the same times with the same
+// time handles (t_i + h/3, t_{i+1} - h/3), so the segment and the curve
+// parameter at a time are those of every curve: they are found once per
+// time, on the key times of any one curve.
+//
+// Where a value comes from (curve value, trajectory.cpp:266-285): the first
+// or last key outside the key range, else segment `seg` at parameter u.
+struct ValueAt {
+  int where;  // -1 first key, 1 last key, 0 segment seg at u
+  int seg;
+  float u;
+};
+__host__ __device__ inline ValueAt value_at(const Key *k, int count, float t) {
+  if (count == 1 || t <= k[0].time) return ValueAt{-1, 0, 0.0f};
+  if (t >= k[count - 1].time) return ValueAt{1, 0, 0.0f};
   const int s = find_segment(k, count, t);
-  const float u = solve_time(k[s], k[s + 1], t);
-  return bezier(k[s].value, k[s].right_value, k[s + 1].left_value, k[s + 1].value, u);
+  return ValueAt{0, s, solve_time(k[s], k[s + 1], t)};
+}
+__host__ __device__ inline float curve_value(const Key *k, int count, const ValueAt &at) {
+  if (at.where < 0) return k[0].value;
+  if (at.where > 0) return k[count - 1].value;
+  const int s = at.seg;
+  return bezier(k[s].value, k[s].right_value, k[s + 1].left_value, k[s + 1].value, at.u);
 }
 
-// dy/dt; the segment comes from t kept 1e-7 inside the key range, the curve
-// parameter from t itself (derivative, trajectory.cpp:287-314).
-__host__ __device__ inline float curve_rate(const Key *k, int count, float t) {
-  if (count < 2) return 0.0f;
+// Where a rate comes from (derivative, trajectory.cpp:287-314): the segment
+// of t kept 1e-7 inside the key range, the curve parameter of t itself, and
+// dx/du there (zero rate when |dx/du| < 1e-12 or with fewer than two keys).
+struct RateAt {
+  int valid;
+  int seg;
+  float u, dx;
+};
+__host__ __device__ inline RateAt rate_at(const Key *k, int count, float t) {
+  if (count < 2) return RateAt{0, 0, 0.0f, 0.0f};
   const float lo = k[0].time, hi = k[count - 1].time, eps = 1e-7f;
   float tc = t;
   if (tc < lo + eps) tc = lo + eps;
@@ -96,8 +119,13 @@ __host__ __device__ inline float curve_rate(const Key *k, int count, float t) {
   const int s = find_segment(k, count, tc);
   const float u = solve_time(k[s], k[s + 1], t);
   const float dx = bezier_derivative(k[s].time, k[s].right_time, k[s + 1].left_time, k[s + 1].time, u);
-  const float dy = bezier_derivative(k[s].value, k[s].right_value, k[s + 1].left_value, k[s + 1].value, u);
-  return fabsf(dx) < 1e-12f ? 0.0f : dy / dx;
+  return RateAt{fabsf(dx) < 1e-12f ? 0 : 1, s, u, dx};
+}
+__host__ __device__ inline float curve_rate(const Key *k, const RateAt &at) {
+  if (!at.valid) return 0.0f;
+  const int s = at.seg;
+  const float dy = bezier_derivative(k[s].value, k[s].right_value, k[s + 1].left_value, k[s + 1].value, at.u);
+  return dy / at.dx;
 }
 
 // Quaternions as (x, y, z, w).
@@ -145,11 +173,12 @@ __host__ __device__ inline void qmatrix(const Quat &q, float m[9]) {
 // trajectory.cpp:474-560): alpha from the phase curve, q = q0 Exp(alpha rho)
 // with rho = Log(q0^-1 q1) on the short arc; the body angular velocity is
 // phase_rate * rho.
-__host__ __device__ inline Quat orientation(const View &v, float t, float rho_rate[3]) {
+__host__ __device__ inline Quat orientation(const View &v, const ValueAt &vat, const RateAt &rat,
+                                            float rho_rate[3]) {
   const Key *ph = v.keys + kPhase * v.count;
   rho_rate[0] = rho_rate[1] = rho_rate[2] = 0;
   if (v.count < 2) return Quat{0, 0, 0, 1};  // one key: no segment, the identity (as the reference)
-  const float phase = curve_value(ph, v.count, t), phase_rate = curve_rate(ph, v.count, t);
+  const float phase = curve_value(ph, v.count, vat), phase_rate = curve_rate(ph, rat);
   int seg = (int)floorf(phase);
   seg = seg < 0 ? 0 : seg > v.count - 2 ? v.count - 2 : seg;
   const float alpha = clampf(phase - (float)seg, 0.0f, 1.0f);
@@ -184,12 +213,14 @@ __host__ __device__ inline Sample evaluate(const View &v, float t, bool full) {
   s.timestamp = tc;
   const Key *k = v.keys;
   const int n = v.count;
+  const ValueAt vat = value_at(k, n, tc);
+  const RateAt rat = rate_at(k, n, tc);
   for (int c = 0; c < 3; ++c) {
-    s.position[c] = curve_value(k + (kPx + c) * n, n, tc);
-    s.velocity[c] = curve_rate(k + (kPx + c) * n, n, tc);
+    s.position[c] = curve_value(k + (kPx + c) * n, n, vat);
+    s.velocity[c] = curve_rate(k + (kPx + c) * n, rat);
   }
   float body[3];
-  s.rotation = orientation(v, tc, body);
+  s.rotation = orientation(v, vat, rat, body);
   if (full) {
     qmatrix(s.rotation, s.orientation);
     for (int r = 0; r < 3; ++r)
@@ -198,8 +229,8 @@ __host__ __device__ inline Sample evaluate(const View &v, float t, bool full) {
   }
   float l[3], ld[3];
   for (int c = 0; c < 3; ++c) {
-    l[c] = curve_value(k + (kLx + c) * n, n, tc);
-    ld[c] = curve_rate(k + (kLx + c) * n, n, tc);
+    l[c] = curve_value(k + (kLx + c) * n, n, vat);
+    ld[c] = curve_rate(k + (kLx + c) * n, rat);
   }
   const float norm = sqrtf(l[0] * l[0] + l[1] * l[1] + l[2] * l[2]);
   if (norm > 1e-12f) {
@@ -213,8 +244,8 @@ __host__ __device__ inline Sample evaluate(const View &v, float t, bool full) {
     s.light[0] = s.light[1] = 0; s.light[2] = 1;
     s.light_rate[0] = s.light_rate[1] = s.light_rate[2] = 0;
   }
-  s.intensity = curve_value(k + kIntensity * n, n, tc);
-  s.intensity_rate = curve_rate(k + kIntensity * n, n, tc);
+  s.intensity = curve_value(k + kIntensity * n, n, vat);
+  s.intensity_rate = curve_rate(k + kIntensity * n, rat);
   return s;
 }
 
