@@ -366,8 +366,8 @@ def request_mode(args) -> dict:
                      "traffic": None, "avg_launch_ms": round(st["circuit_ms"] / launches, 3),
                      "bytes_per_launch": int(bytes_integrate / launches),
                      "note": "one lane per pixel, cells in series (FP64 inverse receptor): latency bound at "
-                             f"{npix / 64:.0f} waves beside the next chunk's photocurrent kernel; photocurrent "
-                             "in is 8 B per pixel cell"},
+                             f"{npix / 64:.0f} waves, one launch per window after the window's photocurrent "
+                             "kernel; photocurrent in is 8 B per pixel cell"},
     }
     sensor.close()
 

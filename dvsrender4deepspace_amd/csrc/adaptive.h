@@ -114,10 +114,12 @@ hipError_t launch_adapt_probe_poses(int nseg, const SegDev *segs, const AdaptDev
                                     const Pose *prev_end, Pose *cur_end, int *reuse, hipStream_t st);
 hipError_t launch_adapt_cell_frac(int cells, const double *t0, const double *t1, const int *cell_begin, int nseg,
                                   const AdaptDev *ad, double *frac, hipStream_t st);
-// Radiance of `items` pixel samples: LAYOUT_PROBE item q is probe q%5 of
-// global pixel q/5 (pose probe_poses[segment*5 + q%5]); LAYOUT_QUERY item q
-// is item_pixel[q] at poses[q]; LAYOUT_CELL item q is item_pixel[q] at
-// poses[item_pose[q]].
+// Radiance of `items` pixel samples: LAYOUT_PROBE covers the five probes of
+// every pixel (items = 5 P; probe j of a pixel of segment s at pose
+// poses[s * 5 + j], written to out[pixel * 5 + j]; where reuse[s] is set,
+// probe 0 is the previous window's end probe, prev_probes[pixel * 5 + 4]);
+// LAYOUT_QUERY item q is item_pixel[q] at poses[q]; LAYOUT_CELL item q is
+// item_pixel[q] at poses[item_pose[q]].
 hipError_t launch_adapt_trace(int layout, const SegDev *segs, const int *pixel_begin, int nseg, int samples,
                               long long items, const int *item_pixel, const int *item_pose, const Pose *poses,
                               size_t lds, double *out, int *status, unsigned long long *shadow_count,

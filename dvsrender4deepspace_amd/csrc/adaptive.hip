@@ -67,9 +67,13 @@ __global__ void adapt_cell_frac(int cells, const double *t0, const double *t1, c
   frac[c] = ((t0[c] + t1[c]) * .5 - a.start) / (a.end - a.start);
 }
 
-// LAYOUT_PROBE items run probe-major (item q is probe q / P of pixel q % P,
-// written to out[pixel * 5 + probe]), so that reused start probes fill whole
-// blocks that trace nothing.
+// LAYOUT_PROBE: segment s owns items [5 b, 5 e) of its pixels [b, e), pixel
+// major, so that the rays of a wave are one pixel's probes at nearby times.
+// Without reuse, local item l is probe l % 5 of pixel b + l / 5.  With reuse
+// (the start pose equals the previous window's end pose), the first 4 (e - b)
+// items are probes 1..4 of each pixel (one wave per pixel at 4x4 sub-rays),
+// and the last e - b items only copy the previous end probe into probe 0, in
+// blocks that trace nothing.  Probe j of pixel p goes to out[p * 5 + j].
 template <int LAYOUT>
 __global__ __launch_bounds__(256) void adapt_trace(const SegDev *segs, const int *pixel_begin, int nseg,
                                                    long long items, const int *item_pixel, const int *item_pose,
@@ -90,13 +94,23 @@ __global__ __launch_bounds__(256) void adapt_trace(const SegDev *segs, const int
     int gp;
     Pose pose;
     if (LAYOUT == LAYOUT_PROBE) {
-      const long long P = items / 5;
-      const int j = (int)(q / P);
-      gp = (int)(q - (long long)j * P);
-      const int si = segment_of<int>(pixel_begin, nseg, gp);
+      const int si = segment_of<int>(pixel_begin, nseg, (int)(q / 5));
+      const int b = pixel_begin[si], n = pixel_begin[si + 1] - b;
+      const int l = (int)(q - 5LL * b);
+      int j;
+      if (!reuse[si]) {
+        gp = b + l / 5;
+        j = l % 5;
+      } else if (l < 4 * n) {
+        gp = b + l / 4;
+        j = 1 + l % 4;
+      } else {
+        gp = b + (l - 4 * n);
+        j = 0;
+        reused = true;
+      }
       pose = poses[si * 5 + j];
       at = (long long)gp * 5 + j;
-      reused = j == 0 && reuse[si];
     } else if (LAYOUT == LAYOUT_QUERY) {
       gp = item_pixel[q];
       pose = poses[q];
@@ -587,10 +601,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void a
   const AdaptDev &A = ad[si];
   const int i = ((int)blockIdx.x - block_begin[si]) * 256 + threadIdx.x;
   if (!A.accepted) return;  // the whole block: blocks do not straddle segments
-  // The serial circuit is latency bound at one or two waves per SIMD while
-  // the next chunk's photocurrent kernel fills the SIMDs beside it: its waves
-  // issue first.
-  __builtin_amdgcn_s_setprio(3);
   // lanes past the segment's pixels stay in the cell loop (they hold cells of
   // the wave's shared tiles) without a pixel of their own
   const bool own = i < g.npix;

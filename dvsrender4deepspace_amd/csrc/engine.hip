@@ -141,18 +141,10 @@ class Engine {
  public:
   explicit Engine(int horizons_per_step, bool use_graph) : horizons_(horizons_per_step), use_graph_(use_graph) {
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "engine stream");
-    // second stream (default priority) for the photocurrent of the next cell
-    // chunk while the serial circuit integrates the current one
-    hip_check(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking), "engine stream");
-    for (hipEvent_t *e : {&ev_start_, &ev_rates_[0], &ev_rates_[1], &ev_int_[0], &ev_int_[1]})
-      hip_check(hipEventCreateWithFlags(e, hipEventDisableTiming), "engine events");
   }
   ~Engine() {
     segments_.clear();
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
-    for (hipEvent_t e : {ev_start_, ev_rates_[0], ev_rates_[1], ev_int_[0], ev_int_[1]})
-      if (e) (void)hipEventDestroy(e);
-    if (side_) (void)hipStreamDestroy(side_);
     if (stream_) (void)hipStreamDestroy(stream_);
   }
 
@@ -1068,20 +1060,20 @@ class Engine {
     std::vector<std::unique_ptr<Timer>> integrate_t;
     int max_cells = 0;
     for (auto &w : work) max_cells = std::max(max_cells, w.cells);
-    // Photocurrent of every pixel for a chunk of cells: knot pixels in
-    // rates_ [cell][P], fallback pixels in radiance_ [cell][fallback list].
-    // Without fallback pixels the photocurrent of chunk k+1 is computed on
-    // the side stream while the circuit integrates chunk k (two rate slots).
-    const bool pipelined = fb_total == 0;
-    const size_t slots_needed = pipelined ? 2 : 1;
+    // Photocurrent of every pixel for a chunk of cells (knot pixels in
+    // rates_ [cell][P], fallback pixels in radiance_ [cell][fallback list]),
+    // then the circuit over the chunk, all on one stream.  The chunk is the
+    // whole window unless the photocurrent would pass the radiance budget.
+    // (Computing the next chunk's photocurrent on a second stream beside the
+    // circuit gains nothing: the circuit is slowed by as much as the
+    // photocurrent would take alone, and each cross-stream wait costs about
+    // ten microseconds.)
     size_t chunk_cap = std::min<size_t>((size_t)max_cells, 65535);
-    chunk_cap = std::min<size_t>(chunk_cap, radiance_budget_ / (sizeof(double) * slots_needed *
-                                                                  ((size_t)P + (size_t)fb_total)));
-    if (pipelined && max_cells >= 256) chunk_cap = std::min<size_t>(chunk_cap, (size_t)(max_cells + 3) / 4);
+    chunk_cap = std::min<size_t>(chunk_cap, radiance_budget_ / (sizeof(double) * ((size_t)P + (size_t)fb_total)));
     const int chunk = (int)std::max<size_t>(1, chunk_cap);
-    rates_.reserve((size_t)chunk * P * slots_needed, "adaptive photocurrent");
+    rates_.reserve((size_t)chunk * P, "adaptive photocurrent");
     b.rates = rates_.ptr;
-    // one flag buffer per stream that runs photocurrent kernels
+    // per-wave flags of the knot pixels' and the fallback pixels' photocurrent
     const size_t rare_knots = adapt_rates_flags(P, chunk), rare_fallback = cell_rates_flags(fb_total, chunk);
     rare_adapt_.reserve(rare_knots, "photocurrent flags");
     rare_native_.reserve(rare_fallback, "photocurrent flags");
@@ -1105,30 +1097,7 @@ class Engine {
       adapt_.upload(ad.data(), ad.size(), "adaptive windows", st, staging_);
       staging_.flush(st);
     }
-    if (pipelined) {
-      hip_check(hipEventRecord(ev_start_, st), "record");
-      int k = 0;
-      for (int c0 = 0; c0 < max_cells; c0 += chunk, ++k) {
-        const int c1 = std::min(max_cells, c0 + chunk), slot = k & 1;
-        AdaptBufs bk = b;
-        bk.rates = rates_.ptr + (size_t)slot * chunk * P;
-        if (k == 0) hip_check(hipStreamWaitEvent(side_, ev_start_, 0), "stream wait");
-        if (k >= 2) hip_check(hipStreamWaitEvent(side_, ev_int_[slot], 0), "stream wait");  // slot free again
-        hip_check(launch_adapt_rates(segtab_.ptr, adapt_.ptr, pbegin_.ptr, nseg, P, bk, c0, c1, status_.ptr,
-                                     rare_adapt_.ptr, rare_knots, side_),
-                  "adaptive photocurrent launch");
-        hip_check(hipEventRecord(ev_rates_[slot], side_), "record");
-        hip_check(hipStreamWaitEvent(st, ev_rates_[slot], 0), "stream wait");
-        integrate_t.emplace_back(new Timer());
-        hip_check(hipEventRecord(integrate_t.back()->start, st), "record");
-        hip_check(launch_adapt_integrate(segtab_.ptr, adapt_.ptr, cblock_.ptr, nseg, circuit_blocks_, P, bk, c0, c1,
-                                         status_.ptr, st),
-                  "adaptive circuit launch");
-        hip_check(hipEventRecord(integrate_t.back()->stop, st), "record");
-        hip_check(hipEventRecord(ev_int_[slot], st), "record");
-      }
-    }
-    for (int c0 = 0; !pipelined && c0 < max_cells; c0 += chunk) {
+    for (int c0 = 0; c0 < max_cells; c0 += chunk) {
       const int c1 = std::min(max_cells, c0 + chunk);
       if (fb_total > 0) {
         fblocks[0] = 0;
@@ -1196,8 +1165,7 @@ class Engine {
 
   int horizons_;
   bool use_graph_;
-  hipStream_t stream_ = nullptr, side_ = nullptr;
-  hipEvent_t ev_start_ = nullptr, ev_rates_[2] = {nullptr, nullptr}, ev_int_[2] = {nullptr, nullptr};
+  hipStream_t stream_ = nullptr;
   std::map<int, std::unique_ptr<Segment>> segments_;
   int next_id_ = 0;
   size_t radiance_budget_ = size_t(4) << 30;
