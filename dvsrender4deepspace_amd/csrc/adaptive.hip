@@ -457,12 +457,14 @@ __global__ void adapt_gate(const SegDev *gate_segs, const int *gate_seg, const A
   const int size = b.kcount[gp];
   const double2 *v = b.knots + (b.koff[gp] - size);
   const double *frac = b.cell_frac + A.cell_offset;
+  // cell fractions ascend, so each cell's bracket walk starts at the last one
+  int a = 0;
   for (int c = 0; c < G.cells; ++c) {
     const double light = G.rad[(size_t)c * G.npix + i];
     double cached = light;
     if (knots) {
       const double f = frac[c];
-      const int a = knot_bracket(v, size, f, 0);
+      a = knot_bracket(v, size, f, a);
       const double w = (f - fabs(v[a].x)) / (fabs(v[a + 1].x) - fabs(v[a].x));
       if (v[a + 1].x >= 0) cached = v[a].y + (v[a + 1].y - v[a].y) * w;
     }
