@@ -457,20 +457,62 @@ __global__ void adapt_gate(const SegDev *gate_segs, const int *gate_seg, const A
   const int size = b.kcount[gp];
   const double2 *v = b.knots + (b.koff[gp] - size);
   const double *frac = b.cell_frac + A.cell_offset;
-  // cell fractions ascend, so each cell's bracket walk starts at the last one
+  const int cells = G.cells;
+  if (cells <= 0) {
+    errors[g * 1024 + i] = 0.0;
+    return;
+  }
+  // Each lane's two receptor chains are serial over the cells: the inputs of
+  // the next group of cells are loaded while this group computes, and the
+  // knot pair around the cell stays in registers (cell fractions ascend, so
+  // the bracket only walks forward).
+  constexpr int kGroup = 4;
+  auto at = [&](int c) { return min(c, cells - 1); };
+  double light_n[kGroup], f_n[kGroup], dt_n[kGroup];
+#pragma unroll
+  for (int j = 0; j < kGroup; ++j) {
+    light_n[j] = G.rad[(size_t)at(j) * G.npix + i];
+    f_n[j] = frac[at(j)];
+    dt_n[j] = G.t1[at(j)] - G.t0[at(j)];
+  }
   int a = 0;
-  for (int c = 0; c < G.cells; ++c) {
-    const double light = G.rad[(size_t)c * G.npix + i];
-    double cached = light;
-    if (knots) {
-      const double f = frac[c];
-      a = knot_bracket(v, size, f, a);
-      const double w = (f - fabs(v[a].x)) / (fabs(v[a + 1].x) - fabs(v[a].x));
-      if (v[a + 1].x >= 0) cached = v[a].y + (v[a + 1].y - v[a].y) * w;
+  double2 va = make_double2(0.0, 0.0), vb = va;
+  if (knots && size >= 2) {
+    va = v[0];
+    vb = v[1];
+  }
+  for (int c0 = 0; c0 < cells; c0 += kGroup) {
+    double light[kGroup], f[kGroup], dt[kGroup];
+#pragma unroll
+    for (int j = 0; j < kGroup; ++j) {
+      light[j] = light_n[j];
+      f[j] = f_n[j];
+      dt[j] = dt_n[j];
     }
-    const double dt = G.t1[c] - G.t0[c];
-    ex = receptor_log_at(ex, light * ph.electrons_per_radiance + ph.dark_electrons, dt, charge);
-    ca = receptor_log_at(ca, fmax(0.0, cached) * ph.electrons_per_radiance + ph.dark_electrons, dt, charge);
+#pragma unroll
+    for (int j = 0; j < kGroup; ++j) {
+      const int c = at(c0 + kGroup + j);
+      light_n[j] = G.rad[(size_t)c * G.npix + i];
+      f_n[j] = frac[c];
+      dt_n[j] = G.t1[c] - G.t0[c];
+    }
+#pragma unroll
+    for (int j = 0; j < kGroup; ++j) {
+      if (c0 + j < cells) {
+        double cached = light[j];
+        if (knots) {
+          while (a + 2 < size && f[j] > fabs(vb.x)) {  // knot_bracket
+            ++a;
+            va = vb;
+            vb = v[a + 1];
+          }
+          const double w = (f[j] - fabs(va.x)) / (fabs(vb.x) - fabs(va.x));
+          if (vb.x >= 0) cached = va.y + (vb.y - va.y) * w;
+        }
+        ex = receptor_log_at(ex, light[j] * ph.electrons_per_radiance + ph.dark_electrons, dt[j], charge);
+        ca = receptor_log_at(ca, fmax(0.0, cached) * ph.electrons_per_radiance + ph.dark_electrons, dt[j], charge);
+      }
+    }
   }
   errors[g * 1024 + i] = fabs(ex - ca);
 }

@@ -660,15 +660,18 @@ class Engine {
         hip_check(launch_fill_int(reinterpret_cast<int *>(ev_depth_.ptr), (int)0xBF800000u, total, st), "event depth");
       }
       // Stable sort by time, then stable by segment: (segment, t, y, x),
-      // since compaction order is (segment, y, x).
+      // since compaction order is (segment, y, x).  The scratch is sized for
+      // event_room items, so that it does not grow with every step.
+      const int sort_room = (int)std::min<size_t>(event_room, (size_t)INT_MAX);
       temp_bytes = 0;
-      hip_check(sort_pairs_u64(nullptr, temp_bytes, keys_.ptr, keys2_.ptr, order_.ptr, order2_.ptr, total, 0, 64, st),
+      hip_check(sort_pairs_u64(nullptr, temp_bytes, keys_.ptr, keys2_.ptr, order_.ptr, order2_.ptr, sort_room, 0, 64,
+                               st),
                 "sort size");
       size_t need = temp_bytes;
       int seg_bits = 1;
       while ((1 << seg_bits) < nseg) ++seg_bits;
-      hip_check(sort_pairs_u64(nullptr, temp_bytes, keys_.ptr, keys2_.ptr, order_.ptr, order2_.ptr, total, 0, seg_bits,
-                               st),
+      hip_check(sort_pairs_u64(nullptr, temp_bytes, keys_.ptr, keys2_.ptr, order_.ptr, order2_.ptr, sort_room, 0,
+                               seg_bits, st),
                 "sort size");
       need = std::max(need, temp_bytes);
       temp_.reserve_ahead(need, "sort scratch");
