@@ -124,7 +124,9 @@ hipError_t launch_adapt_trace(int layout, const SegDev *segs, const int *pixel_b
                               long long items, const int *item_pixel, const int *item_pose, const Pose *poses,
                               size_t lds, double *out, int *status, unsigned long long *shadow_count,
                               hipStream_t st, const int *reuse = nullptr, const double *prev_probes = nullptr);
-hipError_t launch_adapt_init(int pixels, AdaptBufs b, AdaptEntry *entries, hipStream_t st);
+// Also clears clear[0, clear_words) and event_counts[0, pixels).
+hipError_t launch_adapt_init(int pixels, AdaptBufs b, AdaptEntry *entries, int *clear, int clear_words,
+                             int *event_counts, hipStream_t st);
 // One level: resolved segments to terms; a bisected segment whose halves'
 // midpoints were traced ahead writes its halves to `direct` (level_ctr[1]
 // pairs), any other to `bisected` with kBisectQueries queries each
@@ -138,8 +140,10 @@ hipError_t launch_adapt_decide(int n, const int *prev_level, const AdaptEntry *e
 // The halves of the traced bisections, with their halves' midpoints.
 hipError_t launch_adapt_children(int n, const AdaptEntry *bisected, const double *query_rad, AdaptEntry *next,
                                  hipStream_t st);
+// Knot counts and fallback flags; also clears fill (P words), the exact
+// counts when clear_exact, and the nseg statistics records.
 hipError_t launch_adapt_counts(int pixels, const AdaptDev *ad, const int *pixel_begin, int nseg, AdaptBufs b,
-                               hipStream_t st);
+                               int *fill, bool clear_exact, AdaptSegStats *stats, hipStream_t st);
 // Knots of every knot pixel in time order from the resolved segments
 // (terms, keys as written by adapt_decide); fill holds P zeroed counters and
 // bucket at least as many words as knots.
@@ -154,6 +158,7 @@ hipError_t launch_adapt_fallback_list(int pixels, const AdaptDev *ad, const int 
                                       const int *scan, AdaptBufs b, int *list, hipStream_t st);
 // out[s] = scan[pixel_begin[s + 1] - 1]: inclusive prefix at each segment's end.
 hipError_t launch_adapt_segment_ends(int nseg, const int *pixel_begin, const int *scan, int *out, hipStream_t st);
+// Adds each segment's statistics into out (cleared by adapt_counts).
 hipError_t launch_adapt_segment_stats(int nseg, const int *pixel_begin, AdaptBufs b, AdaptSegStats *out,
                                       hipStream_t st);
 // Convergence gate: gate g is segment gate_seg[g] with its probe pixels as

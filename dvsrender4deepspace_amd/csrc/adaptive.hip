@@ -147,9 +147,12 @@ __global__ __launch_bounds__(256) void adapt_trace(const SegDev *segs, const int
 
 // Two root segments per pixel: [0, 1/2] is popped first with one pending
 // sibling on the depth-first stack, [1/2, 1] with none (refine_pixels, :46-49).
-__global__ void adapt_init(int pixels, AdaptBufs b, AdaptEntry *entries) {
+__global__ void adapt_init(int pixels, AdaptBufs b, AdaptEntry *entries, int *clear, int clear_words,
+                           int *event_counts) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < clear_words) clear[p] = 0;
   if (p >= pixels) return;
+  event_counts[p] = 0;
   const double *r = b.probes + (size_t)p * 5;
   entries[2 * p] = AdaptEntry{0.0, .5, r[0], r[1], r[2], -1.0, -1.0, p, 1};
   entries[2 * p + 1] = AdaptEntry{.5, 1.0, r[2], r[3], r[4], -1.0, -1.0, p, 0};
@@ -265,9 +268,16 @@ __global__ void adapt_children(int n, const AdaptEntry *bisected, const double *
   next[2 * k + 1] = AdaptEntry{mid, e.b, e.mid, q[1], e.right, q[4], q[5], e.pixel, e.zeros};
 }
 
-__global__ void adapt_counts(int pixels, const AdaptDev *ad, const int *pixel_begin, int nseg, AdaptBufs b) {
+__global__ void adapt_counts(int pixels, const AdaptDev *ad, const int *pixel_begin, int nseg, AdaptBufs b,
+                             int *fill, int clear_exact, int *stats, int stat_words) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < stat_words) stats[p] = 0;
   if (p >= pixels) return;
+  fill[p] = 0;
+  if (clear_exact) {
+    b.ecount[p] = 0;
+    b.eoff[p] = 0;
+  }
   const AdaptDev &A = ad[segment_of<int>(pixel_begin, nseg, p)];
   int f = b.flags[p];
   if (b.bound[p] > A.capacity) f |= 1;
@@ -712,9 +722,11 @@ hipError_t launch_adapt_trace(int layout, const SegDev *segs, const int *pixel_b
   }
   return hipGetLastError();
 }
-hipError_t launch_adapt_init(int pixels, AdaptBufs b, AdaptEntry *entries, hipStream_t st) {
-  if (pixels <= 0) return hipSuccess;
-  adapt_init<<<grid(pixels), 256, 0, st>>>(pixels, b, entries);
+hipError_t launch_adapt_init(int pixels, AdaptBufs b, AdaptEntry *entries, int *clear, int clear_words,
+                             int *event_counts, hipStream_t st) {
+  const int n = max(pixels, clear_words);
+  if (n <= 0) return hipSuccess;
+  adapt_init<<<grid(n), 256, 0, st>>>(pixels, b, entries, clear, clear_words, event_counts);
   return hipGetLastError();
 }
 hipError_t launch_adapt_decide(int n, const int *prev_level, const AdaptEntry *entries, const SegDev *segs,
@@ -733,9 +745,13 @@ hipError_t launch_adapt_children(int n, const AdaptEntry *bisected, const double
   return hipGetLastError();
 }
 hipError_t launch_adapt_counts(int pixels, const AdaptDev *ad, const int *pixel_begin, int nseg, AdaptBufs b,
-                               hipStream_t st) {
-  if (pixels <= 0) return hipSuccess;
-  adapt_counts<<<grid(pixels), 256, 0, st>>>(pixels, ad, pixel_begin, nseg, b);
+                               int *fill, bool clear_exact, AdaptSegStats *stats, hipStream_t st) {
+  static_assert(sizeof(AdaptSegStats) % sizeof(int) == 0, "statistics are cleared word by word");
+  const int stat_words = nseg * (int)(sizeof(AdaptSegStats) / sizeof(int));
+  const int n = max(pixels, stat_words);
+  if (n <= 0) return hipSuccess;
+  adapt_counts<<<grid(n), 256, 0, st>>>(pixels, ad, pixel_begin, nseg, b, fill, clear_exact ? 1 : 0,
+                                         reinterpret_cast<int *>(stats), stat_words);
   return hipGetLastError();
 }
 hipError_t launch_adapt_knots(int n, const unsigned long long *keys, const AdaptTerm *terms, AdaptBufs b, int *fill,
@@ -776,9 +792,6 @@ hipError_t launch_adapt_segment_ends(int nseg, const int *pixel_begin, const int
 hipError_t launch_adapt_segment_stats(int nseg, const int *pixel_begin, AdaptBufs b, AdaptSegStats *out,
                                       hipStream_t st) {
   if (nseg <= 0) return hipSuccess;
-  static_assert(sizeof(AdaptSegStats) % sizeof(int) == 0, "statistics are cleared word by word");
-  const hipError_t e = launch_fill_int(reinterpret_cast<int *>(out), 0, nseg * (int)(sizeof(AdaptSegStats) / 4), st);
-  if (e != hipSuccess) return e;
   adapt_segment_stats<<<dim3(kStatBlocks, nseg), 256, 0, st>>>(pixel_begin, b, out);
   return hipGetLastError();
 }

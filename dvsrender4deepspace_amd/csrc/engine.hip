@@ -413,8 +413,10 @@ class Engine {
     ends_.reserve(pixel_total, "event offsets");
     status_.reserve(1, "error flag");
     counter_.reserve(1, "ray counter");
-    hip_check(launch_fill_int(status_.ptr, 0, 1, st), "clear status");
-    hip_check(launch_fill_int(reinterpret_cast<int *>(counter_.ptr), 0, 2, st), "clear ray counter");
+    // the status word and the ray counter start at zero: staged with the tables
+    static const int kZeros[2] = {0, 0};
+    staging_.upload(status_.ptr, kZeros, 1, "clear status", st);
+    staging_.upload(reinterpret_cast<int *>(counter_.ptr), kZeros, 2, "clear ray counter", st);
     const int nseg = (int)work.size();
     std::vector<int> cblocks(nseg + 1), pbegin(nseg + 1);
     cblocks[0] = 0;
@@ -782,7 +784,6 @@ class Engine {
     segtab_.upload(tables.data(), tables.size(), "segment table", st, staging_);
     adapt_.upload(ad.data(), ad.size(), "adaptive windows", st, staging_);
     staging_.flush(st);
-    hip_check(launch_fill_int(counts_.ptr, 0, P, st), "clear events");
     hip_check(launch_cell_poses(cell_total, cell_t0_.ptr, cell_t1_.ptr, cell_iv_.ptr, intervals_.ptr, poses_.ptr, st),
               "cell poses");
     cell_frac_.reserve(cell_total, "cell fractions");
@@ -831,9 +832,10 @@ class Engine {
     b.fb_index = fb_index_.ptr;
     b.cell_frac = cell_frac_.ptr;
     ent_[0].reserve((size_t)2 * P, "refinement segments");
-    hip_check(launch_adapt_init(P, b, ent_[0].ptr, st), "refine pixel optical launch");
     ctr_.reserve(kRefineCounterWords, "refinement counters");
-    hip_check(launch_fill_int(ctr_.ptr, 0, kRefineCounterWords, st), "clear refinement counters");
+    // also clears the refinement counters and the event counts
+    hip_check(launch_adapt_init(P, b, ent_[0].ptr, ctr_.ptr, kRefineCounterWords, counts_.ptr, st),
+              "refine pixel optical launch");
     // Level-synchronous refinement.  A traced bisection also traces the
     // midpoints of its halves' halves, so the next level's bisections need
     // no trace: the levels alternate between one ray launch and none, and
@@ -897,8 +899,18 @@ class Engine {
       cur = 1 - cur;
       n = 2 * (nb + nd);
     }
-    // Knot counts, offsets and depth-first knot order.
-    hip_check(launch_adapt_counts(P, adapt_.ptr, pbegin_.ptr, nseg, b, st), "pixel optical knots");
+    // Without exact segments there are no exact cells, and without them and
+    // without a pixel past its capacity no pixel falls back to native
+    // tracing: the exact-cell and fallback lists, and their read-backs, are
+    // skipped.
+    const bool any_exact = exact_terms > 0, any_fallback = any_exact || over_capacity > 0;
+    // Knot counts, offsets and depth-first knot order; adapt_counts also
+    // clears the knot fills, the statistics and (without exact segments)
+    // the exact counts.
+    kfill_.reserve(P, "knot order");
+    seg_stats_.reserve(nseg, "refinement statistics");
+    hip_check(launch_adapt_counts(P, adapt_.ptr, pbegin_.ptr, nseg, b, kfill_.ptr, !any_exact, seg_stats_.ptr, st),
+              "pixel optical knots");
     size_t temp_bytes = 0;
     hip_check(inclusive_scan_int(nullptr, temp_bytes, kcount_.ptr, koff_.ptr, P, st), "scan size");
     temp_.reserve_ahead(temp_bytes, "scan scratch");
@@ -909,17 +921,10 @@ class Engine {
     knots_.reserve_ahead(knots_bound, "pixel optical knots");
     b.knots = knots_.ptr;
     if (terms_total > 0) {
-      kfill_.reserve(P, "knot order");
       kbucket_.reserve_ahead(knots_bound, "knot order");
-      hip_check(launch_fill_int(kfill_.ptr, 0, P, st), "knot order");
       hip_check(launch_adapt_knots(terms_total, tkeys_.ptr, terms_.ptr, b, kfill_.ptr, kbucket_.ptr, st),
                 "pixel optical knots");
     }
-    // Without exact segments there are no exact cells, and without them and
-    // without a pixel past its capacity no pixel falls back to native
-    // tracing: the exact-cell and fallback lists, and their read-backs, are
-    // skipped.
-    const bool any_exact = exact_terms > 0, any_fallback = any_exact || over_capacity > 0;
     // Local exact cells (identify_exact_cells / sample_exact_queries).
     int exact_total = 0;
     if (any_exact) {
@@ -934,9 +939,6 @@ class Engine {
                 "local exact query count");
       stream_wait(st, "local exact query count");
       exact_total = host_ctr[kRefineCounterWords];
-    } else {
-      hip_check(launch_fill_int(ecount_.ptr, 0, P, st), "identify local exact optical cells");
-      hip_check(launch_fill_int(eoff_.ptr, 0, P, st), "identify local exact optical cells");
     }
     ecell_.reserve(std::max(exact_total, 1), "local exact optical cells");
     erad_.reserve(std::max(exact_total, 1), "local exact radiance");
@@ -977,7 +979,6 @@ class Engine {
       hip_check(launch_adapt_fallback_list(P, adapt_.ptr, pbegin_.ptr, nseg, fb_scan_.ptr, b, fb_list_.ptr, st),
                 "compact exact fallback pixels");
     // Refinement statistics of every segment.
-    seg_stats_.reserve(nseg, "refinement statistics");
     hip_check(launch_adapt_segment_stats(nseg, pbegin_.ptr, b, seg_stats_.ptr, st), "refinement statistics");
     host_seg_stats_.reserve(nseg, "refinement statistics");
     const AdaptSegStats *seg_stats = host_seg_stats_.ptr;  // read after the step's final wait
