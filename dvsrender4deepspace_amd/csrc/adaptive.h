@@ -162,9 +162,10 @@ hipError_t launch_adapt_segment_ends(int nseg, const int *pixel_begin, const int
 hipError_t launch_adapt_segment_stats(int nseg, const int *pixel_begin, AdaptBufs b, AdaptSegStats *out,
                                       hipStream_t st);
 // Convergence gate: gate g is segment gate_seg[g] with its probe pixels as
-// gate_segs[g].pixel_list and their native cell radiance in gate_segs[g].rad.
-hipError_t launch_adapt_gate(const SegDev *gate_segs, const int *gate_seg, int ngate, const AdaptDev *ad,
-                             AdaptBufs b, double *errors, hipStream_t st);
+// gate_segs[g].pixel_list and their native cell radiance in gate_segs[g].rad;
+// the error of list entry i goes to errors[g * 1024 + slots[g * 1024 + i]].
+hipError_t launch_adapt_gate(const SegDev *gate_segs, const int *gate_seg, const int *slots, int ngate,
+                             const AdaptDev *ad, AdaptBufs b, double *errors, hipStream_t st);
 // Photocurrent of cells [c0, c1) of the knot pixels; `rare` holds
 // adapt_rates_flags(pixels, c1 - c0) per-wave flags of the second phase.
 size_t adapt_rates_flags(int pixels, int cells);

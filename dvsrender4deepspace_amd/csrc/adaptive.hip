@@ -453,29 +453,35 @@ __global__ __launch_bounds__(256) void adapt_segment_stats(const int *pixel_begi
 
 // Receptor log voltage of each probe pixel under native and under cached
 // (knot-interpolated) radiance (advance_mean_probes, event_sensor.cu:138-150).
-__global__ void adapt_gate(const SegDev *gate_segs, const int *gate_seg, const AdaptDev *ad, AdaptBufs b,
-                           double *errors) {
-  const int g = blockIdx.x / 4, i = (blockIdx.x & 3) * 256 + threadIdx.x;
+// Two lanes per probe pixel (list entry i of gate g): lane 0 integrates the
+// receptor under the native radiance, lane 1 under the interpolated one, on
+// the same instructions with the inputs selected per lane, so the wave does
+// not diverge.  The error goes to errors[g * 1024 + slots[g * 1024 + i]].
+__global__ void adapt_gate(const SegDev *gate_segs, const int *gate_seg, const int *slots, const AdaptDev *ad,
+                           AdaptBufs b, double *errors) {
+  const int g = blockIdx.x / 8, t = (blockIdx.x & 7) * 256 + threadIdx.x;
+  const int i = t >> 1, chain = t & 1;
   const SegDev &G = gate_segs[g];
-  if (i >= G.npix) return;
+  if (i >= G.npix) return;  // both lanes of the pair
   const AdaptDev &A = ad[gate_seg[g]];
   const int local = G.pixel_list[i], gp = G.pixel_offset + local;
   const double charge = G.params.charge_scale;
   const PhotoParams ph = G.photo;
-  double ex = G.states[local].receptor_log, ca = ex;
+  double u = G.states[local].receptor_log;
   const bool knots = !(b.flags[gp] & 1);
   const int size = b.kcount[gp];
   const double2 *v = b.knots + (b.koff[gp] - size);
   const double *frac = b.cell_frac + A.cell_offset;
   const int cells = G.cells;
+  double *out = errors + g * 1024 + slots[g * 1024 + i];
   if (cells <= 0) {
-    errors[g * 1024 + i] = 0.0;
+    if (chain == 0) *out = 0.0;
     return;
   }
-  // Each lane's two receptor chains are serial over the cells: the inputs of
-  // the next group of cells are loaded while this group computes, and the
-  // knot pair around the cell stays in registers (cell fractions ascend, so
-  // the bracket only walks forward).
+  // The receptor chain is serial over the cells: the inputs of the next
+  // group of cells are loaded while this group computes, and the knot pair
+  // around the cell stays in registers (cell fractions ascend, so the
+  // bracket only walks forward).
   constexpr int kGroup = 4;
   auto at = [&](int c) { return min(c, cells - 1); };
   double light_n[kGroup], f_n[kGroup], dt_n[kGroup];
@@ -519,12 +525,13 @@ __global__ void adapt_gate(const SegDev *gate_segs, const int *gate_seg, const A
           const double w = (f[j] - fabs(va.x)) / (fabs(vb.x) - fabs(va.x));
           if (vb.x >= 0) cached = va.y + (vb.y - va.y) * w;
         }
-        ex = receptor_log_at(ex, light[j] * ph.electrons_per_radiance + ph.dark_electrons, dt[j], charge);
-        ca = receptor_log_at(ca, fmax(0.0, cached) * ph.electrons_per_radiance + ph.dark_electrons, dt[j], charge);
+        const double r = chain ? fmax(0.0, cached) : light[j];
+        u = receptor_log_at(u, r * ph.electrons_per_radiance + ph.dark_electrons, dt[j], charge);
       }
     }
   }
-  errors[g * 1024 + i] = fabs(ex - ca);
+  const double other = __shfl_xor(u, 1);
+  if (chain == 0) *out = fabs(u - other);
 }
 
 // Knot bracket by bisection: the index the walk above stops at, i.e. the
@@ -795,10 +802,10 @@ hipError_t launch_adapt_segment_stats(int nseg, const int *pixel_begin, AdaptBuf
   adapt_segment_stats<<<dim3(kStatBlocks, nseg), 256, 0, st>>>(pixel_begin, b, out);
   return hipGetLastError();
 }
-hipError_t launch_adapt_gate(const SegDev *gate_segs, const int *gate_seg, int ngate, const AdaptDev *ad,
-                             AdaptBufs b, double *errors, hipStream_t st) {
+hipError_t launch_adapt_gate(const SegDev *gate_segs, const int *gate_seg, const int *slots, int ngate,
+                             const AdaptDev *ad, AdaptBufs b, double *errors, hipStream_t st) {
   if (ngate <= 0) return hipSuccess;
-  adapt_gate<<<ngate * 4, 256, 0, st>>>(gate_segs, gate_seg, ad, b, errors);
+  adapt_gate<<<ngate * 8, 256, 0, st>>>(gate_segs, gate_seg, slots, ad, b, errors);
   return hipGetLastError();
 }
 size_t adapt_rates_flags(int pixels, int cells) {

@@ -992,17 +992,28 @@ class Engine {
     const int ngate = (int)gate_seg.size();
     uint64_t gate_samples = 0;
     if (ngate > 0) {
-      std::vector<int> ids((size_t)ngate * 1024);
+      // Probe pixel i of a segment is (i * 2654435761) mod npix.  They are
+      // traced in ascending pixel order, so that the rays in flight together
+      // meet nearby parts of the scene; slots[] takes each error back to
+      // position i, where the statistics read it.
+      std::vector<int> ids((size_t)ngate * 1024), slots_of((size_t)ngate * 1024);
       std::vector<SegDev> gtab(ngate);
       std::vector<long long> gblocks(ngate + 1);
       size_t rad_total = 0;
       for (int g = 0; g < ngate; ++g) {
         const StepSegment &w = work[gate_seg[g]];
         const int np = std::min(1024, w.seg->npix);
-        for (int i = 0; i < np; ++i) ids[(size_t)g * 1024 + i] = (int)(((uint64_t)i * 2654435761ULL) % (uint64_t)w.seg->npix);
+        std::vector<std::pair<int, int>> order(np);
+        for (int i = 0; i < np; ++i) order[i] = {(int)(((uint64_t)i * 2654435761ULL) % (uint64_t)w.seg->npix), i};
+        std::sort(order.begin(), order.end());
+        for (int j = 0; j < np; ++j) {
+          ids[(size_t)g * 1024 + j] = order[j].first;
+          slots_of[(size_t)g * 1024 + j] = order[j].second;
+        }
         rad_total += (size_t)np * w.cells;
       }
       gate_ids_.upload(ids.data(), ids.size(), "mean receptor probe IDs", st, staging_);
+      gate_slots_.upload(slots_of.data(), slots_of.size(), "mean receptor probe IDs", st, staging_);
       gate_rad_.reserve(std::max<size_t>(rad_total, 1), "mean receptor probe radiance");
       size_t off = 0;
       gblocks[0] = 0;
@@ -1024,7 +1035,8 @@ class Engine {
                                          counter_.ptr, st),
                 "mean receptor probe optics");
       gate_err_.reserve((size_t)ngate * 1024, "mean receptor probe errors");
-      hip_check(launch_adapt_gate(gate_tab_.ptr, gate_idx_.ptr, ngate, adapt_.ptr, b, gate_err_.ptr, st),
+      hip_check(launch_adapt_gate(gate_tab_.ptr, gate_idx_.ptr, gate_slots_.ptr, ngate, adapt_.ptr, b, gate_err_.ptr,
+                                  st),
                 "mean receptor probe integration");
       std::vector<double> errors((size_t)ngate * 1024);
       hip_check(hipMemcpyAsync(errors.data(), gate_err_.ptr, sizeof(double) * errors.size(), hipMemcpyDeviceToHost, st),
@@ -1191,7 +1203,8 @@ class Engine {
   // adaptive step state
   DeviceArray<AdaptDev> adapt_;
   DeviceArray<int> cellbeg_, bound_, flags_, bisects_, exact_terms_, kcount_, koff_, ecount_, eoff_, fb_index_,
-      fb_scan_, qpix_, kfill_, ecell_, eitem_pix_, eitem_pose_, fb_ends_, fb_list_, gate_ids_, gate_idx_, fb_items_;
+      fb_scan_, qpix_, kfill_, ecell_, eitem_pix_, eitem_pose_, fb_ends_, fb_list_, gate_ids_, gate_slots_, gate_idx_,
+      fb_items_;
   DeviceArray<unsigned> kbucket_;  // per-pixel segment words of the knot order
   DeviceArray<double> cell_frac_, qrad_, erad_, gate_rad_, gate_err_, rates_;
   // probe radiance and end poses of the last two adaptive windows, and the
