@@ -56,39 +56,96 @@ void derive(const dvs_physic &c, bool effects, CircuitParams &p, PhotoParams &ph
 }
 
 // Bounded FIFO readout (EventReadout::submit / drain, src/event_readout.cpp:6-34)
-// over events already in (t, y, x) order.
+// over events already in (t, y, x) order.  The pending events are a ring of
+// {ready time, event} in submission order; their ready times ascend (each is
+// max(previous, t) + 1 / bandwidth + latency).
 struct Readout {
   double bandwidth = 0, latency = 0, available = 0;
   size_t capacity = 1, dropped = 0, peak = 0;
   struct Pending { double ready; dvs_event event; };
-  std::deque<Pending> pending;
+  std::vector<Pending> ring;  // power-of-two size
+  size_t head = 0, count = 0;
+
+  const Pending &at(size_t i) const { return ring[(head + i) & (ring.size() - 1)]; }
+  void push(const Pending &x) {
+    if (count == ring.size()) grow(count + 1);
+    ring[(head + count) & (ring.size() - 1)] = x;
+    ++count;
+  }
+  void grow(size_t need) {
+    size_t n = std::max<size_t>(ring.size(), 4096);
+    while (n < need) n *= 2;
+    if (n == ring.size()) return;
+    std::vector<Pending> bigger(n);
+    for (size_t i = 0; i < count; ++i) bigger[i] = at(i);
+    ring.swap(bigger);
+    head = 0;
+  }
+  // Moves the first n pending events to `ready`.
+  void deliver(size_t n, std::vector<dvs_event> &ready) {
+    for (size_t i = 0; i < n; ++i) ready.push_back(at(i).event);
+    head = (head + n) & (ring.size() - 1);
+    count -= n;
+  }
+
   void submit_sorted(const dvs_event *events, size_t n, double now, std::vector<dvs_event> &ready) {
+    if (bandwidth <= 0) {  // no queue: every event is delivered as it comes
+      ready.insert(ready.end(), events, events + n);
+      return;
+    }
+    if (count + n < capacity) {
+      submit_unbounded(events, n, now, ready);
+      return;
+    }
     for (size_t k = 0; k < n; ++k) {
       const dvs_event &e = events[k];
-      while (!pending.empty() && pending.front().ready <= e.timestamp) {
-        ready.push_back(pending.front().event);
-        pending.pop_front();
-      }
-      if (bandwidth <= 0) { ready.push_back(e); continue; }
-      if (pending.size() >= capacity) { ++dropped; continue; }
+      size_t pops = 0;
+      while (pops < count && at(pops).ready <= e.timestamp) ++pops;
+      deliver(pops, ready);
+      if (count >= capacity) { ++dropped; continue; }
       const double done = std::max(available, (double)e.timestamp) + 1.0 / bandwidth;
       available = done;
       dvs_event d = e;
       d.timestamp = done + latency;
-      pending.push_back({done + latency, d});
-      peak = std::max(peak, pending.size());
+      push(Pending{done + latency, d});
+      peak = std::max(peak, count);
     }
-    while (!pending.empty() && pending.front().ready <= now) {
-      ready.push_back(pending.front().event);
-      pending.pop_front();
-    }
+    size_t pops = 0;
+    while (pops < count && at(pops).ready <= now) ++pops;
+    deliver(pops, ready);
   }
-  void drain(std::vector<dvs_event> &ready) {
-    while (!pending.empty()) {
-      ready.push_back(pending.front().event);
-      pending.pop_front();
+  // The same when no event can be dropped (fewer than `capacity` pending
+  // even if nothing were delivered): all events are queued first, then one
+  // pass over the merged order of ready times and event times replays the
+  // deliveries before each event, which only the peak depends on.  The
+  // merge is branch free: one step delivers or queues.
+  void submit_unbounded(const dvs_event *events, size_t n, double now, std::vector<dvs_event> &ready) {
+    const size_t old = count;
+    grow(count + n);
+    const double step = 1.0 / bandwidth;
+    for (size_t k = 0; k < n; ++k) {
+      const double done = std::max(available, (double)events[k].timestamp) + step;
+      available = done;
+      dvs_event d = events[k];
+      d.timestamp = done + latency;
+      ring[(head + count) & (ring.size() - 1)] = Pending{done + latency, d};
+      ++count;
     }
+    const size_t mask = ring.size() - 1;
+    size_t p = 0, k = 0, size = old, top = peak;
+    while (k < n) {
+      const bool pop = (p < old + k) & (ring[(head + p) & mask].ready <= (double)events[k].timestamp);
+      p += pop;
+      k += !pop;
+      size = pop ? size - 1 : size + 1;
+      top = std::max(top, size);
+    }
+    peak = top;
+    while (p < count && at(p).ready <= now) ++p;
+    ready.reserve(ready.size() + p);
+    deliver(p, ready);
   }
+  void drain(std::vector<dvs_event> &ready) { deliver(count, ready); }
 };
 
 struct Segment {

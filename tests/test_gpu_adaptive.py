@@ -260,3 +260,44 @@ def test_batch_adaptive_policy_equals_driver(oracle, gpu):
         total += len(ref)
         assert mine.tobytes() == ref.tobytes(), f"segment {sid} differs from the driver loop"
     assert total > 0
+
+
+def test_readout_fifo_matches_oracle_with_drops(oracle, gpu):
+    """The readout FIFO (src/event_readout.cpp:6-34) behind the device's
+    comparator: a flash overfills a 40-event queue at 2e5 events/s, later
+    windows drain it.  Every call's delivered events, and the dropped count,
+    peak and final drain, equal the oracle's FIFO fed the sensor's own
+    comparator events of each call (calls both with drops and without)."""
+    g, _, cam, phys = _plane(oracle, gpu)
+    bw, lat, cap = 2e5, 1e-6, 40
+    phys = dataclasses.replace(phys, photon_shot_noise=1, readout_bandwidth=bw, readout_latency=lat,
+                               readout_queue_capacity=cap)
+    pose = scenes.nadir_state()
+    flash = pose.copy()
+    flash["intensity"] = 20000
+    s = gpu.Sensor(g, phys)
+    s.initialize(pose, 0.0)
+    # the sensor configuration holds bandwidth and latency as float32
+    bus = oracle.Readout(float(np.float32(bw)), float(np.float32(lat)), cap)
+
+    def same(got, want):
+        assert len(got) == len(want)
+        assert np.array_equal(got["x"], want["x"]) and np.array_equal(got["y"], want["y"])
+        assert np.array_equal(got["polarity"], want["polarity"])
+        assert np.array_equal(got["timestamp"], want["timestamp"].astype(got["timestamp"].dtype))
+
+    calls = [scenes.interval(pose, flash, 0, .0003), scenes.interval(flash, flash, .0003, .0006),
+             scenes.interval(flash, pose, .0006, .0009)] + [scenes.interval(pose, pose, .0009 + .001 * k,
+                                                                             .0019 + .001 * k) for k in range(4)]
+    submitted = 0
+    for ivs in calls:
+        got = s.process(ivs)
+        raw = s.raw_events()
+        ev = np.zeros(len(raw), oracle.EVENT)
+        for f in ("timestamp", "x", "y", "polarity", "depth"):
+            ev[f] = raw[f]
+        submitted += len(ev)
+        same(got, bus.submit(ev, float(ivs["t1"][-1])))
+    dropped, peak = s.readout_stats()
+    assert dropped == bus.dropped > 0 and peak == bus.peak == cap and submitted > cap
+    same(s.drain(), bus.drain())
