@@ -188,8 +188,14 @@ __global__ __launch_bounds__(256) void adapt_decide(int bound, const int *prev_l
         const double estimate = (e.left + e.right) * .5 * ph.electrons_per_radiance + ph.dark_electrons;
         const double residual = fabs(log(fmax(1e-30, actual) / fmax(1e-30, estimate)));
         const double leaf = (e.b - e.a) * (A.end - A.start);
-        const double verr = fabs(receptor_log_at(st.receptor_log, actual, leaf, charge) -
-                                 receptor_log_at(st.receptor_log, estimate, leaf, charge));
+        // both pieces start from the pixel's voltage: exp(u) once for the two
+        double va = st.receptor_log, vb = va;
+        if (!(leaf <= 0)) {  // receptor_log_at's guard
+          const double x = exp(st.receptor_log);
+          va = receptor_log_from(st.receptor_log, x, actual, leaf, charge);
+          vb = receptor_log_from(st.receptor_log, x, estimate, leaf, charge);
+        }
+        const double verr = fabs(va - vb);
         const double budget = 1e-3 * fmin(st.threshold_pos, st.threshold_neg);
         if (verr <= budget && (residual <= A.tolerance || fmax(actual, estimate) * leaf < 1.0))
           decision = ADAPT_ACCEPT;

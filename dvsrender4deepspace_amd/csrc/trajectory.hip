@@ -17,11 +17,19 @@ namespace dvstraj {
 
 // The n - 1 intervals between consecutive times: each time's optical state
 // (to_optical_state of evaluate((float)t)) is evaluated once and written as
-// the end of the interval before it and the start of the one after it.
-__global__ void intervals_kernel(View v, const double *times, int n, dvs_optical_interval *out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double t = times[i];
+// the end of the interval before it and the start of the one after it.  The
+// times of one launch travel in its arguments (times base .. base + count - 1
+// of n), so a window needs no copy before its launch.
+constexpr int kArgTimes = 240;
+struct TimesArg {
+  double t[kArgTimes];
+  int base, count, n;
+};
+__global__ void intervals_kernel(View v, TimesArg a, dvs_optical_interval *out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.count) return;
+  const int i = a.base + j, n = a.n;
+  const double t = a.t[j];
   const Sample s = evaluate(v, (float)t, false);
   dvs_optical_state o;
   for (int c = 0; c < 3; ++c) {
@@ -164,9 +172,7 @@ struct Trajectory {
   std::vector<float> quats;  // [count][4]
   DeviceArray<Key> device_keys;
   DeviceArray<float> device_quats;
-  DeviceArray<double> device_times;
   DeviceArray<dvs_optical_interval> device_intervals;
-  PinnedArray<double> host_times;
   PinnedArray<dvs_optical_interval> host_intervals;
   hipStream_t stream = nullptr;
   ~Trajectory() {
@@ -350,20 +356,21 @@ int dvs_trajectory_intervals(dvs_trajectory trajectory, const double *times, int
       tr.device_keys.upload(tr.keys.data(), tr.keys.size(), "trajectory keys", st);
       tr.device_quats.upload(tr.quats.data(), tr.quats.size(), "trajectory keys", st);
     }
-    // times in and intervals out through page-locked buffers
+    // times in through the launches' arguments, intervals out through a
+    // page-locked buffer
     stream_wait(st, "trajectory times");  // the previous call's copies are done
-    tr.host_times.reserve((size_t)n, "trajectory times");
     tr.host_intervals.reserve((size_t)n - 1, "trajectory intervals");
-    std::memcpy(tr.host_times.ptr, times, sizeof(double) * (size_t)n);
-    tr.device_times.reserve((size_t)n, "trajectory times");
     tr.device_intervals.reserve((size_t)n - 1, "trajectory intervals");
-    hip_check(hipMemcpyAsync(tr.device_times.ptr, tr.host_times.ptr, sizeof(double) * (size_t)n,
-                             hipMemcpyHostToDevice, st),
-              "trajectory times");
     const dvstraj::View v{tr.device_keys.ptr, tr.device_quats.ptr, tr.count};
-    dvstraj::intervals_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(v, tr.device_times.ptr, (int)n,
-                                                                          tr.device_intervals.ptr);
-    hip_check(hipGetLastError(), "trajectory launch");
+    for (int64_t base = 0; base < n; base += dvstraj::kArgTimes) {
+      dvstraj::TimesArg a;
+      a.base = (int)base;
+      a.count = (int)std::min<int64_t>(dvstraj::kArgTimes, n - base);
+      a.n = (int)n;
+      std::memcpy(a.t, times + base, sizeof(double) * (size_t)a.count);
+      dvstraj::intervals_kernel<<<(unsigned)((a.count + 63) / 64), 64, 0, st>>>(v, a, tr.device_intervals.ptr);
+      hip_check(hipGetLastError(), "trajectory launch");
+    }
     hip_check(hipMemcpyAsync(tr.host_intervals.ptr, tr.device_intervals.ptr,
                              sizeof(dvs_optical_interval) * (size_t)(n - 1), hipMemcpyDeviceToHost, st),
               "trajectory intervals");
