@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -71,6 +72,35 @@ struct ScatterArgs {
 // Queues the copies of `a` from device memory `mirror` on st (one block per
 // copy).
 hipError_t launch_staged_scatter(const char *mirror, const ScatterArgs &a, hipStream_t st);
+
+// Waits for a queued device-to-host copy into page-locked memory by reading
+// its destination: the caller set the n words to `unset` (a value the copy
+// never writes) before queuing the copy, and the wait ends when none of them
+// holds it.  This sees the data several microseconds before the stream
+// reports the copy complete.  Copies on one stream run in order, so the
+// words of the last copy queued also tell that the earlier ones landed.
+// The stream is polled every 1024 reads: an error ends the wait, and so
+// does a drained stream whose copy left a word unset.
+inline void wait_words(const volatile int *host, size_t n, int unset, hipStream_t st, const char *what) {
+  for (unsigned reads = 1;; ++reads) {
+    bool all = true;
+    for (size_t i = 0; i < n; ++i) all = all && host[i] != unset;
+    if (all) {
+      std::atomic_thread_fence(std::memory_order_acquire);
+      return;
+    }
+    if (reads % 1024 == 0) {
+      const hipError_t e = hipStreamQuery(st);
+      if (e == hipSuccess) {
+        for (size_t i = 0; i < n; ++i)
+          if (host[i] == unset) throw std::runtime_error(std::string(what) + ": read-back left a word unwritten");
+        std::atomic_thread_fence(std::memory_order_acquire);
+        return;
+      }
+      if (e != hipErrorNotReady) hip_check(e, what);
+    }
+  }
+}
 
 // Page-locked staging for the small host-to-device copies of one engine
 // step.  A copy from pageable memory blocks the host until the stream has
@@ -251,6 +281,9 @@ struct Timer {
   Timer &operator=(const Timer &) = delete;
   double ms() const {
     float v = 0;
+    // a wait that read its copy's words may return before the runtime has
+    // marked the stream's events complete
+    hip_check(hipEventSynchronize(stop), "timer elapsed");
     hip_check(hipEventElapsedTime(&v, start, stop), "timer elapsed");
     return v;
   }

@@ -675,10 +675,11 @@ class Engine {
     int status = 0, total = 0;
     unsigned long long shadows = 0;
     host_words_.reserve(4, "status");
+    host_words_.ptr[0] = host_words_.ptr[1] = -1;  // the status and the count are never negative
     hip_check(hipMemcpyAsync(host_words_.ptr, status_.ptr, sizeof(int), hipMemcpyDeviceToHost, st), "copy status");
     hip_check(hipMemcpyAsync(host_words_.ptr + 1, ends_.ptr + pixel_total - 1, sizeof(int), hipMemcpyDeviceToHost, st),
               "copy count");
-    stream_wait(st, "copy count");
+    wait_words(host_words_.ptr, 2, -1, st, "copy count");
     status = host_words_.ptr[0];
     total = host_words_.ptr[1];
     if (status & 1) fail(DVS_E_CAPACITY, "pixel event capacity exceeded; reduce numerical window");
@@ -759,17 +760,18 @@ class Engine {
                   "copy events");
     }
     hip_check(hipEventRecord(events_t.stop, st), "record");
-    hip_check(hipMemcpyAsync(host_words_.ptr, status_.ptr, sizeof(int), hipMemcpyDeviceToHost, st), "copy status");
+    // the status word is copied last: when it has landed, so have the events
+    // and the ray counter
     hip_check(hipMemcpyAsync(host_words_.ptr + 2, counter_.ptr, sizeof(shadows), hipMemcpyDeviceToHost, st),
               "ray counter");
-    stream_wait(st, "copy events");
+    host_words_.ptr[0] = -1;
+    hip_check(hipMemcpyAsync(host_words_.ptr, status_.ptr, sizeof(int), hipMemcpyDeviceToHost, st), "copy status");
+    wait_words(host_words_.ptr, 1, -1, st, "copy events");
     status = host_words_.ptr[0];
     std::memcpy(&shadows, host_words_.ptr + 2, sizeof(shadows));
     if (status) fail(DVS_E_OPTICAL, "invalid optical sample or BVH traversal overflow");
     const dvs_event *sorted = host_events_.ptr;
     const int *sorted_seg = host_event_seg_.ptr;
-    const double events_ms = events_t.ms();
-    stats.events_ms += events_ms;
     stats.shadow_rays += shadows;
     stats.comparator_events += (uint64_t)total;
     if (depth) stats.event_rays += (uint64_t)total * samples * samples;
@@ -791,6 +793,8 @@ class Engine {
       s.last_time = work[i].ivs.back().t1;
       at = end;
     }
+    const double events_ms = events_t.ms();  // after the readout: by now the runtime has seen the stream drain
+    stats.events_ms += events_ms;
     return events_ms;
   }
 
@@ -931,8 +935,9 @@ class Engine {
         continue;
       }
       const size_t words = kRefineHeadWords + 2 * (size_t)levels;
+      for (size_t k = 0; k < words; ++k) host_ctr[k] = -1;  // counters are never negative
       hip_check(hipMemcpyAsync(host_ctr, ctr_.ptr, sizeof(int) * words, hipMemcpyDeviceToHost, st), "refinement count");
-      stream_wait(st, "refinement count");
+      wait_words(host_ctr, words, -1, st, "refinement count");
       terms_total = host_ctr[0];
       terms_bound = terms_total;
       exact_terms = host_ctr[1];
