@@ -212,17 +212,32 @@ __global__ __launch_bounds__(256) void adapt_decide(int bound, const int *prev_l
   const unsigned long long emask = __ballot(decision == ADAPT_EXACT);
   const unsigned long long bmask = __ballot(decision == 2 && !ahead);
   const unsigned long long dmask = __ballot(decision == 2 && ahead);
-  const int lane = threadIdx.x & 63;
-  int tbase = 0, bbase = 0, dbase = 0;
+  // One returning atomic per counter and block: atomics on one address are
+  // served one after another, so per-wave ones would cost each wave a round
+  // trip behind all the others.  The block's four waves take consecutive
+  // ranges of its share (no result depends on the compaction order).
+  __shared__ int wave_count[4][3], block_base[3];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (lane == 0) {
-    if (tmask) tbase = atomicAdd(&head[0], __popcll(tmask));
+    wave_count[wave][0] = __popcll(tmask);
+    wave_count[wave][1] = __popcll(bmask);
+    wave_count[wave][2] = __popcll(dmask);
     if (emask) atomicAdd(&head[1], __popcll(emask));
-    if (bmask) bbase = atomicAdd(&level_ctr[0], __popcll(bmask));
-    if (dmask) dbase = atomicAdd(&level_ctr[1], __popcll(dmask));
   }
-  tbase = __shfl(tbase, 0);
-  bbase = __shfl(bbase, 0);
-  dbase = __shfl(dbase, 0);
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int total = wave_count[0][threadIdx.x] + wave_count[1][threadIdx.x] + wave_count[2][threadIdx.x] +
+                      wave_count[3][threadIdx.x];
+    int *ctr = threadIdx.x == 0 ? &head[0] : &level_ctr[threadIdx.x - 1];
+    block_base[threadIdx.x] = total ? atomicAdd(ctr, total) : 0;
+  }
+  __syncthreads();
+  int tbase = block_base[0], bbase = block_base[1], dbase = block_base[2];
+  for (int w = 0; w < wave; ++w) {
+    tbase += wave_count[w][0];
+    bbase += wave_count[w][1];
+    dbase += wave_count[w][2];
+  }
   const unsigned long long below = (1ull << lane) - 1ull;
   if (resolved) {
     const int idx = tbase + __popcll(tmask & below);
