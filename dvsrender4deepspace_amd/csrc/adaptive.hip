@@ -125,13 +125,20 @@ __global__ __launch_bounds__(256) void adapt_trace(const SegDev *segs, const int
   const LdsInts lw = (LdsInts)words;  // the lanes' radiances, low then high words
   lw[threadIdx.x] = __double2loint(r.radiance);
   lw[256 + threadIdx.x] = __double2hiint(r.radiance);
+  // shadow rays counted per block: one atomic on the shared counter per
+  // block instead of per wave
+  __shared__ int block_shadows[4];
   const unsigned long long shadows = __ballot(r.shadow);
   const unsigned long long invalid = __ballot(r.invalid);
   if ((threadIdx.x & 63) == 0) {
-    if (shadows) atomicAdd(shadow_count, (unsigned long long)__popcll(shadows));
+    block_shadows[threadIdx.x >> 6] = __popcll(shadows);
     if (invalid) atomicOr(status, 2);
   }
   __syncthreads();
+  if (threadIdx.x == 0) {
+    const int n = block_shadows[0] + block_shadows[1] + block_shadows[2] + block_shadows[3];
+    if (n) atomicAdd(shadow_count, (unsigned long long)n);
+  }
   if (!active || sub != 0) return;
   if (reused) {
     const double kept = prev_probes[at + 4];  // the previous window's end probe of this pixel

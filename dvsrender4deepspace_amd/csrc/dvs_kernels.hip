@@ -75,13 +75,20 @@ __global__ __launch_bounds__(256) void trace_kernel(DevScene s, const Pose *pose
   stack[0] = __double2loint(r.radiance);
   stack[256] = __double2hiint(r.radiance);
   stack[512] = __float_as_int(r.depth);
+  // shadow rays counted per block: one atomic on the shared counter per
+  // block instead of per wave
+  __shared__ int block_shadows[4];
   const unsigned long long shadows = __ballot(r.shadow);
   const unsigned long long invalid = __ballot(r.invalid);
   if ((threadIdx.x & 63) == 0) {
-    if (shadows) atomicAdd(shadow_count, (unsigned long long)__popcll(shadows));
+    block_shadows[threadIdx.x >> 6] = __popcll(shadows);
     if (invalid) atomicOr(status, 2);
   }
   __syncthreads();
+  if (threadIdx.x == 0) {
+    const int n = block_shadows[0] + block_shadows[1] + block_shadows[2] + block_shadows[3];
+    if (n) atomicAdd(shadow_count, (unsigned long long)n);
+  }
   if (!active || sub != 0) return;
   // Sequential sum in sub-ray order j = sy*ss + sx (sample_pixel,
   // device_optics.cuh:75-93; sample_warps, optical_sampler.cu:19-21).
@@ -166,13 +173,20 @@ __global__ __launch_bounds__(256) void trace_cells_multi(const SegDev *segs, con
   const LdsInts lw = (LdsInts)words;  // the lanes' radiances, low then high words
   lw[threadIdx.x] = __double2loint(r.radiance);
   lw[256 + threadIdx.x] = __double2hiint(r.radiance);
+  // shadow rays counted per block: one atomic on the shared counter per
+  // block instead of per wave
+  __shared__ int block_shadows[4];
   const unsigned long long shadows = __ballot(r.shadow);
   const unsigned long long invalid = __ballot(r.invalid);
   if ((threadIdx.x & 63) == 0) {
-    if (shadows) atomicAdd(shadow_count, (unsigned long long)__popcll(shadows));
+    block_shadows[threadIdx.x >> 6] = __popcll(shadows);
     if (invalid) atomicOr(status, 2);
   }
   __syncthreads();
+  if (threadIdx.x == 0) {
+    const int n = block_shadows[0] + block_shadows[1] + block_shadows[2] + block_shadows[3];
+    if (n) atomicAdd(shadow_count, (unsigned long long)n);
+  }
   if (!active || sub != 0) return;
   double total = 0;
   for (int j = 0; j < R; ++j) total += __hiloint2double(lw[256 + threadIdx.x + j], lw[threadIdx.x + j]);
