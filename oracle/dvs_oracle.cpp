@@ -345,9 +345,16 @@ static inline float porosity_K(float phi) {  // :53-57
   return -logf(1.0f - p) / p;
 }
 static inline float chi(float tp) { const float t = tanf(tp); return 1.0f / sqrtf(1.0f + kPi * t * t); }
+#ifndef ORC_PLANT
+#define ORC_PLANT 0
+#endif
 static inline float E1(float tp, float cy, float sy) {
   const float ss = fmaxf(sy, 1e-6f);
+#if ORC_PLANT == 8  // E1's factor 2/pi taken as 1/pi
+  return expf((-1.0f / kPi) * (1.0f / tanf(tp)) * cy / ss);
+#else
   return expf((-2.0f / kPi) * (1.0f / tanf(tp)) * cy / ss);
+#endif
 }
 static inline float E2(float tp, float cy, float sy) {
   const float ss = fmaxf(sy, 1e-6f);
@@ -356,28 +363,65 @@ static inline float E2(float tp, float cy, float sy) {
 }
 static inline float eta(float tp, float cy, float sy) {
   const float ch = chi(tp), e1 = E1(tp, cy, sy), e2 = E2(tp, cy, sy);
+#if ORC_PLANT == 7  // eta's denominator with E2 in place of E1
+  (void)e1;
+  return ch * (cy + sy * tanf(tp) * e2 / (2.0f - e2));
+#else
   return ch * (cy + sy * tanf(tp) * e2 / (2.0f - e1));
+#endif
 }
 static inline float f_psi(float psi) { return expf(-2.0f * tanf(psi * 0.5f)); }
 struct Rough { float mu0 = 0, mue = 0, S = 0; };
+// ORC_PLANT (defined above E1) selects a deliberately wrong variant of one
+// term of the rough branch (0 = the reference's formula).  The planted
+// builds exist only so the tests can show that their checks of this branch
+// (reciprocity, the smooth limit, the published form) fail on each
+// plausible mistake.
 static inline Rough rough_S(float cosi, float cose, float sini, float sine, float psi,
                             float tp) {  // :86-129
   if (tp <= 1e-8f) return Rough{cosi, cose, 1.0f};
+#if ORC_PLANT == 6
+  const float ch = 1.0f / sqrtf(1.0f + tanf(tp) * tanf(tp)), ttp = tanf(tp);  // chi without pi
+#else
   const float ch = chi(tp), ttp = tanf(tp);
+#endif
   const float sh = sinf(psi * 0.5f), shv = sh * sh;
   Rough o;
   if (cosi >= cose) {
+#if ORC_PLANT == 1  // E2 operands swapped in mu0
+    o.mu0 = ch * (cosi + sini * ttp * (cosf(psi) * E2(tp, cosi, sini) + shv * E2(tp, cose, sine)) /
+                             (2.0f - E1(tp, cose, sine) - (psi / kPi) * E1(tp, cosi, sini)));
+#elif ORC_PLANT == 2  // E1 operands swapped in mu0's denominator
+    o.mu0 = ch * (cosi + sini * ttp * (cosf(psi) * E2(tp, cose, sine) + shv * E2(tp, cosi, sini)) /
+                             (2.0f - E1(tp, cosi, sini) - (psi / kPi) * E1(tp, cose, sine)));
+#else
     o.mu0 = ch * (cosi + sini * ttp * (cosf(psi) * E2(tp, cose, sine) + shv * E2(tp, cosi, sini)) /
                              (2.0f - E1(tp, cose, sine) - (psi / kPi) * E1(tp, cosi, sini)));
+#endif
+#if ORC_PLANT == 5  // sign of the shadowed term flipped in mue
+    o.mue = ch * (cose + sine * ttp * (E2(tp, cose, sine) + shv * E2(tp, cosi, sini)) /
+                             (2.0f - E1(tp, cose, sine) - (psi / kPi) * E1(tp, cosi, sini)));
+#else
     o.mue = ch * (cose + sine * ttp * (E2(tp, cose, sine) - shv * E2(tp, cosi, sini)) /
                              (2.0f - E1(tp, cose, sine) - (psi / kPi) * E1(tp, cosi, sini)));
+#endif
+#if ORC_PLANT == 4  // the other branch's f(psi) denominator
+    o.S = o.mue / eta(tp, cose, sine) * cosi / eta(tp, cosi, sini) * ch /
+          (1.0f - f_psi(psi) + f_psi(psi) * ch * (cose / eta(tp, cose, sine)));
+#else
     o.S = o.mue / eta(tp, cose, sine) * cosi / eta(tp, cosi, sini) * ch /
           (1.0f - f_psi(psi) + f_psi(psi) * ch * (cosi / eta(tp, cosi, sini)));
+#endif
   } else {
     o.mu0 = ch * (cosi + sini * ttp * (E2(tp, cosi, sini) - shv * E2(tp, cose, sine)) /
                              (2.0f - E1(tp, cosi, sini) - (psi / kPi) * E1(tp, cose, sine)));
+#if ORC_PLANT == 3  // cos(psi) dropped from mue
+    o.mue = ch * (cose + sine * ttp * (E2(tp, cosi, sini) + shv * E2(tp, cose, sine)) /
+                             (2.0f - E1(tp, cosi, sini) - (psi / kPi) * E1(tp, cose, sine)));
+#else
     o.mue = ch * (cose + sine * ttp * (cosf(psi) * E2(tp, cosi, sini) + shv * E2(tp, cose, sine)) /
                              (2.0f - E1(tp, cosi, sini) - (psi / kPi) * E1(tp, cose, sine)));
+#endif
     o.S = o.mue / eta(tp, cose, sine) * cosi / eta(tp, cosi, sini) * ch /
           (1.0f - f_psi(psi) + f_psi(psi) * ch * (cose / eta(tp, cose, sine)));
   }

@@ -151,6 +151,23 @@ def lib() -> ctypes.CDLL:
     return _lib
 
 
+PLANTS = range(1, 9)
+
+
+def planted_hapke(n: int):
+    """hapke_reflectance of the oracle built with planted mistake ORC_PLANT=n
+    (a wrong term of the rough-surface branch), for the tests that show
+    their checks reject it."""
+    path = HERE / "_build" / f"liboracle_plant{n}.so"
+    if not path.exists():
+        subprocess.run(["make", "-s", "-C", str(HERE), "-j8", "planted"], check=True)
+    L = ctypes.CDLL(str(path))
+    fn = L.orc_hapke_reflectance
+    fn.restype = ctypes.c_float
+    fn.argtypes = [ctypes.c_float] * 4 + [Hapke]
+    return lambda ci, ce, cg, w, h: fn(ci, ce, cg, w, hapke_struct(h))
+
+
 def ptr(a):
     return None if a is None else a.ctypes.data_as(_P)
 
