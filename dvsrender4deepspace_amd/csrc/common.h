@@ -257,10 +257,10 @@ template <class T> struct PinnedArray {
   }
   void reserve(size_t n, const char *what) {
     if (n <= count) return;
+    const size_t want = std::max<size_t>(std::max(n + n / 2, 2 * count), 1);
     if (ptr) (void)hipHostFree(ptr);
     ptr = nullptr;
     count = 0;
-    const size_t want = std::max<size_t>(std::max(n + n / 2, 2 * count), 1);
     hip_check(hipHostMalloc(reinterpret_cast<void **>(&ptr), want * sizeof(T), hipHostMallocDefault), what);
     count = want;
   }

@@ -759,14 +759,16 @@ class Engine {
                                  st),
                   "copy events");
     }
-    hip_check(hipEventRecord(events_t.stop, st), "record");
-    // the status word is copied last: when it has landed, so have the events
-    // and the ray counter
     hip_check(hipMemcpyAsync(host_words_.ptr + 2, counter_.ptr, sizeof(shadows), hipMemcpyDeviceToHost, st),
               "ray counter");
+    hip_check(hipEventRecord(events_t.stop, st), "record");
     host_words_.ptr[0] = -1;
     hip_check(hipMemcpyAsync(host_words_.ptr, status_.ptr, sizeof(int), hipMemcpyDeviceToHost, st), "copy status");
     wait_words(host_words_.ptr, 1, -1, st, "copy events");
+    // The event records, their segments and the ray counter arrive by other
+    // copies than the polled status word; only the stream orders them, so
+    // their completion is taken from the event recorded after them.
+    hip_check(hipEventSynchronize(events_t.stop), "copy events");
     status = host_words_.ptr[0];
     std::memcpy(&shadows, host_words_.ptr + 2, sizeof(shadows));
     if (status) fail(DVS_E_OPTICAL, "invalid optical sample or BVH traversal overflow");

@@ -35,14 +35,22 @@ def _lattice(seed: int, octave: int, ix: np.ndarray, iy: np.ndarray) -> np.ndarr
 
 
 def _value_noise(seed: int, octave: int, x: np.ndarray, y: np.ndarray) -> np.ndarray:
+    """Value noise on the grid x (columns) by y (rows), both 1-D: the lattice
+    is hashed once per lattice point and the quintic fades once per column
+    and per row (the grid coordinates are separable)."""
     ix, iy = np.floor(x), np.floor(y)
     fx, fy = x - ix, y - iy
-    sx = fx * fx * fx * (fx * (fx * 6 - 15) + 10)  # quintic fade
-    sy = fy * fy * fy * (fy * (fy * 6 - 15) + 10)
-    a = _lattice(seed, octave, ix, iy)
-    b = _lattice(seed, octave, ix + 1, iy)
-    c = _lattice(seed, octave, ix, iy + 1)
-    d = _lattice(seed, octave, ix + 1, iy + 1)
+    sx = (fx * fx * fx * (fx * (fx * 6 - 15) + 10))[None, :]  # quintic fade
+    sy = (fy * fy * fy * (fy * (fy * 6 - 15) + 10))[:, None]
+    x0, y0 = int(ix.min()), int(iy.min())
+    lx = np.arange(x0, int(ix.max()) + 2, dtype=np.float64)
+    ly = np.arange(y0, int(iy.max()) + 2, dtype=np.float64)
+    table = _lattice(seed, octave, lx[None, :], ly[:, None])  # [row, col] over the lattice
+    cx = (ix - x0).astype(np.int64)
+    cy = (iy - y0).astype(np.int64)
+    top, bottom = table[cy], table[cy + 1]
+    a, b = top[:, cx], top[:, cx + 1]
+    c, d = bottom[:, cx], bottom[:, cx + 1]
     return (a + (b - a) * sx) * (1 - sy) + (c + (d - c) * sx) * sy
 
 
@@ -68,12 +76,11 @@ def heightfield(spec: TerrainSpec) -> np.ndarray:
     """(size, size) heights [row, col] in metres, float64."""
     n = spec.size
     coords = np.arange(n, dtype=np.float64) * spec.spacing
-    x, y = np.meshgrid(coords, coords)  # x = col, y = row
     h = np.zeros((n, n))
     gain, norm = 1.0, 0.0
     freq = 1.0 / spec.base_wavelength
     for o in range(spec.octaves):
-        h += gain * _value_noise(spec.seed, o, x * freq, y * freq)
+        h += gain * _value_noise(spec.seed, o, coords * freq, coords * freq)
         norm += gain
         gain *= 2.0 ** -spec.hurst
         freq *= 2.0
@@ -86,7 +93,7 @@ def heightfield(spec: TerrainSpec) -> np.ndarray:
         # parabolic bowl of depth depth_ratio * r, applied on its bounding window
         c0, c1 = max(int(np.floor((cx - r) / spec.spacing)), 0), min(int(np.ceil((cx + r) / spec.spacing)) + 1, n)
         r0, r1 = max(int(np.floor((cy - r) / spec.spacing)), 0), min(int(np.ceil((cy + r) / spec.spacing)) + 1, n)
-        wx, wy = x[r0:r1, c0:c1], y[r0:r1, c0:c1]
+        wx, wy = coords[None, c0:c1], coords[r0:r1, None]
         d2 = ((wx - cx) ** 2 + (wy - cy) ** 2) / (r * r)
         h[r0:r1, c0:c1] -= np.where(d2 < 1.0, spec.depth_ratio * r * (1.0 - d2), 0.0)
     return h
@@ -97,35 +104,68 @@ def mesh(spec: TerrainSpec, heights: np.ndarray | None = None):
     h = heightfield(spec) if heights is None else heights
     n = spec.size
     half = (n - 1) * spec.spacing / 2.0
-    cols, rows = np.meshgrid(np.arange(n), np.arange(n))
-    verts = np.stack([cols * spec.spacing - half, rows * spec.spacing - half, h], axis=-1).reshape(-1, 3)
-    verts = verts.astype(np.float32)
-    c, r = np.meshgrid(np.arange(n - 1), np.arange(n - 1))
-    v1 = (r * n + c).ravel()
+    axis = np.arange(n) * spec.spacing - half
+    grid = np.empty((n, n, 3), np.float32)
+    grid[..., 0] = axis[None, :]
+    grid[..., 1] = axis[:, None]
+    grid[..., 2] = h
+    c, r = np.meshgrid(np.arange(n - 1, dtype=np.uint32), np.arange(n - 1, dtype=np.uint32))
+    v1 = (r * np.uint32(n) + c).ravel()
     v2 = v1 + 1
     v3 = v1 + n
     v4 = v3 + 1
     tris = np.empty((2 * len(v1), 3), np.uint32)
     tris[0::2] = np.stack([v1, v3, v2], axis=1)
     tris[1::2] = np.stack([v2, v3, v4], axis=1)
-    return verts, vertex_normals(verts, tris), tris
+    return grid.reshape(-1, 3), grid_normals(grid), tris
+
+
+def _corner_weights(pa, pb, pc):
+    """Unit face normal (oriented to +Z) of triangle (pa, pb, pc), float64
+    arrays (..., 3), and its angle at each corner."""
+    face = np.cross(pb - pa, pc - pa)
+    face *= np.where(face[..., 2:3] < 0, -1.0, 1.0)
+    length = np.linalg.norm(face, axis=-1, keepdims=True)
+    face = np.divide(face, length, out=np.zeros_like(face), where=length > 0)
+    corners = (pa, pb, pc)
+    angles = []
+    for k in range(3):
+        e1 = corners[(k + 1) % 3] - corners[k]
+        e2 = corners[(k + 2) % 3] - corners[k]
+        cosang = np.einsum("...i,...i->...", e1, e2) / np.maximum(
+            np.linalg.norm(e1, axis=-1) * np.linalg.norm(e2, axis=-1), 1e-30)
+        angles.append(np.arccos(np.clip(cosang, -1.0, 1.0))[..., None])
+    return face, angles
+
+
+def grid_normals(grid: np.ndarray) -> np.ndarray:
+    """Angle-weighted vertex normals of the grid mesh (faces oriented to +Z):
+    each vertex sums face normal x corner angle over the (up to six)
+    triangles around it, cell by cell over the whole grid at once."""
+    p = grid.astype(np.float64)
+    v1, v2, v3, v4 = p[:-1, :-1], p[:-1, 1:], p[1:, :-1], p[1:, 1:]
+    acc = np.zeros_like(p)
+    face, (a1, a3, a2) = _corner_weights(v1, v3, v2)  # T1 = (v1, v3, v2)
+    acc[:-1, :-1] += face * a1
+    acc[1:, :-1] += face * a3
+    acc[:-1, 1:] += face * a2
+    face, (b2, b3, b4) = _corner_weights(v2, v3, v4)  # T2 = (v2, v3, v4)
+    acc[:-1, 1:] += face * b2
+    acc[1:, :-1] += face * b3
+    acc[1:, 1:] += face * b4
+    acc = acc.reshape(-1, 3)
+    length = np.linalg.norm(acc, axis=1, keepdims=True)
+    normals = np.divide(acc, length, out=np.zeros_like(acc), where=length > 1e-12)
+    return normals.astype(np.float32)
 
 
 def vertex_normals(verts: np.ndarray, tris: np.ndarray) -> np.ndarray:
-    """Angle-weighted vertex normals, faces oriented to +Z."""
+    """Angle-weighted vertex normals of an indexed mesh, faces oriented to +Z."""
     p = verts.astype(np.float64)[tris]  # (T, 3 corners, 3)
-    face = np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0])
-    face *= np.where(face[:, 2:3] < 0, -1.0, 1.0)
-    length = np.linalg.norm(face, axis=1, keepdims=True)
-    face = np.divide(face, length, out=np.zeros_like(face), where=length > 0)
+    face, angles = _corner_weights(p[:, 0], p[:, 1], p[:, 2])
     acc = np.zeros((len(verts), 3))
     for k in range(3):
-        e1 = p[:, (k + 1) % 3] - p[:, k]
-        e2 = p[:, (k + 2) % 3] - p[:, k]
-        cosang = np.einsum("ij,ij->i", e1, e2) / np.maximum(
-            np.linalg.norm(e1, axis=1) * np.linalg.norm(e2, axis=1), 1e-30)
-        ang = np.arccos(np.clip(cosang, -1.0, 1.0))
-        w = face * ang[:, None]
+        w = face * angles[k]
         for a in range(3):
             acc[:, a] += np.bincount(tris[:, k], weights=w[:, a], minlength=len(verts))
     length = np.linalg.norm(acc, axis=1, keepdims=True)
