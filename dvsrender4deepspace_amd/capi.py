@@ -42,6 +42,9 @@ TRAJECTORY_STATE_DTYPE = np.dtype([("timestamp", "<f4"), ("position", "<f4", (3,
                                    ("angular_velocity", "<f4", (3,)), ("light_direction", "<f4", (3,)),
                                    ("light_direction_rate", "<f4", (3,)), ("light_intensity", "<f4"),
                                    ("light_intensity_rate", "<f4")])
+PIXEL_PARAMS_DTYPE = np.dtype([(n, "<f8") for n in ("threshold_pos", "threshold_neg", "charge_scale",
+                                                     "refractory_time", "reference_leak", "reset_noise_std")])
+CROSSING_DTYPE = np.dtype([("time", "<f8"), ("polarity", "<i4"), ("pad", "<i4")])
 assert NODE_DTYPE.itemsize == 40 and STATE_DTYPE.itemsize == 72 and INTERVAL_DTYPE.itemsize == 160
 assert EVENT_DTYPE.itemsize == 17 and PIXEL_STATE_DTYPE.itemsize == 64
 assert FRAME_DTYPE.itemsize == 80 and TRAJECTORY_STATE_DTYPE.itemsize == 124
@@ -234,6 +237,7 @@ def load_library() -> ctypes.CDLL:
         "dvs_receptor_eval": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, _I64, _P]),
         "dvs_noise_eval": (ctypes.c_int, [ctypes.c_uint64, _I64, ctypes.c_double, _I32, _P, _P]),
         "dvs_poisson_eval": (ctypes.c_int, [_P, _P, _I64, _P]),
+        "dvs_advance_pixel_eval": (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _I32, ctypes.c_double, _P, _P]),
         "dvs_trace_rays": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P, _P, _P, _P]),
         "dvs_trace_shadow_rays": (ctypes.c_int, [_P, _P, _P, _P, _I64, _P]),
         "dvs_trajectory_create": (ctypes.c_int, [_P, _I32, ctypes.c_float, _P, _P, ctypes.POINTER(_P)]),
@@ -648,6 +652,67 @@ class Batch:
         _check(load_library().dvs_batch_reset_stats(self.handle))
 
 
+_hip = None
+
+
+def hip_runtime() -> ctypes.CDLL:
+    """The HIP runtime libdvs_hip.so is linked against (libamdhip64.so.7 of
+    /opt/rocm), for device buffers handed to the *_device entry points.
+    (A second HIP runtime in the process, such as the one bundled with
+    torch, cannot share the device with it.)"""
+    global _hip
+    if _hip is None:
+        load_library()
+        hip = ctypes.CDLL("libamdhip64.so.7")
+        hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        hip.hipFree.argtypes = [ctypes.c_void_p]
+        hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        hip.hipDeviceSynchronize.argtypes = []
+        _hip = hip
+    return _hip
+
+
+class DeviceBuffer:
+    """Device memory of `nbytes` from hipMalloc, freed on close()."""
+
+    H2D, D2H = 1, 2
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        self.ptr = ctypes.c_void_p()
+        if hip_runtime().hipMalloc(ctypes.byref(self.ptr), max(self.nbytes, 1)) != 0:
+            raise DvsError(DVS_E_HIP, f"hipMalloc of {self.nbytes} bytes failed")
+
+    @property
+    def address(self) -> int:
+        return int(self.ptr.value)
+
+    def upload(self, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        if hip_runtime().hipMemcpy(self.ptr, a.ctypes.data_as(ctypes.c_void_p), a.nbytes, self.H2D) != 0:
+            raise DvsError(DVS_E_HIP, "hipMemcpy to device failed")
+        return self
+
+    def download(self, dtype, count: int) -> np.ndarray:
+        out = np.empty(count, dtype)
+        assert out.nbytes <= self.nbytes
+        if hip_runtime().hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), self.ptr, out.nbytes, self.D2H) != 0:
+            raise DvsError(DVS_E_HIP, "hipMemcpy to host failed")
+        return out
+
+    def close(self) -> None:
+        if self.ptr.value:
+            hip_runtime().hipFree(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def hapke_eval(cosi, cose, cosg, albedo, hapke: Hapke) -> np.ndarray:
     a, b, c, d = (np.ascontiguousarray(x, np.float32) for x in np.broadcast_arrays(cosi, cose, cosg, albedo))
     a, b, c, d = (np.ascontiguousarray(x) for x in (a, b, c, d))
@@ -663,6 +728,22 @@ def receptor_eval(u, rate, dt, charge: float) -> np.ndarray:
     out = np.empty(a.shape, np.float64)
     _check(load_library().dvs_receptor_eval(_ptr(a), _ptr(b), _ptr(c), float(charge), a.size, _ptr(out)))
     return out
+
+
+def advance_pixel_eval(states: np.ndarray, params: np.ndarray, rates, t0, t1, capacity: int = 64,
+                       normal: float = 0.0):
+    """advance_pixel (include/dvs/pixel_model.hpp:35-102) of n independent
+    pixels on the device.  `states` (PIXEL_STATE_DTYPE) is updated in place;
+    returns (counts, crossings[n, capacity]) with count -1 on overflow."""
+    n = len(states)
+    assert states.dtype == PIXEL_STATE_DTYPE and states.flags.c_contiguous
+    params = np.ascontiguousarray(params, PIXEL_PARAMS_DTYPE)
+    r, a, b = (np.ascontiguousarray(np.broadcast_to(x, (n,)), np.float64) for x in (rates, t0, t1))
+    counts = np.empty(n, np.int32)
+    events = np.zeros((n, max(capacity, 1)), CROSSING_DTYPE)
+    _check(load_library().dvs_advance_pixel_eval(_ptr(states), _ptr(params), _ptr(r), _ptr(a), _ptr(b), n,
+                                                 int(capacity), float(normal), _ptr(counts), _ptr(events)))
+    return counts, events[:, :capacity]
 
 
 def poisson_eval(lam, u) -> np.ndarray:

@@ -270,6 +270,31 @@ int dvs_receptor_eval(const double *u, const double *rate, const double *dt, dou
   });
 }
 
+int dvs_advance_pixel_eval(dvs_pixel_state *states, const dvs_pixel_params *params, const double *rates,
+                           const double *t0, const double *t1, int64_t n, int32_t capacity, double normal,
+                           int32_t *counts, dvs_crossing *events) {
+  return guarded([&] {
+    if (!states || !params || !rates || !t0 || !t1 || !counts || n < 0 || capacity < 0 || (capacity > 0 && !events))
+      fail(DVS_E_ARGUMENT, "invalid advance_pixel arguments");
+    DeviceArray<dvs_pixel_state> st;
+    DeviceArray<dvs_pixel_params> pa;
+    DeviceArray<double> r, a, b;
+    DeviceArray<int> c;
+    DeviceArray<dvs_crossing> ev;
+    st.upload(states, n, "advance input"); pa.upload(params, n, "advance input"); r.upload(rates, n, "advance input");
+    a.upload(t0, n, "advance input"); b.upload(t1, n, "advance input");
+    c.reserve(n, "advance output");
+    ev.reserve(std::max<size_t>(1, (size_t)n * capacity), "advance output");
+    hip_check(launch_advance(n, st.ptr, pa.ptr, r.ptr, a.ptr, b.ptr, capacity, normal, c.ptr, ev.ptr, nullptr),
+              "advance kernel");
+    hip_check(hipMemcpy(states, st.ptr, n * sizeof(dvs_pixel_state), hipMemcpyDeviceToHost), "advance output");
+    hip_check(hipMemcpy(counts, c.ptr, n * sizeof(int), hipMemcpyDeviceToHost), "advance output");
+    if (capacity > 0)
+      hip_check(hipMemcpy(events, ev.ptr, (size_t)n * capacity * sizeof(dvs_crossing), hipMemcpyDeviceToHost),
+                "advance output");
+  });
+}
+
 int dvs_poisson_eval(const double *lambda, const double *u, int64_t n, double *out) {
   return guarded([&] {
     if (!lambda || !u || !out || n < 0) fail(DVS_E_ARGUMENT, "invalid poisson arguments");

@@ -400,6 +400,29 @@ __global__ void receptor_kernel(int64_t n, const double *u, const double *rate, 
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = receptor_log_at(u[i], rate[i], dt[i], charge);
 }
+// advance_pixel of n independent pixels, one lane each; `normal` is the
+// value of every reset-noise draw.
+__global__ void advance_kernel(int64_t n, dvs_pixel_state *states, const dvs_pixel_params *params,
+                               const double *rates, const double *t0, const double *t1, int capacity, double normal,
+                               int *counts, dvs_crossing *events) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  dvs_pixel_state st = states[i];
+  PixelRegs r{st.receptor_log, st.reference_log, st.refractory_until, st.threshold_pos, st.threshold_neg,
+              st.reset_pending};
+  const dvs_pixel_params q = params[i];
+  const CircuitParams p{q.threshold_pos, q.threshold_neg, q.charge_scale, q.refractory_time, q.reference_leak,
+                        q.reset_noise_std};
+  CrossingEvent *out = reinterpret_cast<CrossingEvent *>(events + i * capacity);
+  const int count = advance_pixel(r, p, rates[i], t0[i], t1[i], out, capacity, [&] { return normal; });
+  counts[i] = count;
+  st.receptor_log = r.receptor_log;
+  st.reference_log = r.reference_log;
+  st.refractory_until = r.refractory_until;
+  st.reset_pending = r.reset_pending;
+  st.receptor_inverse = exp(-r.receptor_log);
+  states[i] = st;
+}
 __global__ void poisson_kernel(int64_t n, const double *lambda, const double *u, double *out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = poisson_inversion(lambda[i], u[i]);
@@ -599,6 +622,14 @@ hipError_t launch_receptor(int64_t n, const double *u, const double *rate, const
                            double *out, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   receptor_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, u, rate, dt, charge, out);
+  return hipGetLastError();
+}
+hipError_t launch_advance(int64_t n, dvs_pixel_state *states, const dvs_pixel_params *params, const double *rates,
+                          const double *t0, const double *t1, int capacity, double normal, int *counts,
+                          dvs_crossing *events, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  advance_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, states, params, rates, t0, t1, capacity, normal,
+                                                                counts, events);
   return hipGetLastError();
 }
 hipError_t launch_poisson(int64_t n, const double *lambda, const double *u, double *out, hipStream_t st) {

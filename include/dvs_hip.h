@@ -106,6 +106,18 @@ typedef struct {
   double receptor_inverse;
 } dvs_pixel_state;
 
+/* PixelParameters (include/dvs/pixel_model.hpp:7-10) and PixelEvent
+ * (:15-17): the circuit constants and one comparator crossing. */
+typedef struct {
+  double threshold_pos, threshold_neg, charge_scale, refractory_time;
+  double reference_leak, reset_noise_std;
+} dvs_pixel_params;
+typedef struct {
+  double time;
+  int32_t polarity;
+  int32_t pad;
+} dvs_crossing;
+
 /* Device timings and work counters of one sensor or scene handle. */
 typedef struct {
   double trace_ms, circuit_ms, events_ms, total_ms;
@@ -340,6 +352,16 @@ int dvs_hapke_eval(const float *cosi, const float *cose, const float *cosg, cons
 /* receptor_log_at (include/dvs/pixel_model.hpp:18-26) over n inputs. */
 int dvs_receptor_eval(const double *u, const double *rate, const double *dt, double charge,
                       int64_t n, double *out);
+/* advance_pixel (include/dvs/pixel_model.hpp:35-102) of n independent pixels
+ * on the device: pixel i starts from states[i] (updated in place; the first
+ * six fields are the reference's PixelState), integrates the constant
+ * photo-electron rate rates[i] over [t0[i], t1[i]] under params[i], takes
+ * `normal` as every reset-noise draw, and writes its crossings to
+ * events[i * capacity ...]; counts[i] is their number, or -1 past
+ * `capacity` (the reference's overflow). */
+int dvs_advance_pixel_eval(dvs_pixel_state *states, const dvs_pixel_params *params, const double *rates,
+                           const double *t0, const double *t1, int64_t n, int32_t capacity, double normal,
+                           int32_t *counts, dvs_crossing *events);
 /* The device's sequential-search Poisson inversion of uniform u[i] at
  * lambda[i] (0 < lambda < 10), the shot-noise draw of a circuit cell. */
 int dvs_poisson_eval(const double *lambda, const double *u, int64_t n, double *out);
