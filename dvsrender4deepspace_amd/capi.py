@@ -199,6 +199,7 @@ def load_library() -> ctypes.CDLL:
                                           ctypes.POINTER(_I64)]),
         "dvs_frame_render": (ctypes.c_int, [_P, _P, _I32, _P, _P, _P]),
         "dvs_frame_render_device": (ctypes.c_int, [_P, _P, _I32, _P, _P]),
+        "dvs_frame_render_device_noalloc": (ctypes.c_int, [_P, _P, _I32, _P, _P]),
         "dvs_scene_stats": (ctypes.c_int, [_P, ctypes.POINTER(_StatsC)]),
         "dvs_frame_traversal_stats": (ctypes.c_int, [_P, _P, _P]),
         "dvs_sensor_create": (ctypes.c_int, [_P, ctypes.POINTER(_PhysicC), _I32, _I32, ctypes.POINTER(_P)]),
@@ -362,6 +363,13 @@ class Scene:
         _check(load_library().dvs_frame_render_device(self.handle, ctypes.c_void_p(device_poses_ptr), count,
                                                       ctypes.c_void_p(device_image_ptr),
                                                       ctypes.c_void_p(device_depth_ptr)))
+
+    def render_device_noalloc(self, device_poses_ptr: int, count: int, device_image_ptr: int,
+                              device_depth_ptr: int):
+        """render_device without any device allocation (at most 4096 poses)."""
+        _check(load_library().dvs_frame_render_device_noalloc(self.handle, ctypes.c_void_p(device_poses_ptr), count,
+                                                              ctypes.c_void_p(device_image_ptr),
+                                                              ctypes.c_void_p(device_depth_ptr)))
 
     def stats(self) -> Stats:
         s = _StatsC()

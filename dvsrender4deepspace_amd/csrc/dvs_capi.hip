@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -47,6 +48,42 @@ DevScene make_view(const Geometry &g, const dvs_camera &camera, const dvs_hapke 
 }  // namespace
 
 namespace {
+// Scratch of dvs_frame_render_device_noalloc, which may not allocate: the
+// status word (bytes 0-3), the shadow ray counter (bytes 8-15) and the
+// converted poses of up to kNoAllocMaxPoses frames from byte kNoAllocHeader,
+// 196,864 bytes in all, held by every GPU the library runs on until the
+// process ends.  Every call on a GPU shares that GPU's copy: calls that
+// overlap on one GPU race.  In host code `scratch` names a host-side
+// stand-in; its device address on each GPU comes from hipGetSymbolAddress.
+constexpr int kNoAllocMaxPoses = 4096;
+constexpr size_t kNoAllocHeader = 256;
+constexpr size_t kNoAllocScratchBytes = kNoAllocHeader + sizeof(Pose) * kNoAllocMaxPoses;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kNoAllocDevices = 64;
+std::mutex noalloc_mutex;
+void *noalloc_ptr[kNoAllocDevices] = {};
+
+// The device address of `scratch` on the GPU that owns `st`, looked up once
+// per device with that device current.
+void *noalloc_scratch(hipStream_t st) {
+  int device = 0;
+  hip_check(hipStreamGetDevice(st, &device), "scratch device");
+  if (device < 0 || device >= kNoAllocDevices) fail(DVS_E_ARGUMENT, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(noalloc_mutex);
+  if (!noalloc_ptr[device]) {
+    int current = 0;
+    hip_check(hipGetDevice(&current), "scratch device");
+    if (current != device) hip_check(hipSetDevice(device), "scratch device");
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (current != device) hip_check(hipSetDevice(current), "scratch device");
+    hip_check(e, "scratch address");
+    if (!ptr) fail(DVS_E_HIP, "scratch address");
+    noalloc_ptr[device] = ptr;
+  }
+  return noalloc_ptr[device];
+}
+
 // Renders `n` device poses of `s` into device image/depth/radiance buffers.
 void frame_trace(dvs_scene s, const Pose *poses, int n, double *rad, float *img, float *dep, hipStream_t st) {
   const int npix = s->view.width * s->view.height;
@@ -216,6 +253,38 @@ int dvs_frame_render_device(dvs_scene s, const dvs_optical_state *device_poses, 
     s->poses.reserve(n, "optical poses");
     hip_check(launch_state_poses(n, device_poses, s->poses.ptr, s->stream), "pose conversion");
     frame_trace(s, s->poses.ptr, n, nullptr, device_image, device_depth, s->stream);
+  });
+}
+
+int dvs_frame_render_device_noalloc(dvs_scene s, const dvs_optical_state *device_poses, int32_t n,
+                                    float *device_image, float *device_depth) {
+  return guarded([&] {
+    if (!s || !device_poses || !device_image || !device_depth || n <= 0)
+      fail(DVS_E_ARGUMENT, "invalid parallel optical request");
+    if (n > kNoAllocMaxPoses)
+      fail(DVS_E_ARGUMENT, "dvs_frame_render_device_noalloc renders at most " + std::to_string(kNoAllocMaxPoses) +
+                               " poses per call");
+    hipStream_t st = s->stream;
+    unsigned char *base = static_cast<unsigned char *>(noalloc_scratch(st));
+    int *status = reinterpret_cast<int *>(base);
+    unsigned long long *counter = reinterpret_cast<unsigned long long *>(base + 8);
+    Pose *poses = reinterpret_cast<Pose *>(base + kNoAllocHeader);
+    // status word and ray counter start at zero; a fill kernel clears them
+    hip_check(launch_fill_int(status, 0, 4, st), "clear optical status");
+    hip_check(launch_state_poses(n, device_poses, poses, st), "pose conversion");
+    const int npix = s->view.width * s->view.height;
+    hip_check(launch_trace(dvsdev::MODE_FRAME, s->view, poses, n, npix, (long long)npix * n, nullptr, nullptr,
+                           device_image, device_depth, status, counter, st),
+              "parallel optical launch");
+    int words[4] = {0, 0, 0, 0};
+    hip_check(hipMemcpyAsync(words, base, sizeof(words), hipMemcpyDeviceToHost, st), "parallel optical status");
+    hip_check(hipStreamSynchronize(st), "parallel optical wait");
+    unsigned long long shadows = 0;
+    std::memcpy(&shadows, words + 2, sizeof(shadows));
+    s->stats.trace_launches += 1;
+    s->stats.primary_rays += (uint64_t)npix * n * s->view.samples * s->view.samples;
+    s->stats.shadow_rays += shadows;
+    if (words[0]) fail(DVS_E_OPTICAL, "invalid optical sample/BVH stack overflow");
   });
 }
 

@@ -208,6 +208,17 @@ int dvs_frame_render(dvs_scene scene, const dvs_optical_state *poses, int32_t po
 /* Same on device buffers (no host copies); for timing. */
 int dvs_frame_render_device(dvs_scene scene, const dvs_optical_state *device_poses,
                             int32_t pose_count, float *device_image, float *device_depth);
+/* No reference counterpart.  dvs_frame_render_device for callers that cannot
+ * allocate device memory (code that must not call hipMalloc, such as tight
+ * serving loops and harnesses that forbid it): it allocates nothing.  Its
+ * pose table, status word and ray counter live in a fixed device array of
+ * the library (196,864 bytes per GPU, held until the process ends), so
+ * pose_count is at most 4096 (larger requests fail with DVS_E_ARGUMENT).
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * race.  Synchronous like dvs_frame_render_device; it adds rays and launches
+ * to the scene's statistics but no trace time. */
+int dvs_frame_render_device_noalloc(dvs_scene scene, const dvs_optical_state *device_poses,
+                                    int32_t pose_count, float *device_image, float *device_depth);
 int dvs_scene_stats(dvs_scene scene, dvs_stats *out);
 /* Traversal work of every sub-ray of one frame, for the roofline's
  * algorithmic bytes per ray (no reference counterpart): totals[0..7] =
