@@ -146,7 +146,10 @@ class _StatsC(ctypes.Structure):
                 ("comparator_events", ctypes.c_uint64), ("delivered_events", ctypes.c_uint64),
                 ("native_equivalent_rays", ctypes.c_uint64), ("adaptive_windows", ctypes.c_uint64),
                 ("rejected_windows", ctypes.c_uint64), ("rates_ms", ctypes.c_double),
-                ("circuit_launches", ctypes.c_double)]
+                ("circuit_launches", ctypes.c_double), ("probe_ms", ctypes.c_double),
+                ("query_ms", ctypes.c_double), ("probe_launches", ctypes.c_double),
+                ("query_launches", ctypes.c_double), ("probe_samples", ctypes.c_uint64),
+                ("query_samples", ctypes.c_uint64)]
 
 
 class _RefineC(ctypes.Structure):

@@ -202,6 +202,9 @@ __device__ inline Hit trace_nearest(const DevScene &s, f3 origin, f3 ray, int *s
   const GlobalFloats pairs = (GlobalFloats)s.pairs, tris = (GlobalFloats)s.triangles;
   const GlobalInts leaves = (GlobalInts)s.leaves;
   const LdsInts lds = (LdsInts)stack;
+  // copied once: `s` may be read through a generic pointer, which stores to
+  // the LDS stack could alias, so its fields would be reloaded every push
+  const int depth = s.stack_depth;
   int ref = s.root_ref, sp = 0;
   for (;;) {
     if (ref < 0) {
@@ -226,7 +229,7 @@ __device__ inline Hit trace_nearest(const DevScene &s, f3 origin, f3 ray, int *s
       cnt.pair();
       const int mask = test_pair(pairs + 16 * (size_t)ref, origin, idir, kRayEps, closest, d0, d1, l, r);
       if (mask == 3) {
-        if (sp >= s.stack_depth) { h.hit = -1; return h; }
+        if (sp >= depth) { h.hit = -1; return h; }
         const bool left_first = d0 < d1;
         lds[sp * stride] = left_first ? r : l;
         ++sp;
@@ -251,6 +254,7 @@ __device__ inline bool trace_any(const DevScene &s, f3 origin, f3 dir, int self,
   const GlobalFloats pairs = (GlobalFloats)s.pairs, tris = (GlobalFloats)s.triangles;
   const GlobalInts leaves = (GlobalInts)s.leaves;
   const LdsInts lds = (LdsInts)stack;
+  const int depth = s.stack_depth;
   int ref = s.root_ref, sp = 0;
   for (;;) {
     if (ref < 0) {
@@ -269,7 +273,7 @@ __device__ inline bool trace_any(const DevScene &s, f3 origin, f3 dir, int self,
       cnt.pair();
       const int mask = test_pair(pairs + 16 * (size_t)ref, origin, idir, t_min, t_max, d0, d1, l, r);
       if (mask == 3) {
-        if (sp >= s.stack_depth) return true;
+        if (sp >= depth) return true;
         const bool left_first = d0 < d1;
         lds[sp * stride] = left_first ? r : l;
         ++sp;

@@ -873,10 +873,14 @@ class Engine {
                                        probe_reuse_.ptr, st),
               "probe poses");
     probes_[pcur].reserve((size_t)P * 5, "optical probes");
+    std::vector<std::unique_ptr<Timer>> query_t;
+    Timer probe_t;
+    hip_check(hipEventRecord(probe_t.start, st), "record");
     hip_check(launch_adapt_trace(dvsdev::LAYOUT_PROBE, segtab_.ptr, pbegin_.ptr, nseg, samples, (long long)P * 5,
                                  nullptr, nullptr, probe_poses_.ptr, lds, probes_[pcur].ptr, status_.ptr, counter_.ptr,
                                  st, probe_reuse_.ptr, probes_[pprev].ptr),
               "refine pixel optical launch");
+    hip_check(hipEventRecord(probe_t.stop, st), "record");
     probe_layout_ = std::move(layout);
     probe_cur_ = pcur;
     for (auto *a : {&bound_, &flags_, &bisects_, &exact_terms_, &kcount_, &koff_, &ecount_, &eoff_, &fb_index_,
@@ -952,9 +956,13 @@ class Engine {
       if (nb > 0) {
         const long long queries = (long long)dvsdev::kBisectQueries * nb;
         qrad_.reserve_ahead(queries, "optical queries");
+        query_t.emplace_back(new Timer());
+        hip_check(hipEventRecord(query_t.back()->start, st), "record");
         hip_check(launch_adapt_trace(dvsdev::LAYOUT_QUERY, segtab_.ptr, pbegin_.ptr, nseg, samples, queries,
                                      qpix_.ptr, nullptr, qpose_.ptr, lds, qrad_.ptr, status_.ptr, counter_.ptr, st),
                   "refine pixel optical launch");
+        hip_check(hipEventRecord(query_t.back()->stop, st), "record");
+        stats.query_samples += (uint64_t)queries;
         hip_check(launch_adapt_children((int)nb, bis_.ptr, qrad_.ptr, ent_[1 - cur].ptr + 2 * nd, st),
                   "refine pixel optical launch");
       }
@@ -1047,6 +1055,10 @@ class Engine {
     host_seg_stats_.reserve(nseg, "refinement statistics");
     const AdaptSegStats *seg_stats = host_seg_stats_.ptr;  // read after the step's final wait
     hip_check(hipMemcpyAsync(host_seg_stats_.ptr, seg_stats_.ptr, sizeof(AdaptSegStats) * nseg, hipMemcpyDeviceToHost, st),
+              "refinement statistics");
+    host_reuse_.reserve(nseg, "refinement statistics");
+    const int *probe_reused = host_reuse_.ptr;  // read after the step's final wait
+    hip_check(hipMemcpyAsync(host_reuse_.ptr, probe_reuse_.ptr, sizeof(int) * nseg, hipMemcpyDeviceToHost, st),
               "refinement statistics");
     // Convergence gate (verify_receptor, event_sensor.cu:357-374) of the
     // segments that check this window.
@@ -1222,6 +1234,11 @@ class Engine {
     stats.trace_ms += refine_ms;
     stats.circuit_ms += circuit_ms;
     stats.rates_ms += photo_circuit_ms - circuit_ms;
+    stats.probe_ms += probe_t.ms();
+    stats.probe_launches += 1;
+    for (int i = 0; i < nseg; ++i) stats.probe_samples += (uint64_t)work[i].seg->npix * (probe_reused[i] ? 4 : 5);
+    for (auto &t : query_t) stats.query_ms += t->ms();
+    stats.query_launches += (double)query_t.size();
     stats.circuit_launches += (double)integrate_t.size();
     stats.total_ms += step_ms + events_ms;
     uint64_t samples_traced = gate_samples + fallback_samples;
@@ -1276,6 +1293,7 @@ class Engine {
   DeviceArray<double> probes_[2];
   DeviceArray<Pose> end_pose_[2];
   DeviceArray<int> probe_reuse_;
+  PinnedArray<int> host_reuse_;
   std::vector<long long> probe_layout_;
   int probe_cur_ = 0;
   DeviceArray<int> rare_native_, rare_adapt_;  // per-wave flags of the photocurrent's second phase

@@ -130,6 +130,11 @@ typedef struct {
   /* adaptive steps: photocurrent kernels (knot interpolation, fallback
    * tiles, shot noise) apart from the serial circuit in circuit_ms */
   double rates_ms, circuit_launches;
+  /* adaptive refinement's ray launches, timed on their stream: the five
+   * initial probes per pixel (probe) and the bisections' midpoints (query);
+   * pixel samples traced by each (ss*ss primary rays apiece) */
+  double probe_ms, query_ms, probe_launches, query_launches;
+  uint64_t probe_samples, query_samples;
 } dvs_stats;
 
 /* OpticalRefinementStats (include/dvs/optical_sampler.hpp:16) of the last
