@@ -78,7 +78,8 @@ struct AdaptBufs {
   double *erad;          // their radiance
   int *fb_index;         // index of the pixel in its segment's fallback list, -1 otherwise
   const double *cell_frac;  // window fraction of every cell midpoint of the step
-  double *rates;         // [cell - chunk start][pixel] photocurrent of knot pixels
+  double *rates;         // [cell - chunk start][pixel] photocurrent of knot pixels (escaped cells)
+  uint64_t *codes;       // [(cell - chunk start) / 8][pixel] photocurrent codes of knot pixels
 };
 
 // Refinement counters of one window, cleared once per window.  Four head

@@ -581,6 +581,7 @@ __device__ __forceinline__ int knot_bracket_search(const double2 *v, int size, d
 // position, then both walk forward as in the reference, so each knot is read
 // once per run while the rate stores stay coalesced across pixels.
 constexpr int kRateCells = 32;
+static_assert(kRateCells % kCodeCells == 0, "rate runs start at whole code words");
 // Returns whether any cell of the run was left to adapt_rates_rare.
 __device__ __forceinline__ bool adapt_rates_run(const SegDev *segs, const AdaptDev *ad, const int *pixel_begin,
                                                 int nseg, int pixels, const AdaptBufs &b, int c0, int c1, int *status) {
@@ -610,7 +611,9 @@ __device__ __forceinline__ bool adapt_rates_run(const SegDev *segs, const AdaptD
   const uint64_t seed = g.seed;
   const CellMaps *maps = g.maps;
   double *out = b.rates + gp;
+  uint64_t *out_codes = b.codes + gp;
   uint64_t shared[4] = {0, 0, 0, 0}, shared_index = ~0ull;
+  uint64_t word = 0;  // codes of the current group of kCodeCells cells
   bool marked = false;
   // the knot pair around the cell stays in registers (knot_bracket's walk),
   // and the next cell's fraction and map words are loaded a cell ahead
@@ -641,9 +644,19 @@ __device__ __forceinline__ bool adapt_rates_run(const SegDev *segs, const AdaptD
       shared_index = serial / kShotCellsPerBlock;
       shot_block(seed, (uint64_t)i, shared_index, shared);
     }
-    const double rate = cell_rate_first(light, ph, t.y - t.x, r.x, r.y, serial, shared);
-    out[(size_t)(c - c0) * pixels] = rate;
-    marked |= rate < 0;
+    double escape = 0;
+    const int code = cell_code_first(light, ph, t.y - t.x, r.x, r.y, serial, shared, escape);
+    if (code == kCodeEscape) {
+      out[(size_t)(c - c0) * pixels] = escape;
+      marked |= escape < 0;
+    }
+    // cb - c0 is a multiple of kCodeCells: runs start at whole groups
+    const int j = (c - c0) % kCodeCells;
+    word |= (uint64_t)code << (8 * j);
+    if (j == kCodeCells - 1 || c == ce - 1) {
+      out_codes[(size_t)((c - c0) / kCodeCells) * pixels] = word;
+      word = 0;
+    }
   }
   return marked;
 }
@@ -673,7 +686,11 @@ __global__ __launch_bounds__(256) void adapt_rates_rare(const SegDev *segs, cons
   if (cb >= ce || !A.accepted || (b.flags[gp] & 1)) return;
   const int i = gp - pixel_begin[si];
   double *out = b.rates + gp;
+  const uint64_t *codes = b.codes + gp;
   for (int c = cb; c < ce; ++c) {
+    // only escaped cells have a double; the first phase marked its draws -lambda
+    const uint64_t w = codes[(size_t)((c - c0) / kCodeCells) * pixels];
+    if (((w >> (8 * ((c - c0) % kCodeCells))) & 0xff) != (uint64_t)kCodeEscape) continue;
     double &r = out[(size_t)(c - c0) * pixels];
     if (r < 0) r = cell_rate_rest(r, g.t1[c] - g.t0[c], g.seed, (uint64_t)i, A.cell_base + (uint64_t)c);
   }
@@ -699,9 +716,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void a
   const double *rates = fallback ? A.fb_rad + b.fb_index[gp] : b.rates + gp;
   const size_t stride = fallback ? (size_t)A.fb_count : (size_t)pixels;
   const int c_end = min(c1, g.cells);
-  const int count = circuit_cells(st, g.params, g.seed, (uint64_t)i, own, rates, stride, g.t0, g.t1, g.maps, c0,
-                                  c_end, g.slots + (size_t)(own ? i : 0) * g.max_events, own ? g.counts[i] : 0,
-                                  g.max_events);
+  // knot pixels read photocurrent codes (their escapes from b.rates); a
+  // fallback pixel's photocurrents are all doubles in A.fb_rad
+  const int count = circuit_cells<true>(st, g.params, g.seed, (uint64_t)i, own, rates, stride, b.codes + gp,
+                                        (size_t)pixels, !fallback, g.t0, g.t1, g.maps, c0, c_end,
+                                        g.slots + (size_t)(own ? i : 0) * g.max_events, own ? g.counts[i] : 0,
+                                        g.max_events);
   if (!own) return;
   if (count < 0) { atomicOr(status, count == -1 ? 1 : 2); g.counts[i] = 0; return; }
   st.shot_block += (uint64_t)max(0, c_end - c0);

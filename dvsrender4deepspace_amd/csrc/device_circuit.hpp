@@ -342,6 +342,40 @@ __device__ __forceinline__ double cell_rate_first(double light, const PhotoParam
   }
   return -lambda;
 }
+// Photocurrent codes.  With shot noise a cell's photocurrent is k / dt for
+// the drawn electron count k, so the adaptive photocurrent passes a byte per
+// cell to the circuit instead of a double: k itself below kCodeEscape, whose
+// photocurrent is 0 for k = 0 and k / dt otherwise (the quotients
+// cell_rate_first returns, dt = t1 - t0 of the cell's window), and
+// kCodeEscape for every other value, which travels as a double beside the
+// codes (no shot noise, a draw left to the second phase, a NaN, dt <= 0).
+// Eight consecutive cells of a pixel share a 64-bit word, cell j in byte j.
+constexpr int kCodeEscape = 255;
+constexpr int kCodeCells = 8;
+__device__ __forceinline__ int cell_code_first(double light, const PhotoParams &ph, double dt, double one_per_dt,
+                                               double two_per_dt, uint64_t cell_serial, const uint64_t shared[4],
+                                               double &escape) {
+  const double rate = light * ph.electrons_per_radiance + ph.dark_electrons;
+  if (!ph.shot_noise) { escape = rate; return kCodeEscape; }
+  const double lambda = rate * dt;
+  if (lambda != lambda) { escape = lambda; return kCodeEscape; }
+  if (lambda <= 0) {
+    if (dt > 0) return 0;
+    escape = 0.0 / dt;
+    return kCodeEscape;
+  }
+  if (lambda < 10) {
+    const double k = poisson_inversion(lambda, shot_uniform(shared, cell_serial));
+    if (dt > 0 && k < kCodeEscape) return (int)k;
+    escape = k == 0 && dt > 0 ? 0.0 : k == 1 ? one_per_dt : k == 2 ? two_per_dt : k / dt;
+    return kCodeEscape;
+  }
+  escape = -lambda;
+  return kCodeEscape;
+}
+// The photocurrent of code k < kCodeEscape in a cell of length dt > 0.
+__device__ __forceinline__ double code_rate(int k, double dt) { return k == 0 ? 0.0 : (double)k / dt; }
+
 // inverse_map over the whole cell, dt = t1 - t0 > 0, rate >= 0.
 __device__ __forceinline__ double inverse_map_cell(double y, double rate, double dt, double charge,
                                                    const CellMaps &m) {
@@ -479,23 +513,43 @@ static_assert(kBlockWords <= 128, "a block's maps fit two words per lane");
 // block has no crossing, no hold end, no pending reset and tabulated
 // photocurrents only (nearly all of them) keeps that result, which is what
 // the cell-by-cell path gives; the others run the block again cell by cell.
+//
+// CODED: the photocurrents arrive as codes (cell_code_first), word g of the
+// pixel's cells c_begin + 8 g .. c_begin + 8 g + 7 at codes[g * code_stride], and
+// escaped cells at rates[(c - c_begin) * stride] as above; coded = false
+// takes every cell as escaped (the pixel has no codes).  The decoded
+// photocurrent is the double the uncoded path would read.
+typedef const __attribute__((address_space(1))) uint64_t *GlobalWords;
+template <bool CODED>
 __device__ inline int circuit_cells(dvs_pixel_state &st, const CircuitParams &p, uint64_t seed, uint64_t pixel,
-                                    bool active, const double *rates, size_t stride, const double *t0,
-                                    const double *t1, const CellMaps *maps, int c_begin, int c_end,
-                                    CrossingEvent *out_events, int count, int capacity) {
+                                    bool active, const double *rates, size_t stride, const uint64_t *codes,
+                                    size_t code_stride, bool coded, const double *t0, const double *t1, const CellMaps *maps,
+                                    int c_begin, int c_end, CrossingEvent *out_events, int count, int capacity) {
   const GlobalDoubles R = (GlobalDoubles)rates, T0 = (GlobalDoubles)t0, T1 = (GlobalDoubles)t1;
+  const GlobalWords C = (GlobalWords)codes;
   const GlobalEvents out = (GlobalEvents)out_events;
   NoiseStream reset(seed, pixel, kStreamReset, st.reset_block);
   if (c_end <= c_begin) return count;
+  // code of cell c (CODED): its byte of the word of its group of eight
+  auto code_of = [&](int c) -> int {
+    const uint64_t w = coded ? C[(size_t)((c - c_begin) / kCodeCells) * code_stride] : ~0ull;
+    return (int)((w >> (8 * ((c - c_begin) % kCodeCells))) & 0xff);
+  };
   if (p.reference_leak > 0) {
     if (!active) return 0;
     auto normal = [&]() { return reset.normal(); };
     PixelRegs s{st.receptor_log, st.reference_log, st.refractory_until, st.threshold_pos, st.threshold_neg,
                 st.reset_pending};
-    double next = R[0];
+    double next = 0;
+    if (!CODED) next = R[0];
     for (int c = c_begin; c < c_end; ++c) {
-      const double rate = next;
-      if (c + 1 < c_end) next = R[(size_t)(c + 1 - c_begin) * stride];
+      double rate = next;
+      if (CODED) {
+        const int k = code_of(c);
+        rate = k == kCodeEscape ? R[(size_t)(c - c_begin) * stride] : code_rate(k, T1[c] - T0[c]);
+      } else if (c + 1 < c_end) {
+        next = R[(size_t)(c + 1 - c_begin) * stride];
+      }
       if (!isfinite(rate)) return -2;
       const int emitted = advance_pixel(s, p, rate, T0[c], T1[c], out_events + count, capacity - count, normal);
       if (emitted < 0) return -1;
@@ -528,14 +582,26 @@ __device__ inline int circuit_cells(dvs_pixel_state &st, const CircuitParams &p,
   // in flight exactly.
   auto word = [&](int cb, int k) { return W[min((long long)cb * kCellWords + k, last_word)]; };
   auto rate_of = [&](int c) { return R[(size_t)(min(c, c_end - 1) - c_begin) * stride]; };
+  // the code word of the block at cb (CODED; all escapes without codes)
+  auto codes_of = [&](int cb) -> uint64_t {
+    const uint64_t w = C[(size_t)((min(cb, c_end - 1) - c_begin) / kCodeCells) * code_stride];
+    return coded ? w : ~0ull;
+  };
+  static_assert(kCellPrefetch == kCodeCells, "a block is one code word");
   // One block of cells from cb: integrates it with the photocurrents `use`
-  // and map words u0/u1 (loaded one block earlier) while loading the next
-  // block's into `fill` and f0/f1.  The loop below alternates two register
-  // sets, so no value moves between them: a move would wait for the load.
-  auto block = [&](const double (&use)[kCellPrefetch], double u0, double u1, double (&fill)[kCellPrefetch],
-                   double &f0, double &f1, int cb) __attribute__((always_inline)) {
+  // (CODED: the code word `use_codes`) and map words u0/u1 (loaded one block
+  // earlier) while loading the next block's into `fill` (`fill_codes`) and
+  // f0/f1.  The loop below alternates two register sets, so no value moves
+  // between them: a move would wait for the load.
+  auto block = [&](const double (&use)[kCellPrefetch], uint64_t use_codes, double u0, double u1,
+                   double (&fill)[kCellPrefetch], uint64_t &fill_codes, double &f0, double &f1,
+                   int cb) __attribute__((always_inline)) {
+    if (CODED) {
+      fill_codes = codes_of(cb + kCellPrefetch);
+    } else {
 #pragma unroll
-    for (int j = 0; j < kCellPrefetch; ++j) fill[j] = rate_of(cb + kCellPrefetch + j);
+      for (int j = 0; j < kCellPrefetch; ++j) fill[j] = rate_of(cb + kCellPrefetch + j);
+    }
     f0 = word(cb + kCellPrefetch, lane);
     f1 = word(cb + kCellPrefetch, 64 + lane);
     // stage this block's maps; LDS operations of a wave complete in order
@@ -557,10 +623,16 @@ __device__ inline int circuit_cells(dvs_pixel_state &st, const CircuitParams &p,
     int k[kCellPrefetch];
 #pragma unroll
     for (int j = 0; j < kCellPrefetch; ++j) {
-      const double rate = use[j];
-      const bool one = rate == r[j].x, two = rate == r[j].y;
-      k[j] = one ? 1 : two ? 2 : 0;
-      ok = ok && (j >= n || one || two || rate == 0);
+      if (CODED) {
+        const int code = (int)((use_codes >> (8 * j)) & 0xff);
+        k[j] = code <= kMappedElectrons ? code : 0;
+        ok = ok && (j >= n || code <= kMappedElectrons);
+      } else {
+        const double rate = use[j];
+        const bool one = rate == r[j].x, two = rate == r[j].y;
+        k[j] = one ? 1 : two ? 2 : 0;
+        ok = ok && (j >= n || one || two || rate == 0);
+      }
     }
 #pragma unroll
     for (int j = 0; j < kCellPrefetch; ++j) aq[j] = *reinterpret_cast<const double2 *>(L + j * kCellWords + 4 + 2 * k[j]);
@@ -592,7 +664,13 @@ __device__ inline int circuit_cells(dvs_pixel_state &st, const CircuitParams &p,
             m.map[e][0] = w[2 + kMappedElectrons + 2 * e];
             m.map[e][1] = w[3 + kMappedElectrons + 2 * e];
           }
-          const double rate = use[j];
+          double rate;
+          if (CODED) {
+            const int code = (int)((use_codes >> (8 * j)) & 0xff);
+            rate = code == kCodeEscape ? R[(size_t)(cb + j - c_begin) * stride] : code_rate(code, m.t1 - m.t0);
+          } else {
+            rate = use[j];
+          }
           if (!isfinite(rate)) {
             code = -2;
             active = false;
@@ -610,12 +688,17 @@ __device__ inline int circuit_cells(dvs_pixel_state &st, const CircuitParams &p,
     }
   };
   double ra[kCellPrefetch], rb[kCellPrefetch];
+  uint64_t ca = 0, cbw = 0;
+  if (CODED) {
+    ca = codes_of(c_begin);
+  } else {
 #pragma unroll
-  for (int j = 0; j < kCellPrefetch; ++j) ra[j] = rate_of(c_begin + j);
+    for (int j = 0; j < kCellPrefetch; ++j) ra[j] = rate_of(c_begin + j);
+  }
   double a0 = word(c_begin, lane), a1 = word(c_begin, 64 + lane), b0 = 0, b1 = 0;
   for (int cb = c_begin; cb < c_end; cb += 2 * kCellPrefetch) {
-    block(ra, a0, a1, rb, b0, b1, cb);
-    if (cb + kCellPrefetch < c_end) block(rb, b0, b1, ra, a0, a1, cb + kCellPrefetch);
+    block(ra, ca, a0, a1, rb, cbw, b0, b1, cb);
+    if (cb + kCellPrefetch < c_end) block(rb, cbw, b0, b1, ra, ca, a0, a1, cb + kCellPrefetch);
   }
   if (code < 0) return code;
   st.receptor_inverse = s.y;

@@ -254,8 +254,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
   // the wave's shared tiles) without a pixel of their own
   const bool own = i < g.npix;
   dvs_pixel_state st = own ? g.states[i] : dvs_pixel_state{};
-  const int count = circuit_cells(st, g.params, g.seed, (uint64_t)i, own, g.rad + (own ? i : 0), (size_t)g.npix,
-                                  g.t0, g.t1, g.maps, 0, g.cells, g.slots + (size_t)(own ? i : 0) * g.max_events,
+  const int count = circuit_cells<false>(st, g.params, g.seed, (uint64_t)i, own, g.rad + (own ? i : 0),
+                                         (size_t)g.npix, nullptr, 0, false, g.t0, g.t1, g.maps, 0, g.cells, g.slots + (size_t)(own ? i : 0) * g.max_events,
                                   own ? g.counts[i] : 0, g.max_events);
   if (!own) return;
   if (count < 0) { atomicOr(status, count == -1 ? 1 : 2); g.counts[i] = 0; return; }

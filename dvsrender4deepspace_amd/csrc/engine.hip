@@ -1165,6 +1165,8 @@ class Engine {
     const int chunk = (int)std::max<size_t>(1, chunk_cap);
     rates_.reserve((size_t)chunk * P, "adaptive photocurrent");
     b.rates = rates_.ptr;
+    codes_.reserve((size_t)((chunk + dvsdev::kCodeCells - 1) / dvsdev::kCodeCells) * P, "adaptive photocurrent");
+    b.codes = codes_.ptr;
     // per-wave flags of the knot pixels' and the fallback pixels' photocurrent
     const size_t rare_knots = adapt_rates_flags(P, chunk), rare_fallback = cell_rates_flags(fb_total, chunk);
     rare_adapt_.reserve(rare_knots, "photocurrent flags");
@@ -1288,6 +1290,7 @@ class Engine {
       fb_items_;
   DeviceArray<unsigned> kbucket_;  // per-pixel segment words of the knot order
   DeviceArray<double> cell_frac_, qrad_, erad_, gate_rad_, gate_err_, rates_;
+  DeviceArray<uint64_t> codes_;
   // probe radiance and end poses of the last two adaptive windows, and the
   // segment layout (id, pixels) of the last one
   DeviceArray<double> probes_[2];
