@@ -4,33 +4,36 @@
 Workload: the single DAVIS346 request.
 - 346x260 sensor, fx=fy=300, 4x4 sub-rays per pixel.
 - 1024^2 heightfield (2,093,058 triangles, 200 craters).
-- 100 fps trajectory, 10 us host horizons of 1 us circuit cells.
+- 100 fps trajectory (250 frames, 2.49 s), 10 us host horizons of 1 us
+  circuit cells, sun 30 deg above the horizon at intensity 1000.
 - Shot and reset noise on (Philox seed).
 - Adaptive optics under bin/dvs's solver policy (apps/main_dvs.cpp:96-116):
   windows of 200 horizons, the receptor check at the first window and every
-  0.25 s.
+  0.25 s, a rejected window halved.
 
-One step is one driver call as bin/dvs makes it: the window's trajectory
-states (CameraTrajectory over the flight's 100 fps frames, evaluated on the
-device) and EventSensor::process_adaptive over the window.  It covers the
-refinement probes and bisection, local exact cells, the photocurrent with
-shot noise, the FP64 circuit of 2,000 cells x 89,960 pixels, device
-compaction/sort and the readout FIFO.  Warm-up windows run on
-a separate sensor; the timed sensor starts at t=0.  Inputs (BVH, mesh, pixel
-state) are resident in HBM before timing starts.
+One step is one twentieth of the flight: the driver windows (bin/dvs calls)
+that start in that twentieth, each with its trajectory states evaluated on
+the device (CameraTrajectory over the flight's frames) and
+EventSensor::process_adaptive: refinement probes and bisection, local exact
+cells, the photocurrent with shot noise, the FP64 circuit of every pixel,
+device compaction/sort and the readout FIFO.  `--steps 20` times the whole
+flight; more steps fly it again on a re-initialised sensor.  Warm-up steps
+run on a separate sensor.  Inputs (BVH, mesh, pixel state) are resident in
+HBM before timing starts.
 
-Prints one JSON line: comparator events/s (value); Mrays/s traced and
-native-equivalent; trajectory seconds per wall second; per-window p50/p99
-latency; the roofline of the dominant kernel (the serial circuit).  Next to
-them: the native solver on the same request (Mrays/s and the cell-trace
-roofline, with algorithmic bytes per ray from the product's traversal
-counters) and the CPU oracle timed on the host on the request's first
-window.  `--mode served|batched` runs the served and batched forms against
-the plain per-request path.
+Prints one JSON line: comparator events/s over the timed steps (value);
+Mrays/s traced and native-equivalent; trajectory seconds per wall second;
+per-window p50/p99 latency; the roofline of the dominant kernel (the
+refinement's probe trace, VALU instruction issue) and of
+the other kernels of a window; the native solver on the same request; the
+CPU oracle and the GPU timed on the same first windows; and the served
+form against the plain per-request path on a stated request mix.
+`--mode served|batched` runs the full served and batched forms alone.
 """
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -44,7 +47,37 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "Mrays/s + events/s at DAVIS346; p50/p99 latency, served vs plain, 1 GPU"
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+# Wave64 VALU instructions per second at the published FP32 vector peak
+# (157.3 TFLOP/s / (64 lanes x 2 flops per FMA)).  32-bit integer VALU
+# instructions reach 1,150 G/s on the box (a microbenchmark of independent
+# chains); FP32 and FP64 FMAs without packed math issue at half that rate.
+VALU_ISSUE_PEAK_G = 157.3e12 / 128 / 1e9
 PAIR_BYTES, LEAF_BYTES, TRIANGLE_BYTES = 64, 8, 48
+# FP32 operations of the traversal: a node-pair slab test (12 subtractions,
+# 12 multiplications), a Moller-Trumbore test (46 adds/multiplies/one divide
+# on its full path), and the inverse direction of a ray (3 divides)
+PAIR_FLOPS, TRIANGLE_FLOPS, RAY_FLOPS = 24, 46, 3
+SLICES = 20
+
+# Per-unit counter figures of this build's kernels on this workload, from
+# rocprofv3 --pmc runs of `python3 bench.py --steps 1 --warmup 0
+# --no-cpu-baseline --no-served` (the flight's first twentieth; the
+# summary with its commands is profiles/adaptive_window_davis346_counters.md):
+# VALU instructions (SQ_INSTS_VALU), VALUBusy, lane utilisation
+# (VALUUtilization), L2 hit rate (TCC_HIT / (TCC_HIT + TCC_MISS)) and HBM
+# bytes (FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0: a 1 GiB device copy reads
+# FETCH_SIZE 0.5 GiB and WRITE_SIZE 1 GiB), per unit of each kernel's work.
+PROFILE = {
+    "probe": {"unit": "traced pixel sample", "valu_insts": 1914.68, "valu_busy": 1.072, "lane_util": 0.78,
+              "l2_hit": 0.571, "fetch": 156.61, "write": 19.65},
+    "query": {"unit": "traced pixel sample", "valu_insts": 2226.60, "valu_busy": 0.52, "lane_util": 0.655,
+              "l2_hit": 0.607, "fetch": 367.75, "write": 10.0},
+    "rates": {"unit": "pixel cell", "valu_insts": 2.5555, "valu_busy": 0.898, "lane_util": 0.868,
+              "l2_hit": 0.827, "fetch": 0.0723, "write": 1.004},
+    "integrate": {"unit": "pixel cell", "valu_insts": 0.4639, "valu_busy": 0.27, "lane_util": 0.577,
+                  "l2_hit": 0.765, "fetch": 1.1049, "write": 0.0366},
+}
 
 
 def log(msg: str) -> None:
@@ -61,10 +94,47 @@ def bytes_per_ray(stats: dict) -> tuple[float, float]:
     return per("primary"), per("shadow")
 
 
-def cpu_baseline(w, mesh, horizons: int, ivs) -> dict:
-    """The CPU oracle (restatement of the reference path) on the host cores:
-    the request's first adaptive window (intervals `ivs`), checked by the
-    receptor gate as the driver checks it."""
+def flops_per_ray(stats: dict) -> tuple[float, float]:
+    def per(prefix):
+        rays = max(stats[f"{prefix}_rays"], 1)
+        return (stats[f"{prefix}_pairs"] * PAIR_FLOPS + stats[f"{prefix}_triangles"] * TRIANGLE_FLOPS) / rays + \
+            RAY_FLOPS
+    return per("primary"), per("shadow")
+
+
+def valu_roofline(kernel: str, key: str, launch_ms: float, units_per_launch: float, extra: dict) -> dict:
+    """VALU-issue-bound kernel: wave64 VALU instructions per launch (the
+    profile's per-unit count x the launch's units, counted live) over the
+    launch's live HIP-event duration, against VALU_ISSUE_PEAK_G."""
+    prof = PROFILE[key]
+    achieved = prof["valu_insts"] * units_per_launch / max(launch_ms * 1e-3, 1e-30) / 1e9
+    out = {"bound": "valu", "kernel": kernel, "achieved": round(achieved, 2), "peak": round(VALU_ISSUE_PEAK_G, 1),
+           "unit": "G VALU instr/s", "frac": round(achieved / VALU_ISSUE_PEAK_G, 4),
+           "traffic": int((prof["fetch"] + prof["write"]) * units_per_launch),
+           "avg_launch_ms": round(launch_ms, 4), "units_per_launch": round(units_per_launch, 1),
+           "unit_of_work": prof["unit"], "valu_instr_per_unit": prof["valu_insts"],
+           "valu_busy": prof["valu_busy"], "lane_utilisation": prof["lane_util"], "l2_hit_rate": prof["l2_hit"]}
+    out.update(extra)
+    return out
+
+
+def hbm_roofline(kernel: str, key: str, launch_ms: float, bytes_per_launch: float, units_per_launch: float,
+                 extra: dict) -> dict:
+    prof = PROFILE[key]
+    achieved = bytes_per_launch / max(launch_ms * 1e-3, 1e-30) / 1e9
+    out = {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": int((prof["fetch"] + prof["write"]) * units_per_launch),
+           "avg_launch_ms": round(launch_ms, 4), "bytes_per_launch": int(bytes_per_launch),
+           "valu_busy": prof["valu_busy"], "lane_utilisation": prof["lane_util"], "l2_hit_rate": prof["l2_hit"]}
+    out.update(extra)
+    return out
+
+
+def cpu_baseline(w, mesh, windows: list, gpu_windows: dict) -> dict:
+    """The CPU oracle (restatement of the reference path) on the host cores
+    over the request's first driver windows `windows` (intervals, and
+    whether the receptor check ran), from the same initial state as the
+    GPU; `gpu_windows` is the GPU's time and events on the same windows."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle_ctypes as oracle
 
@@ -72,22 +142,30 @@ def cpu_baseline(w, mesh, horizons: int, ivs) -> dict:
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     scene = oracle.Scene(nodes, root, v, n, idx, w.camera, w.hapke)
     sensor = oracle.Sensor(w.camera.width, w.camera.height, w.physic)
-    sensor.initialize(scene, ivs["start"][:1], w.start_time, threads)
-    t0 = time.perf_counter()
-    code, ok, raw, _, st, _ = sensor.process_adaptive(scene, ivs, tolerance=1e-2, check=True, capacity=512,
-                                                      threads=threads)
-    if code == 0 and not ok:
-        code, raw, _ = sensor.process(scene, ivs, threads)
-    wall = time.perf_counter() - t0
-    if code:
-        raise RuntimeError(f"oracle failed with status {code}")
-    sim = float(ivs["t1"][-1] - ivs["t0"][0])
-    return {"value": round(len(raw) / wall, 1), "unit": "events/s", "cores": threads, "kind": "port",
-            "trajectory_s_per_wall_s": round(sim / wall, 6),
-            "sample": f"the request's first {horizons}-horizon window from t=0 on the full "
-                      f"{w.camera.width}x{w.camera.height} sensor with the receptor check, initialisation "
-                      f"excluded ({'accepted' if ok else 'rejected, then native'}): {len(raw)} events in "
-                      f"{wall:.2f} s; refinement {st}"}
+    sensor.initialize(scene, windows[0][0]["start"][:1], w.start_time, threads)
+    events, wall = 0, 0.0
+    notes = []
+    for ivs, check in windows:
+        t0 = time.perf_counter()
+        code, ok, raw, _, st, _ = sensor.process_adaptive(scene, ivs, tolerance=1e-2, check=check, capacity=512,
+                                                          threads=threads)
+        if code == 0 and not ok:
+            code, raw, _ = sensor.process(scene, ivs, threads)
+        wall += time.perf_counter() - t0
+        if code:
+            raise RuntimeError(f"oracle failed with status {code}")
+        events += len(raw)
+        notes.append(f"{len(ivs)} horizons{' checked' if check else ''}{'' if ok else ' (rejected, native)'}: "
+                     f"{len(raw)} events")
+    sim = float(windows[-1][0]["t1"][-1] - windows[0][0]["t0"][0])
+    return {"value": round(events / wall, 1), "unit": "events/s", "cores": threads, "kind": "port",
+            "wall_s": round(wall, 3), "events": events, "trajectory_s_per_wall_s": round(sim / wall, 6),
+            "sample": f"the request's first {len(windows)} driver windows from t=0 on the full "
+                      f"{w.camera.width}x{w.camera.height} sensor ({'; '.join(notes)}), initialisation excluded",
+            "gpu_same_sample": {"value": round(gpu_windows["events"] / gpu_windows["wall_s"], 1),
+                                "unit": "events/s", "events": gpu_windows["events"],
+                                "wall_s": round(gpu_windows["wall_s"], 4),
+                                "trajectory_s_per_wall_s": round(sim / gpu_windows["wall_s"], 4)}}
 
 
 def _latency(reqs) -> dict:
@@ -110,24 +188,24 @@ def _tune(serve, geometries, reqs, adaptive: bool) -> dict:
     return tune["best"]
 
 
-def served_mode(args) -> dict:
-    """Served form: Poisson request arrivals over two resident terrains,
-    continuous batching, HBM-footprint admission; against the plain
-    per-request path (scene built and uploaded per request) and the plain path
-    with the scenes kept resident.  Same requests, same seeds."""
-    import dataclasses
-
-    from dvsrender4deepspace_amd import capi, serve, terrain, workloads
+def served_compare(args, requests: int, horizons: tuple) -> dict:
+    """Served form against the plain per-request path on the same requests:
+    Poisson arrivals at args.rate over two terrains (512^2 and 1024^2),
+    resolutions 64x48 / 173x130 / 346x260, per-request seed, trajectory and
+    noise flag, continuous batching under the HBM admission budget.  The
+    plain path runs each request alone as bin/dvs does (BVH built and
+    uploaded per request), and again with the scenes kept resident."""
+    from dvsrender4deepspace_amd import serve, terrain, workloads
 
     specs = {"terrain512": terrain.TerrainSpec(size=512, amplitude=20.0, craters=100, seed=512),
              "terrain1024": workloads.DAVIS346.terrain}
     base = workloads.DAVIS346
     t = time.perf_counter()
     meshes = serve.terrain_meshes(specs)
-    log(f"meshes in {time.perf_counter() - t:.1f} s")
+    log(f"served mix: meshes in {time.perf_counter() - t:.1f} s")
     adaptive = args.solver == "adaptive"
-    reqs = serve.request_mix(base, args.requests, args.rate, (args.min_horizons, args.max_horizons), specs,
-                             seed=args.seed, speed=tuple(args.speed), adaptive=adaptive)
+    reqs = serve.request_mix(base, requests, args.rate, horizons, specs, seed=args.seed, speed=tuple(args.speed),
+                             adaptive=adaptive)
     geometries = {k: serve.build_scene(meshes[k], base, base.camera) for k in specs}
     best = _tune(serve, geometries, reqs, adaptive)
     served = [dataclasses.replace(r, events=[]) for r in reqs]
@@ -144,10 +222,8 @@ def served_mode(args) -> dict:
     events = sum(len(r.event_array()) for r in served)
     noise_off = [i for i, r in enumerate(reqs) if not r.workload.physic.photon_shot_noise]
     return {
-        "metric": METRIC, "value": round(events / wall_served, 1), "unit": "events/s", "n_gpus": 1,
-        "higher_is_better": True, "data": "synthetic", "mode": "served",
         "config": {"workload": "served mix", "requests": len(reqs), "arrival_rate_hz": args.rate,
-                   "horizons": [args.min_horizons, args.max_horizons], "resolutions": serve.RESOLUTIONS,
+                   "horizons": list(horizons), "resolutions": serve.RESOLUTIONS,
                    "terrains": {k: v.triangles for k, v in specs.items()}, "admission_budget_gb": args.budget_gb,
                    "supersample": base.camera.samples, "solver": args.solver, "autotuned": best},
         "events": events,
@@ -164,13 +240,17 @@ def served_mode(args) -> dict:
     }
 
 
+def served_mode(args) -> dict:
+    out = served_compare(args, args.requests, (args.min_horizons, args.max_horizons))
+    return {"metric": METRIC, "value": out["served"]["events_per_s"], "unit": "events/s", "n_gpus": 1,
+            "higher_is_better": True, "data": "synthetic", "mode": "served", **out}
+
+
 def batched_mode(args) -> dict:
     """Batched form: 64 segments of uneven size over a shared 1024^2 terrain
     advanced together (graph-replayed steps, autotuned step size), checked
     against the same segments run one by one."""
-    import dataclasses
-
-    from dvsrender4deepspace_amd import serve, terrain, workloads
+    from dvsrender4deepspace_amd import serve, workloads
 
     base = workloads.DAVIS346
     specs = {"terrain1024": base.terrain}
@@ -204,8 +284,8 @@ def batched_mode(args) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--mode", choices=["request", "served", "batched"], default="request",
-                    help="request: the single DAVIS346 request (default bench line); served / batched: the "
-                         "served and batched forms against the plain per-request path")
+                    help="request: the single DAVIS346 request with the served comparison (default bench line); "
+                         "served / batched: the full served and batched forms alone")
     ap.add_argument("--requests", type=int, default=60)
     ap.add_argument("--segments", type=int, default=64)
     ap.add_argument("--rate", type=float, default=20.0)
@@ -218,29 +298,26 @@ def main() -> int:
     ap.add_argument("--speed", type=float, nargs=2, default=(100.0, 400.0),
                     help="ground speed range (m/s) of the served/batched mix")
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40, help="timed driver windows (request mode)")
+    ap.add_argument("--steps", type=int, default=SLICES, help="timed twentieths of the flight (request mode)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="davis346-terrain1024-ss4")
     ap.add_argument("--horizons", type=int, default=64, help="host horizons per native-solver step")
+    ap.add_argument("--cpu-windows", type=int, default=3, help="driver windows of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-served", action="store_true", help="skip the served-vs-plain comparison")
+    ap.add_argument("--served-requests", type=int, default=30)
+    ap.add_argument("--served-horizons", type=int, nargs=2, default=(2000, 6000))
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (calibrates rocprofv3 byte counters)")
     args = ap.parse_args()
     if args.copy_probe:
-        import torch
-        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        torch.cuda.synchronize()
-        b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes_read": a.numel() * 4, "copy_probe_bytes_written": b.numel() * 4}))
-        return 0
+        return copy_probe()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU layout "
               "(pixel-row shards, event merge on rank 0) is not launched in this build", file=sys.stderr)
         return 2
 
-    from dvsrender4deepspace_amd import capi, terrain, workloads
+    from dvsrender4deepspace_amd import capi
 
     capi.load_library()
     if capi.device_count() < 1:
@@ -252,10 +329,64 @@ def main() -> int:
     return 0
 
 
+def copy_probe() -> int:
+    """A 1 GiB device-to-device copy through the library's HIP runtime."""
+    from dvsrender4deepspace_amd import capi
+
+    n = 1 << 30
+    a, b = capi.DeviceBuffer(n), capi.DeviceBuffer(n)
+    hip = capi.hip_runtime()
+    hip.hipMemcpy(b.ptr, a.ptr, n, 3)  # device to device
+    hip.hipDeviceSynchronize()
+    print(json.dumps({"copy_probe_bytes_read": n, "copy_probe_bytes_written": n}))
+    return 0
+
+
+class Flight:
+    """bin/dvs's window loop over the flight on one sensor (apps/main_dvs.cpp:
+    96-116): windows of `batch` horizons along the flight's CameraTrajectory,
+    the receptor check at the start and every 0.25 s, a rejected window
+    halved (the smaller window is kept), a rejected one-horizon window
+    integrated natively."""
+
+    def __init__(self, w, traj, sensor, batch):
+        from dvsrender4deepspace_amd import driver
+        self.w, self.traj, self.sensor, self.batch, self.driver = w, traj, sensor, batch, driver
+        self.step, self.t_prev, self.next_check = 0, w.start_time, w.start_time
+        self.windows = []  # (seconds, comparator events, intervals, checked)
+
+    def window(self):
+        w, d = self.w, self.driver
+        from dvsrender4deepspace_amd import workloads
+        t0 = time.perf_counter()
+        while True:
+            ivs = workloads.horizon_intervals(w, self.traj, self.t_prev, self.step, self.batch)
+            check = float(ivs["t0"][0]) >= self.next_check
+            ok, ready = self.sensor.process_adaptive(ivs, d.DEFAULT_TOLERANCE, check)
+            if ok:
+                break
+            if self.batch > 1:
+                self.batch = max(1, self.batch // 2)
+                continue
+            ready = self.sensor.process(ivs)
+            break
+        if check:
+            self.next_check = float(ivs["t1"][-1]) + d.CHECK_PERIOD
+        dt = time.perf_counter() - t0
+        self.step += len(ivs)
+        self.t_prev = float(ivs["t1"][-1])
+        self.windows.append((dt, self.sensor.stats()["comparator_events"], ivs, check))
+        return dt
+
+    def run_until(self, horizon: int):
+        while self.step < min(horizon, self.w.total_steps):
+            self.window()
+
+
 def request_mode(args) -> dict:
-    """The single DAVIS346 request: adaptive optics under the driver's window
-    policy (the headline line), then a short native-solver run of the same
-    request for the cell-trace roofline, then the CPU oracle on a sample."""
+    """The single DAVIS346 request over its whole flight (the headline line),
+    the native solver on the same request, the CPU oracle on the first
+    windows, and the served comparison."""
     from dvsrender4deepspace_amd import capi, driver, terrain, workloads
 
     w = workloads.WORKLOADS[args.workload]
@@ -265,71 +396,90 @@ def request_mode(args) -> dict:
     log(f"scene: {len(idx)} triangles, {len(nodes)} BVH nodes, built in {time.perf_counter() - t:.1f} s")
     scene = capi.Scene(nodes, root, v, n, leaf_idx, w.camera, w.hapke)
     info = scene.info()
-    trav = scene.traversal_stats(workloads.horizon_intervals(w, workloads.trajectory(w), w.start_time, 0, 1)
-                                 ["start"][:1])
-    b_primary, b_shadow = bytes_per_ray(trav)
-    log(f"scene on device: {info}; bytes/ray primary {b_primary:.0f}, shadow {b_shadow:.0f}")
-    npix = w.camera.width * w.camera.height
-    R = w.camera.samples ** 2
-
-    # --- adaptive optics: one step = one driver call (window of `batch`
-    # horizons) flown along the flight's CameraTrajectory (frames at the
-    # workload's fps), the window's states evaluated on the device
-    batch = driver.batch_size(w.physic, w.time_resolution, adaptive=True)
     traj = workloads.trajectory(w)
     first = workloads.horizon_intervals(w, traj, w.start_time, 0, 1)
     initial = first["start"][:1]
+    trav = scene.traversal_stats(initial)
+    b_primary, b_shadow = bytes_per_ray(trav)
+    f_primary, f_shadow = flops_per_ray(trav)
+    log(f"scene on device: {info}; per ray: primary {b_primary:.0f} B {f_primary:.0f} flops, "
+        f"shadow {b_shadow:.0f} B {f_shadow:.0f} flops")
+    npix = w.camera.width * w.camera.height
+    R = w.camera.samples ** 2
+    batch = driver.batch_size(w.physic, w.time_resolution, adaptive=True)
+    H = w.total_steps
+    bounds = [k * H // SLICES for k in range(SLICES + 1)]
 
-    def windows(sensor, count, timed):
-        step, next_check, hz, lat, ready_n, rejected = 0, w.start_time, batch, [], 0, 0
-        done, t_prev = 0, w.start_time
-        while done < count:
-            t0 = time.perf_counter()
-            ivs = workloads.horizon_intervals(w, traj, t_prev, step, hz)
-            check = float(ivs["t0"][0]) >= next_check
-            ok, ready = sensor.process_adaptive(ivs, driver.DEFAULT_TOLERANCE, check)
-            if not ok:
-                rejected += 1
-                if hz > 1:
-                    hz = max(1, hz // 2)
-                    continue
-                ready = sensor.process(ivs)
-            if check:
-                next_check = float(ivs["t1"][-1]) + driver.CHECK_PERIOD
-            lat.append(time.perf_counter() - t0)
-            ready_n += len(ready)
-            step += len(ivs)
-            t_prev = float(ivs["t1"][-1])
-            done += 1
-            if timed and done % 10 == 0:
-                log(f"window {done}: {lat[-1] * 1e3:.1f} ms, {len(ready)} events delivered")
-        return step, lat, ready_n, rejected
-
+    # warm-up slices on a separate sensor
     warm = capi.Sensor(scene, w.physic)
     warm.initialize(initial, w.start_time)
-    windows(warm, max(args.warmup, 1), False)
+    wf = Flight(w, traj, warm, batch)
+    for k in range(args.warmup):
+        wf.run_until(bounds[(k % SLICES) + 1])
     warm.close()
-    log(f"warmup done ({args.warmup} windows on a separate sensor)")
+    log(f"warmup done ({args.warmup} twentieths of the flight on a separate sensor)")
+
     sensor = capi.Sensor(scene, w.physic)
     sensor.initialize(initial, w.start_time)
     sensor.reset_stats()
+    flights = []
+    flight = Flight(w, traj, sensor, batch)
     t_start = time.perf_counter()
-    horizons, latencies, delivered, rejected = windows(sensor, args.steps, True)
+    for k in range(args.steps):
+        if k and k % SLICES == 0:  # the next flight, from a re-initialised sensor
+            flights.append(flight)
+            sensor.initialize(initial, w.start_time)
+            flight = Flight(w, traj, sensor, batch)
+        flight.run_until(bounds[(k % SLICES) + 1])
+        if (k + 1) % 5 == 0:
+            log(f"step {k + 1}: t = {flight.t_prev:.3f} s, {len(flight.windows)} windows, "
+                f"{sensor.stats()['comparator_events']} events")
     wall = time.perf_counter() - t_start
+    flights.append(flight)
     st = sensor.stats()
     refine = sensor.refinement_stats()
+    windows = [x for f in flights for x in f.windows]
     traced = st["primary_rays"] + st["shadow_rays"]
+    horizons = sum(f.step for f in flights)
     sim_s = horizons * float(np.float32(w.time_resolution))
-    cells = horizons * int(round(float(np.float32(w.time_resolution)) / float(np.float32(w.physic.circuit_step))))
-    lat = np.array(latencies) * 1e3
-    # dominant kernel: the serial circuit (adapt_integrate), HIP-event timed
-    launches = max(st["circuit_launches"], 1)
-    circuit_s = st["circuit_ms"] / 1e3
+    lat = np.array([x[0] for x in windows]) * 1e3
+    cells_per_window = int(round(batch * float(np.float32(w.time_resolution)) / float(np.float32(w.physic.circuit_step))))
+
+    # roofline of the dominant kernel (the probe trace), then the others
+    probe_launches = max(st["probe_launches"], 1)
+    probe_ms = st["probe_ms"] / probe_launches
+    probe_units = st["probe_samples"] / probe_launches
+    shadow_frac = st["shadow_rays"] / max(st["primary_rays"], 1)
+    traversal_flops = probe_units * R * (f_primary + shadow_frac * f_shadow)
+    roofline = valu_roofline("adapt_trace (five initial probes per pixel)", "probe", probe_ms, probe_units, {
+        "fp32_flops": {"algorithmic_per_launch": int(traversal_flops),
+                       "achieved_tflops": round(traversal_flops / (probe_ms * 1e-3) / 1e12, 3),
+                       "peak_tflops": FP32_PEAK_TFLOPS,
+                       "frac": round(traversal_flops / (probe_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
+                       "per_ray": {"primary": round(f_primary, 1), "shadow": round(f_shadow, 1)},
+                       "note": "slab tests and triangle tests only: compares, address arithmetic, stack "
+                               "and branch instructions, which set the issue time, are not FLOPs"},
+        "algorithmic_bytes_per_ray": {"primary": round(b_primary, 1), "shadow": round(b_shadow, 1),
+                                      "note": "the 186 MB scene is served from L2 and MALL, so the algorithmic "
+                                              "bytes are not the HBM traffic (traffic: counted HBM bytes)"}})
+    q_launches = max(st["query_launches"], 1)
+    secondary = [
+        valu_roofline("adapt_trace (bisection midpoints)", "query", st["query_ms"] / q_launches,
+                      st["query_samples"] / q_launches, {"launches_per_window": round(st["query_launches"] /
+                                                                                        max(len(windows), 1), 2)}),
+    ]
+    circuit_launches = max(st["circuit_launches"], 1)
+    cells = st["pixel_cells"] / circuit_launches
     state_bytes = capi.PIXEL_STATE_DTYPE.itemsize
-    bytes_integrate = (npix * cells * 8            # photocurrent in, one double per pixel cell
-                       + st["circuit_launches"] * npix * (2 * state_bytes + 2 * 4)  # pixel state and counts in/out
-                       + st["comparator_events"] * 16)                               # crossings out
-    achieved = bytes_integrate / max(circuit_s, 1e-30) / 1e9
+    integrate_bytes = cells * 1 + npix * (2 * state_bytes + 2 * 4) + st["comparator_events"] / circuit_launches * 16
+    secondary.append(hbm_roofline("adapt_integrate (FP64 circuit)", "integrate", st["circuit_ms"] / circuit_launches,
+                                  integrate_bytes, cells,
+                                  {"note": "one lane per pixel, cells in series: latency bound; in: one "
+                                           "photocurrent code byte per pixel cell, pixel state; out: state, "
+                                           "crossings (16 B)"}))
+    secondary.append(valu_roofline("adapt_rates (photocurrent, shot noise)", "rates",
+                                   st["rates_ms"] / circuit_launches, cells, {}))
+
     result = {
         "metric": METRIC,
         "value": round(st["comparator_events"] / wall, 1),
@@ -345,33 +495,36 @@ def request_mode(args) -> dict:
         "data": "synthetic",
         "config": {"workload": w.name, "sensor": f"{w.camera.width}x{w.camera.height}",
                    "supersample": w.camera.samples, "terrain_triangles": int(len(idx)),
+                   "flight_s": round(w.end_time - w.start_time, 4), "frames": w.frames, "fps": w.fps,
+                   "step": f"one twentieth of the flight ({H // SLICES} horizons)",
                    "solver": "adaptive-optics", "window_horizons": batch,
                    "tolerance": driver.DEFAULT_TOLERANCE, "check_period_s": driver.CHECK_PERIOD,
                    "time_resolution_s": w.time_resolution, "circuit_step_s": w.physic.circuit_step,
+                   "sun_elevation_deg": w.sun_elevation_deg, "intensity": w.intensity,
                    "noise": "shot+reset on", "bvh_leaf_size": w.leaf_size, "device_leaf_size": 4},
         "events": st["comparator_events"],
-        "delivered_events_per_s": round(delivered / wall, 1),
+        "delivered_events_per_s": round(st["delivered_events"] / wall, 1),
         "mrays_per_s": round(traced / wall / 1e6, 3),
         "native_equivalent_mrays_per_s": round(st["native_equivalent_rays"] / wall / 1e6, 3),
         "trajectory_s_per_wall_s": round(sim_s / wall, 5),
+        "windows": len(windows),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                        "p99": round(float(np.percentile(lat, 99)), 3)},
-        "breakdown_ms": {"refine_and_probes": round(st["trace_ms"], 3), "photocurrent": round(st["rates_ms"], 3),
-                         "circuit": round(st["circuit_ms"], 3), "events": round(st["events_ms"], 3),
-                         "wall": round(wall * 1e3, 3)},
+        "breakdown_ms_per_window": {k: round(v / max(len(windows), 1), 4) for k, v in (
+            ("probe_trace", st["probe_ms"]), ("query_trace", st["query_ms"]),
+            ("refinement_other", st["trace_ms"] - st["probe_ms"] - st["query_ms"]),
+            ("photocurrent", st["rates_ms"]), ("circuit", st["circuit_ms"]), ("events", st["events_ms"]),
+            ("wall", wall * 1e3))},
         "refinement_last_window": refine,
-        "rejected_windows": rejected,
-        "roofline": {"bound": "hbm", "kernel": "adapt_integrate", "achieved": round(achieved, 2),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": None, "avg_launch_ms": round(st["circuit_ms"] / launches, 3),
-                     "bytes_per_launch": int(bytes_integrate / launches),
-                     "note": "one lane per pixel, cells in series (FP64 inverse receptor): latency bound at "
-                             f"{npix / 64:.0f} waves, one launch per window after the window's photocurrent "
-                             "kernel; photocurrent in is 8 B per pixel cell"},
+        "rejected_windows": int(st["rejected_windows"]),
+        "roofline": roofline,
+        "roofline_other_kernels": secondary,
+        "totals": {k: st[k] for k in ("probe_samples", "query_samples", "pixel_cells", "probe_launches",
+                                      "query_launches", "circuit_launches", "primary_rays", "shadow_rays")},
     }
     sensor.close()
 
-    # --- native solver on the same request: the cell-trace roofline
+    # --- native solver on the same request (Mrays/s of the cell trace)
     nat = capi.Sensor(scene, w.physic)
     nat.initialize(initial, w.start_time)
     ivs = workloads.horizon_intervals(w, traj, w.start_time, 0, args.horizons)
@@ -384,25 +537,23 @@ def request_mode(args) -> dict:
         nat.process(ivs)
     nwall = time.perf_counter() - t0
     ns = nat.stats()
-    trace_s = ns["trace_ms"] / 1e3
-    algorithmic = ns["primary_rays"] * b_primary + ns["shadow_rays"] * b_shadow
-    nachieved = algorithmic / max(trace_s, 1e-30) / 1e9
     result["native_solver"] = {
         "mrays_per_s": round((ns["primary_rays"] + ns["shadow_rays"] + ns["event_rays"]) / nwall / 1e6, 3),
         "ms_per_step": round(nwall / nsteps * 1e3, 3), "horizons_per_step": args.horizons,
         "trajectory_s_per_wall_s": round(nsteps * args.horizons * w.time_resolution / nwall, 6),
-        "roofline": {"bound": "hbm", "kernel": "trace_cells_multi", "achieved": round(nachieved, 2),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(nachieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "avg_launch_ms": round(ns["trace_ms"] / max(ns["trace_launches"], 1), 3),
-                     "bytes_per_ray": {"primary": round(b_primary, 1), "shadow": round(b_shadow, 1)},
-                     "note": "the 186 MB scene is served from L2/MALL: algorithmic bytes exceed HBM peak; "
-                             "the kernel is VALU bound"},
-    }
+        "trace_cells_ms_per_launch": round(ns["trace_ms"] / max(ns["trace_launches"], 1), 3)}
     nat.close()
     if not args.no_cpu_baseline:
-        log("cpu baseline (oracle) ...")
-        result["cpu_baseline"] = cpu_baseline(w, (v, n, nodes, root, leaf_idx), batch,
-                                              workloads.horizon_intervals(w, traj, w.start_time, 0, batch))
+        k = min(args.cpu_windows, len(flights[0].windows))
+        first_windows = flights[0].windows[:k]
+        events_before = 0
+        gpu = {"wall_s": sum(x[0] for x in first_windows), "events": first_windows[-1][1] - events_before}
+        log(f"cpu baseline (oracle) on the first {k} windows ...")
+        result["cpu_baseline"] = cpu_baseline(w, (v, n, nodes, root, leaf_idx),
+                                              [(x[2], x[3]) for x in first_windows], gpu)
+    if not args.no_served:
+        log("served vs plain ...")
+        result["served_vs_plain"] = served_compare(args, args.served_requests, tuple(args.served_horizons))
     return result
 
 
