@@ -164,6 +164,12 @@ class _ProbeC(ctypes.Structure):
                 ("circuit_cells", ctypes.c_int32)]
 
 
+class _OpticalProbeC(ctypes.Structure):
+    _fields_ = [("maximum_log_error", ctypes.c_double), ("p99_log_error", ctypes.c_double),
+                ("mean_log_error", ctypes.c_double), ("probes", ctypes.c_int32), ("over_tolerance", ctypes.c_int32),
+                ("exact_fallback_probes", ctypes.c_int32)]
+
+
 def _as_dict(c) -> dict:
     return {name: getattr(c, name) for name, _ in c._fields_}
 
@@ -198,6 +204,7 @@ def load_library() -> ctypes.CDLL:
                                                ctypes.POINTER(_CameraC), ctypes.POINTER(_HapkeC), _I32,
                                                ctypes.POINTER(_P)]),
         "dvs_scene_destroy": (ctypes.c_int, [_P]),
+        "dvs_scene_camera": (ctypes.c_int, [_P, ctypes.POINTER(_CameraC)]),
         "dvs_scene_info": (ctypes.c_int, [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I64), ctypes.POINTER(_I64),
                                           ctypes.POINTER(_I64)]),
         "dvs_frame_render": (ctypes.c_int, [_P, _P, _I32, _P, _P, _P]),
@@ -242,6 +249,21 @@ def load_library() -> ctypes.CDLL:
         "dvs_noise_eval": (ctypes.c_int, [ctypes.c_uint64, _I64, ctypes.c_double, _I32, _P, _P]),
         "dvs_poisson_eval": (ctypes.c_int, [_P, _P, _I64, _P]),
         "dvs_advance_pixel_eval": (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _I32, ctypes.c_double, _P, _P]),
+        "dvs_invert_hapke_eval": (ctypes.c_int, [_P, _P, _P, _P, _I64, ctypes.POINTER(_HapkeC), _P]),
+        "dvs_hapke_reflectance": (ctypes.c_float, [ctypes.c_float] * 4 + [ctypes.POINTER(_HapkeC)]),
+        "dvs_hapke_directional": (ctypes.c_float, [_P, _P, _P, ctypes.c_float, ctypes.POINTER(_HapkeC)]),
+        "dvs_invert_hapke_if": (ctypes.c_float, [ctypes.c_float] * 4 + [ctypes.POINTER(_HapkeC)]),
+        "dvs_receptor_log_at": (ctypes.c_double, [ctypes.c_double] * 4),
+        "dvs_advance_pixel": (_I32, [_P, _P, ctypes.c_double, ctypes.c_double, ctypes.c_double, _P, _I32, _P, _P]),
+        "dvs_sensor_enable_diagnostics": (ctypes.c_int, [_P]),
+        "dvs_sensor_verify_receptor": (ctypes.c_int, [_P, _P, _I32, ctypes.POINTER(_ProbeC)]),
+        "dvs_sensor_verify_optics": (ctypes.c_int, [_P, _P, _I32, ctypes.c_double, ctypes.POINTER(_OpticalProbeC)]),
+        "dvs_readout_create": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, _I64, ctypes.POINTER(_P)]),
+        "dvs_readout_destroy": (ctypes.c_int, [_P]),
+        "dvs_readout_submit": (ctypes.c_int, [_P, _P, _I64, ctypes.c_double, ctypes.POINTER(_I64)]),
+        "dvs_readout_drain": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),
+        "dvs_readout_take": (ctypes.c_int, [_P, _P, _I64, ctypes.POINTER(_I64)]),
+        "dvs_readout_stats": (ctypes.c_int, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
         "dvs_trace_rays": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P, _P, _P, _P]),
         "dvs_trace_shadow_rays": (ctypes.c_int, [_P, _P, _P, _P, _I64, _P]),
         "dvs_trajectory_create": (ctypes.c_int, [_P, _I32, ctypes.c_float, _P, _P, ctypes.POINTER(_P)]),
@@ -469,6 +491,27 @@ class Sensor:
                                                           int(check_convergence), ctypes.byref(ok),
                                                           ctypes.byref(ready)))
         return bool(ok.value), self._take(ready.value)
+
+    def enable_diagnostics(self) -> None:
+        """EventSensor::enable_numerical_diagnostics: every later adaptive
+        window also measures its receptor check, for verify_receptor."""
+        _check(load_library().dvs_sensor_enable_diagnostics(self.handle))
+
+    def verify_receptor(self, intervals: np.ndarray) -> dict:
+        """EventSensor::verify_receptor of the last adaptive window."""
+        ivs = np.ascontiguousarray(np.atleast_1d(intervals), dtype=INTERVAL_DTYPE)
+        r = _ProbeC()
+        _check(load_library().dvs_sensor_verify_receptor(self.handle, _ptr(ivs), len(ivs), ctypes.byref(r)))
+        return _as_dict(r)
+
+    def verify_optics(self, intervals: np.ndarray, tolerance: float = 1e-2) -> dict:
+        """EventSensor::verify_optics: the last adaptive window's knots against
+        native traces of 1024 hashed pixels at 17 times."""
+        ivs = np.ascontiguousarray(np.atleast_1d(intervals), dtype=INTERVAL_DTYPE)
+        r = _OpticalProbeC()
+        _check(load_library().dvs_sensor_verify_optics(self.handle, _ptr(ivs), len(ivs), float(tolerance),
+                                                       ctypes.byref(r)))
+        return _as_dict(r)
 
     def adaptive_capacity(self) -> int:
         c = _I32()
@@ -739,6 +782,94 @@ def receptor_eval(u, rate, dt, charge: float) -> np.ndarray:
     out = np.empty(a.shape, np.float64)
     _check(load_library().dvs_receptor_eval(_ptr(a), _ptr(b), _ptr(c), float(charge), a.size, _ptr(out)))
     return out
+
+
+class Readout:
+    """EventReadout (include/dvs/event_sensor.hpp:24-37): the bounded FIFO
+    readout on its own (host side)."""
+
+    def __init__(self, bandwidth: float, latency: float, capacity: int):
+        self.handle = _P()
+        _check(load_library().dvs_readout_create(float(bandwidth), float(latency), int(capacity),
+                                                 ctypes.byref(self.handle)))
+
+    def close(self):
+        if self.handle:
+            load_library().dvs_readout_destroy(self.handle)
+            self.handle = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _take(self, count: int) -> np.ndarray:
+        out = np.empty(count, EVENT_DTYPE)
+        n = _I64()
+        _check(load_library().dvs_readout_take(self.handle, _ptr(out), count, ctypes.byref(n)))
+        return out[: n.value]
+
+    def submit(self, events: np.ndarray, now: float) -> np.ndarray:
+        ev = np.ascontiguousarray(events, dtype=EVENT_DTYPE)
+        n = _I64()
+        _check(load_library().dvs_readout_submit(self.handle, _ptr(ev), len(ev), float(now), ctypes.byref(n)))
+        return self._take(n.value)
+
+    def drain(self) -> np.ndarray:
+        n = _I64()
+        _check(load_library().dvs_readout_drain(self.handle, ctypes.byref(n)))
+        return self._take(n.value)
+
+    def stats(self) -> tuple[int, int]:
+        d, p = _I64(), _I64()
+        _check(load_library().dvs_readout_stats(self.handle, ctypes.byref(d), ctypes.byref(p)))
+        return d.value, p.value
+
+
+def hapke_reflectance(cosi: float, cose: float, cosg: float, albedo: float, hapke: Hapke) -> float:
+    """hapke_reflectance (include/dvs/hapke.hpp:131-151) on the host."""
+    hk = _struct(_HapkeC, hapke)
+    return load_library().dvs_hapke_reflectance(cosi, cose, cosg, albedo, ctypes.byref(hk))
+
+
+def hapke_directional(normal, light, view, albedo: float, hapke: Hapke) -> float:
+    n, l, v = (np.ascontiguousarray(x, np.float32) for x in (normal, light, view))
+    hk = _struct(_HapkeC, hapke)
+    return load_library().dvs_hapke_directional(_ptr(n), _ptr(l), _ptr(v), albedo, ctypes.byref(hk))
+
+
+def invert_hapke_if(incidence: float, emission: float, phase: float, target: float, hapke: Hapke) -> float:
+    """invert_hapke_if (include/dvs/hapke.hpp:160-172) on the host."""
+    hk = _struct(_HapkeC, hapke)
+    return load_library().dvs_invert_hapke_if(incidence, emission, phase, target, ctypes.byref(hk))
+
+
+def invert_hapke_eval(incidence, emission, phase, target, hapke: Hapke) -> np.ndarray:
+    """invert_hapke_if over arrays, on the device."""
+    a, b, c, d = (np.ascontiguousarray(x, np.float32) for x in np.broadcast_arrays(incidence, emission, phase, target))
+    a, b, c, d = (np.ascontiguousarray(x) for x in (a, b, c, d))
+    out = np.empty(a.shape, np.float32)
+    hk = _struct(_HapkeC, hapke)
+    _check(load_library().dvs_invert_hapke_eval(_ptr(a), _ptr(b), _ptr(c), _ptr(d), a.size, ctypes.byref(hk),
+                                                _ptr(out)))
+    return out
+
+
+def receptor_log_at(u: float, rate: float, dt: float, charge: float) -> float:
+    return load_library().dvs_receptor_log_at(u, rate, dt, charge)
+
+
+def advance_pixel(state: np.ndarray, params: np.ndarray, rate: float, t0: float, t1: float, capacity: int = 64):
+    """advance_pixel (include/dvs/pixel_model.hpp:35-102) on the host for one
+    pixel (state: one PIXEL_STATE_DTYPE entry, updated in place); reset-noise
+    draws are 0.  Returns (count, crossings)."""
+    assert state.dtype == PIXEL_STATE_DTYPE and state.size == 1 and state.flags.c_contiguous
+    p = np.ascontiguousarray(params, PIXEL_PARAMS_DTYPE).reshape(1)
+    out = np.zeros(max(capacity, 1), CROSSING_DTYPE)
+    n = load_library().dvs_advance_pixel(_ptr(state), _ptr(p), float(rate), float(t0), float(t1), _ptr(out),
+                                         int(capacity), None, None)
+    return n, out[: max(n, 0)]
 
 
 def advance_pixel_eval(states: np.ndarray, params: np.ndarray, rates, t0, t1, capacity: int = 64,

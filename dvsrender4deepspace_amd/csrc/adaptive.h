@@ -175,5 +175,12 @@ hipError_t launch_adapt_rates(const SegDev *segs, const AdaptDev *ad, const int 
                               hipStream_t st);
 hipError_t launch_adapt_integrate(const SegDev *segs, const AdaptDev *ad, const int *block_begin, int nseg,
                                   int blocks, int pixels, AdaptBufs b, int c0, int c1, int *status, hipStream_t st);
+// verify_knots (optical_sampler.cu:86-103): the probe poses of a window, and
+// the knot-interpolation log errors of traced probes.
+hipError_t launch_adapt_verify_poses(const SegDev *seg, double start, double end, int times, Pose *poses,
+                                     hipStream_t st);
+hipError_t launch_adapt_verify_errors(int items, int positions, int times, const int *item_pixel,
+                                      const double *light, AdaptBufs b, const PhotoParams &photo, double *errors,
+                                      hipStream_t st);
 
 }  // namespace dvsamd

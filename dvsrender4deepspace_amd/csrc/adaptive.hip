@@ -729,10 +729,56 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void a
   g.counts[i] = count;
 }
 
+// verify_knots (optical_sampler.cu:86-103), in two parts around a native
+// trace of the probes.  Pose j of `times` sits at window fraction
+// (j + .5) / times.
+__global__ void adapt_verify_poses(const SegDev *seg, double start, double end, int times, Pose *poses) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= times) return;
+  const double fraction = (j + .5) / times;
+  poses[j] = native_pose(seg->ivs, seg->iv_count, start + (end - start) * fraction);
+}
+// Log error of the knot interpolation against the traced radiance of each
+// probe; item q is probe time q / positions.  -1 where the pixel has no
+// knots (native fallback) or the time falls in an exact segment.
+__global__ void adapt_verify_errors(int items, int positions, int times, const int *item_pixel, const double *light,
+                                    AdaptBufs b, PhotoParams photo, double *errors) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= items) return;
+  const int gp = item_pixel[q];
+  const double fraction = (q / positions + .5) / times;
+  if (b.flags[gp] & 1) { errors[q] = -1; return; }
+  const int size = b.kcount[gp];
+  const double2 *v = b.knots + (b.koff[gp] - size);
+  int a = 0;
+  while (a + 2 < size && fraction > fabs(v[a + 1].x)) ++a;
+  if (v[a + 1].x < 0) { errors[q] = -1; return; }
+  const double weight = (fraction - fabs(v[a].x)) / (fabs(v[a + 1].x) - fabs(v[a].x));
+  const double estimate = v[a].y + (v[a + 1].y - v[a].y) * weight;
+  const double actual = light[q] * photo.electrons_per_radiance + photo.dark_electrons;
+  const double predicted = estimate * photo.electrons_per_radiance + photo.dark_electrons;
+  errors[q] = fabs(log(fmax(1e-30, actual) / fmax(1e-30, predicted)));
+}
+
 }  // namespace dvsdev
 
 namespace dvsamd {
 using namespace dvsdev;
+
+hipError_t launch_adapt_verify_poses(const SegDev *seg, double start, double end, int times, Pose *poses,
+                                     hipStream_t st) {
+  if (times <= 0) return hipSuccess;
+  adapt_verify_poses<<<1, 64, 0, st>>>(seg, start, end, times, poses);
+  return hipGetLastError();
+}
+hipError_t launch_adapt_verify_errors(int items, int positions, int times, const int *item_pixel,
+                                      const double *light, AdaptBufs b, const PhotoParams &photo, double *errors,
+                                      hipStream_t st) {
+  if (items <= 0) return hipSuccess;
+  adapt_verify_errors<<<(items + 255) / 256, 256, 0, st>>>(items, positions, times, item_pixel, light, b, photo,
+                                                            errors);
+  return hipGetLastError();
+}
 
 namespace {
 inline unsigned grid(long long n, int block = 256) { return (unsigned)((n + block - 1) / block); }

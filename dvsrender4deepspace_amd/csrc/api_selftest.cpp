@@ -1,20 +1,21 @@
-// Exercises the C++ API (include/dvs_render_amd.hpp) the way the
-// reference's callers use it.
+// Exercises the C++ API (include/dvs_render.hpp over dvs_render_amd.hpp)
+// the way the reference's callers use it.
 //   dvs_api_selftest --cpu <config.yaml> <out_dir>
-//       Config, BVHBuilder, writers, pose conversion and CameraTrajectory
-//       (no GPU).
+//       Config, BVHBuilder, writers, pose conversion, CameraTrajectory, the
+//       host physics (advance_pixel, Hapke) and EventReadout (no GPU).
 //   dvs_api_selftest --gpu <out_dir>
 //       DeviceScene, CamRayCaster and DvsRayCaster on the plane fixture of
 //       tests/test_sensor.cu:66-80 (48x36 grid): frame depth, native and
 //       adaptive events written to events.bin / events.h5 for comparison;
-//       trajectory intervals built on the device driving a DvsRayCaster.
+//       trajectory intervals built on the device driving a DvsRayCaster;
+//       EventSensor, OpticalSampler and the diagnostics.
 // Prints one line per check and exits non-zero on the first failure.
 #include <cstdio>
 #include <iostream>
 #include <stdexcept>
 #include <string>
 
-#include "dvs_render_amd.hpp"
+#include "dvs_render.hpp"
 
 namespace {
 void expect(bool ok, const char *what) {
@@ -117,6 +118,47 @@ int cpu(const std::string &config_path, const std::string &out) {
     }
     expect(threw, "CameraTrajectory rejects duplicate timestamps");
   }
+  {
+    // tests/test_sensor.cu:37-42: the analytic ON crossing of a step from
+    // 1000 to 2000 e-/s, charge 10, thresholds 0.2 (TestRng: normal() = 0)
+    struct TestRng {
+      double normal() { return 0; }
+    } rng;
+    dvs::PixelParameters p;
+    p.threshold_pos = p.threshold_neg = .2;
+    p.charge_scale = 10;
+    p.refractory_time = 0;
+    dvs::PixelState st;
+    st.receptor_log = st.reference_log = std::log(1000.0);
+    st.threshold_pos = st.threshold_neg = .2;
+    dvs::PixelEvent ev[8];
+    const int n = dvs::advance_pixel(st, p, 2000, 0, .1, ev, 8, rng);
+    const double target = 1000 * std::exp(.2);
+    const double expected = -10.0 / 2000 * std::log((1 / target - 1 / 2000.0) / (1 / 1000.0 - 1 / 2000.0));
+    expect(n == 3 && ev[0].polarity == 1 && std::fabs(ev[0].time - expected) < 1e-12,
+           "advance_pixel: analytic step crossing");
+    expect(dvs::advance_pixel(st, p, 2000, .1, .2, ev, 0, rng) >= 0 &&
+               std::fabs(dvs::receptor_log_at(std::log(1000.0), 0, 10, 10) - (std::log(1000.0) - std::log1p(1000.0)))
+                   < 1e-12,
+           "receptor_log_at: zero-current closed form");
+    dvs::HapkeConfig h;
+    const float ci = std::cos(30 * 3.14159265f / 180);
+    const float target_if = 3.14159265f * dvs::hapke_reflectance(ci, 1, ci, .324f, h);
+    expect(std::fabs(dvs::invert_hapke_if(30, 0, 30, target_if, h) - .324f) < 2e-6f &&
+               std::isnan(dvs::invert_hapke_if(20, 10, 90, .1f, h)),
+           "Hapke I/F round trip and invalid geometry");
+    const float dir = dvs::hapke_directional_reflectance(dvs::make_float3(0, 0, 1), dvs::make_float3(0, 0, 1),
+                                                         dvs::make_float3(0, 0, 1), h.w, h);
+    expect(dir > 0 && std::isfinite(dir), "Hapke directional reflectance at opposition");
+    // tests/test_sensor.cu:176-177
+    dvs::EventReadout bus(100, .001, 2);
+    std::vector<dvs::Event> ready;
+    bus.submit(std::vector<dvs::Event>(3, dvs::Event{0, 0, 0, 1, 0}), 0, ready);
+    bus.drain(ready);
+    expect(bus.dropped() == 1 && ready.size() == 2 && std::fabs(ready[0].timestamp - .011) < 1e-12 &&
+               bus.peak_queue() == 2,
+           "EventReadout drops and delivery time");
+  }
   return 0;
 }
 
@@ -214,6 +256,57 @@ int gpu(const std::string &out) {
     moving.advance_many(ivs, ev);
     moving.drain(ev);
     expect(moving.dropped_events() == 0, "trajectory window advances");
+  }
+  {
+    // EventSensor / OpticalSampler on the same scene view, against the
+    // casters above: the same frame and the same native events.
+    dvs::OpticalSampler sampler(48, 36, 1);
+    const std::vector<dvs::OpticalState> poses{pose};
+    const dvs::OpticalValue *values = sampler.evaluate(scene->view(), poses);
+    std::vector<dvs::OpticalValue> down;
+    sampler.download(down);
+    expect(down.size() == image.size() && (float)values[centre].radiance == image[centre] &&
+               values[centre].depth == depth[centre] && values[centre].samples == 1,
+           "OpticalSampler evaluates the frame of CamRayCaster");
+    dvs::EventSensor sensor(48, 36, config.physic);
+    sensor.initialize(scene->view(), pose, 0.0);
+    std::vector<dvs::Event> ev, again;
+    sensor.process_many(scene->view(), {{pose, pose, 0, .002}}, ev);
+    sensor.process(scene->view(), {start, end, .002, .004}, ev);
+    dvs::DvsRayCaster twin(scene, config);
+    twin.initialize(s, 0.0);
+    twin.advance_many({{pose, pose, 0, .002}}, again);
+    twin.advance_many({{start, end, .002, .004}}, again);
+    expect(ev.size() == again.size() && !ev.empty() &&
+               std::memcmp(ev.data(), again.data(), ev.size() * sizeof(dvs::Event)) == 0,
+           "EventSensor gives DvsRayCaster's events");
+    dvs::EventSensor adaptive_sensor(48, 36, config.physic);
+    adaptive_sensor.enable_numerical_diagnostics();
+    adaptive_sensor.initialize(scene->view(), pose, 0.0);
+    bool threw_state = false;
+    try {
+      adaptive_sensor.verify_optics(scene->view(), sampler, {{pose, pose, 0, .002}}, 1e-3);
+    } catch (const std::logic_error &) {
+      threw_state = true;
+    }
+    expect(threw_state, "verify_optics before refinement throws std::logic_error");
+    const std::vector<dvs::OpticalInterval> window{{pose, pose, 0, .002}};
+    expect(adaptive_sensor.process_adaptive(scene->view(), window, sampler, 1e-3, ev) && ev.empty() &&
+               sampler.statistics().maximum_knots == 5 && sampler.knot_capacity() >= 128,
+           "EventSensor::process_adaptive with an OpticalSampler");
+    const auto optics = adaptive_sensor.verify_optics(scene->view(), sampler, window, 1e-3);
+    expect(optics.probes == 1024 * 17 && optics.maximum_log_error == 0 && optics.over_tolerance == 0,
+           "verify_optics: static knots are exact");
+    const auto receptor = adaptive_sensor.verify_receptor(scene->view(), sampler, window);
+    // ceil(0.002 s / 1e-6f) cells: the float circuit step is just below 1 us
+    expect(receptor.pixels == 1024 && receptor.circuit_cells == 2001 && receptor.maximum_log_voltage_error < 1e-9,
+           "verify_receptor after enable_numerical_diagnostics");
+    dvs::DvsRayCaster diag(scene, config);
+    diag.enable_numerical_diagnostics();
+    diag.initialize(s, 0.0);
+    expect(diag.advance_adaptive(window, 1e-3, ev) && diag.verify_optics(window, 1e-3).over_tolerance == 0 &&
+               diag.verify_receptor(window).pixels == 1024,
+           "DvsRayCaster diagnostics");
   }
   return 0;
 }

@@ -23,14 +23,14 @@ struct CrossingEvent { double time; int polarity; int pad; };
 // Exact constant-rate receptor solution (receptor_log_at, pixel_model.hpp:18-26).
 // receptor_log_from takes x = exp(u) from the caller, which evaluates it once
 // for all pieces that start at u.
-__device__ __forceinline__ double receptor_log_from(double u, double x, double rate, double dt, double charge) {
+__host__ __device__ __forceinline__ double receptor_log_from(double u, double x, double rate, double dt, double charge) {
   if (dt <= 0) return u;
   rate = fmax(0.0, rate);
   if (rate == 0) return u - log1p(x * dt / charge);
   const double a = exp(-rate * dt / charge), b = -expm1(-rate * dt / charge);
   return -log(a / x + b / rate);
 }
-__device__ __forceinline__ double receptor_log_at(double u, double rate, double dt, double charge) {
+__host__ __device__ __forceinline__ double receptor_log_at(double u, double rate, double dt, double charge) {
   if (dt <= 0) return u;
   return receptor_log_from(u, exp(u), rate, dt, charge);
 }
@@ -161,7 +161,7 @@ struct PixelRegs {
   int reset_pending;
 };
 template <class Normal>
-__device__ inline int advance_pixel(PixelRegs &s, const CircuitParams &p, double rate, double t0, double t1,
+__host__ __device__ inline int advance_pixel(PixelRegs &s, const CircuitParams &p, double rate, double t0, double t1,
                                     CrossingEvent *out, int capacity, Normal &&normal) {
   if (t1 <= t0) return 0;
   int count = 0;

@@ -13,21 +13,21 @@
 namespace dvsdev {
 
 struct f3 { float x, y, z; };
-__device__ __forceinline__ f3 mk(float x, float y, float z) { return f3{x, y, z}; }
-__device__ __forceinline__ f3 add(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
-__device__ __forceinline__ f3 sub(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ __forceinline__ f3 neg(f3 a) { return mk(-a.x, -a.y, -a.z); }
-__device__ __forceinline__ f3 scale(f3 a, float b) { return mk(a.x * b, a.y * b, a.z * b); }
-__device__ __forceinline__ f3 divide(f3 a, float b) { return mk(a.x / b, a.y / b, a.z / b); }
-__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ f3 cross(f3 a, f3 b) {
+__host__ __device__ __forceinline__ f3 mk(float x, float y, float z) { return f3{x, y, z}; }
+__host__ __device__ __forceinline__ f3 add(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__host__ __device__ __forceinline__ f3 sub(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__host__ __device__ __forceinline__ f3 neg(f3 a) { return mk(-a.x, -a.y, -a.z); }
+__host__ __device__ __forceinline__ f3 scale(f3 a, float b) { return mk(a.x * b, a.y * b, a.z * b); }
+__host__ __device__ __forceinline__ f3 divide(f3 a, float b) { return mk(a.x / b, a.y / b, a.z / b); }
+__host__ __device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__host__ __device__ __forceinline__ f3 cross(f3 a, f3 b) {
   return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
-__device__ __forceinline__ f3 normalize(f3 v) {
+__host__ __device__ __forceinline__ f3 normalize(f3 v) {
   const float len = sqrtf(dot(v, v));
   return len > 0.0f ? divide(v, len) : mk(0.f, 0.f, 0.f);
 }
-__device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+__host__ __device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 
 constexpr float kRayEps = 1e-6f;                  // include/helper_math.hpp:11
 constexpr float kPi = 3.14159265358979323846f;    // include/dvs/math_types.hpp
@@ -313,32 +313,32 @@ __device__ inline f3 shadow_origin(Rec rec, float u, float v, f3 light) {
 // ---------------------------------------------------------------------------
 // Hapke BRDF x cos(i) (include/dvs/hapke.hpp:14-157), float.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float hk_safe_tan(float x) { return tanf(clampf(x, -kPi / 2.0f + 1e-6f, kPi / 2.0f - 1e-6f)); }
-__device__ __forceinline__ float hk_safe_acos(float x) { return acosf(clampf(x, -1.0f, 1.0f)); }
-__device__ __forceinline__ float hk_phase(float cosg, float b, float c) {
+__host__ __device__ __forceinline__ float hk_safe_tan(float x) { return tanf(clampf(x, -kPi / 2.0f + 1e-6f, kPi / 2.0f - 1e-6f)); }
+__host__ __device__ __forceinline__ float hk_safe_acos(float x) { return acosf(clampf(x, -1.0f, 1.0f)); }
+__host__ __device__ __forceinline__ float hk_phase(float cosg, float b, float c) {
   const float omb = 1.0f - b * b;
   const float t1 = ((1.0f + c) * 0.5f) * omb / powf(1.0f - 2.0f * b * cosg + b * b, 1.5f);
   const float t2 = ((1.0f - c) * 0.5f) * omb / powf(1.0f + 2.0f * b * cosg + b * b, 1.5f);
   return t1 + t2;
 }
-__device__ __forceinline__ float hk_shoe(float tg, float h) {
+__host__ __device__ __forceinline__ float hk_shoe(float tg, float h) {
   if (h <= 0.0f) return tg <= 0.0f ? 1.0f : 0.0f;
   return 1.0f / (1.0f + tg / h);
 }
-__device__ __forceinline__ float hk_cboe(float tg, float h) {
+__host__ __device__ __forceinline__ float hk_cboe(float tg, float h) {
   if (h <= 0.0f) return tg <= 0.0f ? 1.0f : 0.0f;
   const float T = tg / h;
   if (T < 1e-8f) return 1.0f;
   return (1.0f + (1.0f - expf(-T)) / T) / (2.0f * (1.0f + T) * (1.0f + T));
 }
-__device__ __forceinline__ float hk_H(float x, float w) {
+__host__ __device__ __forceinline__ float hk_H(float x, float w) {
   if (x <= 1e-8f) return 1.0f;
   const float root = sqrtf(fmaxf(0.0f, 1.0f - w));
   const float r0 = (1.0f - root) / (1.0f + root);
   const float term = r0 + (1.0f - 2.0f * r0 * x) * 0.5f * logf(1.0f + 1.0f / x);
   return 1.0f / (1.0f - w * x * term);
 }
-__device__ __forceinline__ float hk_porosity(float phi) {
+__host__ __device__ __forceinline__ float hk_porosity(float phi) {
   if (phi <= 1e-8f) return 1.0f;
   const float p = 1.209f * powf(phi, 2.0f / 3.0f);
   return -logf(1.0f - p) / p;
@@ -346,18 +346,18 @@ __device__ __forceinline__ float hk_porosity(float phi) {
 // The roughness helpers take tan(thetap) and chi(thetap) precomputed; the
 // reference recomputes the same float expressions (tanf of the same
 // argument), so the values are identical.
-__device__ __forceinline__ float hk_E1(float cot_tp, float cy, float sy) {
+__host__ __device__ __forceinline__ float hk_E1(float cot_tp, float cy, float sy) {
   return expf((-2.0f / kPi) * cot_tp * cy / fmaxf(sy, 1e-6f));
 }
-__device__ __forceinline__ float hk_E2(float cot_tp, float cy, float sy) {
+__host__ __device__ __forceinline__ float hk_E2(float cot_tp, float cy, float sy) {
   const float v = cot_tp * cy / fmaxf(sy, 1e-6f);
   return expf((-1.0f / kPi) * v * v);
 }
-__device__ __forceinline__ float hk_eta(float ch, float tan_tp, float cot_tp, float cy, float sy) {
+__host__ __device__ __forceinline__ float hk_eta(float ch, float tan_tp, float cot_tp, float cy, float sy) {
   const float e1 = hk_E1(cot_tp, cy, sy), e2 = hk_E2(cot_tp, cy, sy);
   return ch * (cy + sy * tan_tp * e2 / (2.0f - e1));
 }
-__device__ inline float hapke_reflectance(float cosi, float cose, float cosg, float albedo, const dvs_hapke &p) {
+__host__ __device__ inline float hapke_reflectance(float cosi, float cose, float cosg, float albedo, const dvs_hapke &p) {
   if (cosi <= 0 || cose <= 0) return 0;
   cosi = clampf(cosi, 0, 1); cose = clampf(cose, 0, 1); cosg = clampf(cosg, -1, 1);
   const float sini = sqrtf(fmaxf(0, 1 - cosi * cosi)), sine = sqrtf(fmaxf(0, 1 - cose * cose));
@@ -396,12 +396,31 @@ __device__ inline float hapke_reflectance(float cosi, float cose, float cosg, fl
   const float LS = mu0 / (mu0 + mue + 1e-6f);
   return (albedo / (4.0f * kPi)) * LS * K * (pg * (1.0f + BS * p.B_S0) + M) * (1.0f + BC * p.B_C0) * S;
 }
-__device__ __forceinline__ float hapke_directional(f3 normal, f3 light, f3 view, float albedo, const dvs_hapke &p) {
+__host__ __device__ __forceinline__ float hapke_directional(f3 normal, f3 light, f3 view, float albedo, const dvs_hapke &p) {
   const f3 n = normalize(normal), l = normalize(light), v = normalize(view);
   return hapke_reflectance(dot(n, l), dot(n, v), dot(l, v), albedo, p);
 }
-__device__ __forceinline__ float material_albedo(float value, float fallback) {
-  return isfinite(value) && value >= 0 && value <= 1 ? value : fallback;
+__host__ __device__ __forceinline__ float material_albedo(float value, float fallback) {
+  return __builtin_isfinite(value) && value >= 0 && value <= 1 ? value : fallback;
+}
+// I/F = pi * hapke_reflectance for the angles in degrees, inverted for the
+// single-scattering albedo by 32 bisection steps on [0, 1]
+// (invert_hapke_if, include/dvs/hapke.hpp:160-172); NaN for invalid
+// geometry, missing values or a target outside [I/F(w=0), I/F(w=1)].
+__host__ __device__ inline float invert_hapke_if(float inc, float emi, float phase, float target, const dvs_hapke &p) {
+  if (!(inc >= 0 && inc < 90 && emi >= 0 && emi < 90 && phase >= 0 && phase <= 180 && target >= 0) ||
+      !__builtin_isfinite(target))
+    return NAN;
+  if (phase > inc + emi + 1e-4f || phase < fabsf(inc - emi) - 1e-4f) return NAN;
+  const float ci = cosf(inc * kPi / 180), ce = cosf(emi * kPi / 180), cg = cosf(phase * kPi / 180);
+  const float top = kPi * hapke_reflectance(ci, ce, cg, 1, p);
+  if (!__builtin_isfinite(top) || target > top) return NAN;
+  float lo = 0, hi = 1;
+  for (int k = 0; k < 32; ++k) {
+    const float w = (lo + hi) * .5f;
+    if (kPi * hapke_reflectance(ci, ce, cg, w, p) < target) lo = w; else hi = w;
+  }
+  return (lo + hi) * .5f;
 }
 
 // ---------------------------------------------------------------------------

@@ -15,6 +15,8 @@
 #include "bvh_host.h"
 #include "common.h"
 #include "dvs_hip.h"
+#include "device_circuit.hpp"
+#include "device_optics.hpp"
 #include "kernels.h"
 #include "scene.h"
 
@@ -215,6 +217,14 @@ int dvs_scene_destroy(dvs_scene scene) {
   return DVS_OK;
 }
 
+int dvs_scene_camera(dvs_scene s, dvs_camera *out) {
+  return guarded([&] {
+    if (!s || !out) fail(DVS_E_ARGUMENT, "null scene");
+    *out = dvs_camera{s->view.width, s->view.height, s->view.fx, s->view.fy, s->view.cx, s->view.cy,
+                      s->view.samples};
+  });
+}
+
 int dvs_scene_info(dvs_scene s, int32_t *max_depth, int64_t *internal_nodes, int64_t *leaves, int64_t *device_bytes) {
   return guarded([&] {
     if (!s) fail(DVS_E_ARGUMENT, "null scene");
@@ -336,6 +346,56 @@ int dvs_receptor_eval(const double *u, const double *rate, const double *dt, dou
     o.reserve(n, "receptor output");
     hip_check(launch_receptor(n, a.ptr, b.ptr, c.ptr, charge, o.ptr, nullptr), "receptor kernel");
     hip_check(hipMemcpy(out, o.ptr, n * sizeof(double), hipMemcpyDeviceToHost), "receptor output");
+  });
+}
+
+float dvs_hapke_reflectance(float cosi, float cose, float cosg, float albedo, const dvs_hapke *hapke) {
+  return hapke ? dvsdev::hapke_reflectance(cosi, cose, cosg, albedo, *hapke) : NAN;
+}
+
+float dvs_hapke_directional(const float normal[3], const float light[3], const float view[3], float albedo,
+                            const dvs_hapke *hapke) {
+  if (!normal || !light || !view || !hapke) return NAN;
+  using dvsdev::mk;
+  return dvsdev::hapke_directional(mk(normal[0], normal[1], normal[2]), mk(light[0], light[1], light[2]),
+                                   mk(view[0], view[1], view[2]), albedo, *hapke);
+}
+
+float dvs_invert_hapke_if(float incidence, float emission, float phase, float target, const dvs_hapke *hapke) {
+  return hapke ? dvsdev::invert_hapke_if(incidence, emission, phase, target, *hapke) : NAN;
+}
+
+double dvs_receptor_log_at(double u, double rate, double dt, double charge) {
+  return dvsdev::receptor_log_at(u, rate, dt, charge);
+}
+
+int32_t dvs_advance_pixel(dvs_pixel_state *state, const dvs_pixel_params *params, double rate, double t0, double t1,
+                          dvs_crossing *out, int32_t capacity, double (*normal)(void *), void *context) {
+  if (!state || !params || capacity < 0 || (capacity > 0 && !out)) return -2;
+  dvsdev::PixelRegs r{state->receptor_log, state->reference_log, state->refractory_until, state->threshold_pos,
+                      state->threshold_neg, state->reset_pending};
+  const dvsdev::CircuitParams p{params->threshold_pos, params->threshold_neg, params->charge_scale,
+                                params->refractory_time, params->reference_leak, params->reset_noise_std};
+  auto draw = [&] { return normal ? normal(context) : 0.0; };
+  const int count = dvsdev::advance_pixel(r, p, rate, t0, t1, reinterpret_cast<dvsdev::CrossingEvent *>(out),
+                                          capacity, draw);
+  state->receptor_log = r.receptor_log;
+  state->reference_log = r.reference_log;
+  state->refractory_until = r.refractory_until;
+  state->reset_pending = r.reset_pending;
+  state->receptor_inverse = std::exp(-r.receptor_log);
+  return count;
+}
+
+int dvs_invert_hapke_eval(const float *inc, const float *emi, const float *phase, const float *target, int64_t n,
+                          const dvs_hapke *h, float *out) {
+  return guarded([&] {
+    if (!inc || !emi || !phase || !target || !h || !out || n < 0) fail(DVS_E_ARGUMENT, "invalid Hapke arguments");
+    DeviceArray<float> a, b, c, d, o;
+    a.upload(inc, n, "hapke input"); b.upload(emi, n, "hapke input"); c.upload(phase, n, "hapke input");
+    d.upload(target, n, "hapke input"); o.reserve(n, "hapke output");
+    hip_check(launch_invert_hapke(n, a.ptr, b.ptr, c.ptr, d.ptr, *h, o.ptr, nullptr), "hapke kernel");
+    hip_check(hipMemcpy(out, o.ptr, n * sizeof(float), hipMemcpyDeviceToHost), "hapke output");
   });
 }
 

@@ -395,6 +395,11 @@ __global__ void hapke_kernel(int64_t n, const float *ci, const float *ce, const 
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = hapke_reflectance(ci[i], ce[i], cg[i], alb[i], h);
 }
+__global__ void invert_hapke_kernel(int64_t n, const float *inc, const float *emi, const float *phase,
+                                    const float *target, dvs_hapke h, float *out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = invert_hapke_if(inc[i], emi[i], phase[i], target[i], h);
+}
 __global__ void receptor_kernel(int64_t n, const double *u, const double *rate, const double *dt, double charge,
                                 double *out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -616,6 +621,12 @@ hipError_t launch_hapke(int64_t n, const float *ci, const float *ce, const float
                         const dvs_hapke &h, float *out, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   hapke_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, ci, ce, cg, alb, h, out);
+  return hipGetLastError();
+}
+hipError_t launch_invert_hapke(int64_t n, const float *inc, const float *emi, const float *phase, const float *target,
+                               const dvs_hapke &h, float *out, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  invert_hapke_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, inc, emi, phase, target, h, out);
   return hipGetLastError();
 }
 hipError_t launch_receptor(int64_t n, const double *u, const double *rate, const double *dt, double charge,

@@ -175,6 +175,17 @@ struct Segment {
   dvs_refinement_stats last_refine{};
   dvs_receptor_probe last_probe{};
   bool last_accepted = true;
+  // the last adaptive window: its engine step, its place in that step and
+  // its bounds (verify_optics reads its knots while no other step ran)
+  uint64_t window_step = ~0ull;
+  int window_index = 0, window_pixel_offset = 0, window_ivs = 0;
+  double window_start = 0, window_end = 0;
+  // numerical diagnostics (EventSensor::enable_numerical_diagnostics): every
+  // adaptive window also measures the receptor check, kept for
+  // verify_receptor
+  bool diagnostics = false;
+  dvs_receptor_probe diag_probe{};
+  uint64_t diag_step = ~0ull;
   ~Segment() { scene_release(scene); }
 };
 
@@ -397,6 +408,78 @@ class Engine {
   // process_adaptive over exactly the queued intervals of one segment, as
   // one window (event_sensor.cu:306-355); returns whether the convergence
   // gate accepted it.  The queue is consumed only when accepted.
+  // EventSensor::verify_optics / OpticalSampler::verify (event_sensor.hpp:
+  // 57-59, optical_sampler.cu:86-103,195-215): the knot interpolation of the
+  // segment's last adaptive window against native traces of 1024 hashed
+  // pixels at 17 times, while no other step has run since.
+  dvs_optical_probe verify_optics(int id, const dvs_optical_interval *ivs, int n, double tolerance) {
+    Segment &s = segment(id);
+    if (s.window_step != step_serial_) fail(DVS_E_STATE, "refine before validating optical knots");
+    if (!ivs || n != s.window_ivs || ivs[0].t0 != s.window_start || ivs[n - 1].t1 != s.window_end)
+      fail(DVS_E_ARGUMENT, "intervals differ from the last adaptive window");
+    constexpr int positions = 1024, times = 17, items = positions * times;
+    hipStream_t st = stream_;
+    std::vector<int> pix(items), pose_of(items);
+    for (int q = 0; q < items; ++q) {
+      pix[q] = s.window_pixel_offset + (int)(((uint64_t)(q % positions) * 2654435761ULL) % (uint64_t)s.npix);
+      pose_of[q] = q / positions;
+    }
+    verify_pix_.upload(pix.data(), pix.size(), "optical convergence probes", st, staging_);
+    verify_pose_.upload(pose_of.data(), pose_of.size(), "optical convergence probes", st, staging_);
+    staging_.flush(st);
+    verify_poses_.reserve(times, "optical convergence probes");
+    verify_rad_.reserve(items, "optical convergence probes");
+    verify_err_.reserve(items, "optical convergence probes");
+    hip_check(launch_fill_int(status_.ptr, 0, 1, st), "clear probe status");
+    const SegDev *seg = segtab_.ptr + s.window_index;
+    hip_check(launch_adapt_verify_poses(seg, s.window_start, s.window_end, times, verify_poses_.ptr, st),
+              "optical convergence probe launch");
+    const int samples = s.scene->view.samples;
+    hip_check(launch_adapt_trace(dvsdev::LAYOUT_CELL, segtab_.ptr, pbegin_.ptr, last_nseg_, samples, items,
+                                 verify_pix_.ptr, verify_pose_.ptr, verify_poses_.ptr, last_lds_, verify_rad_.ptr,
+                                 status_.ptr, counter_.ptr, st),
+              "optical convergence probe launch");
+    AdaptBufs b{};
+    b.flags = flags_.ptr;
+    b.kcount = kcount_.ptr;
+    b.koff = koff_.ptr;
+    b.knots = knots_.ptr;
+    hip_check(launch_adapt_verify_errors(items, positions, times, verify_pix_.ptr, verify_rad_.ptr, b, s.photo,
+                                         verify_err_.ptr, st),
+              "optical convergence probe launch");
+    std::vector<double> values(items);
+    int status = 0;
+    hip_check(hipMemcpyAsync(values.data(), verify_err_.ptr, sizeof(double) * items, hipMemcpyDeviceToHost, st),
+              "download optical convergence probes");
+    hip_check(hipMemcpyAsync(&status, status_.ptr, sizeof(int), hipMemcpyDeviceToHost, st), "probe status");
+    hip_check(hipStreamSynchronize(st), "download optical convergence probes");
+    if (status) fail(DVS_E_OPTICAL, "invalid ray in optical convergence probe");
+    dvs_optical_probe r{};
+    r.probes = items;
+    for (double v : values) {
+      if (v < 0) { ++r.exact_fallback_probes; continue; }
+      r.maximum_log_error = std::max(r.maximum_log_error, v);
+      r.mean_log_error += v;
+      if (v > tolerance) ++r.over_tolerance;
+    }
+    r.mean_log_error /= items - r.exact_fallback_probes > 0 ? items - r.exact_fallback_probes : 1;
+    std::sort(values.begin(), values.end());
+    r.p99_log_error = values[(size_t)(items * .99)];
+    return r;
+  }
+
+  // EventSensor::verify_receptor (event_sensor.cu:357-374): with numerical
+  // diagnostics on, the receptor check's measurement of the last adaptive
+  // window.
+  dvs_receptor_probe verify_receptor(int id, const dvs_optical_interval *ivs, int n) {
+    Segment &s = segment(id);
+    if (!s.diagnostics) fail(DVS_E_STATE, "enable numerical diagnostics before processing");
+    if (s.diag_step != s.window_step || s.window_step == ~0ull) fail(DVS_E_STATE, "no optical knots for receptor diagnostic");
+    if (!ivs || n != s.window_ivs || ivs[0].t0 != s.window_start || ivs[n - 1].t1 != s.window_end)
+      fail(DVS_E_ARGUMENT, "intervals differ from the last adaptive window");
+    return s.diag_probe;
+  }
+
   bool step_adaptive_once(int id, double tolerance, bool check) {
     Segment &s = segment(id);
     if (s.queue.empty()) fail(DVS_E_ARGUMENT, "empty adaptive optical horizon");
@@ -534,6 +617,7 @@ class Engine {
 
   void run(std::vector<StepSegment> &work, int pixel_total, int cell_total, int iv_total) {
     hipStream_t st = stream_;
+    ++step_serial_;
     const size_t lds = prepare(work, pixel_total, cell_total, iv_total);
     // Cells are traced in chunks so the [cell][pixel] radiance scratch stays
     // within budget: chunk k covers cells [k*C, (k+1)*C) of every segment.
@@ -822,7 +906,19 @@ class Engine {
     hipStream_t st = stream_;
     Timer step_t, refine_t, circuit_t;
     hip_check(hipEventRecord(step_t.start, st), "record");
+    ++step_serial_;
     const size_t lds = prepare(work, pixel_total, cell_total, iv_total);
+    last_lds_ = lds;
+    last_nseg_ = (int)work.size();
+    for (int i = 0; i < (int)work.size(); ++i) {
+      Segment &s = *work[i].seg;
+      s.window_step = step_serial_;
+      s.window_index = i;
+      s.window_pixel_offset = work[i].pixel_offset;
+      s.window_ivs = (int)work[i].ivs.size();
+      s.window_start = work[i].ivs.front().t0;
+      s.window_end = work[i].ivs.back().t1;
+    }
     const int nseg = (int)work.size();
     const int P = pixel_total;
     const int samples = work[0].seg->scene->view.samples, R = samples * samples, slots = 256 / R;
@@ -1064,7 +1160,7 @@ class Engine {
     // segments that check this window.
     std::vector<int> gate_seg;
     for (int i = 0; i < nseg; ++i)
-      if (work[i].check) gate_seg.push_back(i);
+      if (work[i].check || work[i].seg->diagnostics) gate_seg.push_back(i);
     const int ngate = (int)gate_seg.size();
     uint64_t gate_samples = 0;
     if (ngate > 0) {
@@ -1132,6 +1228,11 @@ class Engine {
         r.maximum_log_voltage_error = e.back();
         r.p99_log_voltage_error = e[(size_t)(np * .99)];
         r.mean_log_voltage_error /= np;
+        if (s.diagnostics) {
+          s.diag_probe = r;
+          s.diag_step = step_serial_;
+        }
+        if (!w.check) continue;  // measured for the diagnostics only
         s.last_probe = r;
         const double threshold = std::min(s.params.threshold_pos, s.params.threshold_neg);
         if (r.maximum_log_voltage_error > .02 * threshold || r.p99_log_voltage_error > .005 * threshold) {
@@ -1310,6 +1411,13 @@ class Engine {
   PinnedArray<int> host_event_seg_, host_words_, host_small_, host_fb_ends_;
   PinnedArray<AdaptSegStats> host_seg_stats_;
   DeviceArray<double2> knots_;
+  // engine steps run so far, and the last adaptive step's trace LDS bytes and segments
+  uint64_t step_serial_ = 0;
+  size_t last_lds_ = 0;
+  int last_nseg_ = 0;
+  DeviceArray<int> verify_pix_, verify_pose_;
+  DeviceArray<Pose> verify_poses_;
+  DeviceArray<double> verify_rad_, verify_err_;
   DeviceArray<AdaptSegStats> seg_stats_;
   DeviceArray<SegDev> gate_tab_, fb_tab_;
   DeviceArray<long long> gate_blocks_, fb_blocks_;
@@ -1400,6 +1508,68 @@ int dvs_sensor_raw_events(dvs_sensor s, dvs_event *out, int64_t capacity, int64_
   });
 }
 
+struct dvs_readout_s {
+  Readout fifo;
+  std::vector<dvs_event> ready;
+};
+
+int dvs_readout_create(double bandwidth, double latency, int64_t capacity, dvs_readout *out) {
+  return guarded([&] {
+    if (!out) fail(DVS_E_ARGUMENT, "null readout");
+    *out = nullptr;
+    if (!std::isfinite(bandwidth) || !std::isfinite(latency) || capacity < 0)
+      fail(DVS_E_ARGUMENT, "invalid readout bandwidth, latency or capacity");
+    auto r = std::make_unique<dvs_readout_s>();
+    r->fifo.bandwidth = bandwidth;
+    r->fifo.latency = latency;
+    r->fifo.capacity = (size_t)capacity;
+    *out = r.release();
+  });
+}
+
+int dvs_readout_destroy(dvs_readout r) {
+  delete r;
+  return DVS_OK;
+}
+
+int dvs_readout_submit(dvs_readout r, const dvs_event *events, int64_t n, double now, int64_t *ready_count) {
+  return guarded([&] {
+    if (!r || n < 0 || (n > 0 && !events)) fail(DVS_E_ARGUMENT, "invalid readout submission");
+    // stable order by (t, y, x) (event_readout.cpp:8-12), then the FIFO
+    std::vector<dvs_event> sorted(events, events + n);
+    std::stable_sort(sorted.begin(), sorted.end(), [](const dvs_event &a, const dvs_event &b) {
+      if (a.timestamp != b.timestamp) return a.timestamp < b.timestamp;
+      if (a.y != b.y) return a.y < b.y;
+      return a.x < b.x;
+    });
+    r->fifo.submit_sorted(sorted.data(), sorted.size(), now, r->ready);
+    if (ready_count) *ready_count = (int64_t)r->ready.size();
+  });
+}
+
+int dvs_readout_drain(dvs_readout r, int64_t *ready_count) {
+  return guarded([&] {
+    if (!r) fail(DVS_E_ARGUMENT, "null readout");
+    r->fifo.drain(r->ready);
+    if (ready_count) *ready_count = (int64_t)r->ready.size();
+  });
+}
+
+int dvs_readout_take(dvs_readout r, dvs_event *out, int64_t capacity, int64_t *count) {
+  return guarded([&] {
+    if (!r || !count || (capacity > 0 && !out)) fail(DVS_E_ARGUMENT, "null readout");
+    copy_out(r->ready, out, capacity, count, true);
+  });
+}
+
+int dvs_readout_stats(dvs_readout r, int64_t *dropped, int64_t *peak_queue) {
+  return guarded([&] {
+    if (!r) fail(DVS_E_ARGUMENT, "null readout");
+    if (dropped) *dropped = (int64_t)r->fifo.dropped;
+    if (peak_queue) *peak_queue = (int64_t)r->fifo.peak;
+  });
+}
+
 int dvs_sensor_drain(dvs_sensor s, int64_t *ready_count) {
   return guarded([&] {
     if (!s) fail(DVS_E_ARGUMENT, "null sensor");
@@ -1463,6 +1633,28 @@ int dvs_sensor_process_adaptive(dvs_sensor s, const dvs_optical_interval *ivs, i
     seg.fed_until = seg.last_time;
     if (accepted) *accepted = ok ? 1 : 0;
     if (ready_count) *ready_count = (int64_t)seg.ready.size();
+  });
+}
+
+int dvs_sensor_enable_diagnostics(dvs_sensor s) {
+  return guarded([&] {
+    if (!s) fail(DVS_E_ARGUMENT, "null sensor");
+    s->engine->segment(s->id).diagnostics = true;
+  });
+}
+
+int dvs_sensor_verify_receptor(dvs_sensor s, const dvs_optical_interval *ivs, int32_t n, dvs_receptor_probe *out) {
+  return guarded([&] {
+    if (!s || !out) fail(DVS_E_ARGUMENT, "null sensor");
+    *out = s->engine->verify_receptor(s->id, ivs, n);
+  });
+}
+
+int dvs_sensor_verify_optics(dvs_sensor s, const dvs_optical_interval *ivs, int32_t n, double tolerance,
+                             dvs_optical_probe *out) {
+  return guarded([&] {
+    if (!s || !out) fail(DVS_E_ARGUMENT, "null sensor");
+    *out = s->engine->verify_optics(s->id, ivs, n, tolerance);
   });
 }
 

@@ -96,6 +96,8 @@ hipError_t launch_init(int npix, const double *radiance, const PhotoParams &phot
 hipError_t launch_hapke(int64_t n, const float *ci, const float *ce, const float *cg, const float *alb,
                         const dvs_hapke &h, float *out, hipStream_t st);
 hipError_t launch_poisson(int64_t n, const double *lambda, const double *u, double *out, hipStream_t st);
+hipError_t launch_invert_hapke(int64_t n, const float *inc, const float *emi, const float *phase, const float *target,
+                               const dvs_hapke &h, float *out, hipStream_t st);
 hipError_t launch_advance(int64_t n, dvs_pixel_state *states, const dvs_pixel_params *params, const double *rates,
                           const double *t0, const double *t1, int capacity, double normal, int *counts,
                           dvs_crossing *events, hipStream_t st);

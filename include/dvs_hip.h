@@ -8,7 +8,8 @@
  *   DVS_E_ARGUMENT 3, DVS_E_HIP 4, DVS_E_STATE 5,
  * and dvs_last_error() returns the message of the calling thread's last
  * failure, worded as the reference's exceptions.  The C++ layer
- * (dvs/raycast.hpp in this repository) rethrows them as the reference does.
+ * (include/dvs_render.hpp and include/dvs/raycast.hpp in this repository,
+ * over dvs_render_amd.hpp) rethrows them as the reference does.
  *
  * Each declaration names the reference interface it replaces
  * (BugBubbles/DvsRender4DeepSpace, paths relative to its root).
@@ -118,6 +119,13 @@ typedef struct {
   int32_t pad;
 } dvs_crossing;
 
+/* OpticalProbeResult (include/dvs/optical_sampler.hpp:15): knot
+ * interpolation against native traces over 1024 pixels x 17 times. */
+typedef struct {
+  double maximum_log_error, p99_log_error, mean_log_error;
+  int32_t probes, over_tolerance, exact_fallback_probes;
+} dvs_optical_probe;
+
 /* Device timings and work counters of one sensor or scene handle. */
 typedef struct {
   double trace_ms, circuit_ms, events_ms, total_ms;
@@ -199,6 +207,9 @@ int dvs_scene_view(dvs_scene base, const dvs_camera *camera, const dvs_hapke *ha
 /* ~DeviceScene (device_scene.cu:38-43).  The scene is reference counted:
  * sensors created on it keep it alive. */
 int dvs_scene_destroy(dvs_scene scene);
+/* The camera of a scene view (OpticalScene::width, height, fx, fy, cx, cy,
+ * samples, include/dvs/optical_sensor.hpp:5-13). */
+int dvs_scene_camera(dvs_scene scene, dvs_camera *out);
 /* Tree depth and SoA sizes of an uploaded scene (diagnostics). */
 int dvs_scene_info(dvs_scene scene, int32_t *max_depth, int64_t *internal_nodes,
                    int64_t *leaves, int64_t *device_bytes);
@@ -258,6 +269,24 @@ int dvs_sensor_process_adaptive(dvs_sensor sensor, const dvs_optical_interval *i
                                 int32_t *accepted, int64_t *ready_count);
 /* DvsRayCaster::adaptive_cache_capacity (raycast.hpp:36-39): knot and exact
  * cell capacity per pixel (optical_sampler.cu:141-150). */
+/* EventSensor::enable_numerical_diagnostics (include/dvs/event_sensor.hpp:61):
+ * every later adaptive window also measures its receptor check (1024
+ * hashed pixels, native photocurrent against the knot interpolation), for
+ * verify_receptor; only windows processed with check_convergence use it to
+ * reject. */
+int dvs_sensor_enable_diagnostics(dvs_sensor sensor);
+/* EventSensor::verify_receptor (event_sensor.hpp:62, src/cuda/event_sensor.cu:
+ * 357-374) of the last adaptive window `intervals`: DVS_E_STATE "enable
+ * numerical diagnostics before processing" without diagnostics. */
+int dvs_sensor_verify_receptor(dvs_sensor sensor, const dvs_optical_interval *intervals, int32_t count,
+                               dvs_receptor_probe *out);
+/* EventSensor::verify_optics (event_sensor.hpp:57-59) = OpticalSampler::verify
+ * (src/cuda/optical_sampler.cu:86-103,195-215) on the last adaptive window
+ * `intervals`, right after it (DVS_E_STATE "refine before validating optical
+ * knots" otherwise): log errors of the knot-interpolated photocurrent against
+ * native traces of 1024 hashed pixels at 17 times. */
+int dvs_sensor_verify_optics(dvs_sensor sensor, const dvs_optical_interval *intervals, int32_t count,
+                             double tolerance, dvs_optical_probe *out);
 int dvs_sensor_adaptive_capacity(dvs_sensor sensor, int32_t *capacity);
 /* DvsRayCaster::refinement_statistics (raycast.hpp:55). */
 int dvs_sensor_refinement_stats(dvs_sensor sensor, dvs_refinement_stats *out);
@@ -277,6 +306,22 @@ int dvs_sensor_readout_stats(dvs_sensor sensor, int64_t *dropped, int64_t *peak_
 int dvs_sensor_states(dvs_sensor sensor, dvs_pixel_state *out);
 int dvs_sensor_stats(dvs_sensor sensor, dvs_stats *out);
 int dvs_sensor_reset_stats(dvs_sensor sensor);
+
+/* ---- EventReadout (include/dvs/event_sensor.hpp:24-37) on its own ------
+ * The bounded FIFO of src/event_readout.cpp:6-34, host side: submit sorts
+ * the events stably by (t, y, x) and queues them (completion = max(available,
+ * t) + 1 / bandwidth, delivered at completion + latency, dropped while
+ * `capacity` events are pending; bandwidth <= 0 delivers at once), then
+ * delivers every event ready by `now`.  Delivered events accumulate until
+ * dvs_readout_take (count: the number waiting). */
+typedef struct dvs_readout_s *dvs_readout;
+int dvs_readout_create(double bandwidth, double latency, int64_t capacity, dvs_readout *out);
+int dvs_readout_destroy(dvs_readout readout);
+int dvs_readout_submit(dvs_readout readout, const dvs_event *events, int64_t count, double now,
+                       int64_t *ready_count);
+int dvs_readout_drain(dvs_readout readout, int64_t *ready_count);
+int dvs_readout_take(dvs_readout readout, dvs_event *out, int64_t capacity, int64_t *count);
+int dvs_readout_stats(dvs_readout readout, int64_t *dropped, int64_t *peak_queue);
 
 /* ---- Multi-segment engine (batched and served forms) -----------------
  * No reference counterpart: many EventSensors ("segments"), each with its own
@@ -360,11 +405,36 @@ int dvs_trajectory_evaluate(dvs_trajectory trajectory, const float *times, int64
 int dvs_trajectory_intervals(dvs_trajectory trajectory, const double *times, int64_t n,
                              dvs_optical_interval *out);
 
+/* ---- The reference's __host__ __device__ physics, called on the host ---- */
+/* hapke_reflectance (include/dvs/hapke.hpp:131-151): BRDF x cos(i). */
+float dvs_hapke_reflectance(float cosi, float cose, float cosg, float albedo, const dvs_hapke *hapke);
+/* hapke_directional_reflectance (include/dvs/hapke.hpp:153-157). */
+float dvs_hapke_directional(const float normal[3], const float light[3], const float view[3], float albedo,
+                            const dvs_hapke *hapke);
+/* invert_hapke_if (include/dvs/hapke.hpp:160-172): angles in degrees,
+ * target I/F; NaN for invalid geometry or an unreachable target. */
+float dvs_invert_hapke_if(float incidence, float emission, float phase, float target, const dvs_hapke *hapke);
+/* receptor_log_at (include/dvs/pixel_model.hpp:18-26). */
+double dvs_receptor_log_at(double u, double rate, double dt, double charge);
+/* advance_pixel (include/dvs/pixel_model.hpp:35-102): integrates one pixel
+ * (the first six fields of `state` are the reference's PixelState; state
+ * is updated in place) at the constant photo-electron rate over [t0, t1],
+ * writing crossings to out[0 .. capacity).  normal(context) supplies every
+ * reset-noise draw (the reference's Rng::normal; NULL draws 0).  Returns
+ * the number of crossings, -1 past capacity, -2 for invalid arguments. */
+int32_t dvs_advance_pixel(dvs_pixel_state *state, const dvs_pixel_params *params, double rate, double t0,
+                          double t1, dvs_crossing *out, int32_t capacity, double (*normal)(void *context),
+                          void *context);
+
 /* ---- Device kernels exposed for unit parity (no reference counterpart;
  * they expose the reference's __host__ __device__ functions run on the GPU) */
 /* hapke_reflectance (include/dvs/hapke.hpp:131-151) over n inputs. */
 int dvs_hapke_eval(const float *cosi, const float *cose, const float *cosg, const float *albedo,
                    int64_t n, const dvs_hapke *hapke, float *out);
+/* invert_hapke_if (include/dvs/hapke.hpp:160-172) over n inputs (the
+ * per-pixel inversion of tools/cpp_tools/main_albedo_map.cu:97-101). */
+int dvs_invert_hapke_eval(const float *incidence, const float *emission, const float *phase,
+                          const float *target, int64_t n, const dvs_hapke *hapke, float *out);
 /* receptor_log_at (include/dvs/pixel_model.hpp:18-26) over n inputs. */
 int dvs_receptor_eval(const double *u, const double *rate, const double *dt, double charge,
                       int64_t n, double *out);

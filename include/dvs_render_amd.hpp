@@ -100,6 +100,12 @@ namespace detail {
 inline void check(int code) {
   if (code != DVS_OK) raise(code, dvs_last_error());
 }
+// The diagnostics' state errors are std::logic_error, as in the reference
+// (raycast.hpp:47-53, optical_sampler.cu:198).
+inline void check_logic(int code) {
+  if (code == DVS_E_STATE) throw std::logic_error(dvs_last_error());
+  check(code);
+}
 inline void check_io(int code) {
   if (code != DVS_OK) raise(code, dvs_io_last_error());
 }
@@ -447,6 +453,9 @@ inline void render_float_to_png_bytes(const float *linear, int width, int height
 // OpticalRefinementStats / ReceptorProbeResult of the adaptive solver.
 using OpticalRefinementStats = dvs_refinement_stats;
 using ReceptorProbeResult = dvs_receptor_probe;
+using OpticalProbeResult = dvs_optical_probe;  // include/dvs/optical_sampler.hpp:15
+constexpr double kDefaultOpticalLogTolerance = 1e-2;       // optical_sampler.hpp:6
+constexpr double kOpticalVoltageThresholdFraction = 1e-3;  // optical_sampler.hpp:7
 
 // DvsRayCaster (include/dvs/raycast.hpp:23-62).
 class DvsRayCaster {
@@ -503,6 +512,23 @@ class DvsRayCaster {
   OpticalRefinementStats refinement_statistics() const {
     OpticalRefinementStats r{};
     detail::check(dvs_sensor_refinement_stats(sensor_, &r));
+    return r;
+  }
+  // raycast.hpp:45-53: every later adaptive window measures its receptor
+  // check; verify_receptor returns that of the last window `intervals`,
+  // verify_optics its knots against native traces (std::logic_error before
+  // any adaptive window, or without the diagnostics for verify_receptor).
+  void enable_numerical_diagnostics() { detail::check(dvs_sensor_enable_diagnostics(sensor_)); }
+  ReceptorProbeResult verify_receptor(const std::vector<OpticalInterval> &intervals) {
+    ReceptorProbeResult r{};
+    detail::check_logic(dvs_sensor_verify_receptor(sensor_, reinterpret_cast<const dvs_optical_interval *>(intervals.data()),
+                                             (int32_t)intervals.size(), &r));
+    return r;
+  }
+  OpticalProbeResult verify_optics(const std::vector<OpticalInterval> &intervals, double tolerance) {
+    OpticalProbeResult r{};
+    detail::check_logic(dvs_sensor_verify_optics(sensor_, reinterpret_cast<const dvs_optical_interval *>(intervals.data()),
+                                           (int32_t)intervals.size(), tolerance, &r));
     return r;
   }
   void drain(std::vector<Event> &events) {
@@ -600,6 +626,274 @@ class CamRayCaster {
 
  private:
   std::shared_ptr<DeviceScene> scene_;
+};
+
+// PixelParameters / PixelState / PixelEvent and the receptor model
+// (include/dvs/pixel_model.hpp:7-102), host side.
+struct PixelParameters {
+  double threshold_pos = .3, threshold_neg = .3;
+  double charge_scale = 200, refractory_time = .0002;
+  double reference_leak = 0, reset_noise_std = 0;
+};
+struct PixelState {
+  double receptor_log = 0, reference_log = 0, refractory_until = -1;
+  double threshold_pos = .3, threshold_neg = .3;
+  bool reset_pending = false;
+};
+struct PixelEvent { double time; int polarity; };
+inline double receptor_log_at(double u, double rate, double dt, double charge) {
+  return dvs_receptor_log_at(u, rate, dt, charge);
+}
+inline double contrast_at(double u, double rate, double dt, double ref, const PixelParameters &p) {
+  return receptor_log_at(u, rate, dt, p.charge_scale) - ref + p.reference_leak * dt;
+}
+// advance_pixel<Rng> (pixel_model.hpp:35-102): rng.normal() supplies every
+// reset-noise draw.  Returns the crossings written to out, -1 past capacity.
+template <class Rng>
+int advance_pixel(PixelState &s, const PixelParameters &p, double rate, double t0, double t1, PixelEvent *out,
+                  int capacity, Rng &rng) {
+  dvs_pixel_state c{};
+  c.receptor_log = s.receptor_log;
+  c.reference_log = s.reference_log;
+  c.refractory_until = s.refractory_until;
+  c.threshold_pos = s.threshold_pos;
+  c.threshold_neg = s.threshold_neg;
+  c.reset_pending = s.reset_pending ? 1 : 0;
+  const dvs_pixel_params q{p.threshold_pos, p.threshold_neg, p.charge_scale, p.refractory_time, p.reference_leak,
+                           p.reset_noise_std};
+  std::vector<dvs_crossing> buf((size_t)std::max(capacity, 0));
+  double (*normal)(void *) = [](void *context) -> double { return static_cast<Rng *>(context)->normal(); };
+  const int n = dvs_advance_pixel(&c, &q, rate, t0, t1, buf.data(), capacity, normal, &rng);
+  if (n == -2) throw std::invalid_argument("invalid pixel circuit arguments");
+  s.receptor_log = c.receptor_log;
+  s.reference_log = c.reference_log;
+  s.refractory_until = c.refractory_until;
+  s.reset_pending = c.reset_pending != 0;
+  for (int i = 0; i < n; ++i) out[i] = PixelEvent{buf[i].time, buf[i].polarity};
+  return n;
+}
+
+// Hapke BRDF (include/dvs/hapke.hpp:131-172), host side.
+inline float hapke_reflectance(float cosi, float cose, float cosg, float albedo, const HapkeConfig &p) {
+  const dvs_hapke h = detail::to_c(p);
+  return dvs_hapke_reflectance(cosi, cose, cosg, albedo, &h);
+}
+inline float hapke_directional_reflectance(const float3 &normal, const float3 &light, const float3 &view,
+                                           float albedo, const HapkeConfig &p) {
+  const dvs_hapke h = detail::to_c(p);
+  const float n[3] = {normal.x, normal.y, normal.z}, l[3] = {light.x, light.y, light.z},
+              v[3] = {view.x, view.y, view.z};
+  return dvs_hapke_directional(n, l, v, albedo, &h);
+}
+inline float invert_hapke_if(float inc, float emi, float phase, float target, const HapkeConfig &p) {
+  const dvs_hapke h = detail::to_c(p);
+  return dvs_invert_hapke_if(inc, emi, phase, target, &h);
+}
+
+// EventReadout (include/dvs/event_sensor.hpp:24-37): the bounded FIFO on
+// its own.  submit sorts by (t, y, x), queues, and returns in `ready` the
+// events delivered by `now`.
+class EventReadout {
+ public:
+  EventReadout(double bandwidth, double latency, std::size_t capacity) {
+    detail::check(dvs_readout_create(bandwidth, latency, (int64_t)capacity, &r_));
+  }
+  ~EventReadout() { dvs_readout_destroy(r_); }
+  EventReadout(const EventReadout &) = delete;
+  EventReadout &operator=(const EventReadout &) = delete;
+  void submit(std::vector<Event> events, double now, std::vector<Event> &ready) {
+    int64_t n = 0;
+    detail::check(dvs_readout_submit(r_, events.data(), (int64_t)events.size(), now, &n));
+    take(ready, n);
+  }
+  void drain(std::vector<Event> &ready) {
+    int64_t n = 0;
+    detail::check(dvs_readout_drain(r_, &n));
+    take(ready, n);
+  }
+  std::size_t dropped() const {
+    int64_t d = 0, p = 0;
+    detail::check(dvs_readout_stats(r_, &d, &p));
+    return (std::size_t)d;
+  }
+  std::size_t peak_queue() const {
+    int64_t d = 0, p = 0;
+    detail::check(dvs_readout_stats(r_, &d, &p));
+    return (std::size_t)p;
+  }
+
+ private:
+  void take(std::vector<Event> &ready, int64_t n) {
+    ready.resize((size_t)n);
+    int64_t got = 0;
+    detail::check(dvs_readout_take(r_, ready.data(), n, &got));
+    ready.resize((size_t)got);
+  }
+  dvs_readout r_ = nullptr;
+};
+
+// OpticalScene: the scene view the sensor classes take, DeviceScene::view().
+using OpticalScene = dvs_scene;
+// OpticalValue (optical_sampler.hpp:8): mean radiance, mean hit depth (-1
+// without a hit) and the sub-ray grid size of one pixel.
+struct OpticalValue {
+  double radiance = 0;
+  float depth = -1;
+  int samples = 0;
+};
+
+// OpticalSampler (include/dvs/optical_sampler.hpp:17-48).  evaluate renders
+// poses into host memory (the values stay readable through the returned
+// pointer and download).  The adaptive refinement runs inside
+// EventSensor::process_adaptive; statistics() and knot_capacity() report
+// the sensor the sampler last refined with (zero before).
+class OpticalSampler {
+ public:
+  OpticalSampler(int width, int height, int maximum_samples)
+      : width_(width), height_(height), maximum_(maximum_samples) {}
+  OpticalSampler(const OpticalSampler &) = delete;
+  OpticalSampler &operator=(const OpticalSampler &) = delete;
+  const OpticalValue *evaluate(const OpticalScene &scene, const std::vector<OpticalState> &states) {
+    dvs_camera cam{};
+    detail::check(dvs_scene_camera(scene, &cam));
+    const size_t n = (size_t)width_ * height_;
+    if (states.empty() || cam.width * cam.height != width_ * height_ || cam.samples != maximum_)
+      throw std::invalid_argument("invalid parallel optical request");
+    std::vector<float> image(n * states.size()), depth(n * states.size());
+    std::vector<double> radiance(n * states.size());
+    detail::check(dvs_frame_render(scene, reinterpret_cast<const dvs_optical_state *>(states.data()),
+                                   (int32_t)states.size(), image.data(), depth.data(), radiance.data()));
+    values_.resize(n * states.size());
+    for (size_t i = 0; i < values_.size(); ++i) values_[i] = OpticalValue{radiance[i], depth[i], maximum_};
+    return values_.data();
+  }
+  void download(std::vector<OpticalValue> &values) const { values = values_; }
+  OpticalRefinementStats statistics() const {
+    OpticalRefinementStats r{};
+    if (sensor_) detail::check(dvs_sensor_refinement_stats(sensor_, &r));
+    return r;
+  }
+  int knot_capacity() const {
+    int32_t c = 0;
+    if (sensor_) detail::check(dvs_sensor_adaptive_capacity(sensor_, &c));
+    return c;
+  }
+
+ private:
+  friend class EventSensor;
+  int width_, height_, maximum_;
+  std::vector<OpticalValue> values_;
+  dvs_sensor sensor_ = nullptr;  // the sensor of the last refinement
+};
+
+// EventSensor (include/dvs/event_sensor.hpp:40-79).  The device state is
+// created for the scene of the first initialize; later calls take the same
+// scene.
+class EventSensor {
+ public:
+  EventSensor(int width, int height, const PhysicConfig &config)
+      : width_(width), height_(height), physic_(detail::to_c(config)) {
+    if (width <= 0 || height <= 0 || config.max_events_per_step < 1)
+      throw std::invalid_argument("invalid event sensor dimensions or capacity");
+  }
+  ~EventSensor() { dvs_sensor_destroy(sensor_); }
+  EventSensor(const EventSensor &) = delete;
+  EventSensor &operator=(const EventSensor &) = delete;
+  void initialize(const OpticalScene &scene, const OpticalState &state, double timestamp) {
+    bind(scene);
+    detail::check(dvs_sensor_initialize(sensor_, reinterpret_cast<const dvs_optical_state *>(&state), timestamp));
+  }
+  void process(const OpticalScene &scene, const OpticalInterval &interval, std::vector<Event> &events) {
+    process_many(scene, {interval}, events);
+  }
+  void process_many(const OpticalScene &scene, const std::vector<OpticalInterval> &intervals,
+                    std::vector<Event> &events) {
+    bind(scene);
+    int64_t ready = 0;
+    detail::check(dvs_sensor_process(sensor_, reinterpret_cast<const dvs_optical_interval *>(intervals.data()),
+                                     (int32_t)intervals.size(), &ready));
+    take(events, ready);
+  }
+  bool process_adaptive(const OpticalScene &scene, const std::vector<OpticalInterval> &intervals,
+                        OpticalSampler &sampler, double log_tolerance, std::vector<Event> &events,
+                        bool check_convergence = false) {
+    bind(scene);
+    int32_t accepted = 0;
+    int64_t ready = 0;
+    detail::check(dvs_sensor_process_adaptive(sensor_,
+                                              reinterpret_cast<const dvs_optical_interval *>(intervals.data()),
+                                              (int32_t)intervals.size(), log_tolerance, check_convergence ? 1 : 0,
+                                              &accepted, &ready));
+    sampler.sensor_ = sensor_;
+    take(events, ready);
+    return accepted != 0;
+  }
+  OpticalProbeResult verify_optics(const OpticalScene &scene, OpticalSampler &sampler,
+                                   const std::vector<OpticalInterval> &ivs, double tolerance) {
+    bind(scene);
+    if (sampler.sensor_ != sensor_) throw std::logic_error("refine before validating optical knots");
+    OpticalProbeResult r{};
+    detail::check_logic(dvs_sensor_verify_optics(sensor_, reinterpret_cast<const dvs_optical_interval *>(ivs.data()),
+                                           (int32_t)ivs.size(), tolerance, &r));
+    return r;
+  }
+  const ReceptorProbeResult &last_convergence() const {
+    if (sensor_) detail::check(dvs_sensor_last_convergence(sensor_, &convergence_));
+    return convergence_;
+  }
+  void enable_numerical_diagnostics() {
+    diagnostics_ = true;
+    if (sensor_) detail::check(dvs_sensor_enable_diagnostics(sensor_));
+  }
+  ReceptorProbeResult verify_receptor(const OpticalScene &scene, OpticalSampler &,
+                                      const std::vector<OpticalInterval> &intervals) {
+    bind(scene);
+    ReceptorProbeResult r{};
+    detail::check_logic(dvs_sensor_verify_receptor(sensor_, reinterpret_cast<const dvs_optical_interval *>(intervals.data()),
+                                             (int32_t)intervals.size(), &r));
+    return r;
+  }
+  void drain(std::vector<Event> &events) {
+    if (!sensor_) { events.clear(); return; }
+    int64_t ready = 0;
+    detail::check(dvs_sensor_drain(sensor_, &ready));
+    take(events, ready);
+  }
+  std::size_t dropped() const { return readout_stat(false); }
+  std::size_t peak_queue() const { return readout_stat(true); }
+
+ private:
+  void bind(const OpticalScene &scene) {
+    if (sensor_) {
+      if (scene != scene_) throw std::invalid_argument("EventSensor is bound to the scene it was initialized with");
+      return;
+    }
+    dvs_camera cam{};
+    detail::check(dvs_scene_camera(scene, &cam));
+    if (cam.width != width_ || cam.height != height_)
+      throw std::invalid_argument("event sensor dimensions differ from the scene's camera");
+    detail::check(dvs_sensor_create(scene, &physic_, 1, 1, &sensor_));
+    scene_ = scene;
+    if (diagnostics_) detail::check(dvs_sensor_enable_diagnostics(sensor_));
+  }
+  std::size_t readout_stat(bool peak) const {
+    if (!sensor_) return 0;
+    int64_t d = 0, p = 0;
+    detail::check(dvs_sensor_readout_stats(sensor_, &d, &p));
+    return (std::size_t)(peak ? p : d);
+  }
+  void take(std::vector<Event> &events, int64_t ready) {
+    events.resize((size_t)ready);
+    int64_t got = 0;
+    detail::check(dvs_sensor_take(sensor_, events.data(), ready, &got));
+    events.resize((size_t)got);
+  }
+  int width_, height_;
+  dvs_physic physic_;
+  dvs_sensor sensor_ = nullptr;
+  OpticalScene scene_ = nullptr;
+  bool diagnostics_ = false;
+  mutable ReceptorProbeResult convergence_{};
 };
 
 // EventStreamWriter / BinaryEventStreamWriter / Hdf5EventStreamWriter

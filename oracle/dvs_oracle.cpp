@@ -799,6 +799,12 @@ struct Sensor {
   std::vector<uint64_t> shot_block, reset_block;
   Readout readout;
   int64_t shadow_rays = 0;
+  // the last adaptive window's knots ({fraction, radiance} pairs, `capacity`
+  // per pixel; count -1 for a native fallback pixel) for verify_optics
+  std::vector<double> win_knots;
+  std::vector<int> win_kcount;
+  std::vector<orc_interval> win_ivs;
+  int win_capacity = 0;
   Sensor(int w, int h, const orc_physic &c, bool effects, bool aer)
       : width(w), height(h), count(w * h), max_events(c.max_events_per_step),
         seed((uint64_t)(uint32_t)c.random_seed), threshold_std(c.threshold_std),
@@ -1237,6 +1243,11 @@ int orc_sensor_process_adaptive(void *sp, const void *scp, const orc_interval *i
     stats_out->maximum_knots = max_knots; stats_out->exact_segments = exact_segments;
     stats_out->native_cell_queries = native_queries;
   }
+  s.win_knots.resize(knots.size() * 2);
+  std::memcpy(s.win_knots.data(), knots.data(), sizeof(Knot) * knots.size());
+  s.win_kcount = kcount;
+  s.win_ivs.assign(ivs, ivs + n);
+  s.win_capacity = capacity;
   // Convergence gate: capture_mean_probes / advance_mean_probes /
   // verify_receptor (event_sensor.cu:134-150,318-326,357-374).
   if (check) {
@@ -1340,6 +1351,51 @@ int orc_sensor_process_adaptive(void *sp, const void *scp, const orc_interval *i
   if (accepted) *accepted = 1;
   if (ready_count) *ready_count = (int64_t)out.size();
   if (ready) std::memcpy(ready, out.data(), sizeof(orc_event) * std::min<int64_t>(ready_cap, out.size()));
+  return 0;
+}
+// OpticalSampler::verify / verify_knots (optical_sampler.cu:86-103,195-215) on
+// the last adaptive window: 1024 hashed pixels x 17 times, the knot
+// interpolation of the photocurrent against the native trace.  out: max,
+// p99, mean log error; iout: probes, over tolerance, fallback probes.
+int orc_sensor_verify_optics(void *sp, const void *scp, double tolerance, double out[3], int iout[3], int threads) {
+  Sensor &s = *static_cast<Sensor *>(sp);
+  const Scene &sc = *static_cast<const Scene *>(scp);
+  if (s.win_ivs.empty()) return 5;
+  set_threads(threads);
+  constexpr int positions = 1024, times = 17, total = positions * times;
+  const int n = (int)s.win_ivs.size();
+  const double start = s.win_ivs[0].t0, end = s.win_ivs[n - 1].t1;
+  std::vector<double> values(total);
+  int invalid = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(| : invalid)
+  for (int work = 0; work < total; ++work) {
+    const int pixel = (int)((uint64_t)(work % positions) * 2654435761ULL % (uint64_t)s.count), probe = work / positions;
+    if (s.win_kcount[pixel] < 0) { values[work] = -1; continue; }
+    const double fraction = (probe + .5) / times, t = start + (end - start) * fraction;
+    orc_state pose; native_pose(s.win_ivs.data(), n, t, pose);
+    const PixelSample o = sample_pixel(sc, pixel, pose);
+    if (!o.valid) invalid = 1;
+    const Knot *v = reinterpret_cast<const Knot *>(s.win_knots.data()) + (size_t)pixel * s.win_capacity;
+    int a = 0;
+    while (a + 2 < s.win_kcount[pixel] && fraction > fabs(v[a + 1].fraction)) ++a;
+    if (v[a + 1].fraction < 0) { values[work] = -1; continue; }
+    const double weight = (fraction - fabs(v[a].fraction)) / (fabs(v[a + 1].fraction) - fabs(v[a].fraction));
+    const double estimate = v[a].radiance + (v[a + 1].radiance - v[a].radiance) * weight;
+    const double actual = o.radiance * s.photo.electrons_per_radiance + s.photo.dark_electrons;
+    const double predicted = estimate * s.photo.electrons_per_radiance + s.photo.dark_electrons;
+    values[work] = fabs(log(fmax(1e-30, actual) / fmax(1e-30, predicted)));
+  }
+  if (invalid) return 2;
+  double mx = 0, mean = 0; int over = 0, fb = 0;
+  for (double value : values) {
+    if (value < 0) { ++fb; continue; }
+    mx = std::max(mx, value); mean += value;
+    if (value > tolerance) ++over;
+  }
+  mean /= total - fb > 0 ? total - fb : 1;
+  std::sort(values.begin(), values.end());
+  out[0] = mx; out[1] = values[(size_t)(total * .99)]; out[2] = mean;
+  iout[0] = total; iout[1] = over; iout[2] = fb;
   return 0;
 }
 int orc_sensor_drain(void *sp, orc_event *ready, int64_t cap, int64_t *count) {

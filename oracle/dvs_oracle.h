@@ -228,6 +228,11 @@ int orc_sensor_process_adaptive(void *sensor, const void *scene, const orc_inter
                                 int64_t *raw_count, orc_event *ready, int64_t ready_capacity,
                                 int64_t *ready_count, int threads);
 int orc_sensor_drain(void *sensor, orc_event *ready, int64_t capacity, int64_t *count);
+/* OpticalSampler::verify (optical_sampler.cu:86-103,195-215) on the last
+ * adaptive window's knots: out = {max, p99, mean} log error, iout = {probes,
+ * over tolerance, fallback probes}. */
+int orc_sensor_verify_optics(void *sensor, const void *scene, double tolerance, double out[3], int iout[3],
+                             int threads);
 int64_t orc_sensor_dropped(const void *sensor);
 int64_t orc_sensor_peak_queue(const void *sensor);
 int orc_sensor_states(const void *sensor, orc_pixel_state *out);

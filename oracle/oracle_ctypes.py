@@ -127,6 +127,7 @@ def lib() -> ctypes.CDLL:
             "orc_sensor_process_adaptive": (I, [_P, _P, _P, I, ctypes.c_double, I, I, ctypes.POINTER(I), _P, _P,
                                                 _P, I64, ctypes.POINTER(I64), _P, I64, ctypes.POINTER(I64), I]),
             "orc_sensor_drain": (I, [_P, _P, I64, ctypes.POINTER(I64)]),
+            "orc_sensor_verify_optics": (I, [_P, _P, D, _P, _P, I]),
             "orc_sensor_dropped": (I64, [_P]),
             "orc_sensor_peak_queue": (I64, [_P]),
             "orc_sensor_states": (I, [_P, _P]),
@@ -296,6 +297,15 @@ class Sensor:
                                                  ctypes.byref(nd), threads)
         as_dict = lambda c: {name: getattr(c, name) for name, _ in c._fields_}
         return code, bool(acc.value), raw[: nr.value].copy(), ready[: nd.value].copy(), as_dict(st), as_dict(pr)
+
+    def verify_optics(self, scene: Scene, tolerance=1e-2, threads=0):
+        """OpticalSampler::verify of the last adaptive window: (code, dict)."""
+        out = np.zeros(3)
+        iout = np.zeros(3, np.int32)
+        code = lib().orc_sensor_verify_optics(self.handle, scene.handle, float(tolerance), ptr(out), ptr(iout), threads)
+        keys = ("maximum_log_error", "p99_log_error", "mean_log_error")
+        return code, {**dict(zip(keys, map(float, out))),
+                      **dict(zip(("probes", "over_tolerance", "exact_fallback_probes"), map(int, iout)))}
 
     def drain(self):
         cap = self.width * self.height * 64 + 1024
