@@ -97,7 +97,8 @@ API_SELFTEST = BINDIR / "dvs_api_selftest"
 
 # Programs on the C++ API: the self-test and the bin/dvs and bin/cam front
 # ends (apps/main_dvs.cpp, apps/main_cam.cpp).
-PROGRAMS = {"dvs_api_selftest": "api_selftest.cpp", "dvs": "main_dvs.cpp", "cam": "main_cam.cpp"}
+PROGRAMS = {"dvs_api_selftest": "api_selftest.cpp", "dvs": "main_dvs.cpp", "cam": "main_cam.cpp",
+            "cam_blur": "main_cam_blur.cpp"}
 
 
 def _link_program(name: str) -> Path:
@@ -119,8 +120,8 @@ def build_api_selftest() -> Path:
 
 
 def build_apps() -> list[Path]:
-    """Links bin/dvs and bin/cam."""
-    return [_link_program("dvs"), _link_program("cam")]
+    """Links bin/dvs, bin/cam and bin/cam_blur."""
+    return [_link_program(name) for name in ("dvs", "cam", "cam_blur")]
 
 
 def build_oracle() -> Path:

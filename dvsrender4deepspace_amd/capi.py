@@ -210,6 +210,7 @@ def load_library() -> ctypes.CDLL:
         "dvs_frame_render": (ctypes.c_int, [_P, _P, _I32, _P, _P, _P]),
         "dvs_frame_render_device": (ctypes.c_int, [_P, _P, _I32, _P, _P]),
         "dvs_frame_render_device_noalloc": (ctypes.c_int, [_P, _P, _I32, _P, _P]),
+        "dvs_frame_blur": (ctypes.c_int, [_P, _P, _P, _P, _I32, _I32, _P]),
         "dvs_scene_stats": (ctypes.c_int, [_P, ctypes.POINTER(_StatsC)]),
         "dvs_frame_traversal_stats": (ctypes.c_int, [_P, _P, _P]),
         "dvs_sensor_create": (ctypes.c_int, [_P, ctypes.POINTER(_PhysicC), _I32, _I32, ctypes.POINTER(_P)]),
@@ -383,6 +384,22 @@ class Scene:
         rad = np.empty(shape, np.float64) if with_radiance else None
         _check(load_library().dvs_frame_render(self.handle, _ptr(p), len(p), _ptr(img), _ptr(dep), _ptr(rad)))
         return (img, dep, rad) if with_radiance else (img, dep)
+
+    def render_blur(self, poses: np.ndarray, weights, cell_offsets, wide: bool = False) -> np.ndarray:
+        """Motion-blur exposures (apps/main_cam_blur.cpp:64-83): exposure e is
+        the weighted sum, in cell order, of the frames of poses[offsets[e] :
+        offsets[e+1]]; float sums of frame x float(weight), or with `wide`
+        double sums cast at the end (apps/main_cam_blur_windows.cpp:77,81).
+        Returns (exposures, H, W) float32."""
+        p = np.ascontiguousarray(np.atleast_1d(poses), dtype=STATE_DTYPE)
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        o = np.ascontiguousarray(cell_offsets, dtype=np.int32)
+        if o.ndim != 1 or len(o) < 2 or o[-1] != len(p) or len(w) != len(p):
+            raise ValueError("cell_offsets must end at the number of poses, one weight per pose")
+        out = np.empty((len(o) - 1, self.camera.height, self.camera.width), np.float32)
+        _check(load_library().dvs_frame_blur(self.handle, _ptr(p), _ptr(w), _ptr(o), len(o) - 1, int(wide),
+                                             _ptr(out)))
+        return out
 
     def render_device(self, device_poses_ptr: int, count: int, device_image_ptr: int, device_depth_ptr: int):
         _check(load_library().dvs_frame_render_device(self.handle, ctypes.c_void_p(device_poses_ptr), count,

@@ -10,6 +10,7 @@
 //       trajectory intervals built on the device driving a DvsRayCaster;
 //       EventSensor, OpticalSampler and the diagnostics.
 // Prints one line per check and exits non-zero on the first failure.
+#include <algorithm>
 #include <cstdio>
 #include <iostream>
 #include <stdexcept>
@@ -189,6 +190,25 @@ int gpu(const std::string &out) {
   cam.render_main(dvs::make_float3(0, 0, 10), identity, dvs::make_float3(0, 0, 1), 1000, image, depth);
   const int centre = 18 * 48 + 24;
   expect(std::fabs(depth[centre] - 10) < 0.05f && image[centre] > 0, "nadir frame: depth 10 m at the centre");
+  {
+    // One exposure of two cells (both the nadir pose, weights 1/4 and 3/4)
+    // and one of a single unit cell: the float sums of render_main frames.
+    dvs::CamRayCaster::BlurCell cell;
+    cell.camera_origin = dvs::make_float3(0, 0, 10);
+    std::copy(identity, identity + 9, cell.camera_world);
+    cell.light_dir = dvs::make_float3(0, 0, 1);
+    cell.light_intensity = 1000;
+    std::vector<dvs::CamRayCaster::BlurCell> cells(3, cell);
+    cells[0].weight = .25;
+    cells[1].weight = .75;
+    cells[2].weight = 1;
+    std::vector<float> blurred;
+    cam.render_blur(cells, {0, 2, 3}, false, blurred);
+    bool same = blurred.size() == 2 * image.size();
+    for (size_t i = 0; same && i < image.size(); ++i)
+      same = blurred[i] == image[i] * .25f + image[i] * .75f && blurred[image.size() + i] == image[i];
+    expect(same, "render_blur sums render_main frames");
+  }
 
   dvs::CameraTrajectoryState s;
   s.position.v[2] = 10;

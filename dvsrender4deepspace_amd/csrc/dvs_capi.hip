@@ -298,6 +298,62 @@ int dvs_frame_render_device_noalloc(dvs_scene s, const dvs_optical_state *device
   });
 }
 
+int dvs_frame_blur(dvs_scene s, const dvs_optical_state *poses, const double *weights, const int32_t *cell_offsets,
+                   int32_t exposure_count, int32_t wide, float *image) {
+  return guarded([&] {
+    if (!s || !poses || !weights || !cell_offsets || !image || exposure_count <= 0)
+      fail(DVS_E_ARGUMENT, "invalid motion-blur request");
+    if (cell_offsets[0] != 0) fail(DVS_E_ARGUMENT, "motion-blur cell offsets must start at 0");
+    for (int e = 0; e < exposure_count; ++e)
+      if (cell_offsets[e + 1] < cell_offsets[e]) fail(DVS_E_ARGUMENT, "motion-blur cell offsets must not decrease");
+    if (cell_offsets[exposure_count] <= 0) fail(DVS_E_ARGUMENT, "motion-blur request without cells");
+    for (int c = 0; c < cell_offsets[exposure_count]; ++c)
+      if (!std::isfinite(weights[c])) fail(DVS_E_ARGUMENT, "motion-blur weights must be finite");
+    const size_t npix = (size_t)s->view.width * s->view.height;
+    hipStream_t st = s->stream;
+    // Exposures go in groups of whole exposures whose frames fit kBlurFrameBytes
+    // (a single larger exposure forms a group of its own): one multi-pose trace
+    // launch renders every cell of a group, one launch sums each exposure.
+    constexpr size_t kBlurFrameBytes = size_t(1) << 30;
+    const size_t max_cells = std::max<size_t>(1, kBlurFrameBytes / (npix * sizeof(float)));
+    DeviceArray<dvs_optical_state> states;
+    DeviceArray<double> wdev;
+    DeviceArray<int> odev;
+    std::vector<int> local;
+    int e0 = 0;
+    while (e0 < exposure_count) {
+      int e1 = e0 + 1;
+      while (e1 < exposure_count && e1 - e0 < 65535 &&
+             (size_t)(cell_offsets[e1 + 1] - cell_offsets[e0]) <= max_cells)
+        ++e1;
+      const int c0 = cell_offsets[e0], n = cell_offsets[e1] - c0;
+      float *out = image + (size_t)e0 * npix;
+      if (n == 0) {
+        std::fill(out, out + (size_t)(e1 - e0) * npix, 0.0f);
+        e0 = e1;
+        continue;
+      }
+      local.assign(cell_offsets + e0, cell_offsets + e1 + 1);
+      for (int &o : local) o -= c0;
+      states.upload(poses + c0, n, "upload motion-blur poses", st);
+      wdev.upload(weights + c0, n, "upload motion-blur weights", st);
+      odev.upload(local.data(), local.size(), "upload motion-blur offsets", st);
+      s->poses.reserve(n, "optical poses");
+      hip_check(launch_state_poses(n, states.ptr, s->poses.ptr, st), "pose conversion");
+      s->image.reserve(npix * n, "optical image");
+      frame_trace(s, s->poses.ptr, n, nullptr, s->image.ptr, nullptr, st);
+      s->blur.reserve(npix * (e1 - e0), "motion-blur image");
+      hip_check(launch_blur_accumulate((int)npix, e1 - e0, odev.ptr, s->image.ptr, wdev.ptr, wide ? 1 : 0,
+                                       s->blur.ptr, st),
+                "motion-blur accumulation");
+      hip_check(hipMemcpyAsync(out, s->blur.ptr, npix * (e1 - e0) * sizeof(float), hipMemcpyDeviceToHost, st),
+                "download motion-blur image");
+      hip_check(hipStreamSynchronize(st), "motion-blur wait");
+      e0 = e1;
+    }
+  });
+}
+
 int dvs_frame_traversal_stats(dvs_scene s, const dvs_optical_state *pose, uint64_t totals[8]) {
   return guarded([&] {
     if (!s || !pose || !totals) fail(DVS_E_ARGUMENT, "invalid traversal statistics request");

@@ -368,6 +368,32 @@ __global__ void fill_int_kernel(int *p, int value, int n) {
   if (i < n) p[i] = value;
 }
 
+// Motion-blur exposure e, pixel p: the cells [offsets[e], offsets[e+1]) of
+// `frames` (cell-major, npix floats each) summed in cell order with their
+// weights, as bin/cam_blur does (apps/main_cam_blur.cpp:76: float sum of
+// frame x float(weight), each product and sum rounded to float) or, with
+// `wide`, as bin/cam_blur_windows does (apps/main_cam_blur_windows.cpp:77,81:
+// double sum of weight x frame, cast to float at the end).  The products and
+// sums are explicitly rounded so that no multiply-add is contracted.
+__global__ __launch_bounds__(256) void blur_accumulate_kernel(int npix, const int *offsets, const float *frames,
+                                                            const double *weights, int wide, float *out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = blockIdx.y;
+  if (p >= npix) return;
+  const int begin = offsets[e], end = offsets[e + 1];
+  if (wide) {
+    double acc = 0.0;
+    for (int c = begin; c < end; ++c)
+      acc = __dadd_rn(acc, __dmul_rn(weights[c], (double)frames[(size_t)c * npix + p]));
+    out[(size_t)e * npix + p] = (float)acc;
+  } else {
+    float acc = 0.0f;
+    for (int c = begin; c < end; ++c)
+      acc = __fadd_rn(acc, __fmul_rn(frames[(size_t)c * npix + p], (float)weights[c]));
+    out[(size_t)e * npix + p] = acc;
+  }
+}
+
 // Block k moves staged copy k: 16-byte words when source, destination and
 // size allow, else 4-byte words, else bytes.
 __global__ __launch_bounds__(256) void staged_scatter_kernel(const char *mirror, dvsamd::ScatterArgs a) {
@@ -609,6 +635,14 @@ hipError_t launch_permute_multi(const SegDev *segs, int events, const unsigned *
 hipError_t launch_fill_int(int *p, int value, int n, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   fill_int_kernel<<<(n + 255) / 256, 256, 0, st>>>(p, value, n);
+  return hipGetLastError();
+}
+hipError_t launch_blur_accumulate(int npix, int exposures, const int *offsets, const float *frames,
+                                  const double *weights, int wide, float *out, hipStream_t st) {
+  if (npix <= 0 || exposures <= 0) return hipSuccess;
+  if (exposures > 65535) return hipErrorInvalidConfiguration;
+  blur_accumulate_kernel<<<dim3((npix + 255) / 256, exposures), 256, 0, st>>>(npix, offsets, frames, weights, wide,
+                                                                             out);
   return hipGetLastError();
 }
 hipError_t launch_staged_scatter(const char *mirror, const ScatterArgs &a, hipStream_t st) {

@@ -83,6 +83,8 @@ hipError_t launch_permute_multi(const SegDev *segs, int events, const unsigned *
                                 const int *seg, const int *pixel, const int *polarity, const float *depth,
                                 dvs_event *out, int *out_seg, hipStream_t st);
 hipError_t launch_fill_int(int *p, int value, int n, hipStream_t st);
+hipError_t launch_blur_accumulate(int npix, int exposures, const int *offsets, const float *frames,
+                                  const double *weights, int wide, float *out, hipStream_t st);
 
 hipError_t launch_cell_poses(int cells, const double *t0, const double *t1, const int *iv_index,
                              const dvs_optical_interval *ivs, Pose *poses, hipStream_t st);

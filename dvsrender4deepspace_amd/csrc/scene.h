@@ -32,7 +32,7 @@ struct dvs_scene_s {
   dvsamd::DevScene view{};
   hipStream_t stream = nullptr;
   dvsamd::DeviceArray<dvsamd::Pose> poses;
-  dvsamd::DeviceArray<float> image, depth;
+  dvsamd::DeviceArray<float> image, depth, blur;
   dvsamd::DeviceArray<double> radiance;
   dvsamd::DeviceArray<int> status;
   dvsamd::DeviceArray<unsigned long long> counter;

@@ -64,3 +64,44 @@ def drive(sensor: capi.Sensor, intervals: np.ndarray, adaptive: bool = False,
     if drain:
         out.append(sensor.drain())
     return out
+
+
+def blur_schedule(start_time: float, end_time: float, cam_fps: float, blur_resolution: float):
+    """bin/cam_blur's exposures (apps/main_cam_blur.cpp:56-72): one per
+    1/cam_fps from the first to the last frame time (float), each cut into
+    cells of blur_resolution seconds.  Returns a list of (begin, end, times,
+    weights): the cells' midpoint times as float32 (the trajectory is
+    evaluated at float times) and their weights float((right - left) /
+    (end - begin))."""
+    start, stop = float(np.float32(start_time)), float(np.float32(end_time))
+    interval = 1.0 / float(np.float32(cam_fps))
+    res = float(np.float32(blur_resolution))
+    out = []
+    for exposure in range(int(np.ceil((stop - start) / interval))):
+        begin = start + exposure * interval
+        end = min(stop, start + (exposure + 1) * interval)
+        if end <= begin:
+            break
+        cells = int(np.ceil((end - begin) / res))
+        left = begin + np.arange(cells) * res
+        right = np.minimum(end, begin + (np.arange(cells) + 1) * res)
+        times = ((left + right) * 0.5).astype(np.float32)
+        weights = ((right - left) / (end - begin)).astype(np.float32).astype(np.float64)
+        out.append((begin, end, times, weights))
+    return out
+
+
+def blur_window(center: float, blur_resolution: float, exposure: float = 0.050):
+    """bin/cam_blur_windows' exposure around `center` seconds
+    (apps/main_cam_blur_windows.cpp:55-78): [center - exposure/2, center +
+    exposure/2] in ceil(exposure / step - 1e-10) cells, empty cells dropped,
+    weights (right - left) / exposure in double.  Returns (begin, end,
+    midpoint times (float64), weights)."""
+    res = float(np.float32(blur_resolution))
+    begin, end = center - exposure * .5, center + exposure * .5
+    cells = int(np.ceil(exposure / res - 1e-10))
+    left = begin + np.arange(cells) * res
+    right = np.minimum(end, begin + (np.arange(cells) + 1) * res)
+    keep = right > left
+    left, right = left[keep], right[keep]
+    return begin, end, (left + right) * 0.5, (right - left) / exposure

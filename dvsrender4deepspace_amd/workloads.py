@@ -246,7 +246,7 @@ def write_bundle(w: Workload, directory, frames: int | None = None) -> None:
 
 
 def write_config(w: Workload, path, frame_count: int | None = None, imu_fps: float = 200.0, cam_fps: float = 30.0,
-                 physic: Physic | None = None) -> None:
+                 physic: Physic | None = None, blur_resolution: float = 1e-3) -> None:
     """The workload as a configuration file (Config's grammar,
     src/scene_types.cpp): intrinsics, every physic field, scene timing at
     the workload's fps and time resolution, and its Hapke parameters."""
@@ -259,6 +259,6 @@ def write_config(w: Workload, path, frame_count: int | None = None, imu_fps: flo
     lines += ["", "scene:", f"  frame_count: {w.frames if frame_count is None else frame_count}",
               f"  fps: {w.fps!r}", f"  imu_fps: {imu_fps!r}", f"  cam_fps: {cam_fps!r}",
               f"  time_resolution: {w.time_resolution!r}", f"  supersample_factor: {cam.samples}",
-              "  blur_resolution: 1e-3"]
+              f"  blur_resolution: {blur_resolution!r}"]
     lines += ["", "hapke:"] + [f"  {f.name}: {getattr(h, f.name)!r}" for f in dataclasses.fields(h)]
     Path(path).write_text("\n".join(lines) + "\n")

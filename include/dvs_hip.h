@@ -235,6 +235,19 @@ int dvs_frame_render_device(dvs_scene scene, const dvs_optical_state *device_pos
  * to the scene's statistics but no trace time. */
 int dvs_frame_render_device_noalloc(dvs_scene scene, const dvs_optical_state *device_poses,
                                     int32_t pose_count, float *device_image, float *device_depth);
+/* Motion-blur exposures: the render_main loop of apps/main_cam_blur.cpp:64-83
+ * and apps/main_cam_blur_windows.cpp:64-81 with every sub-exposure cell of
+ * an exposure rendered in one multi-pose launch.  Exposure e sums the cells
+ * c in [cell_offsets[e], cell_offsets[e+1]) in cell order:
+ *   wide = 0: acc = float(acc + float(frame_c * float(weights[c])))
+ *             (bin/cam_blur's cv::Mat sum, main_cam_blur.cpp:76);
+ *   wide = 1: acc = acc + weights[c] * double(frame_c) in double, cast to
+ *             float at the end (main_cam_blur_windows.cpp:77,81);
+ * where frame_c is the render_main image of poses[c].  cell_offsets holds
+ * exposure_count + 1 non-decreasing entries starting at 0; image receives
+ * exposure_count * width * height floats (host memory). */
+int dvs_frame_blur(dvs_scene scene, const dvs_optical_state *poses, const double *weights,
+                   const int32_t *cell_offsets, int32_t exposure_count, int32_t wide, float *image);
 int dvs_scene_stats(dvs_scene scene, dvs_stats *out);
 /* Traversal work of every sub-ray of one frame, for the roofline's
  * algorithmic bytes per ray (no reference counterpart): totals[0..7] =

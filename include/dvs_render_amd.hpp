@@ -624,6 +624,45 @@ class CamRayCaster {
     q[3] = w / norm;
   }
 
+  // One sub-exposure cell of a motion-blur exposure: the render_main
+  // arguments of its pose and its weight in the exposure's sum.
+  struct BlurCell {
+    float3 camera_origin{};
+    float camera_world[9]{};
+    float3 light_dir{};
+    float light_intensity = 0;
+    double weight = 0;
+  };
+  // Motion-blur exposures: the render_main loop of apps/main_cam_blur.cpp:
+  // 70-77 (and, with `wide`, apps/main_cam_blur_windows.cpp:70-81) over
+  // whole groups of exposures per launch.  Exposure e sums cells
+  // [offsets[e], offsets[e+1]) in cell order; `frames` receives
+  // (offsets.size() - 1) frames of width * height floats.
+  void render_blur(const std::vector<BlurCell> &cells, const std::vector<int32_t> &offsets, bool wide,
+                   std::vector<float> &frames) {
+    if (offsets.size() < 2 || offsets.back() != (int32_t)cells.size())
+      throw std::invalid_argument("motion-blur offsets must end at the number of cells");
+    std::vector<dvs_optical_state> poses(cells.size());
+    std::vector<double> weights(cells.size());
+    for (size_t c = 0; c < cells.size(); ++c) {
+      dvs_optical_state &p = poses[c];
+      p = dvs_optical_state{};
+      p.position[0] = cells[c].camera_origin.x;
+      p.position[1] = cells[c].camera_origin.y;
+      p.position[2] = cells[c].camera_origin.z;
+      p.light[0] = cells[c].light_dir.x;
+      p.light[1] = cells[c].light_dir.y;
+      p.light[2] = cells[c].light_dir.z;
+      p.intensity = cells[c].light_intensity;
+      quaternion_of(cells[c].camera_world, p.rotation);
+      weights[c] = cells[c].weight;
+    }
+    const int exposures = (int)offsets.size() - 1;
+    frames.resize((size_t)exposures * scene_->width() * scene_->height());
+    detail::check(dvs_frame_blur(scene_->view(), poses.data(), weights.data(), offsets.data(), exposures,
+                                 wide ? 1 : 0, frames.data()));
+  }
+
  private:
   std::shared_ptr<DeviceScene> scene_;
 };

@@ -422,6 +422,31 @@ class Readout:
         return lib().orc_readout_peak(self.handle)
 
 
+def blur_exposures(frames, weights, cell_offsets, wide=False):
+    """The motion-blur sum over sub-exposure frames (cells, H, W) float32:
+    exposure e adds cells [offsets[e], offsets[e+1]) in cell order.
+    wide=False: apps/main_cam_blur.cpp:68,76 -- float accumulator starting at
+    0, acc = float(acc + float(frame * float(weight))) (the cv::Mat
+    `accumulated += image * float(w)`: the scaled image is rounded to float,
+    then the sum); wide=True: apps/main_cam_blur_windows.cpp:69,77,81 --
+    double accumulator, acc += weight * double(frame), cast to float."""
+    frames = np.asarray(frames, np.float32)
+    offsets = np.asarray(cell_offsets, np.int64)
+    out = np.empty((len(offsets) - 1,) + frames.shape[1:], np.float32)
+    for e in range(len(offsets) - 1):
+        if wide:
+            acc = np.zeros(frames.shape[1:], np.float64)
+            for c in range(offsets[e], offsets[e + 1]):
+                acc = acc + np.float64(weights[c]) * frames[c].astype(np.float64)
+            out[e] = acc.astype(np.float32)
+        else:
+            acc = np.zeros(frames.shape[1:], np.float32)
+            for c in range(offsets[e], offsets[e + 1]):
+                acc = acc + frames[c] * np.float32(weights[c])
+            out[e] = acc
+    return out
+
+
 def render_to_bytes(linear, dither=0.0):
     a = np.ascontiguousarray(linear, np.float32)
     h, w = a.shape
