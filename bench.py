@@ -283,9 +283,10 @@ def batched_mode(args) -> dict:
 
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
-    ap.add_argument("--mode", choices=["request", "served", "batched"], default="request",
+    ap.add_argument("--mode", choices=["request", "served", "batched", "blur"], default="request",
                     help="request: the single DAVIS346 request with the served comparison (default bench line); "
-                         "served / batched: the full served and batched forms alone")
+                         "served / batched: the full served and batched forms alone; blur: the request's "
+                         "motion-blur exposures alone")
     ap.add_argument("--requests", type=int, default=60)
     ap.add_argument("--segments", type=int, default=64)
     ap.add_argument("--rate", type=float, default=20.0)
@@ -307,6 +308,8 @@ def main() -> int:
     ap.add_argument("--no-served", action="store_true", help="skip the served-vs-plain comparison")
     ap.add_argument("--served-requests", type=int, default=30)
     ap.add_argument("--served-horizons", type=int, nargs=2, default=(2000, 6000))
+    ap.add_argument("--blur-exposures", type=int, default=16, help="1/30 s exposures of the motion-blur leg")
+    ap.add_argument("--blur-cpu-cells", type=int, default=4, help="cells the CPU oracle renders for its sample")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (calibrates rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -322,6 +325,9 @@ def main() -> int:
     capi.load_library()
     if capi.device_count() < 1:
         raise RuntimeError("no HIP device visible")
+    if args.mode == "blur":
+        print(json.dumps(blur_mode(args)), flush=True)
+        return 0
     if args.mode != "request":
         print(json.dumps(served_mode(args) if args.mode == "served" else batched_mode(args)), flush=True)
         return 0
@@ -381,6 +387,80 @@ class Flight:
     def run_until(self, horizon: int):
         while self.step < min(horizon, self.w.total_steps):
             self.window()
+
+
+def blur_leg(w, scene, traj, mesh, exposures: int, cpu_cells: int) -> dict:
+    """bin/cam_blur on the request (apps/main_cam_blur.cpp:64-83): 1/30 s
+    exposures of 1 ms cells along the flight, every cell of `exposures`
+    exposures rendered by one multi-pose launch and summed on the device,
+    against the reference's form (one render_main call per cell, summed on
+    the host) on the same GPU and, for `cpu_cells` cells, the CPU oracle."""
+    from dvsrender4deepspace_amd import capi, driver
+
+    sched = driver.blur_schedule(w.start_time, w.end_time, 30.0, 1e-3)[:exposures]
+    times = np.concatenate([x[2] for x in sched])
+    weights = np.concatenate([x[3] for x in sched])
+    offsets = np.concatenate([[0], np.cumsum([len(x[2]) for x in sched])]).astype(np.int32)
+    poses = capi.optical_states(traj.evaluate(times))
+    for f in ("velocity", "light_rate", "intensity_rate"):
+        poses[f] = 0
+    scene.render_blur(poses[:offsets[1]], weights[:offsets[1]], offsets[:2])  # warm-up
+    before = scene.stats()
+    t0 = time.perf_counter()
+    batched = scene.render_blur(poses, weights, offsets)
+    wall = time.perf_counter() - t0
+    after = scene.stats()
+    launches = after["trace_launches"] - before["trace_launches"]
+    trace_ms = after["trace_ms"] - before["trace_ms"]
+    rays = (after["primary_rays"] - before["primary_rays"]) + (after["shadow_rays"] - before["shadow_rays"])
+    t0 = time.perf_counter()
+    for e in range(len(sched)):
+        acc = np.zeros((w.camera.height, w.camera.width), np.float32)
+        for c in range(offsets[e], offsets[e + 1]):
+            acc = acc + scene.render(poses[c:c + 1])[0][0] * np.float32(weights[c])
+        if e == 0 and not np.array_equal(acc, batched[0]):
+            raise RuntimeError("batched motion blur differs from the per-cell sum")
+    wall_cells = time.perf_counter() - t0
+    cells = int(offsets[-1])
+    out = {"exposures": len(sched), "cells": cells, "cells_per_exposure": int(offsets[1]),
+           "exposures_per_s": round(len(sched) / wall, 2),
+           "mrays_per_s": round(rays / wall / 1e6, 1),
+           "trace_launches": int(launches), "trace_ms_per_launch": round(trace_ms / max(launches, 1), 3),
+           "trace_mrays_per_s": round(rays / max(trace_ms * 1e-3, 1e-30) / 1e6, 1),
+           "per_cell_form": {"exposures_per_s": round(len(sched) / wall_cells, 2),
+                             "note": "one render_main call per cell and a host float sum, as the reference"}}
+    if cpu_cells > 0:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle_ctypes as oracle
+
+        v, n, nodes, root, idx = mesh
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        o = oracle.Scene(nodes, root, v, n, idx, w.camera, w.hapke)
+        t0 = time.perf_counter()
+        for c in range(cpu_cells):
+            code = o.render(poses[c:c + 1], threads)[0]
+            if code:
+                raise RuntimeError(f"oracle failed with status {code}")
+        cpu = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(cpu_cells / cpu / offsets[1], 4), "unit": "exposures/s",
+                               "cores": threads, "kind": "port", "wall_s": round(cpu, 3),
+                               "sample": f"the oracle's frames of the first {cpu_cells} cells of the first "
+                                         f"exposure ({int(offsets[1])} cells per exposure), the sum excluded"}
+    return out
+
+
+def blur_mode(args) -> dict:
+    from dvsrender4deepspace_amd import capi, terrain, workloads
+
+    w = workloads.WORKLOADS[args.workload]
+    v, n, idx = terrain.mesh(w.terrain)
+    nodes, root, leaf_idx = capi.bvh_build(v, idx, w.leaf_size)
+    scene = capi.Scene(nodes, root, v, n, leaf_idx, w.camera, w.hapke)
+    out = blur_leg(w, scene, workloads.trajectory(w), (v, n, nodes, root, leaf_idx), args.blur_exposures,
+                   0 if args.no_cpu_baseline else args.blur_cpu_cells)
+    return {"metric": "motion-blur exposures/s at DAVIS346, 1 GPU", "value": out["exposures_per_s"],
+            "unit": "exposures/s", "n_gpus": 1, "higher_is_better": True, "data": "synthetic", "mode": "blur",
+            "config": {"workload": w.name, "exposure_s": 1 / 30, "cell_s": 1e-3}, **out}
 
 
 def request_mode(args) -> dict:
@@ -543,6 +623,9 @@ def request_mode(args) -> dict:
         "trajectory_s_per_wall_s": round(nsteps * args.horizons * w.time_resolution / nwall, 6),
         "trace_cells_ms_per_launch": round(ns["trace_ms"] / max(ns["trace_launches"], 1), 3)}
     nat.close()
+    log("motion blur ...")
+    result["motion_blur"] = blur_leg(w, scene, traj, (v, n, nodes, root, leaf_idx), args.blur_exposures,
+                                     0 if args.no_cpu_baseline else args.blur_cpu_cells)
     if not args.no_cpu_baseline:
         k = min(args.cpu_windows, len(flights[0].windows))
         first_windows = flights[0].windows[:k]
