@@ -211,6 +211,8 @@ def load_library() -> ctypes.CDLL:
         "dvs_frame_render_device": (ctypes.c_int, [_P, _P, _I32, _P, _P]),
         "dvs_frame_render_device_noalloc": (ctypes.c_int, [_P, _P, _I32, _P, _P]),
         "dvs_frame_blur": (ctypes.c_int, [_P, _P, _P, _P, _I32, _I32, _P]),
+        "dvs_scan_eval": (ctypes.c_int, [_P, _I64, _P]),
+        "dvs_sort_eval": (ctypes.c_int, [_P, _P, _I64, _I32, _I32, _P, _P]),
         "dvs_scene_stats": (ctypes.c_int, [_P, ctypes.POINTER(_StatsC)]),
         "dvs_frame_traversal_stats": (ctypes.c_int, [_P, _P, _P]),
         "dvs_sensor_create": (ctypes.c_int, [_P, ctypes.POINTER(_PhysicC), _I32, _I32, ctypes.POINTER(_P)]),
@@ -919,3 +921,23 @@ def noise_eval(seed: int, pixels: int, lam: float, draws: int):
     q = np.empty((pixels, draws), np.float64)
     _check(load_library().dvs_noise_eval(int(seed), int(pixels), float(lam), int(draws), _ptr(p), _ptr(q)))
     return p, q
+
+
+def scan_eval(values) -> np.ndarray:
+    """The event compaction's device inclusive prefix sum of int32 values."""
+    a = np.ascontiguousarray(values, dtype=np.int32)
+    out = np.empty_like(a)
+    _check(load_library().dvs_scan_eval(_ptr(a), len(a), _ptr(out)))
+    return out
+
+
+def sort_eval(keys, values, begin_bit: int = 0, end_bit: int = 64):
+    """The device's stable radix sort of (uint64 key, uint32 value) pairs by
+    key bits [begin_bit, end_bit)."""
+    k = np.ascontiguousarray(keys, dtype=np.uint64)
+    v = np.ascontiguousarray(values, dtype=np.uint32)
+    if len(k) != len(v):
+        raise ValueError("one value per key")
+    ko, vo = np.empty_like(k), np.empty_like(v)
+    _check(load_library().dvs_sort_eval(_ptr(k), _ptr(v), len(k), begin_bit, end_bit, _ptr(ko), _ptr(vo)))
+    return ko, vo

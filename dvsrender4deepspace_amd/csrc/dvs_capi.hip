@@ -5,7 +5,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -18,6 +20,7 @@
 #include "device_circuit.hpp"
 #include "device_optics.hpp"
 #include "kernels.h"
+#include "scan_sort.h"
 #include "scene.h"
 
 namespace dvsamd {
@@ -489,6 +492,45 @@ int dvs_poisson_eval(const double *lambda, const double *u, int64_t n, double *o
     o.reserve(n, "poisson output");
     hip_check(launch_poisson(n, a.ptr, b.ptr, o.ptr, nullptr), "poisson kernel");
     hip_check(hipMemcpy(out, o.ptr, n * sizeof(double), hipMemcpyDeviceToHost), "poisson output");
+  });
+}
+
+int dvs_scan_eval(const int32_t *in, int64_t n, int32_t *out) {
+  return guarded([&] {
+    if (n < 0 || (n > 0 && (!in || !out)) || n > INT32_MAX) fail(DVS_E_ARGUMENT, "invalid scan arguments");
+    if (n == 0) return;
+    DeviceArray<int> a;
+    DeviceArray<unsigned char> temp;
+    a.upload(in, n, "scan input");
+    size_t bytes = 0;
+    hip_check(inclusive_scan_int(nullptr, bytes, a.ptr, a.ptr, n, nullptr), "scan size");
+    temp.reserve(bytes, "scan scratch");
+    hip_check(inclusive_scan_int(temp.ptr, bytes, a.ptr, a.ptr, n, nullptr), "scan");
+    hip_check(hipMemcpy(out, a.ptr, n * sizeof(int), hipMemcpyDeviceToHost), "scan output");
+  });
+}
+
+int dvs_sort_eval(const uint64_t *keys, const uint32_t *values, int64_t n, int32_t begin_bit, int32_t end_bit,
+                  uint64_t *keys_out, uint32_t *values_out) {
+  return guarded([&] {
+    if (n < 0 || (n > 0 && (!keys || !values || !keys_out || !values_out)) || n > INT32_MAX || begin_bit < 0 ||
+        end_bit > 64 || begin_bit > end_bit)
+      fail(DVS_E_ARGUMENT, "invalid sort arguments");
+    if (n == 0) return;
+    DeviceArray<unsigned long long> k, k2;
+    DeviceArray<unsigned> v, v2;
+    DeviceArray<unsigned char> temp;
+    k.upload(reinterpret_cast<const unsigned long long *>(keys), n, "sort keys");
+    v.upload(values, n, "sort values");
+    k2.reserve(n, "sort keys");
+    v2.reserve(n, "sort values");
+    size_t bytes = 0;
+    hip_check(sort_pairs_u64(nullptr, bytes, k.ptr, k2.ptr, v.ptr, v2.ptr, n, begin_bit, end_bit, nullptr),
+              "sort size");
+    temp.reserve(bytes, "sort scratch");
+    hip_check(sort_pairs_u64(temp.ptr, bytes, k.ptr, k2.ptr, v.ptr, v2.ptr, n, begin_bit, end_bit, nullptr), "sort");
+    hip_check(hipMemcpy(keys_out, k2.ptr, n * sizeof(uint64_t), hipMemcpyDeviceToHost), "sort output");
+    hip_check(hipMemcpy(values_out, v2.ptr, n * sizeof(uint32_t), hipMemcpyDeviceToHost), "sort output");
   });
 }
 

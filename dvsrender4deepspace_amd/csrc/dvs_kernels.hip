@@ -273,9 +273,12 @@ __global__ void cell_maps_kernel(int cells, const double *t0, const double *t1, 
 
 // Compaction: global pixel p's crossings go to [ends[p] - count, ends[p]);
 // the global pixel order is segment-major, pixel-major within a segment.
+// Compaction of every pixel's crossings to its place in the prefix sum
+// `ends`, with the order-preserving key of its time; a key outside
+// [key_lo, key_hi] (a time outside the step's span) sets *key_flag.
 __global__ void gather_multi(const SegDev *segs, const int *pixel_begin, int nseg, int total_pixels, const int *ends,
-                             double *time, int *seg, int *pixel, int *polarity, unsigned long long *keys,
-                             unsigned *order) {
+                             unsigned long long key_lo, unsigned long long key_hi, double *time, int *seg,
+                             int *pixel, int *polarity, unsigned long long *keys, unsigned *order, int *key_flag) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= total_pixels) return;
   const int si = segment_of<int>(pixel_begin, nseg, p);
@@ -291,8 +294,10 @@ __global__ void gather_multi(const SegDev *segs, const int *pixel_begin, int nse
     polarity[k] = e.polarity;
     // order-preserving key of an IEEE double
     const unsigned long long bits = (unsigned long long)__double_as_longlong(e.time);
-    keys[k] = (bits >> 63) ? ~bits : (bits | 0x8000000000000000ULL);
+    const unsigned long long key = (bits >> 63) ? ~bits : (bits | 0x8000000000000000ULL);
+    keys[k] = key;
     order[k] = (unsigned)k;
+    if (key < key_lo || key > key_hi) atomicOr(key_flag, 1);
   }
 }
 
@@ -597,11 +602,12 @@ hipError_t launch_circuit_multi(const SegDev *segs, const int *block_begin, int 
   return hipGetLastError();
 }
 hipError_t launch_gather_multi(const SegDev *segs, const int *pixel_begin, int nseg, int total_pixels,
-                               const int *ends, double *time, int *seg, int *pixel, int *polarity,
-                               unsigned long long *keys, unsigned *order, hipStream_t st) {
+                               const int *ends, unsigned long long key_lo, unsigned long long key_hi, double *time,
+                               int *seg, int *pixel, int *polarity, unsigned long long *keys, unsigned *order,
+                               int *key_flag, hipStream_t st) {
   if (total_pixels <= 0) return hipSuccess;
-  gather_multi<<<(total_pixels + 255) / 256, 256, 0, st>>>(segs, pixel_begin, nseg, total_pixels, ends, time, seg,
-                                                           pixel, polarity, keys, order);
+  gather_multi<<<(total_pixels + 255) / 256, 256, 0, st>>>(segs, pixel_begin, nseg, total_pixels, ends, key_lo,
+                                                           key_hi, time, seg, pixel, polarity, keys, order, key_flag);
   return hipGetLastError();
 }
 hipError_t launch_event_poses_multi(const SegDev *segs, int events, const double *time, const int *seg,

@@ -7,7 +7,8 @@
 // traces every (pixel, cell) of every segment, integrates every pixel's
 // circuit, compacts the crossings by a device prefix sum, traces event-time
 // depth, sorts the events by (segment, t, y, x) on the device and runs each
-// segment's readout FIFO (src/event_readout.cpp:6-30) on the sorted stream.
+// segment's readout FIFO (src/event_readout.cpp:6-30) on the sorted stream,
+// also on the device (readout.hip).
 // The plain dvs_sensor_* API is a one-segment engine whose every
 // process_many call is one step over exactly the call's intervals.
 #include <hip/hip_runtime.h>
@@ -25,8 +26,9 @@
 #include "adaptive.h"
 #include "common.h"
 #include "dvs_hip.h"
-#include "events_sort.h"
 #include "kernels.h"
+#include "readout.h"
+#include "scan_sort.h"
 #include "scene.h"
 
 namespace dvsamd {
@@ -56,6 +58,7 @@ void derive(const dvs_physic &c, bool effects, CircuitParams &p, PhotoParams &ph
 }
 
 // Bounded FIFO readout (EventReadout::submit / drain, src/event_readout.cpp:6-34)
+// on the host, for the EventReadout API on host events (dvs_readout_*),
 // over events already in (t, y, x) order.  The pending events are a ring of
 // {ready time, event} in submission order; their ready times ascend (each is
 // max(previous, t) + 1 / bandwidth + latency).
@@ -148,6 +151,27 @@ struct Readout {
   void drain(std::vector<dvs_event> &ready) { deliver(count, ready); }
 };
 
+// The order-preserving 64-bit key of an IEEE double (as the compaction
+// kernel builds it for event times).
+inline unsigned long long time_key(double t) {
+  unsigned long long bits;
+  std::memcpy(&bits, &t, sizeof(bits));
+  return (bits >> 63) ? ~bits : (bits | 0x8000000000000000ULL);
+}
+
+// A segment's readout FIFO on the device (readout.hip): the queue lives in
+// a device ring, the rest of its state here between steps.  Without a
+// bandwidth every event is delivered as it comes (event_readout.cpp:18).
+struct DeviceFifo {
+  double bandwidth = 0, latency = 0, step = 0, available = 0;
+  double last_now = -std::numeric_limits<double>::infinity();
+  long long capacity = 1, head = 0, count = 0;
+  unsigned long long dropped = 0, peak = 0;
+  DeviceArray<QueuedEvent> ring;
+  long long ring_size = 0;
+  bool queued() const { return bandwidth > 0; }
+};
+
 struct Segment {
   int id = 0;
   dvs_scene scene = nullptr;
@@ -161,7 +185,7 @@ struct Segment {
   DeviceArray<dvs_pixel_state> states;
   DeviceArray<CrossingEvent> slots;
   std::deque<dvs_optical_interval> queue;
-  Readout readout;
+  DeviceFifo fifo;
   std::vector<dvs_event> ready, raw;
   int64_t horizons_done = 0, events_total = 0;
   // adaptive optics: the host driver policy of apps/main_dvs.cpp:96-116 per
@@ -233,9 +257,10 @@ class Engine {
     derive(c, effects, s->params, s->photo);
     s->threshold_std = effects ? c.threshold_std : 0.0;
     s->threshold_min = c.threshold_min;
-    s->readout.bandwidth = aer ? (double)c.readout_bandwidth : 0.0;
-    s->readout.latency = aer ? (double)c.readout_latency : 0.0;
-    s->readout.capacity = (size_t)std::max(1, c.readout_queue_capacity);
+    s->fifo.bandwidth = aer ? (double)c.readout_bandwidth : 0.0;
+    s->fifo.latency = aer ? (double)c.readout_latency : 0.0;
+    s->fifo.step = s->fifo.bandwidth > 0 ? 1.0 / s->fifo.bandwidth : 0.0;
+    s->fifo.capacity = std::max(1, c.readout_queue_capacity);
     s->states.reserve(s->npix, "pixel states");
     s->slots.reserve((size_t)s->npix * s->max_events, "events");
     scene_retain(scene);
@@ -743,6 +768,103 @@ class Engine {
     }
   }
 
+  // Grows a FIFO's ring to hold `need` entries, moving the queue to the
+  // front of the new ring.
+  void ensure_ring(DeviceFifo &f, long long need) {
+    if (f.ring_size >= need) return;
+    long long size = std::max<long long>(1024, f.ring_size);
+    while (size < need) size *= 2;
+    DeviceArray<QueuedEvent> bigger;
+    bigger.reserve((size_t)size, "readout queue");
+    if (f.count > 0) {
+      const long long slot = f.head & (f.ring_size - 1);
+      const long long first = std::min(f.count, f.ring_size - slot);
+      hip_check(hipMemcpyAsync(bigger.ptr, f.ring.ptr + slot, sizeof(QueuedEvent) * first, hipMemcpyDeviceToDevice,
+                               stream_),
+                "readout queue");
+      if (first < f.count)
+        hip_check(hipMemcpyAsync(bigger.ptr + first, f.ring.ptr, sizeof(QueuedEvent) * (f.count - first),
+                                 hipMemcpyDeviceToDevice, stream_),
+                  "readout queue");
+      hip_check(hipStreamSynchronize(stream_), "readout queue");
+    }
+    std::swap(f.ring.ptr, bigger.ptr);
+    std::swap(f.ring.count, bigger.count);
+    f.ring_size = size;
+    f.head = 0;
+  }
+
+  // The readout tasks of a step: every accepted segment with a bandwidth.
+  // Its delivered events go to host_ready_ at out_offset: the queue holds
+  // only events whose ready time is after the previous call's `now` (every
+  // earlier one was delivered then), and ready times are at least 1 /
+  // bandwidth apart, so at most (now - last now) x bandwidth + 1 of them
+  // are delivered (the room also never exceeds the queue plus the step's
+  // events).
+  int build_fifo_tasks(std::vector<StepSegment> &work, const std::vector<char> &accepted, int total) {
+    host_fifo_tasks_.reserve(work.size(), "readout tasks");
+    int ntask = 0;
+    long long room_total = 0;
+    for (size_t i = 0; i < work.size(); ++i) {
+      Segment &s = *work[i].seg;
+      if (!accepted[i] || !s.fifo.queued()) continue;
+      DeviceFifo &f = s.fifo;
+      const double now = work[i].ivs.back().t1;
+      const long long need = f.count + (long long)total;
+      ensure_ring(f, need);
+      long long room = need;
+      if (std::isfinite(f.last_now)) {
+        const double span = (now - f.last_now) * f.bandwidth;
+        if (span >= 0 && span < 1e15) room = std::min<long long>(room, (long long)(span * (1 + 1e-9)) + 64);
+      }
+      FifoTask &t = host_fifo_tasks_.ptr[ntask++];
+      t.ring = f.ring.ptr;
+      t.mask = f.ring_size - 1;
+      t.head = f.head;
+      t.count = f.count;
+      t.available = f.available;
+      t.step = f.step;
+      t.latency = f.latency;
+      t.now = now;
+      t.capacity = f.capacity;
+      t.dropped = f.dropped;
+      t.peak = f.peak;
+      t.pixel_begin = work[i].pixel_offset;
+      t.pixel_end = work[i].pixel_offset + s.npix;
+      t.out_offset = room_total;
+      t.out_room = room;
+      room_total += room;
+    }
+    if (ntask) {
+      fifo_tasks_.reserve_ahead(ntask, "readout tasks");
+      host_fifo_results_.reserve(ntask, "readout results");
+      host_ready_.reserve((size_t)std::max<long long>(room_total, 1), "delivered events");
+    }
+    return ntask;
+  }
+
+ public:
+  // EventReadout::drain (event_readout.cpp:31-34): every queued event.
+  void drain(Segment &s) {
+    DeviceFifo &f = s.fifo;
+    if (!f.queued() || f.count == 0) return;
+    std::vector<QueuedEvent> queued((size_t)f.count);
+    const long long slot = f.head & (f.ring_size - 1), first = std::min(f.count, f.ring_size - slot);
+    hip_check(hipMemcpyAsync(queued.data(), f.ring.ptr + slot, sizeof(QueuedEvent) * first, hipMemcpyDeviceToHost,
+                             stream_),
+              "drain readout");
+    if (first < f.count)
+      hip_check(hipMemcpyAsync(queued.data() + first, f.ring.ptr, sizeof(QueuedEvent) * (f.count - first),
+                               hipMemcpyDeviceToHost, stream_),
+                "drain readout");
+    hip_check(hipStreamSynchronize(stream_), "drain readout");
+    s.ready.reserve(s.ready.size() + queued.size());
+    for (const QueuedEvent &q : queued) s.ready.push_back(unqueue(q));
+    f.head += f.count;
+    f.count = 0;
+  }
+
+ private:
   // Compaction by a device prefix sum over every segment's pixel counts,
   // event-time depth (native steps), device sort by (segment, t, y, x) and
   // each accepted segment's readout FIFO.  Returns the events time (ms).
@@ -758,7 +880,7 @@ class Engine {
     hip_check(inclusive_scan_int(temp_.ptr, temp_bytes, counts_.ptr, ends_.ptr, pixel_total, st), "event scan");
     int status = 0, total = 0;
     unsigned long long shadows = 0;
-    host_words_.reserve(4, "status");
+    host_words_.reserve(6, "status");
     host_words_.ptr[0] = host_words_.ptr[1] = -1;  // the status and the count are never negative
     hip_check(hipMemcpyAsync(host_words_.ptr, status_.ptr, sizeof(int), hipMemcpyDeviceToHost, st), "copy status");
     hip_check(hipMemcpyAsync(host_words_.ptr + 1, ends_.ptr + pixel_total - 1, sizeof(int), hipMemcpyDeviceToHost, st),
@@ -777,6 +899,51 @@ class Engine {
     host_events_.reserve(event_room, "events");
     if (nseg > 1) host_event_seg_.reserve(event_room, "event segments");
     const int samples = work[0].seg->scene->view.samples;
+    // Every event time of the step lies in the windows' span [tmin, tmax], so
+    // the order-preserving time keys share the bits above the highest bit in
+    // which key(tmin) and key(tmax) differ: only the bits below it are sorted.
+    // The compaction flags a key outside the span (never expected), and then
+    // the sort is repeated on all 64 bits before the readout runs.
+    double tmin = std::numeric_limits<double>::infinity(), tmax = -tmin;
+    for (auto &w : work) {
+      tmin = std::min(tmin, w.ivs.front().t0);
+      tmax = std::max(tmax, w.ivs.back().t1);
+    }
+    const unsigned long long kmin = time_key(tmin), kmax = time_key(tmax);
+    const int time_bits = kmin == kmax ? 1 : 64 - __builtin_clzll(kmin ^ kmax);
+    int seg_bits = 1;
+    while ((1 << seg_bits) < nseg) ++seg_bits;
+    keyflag_.reserve(1, "event key flag");
+    // Sort by time (stably, so ties keep the compaction's (segment, y, x)
+    // order), then stably by segment: (segment, t, y, x); permute the
+    // records and copy them (and their segments) to the host.
+    auto sort_events = [&](int end_bit) {
+      size_t bytes = 0;
+      hip_check(sort_pairs_u64(nullptr, bytes, keys_.ptr, keys2_.ptr, order_.ptr, order2_.ptr, event_room, 0, 64, st),
+                "sort size");
+      temp_.reserve_ahead(bytes, "sort scratch");
+      hip_check(sort_pairs_u64(temp_.ptr, bytes, keys_.ptr, keys2_.ptr, order_.ptr, order2_.ptr, total, 0, end_bit,
+                               st),
+                "event sort");
+      const unsigned *final_order = order2_.ptr;
+      if (nseg > 1) {
+        hip_check(launch_segment_keys(total, order2_.ptr, ev_seg_.ptr, keys_.ptr, st), "segment keys");
+        hip_check(sort_pairs_u64(temp_.ptr, bytes, keys_.ptr, keys2_.ptr, order2_.ptr, order_.ptr, total, 0, seg_bits,
+                                 st),
+                  "event sort");
+        final_order = order_.ptr;
+      }
+      hip_check(launch_permute_multi(segtab_.ptr, total, final_order, ev_time_.ptr, ev_seg_.ptr, ev_pixel_.ptr,
+                                     ev_pol_.ptr, ev_depth_.ptr, events_out_.ptr, events_seg_.ptr, st),
+                "event records");
+      hip_check(hipMemcpyAsync(host_events_.ptr, events_out_.ptr, sizeof(dvs_event) * total, hipMemcpyDeviceToHost,
+                               st),
+                "copy events");
+      if (nseg > 1)
+        hip_check(hipMemcpyAsync(host_event_seg_.ptr, events_seg_.ptr, sizeof(int) * total, hipMemcpyDeviceToHost,
+                                 st),
+                  "copy events");
+    };
     if (total > 0) {
       ev_time_.reserve_ahead(event_room, "event times");
       ev_seg_.reserve_ahead(event_room, "event segments");
@@ -791,8 +958,9 @@ class Engine {
       events_out_.reserve_ahead(event_room, "events");
       events_seg_.reserve_ahead(event_room, "event segments");
       const SegDev *tab = segtab_.ptr;  // chunk 0 table: ivs, scene, counts, slots are chunk independent
-      hip_check(launch_gather_multi(tab, pbegin_.ptr, nseg, pixel_total, ends_.ptr, ev_time_.ptr, ev_seg_.ptr,
-                                    ev_pixel_.ptr, ev_pol_.ptr, keys_.ptr, order_.ptr, st),
+      hip_check(launch_fill_int(keyflag_.ptr, 0, 1, st), "clear event key flag");
+      hip_check(launch_gather_multi(tab, pbegin_.ptr, nseg, pixel_total, ends_.ptr, kmin, kmax, ev_time_.ptr,
+                                    ev_seg_.ptr, ev_pixel_.ptr, ev_pol_.ptr, keys_.ptr, order_.ptr, keyflag_.ptr, st),
                 "event compaction");
       if (depth) {
         hip_check(launch_event_poses_multi(tab, total, ev_time_.ptr, ev_seg_.ptr, ev_poses_.ptr, st), "event poses");
@@ -803,66 +971,52 @@ class Engine {
         // The interpolating path supplies no per-event depth (event_sensor.cu:111-113).
         hip_check(launch_fill_int(reinterpret_cast<int *>(ev_depth_.ptr), (int)0xBF800000u, total, st), "event depth");
       }
-      // Stable sort by time, then stable by segment: (segment, t, y, x),
-      // since compaction order is (segment, y, x).  The scratch is sized for
-      // event_room items, so that it does not grow with every step.
-      const int sort_room = (int)std::min<size_t>(event_room, (size_t)INT_MAX);
-      temp_bytes = 0;
-      hip_check(sort_pairs_u64(nullptr, temp_bytes, keys_.ptr, keys2_.ptr, order_.ptr, order2_.ptr, sort_room, 0, 64,
-                               st),
-                "sort size");
-      size_t need = temp_bytes;
-      int seg_bits = 1;
-      while ((1 << seg_bits) < nseg) ++seg_bits;
-      hip_check(sort_pairs_u64(nullptr, temp_bytes, keys_.ptr, keys2_.ptr, order_.ptr, order2_.ptr, sort_room, 0,
-                               seg_bits, st),
-                "sort size");
-      need = std::max(need, temp_bytes);
-      temp_.reserve_ahead(need, "sort scratch");
-      temp_bytes = need;
-      hip_check(sort_pairs_u64(temp_.ptr, temp_bytes, keys_.ptr, keys2_.ptr, order_.ptr, order2_.ptr, total, 0, 64, st),
-                "event sort");
-      const unsigned *by_time = order2_.ptr;
-      const unsigned *final_order = by_time;
-      if (nseg > 1) {
-        hip_check(launch_segment_keys(total, by_time, ev_seg_.ptr, keys_.ptr, st), "segment keys");
-        temp_bytes = need;
-        hip_check(sort_pairs_u64(temp_.ptr, temp_bytes, keys_.ptr, keys2_.ptr, by_time, order_.ptr, total, 0, seg_bits,
-                                 st),
-                  "event sort");
-        final_order = order_.ptr;
-      }
-      hip_check(launch_permute_multi(tab, total, final_order, ev_time_.ptr, ev_seg_.ptr, ev_pixel_.ptr, ev_pol_.ptr,
-                                     ev_depth_.ptr, events_out_.ptr, events_seg_.ptr, st),
-                "event records");
-      hip_check(hipMemcpyAsync(host_events_.ptr, events_out_.ptr, sizeof(dvs_event) * total, hipMemcpyDeviceToHost,
-                               st),
-                "copy events");
-      if (nseg > 1)
-        hip_check(hipMemcpyAsync(host_event_seg_.ptr, events_seg_.ptr, sizeof(int) * total, hipMemcpyDeviceToHost,
-                                 st),
-                  "copy events");
+      sort_events(time_bits);
     }
+    // The readout FIFO of every accepted segment with a bandwidth, on the
+    // device, over its slice of the sorted events.
+    const int ntask = build_fifo_tasks(work, accepted, total);
+    auto run_fifo = [&](const int *flag) {
+      if (ntask == 0) return;
+      hip_check(hipMemcpyAsync(fifo_tasks_.ptr, host_fifo_tasks_.ptr, sizeof(FifoTask) * ntask, hipMemcpyHostToDevice,
+                               st),
+                "readout tasks");
+      hip_check(launch_readout_fifo(fifo_tasks_.ptr, ntask, ends_.ptr, events_out_.ptr, flag, host_ready_.ptr,
+                                    host_fifo_results_.ptr, st),
+                "readout FIFO");
+    };
+    run_fifo(total > 0 ? keyflag_.ptr : nullptr);
     hip_check(hipMemcpyAsync(host_words_.ptr + 2, counter_.ptr, sizeof(shadows), hipMemcpyDeviceToHost, st),
               "ray counter");
+    host_words_.ptr[4] = 0;
+    if (total > 0)
+      hip_check(hipMemcpyAsync(host_words_.ptr + 4, keyflag_.ptr, sizeof(int), hipMemcpyDeviceToHost, st),
+                "event key flag");
     hip_check(hipEventRecord(events_t.stop, st), "record");
     host_words_.ptr[0] = -1;
     hip_check(hipMemcpyAsync(host_words_.ptr, status_.ptr, sizeof(int), hipMemcpyDeviceToHost, st), "copy status");
     wait_words(host_words_.ptr, 1, -1, st, "copy events");
-    // The event records, their segments and the ray counter arrive by other
-    // copies than the polled status word; only the stream orders them, so
-    // their completion is taken from the event recorded after them.
+    // The event records, their segments, the ray counter and the readout's
+    // results arrive by other copies (or kernel writes) than the polled
+    // status word; only the stream orders them, so their completion is taken
+    // from the event recorded after them.
     hip_check(hipEventSynchronize(events_t.stop), "copy events");
     status = host_words_.ptr[0];
     std::memcpy(&shadows, host_words_.ptr + 2, sizeof(shadows));
     if (status) fail(DVS_E_OPTICAL, "invalid optical sample or BVH traversal overflow");
+    if (host_words_.ptr[4]) {  // a time outside the span: sort on all bits, then the readout
+      sort_events(64);
+      run_fifo(nullptr);
+      stream_wait(st, "copy events");
+    }
     const dvs_event *sorted = host_events_.ptr;
     const int *sorted_seg = host_event_seg_.ptr;
     stats.shadow_rays += shadows;
     stats.comparator_events += (uint64_t)total;
     if (depth) stats.event_rays += (uint64_t)total * samples * samples;
-    // Per-segment readout on its (t, y, x)-ordered slice.
+    // Per-segment raw events and readout results.
     size_t at = 0;
+    int task = 0;
     for (int i = 0; i < nseg; ++i) {
       Segment &s = *work[i].seg;
       size_t end = at;
@@ -872,7 +1026,21 @@ class Engine {
       s.raw.assign(sorted + at, sorted + end);
       s.events_total += (int64_t)(end - at);
       const size_t before = s.ready.size();
-      s.readout.submit_sorted(s.raw.data(), s.raw.size(), work[i].ivs.back().t1, s.ready);
+      if (s.fifo.queued()) {
+        const FifoTask &t = host_fifo_tasks_.ptr[task];
+        const FifoResult &r = host_fifo_results_.ptr[task];
+        ++task;
+        if (r.overflow) fail(DVS_E_STATE, "readout delivered more events than its output room");
+        s.ready.insert(s.ready.end(), host_ready_.ptr + t.out_offset, host_ready_.ptr + t.out_offset + r.delivered);
+        s.fifo.head = r.head;
+        s.fifo.count = r.count;
+        s.fifo.available = r.available;
+        s.fifo.dropped = r.dropped;
+        s.fifo.peak = r.peak;
+        s.fifo.last_now = t.now;
+      } else {
+        s.ready.insert(s.ready.end(), s.raw.begin(), s.raw.end());  // no queue (event_readout.cpp:18)
+      }
       stats.delivered_events += s.ready.size() - before;
       stats.pixel_cells += (uint64_t)s.npix * work[i].cells;
       s.cells_done += (uint64_t)work[i].cells;
@@ -1398,6 +1566,13 @@ class Engine {
   DeviceArray<Pose> end_pose_[2];
   DeviceArray<int> probe_reuse_;
   PinnedArray<int> host_reuse_;
+  // event key flag and the readout FIFO's tasks, results and deliveries
+  // (the last two are written by the kernel into page-locked memory)
+  DeviceArray<int> keyflag_;
+  DeviceArray<FifoTask> fifo_tasks_;
+  PinnedArray<FifoTask> host_fifo_tasks_;
+  PinnedArray<FifoResult> host_fifo_results_;
+  PinnedArray<dvs_event> host_ready_;
   std::vector<long long> probe_layout_;
   int probe_cur_ = 0;
   DeviceArray<int> rare_native_, rare_adapt_;  // per-wave flags of the photocurrent's second phase
@@ -1424,7 +1599,7 @@ class Engine {
  public:
   size_t segment_bytes(int id) {
     Segment &s = segment(id);
-    return s.states.bytes() + s.slots.bytes() + (size_t)s.npix * sizeof(int) * 2;
+    return s.states.bytes() + s.slots.bytes() + s.fifo.ring.bytes() + (size_t)s.npix * sizeof(int) * 2;
   }
   std::map<int, std::unique_ptr<Segment>> &segments() { return segments_; }
 };
@@ -1574,7 +1749,7 @@ int dvs_sensor_drain(dvs_sensor s, int64_t *ready_count) {
   return guarded([&] {
     if (!s) fail(DVS_E_ARGUMENT, "null sensor");
     Segment &seg = s->engine->segment(s->id);
-    seg.readout.drain(seg.ready);
+    s->engine->drain(seg);
     if (ready_count) *ready_count = (int64_t)seg.ready.size();
   });
 }
@@ -1583,8 +1758,8 @@ int dvs_sensor_readout_stats(dvs_sensor s, int64_t *dropped, int64_t *peak) {
   return guarded([&] {
     if (!s) fail(DVS_E_ARGUMENT, "null sensor");
     Segment &seg = s->engine->segment(s->id);
-    if (dropped) *dropped = (int64_t)seg.readout.dropped;
-    if (peak) *peak = (int64_t)seg.readout.peak;
+    if (dropped) *dropped = (int64_t)seg.fifo.dropped;
+    if (peak) *peak = (int64_t)seg.fifo.peak;
   });
 }
 
@@ -1735,7 +1910,7 @@ int dvs_batch_drain(dvs_batch b, int32_t segment, int64_t *ready_count) {
   return guarded([&] {
     if (!b) fail(DVS_E_ARGUMENT, "null batch");
     Segment &seg = b->engine->segment(segment);
-    seg.readout.drain(seg.ready);
+    b->engine->drain(seg);
     if (ready_count) *ready_count = (int64_t)seg.ready.size();
   });
 }

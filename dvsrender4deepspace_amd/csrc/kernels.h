@@ -71,8 +71,9 @@ hipError_t launch_cell_rates_multi(const SegDev *segs, const int *item_begin, in
 hipError_t launch_circuit_multi(const SegDev *segs, const int *block_begin, int nseg, int blocks, int *status,
                                 hipStream_t st);
 hipError_t launch_gather_multi(const SegDev *segs, const int *pixel_begin, int nseg, int total_pixels,
-                               const int *ends, double *time, int *seg, int *pixel, int *polarity,
-                               unsigned long long *keys, unsigned *order, hipStream_t st);
+                               const int *ends, unsigned long long key_lo, unsigned long long key_hi, double *time,
+                               int *seg, int *pixel, int *polarity, unsigned long long *keys, unsigned *order,
+                               int *key_flag, hipStream_t st);
 hipError_t launch_event_poses_multi(const SegDev *segs, int events, const double *time, const int *seg,
                                     Pose *poses, hipStream_t st);
 hipError_t launch_event_depth_multi(const SegDev *segs, int events, int samples, size_t lds, const int *seg,

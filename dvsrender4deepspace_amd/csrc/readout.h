@@ -1,0 +1,64 @@
+// The readout FIFO on the device (readout.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "dvs_hip.h"
+
+namespace dvsamd {
+
+// A queued event: the packed dvs_event record (17 bytes) widened to an
+// aligned 24 bytes for the ring.
+struct QueuedEvent {
+  double timestamp;
+  float depth;
+  int16_t x, y;
+  int8_t polarity;
+  int8_t pad[7];
+};
+static_assert(sizeof(QueuedEvent) == 24, "queued event layout");
+
+__host__ __device__ inline dvs_event unqueue(const QueuedEvent &q) {
+  dvs_event e;
+  e.timestamp = q.timestamp;
+  e.x = q.x;
+  e.y = q.y;
+  e.polarity = q.polarity;
+  e.depth = q.depth;
+  return e;
+}
+
+// One segment's FIFO for one step.  The queue holds `count` events in ring
+// positions [head, head + count) (absolute indices, slot = index & mask);
+// the ring has room for every event queued in the step without reusing a
+// slot, so delivered events stay readable until the step ends.
+struct FifoTask {
+  QueuedEvent *ring;
+  long long mask;
+  long long head, count;
+  double available, step, latency, now;
+  long long capacity;
+  unsigned long long dropped, peak;
+  int pixel_begin, pixel_end;  // the segment's pixels in the step's compaction order
+  long long out_offset, out_room;
+};
+
+struct FifoResult {
+  long long head, count, delivered;
+  double available;
+  unsigned long long dropped, peak;
+  int overflow;  // more deliveries than out_room (the host bound was wrong)
+  int pad;
+};
+
+// One work-group per task: the serial recurrence of EventReadout::submit
+// (src/event_readout.cpp:13-29) over the task's slice of `sorted` (sorted
+// by (segment, t, y, x); the slice is [ends[pixel_begin - 1],
+// ends[pixel_end - 1])), then the delivered events copied to
+// out[out_offset ...] and the queue state to results[task].  Nothing runs
+// while *skip (if given) is set.
+hipError_t launch_readout_fifo(const FifoTask *tasks, int task_count, const int *ends, const dvs_event *sorted,
+                               const int *skip, dvs_event *out, FifoResult *results, hipStream_t st);
+
+}  // namespace dvsamd

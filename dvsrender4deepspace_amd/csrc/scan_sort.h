@@ -1,12 +1,15 @@
-// Device scan / stable radix sort wrappers (compiled in events_sort.hip).
+// Hand-written device prefix sum and stable radix sort (scan_sort.hip).
 #pragma once
 #include <cstddef>
 #include <hip/hip_runtime.h>
 
 namespace dvsamd {
+// Inclusive prefix sum of n ints (in == out allowed).  With temp == nullptr
+// only sets temp_bytes, the scratch the call needs.
 hipError_t inclusive_scan_int(void *temp, size_t &temp_bytes, const int *in, int *out, size_t n,
                               hipStream_t stream);
-// Stable LSD radix sort of (key, value) pairs over bits [begin_bit, end_bit).
+// Stable LSD radix sort of (key, value) pairs by key bits [begin_bit,
+// end_bit); the inputs are left unchanged.  Same temp_bytes protocol.
 hipError_t sort_pairs_u64(void *temp, size_t &temp_bytes, const unsigned long long *keys_in,
                           unsigned long long *keys_out, const unsigned *values_in, unsigned *values_out,
                           size_t n, int begin_bit, int end_bit, hipStream_t stream);

@@ -467,6 +467,14 @@ int dvs_poisson_eval(const double *lambda, const double *u, int64_t n, double *o
 /* Poisson / normal draws of the device noise streams for (seed, pixel). */
 int dvs_noise_eval(uint64_t seed, int64_t pixels, double lambda, int32_t draws, double *poisson,
                    double *normal);
+/* The event compaction's device prefix sum and stable radix sort (no
+ * reference counterpart; they replace the host's std::stable_sort of
+ * EventReadout::submit, src/event_readout.cpp:8-12): the inclusive prefix
+ * sum of n ints, and (key, value) pairs sorted stably by key bits
+ * [begin_bit, end_bit). */
+int dvs_scan_eval(const int32_t *in, int64_t n, int32_t *out);
+int dvs_sort_eval(const uint64_t *keys, const uint32_t *values, int64_t n, int32_t begin_bit, int32_t end_bit,
+                  uint64_t *keys_out, uint32_t *values_out);
 /* Nearest / shadow queries of n rays against a scene (traversal parity). */
 int dvs_trace_rays(dvs_scene scene, const float *origins, const float *directions, int64_t n,
                    int32_t *hit, int32_t *triangle, float *t, float *u, float *v);

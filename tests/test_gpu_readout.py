@@ -1,0 +1,90 @@
+"""GPU: the readout on the device -- the event compaction's hand-written
+prefix sum and stable radix sort (scan_sort.hip) against numpy, and the
+device FIFO (readout.hip, src/event_readout.cpp:6-34) against the oracle's
+FIFO with a backlog of thousands of events: many refills of the staged
+chunks, ring growth, drops once the queue is full, then the final drain.
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+import pytest
+
+import scenes
+from dvsrender4deepspace_amd import capi, driver, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n", [1, 7, 2047, 2048, 2049, 100_003, 5_000_000])
+def test_scan_matches_cumsum(gpu, n):
+    rng = np.random.default_rng(n)
+    a = rng.integers(0, 4, n).astype(np.int32)
+    a[rng.random(n) < 0.7] = 0  # mostly empty pixels, as the compaction sees them
+    assert np.array_equal(gpu.scan_eval(a), np.cumsum(a, dtype=np.int64).astype(np.int32))
+
+
+@pytest.mark.parametrize("n", [1, 255, 256, 2049, 100_000, 1_000_003])
+@pytest.mark.parametrize("bits", [(0, 64), (0, 43), (8, 20), (0, 1)])
+def test_sort_is_stable_radix_sort(gpu, n, bits):
+    rng = np.random.default_rng(n + 1000 * bits[1])
+    keys = rng.integers(0, 2**63, n, dtype=np.uint64) | (rng.integers(0, 2, n, dtype=np.uint64) << np.uint64(63))
+    keys[rng.random(n) < 0.3] = keys[0]  # many equal keys: order among them must be kept
+    values = np.arange(n, dtype=np.uint32)
+    begin, end = bits
+    mask = np.uint64((1 << (end - begin)) - 1) if end - begin < 64 else np.uint64(2**64 - 1)
+    digit = (keys >> np.uint64(begin)) & mask
+    order = np.argsort(digit, kind="stable")
+    ko, vo = gpu.sort_eval(keys, values, begin, end)
+    assert np.array_equal(vo, values[order])
+    assert np.array_equal(ko, keys[order])
+
+
+def test_sort_of_event_time_keys(gpu):
+    """Order-preserving keys of positive doubles within one window sort by
+    the bits below the highest bit in which the window's end keys differ."""
+    rng = np.random.default_rng(5)
+    t0, t1 = 1.25, 1.252
+    times = np.sort(rng.uniform(t0, t1, 50_000))
+    times = rng.permutation(np.concatenate([times, times[:5000]]))  # ties
+    keys = times.view(np.uint64) | np.uint64(1 << 63)
+    lo, hi = np.array([t0]).view(np.uint64)[0], np.array([t1]).view(np.uint64)[0]
+    bits = int(lo ^ hi).bit_length()
+    assert bits < 45
+    ko, vo = gpu.sort_eval(keys, np.arange(len(keys), dtype=np.uint32), 0, bits)
+    assert np.array_equal(vo, np.argsort(times, kind="stable"))
+
+
+def test_device_fifo_backlog_matches_oracle(oracle, gpu):
+    """The small plain request flown fast with low thresholds (thousands of
+    events per call) into a 2e5 events/s readout holding at most 6,000: the
+    backlog grows past the staged chunks and the first ring, the queue fills
+    and drops; every call's deliveries, the drop count, the peak and the
+    final drain equal the oracle's FIFO fed the sensor's comparator events."""
+    w = workloads.small_variant(workloads.PLAIN_SMALL, size=128, width=64, height=48)
+    w = dataclasses.replace(w, speed=4000.0, yaw_rate_deg=300.0, frames=2, fps=100.0)  # a 40 m pass in 10 ms
+    bw, lat, cap = 2e5, 1e-6, 6000
+    phys = dataclasses.replace(w.physic, threshold_pos=0.05, threshold_neg=0.05, readout_bandwidth=bw,
+                               readout_latency=lat, readout_queue_capacity=cap)
+    v, n, nodes, root, idx = scenes.workload_mesh(w)
+    g = gpu.Scene(nodes, root, v, n, idx, w.camera, w.hapke)
+    s = gpu.Sensor(g, phys)
+    s.initialize(w.initial_state(), w.start_time)
+    bus = oracle.Readout(float(np.float32(bw)), float(np.float32(lat)), cap)
+    batch = driver.batch_size(phys, w.time_resolution, adaptive=False)
+    submitted, largest = 0, 0
+    for k in range(0, w.total_steps, batch):
+        got = s.process(w.intervals(k, batch))
+        raw = s.raw_events()
+        ev = np.zeros(len(raw), oracle.EVENT)
+        for f in ("timestamp", "x", "y", "polarity", "depth"):
+            ev[f] = raw[f]
+        submitted += len(ev)
+        want = bus.submit(ev, float(w.intervals(k, batch)["t1"][-1]))
+        largest = max(largest, len(got))
+        assert got.tobytes() == want.tobytes(), f"call at horizon {k}"
+    dropped, peak = s.readout_stats()
+    print(f"fifo backlog: {submitted} submitted, {dropped} dropped, peak {peak}, largest delivery {largest}", flush=True)
+    assert submitted > cap and dropped == bus.dropped > 0 and peak == bus.peak == cap
+    assert s.drain().tobytes() == bus.drain().tobytes()
