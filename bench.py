@@ -601,6 +601,11 @@ def request_mode(args) -> dict:
         "roofline_other_kernels": secondary,
         "totals": {k: st[k] for k in ("probe_samples", "query_samples", "pixel_cells", "probe_launches",
                                       "query_launches", "circuit_launches", "primary_rays", "shadow_rays")},
+        "readout": {"form": "device FIFO: min-plus / max-plus scans, serial recurrence where they do not apply",
+                    "windows": int(st["readout_tasks"]), "serial_windows": int(st["readout_serial_tasks"]),
+                    "dropped": int(sensor.readout_stats()[0]), "peak_queue": int(sensor.readout_stats()[1]),
+                    "bandwidth_events_per_s": w.physic.readout_bandwidth,
+                    "queue_capacity": w.physic.readout_queue_capacity},
     }
     sensor.close()
 

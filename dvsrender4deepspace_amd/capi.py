@@ -149,7 +149,8 @@ class _StatsC(ctypes.Structure):
                 ("circuit_launches", ctypes.c_double), ("probe_ms", ctypes.c_double),
                 ("query_ms", ctypes.c_double), ("probe_launches", ctypes.c_double),
                 ("query_launches", ctypes.c_double), ("probe_samples", ctypes.c_uint64),
-                ("query_samples", ctypes.c_uint64)]
+                ("query_samples", ctypes.c_uint64), ("readout_tasks", ctypes.c_uint64),
+                ("readout_serial_tasks", ctypes.c_uint64)]
 
 
 class _RefineC(ctypes.Structure):

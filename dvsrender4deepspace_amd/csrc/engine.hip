@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <limits>
@@ -233,6 +234,10 @@ class Engine {
  public:
   explicit Engine(int horizons_per_step, bool use_graph) : horizons_(horizons_per_step), use_graph_(use_graph) {
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "engine stream");
+    // DVS_READOUT_SERIAL=1 runs every readout through the serial recurrence
+    // (the scan form's fallback), for comparing the two.
+    const char *serial = std::getenv("DVS_READOUT_SERIAL");
+    force_serial_readout_ = serial && serial[0] == '1';
   }
   ~Engine() {
     segments_.clear();
@@ -795,7 +800,7 @@ class Engine {
   }
 
   // The readout tasks of a step: every accepted segment with a bandwidth.
-  // Its delivered events go to host_ready_ at out_offset: the queue holds
+  // Its delivered events go to ready_out_ at out_offset: the queue holds
   // only events whose ready time is after the previous call's `now` (every
   // earlier one was delivered then), and ready times are at least 1 /
   // bandwidth apart, so at most (now - last now) x bandwidth + 1 of them
@@ -833,12 +838,21 @@ class Engine {
       t.pixel_end = work[i].pixel_offset + s.npix;
       t.out_offset = room_total;
       t.out_room = room;
+      t.force_serial = force_serial_readout_ ? 1 : 0;
+      t.pad = 0;
       room_total += room;
     }
     if (ntask) {
       fifo_tasks_.reserve_ahead(ntask, "readout tasks");
       host_fifo_results_.reserve(ntask, "readout results");
-      host_ready_.reserve((size_t)std::max<long long>(room_total, 1), "delivered events");
+      fifo_results_.reserve_ahead(ntask, "readout results");
+      const size_t room = (size_t)std::max<long long>(room_total, 1);
+      host_ready_.reserve(room, "delivered events");
+      ready_out_.reserve_ahead(room, "delivered events");
+      fifo_room_ = room;
+      fifo_accepted_.reserve_ahead((size_t)std::max(total, 1), "readout scratch");
+      fifo_done_.reserve_ahead((size_t)std::max(total, 1), "readout scratch");
+      fifo_serial_.reserve_ahead(ntask, "readout scratch");
     }
     return ntask;
   }
@@ -981,9 +995,17 @@ class Engine {
       hip_check(hipMemcpyAsync(fifo_tasks_.ptr, host_fifo_tasks_.ptr, sizeof(FifoTask) * ntask, hipMemcpyHostToDevice,
                                st),
                 "readout tasks");
-      hip_check(launch_readout_fifo(fifo_tasks_.ptr, ntask, ends_.ptr, events_out_.ptr, flag, host_ready_.ptr,
-                                    host_fifo_results_.ptr, st),
+      hip_check(launch_readout_fifo(fifo_tasks_.ptr, ntask, ends_.ptr, events_out_.ptr, flag, fifo_accepted_.ptr,
+                                    fifo_done_.ptr, fifo_serial_.ptr, ready_out_.ptr, fifo_results_.ptr, st),
                 "readout FIFO");
+      // the deliveries (the whole room: a copy is cheaper than a wait for the
+      // counts) and the queue states
+      hip_check(hipMemcpyAsync(host_ready_.ptr, ready_out_.ptr, sizeof(dvs_event) * fifo_room_, hipMemcpyDeviceToHost,
+                               st),
+                "delivered events");
+      hip_check(hipMemcpyAsync(host_fifo_results_.ptr, fifo_results_.ptr, sizeof(FifoResult) * ntask,
+                               hipMemcpyDeviceToHost, st),
+                "readout results");
     };
     run_fifo(total > 0 ? keyflag_.ptr : nullptr);
     hip_check(hipMemcpyAsync(host_words_.ptr + 2, counter_.ptr, sizeof(shadows), hipMemcpyDeviceToHost, st),
@@ -997,9 +1019,9 @@ class Engine {
     hip_check(hipMemcpyAsync(host_words_.ptr, status_.ptr, sizeof(int), hipMemcpyDeviceToHost, st), "copy status");
     wait_words(host_words_.ptr, 1, -1, st, "copy events");
     // The event records, their segments, the ray counter and the readout's
-    // results arrive by other copies (or kernel writes) than the polled
-    // status word; only the stream orders them, so their completion is taken
-    // from the event recorded after them.
+    // results arrive by other copies than the polled status word; only the
+    // stream orders them, so their completion is taken from the event
+    // recorded after them.
     hip_check(hipEventSynchronize(events_t.stop), "copy events");
     status = host_words_.ptr[0];
     std::memcpy(&shadows, host_words_.ptr + 2, sizeof(shadows));
@@ -1031,6 +1053,8 @@ class Engine {
         const FifoResult &r = host_fifo_results_.ptr[task];
         ++task;
         if (r.overflow) fail(DVS_E_STATE, "readout delivered more events than its output room");
+        stats.readout_tasks += 1;
+        stats.readout_serial_tasks += (uint64_t)r.serial;
         s.ready.insert(s.ready.end(), host_ready_.ptr + t.out_offset, host_ready_.ptr + t.out_offset + r.delivered);
         s.fifo.head = r.head;
         s.fifo.count = r.count;
@@ -1567,12 +1591,18 @@ class Engine {
   DeviceArray<int> probe_reuse_;
   PinnedArray<int> host_reuse_;
   // event key flag and the readout FIFO's tasks, results and deliveries
-  // (the last two are written by the kernel into page-locked memory)
+  // (written on the device, copied to page-locked memory)
   DeviceArray<int> keyflag_;
   DeviceArray<FifoTask> fifo_tasks_;
   PinnedArray<FifoTask> host_fifo_tasks_;
   PinnedArray<FifoResult> host_fifo_results_;
   PinnedArray<dvs_event> host_ready_;
+  DeviceArray<FifoResult> fifo_results_;
+  DeviceArray<dvs_event> ready_out_;
+  DeviceArray<int> fifo_accepted_, fifo_serial_;
+  DeviceArray<double> fifo_done_;
+  size_t fifo_room_ = 0;
+  bool force_serial_readout_ = false;
   std::vector<long long> probe_layout_;
   int probe_cur_ = 0;
   DeviceArray<int> rare_native_, rare_adapt_;  // per-wave flags of the photocurrent's second phase

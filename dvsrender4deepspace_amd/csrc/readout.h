@@ -42,6 +42,8 @@ struct FifoTask {
   unsigned long long dropped, peak;
   int pixel_begin, pixel_end;  // the segment's pixels in the step's compaction order
   long long out_offset, out_room;
+  int force_serial;  // take the serial recurrence even where the scans apply
+  int pad;
 };
 
 struct FifoResult {
@@ -49,16 +51,19 @@ struct FifoResult {
   double available;
   unsigned long long dropped, peak;
   int overflow;  // more deliveries than out_room (the host bound was wrong)
-  int pad;
+  int serial;    // 1: the serial recurrence ran, 0: the scans did
 };
 
-// One work-group per task: the serial recurrence of EventReadout::submit
-// (src/event_readout.cpp:13-29) over the task's slice of `sorted` (sorted
-// by (segment, t, y, x); the slice is [ends[pixel_begin - 1],
-// ends[pixel_end - 1])), then the delivered events copied to
-// out[out_offset ...] and the queue state to results[task].  Nothing runs
-// while *skip (if given) is set.
+// The readout of every task: the scan form (one 1,024-thread work-group
+// per task), and the serial recurrence (one wave per task) for the tasks the
+// scan form could not certify.  `accepted` (int) and `done` (double) are
+// scratch of one entry per sorted event; `serial` one int per task.  Each
+// task reads its slice of `sorted` (sorted by (segment, t, y, x); the slice
+// is [ends[pixel_begin - 1], ends[pixel_end - 1])), writes its delivered
+// events to out[out_offset ...] and its queue state to results[task].
+// Nothing runs while *skip (if given) is set.
 hipError_t launch_readout_fifo(const FifoTask *tasks, int task_count, const int *ends, const dvs_event *sorted,
-                               const int *skip, dvs_event *out, FifoResult *results, hipStream_t st);
+                               const int *skip, int *accepted, double *done, int *serial, dvs_event *out,
+                               FifoResult *results, hipStream_t st);
 
 }  // namespace dvsamd

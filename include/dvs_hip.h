@@ -143,6 +143,9 @@ typedef struct {
    * pixel samples traced by each (ss*ss primary rays apiece) */
   double probe_ms, query_ms, probe_launches, query_launches;
   uint64_t probe_samples, query_samples;
+  /* readout FIFO on the device: segment-steps run, and those the scan form
+   * handed to the serial recurrence */
+  uint64_t readout_tasks, readout_serial_tasks;
 } dvs_stats;
 
 /* OpticalRefinementStats (include/dvs/optical_sampler.hpp:16) of the last

@@ -56,12 +56,16 @@ def test_sort_of_event_time_keys(gpu):
     assert np.array_equal(vo, np.argsort(times, kind="stable"))
 
 
-def test_device_fifo_backlog_matches_oracle(oracle, gpu):
+@pytest.mark.parametrize("form", ["scan", "serial"])
+def test_device_fifo_backlog_matches_oracle(oracle, gpu, form, monkeypatch):
     """The small plain request flown fast with low thresholds (thousands of
     events per call) into a 2e5 events/s readout holding at most 6,000: the
     backlog grows past the staged chunks and the first ring, the queue fills
     and drops; every call's deliveries, the drop count, the peak and the
-    final drain equal the oracle's FIFO fed the sensor's comparator events."""
+    final drain equal the oracle's FIFO fed the sensor's comparator events,
+    through the scan form (with its serial fallback where it applies) and
+    through the serial form alone."""
+    monkeypatch.setenv("DVS_READOUT_SERIAL", "1" if form == "serial" else "0")
     w = workloads.small_variant(workloads.PLAIN_SMALL, size=128, width=64, height=48)
     w = dataclasses.replace(w, speed=4000.0, yaw_rate_deg=300.0, frames=2, fps=100.0)  # a 40 m pass in 10 ms
     bw, lat, cap = 2e5, 1e-6, 6000
@@ -85,6 +89,12 @@ def test_device_fifo_backlog_matches_oracle(oracle, gpu):
         largest = max(largest, len(got))
         assert got.tobytes() == want.tobytes(), f"call at horizon {k}"
     dropped, peak = s.readout_stats()
-    print(f"fifo backlog: {submitted} submitted, {dropped} dropped, peak {peak}, largest delivery {largest}", flush=True)
+    st = s.stats()
+    print(f"fifo backlog ({form}): {submitted} submitted, {dropped} dropped, peak {peak}, largest delivery {largest}, "
+          f"{st['readout_serial_tasks']} of {st['readout_tasks']} calls serial", flush=True)
+    if form == "serial":
+        assert st["readout_serial_tasks"] == st["readout_tasks"] > 0
+    else:
+        assert st["readout_serial_tasks"] < st["readout_tasks"]
     assert submitted > cap and dropped == bus.dropped > 0 and peak == bus.peak == cap
     assert s.drain().tobytes() == bus.drain().tobytes()
