@@ -80,13 +80,14 @@ def build_library(jobs: int = 4) -> Path:
 
 def build_io() -> Path:
     """Compiles libdvs_io.so (event writers, configuration, scene and
-    trajectory loaders, frame images; host C++)."""
+    trajectory loaders, GeoTIFF rasters, frame images; host C++)."""
     LIBDIR.mkdir(exist_ok=True)
     if not (HDF5_PREFIX / "include" / "hdf5.h").exists():
         raise RuntimeError(f"HDF5 headers not found under {HDF5_PREFIX}; events.h5 cannot be built")
     tmp = IO_LIBRARY.with_suffix(".so.tmp")
     _run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17", f"-I{ROOT / 'include'}",
-          f"-idirafter{HDF5_PREFIX / 'include'}", str(CSRC / "io_host.cpp"), "-o", str(tmp),
+          f"-idirafter{HDF5_PREFIX / 'include'}", str(CSRC / "io_host.cpp"), str(CSRC / "raster_host.cpp"), "-o",
+          str(tmp),
           f"-L{HDF5_PREFIX / 'lib'}", "-lhdf5", "-lz", f"-Wl,-rpath,{HDF5_PREFIX / 'lib'}"])
     os.replace(tmp, IO_LIBRARY)
     return IO_LIBRARY

@@ -24,10 +24,20 @@
 #include <vector>
 
 #include "dvs_io.h"
+#include "raster_host.h"
 
 namespace {
 
 thread_local std::string g_error;
+
+}  // namespace
+
+struct dvs_raster_s {
+  dvsio::Raster raster;
+  size_t missing = 0;
+};
+
+namespace {
 
 struct IoFailure : std::runtime_error {
   int code;
@@ -842,6 +852,65 @@ int dvs_io_write_png(const char *path, const uint8_t *gray, int32_t width, int32
     if (!path || !gray || width <= 0 || height <= 0) fail(DVS_E_ARGUMENT, "invalid PNG arguments");
     write_png(path, gray, width, height);
   });
+}
+
+int dvs_io_raster_load(const char *path, dvs_raster *out) {
+  return guarded([&] {
+    if (!path || !out) fail(DVS_E_ARGUMENT, "invalid raster arguments");
+    auto r = std::make_unique<dvs_raster_s>();
+    try {
+      r->raster = dvsio::read_geotiff(path);
+      r->missing = dvsio::finish_imagery(r->raster, path);
+    } catch (const std::runtime_error &e) {
+      fail(DVS_E_ARGUMENT, e.what());
+    }
+    *out = r.release();
+  });
+}
+
+int dvs_io_raster_info(dvs_raster r, dvs_raster_info *out) {
+  return guarded([&] {
+    if (!r || !out) fail(DVS_E_ARGUMENT, "invalid raster arguments");
+    const dvsio::Raster &x = r->raster;
+    out->width = x.width;
+    out->height = x.height;
+    std::copy(x.transform, x.transform + 6, out->transform);
+    out->has_nodata = x.has_nodata;
+    out->has_scale = x.has_scale;
+    out->has_offset = x.has_offset;
+    out->nodata = x.nodata;
+    out->scale = x.has_scale ? x.scale : 1.0;
+    out->offset = x.has_offset ? x.offset : 0.0;
+    out->missing = (int64_t)r->missing;
+  });
+}
+
+int dvs_io_raster_data(dvs_raster r, float *data, float *x_coords, float *y_coords) {
+  return guarded([&] {
+    if (!r) fail(DVS_E_ARGUMENT, "invalid raster arguments");
+    const dvsio::Raster &x = r->raster;
+    if (data) std::copy(x.data.begin(), x.data.end(), data);
+    if (x_coords) std::copy(x.x_coords.begin(), x.x_coords.end(), x_coords);
+    if (y_coords) std::copy(x.y_coords.begin(), x.y_coords.end(), y_coords);
+  });
+}
+
+int dvs_io_raster_projection(dvs_raster r, char *out, int64_t capacity, int64_t *length) {
+  return guarded([&] {
+    if (!r || (capacity > 0 && !out)) fail(DVS_E_ARGUMENT, "invalid raster arguments");
+    const std::string &p = r->raster.projection;
+    if (length) *length = (int64_t)p.size();
+    if (capacity > 0) {
+      const size_t n = std::min<size_t>(p.size(), (size_t)capacity - 1);
+      std::memcpy(out, p.data(), n);
+      out[n] = '\0';
+    }
+  });
+}
+
+int dvs_io_raster_free(dvs_raster r) {
+  delete r;
+  return DVS_OK;
 }
 
 int dvs_io_write_tiff(const char *path, const float *data, int32_t width, int32_t height) {

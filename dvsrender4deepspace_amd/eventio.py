@@ -69,6 +69,59 @@ class ConfigError(ValueError):
     pass
 
 
+class _RasterInfoC(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("transform", ctypes.c_double * 6),
+                ("has_nodata", ctypes.c_int32), ("has_scale", ctypes.c_int32), ("has_offset", ctypes.c_int32),
+                ("nodata", ctypes.c_double), ("scale", ctypes.c_double), ("offset", ctypes.c_double),
+                ("missing", ctypes.c_int64)]
+
+
+@dataclasses.dataclass
+class Imagery:
+    """tools::Imagery (include/tools/loader.hpp:17-22): band 1 as float32
+    (NoData as NaN, scale and offset applied), pixel-centre coordinates, the
+    geotransform and the projection key."""
+    data: np.ndarray
+    x_coords: np.ndarray
+    y_coords: np.ndarray
+    transform: tuple
+    projection: str
+    nodata: float | None
+    scale: float
+    offset: float
+    missing: int
+
+    @property
+    def width(self) -> int:
+        return self.data.shape[1]
+
+    @property
+    def height(self) -> int:
+        return self.data.shape[0]
+
+
+def load_raster(path) -> Imagery:
+    """tools::loadImageryData (include/tools/loader.hpp:31-100) without GDAL."""
+    lib = load_library()
+    handle = _P()
+    _check(lib.dvs_io_raster_load(str(path).encode(), ctypes.byref(handle)))
+    try:
+        info = _RasterInfoC()
+        _check(lib.dvs_io_raster_info(handle, ctypes.byref(info)))
+        data = np.empty((info.height, info.width), np.float32)
+        xs = np.empty(info.width, np.float32)
+        ys = np.empty(info.height, np.float32)
+        _check(lib.dvs_io_raster_data(handle, data.ctypes.data, xs.ctypes.data, ys.ctypes.data))
+        length = _I64()
+        _check(lib.dvs_io_raster_projection(handle, None, 0, ctypes.byref(length)))
+        buf = ctypes.create_string_buffer(length.value + 1)
+        _check(lib.dvs_io_raster_projection(handle, buf, length.value + 1, ctypes.byref(length)))
+        return Imagery(data, xs, ys, tuple(info.transform), buf.value.decode(),
+                       info.nodata if info.has_nodata else None, info.scale, info.offset, int(info.missing))
+    finally:
+        lib.dvs_io_raster_free(handle)
+
+
 def load_library() -> ctypes.CDLL:
     global _lib
     if _lib is not None:
@@ -93,6 +146,11 @@ def load_library() -> ctypes.CDLL:
         "dvs_io_frame_bytes": (ctypes.c_int, [_P, _I32, _I32, ctypes.c_float, _P]),
         "dvs_io_write_png": (ctypes.c_int, [ctypes.c_char_p, _P, _I32, _I32]),
         "dvs_io_write_tiff": (ctypes.c_int, [ctypes.c_char_p, _P, _I32, _I32]),
+        "dvs_io_raster_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_P)]),
+        "dvs_io_raster_info": (ctypes.c_int, [_P, ctypes.POINTER(_RasterInfoC)]),
+        "dvs_io_raster_data": (ctypes.c_int, [_P, _P, _P, _P]),
+        "dvs_io_raster_projection": (ctypes.c_int, [_P, ctypes.c_char_p, _I64, ctypes.POINTER(_I64)]),
+        "dvs_io_raster_free": (ctypes.c_int, [_P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

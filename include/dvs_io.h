@@ -85,6 +85,33 @@ int dvs_io_frames_binary(const char *dir, dvs_frame *out, int64_t capacity, int6
  * matrix of the quaternion as given (not normalised); timestamps NaN. */
 int dvs_io_frames_text(const char *path, dvs_frame *out, int64_t capacity, int64_t *count);
 
+/* ---- Raster input (include/tools/loader.hpp, without GDAL) -------------
+ * tools::loadImageryData (loader.hpp:31-100) over a minimal GeoTIFF reader:
+ * band 1 as float32, NoData and non-finite samples as NaN, the rest through
+ * the band's scale and offset, pixel-centre x / y coordinates from the
+ * geotransform; rotated, sheared or degenerate transforms are rejected with
+ * the reference's message.  Classic and BigTIFF, strips or tiles, no / LZW /
+ * Deflate compression, horizontal and floating-point predictors, 8-64 bit
+ * samples.  The projection string identifies the GeoKey directory (two
+ * rasters share a CRS when theirs are equal). */
+typedef struct dvs_raster_s *dvs_raster;
+typedef struct {
+  int32_t width, height;
+  double transform[6];
+  int32_t has_nodata, has_scale, has_offset;
+  double nodata, scale, offset;
+  int64_t missing;  /* NoData samples (NaN) */
+} dvs_raster_info;
+int dvs_io_raster_load(const char *path, dvs_raster *out);
+int dvs_io_raster_info(dvs_raster raster, dvs_raster_info *out);
+/* data: width*height floats (row major); x_coords: width; y_coords: height
+ * (any may be NULL). */
+int dvs_io_raster_data(dvs_raster raster, float *data, float *x_coords, float *y_coords);
+/* Copies up to capacity bytes of the projection key (NUL terminated when
+ * room allows); *length receives its length. */
+int dvs_io_raster_projection(dvs_raster raster, char *out, int64_t capacity, int64_t *length);
+int dvs_io_raster_free(dvs_raster raster);
+
 /* ---- Frame images of bin/cam (apps/main_cam.cpp:79-96) ----------------- */
 /* render_float_to_png_bytes (include/dvs/blenderize.hpp:50-69): exposure
  * x/100 clamped to [0,1], sRGB, optional hash dither, to_byte. */
