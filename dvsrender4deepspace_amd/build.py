@@ -33,7 +33,7 @@ HIP_FLAGS = [
 ]
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
 HIP_SOURCES = ["dvs_kernels.hip", "adaptive.hip", "dvs_capi.hip", "engine.hip", "scan_sort.hip", "readout.hip",
-               "trajectory.hip"]
+               "trajectory.hip", "raster.hip"]
 HOST_SOURCES = ["bvh_host.cpp"]
 # Host I/O library (event writers, configuration): HDF5 1.10 C library of the
 # image's conda prefix, searched after the system headers.

@@ -213,6 +213,9 @@ def load_library() -> ctypes.CDLL:
         "dvs_frame_render_device_noalloc": (ctypes.c_int, [_P, _P, _I32, _P, _P]),
         "dvs_frame_blur": (ctypes.c_int, [_P, _P, _P, _P, _I32, _I32, _P]),
         "dvs_scan_eval": (ctypes.c_int, [_P, _I64, _P]),
+        "dvs_raster_mesh": (ctypes.c_int, [_P, _P, _P, _P, _I32, _I32, _P, _P, _P, ctypes.POINTER(_I64), _P,
+                                           ctypes.POINTER(_I64)]),
+        "dvs_raster_align": (ctypes.c_int, [_P, _P, _I32, _I32, _P, _P, _P, _I32, _I32, _P]),
         "dvs_sort_eval": (ctypes.c_int, [_P, _P, _I64, _I32, _I32, _P, _P]),
         "dvs_scene_stats": (ctypes.c_int, [_P, ctypes.POINTER(_StatsC)]),
         "dvs_frame_traversal_stats": (ctypes.c_int, [_P, _P, _P]),
@@ -942,3 +945,33 @@ def sort_eval(keys, values, begin_bit: int = 0, end_bit: int = 64):
     ko, vo = np.empty_like(k), np.empty_like(v)
     _check(load_library().dvs_sort_eval(_ptr(k), _ptr(v), len(k), begin_bit, end_bit, _ptr(ko), _ptr(vo)))
     return ko, vo
+
+
+def raster_mesh(dem, x_coords, y_coords, albedo=None):
+    """The mesh of a DEM on the device (tools::build_from_raster, planar
+    path, and load_from_raster's recentring): (vertices (N,3), normals (N,3),
+    triangles (M,3) uint32, albedos (N,) or None), in raster order."""
+    d = np.ascontiguousarray(dem, dtype=np.float32)
+    h, w = d.shape
+    xs = np.ascontiguousarray(x_coords, dtype=np.float32)
+    ys = np.ascontiguousarray(y_coords, dtype=np.float32)
+    a = None if albedo is None else np.ascontiguousarray(albedo, dtype=np.float32)
+    v = np.empty((w * h, 3), np.float32)
+    n = np.empty((w * h, 3), np.float32)
+    alb = np.empty(w * h, np.float32) if a is not None else None
+    tri = np.empty((max(2 * (w - 1) * (h - 1), 1), 3), np.uint32)
+    nv, nt = _I64(), _I64()
+    _check(load_library().dvs_raster_mesh(_ptr(d), _ptr(a), _ptr(xs), _ptr(ys), w, h, _ptr(v), _ptr(n), _ptr(alb),
+                                          ctypes.byref(nv), _ptr(tri), ctypes.byref(nt)))
+    return v[:nv.value], n[:nv.value], tri[:nt.value], None if alb is None else alb[:nv.value]
+
+
+def raster_align(ref_x, ref_y, source, source_x, source_y) -> np.ndarray:
+    """batch_align_kernel (src/tools/build_from_raster.cu:653-682) on the device."""
+    rx, ry = np.ascontiguousarray(ref_x, dtype=np.float32), np.ascontiguousarray(ref_y, dtype=np.float32)
+    src = np.ascontiguousarray(source, dtype=np.float32)
+    sx, sy = np.ascontiguousarray(source_x, dtype=np.float32), np.ascontiguousarray(source_y, dtype=np.float32)
+    out = np.empty((len(ry), len(rx)), np.float32)
+    _check(load_library().dvs_raster_align(_ptr(rx), _ptr(ry), len(rx), len(ry), _ptr(src), _ptr(sx), _ptr(sy),
+                                           src.shape[1], src.shape[0], _ptr(out)))
+    return out

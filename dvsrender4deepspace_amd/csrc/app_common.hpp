@@ -1,8 +1,9 @@
 // Shared parts of the bin/dvs and bin/cam front ends: the command line
 // (tools::Options, include/tools/cli.hpp:9-100), input loading and the
 // metadata files (load_app_inputs, apps/app_inputs.hpp:6-33) and the
-// progress bar (include/dvs/progress.hpp).  Raster input (--dem/--albedo)
-// needs GDAL, which this build does not link: it is rejected with a message.
+// progress bar (include/dvs/progress.hpp).  Raster input (--dem, --albedo,
+// --traj) is read as GeoTIFF and meshed on the device (dvs_render_amd.hpp's
+// tools::load_from_raster).
 #pragma once
 #include <chrono>
 #include <cstdlib>
@@ -93,14 +94,17 @@ using Frames = std::vector<dvs::FrameData<dvs::Vec3, dvs::Mat3>>;
 inline void load_inputs(const Options &opts, const dvs::Config &config, dvs::SceneBundle<dvs::float3, dvs::uint3> &scene,
                         Frames &frames) {
   std::string error;
+  bool loaded = false;
   if (!opts.dem_path.empty() || !opts.traj_path.empty() || !opts.albedo_path.empty()) {
     if (opts.dem_path.empty() || opts.traj_path.empty())
       throw std::runtime_error("raster input requires both --dem and --traj");
-    throw std::runtime_error("failed to load scene: raster input (--dem/--albedo) needs GDAL, which this build "
-                             "does not include; pass a --scene_dir bundle");
+    loaded = opts.albedo_path.empty()
+                 ? dvs::load_scene_bundle(opts.dem_path, opts.traj_path, scene, frames, &error)
+                 : dvs::load_scene_bundle(opts.dem_path, opts.albedo_path, opts.traj_path, scene, frames, &error);
+  } else {
+    loaded = dvs::load_scene_bundle(opts.scene_dir, scene, frames, &error);
   }
-  if (!dvs::load_scene_bundle(opts.scene_dir, scene, frames, &error))
-    throw std::runtime_error("failed to load scene: " + error);
+  if (!loaded) throw std::runtime_error("failed to load scene: " + error);
   if (frames.empty()) throw std::runtime_error("empty trajectory");
   frames.resize(std::min(frames.size(), static_cast<size_t>(std::max(config.scene.frame_count, 0))));
   namespace fs = std::filesystem;

@@ -20,6 +20,7 @@
 #include "device_circuit.hpp"
 #include "device_optics.hpp"
 #include "kernels.h"
+#include "raster.h"
 #include "scan_sort.h"
 #include "scene.h"
 
@@ -492,6 +493,80 @@ int dvs_poisson_eval(const double *lambda, const double *u, int64_t n, double *o
     o.reserve(n, "poisson output");
     hip_check(launch_poisson(n, a.ptr, b.ptr, o.ptr, nullptr), "poisson kernel");
     hip_check(hipMemcpy(out, o.ptr, n * sizeof(double), hipMemcpyDeviceToHost), "poisson output");
+  });
+}
+
+int dvs_raster_mesh(const float *dem, const float *albedo, const float *x_coords, const float *y_coords,
+                    int32_t width, int32_t height, float *vertices, float *normals, float *albedos,
+                    int64_t *vertex_count, uint32_t *indices, int64_t *triangle_count) {
+  return guarded([&] {
+    if (!dem || !x_coords || !y_coords || width <= 0 || height <= 0 || !vertices || !normals || !indices ||
+        !vertex_count || !triangle_count || (albedo && !albedos) || (int64_t)width * height > INT32_MAX)
+      fail(DVS_E_ARGUMENT, "invalid raster mesh arguments");
+    const size_t npix = (size_t)width * height, ncell = (size_t)std::max(width - 1, 0) * std::max(height - 1, 0);
+    const float centre = dem[(size_t)width * (height / 2) + width / 2];
+    if (!std::isfinite(centre))
+      fail(DVS_E_ARGUMENT, "DEM center is NoData; choose a valid crop/reference origin before rendering");
+    DeviceArray<float> d_dem, d_alb, d_x, d_y, d_v, d_n, d_a;
+    DeviceArray<int> pflags, cflags, pscan, cscan;
+    DeviceArray<unsigned> d_i;
+    DeviceArray<unsigned char> temp;
+    d_dem.upload(dem, npix, "upload DEM");
+    if (albedo) d_alb.upload(albedo, npix, "upload albedo");
+    d_x.upload(x_coords, width, "upload raster axes");
+    d_y.upload(y_coords, height, "upload raster axes");
+    pflags.reserve(npix, "raster flags");
+    cflags.reserve(std::max<size_t>(ncell, 1), "raster flags");
+    pscan.reserve(npix, "raster scan");
+    cscan.reserve(std::max<size_t>(ncell, 1), "raster scan");
+    hip_check(launch_raster_flags(d_dem.ptr, width, height, pflags.ptr, cflags.ptr, nullptr), "raster flags");
+    size_t bytes = 0, bytes2 = 0;
+    hip_check(inclusive_scan_int(nullptr, bytes, pflags.ptr, pscan.ptr, npix, nullptr), "scan size");
+    hip_check(inclusive_scan_int(nullptr, bytes2, cflags.ptr, cscan.ptr, std::max<size_t>(ncell, 1), nullptr),
+              "scan size");
+    temp.reserve(std::max(bytes, bytes2), "scan scratch");
+    bytes = std::max(bytes, bytes2);
+    hip_check(inclusive_scan_int(temp.ptr, bytes, pflags.ptr, pscan.ptr, npix, nullptr), "raster vertex scan");
+    int nv = 0, nc = 0;
+    if (ncell) hip_check(inclusive_scan_int(temp.ptr, bytes, cflags.ptr, cscan.ptr, ncell, nullptr), "raster cell scan");
+    hip_check(hipMemcpy(&nv, pscan.ptr + npix - 1, sizeof(int), hipMemcpyDeviceToHost), "raster vertex count");
+    if (ncell) hip_check(hipMemcpy(&nc, cscan.ptr + ncell - 1, sizeof(int), hipMemcpyDeviceToHost), "raster cell count");
+    d_v.reserve(std::max(nv, 1) * 3, "raster vertices");
+    d_n.reserve(std::max(nv, 1) * 3, "raster normals");
+    if (albedo) d_a.reserve(std::max(nv, 1), "raster albedos");
+    d_i.reserve(std::max(nc, 1) * 6, "raster triangles");
+    const float cx = -x_coords[width / 2], cy = -y_coords[height / 2], cz = -centre;
+    hip_check(launch_raster_mesh(d_dem.ptr, albedo ? d_alb.ptr : nullptr, d_x.ptr, d_y.ptr, width, height, pscan.ptr,
+                                 cscan.ptr, cflags.ptr, cx, cy, cz, d_v.ptr, d_n.ptr, albedo ? d_a.ptr : nullptr,
+                                 d_i.ptr, nullptr),
+              "raster mesh");
+    hip_check(hipMemcpy(vertices, d_v.ptr, sizeof(float) * 3 * nv, hipMemcpyDeviceToHost), "raster vertices");
+    hip_check(hipMemcpy(normals, d_n.ptr, sizeof(float) * 3 * nv, hipMemcpyDeviceToHost), "raster normals");
+    if (albedo) hip_check(hipMemcpy(albedos, d_a.ptr, sizeof(float) * nv, hipMemcpyDeviceToHost), "raster albedos");
+    hip_check(hipMemcpy(indices, d_i.ptr, sizeof(unsigned) * 6 * nc, hipMemcpyDeviceToHost), "raster triangles");
+    *vertex_count = nv;
+    *triangle_count = 2 * (int64_t)nc;
+  });
+}
+
+int dvs_raster_align(const float *ref_x, const float *ref_y, int32_t width, int32_t height, const float *source,
+                     const float *source_x, const float *source_y, int32_t source_width, int32_t source_height,
+                     float *out) {
+  return guarded([&] {
+    if (!ref_x || !ref_y || !source || !source_x || !source_y || !out || width <= 0 || height <= 0 ||
+        source_width <= 0 || source_height <= 0)
+      fail(DVS_E_ARGUMENT, "invalid raster alignment arguments");
+    DeviceArray<float> rx, ry, src, sx, sy, o;
+    rx.upload(ref_x, width, "upload raster axes");
+    ry.upload(ref_y, height, "upload raster axes");
+    src.upload(source, (size_t)source_width * source_height, "upload raster");
+    sx.upload(source_x, source_width, "upload raster axes");
+    sy.upload(source_y, source_height, "upload raster axes");
+    o.reserve((size_t)width * height, "aligned raster");
+    hip_check(launch_raster_align(rx.ptr, ry.ptr, width, height, src.ptr, sx.ptr, sy.ptr, source_width, source_height,
+                                  o.ptr, nullptr),
+              "raster alignment");
+    hip_check(hipMemcpy(out, o.ptr, sizeof(float) * width * height, hipMemcpyDeviceToHost), "aligned raster");
   });
 }
 

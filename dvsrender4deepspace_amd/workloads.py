@@ -245,6 +245,21 @@ def write_bundle(w: Workload, directory, frames: int | None = None) -> None:
     rec.tofile(d / "frames.bin")
 
 
+def write_text_trajectory(w: Workload, path, frames: int | None = None) -> None:
+    """The workload's flight as a --traj text file (load_frames_from_text,
+    src/scene_data.cpp:238-283): "frame_idx x y z qx qy qz qw lx ly lz L"
+    per frame at t = frame_idx / fps."""
+    from pathlib import Path
+
+    n = w.frames if frames is None else frames
+    s = w.states((np.arange(n) / w.fps).astype(np.float32))
+    lines = []
+    for k in range(n):
+        vals = [*s["position"][k], *s["rotation"][k], *s["light"][k], s["intensity"][k]]
+        lines.append(f"{k} " + " ".join(repr(float(v)) for v in vals))
+    Path(path).write_text("\n".join(lines) + "\n")
+
+
 def write_config(w: Workload, path, frame_count: int | None = None, imu_fps: float = 200.0, cam_fps: float = 30.0,
                  physic: Physic | None = None, blur_resolution: float = 1e-3) -> None:
     """The workload as a configuration file (Config's grammar,

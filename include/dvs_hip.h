@@ -421,6 +421,30 @@ int dvs_trajectory_evaluate(dvs_trajectory trajectory, const float *times, int64
 int dvs_trajectory_intervals(dvs_trajectory trajectory, const double *times, int64_t n,
                              dvs_optical_interval *out);
 
+/* ---- Terrain from rasters (src/tools/build_from_raster.cu) -------------
+ * build_from_raster, planar path (build_from_raster.cu:213-245, 333-472,
+ * 580-610), and the recentring of load_from_raster (:786-802), on the device: a vertex per
+ * pixel with a valid elevation (finite, |z| <= 1e8) at (x_coords[col],
+ * y_coords[row], z), triangles (v1, v3, v2) and (v2, v3, v4) per cell with
+ * four valid corners, angle-weighted vertex normals turned to +z; then every
+ * vertex moved by -(x_coords[width/2], y_coords[height/2], dem[centre]).
+ * Vertices and triangles are numbered in raster order (the reference's
+ * atomic counters give a run-dependent order).  albedo (NULL for none) is
+ * per pixel, already on the DEM grid; albedos receives one per vertex.
+ * vertices and normals hold 3*width*height floats, albedos width*height,
+ * indices 6*(width-1)*(height-1); the counts are returned.  A DEM whose
+ * centre pixel is NoData fails (tools::reference_height, loader.hpp:102-106). */
+int dvs_raster_mesh(const float *dem, const float *albedo, const float *x_coords, const float *y_coords,
+                    int32_t width, int32_t height, float *vertices, float *normals, float *albedos,
+                    int64_t *vertex_count, uint32_t *indices, int64_t *triangle_count);
+/* batch_align_kernel (build_from_raster.cu:661-682): `source` (source_width x
+ * source_height, pixel-centre axes source_x / source_y, either direction)
+ * resampled bilinearly at the reference grid's pixel centres (ref_x, ref_y);
+ * NaN outside the source or next to a NoData sample.  out: width*height. */
+int dvs_raster_align(const float *ref_x, const float *ref_y, int32_t width, int32_t height, const float *source,
+                     const float *source_x, const float *source_y, int32_t source_width, int32_t source_height,
+                     float *out);
+
 /* ---- The reference's __host__ __device__ physics, called on the host ---- */
 /* hapke_reflectance (include/dvs/hapke.hpp:131-151): BRDF x cos(i). */
 float dvs_hapke_reflectance(float cosi, float cose, float cosg, float albedo, const dvs_hapke *hapke);

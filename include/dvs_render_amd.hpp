@@ -978,3 +978,174 @@ inline std::unique_ptr<EventStreamWriter> make_event_writer(const std::string &e
 }
 
 }  // namespace dvs
+
+// ---- Terrain from rasters (include/tools/loader.hpp,
+// src/tools/build_from_raster.cu, src/scene_data.cpp:284-395), without GDAL:
+// GeoTIFF rasters through libdvs_io, the mesh built on the device.
+namespace tools {
+
+// Imagery (loader.hpp:17-22); projection holds the GeoKey identity of the
+// raster's CRS rather than GDAL's WKT.
+struct Imagery {
+  std::vector<float> data, lats, lons, x_coords, y_coords;
+  int width = 0, height = 0;
+  double transform[6]{};
+  std::string projection;
+};
+
+// loadImageryData (loader.hpp:31-100): messages on std::cerr and false on
+// failure, the NoData count on std::cout.
+inline bool loadImageryData(const std::string &path, Imagery &imagery) {
+  dvs_raster r = nullptr;
+  if (dvs_io_raster_load(path.c_str(), &r) != DVS_OK) {
+    std::cerr << dvs_io_last_error() << std::endl;
+    return false;
+  }
+  dvs_raster_info info{};
+  dvs_io_raster_info(r, &info);
+  imagery.width = info.width;
+  imagery.height = info.height;
+  imagery.data.resize((size_t)info.width * info.height);
+  imagery.x_coords.resize((size_t)info.width);
+  imagery.y_coords.resize((size_t)info.height);
+  dvs_io_raster_data(r, imagery.data.data(), imagery.x_coords.data(), imagery.y_coords.data());
+  std::copy(info.transform, info.transform + 6, imagery.transform);
+  int64_t length = 0;
+  dvs_io_raster_projection(r, nullptr, 0, &length);
+  std::vector<char> proj((size_t)length + 1);
+  dvs_io_raster_projection(r, proj.data(), length + 1, &length);
+  imagery.projection.assign(proj.data(), (size_t)length);
+  dvs_io_raster_free(r);
+  if (info.missing)
+    std::cout << path << ": " << info.missing << " NoData samples (NaN), scale=" << info.scale
+              << ", offset=" << info.offset << "\n";
+  return true;
+}
+
+// reference_height (loader.hpp:102-106).
+inline float reference_height(const Imagery &dem) {
+  const float center = dem.data[(size_t)dem.width * (dem.height / 2) + dem.width / 2];
+  if (std::isfinite(center)) return center;
+  throw std::runtime_error("DEM center is NoData; choose a valid crop/reference origin before rendering");
+}
+
+// require_same_projection (loader.hpp:107-114).
+inline void require_same_projection(const Imagery &reference, const Imagery &source) {
+  if (reference.projection == source.projection) return;
+  throw std::runtime_error("Raster CRS differ; reproject to a common CRS before alignment");
+}
+
+namespace detail {
+inline void raster_mesh(const Imagery &dem, const float *albedo, std::vector<dvs::float3> &vertices,
+                        std::vector<dvs::float3> &normals, std::vector<dvs::uint3> &indices,
+                        std::vector<float> *albedos) {
+  const size_t npix = (size_t)dem.width * dem.height;
+  const size_t ncell = (size_t)std::max(dem.width - 1, 0) * std::max(dem.height - 1, 0);
+  vertices.resize(std::max<size_t>(npix, 1));
+  normals.resize(std::max<size_t>(npix, 1));
+  indices.resize(std::max<size_t>(2 * ncell, 1));
+  if (albedos) albedos->resize(std::max<size_t>(npix, 1));
+  int64_t nv = 0, nt = 0;
+  const int code = dvs_raster_mesh(dem.data.data(), albedo, dem.x_coords.data(), dem.y_coords.data(), dem.width,
+                                   dem.height, &vertices[0].x, &normals[0].x, albedos ? albedos->data() : nullptr,
+                                   &nv, &indices[0].x, &nt);
+  if (code != DVS_OK) throw std::runtime_error(dvs_last_error());
+  vertices.resize((size_t)nv);
+  normals.resize((size_t)nv);
+  indices.resize((size_t)nt);
+  if (albedos) albedos->resize((size_t)nv);
+  std::cout << "Generated " << nv << " vertices, " << nv << " normals, and " << nt << " triangles." << std::endl;
+  std::cout << "DEM center x: " << dem.x_coords[dem.width / 2] << ", y: " << dem.y_coords[dem.height / 2]
+            << ", depth: " << reference_height(dem) << std::endl;
+  std::cout << "Converted triangles to DVS format." << std::endl;
+}
+}  // namespace detail
+
+// load_from_raster(dem, albedo, ...) (build_from_raster.cu:683-823): the
+// albedo raster, in the DEM's CRS, resampled onto the DEM grid
+// (batch_align_kernel), then the DEM's mesh with a per-vertex albedo.
+inline bool load_from_raster(const std::string &dem_path, const std::string &albedo_path,
+                             std::vector<dvs::float3> &vertices, std::vector<dvs::float3> &normals,
+                             std::vector<dvs::uint3> &indices, std::vector<float> &albedos) {
+  Imagery dem, albedo;
+  if (!loadImageryData(dem_path, dem)) throw std::runtime_error("failed to load DEM data from: " + dem_path);
+  reference_height(dem);
+  std::cout << "DEM dimensions: " << dem.width << "x" << dem.height << std::endl;
+  if (!loadImageryData(albedo_path, albedo)) throw std::runtime_error("failed to load albedo data from: " + albedo_path);
+  require_same_projection(dem, albedo);
+  std::cout << "Albedo dimensions: " << albedo.width << "x" << albedo.height << std::endl;
+  std::vector<float> aligned((size_t)dem.width * dem.height);
+  if (dvs_raster_align(dem.x_coords.data(), dem.y_coords.data(), dem.width, dem.height, albedo.data.data(),
+                       albedo.x_coords.data(), albedo.y_coords.data(), albedo.width, albedo.height,
+                       aligned.data()) != DVS_OK)
+    throw std::runtime_error(dvs_last_error());
+  detail::raster_mesh(dem, aligned.data(), vertices, normals, indices, &albedos);
+  return true;
+}
+
+// load_from_raster(dem, ...) (build_from_raster.cu:825-920).
+inline bool load_from_raster(const std::string &dem_path, std::vector<dvs::float3> &vertices,
+                             std::vector<dvs::float3> &normals, std::vector<dvs::uint3> &indices) {
+  Imagery dem;
+  if (!loadImageryData(dem_path, dem)) throw std::runtime_error("failed to load DEM data from: " + dem_path);
+  reference_height(dem);
+  std::cout << "DEM dimensions: " << dem.width << "x" << dem.height << std::endl;
+  detail::raster_mesh(dem, nullptr, vertices, normals, indices, nullptr);
+  return true;
+}
+
+}  // namespace tools
+
+namespace dvs {
+namespace detail {
+inline bool finish_raster_bundle(SceneBundle<float3, uint3> &out, std::string *err) {
+  if (!out.albedos.empty()) out.use_albedo = true;
+  out.bvh_nodes.clear();
+  if (out.bvh_triangles.empty()) {
+    if (err) *err = "mesh is empty after triangulation";
+    return false;
+  }
+  return true;
+}
+}  // namespace detail
+
+// load_scene_bundle over rasters (scene_data.cpp:284-395).
+inline bool load_scene_bundle(const std::string &dem_path, const std::string &albedo_path,
+                              SceneBundle<float3, uint3> &out_bundle, std::string *err) {
+  if (!tools::load_from_raster(dem_path, albedo_path, out_bundle.vertices, out_bundle.normals,
+                               out_bundle.bvh_triangles, out_bundle.albedos))
+    return false;
+  return detail::finish_raster_bundle(out_bundle, err);
+}
+inline bool load_scene_bundle(const std::string &dem_path, SceneBundle<float3, uint3> &out_bundle,
+                              std::string *err) {
+  if (!tools::load_from_raster(dem_path, out_bundle.vertices, out_bundle.normals, out_bundle.bvh_triangles))
+    return false;
+  return detail::finish_raster_bundle(out_bundle, err);
+}
+inline bool load_scene_bundle(const std::string &dem_path, const std::string &albedo_path,
+                              const std::string &traj_path, SceneBundle<float3, uint3> &out_bundle,
+                              std::vector<FrameData<Vec3, Mat3>> &frames, std::string *err) {
+  if (!tools::load_from_raster(dem_path, albedo_path, out_bundle.vertices, out_bundle.normals,
+                               out_bundle.bvh_triangles, out_bundle.albedos))
+    return false;
+  if (!load_frames_from_text(traj_path, frames, err)) return false;
+  if (frames.empty()) {
+    if (err) *err = "trajectory file is empty";
+    return false;
+  }
+  return detail::finish_raster_bundle(out_bundle, err);
+}
+inline bool load_scene_bundle(const std::string &dem_path, const std::string &traj_path,
+                              SceneBundle<float3, uint3> &out_bundle, std::vector<FrameData<Vec3, Mat3>> &frames,
+                              std::string *err) {
+  if (!tools::load_from_raster(dem_path, out_bundle.vertices, out_bundle.normals, out_bundle.bvh_triangles))
+    return false;
+  if (!load_frames_from_text(traj_path, frames, err)) return false;
+  if (frames.empty()) {
+    if (err) *err = "trajectory file is empty";
+    return false;
+  }
+  return detail::finish_raster_bundle(out_bundle, err);
+}
+}  // namespace dvs

@@ -1804,3 +1804,167 @@ void orc_trajectory_evaluate(const void *tp, const float *times, int64_t n, orc_
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Terrain from rasters (src/tools/build_from_raster.cu), planar path
+// ---------------------------------------------------------------------------
+namespace orc {
+namespace raster {
+
+struct P3 {
+  float x, y, z;
+};
+inline P3 operator-(P3 a, P3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline float dot(P3 a, P3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline float length(P3 a) { return std::sqrt(dot(a, a)); }
+inline P3 cross(P3 a, P3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline P3 scaled_div(P3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+
+// accumulate_triangle_normal_for_vertex (build_from_raster.cu:37-127) with
+// NORMAL_ORIENT_PLANAR_Z_UP.
+void accumulate(int target, int ta, int tb, int tc, const std::vector<P3> &v, P3 &acc) {
+  if (target != ta && target != tb && target != tc) return;
+  const P3 pa = v[ta], pb = v[tb], pc = v[tc];
+  P3 face = cross(pb - pa, pc - pa);
+  const float face_len = length(face);
+  if (!std::isfinite(face_len) || face_len < 1e-12f) return;
+  face = scaled_div(face, face_len);
+  P3 p, o1, o2;
+  if (target == ta) {
+    p = pa; o1 = pb; o2 = pc;
+  } else if (target == tb) {
+    p = pb; o1 = pc; o2 = pa;
+  } else {
+    p = pc; o1 = pa; o2 = pb;
+  }
+  P3 e1 = o1 - p, e2 = o2 - p;
+  const float l1 = length(e1), l2 = length(e2);
+  if (!std::isfinite(l1) || !std::isfinite(l2) || l1 < 1e-12f || l2 < 1e-12f) return;
+  e1 = scaled_div(e1, l1);
+  e2 = scaled_div(e2, l2);
+  const float w = std::acos(std::fmin(1.0f, std::fmax(-1.0f, dot(e1, e2))));
+  if (!std::isfinite(w)) return;
+  if (face.z < 0.0f) face = {face.x * -1.0f, face.y * -1.0f, face.z * -1.0f};
+  acc = {acc.x + face.x * w, acc.y + face.y * w, acc.z + face.z * w};
+}
+
+}  // namespace raster
+}  // namespace orc
+
+extern "C" {
+
+// load_from_raster (build_from_raster.cu:825-920) after loadImageryData:
+// build_from_raster's vertices (raster order here), triangles and normals
+// (:213-245, 333-472), then the recentring on the DEM's centre pixel
+// (:898-914, reference_height, loader.hpp:102-106).  Returns 0, or -1 for a
+// NoData centre.
+int orc_raster_mesh(const float *dem, const float *albedo, const float *xs, const float *ys, int width, int height,
+                    float *vertices, float *normals, float *albedos, int64_t *vertex_count, uint32_t *indices,
+                    int64_t *triangle_count) {
+  using orc::raster::P3;
+  const float centre = dem[(size_t)width * (height / 2) + width / 2];
+  if (!std::isfinite(centre)) return -1;
+  std::vector<int> hash((size_t)width * height, -1);
+  std::vector<P3> v;
+  for (int y = 0; y < height; ++y)  // batch_build_vertices_kernel (:213-245)
+    for (int x = 0; x < width; ++x) {
+      const float z = dem[(size_t)y * width + x];
+      if (!std::isfinite(z) || std::fabs(z) > 1e8f) continue;
+      hash[(size_t)y * width + x] = (int)v.size();
+      if (albedo) albedos[v.size()] = albedo[(size_t)y * width + x];
+      v.push_back({xs[x], ys[y], z});
+    }
+  auto cell = [&](int cx, int cy, int &v1, int &v2, int &v3, int &v4) {  // fetch_cell_vertices (:129-159)
+    if (cx < 0 || cy < 0 || cx >= width - 1 || cy >= height - 1) return false;
+    v1 = hash[(size_t)cy * width + cx];
+    v2 = hash[(size_t)cy * width + cx + 1];
+    v3 = hash[(size_t)(cy + 1) * width + cx];
+    v4 = hash[(size_t)(cy + 1) * width + cx + 1];
+    return v1 >= 0 && v2 >= 0 && v3 >= 0 && v4 >= 0;
+  };
+  int64_t nt = 0;
+  for (int cy = 0; cy + 1 < height; ++cy)  // batch_build_triangles_kernel (:333-386)
+    for (int cx = 0; cx + 1 < width; ++cx) {
+      int v1, v2, v3, v4;
+      if (!cell(cx, cy, v1, v2, v3, v4)) continue;
+      const uint32_t t[6] = {(uint32_t)v1, (uint32_t)v3, (uint32_t)v2, (uint32_t)v2, (uint32_t)v3, (uint32_t)v4};
+      std::memcpy(indices + 3 * nt, t, sizeof(t));
+      nt += 2;
+    }
+  for (int y = 0; y < height; ++y)  // batch_gather_vertex_normals_kernel (:388-472)
+    for (int x = 0; x < width; ++x) {
+      const int me = hash[(size_t)y * width + x];
+      if (me < 0) continue;
+      P3 acc = {0, 0, 0};
+      for (int oy = -1; oy <= 0; ++oy)
+        for (int ox = -1; ox <= 0; ++ox) {
+          int v1, v2, v3, v4;
+          if (!cell(x + ox, y + oy, v1, v2, v3, v4)) continue;
+          orc::raster::accumulate(me, v1, v3, v2, v, acc);
+          orc::raster::accumulate(me, v2, v3, v4, v, acc);
+        }
+      const float len = orc::raster::length(acc);
+      P3 n = {0, 0, 0};
+      if (std::isfinite(len) && len >= 1e-12f) n = orc::raster::scaled_div(acc, len);
+      // apply_rigid_transform with the identity quaternion (:612-622): the
+      // rotation's zero terms and the zero translation leave no -0
+      normals[3 * me] = n.x + 0.0f;
+      normals[3 * me + 1] = n.y + 0.0f;
+      normals[3 * me + 2] = n.z + 0.0f;
+    }
+  const float tx = -xs[width / 2], ty = -ys[height / 2], tz = -centre;
+  for (size_t i = 0; i < v.size(); ++i) {
+    vertices[3 * i] = v[i].x + tx;
+    vertices[3 * i + 1] = v[i].y + ty;
+    vertices[3 * i + 2] = v[i].z + tz;
+  }
+  *vertex_count = (int64_t)v.size();
+  *triangle_count = nt;
+  return 0;
+}
+
+// batch_align_kernel (build_from_raster.cu:653-682).
+void orc_raster_align(const float *ref_x, const float *ref_y, int width, int height, const float *source,
+                      const float *source_x, const float *source_y, int source_width, int source_height, float *out) {
+  auto bracket = [](const float *axis, int n, float value) {
+    if (n < 2 || !std::isfinite(value)) return -1;
+    const bool increasing = axis[n - 1] > axis[0];
+    if (value < std::fmin(axis[0], axis[n - 1]) || value > std::fmax(axis[0], axis[n - 1])) return -1;
+    int lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) / 2;
+      if ((axis[mid] <= value) == increasing) lo = mid;
+      else hi = mid;
+    }
+    return lo;
+  };
+  for (int y = 0; y < height; ++y)
+    for (int x = 0; x < width; ++x) {
+      const int row = bracket(source_y, source_height, ref_y[y]), col = bracket(source_x, source_width, ref_x[x]);
+      float value = std::numeric_limits<float>::quiet_NaN();
+      if (row >= 0 && col >= 0) {
+        const float u = (ref_x[x] - source_x[col]) / (source_x[col + 1] - source_x[col]);
+        const float v = (ref_y[y] - source_y[row]) / (source_y[row + 1] - source_y[row]);
+        double total = 0, weight = 0;
+        bool valid = true;
+        for (int dy = 0; dy < 2; ++dy)
+          for (int dx = 0; dx < 2; ++dx) {
+            const float w = (dx ? u : 1 - u) * (dy ? v : 1 - v);
+            const float s = source[(size_t)(row + dy) * source_width + col + dx];
+            if (w > 0) {
+              if (!std::isfinite(s)) {
+                valid = false;
+                continue;
+              }
+              const float product = w * s;
+              total += product;
+              weight += w;
+            }
+          }
+        if (valid && weight > 0) value = (float)(total / weight);
+      }
+      out[(size_t)y * width + x] = value;
+    }
+}
+
+}  // extern "C"

@@ -276,6 +276,13 @@ void orc_trajectory_destroy(void *t);
 void orc_trajectory_range(const void *t, float *start, float *end, int *size);
 void orc_trajectory_evaluate(const void *t, const float *times, int64_t n, orc_trajectory_state *out);
 
+/* ---- Terrain from rasters (src/tools/build_from_raster.cu) ---- */
+int orc_raster_mesh(const float *dem, const float *albedo, const float *xs, const float *ys, int width, int height,
+                    float *vertices, float *normals, float *albedos, int64_t *vertex_count, uint32_t *indices,
+                    int64_t *triangle_count);
+void orc_raster_align(const float *ref_x, const float *ref_y, int width, int height, const float *source,
+                      const float *source_x, const float *source_y, int source_width, int source_height, float *out);
+
 /* ---- Image encode (include/dvs/blenderize.hpp) ---- */
 void orc_render_to_bytes(const float *linear, int width, int height, float dither,
                          uint8_t *out);

@@ -139,6 +139,8 @@ def lib() -> ctypes.CDLL:
             "orc_readout_dropped": (I64, [_P]),
             "orc_readout_peak": (I64, [_P]),
             "orc_render_to_bytes": (None, [_P, I, I, F, _P]),
+            "orc_raster_mesh": (I, [_P, _P, _P, _P, I, I, _P, _P, _P, ctypes.POINTER(I64), _P, ctypes.POINTER(I64)]),
+            "orc_raster_align": (None, [_P, _P, I, I, _P, _P, _P, I, I, _P]),
             "orc_trajectory_create": (_P, [_P, I, F, _P, _P, ctypes.c_char_p, I]),
             "orc_trajectory_destroy": (None, [_P]),
             "orc_trajectory_range": (None, [_P, ctypes.POINTER(F), ctypes.POINTER(F), ctypes.POINTER(I)]),
@@ -444,6 +446,37 @@ def blur_exposures(frames, weights, cell_offsets, wide=False):
             for c in range(offsets[e], offsets[e + 1]):
                 acc = acc + frames[c] * np.float32(weights[c])
             out[e] = acc
+    return out
+
+
+def raster_mesh(dem, x_coords, y_coords, albedo=None):
+    """load_from_raster's mesh of a DEM (src/tools/build_from_raster.cu,
+    planar path): (vertices (N,3), normals (N,3), triangles (M,3) uint32,
+    albedos (N,) or None), or None for a NoData centre pixel."""
+    d = np.ascontiguousarray(dem, np.float32)
+    h, w = d.shape
+    xs = np.ascontiguousarray(x_coords, np.float32)
+    ys = np.ascontiguousarray(y_coords, np.float32)
+    a = None if albedo is None else np.ascontiguousarray(albedo, np.float32)
+    v = np.empty((w * h, 3), np.float32)
+    n = np.empty((w * h, 3), np.float32)
+    alb = np.empty(w * h, np.float32) if a is not None else None
+    t = np.empty((max(2 * (w - 1) * (h - 1), 1), 3), np.uint32)
+    nv, nt = ctypes.c_int64(), ctypes.c_int64()
+    if lib().orc_raster_mesh(ptr(d), ptr(a), ptr(xs), ptr(ys), w, h, ptr(v), ptr(n), ptr(alb), ctypes.byref(nv),
+                             ptr(t), ctypes.byref(nt)):
+        return None
+    return v[:nv.value], n[:nv.value], t[:nt.value], None if alb is None else alb[:nv.value]
+
+
+def raster_align(ref_x, ref_y, source, source_x, source_y):
+    """batch_align_kernel (src/tools/build_from_raster.cu:653-682)."""
+    rx, ry = np.ascontiguousarray(ref_x, np.float32), np.ascontiguousarray(ref_y, np.float32)
+    src = np.ascontiguousarray(source, np.float32)
+    sx, sy = np.ascontiguousarray(source_x, np.float32), np.ascontiguousarray(source_y, np.float32)
+    out = np.empty((len(ry), len(rx)), np.float32)
+    lib().orc_raster_align(ptr(rx), ptr(ry), len(rx), len(ry), ptr(src), ptr(sx), ptr(sy), src.shape[1], src.shape[0],
+                           ptr(out))
     return out
 
 
