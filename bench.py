@@ -449,6 +449,35 @@ def blur_leg(w, scene, traj, mesh, exposures: int, cpu_cells: int) -> dict:
     return out
 
 
+def terrain_leg(w, heights, cpu: bool) -> dict:
+    """Terrain ingest (tools::load_from_raster, src/tools/build_from_raster.cu):
+    the request's DEM meshed on the device, host buffers in and out, against
+    the CPU oracle's restatement (one core)."""
+    from dvsrender4deepspace_amd import capi
+
+    n = heights.shape[0]
+    axis = (np.arange(n) * w.terrain.spacing + 0.5).astype(np.float32)
+    ys = axis[::-1].copy()
+    capi.raster_mesh(heights[:64, :64], axis[:64], ys[:64])  # warm-up
+    t0 = time.perf_counter()
+    v, _, tri, _ = capi.raster_mesh(heights, axis, ys)
+    wall = time.perf_counter() - t0
+    out = {"dem": f"{n}x{n}", "vertices": int(len(v)), "triangles": int(len(tri)), "ms": round(wall * 1e3, 2),
+           "mtriangles_per_s": round(len(tri) / wall / 1e6, 2),
+           "note": "host DEM in, host mesh out (uploads, flags, two prefix sums, vertices, normals, triangles, "
+                   "downloads)"}
+    if cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle_ctypes as oracle
+
+        t0 = time.perf_counter()
+        oracle.raster_mesh(heights, axis, ys)
+        cpu_s = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(len(tri) / cpu_s / 1e6, 3), "unit": "Mtriangles/s", "cores": 1,
+                               "kind": "port", "ms": round(cpu_s * 1e3, 1), "sample": f"the whole {n}x{n} DEM"}
+    return out
+
+
 def blur_mode(args) -> dict:
     from dvsrender4deepspace_amd import capi, terrain, workloads
 
@@ -628,6 +657,9 @@ def request_mode(args) -> dict:
         "trajectory_s_per_wall_s": round(nsteps * args.horizons * w.time_resolution / nwall, 6),
         "trace_cells_ms_per_launch": round(ns["trace_ms"] / max(ns["trace_launches"], 1), 3)}
     nat.close()
+    log("terrain ingest ...")
+    result["terrain_ingest"] = terrain_leg(w, terrain.heightfield(w.terrain).astype(np.float32),
+                                           not args.no_cpu_baseline)
     log("motion blur ...")
     result["motion_blur"] = blur_leg(w, scene, traj, (v, n, nodes, root, leaf_idx), args.blur_exposures,
                                      0 if args.no_cpu_baseline else args.blur_cpu_cells)

@@ -100,7 +100,7 @@ API_SELFTEST = BINDIR / "dvs_api_selftest"
 # Programs on the C++ API: the self-test and the bin/dvs and bin/cam front
 # ends (apps/main_dvs.cpp, apps/main_cam.cpp).
 PROGRAMS = {"dvs_api_selftest": "api_selftest.cpp", "dvs": "main_dvs.cpp", "cam": "main_cam.cpp",
-            "cam_blur": "main_cam_blur.cpp"}
+            "cam_blur": "main_cam_blur.cpp", "sequence": "main_sequence.cpp"}
 
 
 def _link_program(name: str) -> Path:
@@ -122,8 +122,8 @@ def build_api_selftest() -> Path:
 
 
 def build_apps() -> list[Path]:
-    """Links bin/dvs, bin/cam and bin/cam_blur."""
-    return [_link_program(name) for name in ("dvs", "cam", "cam_blur")]
+    """Links bin/dvs, bin/cam, bin/cam_blur and bin/sequence."""
+    return [_link_program(name) for name in ("dvs", "cam", "cam_blur", "sequence")]
 
 
 def build_oracle() -> Path:

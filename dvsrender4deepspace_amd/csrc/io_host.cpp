@@ -68,6 +68,8 @@ class Writer {
   virtual ~Writer() = default;
   virtual void append(std::vector<dvs_event> events) = 0;
   virtual void close(double end_time_seconds) = 0;
+  // Pushes buffered data to the file, so that its size on disk is current.
+  virtual void flush() {}
   int64_t count = 0;
 };
 
@@ -118,6 +120,9 @@ class H5Writer final : public Writer {
     check_rc(H5Fflush(file_, H5F_SCOPE_GLOBAL), "failed to flush new event file");
   }
   ~H5Writer() override { release(); }
+  void flush() override {
+    if (file_ >= 0) check_rc(H5Fflush(file_, H5F_SCOPE_GLOBAL), "failed to flush event batch");
+  }
 
   void append(std::vector<dvs_event> events) override {
     if (closed_) fail(DVS_E_STATE, "cannot append to closed HDF5 event file");
@@ -235,17 +240,16 @@ class H5Writer final : public Writer {
   bool closed_ = false;
 };
 
-// Reads a whole 1-D dataset.
+// Reads a whole dataset (any rank) in row-major order.
 template <class T>
 std::vector<T> read_column(hid_t file, const char *name, hid_t type) {
   const hid_t set = H5Dopen2(file, name, H5P_DEFAULT);
   if (set < 0) fail(DVS_E_ARGUMENT, std::string("missing HDF5 dataset ") + name);
   const hid_t space = H5Dget_space(set);
-  hsize_t dims[1] = {0};
-  H5Sget_simple_extent_dims(space, dims, nullptr);
-  std::vector<T> out(dims[0]);
+  const hssize_t n = H5Sget_simple_extent_npoints(space);
+  std::vector<T> out(n > 0 ? (size_t)n : 0);
   herr_t rc = 0;
-  if (dims[0]) rc = H5Dread(set, type, H5S_ALL, H5S_ALL, H5P_DEFAULT, out.data());
+  if (n > 0) rc = H5Dread(set, type, H5S_ALL, H5S_ALL, H5P_DEFAULT, out.data());
   H5Sclose(space);
   H5Dclose(set);
   if (rc < 0) fail(DVS_E_STATE, std::string("failed to read HDF5 dataset ") + name);
@@ -503,8 +507,113 @@ void load_config(const std::string &path, dvs_config &c) {
 
 }  // namespace
 
+namespace {
+uint8_t encode_pixel(float linear, int x, int y, float dither);  // render_float_to_png_bytes, per pixel
+}  // namespace
+
+// frames.h5 (Hdf5FrameWriter, include/dvs/frame_writer.hpp:12-77): t (int64
+// microseconds, chunks of 4096), images (uint8, N x H x W) and depth
+// (float32, N x H x W), one frame per chunk, byte shuffle and Deflate level
+// 1 on the frame datasets when the library has the filter.
+class FrameFile {
+ public:
+  FrameFile(const std::string &path, int width, int height) : width_(width), height_(height) {
+    if (width <= 0 || height <= 0) fail(DVS_E_ARGUMENT, "invalid frame dimensions");
+    file_ = H5Fcreate(path.c_str(), H5F_ACC_EXCL, H5P_DEFAULT, H5P_DEFAULT);
+    check(file_, "create frames file");
+    times_ = create("t", H5T_STD_I64LE, 1);
+    images_ = create("images", H5T_STD_U8LE, 3);
+    depths_ = create("depth", H5T_IEEE_F32LE, 3);
+    check(H5Fflush(file_, H5F_SCOPE_GLOBAL), "flush frames header");
+  }
+  ~FrameFile() { release(); }
+  void append(double time, const float *radiance, const float *depth) {
+    const size_t pixels = (size_t)width_ * height_;
+    if (closed_ || !std::isfinite(time) || time <= last_time_ || !radiance || !depth)
+      fail(DVS_E_ARGUMENT, "invalid frame timestamp or array size");
+    for (size_t i = 0; i < pixels; ++i)
+      if (!std::isfinite(radiance[i]) || !std::isfinite(depth[i]) || (depth[i] < 0 && depth[i] != -1))
+        fail(DVS_E_ARGUMENT, "invalid radiance or depth value");
+    std::vector<uint8_t> image(pixels);
+    for (int y = 0; y < height_; ++y)
+      for (int x = 0; x < width_; ++x)
+        image[(size_t)y * width_ + x] = encode_pixel(radiance[(size_t)y * width_ + x], x, y, 0.0f);
+    const int64_t stamp = std::llround(time * 1e6);
+    put(times_, H5T_NATIVE_INT64, &stamp, 1);
+    put(images_, H5T_NATIVE_UINT8, image.data(), 3);
+    put(depths_, H5T_NATIVE_FLOAT, depth, 3);
+    check(H5Fflush(file_, H5F_SCOPE_GLOBAL), "flush frame");
+    ++count_;
+    last_time_ = time;
+  }
+  void close() {
+    if (closed_) return;
+    check(H5Fflush(file_, H5F_SCOPE_GLOBAL), "flush final frame");
+    release();
+    closed_ = true;
+  }
+  int64_t count() const { return (int64_t)count_; }
+
+ private:
+  static void check(int64_t code, const char *message) {
+    if (code < 0) fail(DVS_E_STATE, message);
+  }
+  hid_t create(const char *name, hid_t type, int rank) {
+    const hsize_t dims[3] = {0, (hsize_t)height_, (hsize_t)width_};
+    const hsize_t maxdims[3] = {H5S_UNLIMITED, (hsize_t)height_, (hsize_t)width_};
+    const hsize_t chunks[3] = {rank == 1 ? 4096ULL : 1ULL, (hsize_t)height_, (hsize_t)width_};
+    const hid_t space = H5Screate_simple(rank, dims, maxdims), plist = H5Pcreate(H5P_DATASET_CREATE);
+    check(space, "create frame dataspace");
+    check(plist, "create frame properties");
+    check(H5Pset_chunk(plist, rank, chunks), "set frame chunks");
+    if (rank == 3 && H5Zfilter_avail(H5Z_FILTER_DEFLATE) > 0) {
+      check(H5Pset_shuffle(plist), "set lossless shuffle");
+      check(H5Pset_deflate(plist, 1), "set lossless deflate");
+    }
+    const hid_t set = H5Dcreate2(file_, name, type, space, H5P_DEFAULT, plist, H5P_DEFAULT);
+    H5Pclose(plist);
+    H5Sclose(space);
+    check(set, "create frame dataset");
+    return set;
+  }
+  void put(hid_t set, hid_t type, const void *data, int rank) {
+    const hsize_t size[3] = {count_ + 1, (hsize_t)height_, (hsize_t)width_};
+    const hsize_t start[3] = {count_, 0, 0}, block[3] = {1, (hsize_t)height_, (hsize_t)width_};
+    check(H5Dset_extent(set, size), "extend frame dataset");
+    const hid_t space = H5Dget_space(set);
+    check(space, "get frame dataspace");
+    check(H5Sselect_hyperslab(space, H5S_SELECT_SET, start, nullptr, block, nullptr), "select frame hyperslab");
+    const hid_t memory = H5Screate_simple(rank, block, nullptr);
+    check(memory, "create frame memory space");
+    const herr_t rc = H5Dwrite(set, type, memory, space, H5P_DEFAULT, data);
+    H5Sclose(memory);
+    H5Sclose(space);
+    check(rc, "write frame dataset");
+  }
+  void release() {
+    for (hid_t *p : {&times_, &images_, &depths_})
+      if (*p >= 0) {
+        H5Dclose(*p);
+        *p = -1;
+      }
+    if (file_ >= 0) {
+      H5Fclose(file_);
+      file_ = -1;
+    }
+  }
+  int width_, height_;
+  hid_t file_ = -1, times_ = -1, images_ = -1, depths_ = -1;
+  hsize_t count_ = 0;
+  double last_time_ = -std::numeric_limits<double>::infinity();
+  bool closed_ = false;
+};
+
 struct dvs_writer_s {
   std::unique_ptr<Writer> impl;
+};
+
+struct dvs_frames_s {
+  std::unique_ptr<FrameFile> impl;
 };
 
 extern "C" {
@@ -534,6 +643,13 @@ int dvs_writer_close(dvs_writer w, double end_time_seconds) {
   return guarded([&] {
     if (!w) fail(DVS_E_ARGUMENT, "null event writer");
     w->impl->close(end_time_seconds);
+  });
+}
+
+int dvs_writer_flush(dvs_writer w) {
+  return guarded([&] {
+    if (!w) fail(DVS_E_ARGUMENT, "null event writer");
+    w->impl->flush();
   });
 }
 
@@ -847,6 +963,68 @@ int dvs_io_frame_bytes(const float *linear, int32_t width, int32_t height, float
   });
 }
 
+int dvs_frames_open(const char *path, int32_t width, int32_t height, dvs_frames *out) {
+  return guarded([&] {
+    if (!path || !out) fail(DVS_E_ARGUMENT, "invalid frame writer arguments");
+    *out = nullptr;
+    auto f = std::make_unique<dvs_frames_s>();
+    f->impl = std::make_unique<FrameFile>(path, width, height);
+    *out = f.release();
+  });
+}
+
+int dvs_frames_append(dvs_frames f, double time, const float *radiance, const float *depth) {
+  return guarded([&] {
+    if (!f) fail(DVS_E_ARGUMENT, "null frame writer");
+    f->impl->append(time, radiance, depth);
+  });
+}
+
+int dvs_frames_close(dvs_frames f) {
+  return guarded([&] {
+    if (!f) fail(DVS_E_ARGUMENT, "null frame writer");
+    f->impl->close();
+  });
+}
+
+int dvs_frames_count(dvs_frames f, int64_t *count) {
+  return guarded([&] {
+    if (!f || !count) fail(DVS_E_ARGUMENT, "null frame writer");
+    *count = f->impl->count();
+  });
+}
+
+int dvs_frames_destroy(dvs_frames f) {
+  delete f;
+  return DVS_OK;
+}
+
+int dvs_frames_h5_read(const char *path, int64_t *times, uint8_t *images, float *depth, int64_t capacity,
+                       int32_t *width, int32_t *height, int64_t *count) {
+  return guarded([&] {
+    if (!path || !count || !width || !height) fail(DVS_E_ARGUMENT, "invalid HDF5 read arguments");
+    H5File f(path);
+    const hid_t set = H5Dopen2(f.id, "images", H5P_DEFAULT);
+    if (set < 0) fail(DVS_E_STATE, "failed to open HDF5 dataset images");
+    const hid_t space = H5Dget_space(set);
+    hsize_t dims[3] = {0, 0, 0};
+    const int rank = H5Sget_simple_extent_dims(space, dims, nullptr);
+    H5Sclose(space);
+    H5Dclose(set);
+    if (rank != 3) fail(DVS_E_STATE, "frames images must be N x H x W");
+    *count = (int64_t)dims[0];
+    *height = (int32_t)dims[1];
+    *width = (int32_t)dims[2];
+    if (capacity < *count || !times || !images || !depth) return;
+    const auto t = read_column<int64_t>(f.id, "t", H5T_NATIVE_INT64);
+    const auto im = read_column<uint8_t>(f.id, "images", H5T_NATIVE_UINT8);
+    const auto d = read_column<float>(f.id, "depth", H5T_NATIVE_FLOAT);
+    std::copy(t.begin(), t.end(), times);
+    std::copy(im.begin(), im.end(), images);
+    std::copy(d.begin(), d.end(), depth);
+  });
+}
+
 int dvs_io_write_png(const char *path, const uint8_t *gray, int32_t width, int32_t height) {
   return guarded([&] {
     if (!path || !gray || width <= 0 || height <= 0) fail(DVS_E_ARGUMENT, "invalid PNG arguments");
@@ -882,6 +1060,7 @@ int dvs_io_raster_info(dvs_raster r, dvs_raster_info *out) {
     out->scale = x.has_scale ? x.scale : 1.0;
     out->offset = x.has_offset ? x.offset : 0.0;
     out->missing = (int64_t)r->missing;
+    out->center_raw = x.center_raw;
   });
 }
 
@@ -905,6 +1084,13 @@ int dvs_io_raster_projection(dvs_raster r, char *out, int64_t capacity, int64_t 
       std::memcpy(out, p.data(), n);
       out[n] = '\0';
     }
+  });
+}
+
+int dvs_io_raster_geokey(dvs_raster r, int32_t key, double *value) {
+  return guarded([&] {
+    if (!r || !value) fail(DVS_E_ARGUMENT, "invalid raster arguments");
+    if (!dvsio::geokey_value(r->raster, key, *value)) fail(DVS_E_ARGUMENT, "raster has no GeoKey " + std::to_string(key));
   });
 }
 

@@ -394,10 +394,13 @@ Raster read_geotiff(const std::string &path) {
         r.transform[3] -= 0.5 * r.transform[4] + 0.5 * r.transform[5];
       }
     }
+    r.geokeys = keys;
     proj << "geokeys";
     for (double k : keys) proj << ' ' << (long long)k;
-    if (t.has(34736))
-      for (double d : t.numbers(34736)) proj << ' ' << std::hexfloat << d << std::defaultfloat;
+    if (t.has(34736)) {
+      r.geodoubles = t.numbers(34736);
+      for (double d : r.geodoubles) proj << ' ' << std::hexfloat << d << std::defaultfloat;
+    }
     if (t.has(34737)) proj << " '" << t.ascii(34737) << "'";
   }
   r.projection = proj.str();
@@ -434,6 +437,24 @@ Raster read_geotiff(const std::string &path) {
   return r;
 }
 
+bool geokey_value(const Raster &r, int key, double &value) {
+  const auto &k = r.geokeys;
+  for (size_t i = 4; i + 3 < k.size(); i += 4) {
+    if ((int)k[i] != key) continue;
+    const int location = (int)k[i + 1], count = (int)k[i + 2], offset = (int)k[i + 3];
+    if (location == 0 && count == 1) {
+      value = k[i + 3];
+      return true;
+    }
+    if (location == 34736 && offset >= 0 && (size_t)offset < r.geodoubles.size()) {
+      value = r.geodoubles[offset];
+      return true;
+    }
+    return false;
+  }
+  return false;
+}
+
 // loadImageryData's post-processing (loader.hpp:68-96): rejects rotated or
 // degenerate transforms, maps NoData and non-finite samples to NaN and the
 // rest through scale and offset, and builds the pixel-centre coordinates.
@@ -444,6 +465,7 @@ size_t finish_imagery(Raster &r, const std::string &path) {
                              path);
   const float nodata = (float)r.nodata;
   const double scale = r.has_scale ? r.scale : 1.0, offset = r.has_offset ? r.offset : 0.0;
+  r.center_raw = r.data[(size_t)r.width * (r.height / 2) + r.width / 2];
   size_t missing = 0;
   for (float &v : r.data) {
     if (!std::isfinite(v) || (r.has_nodata && v == nodata)) {

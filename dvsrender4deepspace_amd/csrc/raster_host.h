@@ -15,7 +15,12 @@ struct Raster {
   bool has_nodata = false, has_scale = false, has_offset = false;
   double nodata = 0, scale = 1, offset = 0;
   std::string projection;
+  std::vector<double> geokeys, geodoubles;  // GeoKeyDirectoryTag, GeoDoubleParamsTag
+  float center_raw = 0;  // the centre pixel's stored sample, before NoData / scale / offset
 };
+
+// The value of a GeoKey: its SHORT value or its GeoDoubleParams entry.
+bool geokey_value(const Raster &r, int key, double &value);
 
 // Reads the file; throws std::runtime_error with loadImageryData's messages.
 Raster read_geotiff(const std::string &path);

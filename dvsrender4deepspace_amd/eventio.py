@@ -73,7 +73,7 @@ class _RasterInfoC(ctypes.Structure):
     _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("transform", ctypes.c_double * 6),
                 ("has_nodata", ctypes.c_int32), ("has_scale", ctypes.c_int32), ("has_offset", ctypes.c_int32),
                 ("nodata", ctypes.c_double), ("scale", ctypes.c_double), ("offset", ctypes.c_double),
-                ("missing", ctypes.c_int64)]
+                ("missing", ctypes.c_int64), ("center_raw", ctypes.c_float)]
 
 
 @dataclasses.dataclass
@@ -151,6 +151,15 @@ def load_library() -> ctypes.CDLL:
         "dvs_io_raster_data": (ctypes.c_int, [_P, _P, _P, _P]),
         "dvs_io_raster_projection": (ctypes.c_int, [_P, ctypes.c_char_p, _I64, ctypes.POINTER(_I64)]),
         "dvs_io_raster_free": (ctypes.c_int, [_P]),
+        "dvs_io_raster_geokey": (ctypes.c_int, [_P, _I32, ctypes.POINTER(ctypes.c_double)]),
+        "dvs_frames_open": (ctypes.c_int, [ctypes.c_char_p, _I32, _I32, ctypes.POINTER(_P)]),
+        "dvs_frames_append": (ctypes.c_int, [_P, ctypes.c_double, _P, _P]),
+        "dvs_frames_close": (ctypes.c_int, [_P]),
+        "dvs_frames_count": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),
+        "dvs_frames_destroy": (ctypes.c_int, [_P]),
+        "dvs_frames_h5_read": (ctypes.c_int, [ctypes.c_char_p, _P, _P, _P, _I64, ctypes.POINTER(_I32),
+                                              ctypes.POINTER(_I32), ctypes.POINTER(_I64)]),
+        "dvs_writer_flush": (ctypes.c_int, [_P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -288,3 +297,48 @@ def write_png(path, gray: np.ndarray) -> None:
 def write_tiff(path, data: np.ndarray) -> None:
     d = np.ascontiguousarray(data, dtype=np.float32)
     _check(load_library().dvs_io_write_tiff(str(path).encode(), d.ctypes.data, d.shape[1], d.shape[0]))
+
+
+class FrameWriter:
+    """Hdf5FrameWriter (include/dvs/frame_writer.hpp:12-77) without the
+    dataset budget: frames.h5 with t (int64 us), images (uint8 sRGB preview)
+    and depth (float32)."""
+
+    def __init__(self, path, width: int, height: int):
+        self.width, self.height = width, height
+        self._h = _P()
+        _check(load_library().dvs_frames_open(str(path).encode(), width, height, ctypes.byref(self._h)))
+
+    def append(self, time: float, radiance, depth) -> None:
+        r = np.ascontiguousarray(radiance, dtype=np.float32).reshape(-1)
+        d = np.ascontiguousarray(depth, dtype=np.float32).reshape(-1)
+        if r.size != self.width * self.height or d.size != r.size:
+            raise ConfigError("invalid frame timestamp or array size")
+        _check(load_library().dvs_frames_append(self._h, float(time), r.ctypes.data, d.ctypes.data))
+
+    def count(self) -> int:
+        n = _I64()
+        _check(load_library().dvs_frames_count(self._h, ctypes.byref(n)))
+        return n.value
+
+    def close(self) -> None:
+        _check(load_library().dvs_frames_close(self._h))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            load_library().dvs_frames_destroy(self._h)
+            self._h = None
+
+
+def read_frames_h5(path) -> dict:
+    """frames.h5 back: t (N,) int64, images (N, H, W) uint8, depth (N, H, W) float32."""
+    lib = load_library()
+    w, h, n = _I32(), _I32(), _I64()
+    _check(lib.dvs_frames_h5_read(str(path).encode(), None, None, None, 0, ctypes.byref(w), ctypes.byref(h),
+                                  ctypes.byref(n)))
+    t = np.empty(n.value, np.int64)
+    im = np.empty((n.value, h.value, w.value), np.uint8)
+    d = np.empty((n.value, h.value, w.value), np.float32)
+    _check(lib.dvs_frames_h5_read(str(path).encode(), t.ctypes.data, im.ctypes.data, d.ctypes.data, n.value,
+                                  ctypes.byref(w), ctypes.byref(h), ctypes.byref(n)))
+    return {"t": t, "images": im, "depth": d}

@@ -32,6 +32,8 @@ int dvs_writer_open(const char *path, int32_t format, dvs_writer *out);
 int dvs_writer_append(dvs_writer writer, const dvs_event *events, int64_t count);
 /* close(end_time_seconds) (:127-143); closing twice is a no-op. */
 int dvs_writer_close(dvs_writer writer, double end_time_seconds);
+/* Flushes buffered records to the file (for size accounting). */
+int dvs_writer_flush(dvs_writer writer);
 int dvs_writer_count(dvs_writer writer, int64_t *count);
 int dvs_writer_destroy(dvs_writer writer);
 
@@ -85,6 +87,24 @@ int dvs_io_frames_binary(const char *dir, dvs_frame *out, int64_t capacity, int6
  * matrix of the quaternion as given (not normalised); timestamps NaN. */
 int dvs_io_frames_text(const char *path, dvs_frame *out, int64_t capacity, int64_t *count);
 
+/* ---- frames.h5 (Hdf5FrameWriter, include/dvs/frame_writer.hpp:12-77) ----
+ * Datasets t (int64 microseconds, llround(time * 1e6)), images (uint8
+ * N x H x W, render_float_to_png_bytes of the radiance) and depth (float32
+ * N x H x W), extendable, one frame per chunk, byte shuffle + Deflate 1 on
+ * images and depth.  The file must not exist.  append checks strictly
+ * increasing finite times and finite radiance and depth (depth >= 0 or -1):
+ * DVS_E_ARGUMENT with the reference's messages. */
+typedef struct dvs_frames_s *dvs_frames;
+int dvs_frames_open(const char *path, int32_t width, int32_t height, dvs_frames *out);
+int dvs_frames_append(dvs_frames frames, double time, const float *radiance, const float *depth);
+int dvs_frames_close(dvs_frames frames);
+int dvs_frames_count(dvs_frames frames, int64_t *count);
+int dvs_frames_destroy(dvs_frames frames);
+/* Reads a frames.h5 back: the frame count and size always; with
+ * capacity >= count and all buffers given, the three datasets too. */
+int dvs_frames_h5_read(const char *path, int64_t *times, uint8_t *images, float *depth, int64_t capacity,
+                       int32_t *width, int32_t *height, int64_t *count);
+
 /* ---- Raster input (include/tools/loader.hpp, without GDAL) -------------
  * tools::loadImageryData (loader.hpp:31-100) over a minimal GeoTIFF reader:
  * band 1 as float32, NoData and non-finite samples as NaN, the rest through
@@ -101,6 +121,7 @@ typedef struct {
   int32_t has_nodata, has_scale, has_offset;
   double nodata, scale, offset;
   int64_t missing;  /* NoData samples (NaN) */
+  float center_raw;  /* the stored sample at (width/2, height/2), before NoData / scale / offset */
 } dvs_raster_info;
 int dvs_io_raster_load(const char *path, dvs_raster *out);
 int dvs_io_raster_info(dvs_raster raster, dvs_raster_info *out);
@@ -110,6 +131,9 @@ int dvs_io_raster_data(dvs_raster raster, float *data, float *x_coords, float *y
 /* Copies up to capacity bytes of the projection key (NUL terminated when
  * room allows); *length receives its length. */
 int dvs_io_raster_projection(dvs_raster raster, char *out, int64_t capacity, int64_t *length);
+/* The value of GeoKey `key` (its SHORT value or its GeoDoubleParams
+ * entry); DVS_E_ARGUMENT when the raster has none. */
+int dvs_io_raster_geokey(dvs_raster raster, int32_t key, double *value);
 int dvs_io_raster_free(dvs_raster raster);
 
 /* ---- Frame images of bin/cam (apps/main_cam.cpp:79-96) ----------------- */

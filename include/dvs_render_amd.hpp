@@ -14,13 +14,21 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <cerrno>
+#include <filesystem>
+#include <fstream>
 #include <iostream>
+#include <sstream>
 #include <limits>
 #include <memory>
 #include <stdexcept>
 #include <string>
 #include <utility>
 #include <vector>
+
+#include <fcntl.h>
+#include <sys/file.h>
+#include <unistd.h>
 
 #include "dvs_hip.h"
 #include "dvs_io.h"
@@ -937,18 +945,157 @@ class EventSensor {
 
 // EventStreamWriter / BinaryEventStreamWriter / Hdf5EventStreamWriter
 // (include/dvs/dvs_writer.hpp) and make_event_writer (apps/main_dvs.cpp:23-34).
+// DatasetSizeLimit / DatasetBudget (include/dvs/dataset_budget.hpp:18-114):
+// every write to a guarded dataset root is checked, under a cross-process
+// lock, against a stop level and a hard ceiling on the apparent bytes below
+// the root, with the caller's bound on the growth.
+class DatasetSizeLimit : public std::runtime_error {
+ public:
+  using std::runtime_error::runtime_error;
+};
+
+class DatasetBudget {
+ public:
+  explicit DatasetBudget(const std::filesystem::path &root, uint64_t hard = 500000000000ULL,
+                         uint64_t stop = 480000000000ULL)
+      : root_(std::filesystem::canonical(root)), hard_(hard), stop_(stop) {
+    if (stop_ == 0 || stop_ > hard_ || hard_ > 500000000000ULL)
+      throw std::invalid_argument("invalid dataset capacity limits");
+    uint64_t key = 14695981039346656037ULL;  // FNV-1a of the root, names the lock outside it
+    for (unsigned char c : root_.string()) {
+      key ^= c;
+      key *= 1099511628211ULL;
+    }
+    std::ostringstream name;
+    name << "dvs-dataset-" << std::hex << key << ".lock";
+    lock_path_ = std::filesystem::temp_directory_path() / name.str();
+  }
+  void require_path(const std::filesystem::path &p) const {
+    const auto relative = std::filesystem::weakly_canonical(p).lexically_relative(root_);
+    if (relative.empty() || relative.is_absolute() || *relative.begin() == "..")
+      throw std::runtime_error("output is outside the guarded dataset: " + p.string());
+  }
+  uint64_t logical_bytes() const {
+    uint64_t total = 0;
+    for (const auto &e : std::filesystem::recursive_directory_iterator(root_)) {
+      if (e.is_symlink()) throw std::runtime_error("dataset symlinks are not supported by capacity accounting");
+      if (e.is_regular_file()) {
+        const uint64_t n = e.file_size();
+        if (n > std::numeric_limits<uint64_t>::max() - total) throw DatasetSizeLimit("dataset size overflow");
+        total += n;
+      }
+    }
+    return total;
+  }
+  const std::filesystem::path &root() const { return root_; }
+  uint64_t hard_limit() const { return hard_; }
+  uint64_t stop_limit() const { return stop_; }
+  // The operation must flush what it writes; max_growth bounds the growth.
+  template <class F> void write(uint64_t max_growth, F operation) const {
+    Lock lock(lock_path_);
+    const uint64_t before = logical_bytes();
+    if (before > hard_ || before >= stop_ || max_growth > stop_ - before)
+      throw DatasetSizeLimit("dataset capacity stop: current=" + std::to_string(before) +
+                             ", reserved_write=" + std::to_string(max_growth) + ", stop=" + std::to_string(stop_) +
+                             ", hard=" + std::to_string(hard_) + " bytes");
+    const uint64_t available = std::filesystem::space(root_).available;
+    if (max_growth > available || available - max_growth < 128ULL * 1024 * 1024)
+      throw DatasetSizeLimit("insufficient filesystem space for reserved dataset write");
+    operation();
+    const uint64_t after = logical_bytes();
+    if (after > before + max_growth)
+      throw DatasetSizeLimit("writer exceeded its declared storage reservation; batch must stop");
+    if (after > hard_) throw DatasetSizeLimit("dataset hard size limit reached; batch must stop");
+  }
+  // Metadata through a temporary sibling, renamed over the old file.
+  void write_text(const std::filesystem::path &p, const std::string &text) const {
+    require_path(p);
+    write(text.size(), [&] {
+      auto partial = p;
+      partial += ".partial";
+      std::ofstream out(partial, std::ios::binary | std::ios::trunc);
+      if (!out) throw std::runtime_error("cannot open dataset metadata: " + partial.string());
+      out.write(text.data(), (std::streamsize)text.size());
+      out.flush();
+      if (!out) throw std::runtime_error("cannot write dataset metadata: " + partial.string());
+      out.close();
+      std::filesystem::rename(partial, p);
+    });
+  }
+
+ private:
+  class Lock {
+   public:
+    explicit Lock(const std::filesystem::path &path) {
+      fd_ = ::open(path.c_str(), O_CREAT | O_RDWR | O_CLOEXEC, 0600);
+      if (fd_ < 0) throw std::runtime_error("cannot open dataset capacity lock: " + std::string(std::strerror(errno)));
+      int result;
+      do {
+        result = flock(fd_, LOCK_EX);
+      } while (result < 0 && errno == EINTR);
+      if (result < 0) {
+        ::close(fd_);
+        fd_ = -1;
+        throw std::runtime_error("cannot lock dataset capacity budget");
+      }
+    }
+    ~Lock() {
+      if (fd_ >= 0) {
+        flock(fd_, LOCK_UN);
+        ::close(fd_);
+      }
+    }
+    Lock(const Lock &) = delete;
+    Lock &operator=(const Lock &) = delete;
+
+   private:
+    int fd_ = -1;
+  };
+  std::filesystem::path root_, lock_path_;
+  uint64_t hard_, stop_;
+};
+
+// EventStreamWriter and its events.bin / events.h5 forms
+// (include/dvs/dvs_writer.hpp:13-264); with a budget every file operation
+// is a budgeted write with the reference's reservations.
 class EventStreamWriter {
  public:
-  EventStreamWriter(std::string output_path, int32_t format) : path_(std::move(output_path)) {
-    detail::check_io(dvs_writer_open(path_.c_str(), format, &writer_));
+  EventStreamWriter(std::string output_path, int32_t format, const DatasetBudget *budget = nullptr)
+      : path_(std::move(output_path)), format_(format), budget_(budget) {
+    if (budget_) budget_->require_path(path_);
+    auto open = [&] { detail::check_io(dvs_writer_open(path_.c_str(), format, &writer_)); };
+    if (budget_) budget_->write(format == DVS_EVENTS_HDF5 ? 4 * 1024 * 1024 : 0, open);
+    else open();
   }
   virtual ~EventStreamWriter() { dvs_writer_destroy(writer_); }
   EventStreamWriter(const EventStreamWriter &) = delete;
   EventStreamWriter &operator=(const EventStreamWriter &) = delete;
   void append(std::vector<Event> events) {
-    detail::check_io(dvs_writer_append(writer_, events.data(), (int64_t)events.size()));
+    if (events.empty()) return;
+    auto write = [&] {
+      detail::check_io(dvs_writer_append(writer_, events.data(), (int64_t)events.size()));
+      if (budget_) detail::check_io(dvs_writer_flush(writer_));
+    };
+    uint64_t reservation = events.size() * 13;  // BinaryRecord
+    if (format_ == DVS_EVENTS_HDF5) {
+      const uint64_t chunk = 1 << 16;
+      reservation = ((events.size() + chunk - 1) / chunk + 1) * chunk * 13 + 4 * 1024 * 1024;
+    }
+    if (budget_) budget_->write(reservation, write);
+    else write();
+    for (const Event &e : events) last_time_ = std::max(last_time_, e.timestamp);
   }
-  void close(double end_time_seconds) { detail::check_io(dvs_writer_close(writer_, end_time_seconds)); }
+  void close(double end_time_seconds) {
+    auto finish = [&] { detail::check_io(dvs_writer_close(writer_, end_time_seconds)); };
+    uint64_t reservation = 0;
+    if (format_ == DVS_EVENTS_HDF5) {
+      const double end_ms = std::max(end_time_seconds, last_time_) * 1000.0;
+      const uint64_t index_count = (uint64_t)std::max(0.0, std::ceil(end_ms) + 2.0);
+      reservation = index_count * sizeof(int64_t) + 4 * 1024 * 1024;
+    }
+    if (budget_) budget_->write(reservation, finish);
+    else finish();
+  }
   const std::string &path() const { return path_; }
   std::size_t event_count() const {
     int64_t n = 0;
@@ -958,17 +1105,20 @@ class EventStreamWriter {
 
  private:
   std::string path_;
+  int32_t format_;
+  const DatasetBudget *budget_;
   dvs_writer writer_ = nullptr;
+  double last_time_ = 0;
 };
 class BinaryEventStreamWriter final : public EventStreamWriter {
  public:
-  explicit BinaryEventStreamWriter(std::string output_path)
-      : EventStreamWriter(std::move(output_path), DVS_EVENTS_BIN) {}
+  explicit BinaryEventStreamWriter(std::string output_path, const DatasetBudget *budget = nullptr)
+      : EventStreamWriter(std::move(output_path), DVS_EVENTS_BIN, budget) {}
 };
 class Hdf5EventStreamWriter final : public EventStreamWriter {
  public:
-  explicit Hdf5EventStreamWriter(std::string output_path)
-      : EventStreamWriter(std::move(output_path), DVS_EVENTS_HDF5) {}
+  explicit Hdf5EventStreamWriter(std::string output_path, const DatasetBudget *budget = nullptr)
+      : EventStreamWriter(std::move(output_path), DVS_EVENTS_HDF5, budget) {}
 };
 inline std::unique_ptr<EventStreamWriter> make_event_writer(const std::string &event_format,
                                                             const std::string &output_dir) {
@@ -976,6 +1126,86 @@ inline std::unique_ptr<EventStreamWriter> make_event_writer(const std::string &e
   if (event_format == "bin") return std::make_unique<BinaryEventStreamWriter>(output_dir + "/events.bin");
   throw std::runtime_error("unsupported event format: " + event_format);
 }
+
+// Hdf5FrameWriter (include/dvs/frame_writer.hpp:12-77): frames.h5 with t,
+// images (the sRGB preview bytes) and depth, every write budgeted.
+class Hdf5FrameWriter {
+ public:
+  Hdf5FrameWriter(const std::filesystem::path &path, int width, int height, const DatasetBudget &budget)
+      : width_(width), height_(height), budget_(budget) {
+    if (width <= 0 || height <= 0) throw std::invalid_argument("invalid frame dimensions");
+    budget.require_path(path);
+    budget.write(4 * 1024 * 1024, [&] { detail::check_io(dvs_frames_open(path.c_str(), width, height, &frames_)); });
+  }
+  ~Hdf5FrameWriter() { dvs_frames_destroy(frames_); }
+  Hdf5FrameWriter(const Hdf5FrameWriter &) = delete;
+  Hdf5FrameWriter &operator=(const Hdf5FrameWriter &) = delete;
+  void append(double time, const std::vector<float> &radiance, const std::vector<float> &depth) {
+    const size_t pixels = (size_t)width_ * height_;
+    if (radiance.size() != pixels || depth.size() != pixels)
+      throw std::invalid_argument("invalid frame timestamp or array size");
+    budget_.write(pixels * 5 + 4 * 1024 * 1024,
+                  [&] { detail::check_io(dvs_frames_append(frames_, time, radiance.data(), depth.data())); });
+  }
+  void close() {
+    budget_.write(4 * 1024 * 1024, [&] { detail::check_io(dvs_frames_close(frames_)); });
+  }
+  std::size_t count() const {
+    int64_t n = 0;
+    detail::check_io(dvs_frames_count(frames_, &n));
+    return (std::size_t)n;
+  }
+
+ private:
+  int width_, height_;
+  const DatasetBudget &budget_;
+  dvs_frames frames_ = nullptr;
+};
+
+// RepeatedTrajectory (include/dvs/repeated_trajectory.hpp:8-35): the source
+// path at its own time scale, back and forth, with C2 reversals of
+// turn_seconds at each end (phase = turn * (3x^2 - 3x^3 + x^4), rate 6x -
+// 9x^2 + 4x^3); the state's rates are scaled by the phase rate.
+class RepeatedTrajectory {
+ public:
+  explicit RepeatedTrajectory(const CameraTrajectory &source, double turn_seconds = .5)
+      : source_(source), duration_(source.end_time() - source.start_time()),
+        turn_(std::min(turn_seconds, duration_ / 4)) {
+    if (!std::isfinite(duration_) || !std::isfinite(turn_seconds) || duration_ <= 0 || turn_seconds <= 0)
+      throw std::invalid_argument("finite positive trajectory and turn duration required");
+  }
+  CameraTrajectoryState evaluate(double time) const {
+    if (!std::isfinite(time)) throw std::invalid_argument("finite trajectory query time required");
+    double r = std::fmod(time, 2 * duration_);
+    if (r < 0) r += 2 * duration_;
+    double phase = r <= duration_ ? r : 2 * duration_ - r;
+    double rate = r <= duration_ ? 1 : -1;
+    if (phase < turn_) {
+      const double x = phase / turn_;
+      phase = turn_ * (3 * x * x - 3 * x * x * x + x * x * x * x);
+      rate *= 6 * x - 9 * x * x + 4 * x * x * x;
+    } else if (duration_ - phase < turn_) {
+      const double x = (duration_ - phase) / turn_;
+      phase = duration_ - turn_ * (3 * x * x - 3 * x * x * x + x * x * x * x);
+      rate *= 6 * x - 9 * x * x + 4 * x * x * x;
+    }
+    CameraTrajectoryState state = source_.evaluate((float)(source_.start_time() + phase));
+    state.timestamp = (float)time;
+    const float f = (float)rate;  // a float vector scaled by a double: the scale is a float
+    for (int i = 0; i < 3; ++i) {
+      state.linear_velocity.v[i] *= f;
+      state.angular_velocity.v[i] *= f;
+      state.light_direction_rate.v[i] *= f;
+    }
+    state.light_intensity_rate = (float)(state.light_intensity_rate * rate);
+    return state;
+  }
+  double source_duration() const { return duration_; }
+
+ private:
+  const CameraTrajectory &source_;
+  double duration_, turn_;
+};
 
 }  // namespace dvs
 

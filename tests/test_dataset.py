@@ -1,0 +1,203 @@
+"""CPU: the dataset writer path (include/dvs/frame_writer.hpp,
+include/dvs/repeated_trajectory.hpp, apps/main_sequence.cpp:48-109).
+
+* frames.h5: t (int64 microseconds), images (the sRGB preview bytes of
+  render_float_to_png_bytes) and depth read back equal to what was written;
+  the layout h5dump reports (extendable N x H x W, one frame per chunk,
+  shuffle + Deflate 1, t in chunks of 4096); the file must be new; the
+  reference's checks on times, radiance and depth;
+* bin/sequence --prepare-poses (no GPU: the poses come from the host
+  trajectory): poses.csv / motion.csv / pose_geographic.csv /
+  camera_used.yaml at imu_fps over the duration, against the repeated
+  trajectory restated from repeated_trajectory.hpp:15-31 over the oracle's
+  CameraTrajectory -- and properties the repetition must have: the path
+  reversed about each end, C2 reversals (velocity through zero), an
+  equirectangular map's longitude / latitude.
+"""
+from __future__ import annotations
+
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import geotiff
+from dvsrender4deepspace_amd import build, eventio, workloads
+
+MOON_RADIUS = 1737400.0
+
+
+def test_frames_h5_round_trip(tmp_path):
+    rng = np.random.default_rng(1)
+    w, h = 7, 5
+    path = tmp_path / "frames.h5"
+    f = eventio.FrameWriter(path, w, h)
+    times = [0.0, 0.0333333333, 0.5]
+    rads = [rng.uniform(0, 300, (h, w)).astype(np.float32) for _ in times]
+    deps = [np.where(rng.random((h, w)) < 0.2, -1, rng.uniform(0, 900, (h, w))).astype(np.float32) for _ in times]
+    for t, r, d in zip(times, rads, deps):
+        f.append(t, r, d)
+    assert f.count() == 3
+    f.close()
+    back = eventio.read_frames_h5(path)
+    assert list(back["t"]) == [round(t * 1e6) for t in times]
+    assert np.array_equal(back["images"], np.array([eventio.frame_bytes(r) for r in rads]))
+    assert np.array_equal(back["depth"], np.array(deps))
+    dump = subprocess.run(["/opt/conda/bin/h5dump", "-p", "-H", str(path)], capture_output=True, text=True).stdout
+    assert "DATASET \"images\"" in dump and "H5T_STD_U8LE" in dump and "H5T_IEEE_F32LE" in dump
+    assert "( 3, 5, 7 ) / ( H5S_UNLIMITED, 5, 7 )" in dump and "CHUNKED ( 1, 5, 7 )" in dump
+    assert "SHUFFLE" in dump and "DEFLATE" in dump and "LEVEL 1" in dump and "CHUNKED ( 4096 )" in dump
+
+
+def test_frames_h5_checks(tmp_path):
+    w, h = 3, 2
+    ok = np.ones((h, w), np.float32)
+    f = eventio.FrameWriter(tmp_path / "f.h5", w, h)
+    f.append(0.1, ok, ok)
+    for t, r, d, msg in ((0.1, ok, ok, "invalid frame timestamp or array size"),
+                         (float("nan"), ok, ok, "invalid frame timestamp or array size"),
+                         (0.2, np.full((h, w), np.inf, np.float32), ok, "invalid radiance or depth value"),
+                         (0.2, ok, np.full((h, w), -0.5, np.float32), "invalid radiance or depth value")):
+        with pytest.raises(eventio.ConfigError, match=msg):
+            f.append(t, r, d)
+    f.close()
+    with pytest.raises(eventio.IoError, match="create frames file"):
+        eventio.FrameWriter(tmp_path / "f.h5", w, h)  # exists
+    with pytest.raises(eventio.ConfigError, match="invalid frame dimensions"):
+        eventio.FrameWriter(tmp_path / "g.h5", 0, h)
+
+
+def _sequence_dir(tmp_path, frames=40, imu_fps=50.0):
+    """A dataset root with one sequence: config, text trajectory (the small
+    plain flight at 100 fps), and an equirectangular lunar DEM."""
+    w = workloads.small_variant(workloads.PLAIN_SMALL, size=64, width=32, height=24)
+    root = tmp_path / "dataset"
+    seq = root / "seq0"
+    seq.mkdir(parents=True)
+    workloads.write_config(w, seq / "source_config.yaml", frame_count=frames, imu_fps=imu_fps, cam_fps=30.0)
+    workloads.write_text_trajectory(w, seq / "source_trajectory.txt", frames=frames)
+    dem = (np.add.outer(np.arange(33), np.arange(33)) * 0.1).astype(np.float32)
+    # equirectangular (CT 17) on a 1737.4 km sphere, standard parallel 30, origin lon 10 lat 0
+    geotiff.write(root / "dem.tif", dem, transform=(20000.0, 5.0, 0.0, 91000.0, 0.0, -5.0), scale=2.0, offset=-3.0,
+                  geokeys=(1024, 0, 1, 3075, 0, 17))
+    _add_equirectangular(root / "dem.tif")
+    return w, root, seq
+
+
+def _add_equirectangular(path):
+    """Rewrites the DEM's GeoKeys with GeoDoubleParams: semi-major axis,
+    standard parallel, natural origin longitude."""
+    data = eventio.load_raster(path)
+    keys = [(1024, 0, 1, 1), (1025, 0, 1, 1), (2057, 34736, 1, 0), (3075, 0, 1, 17), (3078, 34736, 1, 1),
+            (3080, 34736, 1, 2), (3081, 34736, 1, 3)]
+    doubles = [MOON_RADIUS, 30.0, 10.0, 0.0]
+    raw = (data.data - np.float32(-3.0)) / np.float32(2.0)
+    _write_with_keys(path, raw.astype(np.float32), data.transform, keys, doubles)
+
+
+def _write_with_keys(path, data, gt, keys, doubles):
+    import struct
+    h, w = data.shape
+    body = data.astype("<f4").tobytes()
+    key_dir = [1, 1, 0, len(keys)] + [v for k in keys for v in k]
+    tags = {256: (4, [w]), 257: (4, [h]), 258: (3, [32]), 259: (3, [1]), 262: (3, [1]), 273: (4, [8]),
+            277: (3, [1]), 278: (4, [h]), 279: (4, [len(body)]), 339: (3, [3]),
+            33550: (12, [gt[1], -gt[5], 0.0]), 33922: (12, [0.0, 0.0, 0.0, gt[0], gt[3], 0.0]),
+            34735: (3, key_dir), 34736: (12, doubles),
+            42112: (2, b'<GDALMetadata><Item name="OFFSET" sample="0" role="offset">-3</Item>'
+                       b'<Item name="SCALE" sample="0" role="scale">2</Item></GDALMetadata>\0')}
+    fmt = {3: "H", 4: "I", 12: "d"}
+    extra, entries = bytearray(), []
+    ifd_at = 8 + len(body)
+    extra_at = ifd_at + 2 + 12 * len(tags) + 4
+    for tag in sorted(tags):
+        typ, vals = tags[tag]
+        raw = vals if typ == 2 else struct.pack("<" + fmt[typ] * len(vals), *vals)
+        count = len(raw) if typ == 2 else len(vals)
+        if len(raw) <= 4:
+            field = raw.ljust(4, b"\0")
+        else:
+            field = struct.pack("<I", extra_at + len(extra))
+            extra += raw + (b"\0" if len(raw) % 2 else b"")
+        entries.append(struct.pack("<HHI", tag, typ, count) + field)
+    blob = b"II" + struct.pack("<HI", 42, ifd_at) + body + struct.pack("<H", len(tags)) + b"".join(entries)
+    blob += struct.pack("<I", 0) + bytes(extra)
+    path.write_bytes(blob)
+
+
+def _repeated_state(traj, t, start, duration, turn=0.5):
+    """repeated_trajectory.hpp:15-31: the phase of t on the back-and-forth
+    path, C2 reversals of `turn` seconds, rates scaled by the phase rate."""
+    turn = min(turn, duration / 4)
+    r = math.fmod(t, 2 * duration)
+    phase = r if r <= duration else 2 * duration - r
+    rate = 1.0 if r <= duration else -1.0
+    if phase < turn:
+        x = phase / turn
+        phase, rate = turn * (3 * x * x - 3 * x ** 3 + x ** 4), rate * (6 * x - 9 * x * x + 4 * x ** 3)
+    elif duration - phase < turn:
+        x = (duration - phase) / turn
+        phase, rate = duration - turn * (3 * x * x - 3 * x ** 3 + x ** 4), rate * (6 * x - 9 * x * x + 4 * x ** 3)
+    s = traj.evaluate(np.array([np.float32(start + phase)], np.float32))[0]
+    return s, rate
+
+
+def test_prepare_poses(tmp_path, oracle):
+    w, root, seq = _sequence_dir(tmp_path)
+    env = dict(os.environ, DVS_SEQUENCE_SECONDS="2.5")
+    out = subprocess.run([str(build.BINDIR / "sequence"), "--prepare-poses", str(root), str(seq), str(root / "dem.tif"),
+                          ""], capture_output=True, text=True, timeout=120, env=env)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "Prepared 2.5 s native-camera poses" in out.stdout
+    poses = np.loadtxt(seq / "poses.csv", delimiter=",", skiprows=1)
+    motion = np.loadtxt(seq / "motion.csv", delimiter=",", skiprows=1)
+    geo = np.loadtxt(seq / "pose_geographic.csv", delimiter=",", skiprows=1)
+    assert len(poses) == len(motion) == len(geo) == math.ceil(2.5 * 50) + 1
+    assert np.allclose(poses[:, 0], np.minimum(2.5, np.arange(len(poses)) / 50.0))
+    frames = eventio.load_frames_text(seq / "source_trajectory.txt")
+    traj = oracle.Trajectory(frames, 100.0)
+    start, end, _ = traj.range()
+    duration = float(np.float32(end) - np.float32(start))
+    for i in (0, 7, 20, 40, 55, 80, 98, 99, 100, 125):
+        t = poses[i, 0]
+        s, rate = _repeated_state(traj, t, start, duration)
+        assert np.allclose(poses[i, 1:4], s["position"], rtol=0, atol=1e-9 + 1e-9 * np.abs(s["position"]).max())
+        v = s["linear_velocity"].astype(np.float32) * np.float32(rate)
+        assert np.allclose(motion[i, 1:4], v, rtol=1e-9, atol=1e-9)
+        assert np.allclose(motion[i, 7:11], [*s["light_direction"], s["light_intensity"]], rtol=1e-9)
+        q = poses[i, 4:8]
+        assert abs(np.linalg.norm(q) - 1) < 1e-6
+    # reflection about the path's end: the state at duration + x is the state at duration - x
+    a = _repeated_state(traj, duration - 0.3, start, duration)[0]["position"]
+    b = _repeated_state(traj, duration + 0.3, start, duration)[0]["position"]
+    assert np.array_equal(a, b)
+    # the C2 reversal brings the velocity through zero at the turn
+    assert _repeated_state(traj, duration, start, duration)[1] == 0
+    # geographic: x, y of the DEM centre plus the pose; equirectangular inverse
+    cx, cy = 20000.0 + 16.5 * 5.0, 91000.0 - 16.5 * 5.0
+    assert np.allclose(geo[:, 1], cx + poses[:, 1], atol=1e-6) and np.allclose(geo[:, 2], cy + poses[:, 2], atol=1e-6)
+    z0 = float(np.float32(3.2)) * 2 - 3  # the centre's stored sample 3.2 through scale 2, offset -3
+    assert np.allclose(geo[:, 3], z0 + poses[:, 3], rtol=0, atol=1e-9)
+    lon = 10.0 + np.degrees(geo[:, 1] / (MOON_RADIUS * math.cos(math.radians(30.0))))
+    lat = np.degrees(geo[:, 2] / MOON_RADIUS)
+    assert np.allclose(geo[:, 4], lon, rtol=0, atol=1e-10) and np.allclose(geo[:, 5], lat, rtol=0, atol=1e-10)
+    cam = (seq / "camera_used.yaml").read_text()
+    assert "width: 32" in cam and "duration_seconds: 2.5" in cam and "pose_axes: opencv_x_right_y_down_z_forward" in cam
+    # an existing render is never overwritten
+    (seq / "frames.h5").write_bytes(b"")
+    out = subprocess.run([str(build.BINDIR / "sequence"), "--prepare-poses", str(root), str(seq),
+                          str(root / "dem.tif"), ""], capture_output=True, text=True, timeout=120, env=env)
+    assert out.returncode == 1 and "refusing to overwrite an existing rendered or partial sequence" in out.stderr
+
+
+def test_sequence_command_line(tmp_path):
+    out = subprocess.run([str(build.BINDIR / "sequence"), "a", "b"], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 1
+    assert "sequence [--prepare-poses|--optimized] dataset_root sequence_dir dem albedo_or_empty" in out.stderr
+    root = tmp_path / "d"
+    root.mkdir()
+    out = subprocess.run([str(build.BINDIR / "sequence"), str(root), str(tmp_path / "outside"), "dem.tif", ""],
+                         capture_output=True, text=True, timeout=60)
+    assert out.returncode == 1 and "output is outside the guarded dataset" in out.stderr
