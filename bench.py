@@ -205,7 +205,7 @@ def served_compare(args, requests: int, horizons: tuple) -> dict:
     log(f"served mix: meshes in {time.perf_counter() - t:.1f} s")
     adaptive = args.solver == "adaptive"
     reqs = serve.request_mix(base, requests, args.rate, horizons, specs, seed=args.seed, speed=tuple(args.speed),
-                             adaptive=adaptive)
+                             adaptive=adaptive, trajectories=True)
     geometries = {k: serve.build_scene(meshes[k], base, base.camera) for k in specs}
     best = _tune(serve, geometries, reqs, adaptive)
     served = [dataclasses.replace(r, events=[]) for r in reqs]
@@ -225,7 +225,9 @@ def served_compare(args, requests: int, horizons: tuple) -> dict:
         "config": {"workload": "served mix", "requests": len(reqs), "arrival_rate_hz": args.rate,
                    "horizons": list(horizons), "resolutions": serve.RESOLUTIONS,
                    "terrains": {k: v.triangles for k, v in specs.items()}, "admission_budget_gb": args.budget_gb,
-                   "supersample": base.camera.samples, "solver": args.solver, "autotuned": best},
+                   "supersample": base.camera.samples, "solver": args.solver, "autotuned": best,
+                   "feed": "horizon times of each request's CameraTrajectory; the engine evaluates the states "
+                           "on the device in each step"},
         "events": events,
         "served": {"wall_s": round(wall_served, 3), "events_per_s": round(events / wall_served, 1),
                    "latency_ms": _latency(served), "steps": server.steps, "peak_active": server.peak_active},
@@ -257,7 +259,7 @@ def batched_mode(args) -> dict:
     meshes = serve.terrain_meshes(specs)
     adaptive = args.solver == "adaptive"
     reqs = serve.request_mix(base, args.segments, None, (args.min_horizons, args.max_horizons), specs,
-                             seed=args.seed, speed=tuple(args.speed), adaptive=adaptive)
+                             seed=args.seed, speed=tuple(args.speed), adaptive=adaptive, trajectories=True)
     geometries = {"terrain1024": serve.build_scene(meshes["terrain1024"], base, base.camera)}
     best = _tune(serve, geometries, reqs, adaptive)
     batched = [dataclasses.replace(r, events=[]) for r in reqs]

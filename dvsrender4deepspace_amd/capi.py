@@ -241,6 +241,7 @@ def load_library() -> ctypes.CDLL:
         "dvs_batch_add": (ctypes.c_int, [_P, _P, ctypes.POINTER(_PhysicC), _I32, _I32, _P, ctypes.c_double,
                                          ctypes.POINTER(_I32)]),
         "dvs_batch_feed": (ctypes.c_int, [_P, _I32, _P, _I64]),
+        "dvs_batch_feed_trajectory": (ctypes.c_int, [_P, _I32, _P, _P, _I64]),
         "dvs_batch_step": (ctypes.c_int, [_P, ctypes.POINTER(_I32)]),
         "dvs_batch_take": (ctypes.c_int, [_P, _I32, _P, _I64, ctypes.POINTER(_I64)]),
         "dvs_batch_drain": (ctypes.c_int, [_P, _I32, ctypes.POINTER(_I64)]),
@@ -657,6 +658,7 @@ class Batch:
         self.handle = handle
         self.horizons_per_step = horizons_per_step
         self._scenes = {}
+        self._trajectories = {}
 
     def close(self):
         if getattr(self, "handle", None):
@@ -678,6 +680,16 @@ class Batch:
     def feed(self, segment: int, intervals: np.ndarray) -> None:
         iv = np.ascontiguousarray(np.atleast_1d(intervals), dtype=INTERVAL_DTYPE)
         _check(load_library().dvs_batch_feed(self.handle, int(segment), _ptr(iv), len(iv)))
+
+    def feed_trajectory(self, segment: int, trajectory: "Trajectory", times) -> None:
+        """Queues the intervals between consecutive `times`, their states
+        evaluated from `trajectory` on the device when their step runs (the
+        states Trajectory.intervals gives).  The batch keeps the trajectory
+        alive while the segment exists."""
+        t = np.ascontiguousarray(times, dtype=np.float64)
+        _check(load_library().dvs_batch_feed_trajectory(self.handle, int(segment), trajectory.handle, _ptr(t),
+                                                        len(t)))
+        self._trajectories.setdefault(int(segment), []).append(trajectory)
 
     def step(self) -> int:
         pending = _I32()
@@ -707,6 +719,7 @@ class Batch:
     def remove(self, segment: int) -> None:
         _check(load_library().dvs_batch_remove(self.handle, int(segment)))
         self._scenes.pop(segment, None)
+        self._trajectories.pop(segment, None)
 
     def set_adaptive(self, segment: int, tolerance: float = 1e-2, horizons: int = 200,
                      check_period: float = 0.25) -> None:

@@ -31,6 +31,7 @@
 #include "readout.h"
 #include "scan_sort.h"
 #include "scene.h"
+#include "trajectory_device.h"
 
 namespace dvsamd {
 namespace {
@@ -186,6 +187,13 @@ struct Segment {
   DeviceArray<dvs_pixel_state> states;
   DeviceArray<CrossingEvent> slots;
   std::deque<dvs_optical_interval> queue;
+  // per queued interval: the trajectory whose device evaluation supplies its
+  // states when its step runs, or null (the states came with the interval)
+  std::deque<dvs_trajectory> source;
+  void clear_queue() {
+    queue.clear();
+    source.clear();
+  }
   DeviceFifo fifo;
   std::vector<dvs_event> ready, raw;
   int64_t horizons_done = 0, events_total = 0;
@@ -218,6 +226,7 @@ struct Segment {
 struct StepSegment {
   Segment *seg;
   std::vector<dvs_optical_interval> ivs;
+  std::vector<dvs_trajectory> src;  // Segment::source of each interval
   int cells = 0, cell_offset = 0, iv_offset = 0, pixel_offset = 0;
   double tolerance = 0;  // adaptive windows only
   bool check = false;
@@ -324,6 +333,29 @@ class Engine {
       previous = iv.t1;
     }
     s.queue.insert(s.queue.end(), ivs, ivs + n);
+    s.source.insert(s.source.end(), (size_t)n, nullptr);
+    s.fed_until = previous;
+  }
+
+  // Queues the n - 1 intervals between consecutive times, their states to be
+  // evaluated from `trajectory` on the device when their step runs.
+  void feed_trajectory(int id, dvs_trajectory trajectory, const double *times, int64_t n) {
+    Segment &s = segment(id);
+    if (!s.initialized || n < 2 || !times || !trajectory) fail(DVS_E_ARGUMENT, "sensor not initialized or no intervals");
+    double previous = s.fed_until;
+    if (times[0] != previous) fail(DVS_E_ARGUMENT, "sensor intervals must be positive and contiguous");
+    std::vector<dvs_optical_interval> ivs((size_t)n - 1);
+    for (int64_t k = 0; k + 1 < n; ++k) {
+      if (!std::isfinite(times[k + 1]) || times[k + 1] <= times[k])
+        fail(DVS_E_ARGUMENT, "sensor intervals must be positive and contiguous");
+      dvs_optical_interval &iv = ivs[(size_t)k];
+      iv = dvs_optical_interval{};
+      iv.t0 = times[k];
+      iv.t1 = times[k + 1];
+      previous = iv.t1;
+    }
+    s.queue.insert(s.queue.end(), ivs.begin(), ivs.end());
+    s.source.insert(s.source.end(), ivs.size(), trajectory);
     s.fed_until = previous;
   }
 
@@ -351,6 +383,7 @@ class Engine {
       w.seg = &s;
       const size_t take = std::min<size_t>(s.queue.size(), (size_t)(s.adaptive ? s.abatch : horizons_));
       w.ivs.assign(s.queue.begin(), s.queue.begin() + take);
+      w.src.assign(s.source.begin(), s.source.begin() + take);
       if (s.adaptive) {
         w.tolerance = s.tolerance;
         w.check = s.check_period >= 0 && w.ivs.front().t0 >= s.next_check;
@@ -365,6 +398,7 @@ class Engine {
     }
     auto consume = [](StepSegment &w) {
       w.seg->queue.erase(w.seg->queue.begin(), w.seg->queue.begin() + w.ivs.size());
+      w.seg->source.erase(w.seg->source.begin(), w.seg->source.begin() + w.ivs.size());
       w.seg->horizons_done += (int64_t)w.ivs.size();
     };
     if (!native.work.empty()) {
@@ -387,6 +421,7 @@ class Engine {
           StepSegment again;
           again.seg = &s;
           again.ivs = w.ivs;
+          again.src = w.src;
           again.check = w.check;
           retry.add(std::move(again));
         }
@@ -516,6 +551,7 @@ class Engine {
     StepSegment w;
     w.seg = &s;
     w.ivs.assign(s.queue.begin(), s.queue.end());
+    w.src.assign(s.source.begin(), s.source.end());
     w.tolerance = tolerance;
     w.check = check;
     check_adaptive_window(w);
@@ -526,7 +562,7 @@ class Engine {
     std::vector<char> accepted;
     run_adaptive(work, s.npix, cells, (int)work[0].ivs.size(), accepted);
     if (accepted[0]) {
-      s.queue.clear();
+      s.clear_queue();
       s.horizons_done += (int64_t)work[0].ivs.size();
     }
     return accepted[0] != 0;
@@ -612,6 +648,21 @@ class Engine {
     cellbeg_.upload(cell_begin.data(), cell_begin.size(), "segment cells", st, staging_);
     charge_.upload(charge.data(), charge.size(), "segment charge", st, staging_);
     staging_.flush(st);
+    // intervals fed as trajectory times: their states evaluated on the device
+    // over the staged copies, which carry only the times
+    std::vector<double> times;
+    for (auto &w : work) {
+      for (size_t k = 0; k < w.src.size();) {
+        size_t e = k + 1;
+        while (e < w.src.size() && w.src[e] == w.src[k]) ++e;
+        if (w.src[k]) {
+          times.assign(1, w.ivs[k].t0);
+          for (size_t j = k; j < e; ++j) times.push_back(w.ivs[j].t1);
+          trajectory_intervals_on(w.src[k], times.data(), (int64_t)times.size(), intervals_.ptr + w.iv_offset + k, st);
+        }
+        k = e;
+      }
+    }
     maps_.reserve(cell_total, "cell maps");
     hip_check(launch_cell_maps(cell_total, cell_t0_.ptr, cell_t1_.ptr, cellbeg_.ptr, nseg, charge_.ptr, maps_.ptr, st),
               "cell maps");
@@ -1685,13 +1736,13 @@ int dvs_sensor_process(dvs_sensor s, const dvs_optical_interval *ivs, int32_t n,
   return guarded([&] {
     if (!s || !ivs) fail(DVS_E_ARGUMENT, "sensor not initialized or no intervals");
     Segment &seg = s->engine->segment(s->id);
-    seg.queue.clear();
+    seg.clear_queue();
     seg.fed_until = seg.last_time;
     s->engine->feed(s->id, ivs, n);
     try {
       s->engine->step(s->id);
     } catch (...) {
-      seg.queue.clear();
+      seg.clear_queue();
       seg.fed_until = seg.last_time;
       throw;
     }
@@ -1823,18 +1874,18 @@ int dvs_sensor_process_adaptive(dvs_sensor s, const dvs_optical_interval *ivs, i
     if (accepted) *accepted = 0;
     s->engine->validate_adaptive(s->id, ivs, n, tolerance);
     Segment &seg = s->engine->segment(s->id);
-    seg.queue.clear();
+    seg.clear_queue();
     seg.fed_until = seg.last_time;
     bool ok = false;
     try {
       s->engine->feed(s->id, ivs, n);
       ok = s->engine->step_adaptive_once(s->id, tolerance, check_convergence != 0);
     } catch (...) {
-      seg.queue.clear();
+      seg.clear_queue();
       seg.fed_until = seg.last_time;
       throw;
     }
-    seg.queue.clear();
+    seg.clear_queue();
     seg.fed_until = seg.last_time;
     if (accepted) *accepted = ok ? 1 : 0;
     if (ready_count) *ready_count = (int64_t)seg.ready.size();
@@ -1918,6 +1969,14 @@ int dvs_batch_feed(dvs_batch b, int32_t segment, const dvs_optical_interval *ivs
   return guarded([&] {
     if (!b) fail(DVS_E_ARGUMENT, "null batch");
     b->engine->feed(segment, ivs, n);
+  });
+}
+
+int dvs_batch_feed_trajectory(dvs_batch b, int32_t segment, dvs_trajectory trajectory, const double *times,
+                              int64_t n) {
+  return guarded([&] {
+    if (!b) fail(DVS_E_ARGUMENT, "null batch");
+    b->engine->feed_trajectory(segment, trajectory, times, n);
   });
 }
 

@@ -4,6 +4,7 @@
 // shared curve math of trajectory.hpp.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -12,6 +13,7 @@
 #include "common.h"
 #include "dvs_hip.h"
 #include "trajectory.hpp"
+#include "trajectory_device.h"
 
 namespace dvstraj {
 
@@ -275,7 +277,46 @@ Trajectory &get(dvs_trajectory t) {
   if (!t) throw std::invalid_argument("null trajectory handle");
   return *reinterpret_cast<Trajectory *>(t);
 }
+// The intervals of n times (2 <= n <= INT32_MAX, tr.count > 0) into device
+// memory `out`, by launches on st; the keys are uploaded on st at first use.
+void launch_intervals(Trajectory &tr, const double *times, int64_t n, dvs_optical_interval *out, hipStream_t st) {
+  if (!tr.device_keys.ptr) {
+    tr.device_keys.upload(tr.keys.data(), tr.keys.size(), "trajectory keys", st);
+    tr.device_quats.upload(tr.quats.data(), tr.quats.size(), "trajectory keys", st);
+    // other streams may use the keys next: the copies are done before return
+    hip_check(hipStreamSynchronize(st), "trajectory keys");
+  }
+  const dvstraj::View v{tr.device_keys.ptr, tr.device_quats.ptr, tr.count};
+  for (int64_t base = 0; base < n; base += dvstraj::kArgTimes) {
+    dvstraj::TimesArg a;
+    a.base = (int)base;
+    a.count = (int)std::min<int64_t>(dvstraj::kArgTimes, n - base);
+    a.n = (int)n;
+    std::memcpy(a.t, times + base, sizeof(double) * (size_t)a.count);
+    dvstraj::intervals_kernel<<<(unsigned)((a.count + 63) / 64), 64, 0, st>>>(v, a, out);
+    hip_check(hipGetLastError(), "trajectory launch");
+  }
+}
 }  // namespace
+
+void trajectory_intervals_on(dvs_trajectory trajectory, const double *times, int64_t n, dvs_optical_interval *out,
+                             hipStream_t stream) {
+  Trajectory &tr = get(trajectory);
+  if (n < 2 || !times || !out) throw std::invalid_argument("invalid trajectory arguments");
+  if (n > (int64_t)INT32_MAX) throw std::invalid_argument("too many trajectory times");
+  if (tr.count == 0) {
+    std::vector<dvs_optical_interval> host((size_t)n - 1);
+    for (int64_t i = 0; i + 1 < n; ++i)
+      host[(size_t)i] = dvs_optical_interval{optical_of(default_state((float)times[i])),
+                                             optical_of(default_state((float)times[i + 1])), times[i], times[i + 1]};
+    hip_check(hipMemcpyAsync(out, host.data(), sizeof(dvs_optical_interval) * host.size(), hipMemcpyHostToDevice,
+                             stream),
+              "trajectory intervals");
+    hip_check(hipStreamSynchronize(stream), "trajectory intervals");  // `host` is released on return
+    return;
+  }
+  launch_intervals(tr, times, n, out, stream);
+}
 }  // namespace dvsamd
 
 using namespace dvsamd;
@@ -352,25 +393,12 @@ int dvs_trajectory_intervals(dvs_trajectory trajectory, const double *times, int
     }
     if (!tr.stream) hip_check(hipStreamCreateWithFlags(&tr.stream, hipStreamNonBlocking), "trajectory stream");
     hipStream_t st = tr.stream;
-    if (!tr.device_keys.ptr) {
-      tr.device_keys.upload(tr.keys.data(), tr.keys.size(), "trajectory keys", st);
-      tr.device_quats.upload(tr.quats.data(), tr.quats.size(), "trajectory keys", st);
-    }
     // times in through the launches' arguments, intervals out through a
     // page-locked buffer
     stream_wait(st, "trajectory times");  // the previous call's copies are done
     tr.host_intervals.reserve((size_t)n - 1, "trajectory intervals");
     tr.device_intervals.reserve((size_t)n - 1, "trajectory intervals");
-    const dvstraj::View v{tr.device_keys.ptr, tr.device_quats.ptr, tr.count};
-    for (int64_t base = 0; base < n; base += dvstraj::kArgTimes) {
-      dvstraj::TimesArg a;
-      a.base = (int)base;
-      a.count = (int)std::min<int64_t>(dvstraj::kArgTimes, n - base);
-      a.n = (int)n;
-      std::memcpy(a.t, times + base, sizeof(double) * (size_t)a.count);
-      dvstraj::intervals_kernel<<<(unsigned)((a.count + 63) / 64), 64, 0, st>>>(v, a, tr.device_intervals.ptr);
-      hip_check(hipGetLastError(), "trajectory launch");
-    }
+    launch_intervals(tr, times, n, tr.device_intervals.ptr, st);
     hip_check(hipMemcpyAsync(tr.host_intervals.ptr, tr.device_intervals.ptr,
                              sizeof(dvs_optical_interval) * (size_t)(n - 1), hipMemcpyDeviceToHost, st),
               "trajectory intervals");

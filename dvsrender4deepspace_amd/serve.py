@@ -47,6 +47,9 @@ class Request:
     horizons: int = 0
     frames: int = 0
     adaptive: bool = False
+    # CameraTrajectory of the flight: event jobs with one are fed as horizon
+    # times whose states the engine evaluates on the device
+    trajectory: object = None
     # filled by the runners
     events: list = dataclasses.field(default_factory=list)
     images: object = None
@@ -59,6 +62,12 @@ class Request:
 
     def event_array(self) -> np.ndarray:
         return np.concatenate(self.events) if self.events else np.zeros(0, capi.EVENT_DTYPE)
+
+
+def horizon_grid(w: Workload, horizons: int) -> np.ndarray:
+    """The request's horizon times from its start (workloads.horizon_grid)."""
+    from .workloads import horizon_grid as grid
+    return grid(w, w.start_time, 0, horizons)
 
 
 def footprint_bytes(req: Request, horizons_per_step: int) -> int:
@@ -119,11 +128,19 @@ class Server:
                 done.append(req)
                 continue
             w = req.workload
-            seg = self.batch.add(view, w.physic, w.initial_state(), w.start_time)
+            if req.trajectory is not None:
+                times = horizon_grid(w, req.horizons)
+                first = req.trajectory.intervals(times[:2])["start"][:1]
+                seg = self.batch.add(view, w.physic, first, w.start_time)
+            else:
+                seg = self.batch.add(view, w.physic, w.initial_state(), w.start_time)
             if req.adaptive:
                 self.batch.set_adaptive(seg, driver.DEFAULT_TOLERANCE,
                                         driver.batch_size(w.physic, w.time_resolution, True), driver.CHECK_PERIOD)
-            self.batch.feed(seg, w.intervals(0, req.horizons))
+            if req.trajectory is not None:
+                self.batch.feed_trajectory(seg, req.trajectory, times)
+            else:
+                self.batch.feed(seg, w.intervals(0, req.horizons))
             self.active[seg] = (req, need)
             self.used += need
         self.peak_active = max(self.peak_active, len(self.active))
@@ -198,10 +215,14 @@ def run_plain(requests: list, meshes: dict, realtime: bool = True, scenes_reside
             times = np.linspace(w.start_time, w.end_time, req.frames)
             req.images = scene.render(w.states(times))
         else:
+            if req.trajectory is not None:
+                iv = req.trajectory.intervals(horizon_grid(w, req.horizons))
+            else:
+                iv = w.intervals(0, req.horizons)
             sensor = capi.Sensor(scene, w.physic)
-            sensor.initialize(w.initial_state(), w.start_time)
+            sensor.initialize(iv["start"][:1], w.start_time)
             batch = driver.batch_size(w.physic, w.time_resolution, req.adaptive)
-            req.events.extend(driver.drive(sensor, w.intervals(0, req.horizons), adaptive=req.adaptive, batch=batch))
+            req.events.extend(driver.drive(sensor, iv, adaptive=req.adaptive, batch=batch))
             sensor.close()
         scene.close()
         req.finished = clock()
@@ -237,12 +258,14 @@ RESOLUTIONS = ((64, 48), (173, 130), (346, 260))
 
 def request_mix(base: Workload, count: int, rate_hz: float | None, horizons: tuple, terrains: dict, seed: int,
                 noise_fraction: float = 0.5, render_fraction: float = 0.0, speed: tuple = (20.0, 80.0),
-                adaptive: bool = False) -> list:
+                adaptive: bool = False, trajectories: bool = False) -> list:
     """Seeded request mix: resolution from RESOLUTIONS (fx scaled with width),
     length U(horizons), terrain uniform over `terrains`, per-request seed and
     noise flag, ground speed U(speed) and yaw rate U(-5, 5) deg/s seeded;
     Poisson arrivals at rate_hz (all at t=0 when None); event jobs use the
-    adaptive solver when `adaptive`."""
+    adaptive solver when `adaptive`, and with `trajectories` fly a
+    CameraTrajectory through the flight's two end frames, evaluated on the
+    device (served: inside the engine's steps)."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
@@ -271,8 +294,12 @@ def request_mix(base: Workload, count: int, rate_hz: float | None, horizons: tup
                                 frames=n_f if render else 2, fps=fps,
                                 speed=float(rng.uniform(*speed)), yaw_rate_deg=float(rng.uniform(-5, 5)),
                                 altitude=float(min(base.altitude, 0.8 * half * f / wpx * 2)))
+        traj = None
+        if trajectories and not render:
+            from .workloads import trajectory
+            traj = trajectory(w)
         out.append(Request(id=i, arrival=t, terrain=key, camera=cam, workload=w, horizons=n_h, frames=n_f,
-                           adaptive=adaptive and not render))
+                           adaptive=adaptive and not render, trajectory=traj))
     return out
 
 

@@ -204,14 +204,21 @@ def trajectory(w: Workload) -> "Trajectory":
     return Trajectory(flight_frames(w), w.fps)
 
 
-def horizon_intervals(w: Workload, traj, start_time: float, first_step: int, count: int) -> np.ndarray:
-    """The OpticalIntervals of host horizons first_step+1 .. first_step+count
-    flown along `traj`, from `start_time` (the end of the previous horizon):
-    bin/dvs's window, each state evaluated once on the device."""
+def horizon_grid(w: Workload, start_time: float, first_step: int, count: int) -> np.ndarray:
+    """`start_time` (the end of the previous horizon) and the ends of host
+    horizons first_step+1 .. first_step+count, a horizon that does not
+    advance dropped (apps/main_dvs.cpp:100-107)."""
     ends = w.horizon_times(first_step, count)
     times = np.concatenate([[start_time], ends])
-    keep = np.concatenate([[True], np.diff(times) > 0])  # a horizon that does not advance is dropped
-    return traj.intervals(times[keep])
+    keep = np.concatenate([[True], np.diff(times) > 0])
+    return times[keep]
+
+
+def horizon_intervals(w: Workload, traj, start_time: float, first_step: int, count: int) -> np.ndarray:
+    """The OpticalIntervals of host horizons first_step+1 .. first_step+count
+    flown along `traj`, from `start_time`: bin/dvs's window, each state
+    evaluated once on the device."""
+    return traj.intervals(horizon_grid(w, start_time, first_step, count))
 
 
 def write_bundle(w: Workload, directory, frames: int | None = None) -> None:

@@ -347,6 +347,7 @@ int dvs_readout_stats(dvs_readout readout, int64_t *dropped, int64_t *peak_queue
  * captured HIP graph while the step shape repeats).  A segment's events are
  * exactly those of a dvs_sensor fed the same intervals in the same calls. */
 typedef struct dvs_batch_s *dvs_batch;
+typedef struct dvs_trajectory_s *dvs_trajectory;
 int dvs_batch_create(int32_t horizons_per_step, int32_t use_graph, dvs_batch *out);
 int dvs_batch_destroy(dvs_batch batch);
 /* Adds and initialises a segment (EventSensor ctor + initialize). */
@@ -354,6 +355,15 @@ int dvs_batch_add(dvs_batch batch, dvs_scene scene, const dvs_physic *physic, in
                   int32_t aer_delay, const dvs_optical_state *initial_pose, double t0, int32_t *segment);
 /* Queues contiguous intervals of a segment. */
 int dvs_batch_feed(dvs_batch batch, int32_t segment, const dvs_optical_interval *intervals, int64_t count);
+/* Queues the n - 1 intervals between consecutive times (n >= 2, times[0]
+ * the end of what the segment has queued so far, strictly increasing) whose
+ * states the engine evaluates from `trajectory` on the device, in the step
+ * that integrates them: the states dvs_trajectory_intervals gives for the
+ * same times (apps/main_dvs.cpp:103-107 builds each window's intervals this
+ * way on the host).  The trajectory must outlive the segment's queued
+ * intervals.  No reference counterpart: the reference feeds host intervals. */
+int dvs_batch_feed_trajectory(dvs_batch batch, int32_t segment, dvs_trajectory trajectory, const double *times,
+                              int64_t n);
 /* One step; *pending_segments = segments with queued horizons left. */
 int dvs_batch_step(dvs_batch batch, int32_t *pending_segments);
 int dvs_batch_take(dvs_batch batch, int32_t segment, dvs_event *out, int64_t capacity, int64_t *count);
@@ -399,7 +409,6 @@ typedef struct {
   float light_intensity, light_intensity_rate;
 } dvs_trajectory_state;
 
-typedef struct dvs_trajectory_s *dvs_trajectory;
 
 /* CameraTrajectory(frames, scene_config) / build (trajectory.cpp:604-716):
  * auto-clamped Bezier keys of the mounted camera position, continuous

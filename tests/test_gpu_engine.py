@@ -109,6 +109,51 @@ def test_segments_join_and_leave(gpu):
     assert np.concatenate(got_c).tobytes() == _alone(gpu, views[2], w2, p2, n2, 5).tobytes()
 
 
+@pytest.mark.parametrize("adaptive", [False, True])
+def test_trajectory_fed_segments_equal_host_fed(gpu, adaptive):
+    """Segments fed horizon times (dvs_batch_feed_trajectory: the engine
+    evaluates the states on the device in each step) give the events of the
+    same intervals evaluated by dvs_trajectory_intervals and fed as
+    intervals, byte for byte; one segment switches from intervals to times
+    mid-run."""
+    specs = _segments()[:3]
+    geo, views = _scenes(gpu, specs)
+    trajs = [workloads.trajectory(w) for w, *_ in specs]
+    grids = [workloads.horizon_grid(w, w.start_time, 0, n) for w, _, _, n in specs]
+
+    def run(mode):
+        batch = gpu.Batch(horizons_per_step=8)
+        ids = []
+        for k, ((w, cam, phys, n), view, traj, times) in enumerate(zip(specs, views, trajs, grids)):
+            iv = traj.intervals(times)
+            sid = batch.add(view, phys, iv["start"][:1], w.start_time)
+            if adaptive:
+                batch.set_adaptive(sid, 1e-2, 16, 0.25)
+            if mode == "intervals":
+                batch.feed(sid, iv)
+            elif k == 1:
+                batch.feed(sid, iv[:11])
+                batch.feed_trajectory(sid, traj, times[11:])
+            else:
+                batch.feed_trajectory(sid, traj, times)
+            ids.append(sid)
+        got = {sid: [] for sid in ids}
+        while batch.step():
+            for sid in ids:
+                got[sid].append(batch.take(sid))
+        out = []
+        for sid in ids:
+            got[sid] += [batch.take(sid), batch.drain(sid)]
+            out.append(np.concatenate(got[sid]))
+        batch.close()
+        return out
+
+    want, got = run("intervals"), run("times")
+    assert sum(len(e) for e in want) > 50
+    for a, b in zip(got, want):
+        assert a.tobytes() == b.tobytes()
+
+
 def test_batch_rejects_mixed_subray_grids(gpu):
     w = workloads.small_variant(workloads.PLAIN_SMALL, size=48, craters=2)
     v, n, nodes, root, idx = scenes.workload_mesh(w)

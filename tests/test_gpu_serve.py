@@ -32,11 +32,18 @@ def _setup(count=10, render_fraction=0.2, adaptive=False, horizons=(3, 12)):
     return reqs, meshes, geometries
 
 
-@pytest.mark.parametrize("adaptive", [False, True])
-def test_served_equals_plain_per_request(gpu, adaptive):
+@pytest.mark.parametrize("adaptive,trajectories", [(False, False), (True, False), (True, True)],
+                         ids=["native", "adaptive", "adaptive-device-trajectory"])
+def test_served_equals_plain_per_request(gpu, adaptive, trajectories):
     """Every request's events are the same served (continuous batching, the
-    engine's per-segment solver policy) as run alone through the driver loop."""
+    engine's per-segment solver policy) as run alone through the driver loop;
+    with trajectories, served segments are fed horizon times and the engine
+    evaluates their states on the device inside its steps."""
     reqs, meshes, geometries = _setup(adaptive=adaptive, horizons=(40, 260) if adaptive else (3, 12))
+    if trajectories:
+        for r in reqs:
+            if not r.frames:
+                r.trajectory = workloads.trajectory(r.workload)
     served = [dataclasses.replace(r, events=[]) for r in reqs]
     # a budget that admits about two requests at a time forces queueing
     budget = 2.5 * max(serve.footprint_bytes(r, 4) for r in reqs)
