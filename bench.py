@@ -178,13 +178,17 @@ def _same_streams(a, b) -> bool:
 
 
 def _tune(serve, geometries, reqs, adaptive: bool) -> dict:
-    """Native steps: the autotuned (horizons per step, graph replay).  Adaptive
-    segments step by the driver's windows, so there is nothing to tune."""
+    """The autotuned launch configuration on the mix's first requests: native
+    steps over (horizons per step, graph replay); adaptive windows (their
+    length set by the driver policy) over the photocurrent run length and
+    the trace block size."""
+    probe = [dataclasses.replace(r, horizons=min(r.horizons, 800)) for r in reqs[: min(6, len(reqs))]]
     if adaptive:
-        return {"horizons_per_step": 64, "use_graph": False, "note": "adaptive windows of the driver policy"}
-    tune = serve.autotune(geometries, reqs[: min(6, len(reqs))],
-                          candidates=((16, False), (32, False), (32, True), (64, False)))
-    log(f"autotune: {tune}")
+        cands = tuple({"rate_cells": rc, "trace_threads": tt} for rc in (16, 32, 64, 128) for tt in (128, 256))
+    else:
+        cands = ((16, False), (32, False), (32, True), (64, False))
+    tune = serve.autotune(geometries, probe, candidates=cands)
+    log("autotune: " + ", ".join(f"{ {k: v for k, v in c.items() if k != 'probe_s'} }" for c in tune["candidates"]))
     return tune["best"]
 
 
@@ -210,7 +214,8 @@ def served_compare(args, requests: int, horizons: tuple) -> dict:
     best = _tune(serve, geometries, reqs, adaptive)
     served = [dataclasses.replace(r, events=[]) for r in reqs]
     server = serve.Server(geometries, budget_bytes=args.budget_gb * 1e9,
-                          horizons_per_step=best["horizons_per_step"], use_graph=best["use_graph"])
+                          horizons_per_step=best["horizons_per_step"], use_graph=best["use_graph"],
+                          tuning={"rate_cells": best["rate_cells"], "trace_threads": best["trace_threads"]})
     wall_served = server.run(served, realtime=True)
     log(f"served: {wall_served:.2f} s, {server.steps} steps, peak {server.peak_active} active")
     resident = [dataclasses.replace(r, events=[]) for r in reqs]
@@ -263,7 +268,8 @@ def batched_mode(args) -> dict:
     geometries = {"terrain1024": serve.build_scene(meshes["terrain1024"], base, base.camera)}
     best = _tune(serve, geometries, reqs, adaptive)
     batched = [dataclasses.replace(r, events=[]) for r in reqs]
-    server = serve.Server(geometries, horizons_per_step=best["horizons_per_step"], use_graph=best["use_graph"])
+    server = serve.Server(geometries, horizons_per_step=best["horizons_per_step"], use_graph=best["use_graph"],
+                          tuning={"rate_cells": best["rate_cells"], "trace_threads": best["trace_threads"]})
     wall_batched = server.run(batched, realtime=False)
     st = server.batch.stats()
     alone = [dataclasses.replace(r, events=[]) for r in reqs]
@@ -306,6 +312,10 @@ def main() -> int:
     ap.add_argument("--workload", default="davis346-terrain1024-ss4")
     ap.add_argument("--horizons", type=int, default=64, help="host horizons per native-solver step")
     ap.add_argument("--cpu-windows", type=int, default=3, help="driver windows of the CPU baseline sample")
+    ap.add_argument("--rate-cells", type=int, default=None,
+                    help="photocurrent cells per thread run of the adaptive solver (library default 128)")
+    ap.add_argument("--trace-threads", type=int, default=None,
+                    help="lanes per refinement trace block: 64, 128 or 256 (library default 128)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-served", action="store_true", help="skip the served-vs-plain comparison")
     ap.add_argument("--served-requests", type=int, default=30)
@@ -531,6 +541,10 @@ def request_mode(args) -> dict:
     log(f"warmup done ({args.warmup} twentieths of the flight on a separate sensor)")
 
     sensor = capi.Sensor(scene, w.physic)
+    tuning = None
+    if args.rate_cells or args.trace_threads:
+        tuning = {"rate_cells": args.rate_cells or 128, "trace_threads": args.trace_threads or 128}
+        sensor.set_tuning(**tuning)
     sensor.initialize(initial, w.start_time)
     sensor.reset_stats()
     flights = []
@@ -612,7 +626,8 @@ def request_mode(args) -> dict:
                    "tolerance": driver.DEFAULT_TOLERANCE, "check_period_s": driver.CHECK_PERIOD,
                    "time_resolution_s": w.time_resolution, "circuit_step_s": w.physic.circuit_step,
                    "sun_elevation_deg": w.sun_elevation_deg, "intensity": w.intensity,
-                   "noise": "shot+reset on", "bvh_leaf_size": w.leaf_size, "device_leaf_size": 4},
+                   "noise": "shot+reset on", "bvh_leaf_size": w.leaf_size, "device_leaf_size": 4,
+                   "launch_tuning": tuning or "library default"},
         "events": st["comparator_events"],
         "delivered_events_per_s": round(st["delivered_events"] / wall, 1),
         "mrays_per_s": round(traced / wall / 1e6, 3),

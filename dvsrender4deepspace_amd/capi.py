@@ -138,6 +138,10 @@ class _PhysicC(ctypes.Structure):
                 for f in dataclasses.fields(Physic)]
 
 
+class _TuningC(ctypes.Structure):
+    _fields_ = [("rate_cells", ctypes.c_int32), ("trace_threads", ctypes.c_int32)]
+
+
 class _StatsC(ctypes.Structure):
     _fields_ = [("trace_ms", ctypes.c_double), ("circuit_ms", ctypes.c_double), ("events_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double), ("trace_launches", ctypes.c_double),
@@ -242,6 +246,9 @@ def load_library() -> ctypes.CDLL:
                                          ctypes.POINTER(_I32)]),
         "dvs_batch_feed": (ctypes.c_int, [_P, _I32, _P, _I64]),
         "dvs_batch_feed_trajectory": (ctypes.c_int, [_P, _I32, _P, _P, _I64]),
+        "dvs_batch_set_tuning": (ctypes.c_int, [_P, ctypes.POINTER(_TuningC)]),
+        "dvs_batch_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(_TuningC)]),
+        "dvs_sensor_set_tuning": (ctypes.c_int, [_P, ctypes.POINTER(_TuningC)]),
         "dvs_batch_step": (ctypes.c_int, [_P, ctypes.POINTER(_I32)]),
         "dvs_batch_take": (ctypes.c_int, [_P, _I32, _P, _I64, ctypes.POINTER(_I64)]),
         "dvs_batch_drain": (ctypes.c_int, [_P, _I32, ctypes.POINTER(_I64)]),
@@ -569,6 +576,11 @@ class Sensor:
         _check(load_library().dvs_sensor_drain(self.handle, ctypes.byref(ready)))
         return self._take(ready.value)
 
+    def set_tuning(self, rate_cells: int = 128, trace_threads: int = 128) -> None:
+        """Launch shapes of the adaptive kernels (dvs_sensor_set_tuning);
+        results do not depend on them."""
+        _check(load_library().dvs_sensor_set_tuning(self.handle, ctypes.byref(_TuningC(rate_cells, trace_threads))))
+
     def readout_stats(self) -> tuple[int, int]:
         d, p = _I64(), _I64()
         _check(load_library().dvs_sensor_readout_stats(self.handle, ctypes.byref(d), ctypes.byref(p)))
@@ -680,6 +692,16 @@ class Batch:
     def feed(self, segment: int, intervals: np.ndarray) -> None:
         iv = np.ascontiguousarray(np.atleast_1d(intervals), dtype=INTERVAL_DTYPE)
         _check(load_library().dvs_batch_feed(self.handle, int(segment), _ptr(iv), len(iv)))
+
+    def set_tuning(self, rate_cells: int = 128, trace_threads: int = 128) -> None:
+        """Launch shapes of the adaptive kernels (dvs_batch_set_tuning):
+        photocurrent cells per thread run and trace lanes per block."""
+        _check(load_library().dvs_batch_set_tuning(self.handle, ctypes.byref(_TuningC(rate_cells, trace_threads))))
+
+    def tuning(self) -> dict:
+        t = _TuningC()
+        _check(load_library().dvs_batch_get_tuning(self.handle, ctypes.byref(t)))
+        return {"rate_cells": t.rate_cells, "trace_threads": t.trace_threads}
 
     def feed_trajectory(self, segment: int, trajectory: "Trajectory", times) -> None:
         """Queues the intervals between consecutive `times`, their states

@@ -120,11 +120,13 @@ hipError_t launch_adapt_cell_frac(int cells, const double *t0, const double *t1,
 // poses[s * 5 + j], written to out[pixel * 5 + j]; where reuse[s] is set,
 // probe 0 is the previous window's end probe, prev_probes[pixel * 5 + 4]);
 // LAYOUT_QUERY item q is item_pixel[q] at poses[q]; LAYOUT_CELL item q is
-// item_pixel[q] at poses[item_pose[q]].
+// item_pixel[q] at poses[item_pose[q]].  `lds` is the block's LDS for 256
+// lanes; blocks of `threads` lanes (64, 128 or 256) take a proportional share.
 hipError_t launch_adapt_trace(int layout, const SegDev *segs, const int *pixel_begin, int nseg, int samples,
                               long long items, const int *item_pixel, const int *item_pose, const Pose *poses,
                               size_t lds, double *out, int *status, unsigned long long *shadow_count,
-                              hipStream_t st, const int *reuse = nullptr, const double *prev_probes = nullptr);
+                              hipStream_t st, const int *reuse = nullptr, const double *prev_probes = nullptr,
+                              int threads = 256);
 // Also clears clear[0, clear_words) and event_counts[0, pixels).
 hipError_t launch_adapt_init(int pixels, AdaptBufs b, AdaptEntry *entries, int *clear, int clear_words,
                              int *event_counts, hipStream_t st);
@@ -167,12 +169,13 @@ hipError_t launch_adapt_segment_stats(int nseg, const int *pixel_begin, AdaptBuf
 // the error of list entry i goes to errors[g * 1024 + slots[g * 1024 + i]].
 hipError_t launch_adapt_gate(const SegDev *gate_segs, const int *gate_seg, const int *slots, int ngate,
                              const AdaptDev *ad, AdaptBufs b, double *errors, hipStream_t st);
-// Photocurrent of cells [c0, c1) of the knot pixels; `rare` holds
-// adapt_rates_flags(pixels, c1 - c0) per-wave flags of the second phase.
-size_t adapt_rates_flags(int pixels, int cells);
+// Photocurrent of cells [c0, c1) of the knot pixels in runs of `run` cells
+// per thread (a multiple of 8); `rare` holds adapt_rates_flags(pixels, c1 -
+// c0, run) per-wave flags of the second phase.
+size_t adapt_rates_flags(int pixels, int cells, int run);
 hipError_t launch_adapt_rates(const SegDev *segs, const AdaptDev *ad, const int *pixel_begin, int nseg, int pixels,
                               AdaptBufs b, int c0, int c1, int *status, int *rare, size_t rare_capacity,
-                              hipStream_t st);
+                              hipStream_t st, int run);
 hipError_t launch_adapt_integrate(const SegDev *segs, const AdaptDev *ad, const int *block_begin, int nseg,
                                   int blocks, int pixels, AdaptBufs b, int c0, int c1, int *status, hipStream_t st);
 // verify_knots (optical_sampler.cu:86-103): the probe poses of a window, and

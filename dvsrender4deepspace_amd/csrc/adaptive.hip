@@ -74,8 +74,10 @@ __global__ void adapt_cell_frac(int cells, const double *t0, const double *t1, c
 // items are probes 1..4 of each pixel (one wave per pixel at 4x4 sub-rays),
 // and the last e - b items only copy the previous end probe into probe 0, in
 // blocks that trace nothing.  Probe j of pixel p goes to out[p * 5 + j].
-template <int LAYOUT>
-__global__ __launch_bounds__(256) void adapt_trace(const SegDev *segs, const int *pixel_begin, int nseg,
+// THREADS lanes per block (64, 128 or 256): THREADS / ss^2 pixel samples,
+// each lane one sub-ray with its traversal stack in LDS at stride THREADS.
+template <int LAYOUT, int THREADS>
+__global__ __launch_bounds__(THREADS) void adapt_trace(const SegDev *segs, const int *pixel_begin, int nseg,
                                                    long long items, const int *item_pixel, const int *item_pose,
                                                    const Pose *poses, double *out, int *status,
                                                    unsigned long long *shadow_count, const int *reuse,
@@ -83,7 +85,7 @@ __global__ __launch_bounds__(256) void adapt_trace(const SegDev *segs, const int
   extern __shared__ double lds_words[];
   int *words = reinterpret_cast<int *>(lds_words);
   int *stack = words + threadIdx.x;
-  const int samples = segs[0].scene.samples, R = samples * samples, slots = 256 / R;
+  const int samples = segs[0].scene.samples, R = samples * samples, slots = THREADS / R;
   const int slot = threadIdx.x / R, sub = threadIdx.x - slot * R;
   const long long q = (long long)blockIdx.x * slots + slot;
   const bool active = slot < slots && q < items;
@@ -120,14 +122,15 @@ __global__ __launch_bounds__(256) void adapt_trace(const SegDev *segs, const int
     }
     const int si = segment_of<int>(pixel_begin, nseg, gp);
     if (!reused)
-      r = sample_subray<true>(segs[si].scene, gp - pixel_begin[si], pose, sub % samples, sub / samples, stack, 256);
+      r = sample_subray<true>(segs[si].scene, gp - pixel_begin[si], pose, sub % samples, sub / samples, stack,
+                              THREADS);
   }
   const LdsInts lw = (LdsInts)words;  // the lanes' radiances, low then high words
   lw[threadIdx.x] = __double2loint(r.radiance);
-  lw[256 + threadIdx.x] = __double2hiint(r.radiance);
+  lw[THREADS + threadIdx.x] = __double2hiint(r.radiance);
   // shadow rays counted per block: one atomic on the shared counter per
   // block instead of per wave
-  __shared__ int block_shadows[4];
+  __shared__ int block_shadows[THREADS / 64];
   const unsigned long long shadows = __ballot(r.shadow);
   const unsigned long long invalid = __ballot(r.invalid);
   if ((threadIdx.x & 63) == 0) {
@@ -136,7 +139,9 @@ __global__ __launch_bounds__(256) void adapt_trace(const SegDev *segs, const int
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    const int n = block_shadows[0] + block_shadows[1] + block_shadows[2] + block_shadows[3];
+    int n = 0;
+#pragma unroll
+    for (int w = 0; w < THREADS / 64; ++w) n += block_shadows[w];
     if (n) atomicAdd(shadow_count, (unsigned long long)n);
   }
   if (!active || sub != 0) return;
@@ -147,7 +152,7 @@ __global__ __launch_bounds__(256) void adapt_trace(const SegDev *segs, const int
     return;
   }
   double total = 0;
-  for (int j = 0; j < R; ++j) total += __hiloint2double(lw[256 + threadIdx.x + j], lw[threadIdx.x + j]);
+  for (int j = 0; j < R; ++j) total += __hiloint2double(lw[THREADS + threadIdx.x + j], lw[threadIdx.x + j]);
   if (!isfinite(total)) atomicOr(status, 2);
   out[at] = total / R;
 }
@@ -576,21 +581,22 @@ __device__ __forceinline__ int knot_bracket_search(const double2 *v, int size, d
 // Photocurrent of every (cell, pixel) of cells [c0, c1) for pixels that
 // interpolate knots (integrate_optical_knots, event_sensor.cu:97-105): knot
 // or exact-cell radiance, then the cell's shot-noise draw.  A thread takes
-// one pixel and kRateCells consecutive cells (x over pixels, y over cell
-// runs): one bisection finds the first cell's knot bracket and exact-cell
+// one pixel and `run` consecutive cells (x over pixels, y over cell runs; a
+// multiple of kCodeCells, kRateCells unless tuned): one bisection finds the first cell's knot bracket and exact-cell
 // position, then both walk forward as in the reference, so each knot is read
 // once per run while the rate stores stay coalesced across pixels.
-constexpr int kRateCells = 32;
+constexpr int kRateCells = 32;  // the default run; launches take any multiple of kCodeCells
 static_assert(kRateCells % kCodeCells == 0, "rate runs start at whole code words");
 // Returns whether any cell of the run was left to adapt_rates_rare.
 __device__ __forceinline__ bool adapt_rates_run(const SegDev *segs, const AdaptDev *ad, const int *pixel_begin,
-                                                int nseg, int pixels, const AdaptBufs &b, int c0, int c1, int *status) {
+                                                int nseg, int pixels, const AdaptBufs &b, int c0, int c1, int run,
+                                                int *status) {
   const int gp = blockIdx.x * 256 + threadIdx.x;
   if (gp >= pixels) return false;
   const int si = segment_of<int>(pixel_begin, nseg, gp);
   const SegDev &g = segs[si];
   const AdaptDev &A = ad[si];
-  const int cb = c0 + (int)blockIdx.y * kRateCells, ce = min(min(c1, g.cells), cb + kRateCells);
+  const int cb = c0 + (int)blockIdx.y * run, ce = min(min(c1, g.cells), cb + run);
   if (cb >= ce || !A.accepted || (b.flags[gp] & 1)) return false;
   const int i = gp - pixel_begin[si];
   const int size = b.kcount[gp];
@@ -664,9 +670,9 @@ __device__ __forceinline__ size_t rate_wave() {
   return ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 4 + (threadIdx.x >> 6);
 }
 __global__ __launch_bounds__(256) void adapt_rates(const SegDev *segs, const AdaptDev *ad, const int *pixel_begin,
-                                                   int nseg, int pixels, AdaptBufs b, int c0, int c1, int *status,
-                                                   int *rare) {
-  const bool marked = adapt_rates_run(segs, ad, pixel_begin, nseg, pixels, b, c0, c1, status);
+                                                   int nseg, int pixels, AdaptBufs b, int c0, int c1, int run,
+                                                   int *status, int *rare) {
+  const bool marked = adapt_rates_run(segs, ad, pixel_begin, nseg, pixels, b, c0, c1, run, status);
   // one flag per wave: does any lane leave a draw to adapt_rates_rare
   const unsigned long long any = __ballot(marked);
   if ((threadIdx.x & 63) == 0) rare[rate_wave()] = any != 0;
@@ -675,14 +681,14 @@ __global__ __launch_bounds__(256) void adapt_rates(const SegDev *segs, const Ada
 // the waves that flagged any.
 __global__ __launch_bounds__(256) void adapt_rates_rare(const SegDev *segs, const AdaptDev *ad,
                                                         const int *pixel_begin, int nseg, int pixels, AdaptBufs b,
-                                                        int c0, int c1, const int *rare) {
+                                                        int c0, int c1, int run, const int *rare) {
   if (!rare[rate_wave()]) return;
   const int gp = blockIdx.x * 256 + threadIdx.x;
   if (gp >= pixels) return;
   const int si = segment_of<int>(pixel_begin, nseg, gp);
   const SegDev &g = segs[si];
   const AdaptDev &A = ad[si];
-  const int cb = c0 + (int)blockIdx.y * kRateCells, ce = min(min(c1, g.cells), cb + kRateCells);
+  const int cb = c0 + (int)blockIdx.y * run, ce = min(min(c1, g.cells), cb + run);
   if (cb >= ce || !A.accepted || (b.flags[gp] & 1)) return;
   const int i = gp - pixel_begin[si];
   double *out = b.rates + gp;
@@ -796,31 +802,55 @@ hipError_t launch_adapt_cell_frac(int cells, const double *t0, const double *t1,
   adapt_cell_frac<<<grid(cells), 256, 0, st>>>(cells, t0, t1, cell_begin, nseg, ad, frac);
   return hipGetLastError();
 }
+namespace {
+template <int THREADS>
+void adapt_trace_as(int layout, long long blocks, size_t lds, const SegDev *segs, const int *pixel_begin, int nseg,
+                    long long items, const int *item_pixel, const int *item_pose, const Pose *poses, double *out,
+                    int *status, unsigned long long *shadow_count, hipStream_t st, const int *reuse,
+                    const double *prev_probes) {
+  const dim3 g((unsigned)blocks);
+  switch (layout) {
+    case LAYOUT_PROBE:
+      adapt_trace<LAYOUT_PROBE, THREADS><<<g, THREADS, lds, st>>>(segs, pixel_begin, nseg, items, item_pixel,
+                                                                    item_pose, poses, out, status, shadow_count,
+                                                                    reuse, prev_probes);
+      break;
+    case LAYOUT_QUERY:
+      adapt_trace<LAYOUT_QUERY, THREADS><<<g, THREADS, lds, st>>>(segs, pixel_begin, nseg, items, item_pixel,
+                                                                    item_pose, poses, out, status, shadow_count,
+                                                                    reuse, prev_probes);
+      break;
+    default:
+      adapt_trace<LAYOUT_CELL, THREADS><<<g, THREADS, lds, st>>>(segs, pixel_begin, nseg, items, item_pixel,
+                                                                   item_pose, poses, out, status, shadow_count,
+                                                                   reuse, prev_probes);
+  }
+}
+}  // namespace
+
 hipError_t launch_adapt_trace(int layout, const SegDev *segs, const int *pixel_begin, int nseg, int samples,
                               long long items, const int *item_pixel, const int *item_pose, const Pose *poses,
                               size_t lds, double *out, int *status, unsigned long long *shadow_count,
-                              hipStream_t st, const int *reuse, const double *prev_probes) {
+                              hipStream_t st, const int *reuse, const double *prev_probes, int threads) {
   if (layout == LAYOUT_PROBE && (items % 5 != 0 || !reuse)) return hipErrorInvalidValue;
+  if (threads != 64 && threads != 128 && threads != 256) return hipErrorInvalidValue;
+  if (samples * samples > threads) threads = 256;  // a block holds at least one pixel sample
   if (items <= 0) return hipSuccess;
-  const int slots = 256 / (samples * samples);
+  const int slots = threads / (samples * samples);
   const long long blocks = (items + slots - 1) / slots;
   if (blocks > kMaxBlocks256) return hipErrorInvalidConfiguration;
-  switch (layout) {
-    case LAYOUT_PROBE:
-      adapt_trace<LAYOUT_PROBE><<<(unsigned)blocks, 256, lds, st>>>(segs, pixel_begin, nseg, items, item_pixel,
-                                                                     item_pose, poses, out, status, shadow_count,
-                                                                     reuse, prev_probes);
-      break;
-    case LAYOUT_QUERY:
-      adapt_trace<LAYOUT_QUERY><<<(unsigned)blocks, 256, lds, st>>>(segs, pixel_begin, nseg, items, item_pixel,
-                                                                     item_pose, poses, out, status, shadow_count,
-                                                                     reuse, prev_probes);
-      break;
-    default:
-      adapt_trace<LAYOUT_CELL><<<(unsigned)blocks, 256, lds, st>>>(segs, pixel_begin, nseg, items, item_pixel,
-                                                                    item_pose, poses, out, status, shadow_count,
-                                                                     reuse, prev_probes);
-  }
+  // lds holds a stack of lds / 1024 levels for 256 lanes: the same levels
+  // for `threads` lanes
+  const size_t bytes = lds / 256 * (size_t)threads;
+  if (threads == 64)
+    adapt_trace_as<64>(layout, blocks, bytes, segs, pixel_begin, nseg, items, item_pixel, item_pose, poses, out,
+                       status, shadow_count, st, reuse, prev_probes);
+  else if (threads == 128)
+    adapt_trace_as<128>(layout, blocks, bytes, segs, pixel_begin, nseg, items, item_pixel, item_pose, poses, out,
+                        status, shadow_count, st, reuse, prev_probes);
+  else
+    adapt_trace_as<256>(layout, blocks, bytes, segs, pixel_begin, nseg, items, item_pixel, item_pose, poses, out,
+                        status, shadow_count, st, reuse, prev_probes);
   return hipGetLastError();
 }
 hipError_t launch_adapt_init(int pixels, AdaptBufs b, AdaptEntry *entries, int *clear, int clear_words,
@@ -902,17 +932,20 @@ hipError_t launch_adapt_gate(const SegDev *gate_segs, const int *gate_seg, const
   adapt_gate<<<ngate * 8, 256, 0, st>>>(gate_segs, gate_seg, slots, ad, b, errors);
   return hipGetLastError();
 }
-size_t adapt_rates_flags(int pixels, int cells) {
-  return pixels <= 0 || cells <= 0 ? 0 : (size_t)grid(pixels) * 4 * (size_t)((cells + kRateCells - 1) / kRateCells);
+size_t adapt_rates_flags(int pixels, int cells, int run) {
+  if (run < kCodeCells) run = kCodeCells;
+  return pixels <= 0 || cells <= 0 ? 0 : (size_t)grid(pixels) * 4 * (size_t)((cells + run - 1) / run);
 }
 hipError_t launch_adapt_rates(const SegDev *segs, const AdaptDev *ad, const int *pixel_begin, int nseg, int pixels,
                               AdaptBufs b, int c0, int c1, int *status, int *rare, size_t rare_capacity,
-                              hipStream_t st) {
+                              hipStream_t st, int run) {
+  if (run < kCodeCells || run % kCodeCells != 0) return hipErrorInvalidValue;
   if (pixels <= 0 || c1 <= c0) return hipSuccess;
-  if (adapt_rates_flags(pixels, c1 - c0) > rare_capacity) return hipErrorInvalidConfiguration;
-  const dim3 g(grid(pixels), (unsigned)((c1 - c0 + kRateCells - 1) / kRateCells));
-  adapt_rates<<<g, 256, 0, st>>>(segs, ad, pixel_begin, nseg, pixels, b, c0, c1, status, rare);
-  adapt_rates_rare<<<g, 256, 0, st>>>(segs, ad, pixel_begin, nseg, pixels, b, c0, c1, rare);
+  if (adapt_rates_flags(pixels, c1 - c0, run) > rare_capacity) return hipErrorInvalidConfiguration;
+  const dim3 g(grid(pixels), (unsigned)((c1 - c0 + run - 1) / run));
+  if (g.y > 65535) return hipErrorInvalidConfiguration;
+  adapt_rates<<<g, 256, 0, st>>>(segs, ad, pixel_begin, nseg, pixels, b, c0, c1, run, status, rare);
+  adapt_rates_rare<<<g, 256, 0, st>>>(segs, ad, pixel_begin, nseg, pixels, b, c0, c1, run, rare);
   return hipGetLastError();
 }
 hipError_t launch_adapt_integrate(const SegDev *segs, const AdaptDev *ad, const int *block_begin, int nseg,

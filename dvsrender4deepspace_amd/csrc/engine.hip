@@ -320,6 +320,15 @@ class Engine {
     s.initialized = true;
   }
 
+  // Launch shapes of the adaptive kernels (results do not depend on them).
+  void set_tuning(const dvs_tuning &t) {
+    if (t.rate_cells < 8 || t.rate_cells > 1024 || t.rate_cells % 8 != 0 ||
+        (t.trace_threads != 64 && t.trace_threads != 128 && t.trace_threads != 256))
+      fail(DVS_E_ARGUMENT, "invalid launch tuning");
+    tuning_ = t;
+  }
+  const dvs_tuning &tuning() const { return tuning_; }
+
   // Queues contiguous intervals (EventSensor::upload_intervals checks,
   // event_sensor.cu:274-282).
   void feed(int id, const dvs_optical_interval *ivs, int64_t n) {
@@ -1217,7 +1226,7 @@ class Engine {
     hip_check(hipEventRecord(probe_t.start, st), "record");
     hip_check(launch_adapt_trace(dvsdev::LAYOUT_PROBE, segtab_.ptr, pbegin_.ptr, nseg, samples, (long long)P * 5,
                                  nullptr, nullptr, probe_poses_.ptr, lds, probes_[pcur].ptr, status_.ptr, counter_.ptr,
-                                 st, probe_reuse_.ptr, probes_[pprev].ptr),
+                                 st, probe_reuse_.ptr, probes_[pprev].ptr, tuning_.trace_threads),
               "refine pixel optical launch");
     hip_check(hipEventRecord(probe_t.stop, st), "record");
     probe_layout_ = std::move(layout);
@@ -1298,7 +1307,8 @@ class Engine {
         query_t.emplace_back(new Timer());
         hip_check(hipEventRecord(query_t.back()->start, st), "record");
         hip_check(launch_adapt_trace(dvsdev::LAYOUT_QUERY, segtab_.ptr, pbegin_.ptr, nseg, samples, queries,
-                                     qpix_.ptr, nullptr, qpose_.ptr, lds, qrad_.ptr, status_.ptr, counter_.ptr, st),
+                                     qpix_.ptr, nullptr, qpose_.ptr, lds, qrad_.ptr, status_.ptr, counter_.ptr, st,
+                                     nullptr, nullptr, tuning_.trace_threads),
                   "refine pixel optical launch");
         hip_check(hipEventRecord(query_t.back()->stop, st), "record");
         stats.query_samples += (uint64_t)queries;
@@ -1363,7 +1373,7 @@ class Engine {
                 "identify local exact optical cells");
       hip_check(launch_adapt_trace(dvsdev::LAYOUT_CELL, segtab_.ptr, pbegin_.ptr, nseg, samples, exact_total,
                                    eitem_pix_.ptr, eitem_pose_.ptr, poses_.ptr, lds, erad_.ptr, status_.ptr,
-                                   counter_.ptr, st),
+                                   counter_.ptr, st, nullptr, nullptr, tuning_.trace_threads),
                 "trace local exact ray cells");
     }
     // Whole-pixel fallback list (compact_fallback, event_sensor.cu:116-118).
@@ -1512,7 +1522,7 @@ class Engine {
     codes_.reserve((size_t)((chunk + dvsdev::kCodeCells - 1) / dvsdev::kCodeCells) * P, "adaptive photocurrent");
     b.codes = codes_.ptr;
     // per-wave flags of the knot pixels' and the fallback pixels' photocurrent
-    const size_t rare_knots = adapt_rates_flags(P, chunk), rare_fallback = cell_rates_flags(fb_total, chunk);
+    const size_t rare_knots = adapt_rates_flags(P, chunk, tuning_.rate_cells), rare_fallback = cell_rates_flags(fb_total, chunk);
     rare_adapt_.reserve(rare_knots, "photocurrent flags");
     rare_native_.reserve(rare_fallback, "photocurrent flags");
     uint64_t fallback_samples = 0;
@@ -1560,7 +1570,7 @@ class Engine {
         }
       }
       hip_check(launch_adapt_rates(segtab_.ptr, adapt_.ptr, pbegin_.ptr, nseg, P, b, c0, c1, status_.ptr,
-                                   rare_adapt_.ptr, rare_knots, st),
+                                   rare_adapt_.ptr, rare_knots, st, tuning_.rate_cells),
                 "adaptive photocurrent launch");
       integrate_t.emplace_back(new Timer());
       hip_check(hipEventRecord(integrate_t.back()->start, st), "record");
@@ -1670,6 +1680,9 @@ class Engine {
   // engine steps run so far, and the last adaptive step's trace LDS bytes and segments
   uint64_t step_serial_ = 0;
   size_t last_lds_ = 0;
+  // measured on the DAVIS346 request's first windows: 128-lane trace blocks
+  // and 128-cell photocurrent runs were 1.5% faster per window than 256 / 32
+  dvs_tuning tuning_{128, 128};
   int last_nseg_ = 0;
   DeviceArray<int> verify_pix_, verify_pose_;
   DeviceArray<Pose> verify_poses_;
@@ -1977,6 +1990,27 @@ int dvs_batch_feed_trajectory(dvs_batch b, int32_t segment, dvs_trajectory traje
   return guarded([&] {
     if (!b) fail(DVS_E_ARGUMENT, "null batch");
     b->engine->feed_trajectory(segment, trajectory, times, n);
+  });
+}
+
+int dvs_batch_set_tuning(dvs_batch b, const dvs_tuning *tuning) {
+  return guarded([&] {
+    if (!b || !tuning) fail(DVS_E_ARGUMENT, "null batch or tuning");
+    b->engine->set_tuning(*tuning);
+  });
+}
+
+int dvs_batch_get_tuning(dvs_batch b, dvs_tuning *tuning) {
+  return guarded([&] {
+    if (!b || !tuning) fail(DVS_E_ARGUMENT, "null batch or tuning");
+    *tuning = b->engine->tuning();
+  });
+}
+
+int dvs_sensor_set_tuning(dvs_sensor s, const dvs_tuning *tuning) {
+  return guarded([&] {
+    if (!s || !tuning) fail(DVS_E_ARGUMENT, "null sensor or tuning");
+    s->engine->set_tuning(*tuning);
   });
 }
 

@@ -18,8 +18,11 @@ windows with the receptor check) and the readout is drained.
 of the adaptive solver run under the same window policy inside the engine,
 so both paths give every request the same events.
 
-``autotune`` picks the engine's horizons per step and graph replay by timing
-short probes of the actual segment mix.
+``autotune`` times short probes of the actual segment mix per candidate
+launch configuration: horizons per step and graph replay for native steps;
+for adaptive windows (whose length the driver policy fixes) the photocurrent
+cells per thread run and the refinement trace's lanes per block (pixel
+samples per block at 4x4 sub-rays: 4, 8 or 16).
 """
 from __future__ import annotations
 
@@ -89,11 +92,13 @@ class Server:
     """Continuous-batching request loop with HBM-footprint admission."""
 
     def __init__(self, geometries: dict, budget_bytes: float = 200e9, horizons_per_step: int = 16,
-                 use_graph: bool = False):
+                 use_graph: bool = False, tuning: dict | None = None):
         self.geometries = geometries          # terrain key -> capi.Scene (resident BVH + mesh)
         self.budget = budget_bytes
         self.horizons_per_step = horizons_per_step
         self.batch = capi.Batch(horizons_per_step=horizons_per_step, use_graph=use_graph)
+        if tuning:
+            self.batch.set_tuning(**tuning)
         self.views = {}
         self.waiting = deque()
         self.active = {}                      # segment id -> (request, bytes)
@@ -229,15 +234,30 @@ def run_plain(requests: list, meshes: dict, realtime: bool = True, scenes_reside
     return clock()
 
 
-def autotune(geometries: dict, requests: list, candidates=((8, False), (16, False), (32, False), (16, True),
-                                                            (32, True), (64, False))) -> dict:
-    """Runs the probe requests through a Server twice per (horizons per step,
-    graph replay) candidate, the first pass as warm-up (allocations, graph
-    capture), and returns the candidate whose second pass traced the most rays
-    per second."""
+NATIVE_CANDIDATES = ((8, False), (16, False), (32, False), (16, True), (32, True), (64, False))
+ADAPTIVE_CANDIDATES = tuple({"rate_cells": rc, "trace_threads": tt} for rc in (16, 32, 64, 128) for tt in (64, 128, 256))
+
+
+def autotune(geometries: dict, requests: list, candidates=None) -> dict:
+    """Runs the probe requests through a Server twice per candidate, the
+    first pass as warm-up (allocations, graph capture), and returns the
+    candidate whose second pass traced the most rays per second.  A
+    candidate is (horizons per step, graph replay) or a dict of Server
+    settings and launch tuning (horizons_per_step, use_graph, rate_cells,
+    trace_threads); by default NATIVE_CANDIDATES, or ADAPTIVE_CANDIDATES when
+    the probe requests are adaptive."""
+    if candidates is None:
+        candidates = ADAPTIVE_CANDIDATES if any(r.adaptive for r in requests) else NATIVE_CANDIDATES
     results = []
-    for hps, graph in candidates:
-        server = Server(geometries, horizons_per_step=hps, use_graph=graph)
+    for cand in candidates:
+        cfg = {"horizons_per_step": 16, "use_graph": False, "rate_cells": 128, "trace_threads": 128}
+        if isinstance(cand, dict):
+            cfg.update(cand)
+        else:
+            cfg["horizons_per_step"], cfg["use_graph"] = cand
+        hps, graph = cfg["horizons_per_step"], cfg["use_graph"]
+        tuning = {"rate_cells": cfg["rate_cells"], "trace_threads": cfg["trace_threads"]}
+        server = Server(geometries, horizons_per_step=hps, use_graph=graph, tuning=tuning)
         for rep in range(2):
             probe = [dataclasses.replace(r, events=[], arrival=0.0) for r in requests]
             server.batch.reset_stats()
@@ -246,8 +266,7 @@ def autotune(geometries: dict, requests: list, candidates=((8, False), (16, Fals
             dt = time.perf_counter() - t
         st = server.batch.stats()
         rays = st["primary_rays"] + st["shadow_rays"] + st["event_rays"]
-        results.append({"horizons_per_step": hps, "use_graph": graph, "mrays_per_s": rays / dt / 1e6,
-                        "probe_s": dt})
+        results.append({**cfg, "mrays_per_s": rays / dt / 1e6, "probe_s": dt})
         server.batch.close()
     best = max(results, key=lambda r: r["mrays_per_s"])
     return {"best": best, "candidates": results}

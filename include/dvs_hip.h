@@ -364,6 +364,21 @@ int dvs_batch_feed(dvs_batch batch, int32_t segment, const dvs_optical_interval 
  * intervals.  No reference counterpart: the reference feeds host intervals. */
 int dvs_batch_feed_trajectory(dvs_batch batch, int32_t segment, dvs_trajectory trajectory, const double *times,
                               int64_t n);
+/* Launch shapes of the adaptive solver's kernels, for autotuning; results
+ * do not depend on them.  No reference counterpart.
+ *   rate_cells     circuit cells each photocurrent thread draws in one run
+ *                  (a multiple of 8 in [8, 1024]; 128 by default);
+ *   trace_threads  lanes of a refinement trace block, one sub-ray each (64,
+ *                  128, the default, or 256; 256 where a pixel's sub-rays
+ *                  outnumber the block). */
+typedef struct dvs_tuning {
+  int32_t rate_cells;
+  int32_t trace_threads;
+} dvs_tuning;
+int dvs_batch_set_tuning(dvs_batch batch, const dvs_tuning *tuning);
+int dvs_batch_get_tuning(dvs_batch batch, dvs_tuning *tuning);
+/* The same for a sensor (a one-segment engine). */
+int dvs_sensor_set_tuning(dvs_sensor sensor, const dvs_tuning *tuning);
 /* One step; *pending_segments = segments with queued horizons left. */
 int dvs_batch_step(dvs_batch batch, int32_t *pending_segments);
 int dvs_batch_take(dvs_batch batch, int32_t segment, dvs_event *out, int64_t capacity, int64_t *count);

@@ -154,6 +154,43 @@ def test_trajectory_fed_segments_equal_host_fed(gpu, adaptive):
         assert a.tobytes() == b.tobytes()
 
 
+def test_launch_tuning_keeps_events(gpu):
+    """The adaptive kernels' launch shapes (photocurrent cells per thread
+    run, trace lanes per block) change no event; invalid shapes are refused."""
+    specs = _segments()[:3]
+    geo, views = _scenes(gpu, specs)
+
+    def run(tuning):
+        batch = gpu.Batch(horizons_per_step=8)
+        if tuning:
+            batch.set_tuning(**tuning)
+            assert batch.tuning() == tuning
+        ids = []
+        for (w, cam, phys, n), view in zip(specs, views):
+            sid = batch.add(view, phys, w.initial_state(), w.start_time)
+            batch.set_adaptive(sid, 1e-2, 16, 0.25)
+            batch.feed(sid, w.intervals(0, n))
+            ids.append(sid)
+        got = {sid: [] for sid in ids}
+        while batch.step():
+            for sid in ids:
+                got[sid].append(batch.take(sid))
+        out = [np.concatenate(got[sid] + [batch.take(sid), batch.drain(sid)]) for sid in ids]
+        batch.close()
+        return out
+
+    want = run(None)
+    assert sum(len(e) for e in want) > 20
+    for tuning in ({"rate_cells": 8, "trace_threads": 64}, {"rate_cells": 128, "trace_threads": 128},
+                   {"rate_cells": 1024, "trace_threads": 256}):
+        for a, b in zip(run(tuning), want):
+            assert a.tobytes() == b.tobytes(), tuning
+    batch = gpu.Batch()
+    for bad in ({"rate_cells": 12, "trace_threads": 256}, {"rate_cells": 32, "trace_threads": 96}):
+        with pytest.raises(capi.InvalidArgument, match="invalid launch tuning"):
+            batch.set_tuning(**bad)
+
+
 def test_batch_rejects_mixed_subray_grids(gpu):
     w = workloads.small_variant(workloads.PLAIN_SMALL, size=48, craters=2)
     v, n, nodes, root, idx = scenes.workload_mesh(w)

@@ -71,3 +71,18 @@ def test_autotune_reports_candidates(gpu):
     reqs, meshes, geometries = _setup(count=4, render_fraction=0.0)
     tune = serve.autotune(geometries, reqs, candidates=((2, False), (4, True)))
     assert len(tune["candidates"]) == 2 and tune["best"]["mrays_per_s"] > 0
+
+
+def test_autotune_adaptive_launch_shapes(gpu):
+    """Adaptive mixes are tuned over the photocurrent run length and the
+    trace block size; every candidate serves the same events."""
+    reqs, meshes, geometries = _setup(count=4, render_fraction=0.0, adaptive=True, horizons=(40, 120))
+    cands = ({"rate_cells": 16, "trace_threads": 64}, {"rate_cells": 64, "trace_threads": 256})
+    tune = serve.autotune(geometries, reqs, candidates=cands)
+    assert [c["rate_cells"] for c in tune["candidates"]] == [16, 64] and tune["best"]["mrays_per_s"] > 0
+    streams = []
+    for c in cands:
+        served = [dataclasses.replace(r, events=[]) for r in reqs]
+        serve.Server(geometries, tuning=c).run(served, realtime=False)
+        streams.append([r.event_array().tobytes() for r in served])
+    assert streams[0] == streams[1]
