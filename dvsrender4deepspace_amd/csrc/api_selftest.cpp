@@ -2,7 +2,9 @@
 // the way the reference's callers use it.
 //   dvs_api_selftest --cpu <config.yaml> <out_dir>
 //       Config, BVHBuilder, writers, pose conversion, CameraTrajectory, the
-//       host physics (advance_pixel, Hapke) and EventReadout (no GPU).
+//       host physics (advance_pixel, Hapke), EventReadout, the dataset
+//       budget (tests/test_dataset_budget.cpp) and frames.h5
+//       (tests/test_frames.cpp) (no GPU).
 //   dvs_api_selftest --gpu <out_dir>
 //       DeviceScene, CamRayCaster and DvsRayCaster on the plane fixture of
 //       tests/test_sensor.cu:66-80 (48x36 grid): frame depth, native and
@@ -10,8 +12,12 @@
 //       trajectory intervals built on the device driving a DvsRayCaster;
 //       EventSensor, OpticalSampler and the diagnostics.
 // Prints one line per check and exits non-zero on the first failure.
+#include <sys/wait.h>
+#include <unistd.h>
+
 #include <algorithm>
 #include <cstdio>
+#include <fstream>
 #include <iostream>
 #include <stdexcept>
 #include <string>
@@ -30,6 +36,127 @@ dvs::SceneBundle<dvs::float3, dvs::uint3> plane() {
   b.normals.assign(4, dvs::float3{0, 0, 1});
   b.bvh_triangles = {{0, 1, 2}, {0, 2, 3}};
   return b;
+}
+
+// tests/test_dataset_budget.cpp: concurrent producers share one allowance,
+// sparse files count their logical size, paths stay inside the root, and an
+// HDF5 append past the stop is refused before it writes.
+void dataset_budget(const std::filesystem::path &root) {
+  std::filesystem::create_directories(root);
+  dvs::DatasetBudget budget(root, 10000, 9000);
+  pid_t children[3];
+  for (int i = 0; i < 3; ++i) {
+    children[i] = fork();
+    if (children[i] < 0) throw std::runtime_error("fork failed");
+    if (children[i] == 0) {
+      try {
+        budget.write_text(root / (std::to_string(i) + ".txt"), std::string(4000, 'x'));
+        _exit(0);
+      } catch (const dvs::DatasetSizeLimit &) {
+        _exit(17);
+      } catch (...) {
+        _exit(1);
+      }
+    }
+  }
+  int accepted = 0, stopped = 0;
+  for (pid_t child : children) {
+    int status = 0;
+    waitpid(child, &status, 0);
+    accepted += WIFEXITED(status) && WEXITSTATUS(status) == 0;
+    stopped += WIFEXITED(status) && WEXITSTATUS(status) == 17;
+  }
+  expect(accepted == 2 && stopped == 1, "DatasetBudget: two of three concurrent 4000-byte writers fit 9000");
+  expect(budget.logical_bytes() == 8000, "DatasetBudget: aggregate logical bytes");
+  bool rejected = false;
+  try {
+    budget.require_path(root / "../outside.txt");
+  } catch (const std::runtime_error &) {
+    rejected = true;
+  }
+  expect(rejected, "DatasetBudget: a path outside the dataset is refused");
+  {
+    std::ofstream f(root / "sparse.bin");
+    f.seekp(4000);
+    f.put('x');
+  }
+  expect(budget.logical_bytes() == 12001, "DatasetBudget: sparse files count their logical size");
+  rejected = false;
+  try {
+    budget.write_text(root / "blocked.txt", "x");
+  } catch (const dvs::DatasetSizeLimit &) {
+    rejected = true;
+  }
+  expect(rejected && !std::filesystem::exists(root / "blocked.txt"), "DatasetBudget: nothing is written past the stop");
+  std::filesystem::remove_all(root);
+  std::filesystem::create_directories(root);
+  dvs::DatasetBudget hdf(root, 32000000, 30000000);
+  {
+    dvs::Hdf5EventStreamWriter writer((root / "events.h5").string(), &hdf);
+    writer.append({dvs::Event{.001234, 2, 3, 1, 10}});
+    const uint64_t before = hdf.logical_bytes();
+    {
+      std::ofstream f(root / "filler.bin");
+      f.seekp((std::streamoff)(29500000 - before - 1));
+      f.put('x');
+    }
+    const auto size_before = std::filesystem::file_size(root / "events.h5");
+    rejected = false;
+    try {
+      writer.append({dvs::Event{.002, 2, 3, 0, 10}, dvs::Event{.020, 2, 3, 0, 10}, dvs::Event{.030, 2, 3, 0, 10}});
+    } catch (const dvs::DatasetSizeLimit &) {
+      rejected = true;
+    }
+    expect(rejected && writer.event_count() == 1 && std::filesystem::file_size(root / "events.h5") == size_before,
+           "HDF5 events: an append past the stop is refused unwritten");
+    std::filesystem::remove(root / "filler.bin");
+    writer.append({dvs::Event{.0021, 2, 3, 0, 10}});
+    writer.close(.003);
+  }
+  int64_t index[8] = {}, count = 0;
+  dvs::detail::check_io(dvs_events_h5_ms_to_idx((root / "events.h5").c_str(), index, 8, &count));
+  expect(count == 4 && index[0] == 0 && index[1] == 0 && index[2] == 1 && index[3] == 2,
+         "HDF5 events: the refused append left the millisecond index intact");
+  std::filesystem::remove_all(root);
+}
+
+// tests/test_frames.cpp: frames.h5 at a camera's native size, times in
+// microseconds, ordered appends, the fixed image encoding, lossless depth.
+void frames_file(const std::filesystem::path &root, int width, int height) {
+  std::filesystem::create_directories(root);
+  dvs::DatasetBudget budget(root);
+  const auto path = root / "frames.h5";
+  const size_t pixels = (size_t)width * height;
+  std::vector<float> light(pixels, 5), depth(pixels, 25);
+  depth[0] = -1;
+  {
+    dvs::Hdf5FrameWriter writer(path, width, height, budget);
+    writer.append(0, light, depth);
+    writer.append(60, light, depth);
+    bool rejected = false;
+    try {
+      writer.append(59, light, depth);
+    } catch (const std::invalid_argument &) {
+      rejected = true;
+    }
+    expect(rejected && writer.count() == 2, "Hdf5FrameWriter refuses an unordered frame");
+    writer.close();
+  }
+  int32_t w = 0, h = 0;
+  int64_t n = 0;
+  dvs::detail::check_io(dvs_frames_h5_read(path.c_str(), nullptr, nullptr, nullptr, 0, &w, &h, &n));
+  std::vector<int64_t> times(2);
+  std::vector<uint8_t> images(2 * pixels);
+  std::vector<float> read_depth(2 * pixels);
+  dvs::detail::check_io(
+      dvs_frames_h5_read(path.c_str(), times.data(), images.data(), read_depth.data(), 2, &w, &h, &n));
+  expect(n == 2 && w == width && h == height, "frames.h5: native image axes and dimensions");
+  expect(times[0] == 0 && times[1] == 60000000, "frames.h5: times in int64 microseconds");
+  expect(read_depth[0] == -1 && read_depth[pixels] == -1 && read_depth[1] == 25, "frames.h5: lossless depth");
+  std::vector<uint8_t> expected;
+  dvs::render_float_to_png_bytes(light.data(), 1, 1, 0, expected);
+  expect(images[0] == expected[0] && images[pixels + 1] == expected[0], "frames.h5: the fixed image encoding");
+  std::filesystem::remove_all(root);
 }
 
 int cpu(const std::string &config_path, const std::string &out) {
@@ -160,6 +287,9 @@ int cpu(const std::string &config_path, const std::string &out) {
                bus.peak_queue() == 2,
            "EventReadout drops and delivery time");
   }
+  dataset_budget(std::filesystem::path(out) / "budget");
+  frames_file(std::filesystem::path(out) / "frames346", config.camera.width, config.camera.height);
+  frames_file(std::filesystem::path(out) / "frames1280", 1280, 720);
   return 0;
 }
 

@@ -18,7 +18,7 @@ def test_cpp_api_host_parts(tmp_path):
     out = subprocess.run([str(exe), "--cpu", str(GOLDEN / "davis346_profile.yaml"), str(tmp_path)],
                          capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
-    assert out.stdout.count("ok ") == 16
+    assert out.stdout.count("ok ") == 33
     assert (tmp_path / "events.bin").stat().st_size == 26
     assert eventio.read_h5(tmp_path / "events.h5")["t"].tolist() == [1234, 2100]
     rec = eventio.read_bin(tmp_path / "events.bin")
