@@ -100,7 +100,8 @@ API_SELFTEST = BINDIR / "dvs_api_selftest"
 # Programs on the C++ API: the self-test and the bin/dvs and bin/cam front
 # ends (apps/main_dvs.cpp, apps/main_cam.cpp).
 PROGRAMS = {"dvs_api_selftest": "api_selftest.cpp", "dvs": "main_dvs.cpp", "cam": "main_cam.cpp",
-            "cam_blur": "main_cam_blur.cpp", "sequence": "main_sequence.cpp"}
+            "cam_blur": "main_cam_blur.cpp", "cam_blur_windows": "main_cam_blur_windows.cpp",
+            "sequence": "main_sequence.cpp"}
 
 
 def _link_program(name: str) -> Path:
@@ -122,8 +123,8 @@ def build_api_selftest() -> Path:
 
 
 def build_apps() -> list[Path]:
-    """Links bin/dvs, bin/cam, bin/cam_blur and bin/sequence."""
-    return [_link_program(name) for name in ("dvs", "cam", "cam_blur", "sequence")]
+    """Links bin/dvs, bin/cam, bin/cam_blur, bin/cam_blur_windows and bin/sequence."""
+    return [_link_program(name) for name in ("dvs", "cam", "cam_blur", "cam_blur_windows", "sequence")]
 
 
 def build_oracle() -> Path:

@@ -201,3 +201,14 @@ def test_sequence_command_line(tmp_path):
     out = subprocess.run([str(build.BINDIR / "sequence"), str(root), str(tmp_path / "outside"), "dem.tif", ""],
                          capture_output=True, text=True, timeout=60)
     assert out.returncode == 1 and "output is outside the guarded dataset" in out.stderr
+
+
+def test_cam_blur_windows_command_line(tmp_path):
+    """apps/main_cam_blur_windows.cpp:24-28: seven arguments or the usage and
+    status 2; a sequence without its configuration fails with status 1."""
+    exe = str(build.BINDIR / "cam_blur_windows")
+    out = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 2 and out.stderr.startswith("Usage: cam_blur_windows <sequence_dir>")
+    out = subprocess.run([exe, str(tmp_path), "dem.tif", "", str(tmp_path / "o"), "1", "2", "3"], capture_output=True,
+                         text=True, timeout=60)
+    assert out.returncode == 1 and "cannot load source camera config" in out.stderr

@@ -152,3 +152,55 @@ def test_sequence_equals_binding(gpu, tmp_path, optimized):
         assert (seq / name).exists()
     print(f"sequence ({'optimized' if optimized else 'native'}): {len(events)} events, {len(frame_times)} frames",
           flush=True)
+
+
+def test_cam_blur_windows_equals_binding(gpu, tmp_path):
+    """bin/cam_blur_windows (apps/main_cam_blur_windows.cpp): three 50 ms
+    windows of the repeated trajectory, cells of blur_resolution, double
+    sums of weight x frame; PNGs equal the binding's render_blur(wide=True)
+    of the same cell poses."""
+    from PIL import Image
+
+    root, seq = _dataset(tmp_path)
+    centers = ("0.03", "0.5", "1.234")
+    out_dir = tmp_path / "blur"
+    args = [str(build.BINDIR / "cam_blur_windows"), str(seq), str(root / "dem.tif"), str(root / "albedo.tif"),
+            str(out_dir), *centers]
+    print(f"running {' '.join(args)}", flush=True)
+    out = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.count("WINDOW center_s=") == 3
+    cfg = eventio.load_config(seq / "source_config.yaml")
+    dem, albedo = eventio.load_raster(root / "dem.tif"), eventio.load_raster(root / "albedo.tif")
+    aligned = capi.raster_align(dem.x_coords, dem.y_coords, albedo.data, albedo.x_coords, albedo.y_coords)
+    v, n, idx, a = capi.raster_mesh(dem.data, dem.x_coords, dem.y_coords, aligned)
+    nodes, root_node, leaf_idx = capi.bvh_build(v, idx, 64)
+    cam = capi.Camera(cfg.camera.width, cfg.camera.height, cfg.camera.fx, cfg.camera.fy, cfg.camera.cx,
+                      cfg.camera.cy, cfg.scene.supersample_factor)
+    scene = capi.Scene(nodes, root_node, v, n, leaf_idx, cam, cfg.hapke, albedos=a)
+    traj = capi.Trajectory(eventio.load_frames_text(seq / "source_trajectory.txt"), cfg.scene.fps)
+    start, end, _ = traj.range()
+    duration = float(np.float32(end) - np.float32(start))
+    step = float(np.float32(cfg.scene.blur_resolution))
+    poses, weights, offsets, names = [], [], [0], []
+    for c in centers:
+        center = float(c)
+        begin, finish = center - 0.025, center + 0.025
+        for k in range(math.ceil(0.05 / step - 1e-10)):
+            left, right = begin + k * step, min(finish, begin + (k + 1) * step)
+            if right <= left:
+                continue
+            s = _states(traj, [(left + right) * 0.5], start, duration)
+            pose = capi.optical_states(s)
+            pose["velocity"] = pose["light_rate"] = pose["intensity_rate"] = 0
+            pose["rotation"][0] = shepperd(s["orientation"][0])
+            poses.append(pose[0])
+            weights.append((right - left) / 0.05)
+        offsets.append(len(poses))
+        names.append(f"motion_blur_{begin:.3f}_{finish:.3f}s.png")
+    frames = scene.render_blur(np.array(poses), weights, offsets, wide=True)
+    for name, frame in zip(names, frames):
+        with Image.open(out_dir / name) as im:
+            got = np.array(im)
+        assert np.array_equal(got, eventio.frame_bytes(frame)), name
+        assert np.count_nonzero(got) > 0
