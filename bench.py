@@ -316,6 +316,8 @@ def main() -> int:
                     help="photocurrent cells per thread run of the adaptive solver (library default 128)")
     ap.add_argument("--trace-threads", type=int, default=None,
                     help="lanes per refinement trace block: 64, 128 or 256 (library default 128)")
+    ap.add_argument("--traversal", type=int, choices=[0, 1], default=None,
+                    help="0: per-lane traversal stacks, 1: wave packets (the library default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-served", action="store_true", help="skip the served-vs-plain comparison")
     ap.add_argument("--served-requests", type=int, default=30)
@@ -542,9 +544,12 @@ def request_mode(args) -> dict:
 
     sensor = capi.Sensor(scene, w.physic)
     tuning = None
-    if args.rate_cells or args.trace_threads:
-        tuning = {"rate_cells": args.rate_cells or 128, "trace_threads": args.trace_threads or 128}
+    if args.rate_cells or args.trace_threads or args.traversal is not None:
+        tuning = {"rate_cells": args.rate_cells or 128, "trace_threads": args.trace_threads or 128,
+                  "traversal": 1 if args.traversal is None else args.traversal}
         sensor.set_tuning(**tuning)
+        if args.traversal is not None:
+            scene.set_traversal(args.traversal)
     sensor.initialize(initial, w.start_time)
     sensor.reset_stats()
     flights = []

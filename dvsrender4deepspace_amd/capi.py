@@ -139,7 +139,7 @@ class _PhysicC(ctypes.Structure):
 
 
 class _TuningC(ctypes.Structure):
-    _fields_ = [("rate_cells", ctypes.c_int32), ("trace_threads", ctypes.c_int32)]
+    _fields_ = [("rate_cells", ctypes.c_int32), ("trace_threads", ctypes.c_int32), ("traversal", ctypes.c_int32)]
 
 
 class _StatsC(ctypes.Structure):
@@ -240,6 +240,7 @@ def load_library() -> ctypes.CDLL:
         "dvs_sensor_stats": (ctypes.c_int, [_P, ctypes.POINTER(_StatsC)]),
         "dvs_sensor_reset_stats": (ctypes.c_int, [_P]),
         "dvs_scene_view": (ctypes.c_int, [_P, ctypes.POINTER(_CameraC), _P, ctypes.POINTER(_P)]),
+        "dvs_scene_set_traversal": (ctypes.c_int, [_P, _I32, ctypes.POINTER(_I32)]),
         "dvs_batch_create": (ctypes.c_int, [_I32, _I32, ctypes.POINTER(_P)]),
         "dvs_batch_destroy": (ctypes.c_int, [_P]),
         "dvs_batch_add": (ctypes.c_int, [_P, _P, ctypes.POINTER(_PhysicC), _I32, _I32, _P, ctypes.c_double,
@@ -432,6 +433,14 @@ class Scene:
         _check(load_library().dvs_scene_stats(self.handle, ctypes.byref(s)))
         return _stats(s)
 
+    def set_traversal(self, form: int) -> int:
+        """Traversal form of this scene's frame, depth and ray launches
+        (dvs_scene_set_traversal): 1 wave packets, 0 per-lane stacks; returns
+        the form in use."""
+        eff = _I32()
+        _check(load_library().dvs_scene_set_traversal(self.handle, int(form), ctypes.byref(eff)))
+        return eff.value
+
     def view(self, camera: Camera, hapke: Hapke | None = None) -> "Scene":
         """Another camera on this scene's resident geometry (shared BVH and mesh)."""
         out = Scene.__new__(Scene)
@@ -576,10 +585,11 @@ class Sensor:
         _check(load_library().dvs_sensor_drain(self.handle, ctypes.byref(ready)))
         return self._take(ready.value)
 
-    def set_tuning(self, rate_cells: int = 128, trace_threads: int = 128) -> None:
-        """Launch shapes of the adaptive kernels (dvs_sensor_set_tuning);
-        results do not depend on them."""
-        _check(load_library().dvs_sensor_set_tuning(self.handle, ctypes.byref(_TuningC(rate_cells, trace_threads))))
+    def set_tuning(self, rate_cells: int = 128, trace_threads: int = 128, traversal: int = 1) -> None:
+        """Launch shapes and traversal form of the sensor's kernels
+        (dvs_sensor_set_tuning); results do not depend on them."""
+        _check(load_library().dvs_sensor_set_tuning(
+            self.handle, ctypes.byref(_TuningC(rate_cells, trace_threads, traversal))))
 
     def readout_stats(self) -> tuple[int, int]:
         d, p = _I64(), _I64()
@@ -693,15 +703,17 @@ class Batch:
         iv = np.ascontiguousarray(np.atleast_1d(intervals), dtype=INTERVAL_DTYPE)
         _check(load_library().dvs_batch_feed(self.handle, int(segment), _ptr(iv), len(iv)))
 
-    def set_tuning(self, rate_cells: int = 128, trace_threads: int = 128) -> None:
-        """Launch shapes of the adaptive kernels (dvs_batch_set_tuning):
-        photocurrent cells per thread run and trace lanes per block."""
-        _check(load_library().dvs_batch_set_tuning(self.handle, ctypes.byref(_TuningC(rate_cells, trace_threads))))
+    def set_tuning(self, rate_cells: int = 128, trace_threads: int = 128, traversal: int = 1) -> None:
+        """Launch shapes of the adaptive kernels and the traversal form
+        (dvs_batch_set_tuning): photocurrent cells per thread run, trace lanes
+        per block, wave packets (1) or per-lane stacks (0)."""
+        _check(load_library().dvs_batch_set_tuning(
+            self.handle, ctypes.byref(_TuningC(rate_cells, trace_threads, traversal))))
 
     def tuning(self) -> dict:
         t = _TuningC()
         _check(load_library().dvs_batch_get_tuning(self.handle, ctypes.byref(t)))
-        return {"rate_cells": t.rate_cells, "trace_threads": t.trace_threads}
+        return {"rate_cells": t.rate_cells, "trace_threads": t.trace_threads, "traversal": t.traversal}
 
     def feed_trajectory(self, segment: int, trajectory: "Trajectory", times) -> None:
         """Queues the intervals between consecutive `times`, their states

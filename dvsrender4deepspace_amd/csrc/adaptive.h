@@ -122,11 +122,13 @@ hipError_t launch_adapt_cell_frac(int cells, const double *t0, const double *t1,
 // LAYOUT_QUERY item q is item_pixel[q] at poses[q]; LAYOUT_CELL item q is
 // item_pixel[q] at poses[item_pose[q]].  `lds` is the block's LDS for 256
 // lanes; blocks of `threads` lanes (64, 128 or 256) take a proportional share.
+// `wave`: the wave-packet traversal (every segment's scene must allow it),
+// which needs no LDS stack.
 hipError_t launch_adapt_trace(int layout, const SegDev *segs, const int *pixel_begin, int nseg, int samples,
                               long long items, const int *item_pixel, const int *item_pose, const Pose *poses,
                               size_t lds, double *out, int *status, unsigned long long *shadow_count,
                               hipStream_t st, const int *reuse = nullptr, const double *prev_probes = nullptr,
-                              int threads = 256);
+                              int threads = 256, bool wave = false);
 // Also clears clear[0, clear_words) and event_counts[0, pixels).
 hipError_t launch_adapt_init(int pixels, AdaptBufs b, AdaptEntry *entries, int *clear, int clear_words,
                              int *event_counts, hipStream_t st);

@@ -75,8 +75,10 @@ __global__ void adapt_cell_frac(int cells, const double *t0, const double *t1, c
 // and the last e - b items only copy the previous end probe into probe 0, in
 // blocks that trace nothing.  Probe j of pixel p goes to out[p * 5 + j].
 // THREADS lanes per block (64, 128 or 256): THREADS / ss^2 pixel samples,
-// each lane one sub-ray with its traversal stack in LDS at stride THREADS.
-template <int LAYOUT, int THREADS>
+// each lane one sub-ray, with its traversal stack in LDS at stride THREADS
+// (WAVE false) or in the wave-packet traversal (WAVE true, every lane of
+// the block joins it).
+template <int LAYOUT, int THREADS, bool WAVE>
 __global__ __launch_bounds__(THREADS) void adapt_trace(const SegDev *segs, const int *pixel_begin, int nseg,
                                                    long long items, const int *item_pixel, const int *item_pose,
                                                    const Pose *poses, double *out, int *status,
@@ -92,9 +94,9 @@ __global__ __launch_bounds__(THREADS) void adapt_trace(const SegDev *segs, const
   SubSample r{0.0, -1.f, 0, 0};
   long long at = q;  // output index
   bool reused = false;
+  int gp = 0;
+  Pose pose{};
   if (active) {
-    int gp;
-    Pose pose;
     if (LAYOUT == LAYOUT_PROBE) {
       const int si = segment_of<int>(pixel_begin, nseg, (int)(q / 5));
       const int b = pixel_begin[si], n = pixel_begin[si + 1] - b;
@@ -120,11 +122,14 @@ __global__ __launch_bounds__(THREADS) void adapt_trace(const SegDev *segs, const
       gp = item_pixel[q];
       pose = poses[item_pose[q]];
     }
-    const int si = segment_of<int>(pixel_begin, nseg, gp);
-    if (!reused)
-      r = sample_subray<true>(segs[si].scene, gp - pixel_begin[si], pose, sub % samples, sub / samples, stack,
-                              THREADS);
   }
+  const int si = segment_of<int>(pixel_begin, nseg, gp);
+  if (WAVE)
+    r = sample_subray_wave<true>(segs[si].scene, gp - pixel_begin[si], pose, sub % samples, sub / samples,
+                                 active && !reused);
+  else if (active && !reused)
+    r = sample_subray<true>(segs[si].scene, gp - pixel_begin[si], pose, sub % samples, sub / samples, stack,
+                            THREADS);
   const LdsInts lw = (LdsInts)words;  // the lanes' radiances, low then high words
   lw[threadIdx.x] = __double2loint(r.radiance);
   lw[THREADS + threadIdx.x] = __double2hiint(r.radiance);
@@ -803,7 +808,7 @@ hipError_t launch_adapt_cell_frac(int cells, const double *t0, const double *t1,
   return hipGetLastError();
 }
 namespace {
-template <int THREADS>
+template <int THREADS, bool WAVE>
 void adapt_trace_as(int layout, long long blocks, size_t lds, const SegDev *segs, const int *pixel_begin, int nseg,
                     long long items, const int *item_pixel, const int *item_pose, const Pose *poses, double *out,
                     int *status, unsigned long long *shadow_count, hipStream_t st, const int *reuse,
@@ -811,27 +816,40 @@ void adapt_trace_as(int layout, long long blocks, size_t lds, const SegDev *segs
   const dim3 g((unsigned)blocks);
   switch (layout) {
     case LAYOUT_PROBE:
-      adapt_trace<LAYOUT_PROBE, THREADS><<<g, THREADS, lds, st>>>(segs, pixel_begin, nseg, items, item_pixel,
-                                                                    item_pose, poses, out, status, shadow_count,
-                                                                    reuse, prev_probes);
+      adapt_trace<LAYOUT_PROBE, THREADS, WAVE><<<g, THREADS, lds, st>>>(segs, pixel_begin, nseg, items, item_pixel,
+                                                                          item_pose, poses, out, status,
+                                                                          shadow_count, reuse, prev_probes);
       break;
     case LAYOUT_QUERY:
-      adapt_trace<LAYOUT_QUERY, THREADS><<<g, THREADS, lds, st>>>(segs, pixel_begin, nseg, items, item_pixel,
-                                                                    item_pose, poses, out, status, shadow_count,
-                                                                    reuse, prev_probes);
+      adapt_trace<LAYOUT_QUERY, THREADS, WAVE><<<g, THREADS, lds, st>>>(segs, pixel_begin, nseg, items, item_pixel,
+                                                                          item_pose, poses, out, status,
+                                                                          shadow_count, reuse, prev_probes);
       break;
     default:
-      adapt_trace<LAYOUT_CELL, THREADS><<<g, THREADS, lds, st>>>(segs, pixel_begin, nseg, items, item_pixel,
-                                                                   item_pose, poses, out, status, shadow_count,
-                                                                   reuse, prev_probes);
+      adapt_trace<LAYOUT_CELL, THREADS, WAVE><<<g, THREADS, lds, st>>>(segs, pixel_begin, nseg, items, item_pixel,
+                                                                         item_pose, poses, out, status,
+                                                                         shadow_count, reuse, prev_probes);
   }
+}
+template <int THREADS>
+void adapt_trace_form(bool wave, int layout, long long blocks, size_t lds, const SegDev *segs,
+                      const int *pixel_begin, int nseg, long long items, const int *item_pixel,
+                      const int *item_pose, const Pose *poses, double *out, int *status,
+                      unsigned long long *shadow_count, hipStream_t st, const int *reuse,
+                      const double *prev_probes) {
+  if (wave)
+    adapt_trace_as<THREADS, true>(layout, blocks, lds, segs, pixel_begin, nseg, items, item_pixel, item_pose, poses,
+                                  out, status, shadow_count, st, reuse, prev_probes);
+  else
+    adapt_trace_as<THREADS, false>(layout, blocks, lds, segs, pixel_begin, nseg, items, item_pixel, item_pose,
+                                   poses, out, status, shadow_count, st, reuse, prev_probes);
 }
 }  // namespace
 
 hipError_t launch_adapt_trace(int layout, const SegDev *segs, const int *pixel_begin, int nseg, int samples,
                               long long items, const int *item_pixel, const int *item_pose, const Pose *poses,
                               size_t lds, double *out, int *status, unsigned long long *shadow_count,
-                              hipStream_t st, const int *reuse, const double *prev_probes, int threads) {
+                              hipStream_t st, const int *reuse, const double *prev_probes, int threads, bool wave) {
   if (layout == LAYOUT_PROBE && (items % 5 != 0 || !reuse)) return hipErrorInvalidValue;
   if (threads != 64 && threads != 128 && threads != 256) return hipErrorInvalidValue;
   if (samples * samples > threads) threads = 256;  // a block holds at least one pixel sample
@@ -839,18 +857,19 @@ hipError_t launch_adapt_trace(int layout, const SegDev *segs, const int *pixel_b
   const int slots = threads / (samples * samples);
   const long long blocks = (items + slots - 1) / slots;
   if (blocks > kMaxBlocks256) return hipErrorInvalidConfiguration;
-  // lds holds a stack of lds / 1024 levels for 256 lanes: the same levels
-  // for `threads` lanes
-  const size_t bytes = lds / 256 * (size_t)threads;
+  // per-lane stacks: lds holds a stack of lds / 1024 levels for 256 lanes,
+  // the same levels for `threads` lanes; wave packets: the two words of each
+  // lane's radiance hand-off
+  const size_t bytes = wave ? 2 * sizeof(int) * (size_t)threads : lds / 256 * (size_t)threads;
   if (threads == 64)
-    adapt_trace_as<64>(layout, blocks, bytes, segs, pixel_begin, nseg, items, item_pixel, item_pose, poses, out,
-                       status, shadow_count, st, reuse, prev_probes);
+    adapt_trace_form<64>(wave, layout, blocks, bytes, segs, pixel_begin, nseg, items, item_pixel, item_pose, poses,
+                         out, status, shadow_count, st, reuse, prev_probes);
   else if (threads == 128)
-    adapt_trace_as<128>(layout, blocks, bytes, segs, pixel_begin, nseg, items, item_pixel, item_pose, poses, out,
-                        status, shadow_count, st, reuse, prev_probes);
+    adapt_trace_form<128>(wave, layout, blocks, bytes, segs, pixel_begin, nseg, items, item_pixel, item_pose, poses,
+                          out, status, shadow_count, st, reuse, prev_probes);
   else
-    adapt_trace_as<256>(layout, blocks, bytes, segs, pixel_begin, nseg, items, item_pixel, item_pose, poses, out,
-                        status, shadow_count, st, reuse, prev_probes);
+    adapt_trace_form<256>(wave, layout, blocks, bytes, segs, pixel_begin, nseg, items, item_pixel, item_pose, poses,
+                          out, status, shadow_count, st, reuse, prev_probes);
   return hipGetLastError();
 }
 hipError_t launch_adapt_init(int pixels, AdaptBufs b, AdaptEntry *entries, int *clear, int clear_words,

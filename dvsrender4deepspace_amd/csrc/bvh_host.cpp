@@ -318,19 +318,36 @@ SoaBvh relayout(const dvs_bvh_node *nodes, int64_t count, int32_t root, const fl
     return split(ids, n.start, n.start + n.count, depth);
   };
   // Reference tree in pre-order (left first); its internal nodes become pair
-  // records holding both children's reference boxes.
-  std::function<int32_t(int32_t, int)> emit = [&](int32_t node, int depth) -> int32_t {
+  // records holding both children's reference boxes.  Word 14 of a pair
+  // holds the first reference triangle index of its right subtree: where
+  // every subtree covers one index range and a left subtree's range precedes
+  // its right one's (the SAH build partitions its index range in place, so
+  // it always does), that index tells on which side of the pair a reference
+  // leaf lies (`ordered_ranges`).
+  out.ordered_ranges = true;
+  std::function<int32_t(int32_t, int, int32_t &, int32_t &)> emit = [&](int32_t node, int depth, int32_t &lo,
+                                                                         int32_t &hi) -> int32_t {
     const dvs_bvh_node &n = nodes[node];
-    if (n.count > 0) return leaf_ref(n, depth);
+    if (n.count > 0) {
+      lo = n.start;
+      hi = n.start + n.count;
+      return leaf_ref(n, depth);
+    }
     const int32_t pair = new_pair();
-    const int32_t l = emit(n.left, depth + 1);
-    const int32_t r = emit(n.right, depth + 1);
+    int32_t llo, lhi, rlo, rhi;
+    const int32_t l = emit(n.left, depth + 1, llo, lhi);
+    const int32_t r = emit(n.right, depth + 1, rlo, rhi);
     set_child(pair, 0, nodes[n.left].min, nodes[n.left].max, l);
     set_child(pair, 1, nodes[n.right].min, nodes[n.right].max, r);
+    if (lhi != rlo) out.ordered_ranges = false;
+    std::memcpy(&out.pairs[16 * static_cast<size_t>(pair) + 14], &rlo, 4);
+    lo = llo;
+    hi = rhi;
     return pair;
   };
   out.triangles.reserve(12 * static_cast<size_t>(triangle_count));
-  out.root_ref = emit(root, 0);
+  int32_t all_lo = 0, all_hi = 0;
+  out.root_ref = emit(root, 0, all_lo, all_hi);
   out.internal_nodes = static_cast<int64_t>(out.pairs.size() / 16);
   out.leaf_count = static_cast<int64_t>(out.leaf_ranges.size() / 2);
   return out;

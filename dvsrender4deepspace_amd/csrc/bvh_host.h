@@ -42,6 +42,9 @@ struct SoaBvh {
   std::vector<float> triangles;
   int32_t root_ref = 0;
   int32_t max_depth = 0;  // deepest leaf below the root
+  // every reference subtree covers one triangle index range, left before
+  // right (word 14 of each reference pair: the right subtree's first index)
+  bool ordered_ranges = false;
   int64_t internal_nodes = 0, leaf_count = 0;
 };
 

@@ -44,6 +44,10 @@ struct DevScene {
   int width, height, samples;
   float fx, fy, cx, cy;
   dvs_hapke hapke;
+  // 1: the wave-packet traversal may serve this tree (every reference
+  // subtree covers one triangle index range, left before right, and word 14
+  // of each pair holds the right subtree's first index)
+  int packet;
 };
 
 // Pose used by the ray kernels: the interpolated OpticalState fields rays read.
@@ -125,9 +129,16 @@ __device__ __forceinline__ f3 inverse_direction(f3 d) {  // device_ray.hpp:6-12
 typedef const __attribute__((address_space(1))) float *GlobalFloats;
 typedef const __attribute__((address_space(1))) int *GlobalInts;
 typedef __attribute__((address_space(3))) int *LdsInts;
-// float4 i of a record (16-byte aligned), generic or global.
+// Wave-uniform reads of the scene: through the constant address space the
+// compiler issues scalar loads (one per wave) for a uniform address.
+typedef const __attribute__((address_space(4))) float *ConstFloats;
+typedef const __attribute__((address_space(4))) int *ConstInts;
+// float4 i of a record (16-byte aligned), generic, global or constant.
 __device__ __forceinline__ float4 record4(const float4 *rec, int i) { return rec[i]; }
 __device__ __forceinline__ float4 record4(GlobalFloats rec, int i) {
+  return make_float4(rec[4 * i], rec[4 * i + 1], rec[4 * i + 2], rec[4 * i + 3]);
+}
+__device__ __forceinline__ float4 record4(ConstFloats rec, int i) {
   return make_float4(rec[4 * i], rec[4 * i + 1], rec[4 * i + 2], rec[4 * i + 3]);
 }
 
@@ -289,6 +300,218 @@ __device__ inline bool trace_any(const DevScene &s, f3 origin, f3 dir, int self,
   return false;
 }
 
+// ---------------------------------------------------------------------------
+// Wave-packet traversal.  The 64 lanes of a wave walk the tree together: a
+// node's record is read once per wave by scalar loads (its address is
+// wave-uniform), every lane runs the slab and Moller-Trumbore arithmetic of
+// the per-lane traversal above on its own ray, and a child is entered by the
+// lanes whose own test passed.  The stack is wave-uniform: entry k (a node
+// reference and the 64-bit mask of the lanes that entered it) sits in lane
+// k of three registers, written by v_writelane and read by v_readlane, so
+// it costs no LDS.  A lane tests a box only where its own tests let it in,
+// with its own current nearest distance, and a popped entry is not tested
+// again, as in the per-lane traversal.
+//
+// Why the results are those of the per-lane (reference) order:
+// * any hit: whether some triangle other than `self` in a leaf under boxes
+//   the ray passes (t_min and t_max fixed) hits inside (t_min, t_max) does
+//   not depend on the order of the visits;
+// * nearest hit: a box holding a hit at distance t is entered whenever the
+//   lane's nearest distance is at least t, so every order finds the same
+//   nearest distance.  At equal distances the reference keeps the first
+//   triangle of its near-first order: within one reference leaf the lower
+//   index (its sequential scan), and across reference leaves the one in the
+//   subtree entered first at the two leaves' lowest common ancestor, the
+//   child with the smaller entry distance, the right one at a tie
+//   (device_ray.hpp:120-127).  earlier_leaf() finds that ancestor by the
+//   first reference index of each pair's right subtree (pair word 14) and
+//   tests its two boxes for the lane's ray.
+// The stack holds at most one entry per level of the current path, and the
+// scene's stack depth is its deepest leaf (at most 64), so no traversal
+// overflows; an overflow would still mark the wave's lanes invalid.
+// ---------------------------------------------------------------------------
+constexpr int kWaveStackDepth = 64;  // dvs_scene_create rejects deeper trees
+__device__ __forceinline__ int uniform_int(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ unsigned long long lane_bit() { return 1ull << __lane_id(); }
+
+// The wave-uniform stack in lanes of three registers (lane sp takes entry
+// sp by a compare and select; v_readlane reads it back).
+struct WaveStack {
+  int ref = 0, lo = 0, hi = 0;
+  __device__ __forceinline__ void push(int sp, int node, unsigned long long mask) {
+    const bool here = (int)__lane_id() == sp;
+    ref = here ? node : ref;
+    lo = here ? (int)(unsigned)mask : lo;
+    hi = here ? (int)(unsigned)(mask >> 32) : hi;
+  }
+  __device__ __forceinline__ int node(int sp) const { return __builtin_amdgcn_readlane(ref, sp); }
+  __device__ __forceinline__ unsigned long long mask(int sp) const {
+    return ((unsigned long long)(unsigned)__builtin_amdgcn_readlane(hi, sp) << 32) |
+           (unsigned)__builtin_amdgcn_readlane(lo, sp);
+  }
+};
+
+// One geometry, wave-uniform.
+struct WaveGeom {
+  ConstFloats pairs, triangles;
+  ConstInts leaves;
+  int root_ref, depth;
+};
+__device__ __forceinline__ unsigned long long lane_u64(unsigned long long x, int lane) {
+  return ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(unsigned)(x >> 32), lane) << 32) |
+         (unsigned)__builtin_amdgcn_readlane((int)(unsigned)x, lane);
+}
+// Every lane's geometry is the scene `s` it traces (per lane); the lanes of
+// `todo` are served geometry by geometry, each group by one walk.
+template <class F>
+__device__ __forceinline__ void for_each_geometry(const DevScene &s, unsigned long long todo, F &&walk) {
+  while (todo) {
+    const int leader = __builtin_ctzll(todo);
+    const unsigned long long p = (unsigned long long)(uintptr_t)s.pairs;
+    const unsigned long long leader_pairs = lane_u64(p, leader);
+    const unsigned long long group = __ballot(p == leader_pairs) & todo;
+    WaveGeom g;
+    g.pairs = (ConstFloats)(uintptr_t)leader_pairs;
+    g.triangles = (ConstFloats)(uintptr_t)lane_u64((unsigned long long)(uintptr_t)s.triangles, leader);
+    g.leaves = (ConstInts)(uintptr_t)lane_u64((unsigned long long)(uintptr_t)s.leaves, leader);
+    g.root_ref = __builtin_amdgcn_readlane(s.root_ref, leader);
+    g.depth = __builtin_amdgcn_readlane(s.stack_depth, leader);
+    walk(g, group);
+    todo &= ~group;
+  }
+}
+
+// Whether reference leaf lo_a comes before reference leaf lo_b (distinct
+// leaves, given by their first reference triangle index) in the near-first
+// order of the ray (origin, idir): the side entered first at their lowest
+// common ancestor.  Per lane, only at exact distance ties.
+__device__ inline bool earlier_leaf(const WaveGeom &g, int lo_a, int lo_b, f3 origin, f3 idir) {
+  const GlobalFloats pairs = (GlobalFloats)(const float *)g.pairs;
+  int ref = g.root_ref;
+  for (int k = 0; k < kWaveStackDepth && ref >= 0; ++k) {
+    const GlobalFloats rec = pairs + 16 * (size_t)ref;
+    const int split = __float_as_int(rec[14]);
+    const bool a_left = lo_a < split, b_left = lo_b < split;
+    if (a_left == b_left) {
+      ref = __float_as_int(rec[a_left ? 12 : 13]);
+      continue;
+    }
+    float d0, d1;
+    int l, r;
+    test_pair(rec, origin, idir, kRayEps, 1e30f, d0, d1, l, r);
+    return a_left == (d0 < d1);
+  }
+  return false;
+}
+
+// The side of a pair most of the lanes entering both sides would take first.
+__device__ __forceinline__ bool wave_left_first(bool mine, int mask, float d0, float d1, unsigned long long m0,
+                                                unsigned long long m1) {
+  const unsigned long long lf = __ballot(mine && mask == 3 && d0 < d1);
+  return 2 * __popcll(lf) >= __popcll(m0 & m1);
+}
+
+// Nearest hit of the lanes of `group` (trace_nearest's result per lane);
+// the group's lanes are added to `overflow` if the stack overflows.
+__device__ inline void wave_nearest(const WaveGeom &g, unsigned long long group, f3 origin, f3 ray, f3 idir, Hit &h,
+                                    unsigned long long &overflow) {
+  WaveStack st;
+  float closest = 1e30f;
+  int best_lo = -1;
+  int ref = g.root_ref, sp = 0;
+  unsigned long long m = group;
+  for (;;) {
+    const bool mine = (m & lane_bit()) != 0;
+    if (ref < 0) {
+      const int2 leaf = make_int2(g.leaves[2 * ~ref], g.leaves[2 * ~ref + 1]);
+      for (int i = 0; i < leaf.y; ++i) {
+        const ConstFloats rec = g.triangles + 12 * (leaf.x + i);
+        float t, u, v;
+        float4 meta;
+        if (mine && intersect(rec, ray, origin, t, u, v, meta)) {
+          const int id = __float_as_int(meta.y), lo = __float_as_int(meta.z);
+          bool take = t < closest;
+          if (!take && t == closest && best_lo >= 0)
+            take = lo == best_lo ? id < h.tri : earlier_leaf(g, lo, best_lo, origin, idir);
+          if (take) {
+            closest = t; best_lo = lo;
+            h.hit = 1; h.tri = id; h.rec = leaf.x + i; h.t = t; h.u = u; h.v = v;
+          }
+        }
+      }
+    } else {
+      float d0 = 0, d1 = 0;
+      int l, r;
+      const int mask = test_pair(g.pairs + 16 * ref, origin, idir, kRayEps, closest, d0, d1, l, r);
+      const unsigned long long m0 = __ballot(mine && (mask & 1)), m1 = __ballot(mine && (mask & 2));
+      if (m0 && m1) {
+        if (sp >= g.depth) { overflow |= group; return; }
+        const bool left_first = wave_left_first(mine, mask, d0, d1, m0, m1);
+        st.push(sp++, left_first ? r : l, left_first ? m1 : m0);
+        ref = left_first ? l : r;
+        m = left_first ? m0 : m1;
+        continue;
+      }
+      if (m0 | m1) { ref = m0 ? l : r; m = m0 | m1; continue; }
+    }
+    if (sp == 0) break;
+    --sp;
+    ref = st.node(sp);
+    m = st.mask(sp);
+  }
+}
+
+// Any hit toward `dir` of the lanes of `group`, skipping each lane's `self`
+// triangle (trace_any's result per lane); returns the occluded lanes.  The
+// group's lanes are added to `overflow` if the stack overflows.
+__device__ inline unsigned long long wave_any(const WaveGeom &g, unsigned long long group, f3 origin, f3 dir, f3 idir,
+                                              int self, float t_min, float t_max, unsigned long long &overflow) {
+  WaveStack st;
+  unsigned long long done = 0;
+  int ref = g.root_ref, sp = 0;
+  unsigned long long m = group;
+  for (;;) {
+    const bool mine = (m & lane_bit()) != 0;
+    if (ref < 0) {
+      const int2 leaf = make_int2(g.leaves[2 * ~ref], g.leaves[2 * ~ref + 1]);
+      bool hit = false;
+      for (int i = 0; i < leaf.y; ++i) {
+        const ConstFloats rec = g.triangles + 12 * (leaf.x + i);
+        float t, u, v;
+        float4 meta;
+        // record word 9: the reference triangle index
+        if (mine && !hit && __float_as_int(rec[9]) != self && intersect(rec, dir, origin, t, u, v, meta) &&
+            t > t_min && t < t_max)
+          hit = true;
+      }
+      done |= __ballot(hit);
+    } else {
+      float d0, d1;
+      int l, r;
+      const int mask = test_pair(g.pairs + 16 * ref, origin, idir, t_min, t_max, d0, d1, l, r);
+      const unsigned long long m0 = __ballot(mine && (mask & 1)), m1 = __ballot(mine && (mask & 2));
+      if (m0 && m1) {
+        if (sp >= g.depth) { overflow |= group; return group; }
+        const bool left_first = wave_left_first(mine, mask, d0, d1, m0, m1);
+        st.push(sp++, left_first ? r : l, left_first ? m1 : m0);
+        ref = left_first ? l : r;
+        m = left_first ? m0 : m1;
+        continue;
+      }
+      if (m0 | m1) { ref = m0 ? l : r; m = m0 | m1; continue; }
+    }
+    // the next entry with a lane still looking
+    m = 0;
+    while (sp > 0 && !m) {
+      --sp;
+      ref = st.node(sp);
+      m = st.mask(sp) & ~done;
+    }
+    if (!m) break;
+  }
+  return done;
+}
+
 // shadow_origin (device_optics.cuh:37-51)
 template <class Rec>
 __device__ inline f3 shadow_origin(Rec rec, float u, float v, f3 light) {
@@ -433,6 +656,58 @@ struct SubSample {
   int invalid;   // traversal overflow
   int shadow;    // a shadow ray was traced
 };
+// The sub-ray through sub-sample (sx, sy) of pixel `pixel` (sample_subray,
+// device_optics.cuh:58-63).
+struct SubRay {
+  float px, py;
+  f3 ray, origin;
+};
+__device__ __forceinline__ SubRay subray_of(const DevScene &s, int pixel, const Pose &pose, int sx, int sy) {
+  SubRay r;
+  const int ss = s.samples;
+  r.px = (pixel % s.width + (sx + .5f) / ss - .5f - s.cx) / s.fx;
+  r.py = (pixel / s.width + (sy + .5f) / ss - .5f - s.cy) / s.fy;
+  r.ray = normalize(rotate_ray(pose, mk(r.px, -r.py, -1.f)));
+  r.origin = mk(pose.px, pose.py, pose.pz);
+  return r;
+}
+// The hit's interpolated normal and the light direction; true when the
+// surface faces both the light and the camera, i.e. a shadow ray is traced
+// from `so` (device_optics.cuh:64-70).
+struct Surface {
+  f3 normal, light, so;
+  uint32_t i0, i1, i2;
+  float w;
+};
+__device__ inline bool lit_surface(const DevScene &s, const Hit &hit, const Pose &pose, f3 ray, Surface &f) {
+  const GlobalInts indices = (GlobalInts)s.indices;
+  const GlobalFloats normals = (GlobalFloats)s.normals;
+  f.i0 = (uint32_t)indices[3 * (size_t)hit.tri];
+  f.i1 = (uint32_t)indices[3 * (size_t)hit.tri + 1];
+  f.i2 = (uint32_t)indices[3 * (size_t)hit.tri + 2];
+  f.w = 1 - hit.u - hit.v;
+  const f3 n0 = mk(normals[3 * (size_t)f.i0], normals[3 * (size_t)f.i0 + 1], normals[3 * (size_t)f.i0 + 2]);
+  const f3 n1 = mk(normals[3 * (size_t)f.i1], normals[3 * (size_t)f.i1 + 1], normals[3 * (size_t)f.i1 + 2]);
+  const f3 n2 = mk(normals[3 * (size_t)f.i2], normals[3 * (size_t)f.i2 + 1], normals[3 * (size_t)f.i2 + 2]);
+  f.normal = normalize(add(add(scale(n0, f.w), scale(n1, hit.u)), scale(n2, hit.v)));
+  f.light = normalize(mk(pose.lx, pose.ly, pose.lz));
+  if (dot(f.normal, f.light) <= 0 || dot(f.normal, neg(ray)) <= 0) return false;
+  f.so = shadow_origin((GlobalFloats)s.triangles + 12 * (size_t)hit.rec, hit.u, hit.v, f.light);
+  return true;
+}
+// Radiance of an unshadowed lit hit: albedo, Hapke, intensity and cos^4
+// (device_optics.cuh:71-74, material_albedo :31-33).
+__device__ inline double lit_radiance(const DevScene &s, const Hit &hit, const Surface &f, const SubRay &r,
+                                      const Pose &pose) {
+  const float albedo = s.albedos ? material_albedo(s.albedos[f.i0], s.hapke.w) * f.w +
+                                       material_albedo(s.albedos[f.i1], s.hapke.w) * hit.u +
+                                       material_albedo(s.albedos[f.i2], s.hapke.w) * hit.v
+                                 : s.hapke.w;
+  const double a = 1.0 + (double)(r.px * r.px) + (double)(r.py * r.py);
+  const double cos4 = 1.0 / (a * a);
+  return (double)fmaxf(0.f, hapke_directional(f.normal, f.light, neg(r.ray), albedo, s.hapke) * pose.intensity) * cos4;
+}
+
 template <bool SHADE, class Count = NoCount>
 __device__ inline SubSample sample_subray(const DevScene &s, int pixel, const Pose &pose, int sx, int sy,
                                           int *stack, int stride, Count *primary = nullptr,
@@ -441,37 +716,51 @@ __device__ inline SubSample sample_subray(const DevScene &s, int pixel, const Po
   Count &cp = primary ? *primary : unused_primary;
   Count &cs = shadow ? *shadow : unused_shadow;
   SubSample out{0.0, -1.f, 0, 0};
-  const int ss = s.samples;
-  const float px = (pixel % s.width + (sx + .5f) / ss - .5f - s.cx) / s.fx;
-  const float py = (pixel / s.width + (sy + .5f) / ss - .5f - s.cy) / s.fy;
-  const f3 ray = normalize(rotate_ray(pose, mk(px, -py, -1.f)));
-  const f3 origin = mk(pose.px, pose.py, pose.pz);
-  const Hit hit = trace_nearest(s, origin, ray, stack, stride, cp);
+  const SubRay r = subray_of(s, pixel, pose, sx, sy);
+  const Hit hit = trace_nearest(s, r.origin, r.ray, stack, stride, cp);
   if (hit.hit < 0) { out.invalid = 1; return out; }
   if (!hit.hit) return out;
   out.depth = hit.t;
   if (!SHADE) return out;
-  const GlobalInts indices = (GlobalInts)s.indices;
-  const GlobalFloats normals = (GlobalFloats)s.normals;
-  const uint32_t i0 = (uint32_t)indices[3 * (size_t)hit.tri], i1 = (uint32_t)indices[3 * (size_t)hit.tri + 1],
-                 i2 = (uint32_t)indices[3 * (size_t)hit.tri + 2];
-  const float w = 1 - hit.u - hit.v;
-  const f3 n0 = mk(normals[3 * (size_t)i0], normals[3 * (size_t)i0 + 1], normals[3 * (size_t)i0 + 2]);
-  const f3 n1 = mk(normals[3 * (size_t)i1], normals[3 * (size_t)i1 + 1], normals[3 * (size_t)i1 + 2]);
-  const f3 n2 = mk(normals[3 * (size_t)i2], normals[3 * (size_t)i2 + 1], normals[3 * (size_t)i2 + 2]);
-  const f3 normal = normalize(add(add(scale(n0, w), scale(n1, hit.u)), scale(n2, hit.v)));
-  const f3 light = normalize(mk(pose.lx, pose.ly, pose.lz));
-  if (dot(normal, light) <= 0 || dot(normal, neg(ray)) <= 0) return out;
-  const f3 so = shadow_origin((GlobalFloats)s.triangles + 12 * (size_t)hit.rec, hit.u, hit.v, light);
+  Surface f;
+  if (!lit_surface(s, hit, pose, r.ray, f)) return out;
   out.shadow = 1;
-  if (trace_any(s, so, light, hit.tri, kRayEps, 1e30f, stack, stride, cs)) return out;
-  const float albedo = s.albedos ? material_albedo(s.albedos[i0], s.hapke.w) * w +
-                                       material_albedo(s.albedos[i1], s.hapke.w) * hit.u +
-                                       material_albedo(s.albedos[i2], s.hapke.w) * hit.v
-                                 : s.hapke.w;
-  const double a = 1.0 + (double)(px * px) + (double)(py * py);
-  const double cos4 = 1.0 / (a * a);
-  out.radiance = (double)fmaxf(0.f, hapke_directional(normal, light, neg(ray), albedo, s.hapke) * pose.intensity) * cos4;
+  if (trace_any(s, f.so, f.light, hit.tri, kRayEps, 1e30f, stack, stride, cs)) return out;
+  out.radiance = lit_radiance(s, hit, f, r, pose);
+  return out;
+}
+
+// The same sub-ray sample by the wave-packet traversal.  Every lane of the
+// wave calls it (`active` marks the lanes with a sub-ray); the lanes' scenes
+// may differ.  A traversal overflow (which the stack depth rules out) marks
+// the sample invalid, also for a shadow ray.
+template <bool SHADE>
+__device__ inline SubSample sample_subray_wave(const DevScene &s, int pixel, const Pose &pose, int sx, int sy,
+                                               bool active) {
+  SubSample out{0.0, -1.f, 0, 0};
+  const SubRay r = subray_of(s, pixel, pose, sx, sy);
+  const f3 idir = inverse_direction(r.ray);
+  Hit hit{0, -1, -1, 0.f, 0.f, 0.f};
+  unsigned long long overflow = 0;
+  for_each_geometry(s, __ballot(active), [&](const WaveGeom &g, unsigned long long group) {
+    wave_nearest(g, group, r.origin, r.ray, idir, hit, overflow);
+  });
+  const bool over = (overflow & lane_bit()) != 0;
+  if (active && over) out.invalid = 1;
+  const bool surface = active && !over && hit.hit > 0;
+  if (surface) out.depth = hit.t;
+  if (!SHADE) return out;
+  Surface f{};
+  const bool need = surface && lit_surface(s, hit, pose, r.ray, f);
+  unsigned long long occluded = 0, shadow_overflow = 0;
+  const f3 light_idir = inverse_direction(f.light);
+  for_each_geometry(s, __ballot(need), [&](const WaveGeom &g, unsigned long long group) {
+    occluded |= wave_any(g, group, f.so, f.light, light_idir, hit.tri, kRayEps, 1e30f, shadow_overflow);
+  });
+  if (!need) return out;
+  out.shadow = 1;
+  if (shadow_overflow & lane_bit()) { out.invalid = 1; return out; }
+  if (!(occluded & lane_bit())) out.radiance = lit_radiance(s, hit, f, r, pose);
   return out;
 }
 

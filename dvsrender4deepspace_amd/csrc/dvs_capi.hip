@@ -49,6 +49,7 @@ DevScene make_view(const Geometry &g, const dvs_camera &camera, const dvs_hapke 
   v.samples = camera.samples;
   v.fx = camera.fx; v.fy = camera.fy; v.cx = camera.cx; v.cy = camera.cy;
   v.hapke = hapke;
+  v.packet = g.ordered_ranges ? 1 : 0;
   return v;
 }
 }  // namespace
@@ -190,6 +191,7 @@ int dvs_scene_create_ex(const dvs_bvh_node *nodes, int64_t node_count, int32_t r
     if (albedos) g->albedos.upload(albedos, (size_t)vertex_count, "vertex albedo", s->stream);
     hip_check(hipStreamSynchronize(s->stream), "scene upload");
     g->has_albedo = albedos != nullptr;
+    g->ordered_ranges = soa.ordered_ranges;
     g->root_ref = soa.root_ref;
     g->stack_depth = std::max(1, soa.max_depth);
     g->internal_nodes = soa.internal_nodes;
@@ -213,6 +215,14 @@ int dvs_scene_view(dvs_scene base, const dvs_camera *camera, const dvs_hapke *ha
     s->geometry = base->geometry;
     s->view = make_view(*s->geometry, *camera, hapke ? *hapke : base->view.hapke);
     *out = s.release();
+  });
+}
+
+int dvs_scene_set_traversal(dvs_scene s, int32_t form, int32_t *effective) {
+  return guarded([&] {
+    if (!s || (form != 0 && form != 1)) fail(DVS_E_ARGUMENT, "invalid traversal form");
+    s->view.packet = form == 1 && s->geometry->ordered_ranges ? 1 : 0;
+    if (effective) *effective = s->view.packet;
   });
 }
 

@@ -37,7 +37,7 @@ __global__ void state_pose_kernel(int n, const dvs_optical_state *states, Pose *
   if (i < n) poses[i] = pose_of(states[i]);
 }
 
-template <int MODE>
+template <int MODE, bool WAVE>
 __global__ __launch_bounds__(256) void trace_kernel(DevScene s, const Pose *poses, int cells, int npix,
                                                     long long items, const int *item_pixel, double *rad,
                                                     float *img, float *dep, int *status,
@@ -54,8 +54,8 @@ __global__ __launch_bounds__(256) void trace_kernel(DevScene s, const Pose *pose
   const bool active = slot < slots && q < items;
   SubSample r{0.0, -1.f, 0, 0};
   long long out_index = 0;
+  int pixel = 0, pose_index = 0;
   if (active) {
-    int pixel, pose_index;
     if (MODE == MODE_CELLS) {
       pixel = (int)(q / cells);
       pose_index = (int)(q - (long long)pixel * cells);
@@ -69,6 +69,11 @@ __global__ __launch_bounds__(256) void trace_kernel(DevScene s, const Pose *pose
       pose_index = (int)q;
       out_index = q;
     }
+  }
+  if (WAVE) {
+    const Pose pose = poses[pose_index];
+    r = sample_subray_wave<MODE != MODE_DEPTH>(s, pixel, pose, sub % s.samples, sub / s.samples, active);
+  } else if (active) {
     const Pose pose = poses[pose_index];
     r = sample_subray<MODE != MODE_DEPTH>(s, pixel, pose, sub % s.samples, sub / s.samples, stack, 256);
   }
@@ -148,6 +153,7 @@ __global__ void init_kernel(int npix, const double *radiance, PhotoParams photo,
 // Radiance of every (pixel, circuit cell) of every segment: the optical half
 // of generate_pixels (src/cuda/event_sensor.cu:39-42).  Items of one block
 // are consecutive cells of one pixel (temporally coherent rays).
+template <bool WAVE>
 __global__ __launch_bounds__(256) void trace_cells_multi(const SegDev *segs, const long long *block_begin, int nseg,
                                                          int *status, unsigned long long *shadow_count) {
   extern __shared__ double lds_words[];
@@ -162,14 +168,18 @@ __global__ __launch_bounds__(256) void trace_cells_multi(const SegDev *segs, con
   const int cells = g.cells, npix = g.npix;
   const bool active = slot < slots && q < (long long)npix * cells;
   SubSample r{0.0, -1.f, 0, 0};
-  int pixel = 0, cell = 0;
+  int pixel = 0, cell = 0, px = 0;
   if (active) {
     pixel = (int)(q / cells);
     cell = (int)(q - (long long)pixel * cells);
-    const Pose pose = g.poses[cell];
-    const int px = g.pixel_list ? g.pixel_list[pixel] : pixel;
-    r = sample_subray<true>(s, px, pose, sub % s.samples, sub / s.samples, stack, 256);
+    px = g.pixel_list ? g.pixel_list[pixel] : pixel;
   }
+  // a block's lanes belong to one segment: the pose of cell 0 stands in for
+  // the lanes without a sample
+  if (WAVE)
+    r = sample_subray_wave<true>(s, px, g.poses[cell], sub % s.samples, sub / s.samples, active);
+  else if (active)
+    r = sample_subray<true>(s, px, g.poses[cell], sub % s.samples, sub / s.samples, stack, 256);
   const LdsInts lw = (LdsInts)words;  // the lanes' radiances, low then high words
   lw[threadIdx.x] = __double2loint(r.radiance);
   lw[256 + threadIdx.x] = __double2hiint(r.radiance);
@@ -318,6 +328,7 @@ __global__ void event_poses_multi(const SegDev *segs, int events, const double *
 
 // Event-time depth (sample_pixel at the event time, event_sensor.cu:55-58):
 // primary rays only, mean hit range of the sub-rays, -1 if none hit.
+template <bool WAVE>
 __global__ __launch_bounds__(256) void event_depth_multi(const SegDev *segs, int events, int samples, const int *seg,
                                                          const int *pixel, const Pose *poses, float *depth,
                                                          int *status) {
@@ -329,7 +340,10 @@ __global__ __launch_bounds__(256) void event_depth_multi(const SegDev *segs, int
   const long long e = (long long)blockIdx.x * slots + slot;
   const bool active = slot < slots && e < events;
   SubSample r{0.0, -1.f, 0, 0};
-  if (active) {
+  if (WAVE) {
+    const long long k = active ? e : 0;  // lanes without an event stand in for event 0
+    r = sample_subray_wave<false>(segs[seg[k]].scene, pixel[k], poses[k], sub % samples, sub / samples, active);
+  } else if (active) {
     const SegDev &g = segs[seg[e]];
     r = sample_subray<false>(g.scene, pixel[e], poses[e], sub % samples, sub / samples, stack, 256);
   }
@@ -473,12 +487,37 @@ __global__ void noise_kernel(uint64_t seed, int64_t pixels, double lambda, int d
     if (normal) normal[i * draws + d] = reset.normal();
   }
 }
+template <bool WAVE>
 __global__ __launch_bounds__(256) void rays_kernel(DevScene s, int64_t n, const float *o, const float *d,
                                                    const int *self, int shadow, int *hit, int *tri, float *t,
                                                    float *u, float *v) {
   extern __shared__ double lds_words[];
   int *stack = reinterpret_cast<int *>(lds_words) + threadIdx.x;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (WAVE) {
+    const bool active = i < n;
+    const int64_t k = active ? i : 0;
+    const f3 origin = mk(o[3 * k], o[3 * k + 1], o[3 * k + 2]);
+    const f3 dir = mk(d[3 * k], d[3 * k + 1], d[3 * k + 2]);
+    const f3 idir = inverse_direction(dir);
+    unsigned long long overflow = 0;
+    if (shadow) {
+      const int me = self[k];
+      unsigned long long occluded = 0;
+      for_each_geometry(s, __ballot(active), [&](const WaveGeom &g, unsigned long long group) {
+        occluded |= wave_any(g, group, origin, dir, idir, me, kRayEps, 1e30f, overflow);
+      });
+      if (active) hit[i] = (occluded & lane_bit()) ? 1 : 0;
+      return;
+    }
+    Hit h{0, -1, -1, 0.f, 0.f, 0.f};
+    for_each_geometry(s, __ballot(active), [&](const WaveGeom &g, unsigned long long group) {
+      wave_nearest(g, group, origin, dir, idir, h, overflow);
+    });
+    if (overflow & lane_bit()) h.hit = -1;
+    if (active) { hit[i] = h.hit; tri[i] = h.tri; t[i] = h.t; u[i] = h.u; v[i] = h.v; }
+    return;
+  }
   if (i >= n) return;
   const f3 origin = mk(o[3 * i], o[3 * i + 1], o[3 * i + 2]);
   const f3 dir = mk(d[3 * i], d[3 * i + 1], d[3 * i + 2]);
@@ -547,19 +586,33 @@ hipError_t launch_trace(int mode, const DevScene &s, const Pose *poses, int cell
   const int R = s.samples * s.samples, slots = 256 / R;
   const long long blocks = (items + slots - 1) / slots;
   if (blocks > kMaxBlocks256) return hipErrorInvalidConfiguration;
-  const size_t lds = trace_lds_bytes(s);
-  switch (mode) {
-    case MODE_CELLS:
-      trace_kernel<MODE_CELLS><<<(unsigned)blocks, 256, lds, st>>>(s, poses, cells, npix, items, item_pixel, rad,
-                                                                 img, dep, status, shadow_count);
+  // wave packets: only the three words of each lane's hand-off
+  const size_t lds = s.packet ? 3 * 256 * sizeof(int) : trace_lds_bytes(s);
+  const unsigned b = (unsigned)blocks;
+  switch (mode * 2 + (s.packet ? 1 : 0)) {
+    case MODE_CELLS * 2:
+      trace_kernel<MODE_CELLS, false><<<b, 256, lds, st>>>(s, poses, cells, npix, items, item_pixel, rad, img, dep,
+                                                          status, shadow_count);
       break;
-    case MODE_FRAME:
-      trace_kernel<MODE_FRAME><<<(unsigned)blocks, 256, lds, st>>>(s, poses, cells, npix, items, item_pixel, rad,
-                                                                 img, dep, status, shadow_count);
+    case MODE_CELLS * 2 + 1:
+      trace_kernel<MODE_CELLS, true><<<b, 256, lds, st>>>(s, poses, cells, npix, items, item_pixel, rad, img, dep,
+                                                         status, shadow_count);
+      break;
+    case MODE_FRAME * 2:
+      trace_kernel<MODE_FRAME, false><<<b, 256, lds, st>>>(s, poses, cells, npix, items, item_pixel, rad, img, dep,
+                                                          status, shadow_count);
+      break;
+    case MODE_FRAME * 2 + 1:
+      trace_kernel<MODE_FRAME, true><<<b, 256, lds, st>>>(s, poses, cells, npix, items, item_pixel, rad, img, dep,
+                                                         status, shadow_count);
+      break;
+    case MODE_DEPTH * 2:
+      trace_kernel<MODE_DEPTH, false><<<b, 256, lds, st>>>(s, poses, cells, npix, items, item_pixel, rad, img, dep,
+                                                          status, shadow_count);
       break;
     default:
-      trace_kernel<MODE_DEPTH><<<(unsigned)blocks, 256, lds, st>>>(s, poses, cells, npix, items, item_pixel, rad,
-                                                                 img, dep, status, shadow_count);
+      trace_kernel<MODE_DEPTH, true><<<b, 256, lds, st>>>(s, poses, cells, npix, items, item_pixel, rad, img, dep,
+                                                         status, shadow_count);
   }
   return hipGetLastError();
 }
@@ -571,10 +624,15 @@ hipError_t launch_init(int npix, const double *radiance, const PhotoParams &phot
   return hipGetLastError();
 }
 hipError_t launch_trace_cells_multi(const SegDev *segs, const long long *block_begin, int nseg, long long blocks,
-                                    size_t lds, int *status, unsigned long long *shadow_count, hipStream_t st) {
+                                    size_t lds, int *status, unsigned long long *shadow_count, hipStream_t st,
+                                    bool wave) {
   if (blocks <= 0) return hipSuccess;
   if (blocks > kMaxBlocks256) return hipErrorInvalidConfiguration;
-  trace_cells_multi<<<(unsigned)blocks, 256, lds, st>>>(segs, block_begin, nseg, status, shadow_count);
+  if (wave)  // the two words of each lane's radiance hand-off
+    trace_cells_multi<true><<<(unsigned)blocks, 256, 2 * 256 * sizeof(int), st>>>(segs, block_begin, nseg, status,
+                                                                                  shadow_count);
+  else
+    trace_cells_multi<false><<<(unsigned)blocks, 256, lds, st>>>(segs, block_begin, nseg, status, shadow_count);
   return hipGetLastError();
 }
 hipError_t launch_cell_maps(int cells, const double *t0, const double *t1, const int *cell_begin, int nseg,
@@ -617,11 +675,16 @@ hipError_t launch_event_poses_multi(const SegDev *segs, int events, const double
   return hipGetLastError();
 }
 hipError_t launch_event_depth_multi(const SegDev *segs, int events, int samples, size_t lds, const int *seg,
-                                    const int *pixel, const Pose *poses, float *depth, int *status, hipStream_t st) {
+                                    const int *pixel, const Pose *poses, float *depth, int *status, hipStream_t st,
+                                    bool wave) {
   if (events <= 0) return hipSuccess;
   const int slots = 256 / (samples * samples);
-  event_depth_multi<<<(events + slots - 1) / slots, 256, lds, st>>>(segs, events, samples, seg, pixel, poses, depth,
-                                                                   status);
+  const unsigned blocks = (unsigned)((events + slots - 1) / slots);
+  if (wave)  // the one word of each lane's depth hand-off
+    event_depth_multi<true><<<blocks, 256, 256 * sizeof(int), st>>>(segs, events, samples, seg, pixel, poses, depth,
+                                                                    status);
+  else
+    event_depth_multi<false><<<blocks, 256, lds, st>>>(segs, events, samples, seg, pixel, poses, depth, status);
   return hipGetLastError();
 }
 hipError_t launch_segment_keys(int events, const unsigned *order, const int *seg, unsigned long long *keys,
@@ -704,8 +767,12 @@ hipError_t launch_traversal_stats(const DevScene &s, const Pose *pose, int npix,
 hipError_t launch_rays(const DevScene &s, int64_t n, const float *o, const float *d, const int *self, int shadow,
                        int *hit, int *tri, float *t, float *u, float *v, hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  const size_t lds = (size_t)s.stack_depth * 256 * sizeof(int);
-  rays_kernel<<<(unsigned)((n + 255) / 256), 256, lds, st>>>(s, n, o, d, self, shadow, hit, tri, t, u, v);
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  if (s.packet)
+    rays_kernel<true><<<blocks, 256, 0, st>>>(s, n, o, d, self, shadow, hit, tri, t, u, v);
+  else
+    rays_kernel<false><<<blocks, 256, (size_t)s.stack_depth * 256 * sizeof(int), st>>>(s, n, o, d, self, shadow, hit,
+                                                                                       tri, t, u, v);
   return hipGetLastError();
 }
 

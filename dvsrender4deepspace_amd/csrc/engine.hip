@@ -323,7 +323,8 @@ class Engine {
   // Launch shapes of the adaptive kernels (results do not depend on them).
   void set_tuning(const dvs_tuning &t) {
     if (t.rate_cells < 8 || t.rate_cells > 1024 || t.rate_cells % 8 != 0 ||
-        (t.trace_threads != 64 && t.trace_threads != 128 && t.trace_threads != 256))
+        (t.trace_threads != 64 && t.trace_threads != 128 && t.trace_threads != 256) ||
+        (t.traversal != 0 && t.traversal != 1))
       fail(DVS_E_ARGUMENT, "invalid launch tuning");
     tuning_ = t;
   }
@@ -511,7 +512,7 @@ class Engine {
     const int samples = s.scene->view.samples;
     hip_check(launch_adapt_trace(dvsdev::LAYOUT_CELL, segtab_.ptr, pbegin_.ptr, last_nseg_, samples, items,
                                  verify_pix_.ptr, verify_pose_.ptr, verify_poses_.ptr, last_lds_, verify_rad_.ptr,
-                                 status_.ptr, counter_.ptr, st),
+                                 status_.ptr, counter_.ptr, st, nullptr, nullptr, 256, wave_),
               "optical convergence probe launch");
     AdaptBufs b{};
     b.flags = flags_.ptr;
@@ -637,11 +638,13 @@ class Engine {
     cblocks[0] = 0;
     pbegin[0] = 0;
     size_t lds = 0;
+    wave_ = tuning_.traversal != 0;
     for (int i = 0; i < nseg; ++i) {
       cblocks[i + 1] = cblocks[i] + (work[i].seg->npix + 255) / 256;
       pbegin[i + 1] = pbegin[i] + work[i].seg->npix;
       const int depth = std::max(3, work[i].seg->scene->view.stack_depth);
       lds = std::max(lds, (size_t)depth * 256 * sizeof(int));
+      wave_ = wave_ && work[i].seg->scene->view.packet != 0;
     }
     cblock_.upload(cblocks.data(), cblocks.size(), "segment blocks", st, staging_);
     pbegin_.upload(pbegin.data(), pbegin.size(), "segment pixels", st, staging_);
@@ -759,6 +762,7 @@ class Engine {
                             (const void *)rare_native_.ptr})
       shape.push_back((long long)(uintptr_t)ptr);
     shape.push_back((long long)lds);
+    shape.push_back(wave_ ? 1 : 0);
     shape.push_back(pixel_total);
     shape.push_back(cell_total);
     shape.push_back(iv_total);
@@ -782,7 +786,7 @@ class Engine {
           hip_check(hipEventRecord(trace_t.back()->start, st), "record");
         }
         hip_check(launch_trace_cells_multi(segtab_.ptr + (size_t)k * nseg, tblock_.ptr + (size_t)k * (nseg + 1), nseg,
-                                           blocks, lds, status_.ptr, counter_.ptr, st),
+                                           blocks, lds, status_.ptr, counter_.ptr, st, wave_),
                   "trace and integrate circuits");
         if (!graph) {
           hip_check(hipEventRecord(trace_t.back()->stop, st), "record");
@@ -1039,7 +1043,7 @@ class Engine {
       if (depth) {
         hip_check(launch_event_poses_multi(tab, total, ev_time_.ptr, ev_seg_.ptr, ev_poses_.ptr, st), "event poses");
         hip_check(launch_event_depth_multi(tab, total, samples, lds, ev_seg_.ptr, ev_pixel_.ptr, ev_poses_.ptr,
-                                           ev_depth_.ptr, status_.ptr, st),
+                                           ev_depth_.ptr, status_.ptr, st, wave_),
                   "event depth rays");
       } else {
         // The interpolating path supplies no per-event depth (event_sensor.cu:111-113).
@@ -1226,7 +1230,7 @@ class Engine {
     hip_check(hipEventRecord(probe_t.start, st), "record");
     hip_check(launch_adapt_trace(dvsdev::LAYOUT_PROBE, segtab_.ptr, pbegin_.ptr, nseg, samples, (long long)P * 5,
                                  nullptr, nullptr, probe_poses_.ptr, lds, probes_[pcur].ptr, status_.ptr, counter_.ptr,
-                                 st, probe_reuse_.ptr, probes_[pprev].ptr, tuning_.trace_threads),
+                                 st, probe_reuse_.ptr, probes_[pprev].ptr, tuning_.trace_threads, wave_),
               "refine pixel optical launch");
     hip_check(hipEventRecord(probe_t.stop, st), "record");
     probe_layout_ = std::move(layout);
@@ -1308,7 +1312,7 @@ class Engine {
         hip_check(hipEventRecord(query_t.back()->start, st), "record");
         hip_check(launch_adapt_trace(dvsdev::LAYOUT_QUERY, segtab_.ptr, pbegin_.ptr, nseg, samples, queries,
                                      qpix_.ptr, nullptr, qpose_.ptr, lds, qrad_.ptr, status_.ptr, counter_.ptr, st,
-                                     nullptr, nullptr, tuning_.trace_threads),
+                                     nullptr, nullptr, tuning_.trace_threads, wave_),
                   "refine pixel optical launch");
         hip_check(hipEventRecord(query_t.back()->stop, st), "record");
         stats.query_samples += (uint64_t)queries;
@@ -1373,7 +1377,7 @@ class Engine {
                 "identify local exact optical cells");
       hip_check(launch_adapt_trace(dvsdev::LAYOUT_CELL, segtab_.ptr, pbegin_.ptr, nseg, samples, exact_total,
                                    eitem_pix_.ptr, eitem_pose_.ptr, poses_.ptr, lds, erad_.ptr, status_.ptr,
-                                   counter_.ptr, st, nullptr, nullptr, tuning_.trace_threads),
+                                   counter_.ptr, st, nullptr, nullptr, tuning_.trace_threads, wave_),
                 "trace local exact ray cells");
     }
     // Whole-pixel fallback list (compact_fallback, event_sensor.cu:116-118).
@@ -1457,7 +1461,7 @@ class Engine {
       gate_idx_.upload(gate_seg.data(), gate_seg.size(), "mean receptor probes", st, staging_);
       staging_.flush(st);
       hip_check(launch_trace_cells_multi(gate_tab_.ptr, gate_blocks_.ptr, ngate, gblocks[ngate], lds, status_.ptr,
-                                         counter_.ptr, st),
+                                         counter_.ptr, st, wave_),
                 "mean receptor probe optics");
       gate_err_.reserve((size_t)ngate * 1024, "mean receptor probe errors");
       hip_check(launch_adapt_gate(gate_tab_.ptr, gate_idx_.ptr, gate_slots_.ptr, ngate, adapt_.ptr, b, gate_err_.ptr,
@@ -1562,7 +1566,7 @@ class Engine {
         staging_.flush(st);
         if (fblocks[nseg] > 0) {
           hip_check(launch_trace_cells_multi(fb_tab_.ptr, fb_blocks_.ptr, nseg, fblocks[nseg], lds, status_.ptr,
-                                             counter_.ptr, st),
+                                             counter_.ptr, st, wave_),
                     "parallel exact cell optics launch");
           hip_check(launch_cell_rates_multi(fb_tab_.ptr, fb_items_.ptr, nseg, fb_total, c1 - c0, status_.ptr,
                                             rare_native_.ptr, rare_fallback, st),
@@ -1682,7 +1686,8 @@ class Engine {
   size_t last_lds_ = 0;
   // measured on the DAVIS346 request's first windows: 128-lane trace blocks
   // and 128-cell photocurrent runs were 1.5% faster per window than 256 / 32
-  dvs_tuning tuning_{128, 128};
+  dvs_tuning tuning_{128, 128, 1};
+  bool wave_ = false;  // this step's traces take the wave-packet traversal
   int last_nseg_ = 0;
   DeviceArray<int> verify_pix_, verify_pose_;
   DeviceArray<Pose> verify_poses_;

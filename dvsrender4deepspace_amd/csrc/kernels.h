@@ -56,8 +56,11 @@ using dvsdev::SegDev;
 
 // Multi-segment launches.  block_begin / pixel_begin hold nseg+1 prefix
 // offsets (in blocks) of each segment's work; the grids are their totals.
+// `wave`: the wave-packet traversal (every segment's scene must allow it);
+// `lds` (per-lane stacks) is then not used.
 hipError_t launch_trace_cells_multi(const SegDev *segs, const long long *block_begin, int nseg, long long blocks,
-                                    size_t lds, int *status, unsigned long long *shadow_count, hipStream_t st);
+                                    size_t lds, int *status, unsigned long long *shadow_count, hipStream_t st,
+                                    bool wave = false);
 // cell_maps of every cell of a step; cell_begin holds nseg+1 prefix offsets
 // of the segments' cells and charge their charge scales.
 hipError_t launch_cell_maps(int cells, const double *t0, const double *t1, const int *cell_begin, int nseg,
@@ -77,7 +80,8 @@ hipError_t launch_gather_multi(const SegDev *segs, const int *pixel_begin, int n
 hipError_t launch_event_poses_multi(const SegDev *segs, int events, const double *time, const int *seg,
                                     Pose *poses, hipStream_t st);
 hipError_t launch_event_depth_multi(const SegDev *segs, int events, int samples, size_t lds, const int *seg,
-                                    const int *pixel, const Pose *poses, float *depth, int *status, hipStream_t st);
+                                    const int *pixel, const Pose *poses, float *depth, int *status, hipStream_t st,
+                                    bool wave = false);
 hipError_t launch_segment_keys(int events, const unsigned *order, const int *seg, unsigned long long *keys,
                                hipStream_t st);
 hipError_t launch_permute_multi(const SegDev *segs, int events, const unsigned *order, const double *time,

@@ -16,6 +16,7 @@ struct Geometry {
   DeviceArray<uint32_t> indices;
   DeviceArray<float> normals, albedos;
   bool has_albedo = false;
+  bool ordered_ranges = false;  // the wave-packet traversal may serve the tree
   int32_t root_ref = 0, stack_depth = 1;
   int64_t internal_nodes = 0, leaf_count = 0, triangle_count = 0, vertex_count = 0;
   int64_t device_bytes() const {

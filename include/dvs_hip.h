@@ -207,6 +207,14 @@ int dvs_scene_create_ex(const dvs_bvh_node *nodes, int64_t node_count, int32_t r
  * the reference builds one DeviceScene per camera configuration).  The view
  * shares the uploaded BVH and mesh; hapke may be NULL to keep the base's. */
 int dvs_scene_view(dvs_scene base, const dvs_camera *camera, const dvs_hapke *hapke, dvs_scene *out);
+/* Traversal form of the scene's frame, depth and ray launches (no reference
+ * counterpart; results do not depend on it): 1 (the default) the
+ * wave-packet traversal, 0 one traversal per lane with its stack in LDS.
+ * A tree whose reference subtrees do not cover ordered triangle index
+ * ranges (not built by dvs_bvh_build) always takes 0; *effective (may be
+ * NULL) receives the form in use.  Sensors and batches choose theirs by
+ * dvs_tuning::traversal. */
+int dvs_scene_set_traversal(dvs_scene scene, int32_t form, int32_t *effective);
 /* ~DeviceScene (device_scene.cu:38-43).  The scene is reference counted:
  * sensors created on it keep it alive. */
 int dvs_scene_destroy(dvs_scene scene);
@@ -370,10 +378,16 @@ int dvs_batch_feed_trajectory(dvs_batch batch, int32_t segment, dvs_trajectory t
  *                  (a multiple of 8 in [8, 1024]; 128 by default);
  *   trace_threads  lanes of a refinement trace block, one sub-ray each (64,
  *                  128, the default, or 256; 256 where a pixel's sub-rays
- *                  outnumber the block). */
+ *                  outnumber the block);
+ *   traversal      1 (the default): the wave-packet traversal, the 64 lanes
+ *                  of a wave walking the tree together; 0: one traversal
+ *                  per lane with its stack in LDS.  Scenes whose reference
+ *                  subtrees do not cover ordered index ranges (not built by
+ *                  dvs_bvh_build) always take 0. */
 typedef struct dvs_tuning {
   int32_t rate_cells;
   int32_t trace_threads;
+  int32_t traversal;
 } dvs_tuning;
 int dvs_batch_set_tuning(dvs_batch batch, const dvs_tuning *tuning);
 int dvs_batch_get_tuning(dvs_batch batch, dvs_tuning *tuning);

@@ -156,7 +156,8 @@ def test_trajectory_fed_segments_equal_host_fed(gpu, adaptive):
 
 def test_launch_tuning_keeps_events(gpu):
     """The adaptive kernels' launch shapes (photocurrent cells per thread
-    run, trace lanes per block) change no event; invalid shapes are refused."""
+    run, trace lanes per block) and the traversal form (per-lane stacks or
+    wave packets) change no event; invalid settings are refused."""
     specs = _segments()[:3]
     geo, views = _scenes(gpu, specs)
 
@@ -181,12 +182,15 @@ def test_launch_tuning_keeps_events(gpu):
 
     want = run(None)
     assert sum(len(e) for e in want) > 20
-    for tuning in ({"rate_cells": 8, "trace_threads": 64}, {"rate_cells": 128, "trace_threads": 128},
-                   {"rate_cells": 1024, "trace_threads": 256}):
+    for tuning in ({"rate_cells": 8, "trace_threads": 64, "traversal": 1},
+                   {"rate_cells": 128, "trace_threads": 128, "traversal": 0},
+                   {"rate_cells": 1024, "trace_threads": 256, "traversal": 1},
+                   {"rate_cells": 64, "trace_threads": 64, "traversal": 0}):
         for a, b in zip(run(tuning), want):
             assert a.tobytes() == b.tobytes(), tuning
     batch = gpu.Batch()
-    for bad in ({"rate_cells": 12, "trace_threads": 256}, {"rate_cells": 32, "trace_threads": 96}):
+    for bad in ({"rate_cells": 12, "trace_threads": 256}, {"rate_cells": 32, "trace_threads": 96},
+                {"rate_cells": 32, "trace_threads": 128, "traversal": 2}):
         with pytest.raises(capi.InvalidArgument, match="invalid launch tuning"):
             batch.set_tuning(**bad)
 

@@ -89,13 +89,17 @@ def test_terrain_frame_parity(oracle, gpu, name):
         _frame_check(oracle, g, o, w.states(np.array([t])))
 
 
+@pytest.mark.parametrize("traversal", [0, 1], ids=["lane", "wave"])
 @pytest.mark.parametrize("device_leaf", [0, 1, 4, 16])
-def test_traversal_bit_exact(oracle, gpu, device_leaf):
+def test_traversal_bit_exact(oracle, gpu, device_leaf, traversal):
     """Nearest hits (triangle, t, u, v) and any-hits equal the reference
-    traversal of the leaf-64 tree, with or without device leaf subdivision."""
+    traversal of the leaf-64 tree, with or without device leaf subdivision,
+    by per-lane stacks and by wave packets (incoherent rays, and rays aimed
+    at shared vertices, where triangles of different reference leaves tie)."""
     w = workloads.PLAIN_SMALL
     v, n, nodes, root, idx = scenes.workload_mesh(w)
     g = gpu.Scene(nodes, root, v, n, idx, w.camera, w.hapke, device_leaf_size=device_leaf)
+    assert g.set_traversal(traversal) == traversal
     o = oracle.Scene(nodes, root, v, n, idx, w.camera, w.hapke)
     rng = np.random.default_rng(7)
     n = 4096
@@ -132,6 +136,35 @@ def test_traversal_bit_exact(oracle, gpu, device_leaf):
     ref = np.array([o.shadow(lifted[i], light[i], -1) for i in range(n)])
     assert np.array_equal(occ, ref)
     assert 0 < occ.sum() < n
+
+
+def test_traversal_forms_equal_full_size(gpu):
+    """The DAVIS346 request's frames (346x260, 4x4 sub-rays, 1024^2 terrain)
+    are bit-identical by per-lane stacks and by wave packets, as are the
+    hits of rays aimed at every 37th mesh vertex from one camera position
+    (coherent waves full of ties between reference leaves)."""
+    w = workloads.DAVIS346
+    v, n, nodes, root, idx = scenes.workload_mesh(w)
+    g = gpu.Scene(nodes, root, v, n, idx, w.camera, w.hapke)
+    for t in (0.0, 0.5 * w.end_time):
+        pose = w.states(np.array([t]))
+        assert g.set_traversal(0) == 0
+        img0, dep0, rad0 = g.render(pose, with_radiance=True)
+        assert g.set_traversal(1) == 1
+        img1, dep1, rad1 = g.render(pose, with_radiance=True)
+        assert np.array_equal(rad0, rad1) and np.array_equal(dep0, dep1)
+    pick = np.arange(0, len(v), 37)
+    eye = np.array([1.5, -2.5, 180.0], np.float32)
+    dirs = (v[pick] - eye).astype(np.float32)
+    dirs = (dirs / np.linalg.norm(dirs, axis=1, keepdims=True)).astype(np.float32)
+    orgs = np.tile(eye, (len(pick), 1))
+    g.set_traversal(0)
+    lane = g.trace(orgs, dirs)
+    g.set_traversal(1)
+    wave = g.trace(orgs, dirs)
+    for a, b in zip(lane, wave):
+        assert np.array_equal(a, b)
+    assert np.mean(lane[0] == 1) > 0.9
 
 
 def test_hapke_device_parity(oracle, gpu):
