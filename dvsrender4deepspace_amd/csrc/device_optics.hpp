@@ -142,10 +142,13 @@ __device__ __forceinline__ float4 record4(ConstFloats rec, int i) {
   return make_float4(rec[4 * i], rec[4 * i + 1], rec[4 * i + 2], rec[4 * i + 3]);
 }
 
-// Slab test of both children of one pair record (device_ray.hpp:13-53).
+
+// Slab test of both children of one pair record (device_ray.hpp:13-53):
+// whether each child's box is entered (hit0, hit1), its entry distances and
+// the child references.
 template <class Rec>
-__device__ __forceinline__ int test_pair(Rec rec, f3 P, f3 idir, float t_min, float t_max,
-                                         float &d0, float &d1, int &left, int &right) {
+__device__ __forceinline__ void slab_pair(Rec rec, f3 P, f3 idir, float t_min, float t_max, float &d0, float &d1,
+                                          int &left, int &right, bool &hit0, bool &hit1) {
   const float4 a = record4(rec, 0), b = record4(rec, 1), c = record4(rec, 2), r = record4(rec, 3);
   const float t1x0 = (a.x - P.x) * idir.x, t1x1 = (a.w - P.x) * idir.x;
   const float t1y0 = (a.y - P.y) * idir.y, t1y1 = (b.x - P.y) * idir.y;
@@ -161,7 +164,16 @@ __device__ __forceinline__ int test_pair(Rec rec, f3 P, f3 idir, float t_min, fl
   t2min = fmaxf(t_min, t2min); t2max = fminf(t_max, t2max);
   d0 = t1min; d1 = t2min;
   left = __float_as_int(r.x); right = __float_as_int(r.y);
-  return (t1max >= t1min ? 1 : 0) | (t2max >= t2min ? 2 : 0);
+  hit0 = t1max >= t1min;
+  hit1 = t2max >= t2min;
+}
+// The same as a mask: bit 0 the left child, bit 1 the right one.
+template <class Rec>
+__device__ __forceinline__ int test_pair(Rec rec, f3 P, f3 idir, float t_min, float t_max,
+                                         float &d0, float &d1, int &left, int &right) {
+  bool hit0, hit1;
+  slab_pair(rec, P, idir, t_min, t_max, d0, d1, left, right, hit0, hit1);
+  return (hit0 ? 1 : 0) | (hit1 ? 2 : 0);
 }
 // Moller-Trumbore on a pre-gathered record (device_ray.hpp:54-80).  `meta`
 // receives the record's last float4: (e2.z, reference index, leaf lo, leaf hi).
@@ -326,9 +338,11 @@ __device__ inline bool trace_any(const DevScene &s, f3 origin, f3 dir, int self,
 //   (device_ray.hpp:120-127).  earlier_leaf() finds that ancestor by the
 //   first reference index of each pair's right subtree (pair word 14) and
 //   tests its two boxes for the lane's ray.
-// The stack holds at most one entry per level of the current path, and the
-// scene's stack depth is its deepest leaf (at most 64), so no traversal
-// overflows; an overflow would still mark the wave's lanes invalid.
+// The stack holds at most one entry per level of the current path: a push
+// at a node of depth k finds at most k entries (one sibling per level above
+// it), and the node has children, so the tree is deeper than k.  Trees are
+// at most 64 deep (dvs_scene_create), so entry k always has its lane k and
+// the stack never overflows.
 // ---------------------------------------------------------------------------
 constexpr int kWaveStackDepth = 64;  // dvs_scene_create rejects deeper trees
 __device__ __forceinline__ int uniform_int(int x) { return __builtin_amdgcn_readfirstlane(x); }
@@ -338,8 +352,9 @@ __device__ __forceinline__ unsigned long long lane_bit() { return 1ull << __lane
 // sp by a compare and select; v_readlane reads it back).
 struct WaveStack {
   int ref = 0, lo = 0, hi = 0;
-  __device__ __forceinline__ void push(int sp, int node, unsigned long long mask) {
-    const bool here = (int)__lane_id() == sp;
+  // push when `when` (wave-uniform) holds, without a branch
+  __device__ __forceinline__ void push_if(bool when, int sp, int node, unsigned long long mask) {
+    const bool here = when && (int)__lane_id() == sp;
     ref = here ? node : ref;
     lo = here ? (int)(unsigned)mask : lo;
     hi = here ? (int)(unsigned)(mask >> 32) : hi;
@@ -404,25 +419,25 @@ __device__ inline bool earlier_leaf(const WaveGeom &g, int lo_a, int lo_b, f3 or
   return false;
 }
 
-// The side of a pair most of the lanes entering both sides would take first.
-__device__ __forceinline__ bool wave_left_first(bool mine, int mask, float d0, float d1, unsigned long long m0,
-                                                unsigned long long m1) {
-  const unsigned long long lf = __ballot(mine && mask == 3 && d0 < d1);
-  return 2 * __popcll(lf) >= __popcll(m0 & m1);
+// The side of a pair most of the lanes entering both sides (m0 & m1) would
+// take first.  Lanes outside the walk's mask test the pair too (their results
+// are masked off: a lane's own state is all its test reads), which keeps the
+// lane mask out of the vector registers.
+__device__ __forceinline__ bool wave_left_first(float d0, float d1, unsigned long long m0, unsigned long long m1) {
+  const unsigned long long both = m0 & m1;
+  return 2 * __popcll(both & __ballot(d0 < d1)) >= __popcll(both);
 }
 
-// Nearest hit of the lanes of `group` (trace_nearest's result per lane);
-// the group's lanes are added to `overflow` if the stack overflows.
-__device__ inline void wave_nearest(const WaveGeom &g, unsigned long long group, f3 origin, f3 ray, f3 idir, Hit &h,
-                                    unsigned long long &overflow) {
+// Nearest hit of the lanes of `group` (trace_nearest's result per lane).
+__device__ inline void wave_nearest(const WaveGeom &g, unsigned long long group, f3 origin, f3 ray, f3 idir, Hit &h) {
   WaveStack st;
   float closest = 1e30f;
   int best_lo = -1;
   int ref = g.root_ref, sp = 0;
   unsigned long long m = group;
   for (;;) {
-    const bool mine = (m & lane_bit()) != 0;
     if (ref < 0) {
+      const bool mine = (m & lane_bit()) != 0;
       const int2 leaf = make_int2(g.leaves[2 * ~ref], g.leaves[2 * ~ref + 1]);
       for (int i = 0; i < leaf.y; ++i) {
         const ConstFloats rec = g.triangles + 12 * (leaf.x + i);
@@ -440,19 +455,22 @@ __device__ inline void wave_nearest(const WaveGeom &g, unsigned long long group,
         }
       }
     } else {
-      float d0 = 0, d1 = 0;
+      float d0, d1;
       int l, r;
-      const int mask = test_pair(g.pairs + 16 * ref, origin, idir, kRayEps, closest, d0, d1, l, r);
-      const unsigned long long m0 = __ballot(mine && (mask & 1)), m1 = __ballot(mine && (mask & 2));
-      if (m0 && m1) {
-        if (sp >= g.depth) { overflow |= group; return; }
-        const bool left_first = wave_left_first(mine, mask, d0, d1, m0, m1);
-        st.push(sp++, left_first ? r : l, left_first ? m1 : m0);
-        ref = left_first ? l : r;
-        m = left_first ? m0 : m1;
+      bool hit0, hit1;
+      slab_pair(g.pairs + 16 * ref, origin, idir, kRayEps, closest, d0, d1, l, r, hit0, hit1);
+      const unsigned long long m0 = m & __ballot(hit0), m1 = m & __ballot(hit1);
+      if (m0 | m1) {
+        // both sides: the far one waits on the stack; one side: go there
+        const bool both = m0 && m1, left_first = wave_left_first(d0, d1, m0, m1);
+        const int near = left_first ? l : r, far = left_first ? r : l;
+        const unsigned long long near_m = left_first ? m0 : m1, far_m = left_first ? m1 : m0;
+        st.push_if(both, sp, far, far_m);
+        sp += both ? 1 : 0;
+        ref = near_m ? near : far;
+        m = near_m ? near_m : far_m;
         continue;
       }
-      if (m0 | m1) { ref = m0 ? l : r; m = m0 | m1; continue; }
     }
     if (sp == 0) break;
     --sp;
@@ -462,17 +480,16 @@ __device__ inline void wave_nearest(const WaveGeom &g, unsigned long long group,
 }
 
 // Any hit toward `dir` of the lanes of `group`, skipping each lane's `self`
-// triangle (trace_any's result per lane); returns the occluded lanes.  The
-// group's lanes are added to `overflow` if the stack overflows.
+// triangle (trace_any's result per lane); returns the occluded lanes.
 __device__ inline unsigned long long wave_any(const WaveGeom &g, unsigned long long group, f3 origin, f3 dir, f3 idir,
-                                              int self, float t_min, float t_max, unsigned long long &overflow) {
+                                              int self, float t_min, float t_max) {
   WaveStack st;
   unsigned long long done = 0;
   int ref = g.root_ref, sp = 0;
   unsigned long long m = group;
   for (;;) {
-    const bool mine = (m & lane_bit()) != 0;
     if (ref < 0) {
+      const bool mine = (m & lane_bit()) != 0;
       const int2 leaf = make_int2(g.leaves[2 * ~ref], g.leaves[2 * ~ref + 1]);
       bool hit = false;
       for (int i = 0; i < leaf.y; ++i) {
@@ -488,17 +505,19 @@ __device__ inline unsigned long long wave_any(const WaveGeom &g, unsigned long l
     } else {
       float d0, d1;
       int l, r;
-      const int mask = test_pair(g.pairs + 16 * ref, origin, idir, t_min, t_max, d0, d1, l, r);
-      const unsigned long long m0 = __ballot(mine && (mask & 1)), m1 = __ballot(mine && (mask & 2));
-      if (m0 && m1) {
-        if (sp >= g.depth) { overflow |= group; return group; }
-        const bool left_first = wave_left_first(mine, mask, d0, d1, m0, m1);
-        st.push(sp++, left_first ? r : l, left_first ? m1 : m0);
-        ref = left_first ? l : r;
-        m = left_first ? m0 : m1;
+      bool hit0, hit1;
+      slab_pair(g.pairs + 16 * ref, origin, idir, t_min, t_max, d0, d1, l, r, hit0, hit1);
+      const unsigned long long m0 = m & __ballot(hit0), m1 = m & __ballot(hit1);
+      if (m0 | m1) {
+        const bool both = m0 && m1, left_first = wave_left_first(d0, d1, m0, m1);
+        const int near = left_first ? l : r, far = left_first ? r : l;
+        const unsigned long long near_m = left_first ? m0 : m1, far_m = left_first ? m1 : m0;
+        st.push_if(both, sp, far, far_m);
+        sp += both ? 1 : 0;
+        ref = near_m ? near : far;
+        m = near_m ? near_m : far_m;
         continue;
       }
-      if (m0 | m1) { ref = m0 ? l : r; m = m0 | m1; continue; }
     }
     // the next entry with a lane still looking
     m = 0;
@@ -732,8 +751,7 @@ __device__ inline SubSample sample_subray(const DevScene &s, int pixel, const Po
 
 // The same sub-ray sample by the wave-packet traversal.  Every lane of the
 // wave calls it (`active` marks the lanes with a sub-ray); the lanes' scenes
-// may differ.  A traversal overflow (which the stack depth rules out) marks
-// the sample invalid, also for a shadow ray.
+// may differ.  The wave traversal cannot overflow (see above).
 template <bool SHADE>
 __device__ inline SubSample sample_subray_wave(const DevScene &s, int pixel, const Pose &pose, int sx, int sy,
                                                bool active) {
@@ -741,25 +759,21 @@ __device__ inline SubSample sample_subray_wave(const DevScene &s, int pixel, con
   const SubRay r = subray_of(s, pixel, pose, sx, sy);
   const f3 idir = inverse_direction(r.ray);
   Hit hit{0, -1, -1, 0.f, 0.f, 0.f};
-  unsigned long long overflow = 0;
   for_each_geometry(s, __ballot(active), [&](const WaveGeom &g, unsigned long long group) {
-    wave_nearest(g, group, r.origin, r.ray, idir, hit, overflow);
+    wave_nearest(g, group, r.origin, r.ray, idir, hit);
   });
-  const bool over = (overflow & lane_bit()) != 0;
-  if (active && over) out.invalid = 1;
-  const bool surface = active && !over && hit.hit > 0;
+  const bool surface = active && hit.hit > 0;
   if (surface) out.depth = hit.t;
   if (!SHADE) return out;
   Surface f{};
   const bool need = surface && lit_surface(s, hit, pose, r.ray, f);
-  unsigned long long occluded = 0, shadow_overflow = 0;
+  unsigned long long occluded = 0;
   const f3 light_idir = inverse_direction(f.light);
   for_each_geometry(s, __ballot(need), [&](const WaveGeom &g, unsigned long long group) {
-    occluded |= wave_any(g, group, f.so, f.light, light_idir, hit.tri, kRayEps, 1e30f, shadow_overflow);
+    occluded |= wave_any(g, group, f.so, f.light, light_idir, hit.tri, kRayEps, 1e30f);
   });
   if (!need) return out;
   out.shadow = 1;
-  if (shadow_overflow & lane_bit()) { out.invalid = 1; return out; }
   if (!(occluded & lane_bit())) out.radiance = lit_radiance(s, hit, f, r, pose);
   return out;
 }

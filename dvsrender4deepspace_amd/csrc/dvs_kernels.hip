@@ -500,21 +500,19 @@ __global__ __launch_bounds__(256) void rays_kernel(DevScene s, int64_t n, const 
     const f3 origin = mk(o[3 * k], o[3 * k + 1], o[3 * k + 2]);
     const f3 dir = mk(d[3 * k], d[3 * k + 1], d[3 * k + 2]);
     const f3 idir = inverse_direction(dir);
-    unsigned long long overflow = 0;
     if (shadow) {
       const int me = self[k];
       unsigned long long occluded = 0;
       for_each_geometry(s, __ballot(active), [&](const WaveGeom &g, unsigned long long group) {
-        occluded |= wave_any(g, group, origin, dir, idir, me, kRayEps, 1e30f, overflow);
+        occluded |= wave_any(g, group, origin, dir, idir, me, kRayEps, 1e30f);
       });
       if (active) hit[i] = (occluded & lane_bit()) ? 1 : 0;
       return;
     }
     Hit h{0, -1, -1, 0.f, 0.f, 0.f};
     for_each_geometry(s, __ballot(active), [&](const WaveGeom &g, unsigned long long group) {
-      wave_nearest(g, group, origin, dir, idir, h, overflow);
+      wave_nearest(g, group, origin, dir, idir, h);
     });
-    if (overflow & lane_bit()) h.hit = -1;
     if (active) { hit[i] = h.hit; tri[i] = h.tri; t[i] = h.t; u[i] = h.u; v[i] = h.v; }
     return;
   }
