@@ -88,6 +88,28 @@ __device__ __forceinline__ double poisson_inversion(double lambda, double u) {
   while (u > F && k < 1000) { ++k; p *= lambda / k; F += p; }
   return k;
 }
+// The count poisson_inversion(lambda, u) returns, for 0 < lambda < 10,
+// mostly without FP64 arithmetic: the cumulative sums run in float and are
+// compared with u outside a band of relative half-width kScreenBand around
+// each sum.  The float sums stay within 1.1e-5 of the double ones (expf
+// within 2 ulp; lambda's float rounding moves exp(-lambda) by at most
+// 2^-24 lambda < 6e-7; each further term adds at most 2.5 ulp: the
+// reciprocal, two products and lambda's rounding; kScreenTerms terms), so
+// a decision outside the band is the double search's.  Inside the band, or
+// past kScreenTerms terms, the double search itself decides.
+constexpr float kScreenBand = 4e-5f;
+constexpr int kScreenTerms = 30;
+__device__ __forceinline__ double poisson_inversion_screened(double lambda, double u) {
+  const float lf = (float)lambda, uf = (float)u;
+  float p = expf(-lf), F = p;
+  for (int k = 0; k < kScreenTerms; ++k) {
+    if (uf < F * (1.0f - kScreenBand)) return k;  // u <= F_k for certain
+    if (uf <= F * (1.0f + kScreenBand)) break;    // too close to call in float
+    p *= lf * __builtin_amdgcn_rcpf((float)(k + 1));
+    F += p;
+  }
+  return poisson_inversion(lambda, u);
+}
 struct NoiseStream {
   uint64_t seed, pixel, stream, block, word;
   uint64_t buf[4];
@@ -334,7 +356,7 @@ __device__ __forceinline__ double cell_rate_first(double light, const PhotoParam
   if (lambda != lambda) return lambda;  // NaN photocurrent: the circuit reports it
   if (lambda <= 0) return 0.0 / dt;
   if (lambda < 10) {
-    const double k = poisson_inversion(lambda, shot_uniform(shared, cell_serial));
+    const double k = poisson_inversion_screened(lambda, shot_uniform(shared, cell_serial));
     if (k == 0 && dt > 0) return 0.0;  // 0 / dt
     if (k == 1) return one_per_dt;
     if (k == 2) return two_per_dt;
@@ -365,7 +387,7 @@ __device__ __forceinline__ int cell_code_first(double light, const PhotoParams &
     return kCodeEscape;
   }
   if (lambda < 10) {
-    const double k = poisson_inversion(lambda, shot_uniform(shared, cell_serial));
+    const double k = poisson_inversion_screened(lambda, shot_uniform(shared, cell_serial));
     if (dt > 0 && k < kCodeEscape) return (int)k;
     escape = k == 0 && dt > 0 ? 0.0 : k == 1 ? one_per_dt : k == 2 ? two_per_dt : k / dt;
     return kCodeEscape;

@@ -475,7 +475,7 @@ __global__ void advance_kernel(int64_t n, dvs_pixel_state *states, const dvs_pix
 }
 __global__ void poisson_kernel(int64_t n, const double *lambda, const double *u, double *out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = poisson_inversion(lambda[i], u[i]);
+  if (i < n) out[i] = poisson_inversion_screened(lambda[i], u[i]);
 }
 __global__ void noise_kernel(uint64_t seed, int64_t pixels, double lambda, int draws, double *poisson,
                              double *normal) {

@@ -384,8 +384,10 @@ def test_noise_on_reproducible_per_seed(gpu):
 def test_poisson_inversion_at_thresholds(gpu, oracle):
     """The device's sequential-search Poisson inversion equals the oracle's
     at uniforms 1e-12 .. 1e-3 either side of each of its first six
-    cumulative thresholds and at random ones, for lambda from 1e-4 to 10
-    (offsets stay above the last-place differences of the two exp)."""
+    cumulative thresholds, at relative offsets either side of the device's
+    float screening band (the screen decides just outside it, the double
+    search inside), and at random ones, for lambda from 1e-4 to 10 (offsets
+    stay above the last-place differences of the two exp)."""
     import math
     rng = np.random.default_rng(8)
     lams = np.concatenate([np.geomspace(1e-4, 0.999, 400), np.linspace(1.0, 9.9, 20)])
@@ -401,6 +403,11 @@ def test_poisson_inversion_at_thresholds(gpu, oracle):
         for F in thresholds:
             for d in (1e-12, 1e-9, 1e-6, 1.9e-5, 2.1e-5, 1e-4, 1e-3):
                 for u in (F - d, F + d):
+                    if 0 < u < 1:
+                        lam_list.append(lam)
+                        u_list.append(u)
+            for r in (3e-5, 3.9e-5, 4.1e-5, 4.5e-5, 6e-5):
+                for u in (F * (1 - r), F * (1 + r)):
                     if 0 < u < 1:
                         lam_list.append(lam)
                         u_list.append(u)
