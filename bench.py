@@ -69,14 +69,14 @@ SLICES = 20
 # bytes (FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0: a 1 GiB device copy reads
 # FETCH_SIZE 0.5 GiB and WRITE_SIZE 1 GiB), per unit of each kernel's work.
 PROFILE = {
-    "probe": {"unit": "traced pixel sample", "valu_insts": 1914.68, "valu_busy": 1.072, "lane_util": 0.78,
-              "l2_hit": 0.571, "fetch": 156.61, "write": 19.65},
-    "query": {"unit": "traced pixel sample", "valu_insts": 2226.60, "valu_busy": 0.52, "lane_util": 0.655,
-              "l2_hit": 0.607, "fetch": 367.75, "write": 10.0},
-    "rates": {"unit": "pixel cell", "valu_insts": 2.5555, "valu_busy": 0.898, "lane_util": 0.868,
-              "l2_hit": 0.827, "fetch": 0.0723, "write": 1.004},
-    "integrate": {"unit": "pixel cell", "valu_insts": 0.4639, "valu_busy": 0.27, "lane_util": 0.577,
-                  "l2_hit": 0.765, "fetch": 1.1049, "write": 0.0366},
+    "probe": {"unit": "traced pixel sample", "valu_insts": 1757.05, "valu_busy": 1.273, "lane_util": 0.867,
+              "l2_hit": 0.871, "fetch": 293.49, "write": 23.73},
+    "query": {"unit": "traced pixel sample", "valu_insts": 2082.73, "valu_busy": 0.574, "lane_util": 0.775,
+              "l2_hit": 0.830, "fetch": 660.0, "write": 12.0},
+    "rates": {"unit": "pixel cell", "valu_insts": 2.2363, "valu_busy": 0.821, "lane_util": 0.903,
+              "l2_hit": 0.682, "fetch": 0.1578, "write": 1.001},
+    "integrate": {"unit": "pixel cell", "valu_insts": 0.4639, "valu_busy": 0.268, "lane_util": 0.581,
+                  "l2_hit": 0.789, "fetch": 1.1044, "write": 0.0366},
 }
 
 
