@@ -32,6 +32,15 @@ __host__ __device__ __forceinline__ float clampf(float x, float lo, float hi) { 
 constexpr float kRayEps = 1e-6f;                  // include/helper_math.hpp:11
 constexpr float kPi = 3.14159265358979323846f;    // include/dvs/math_types.hpp
 
+// The Hapke parameters' per-scene constants (the terms of
+// hapke_reflectance that depend on the parameters alone), computed once on
+// the host by hapke_constants() with the same float expressions.
+struct HapkeConsts {
+  float tan_tp, cot_tp, ch;  // tan, cot and chi of theta-bar (chi: 1/sqrt(1 + pi tan^2))
+  float K;                   // porosity factor
+  int rough;                 // theta-bar > 1e-8
+};
+
 // Device view of one scene (SoA BVH + indexed mesh), passed by value.
 struct DevScene {
   const float4 *pairs;       // 4 x float4 per internal node
@@ -44,6 +53,7 @@ struct DevScene {
   int width, height, samples;
   float fx, fy, cx, cy;
   dvs_hapke hapke;
+  HapkeConsts hk;  // hapke_constants(hapke)
   // 1: the wave-packet traversal may serve this tree (every reference
   // subtree covers one triangle index range, left before right, and word 14
   // of each pair holds the right subtree's first index)
@@ -642,6 +652,64 @@ __host__ __device__ __forceinline__ float hapke_directional(f3 normal, f3 light,
   const f3 n = normalize(normal), l = normalize(light), v = normalize(view);
   return hapke_reflectance(dot(n, l), dot(n, v), dot(l, v), albedo, p);
 }
+// Per-scene constants of hapke_reflectance (host).
+__host__ __device__ inline HapkeConsts hapke_constants(const dvs_hapke &p) {
+  HapkeConsts c{};
+  c.rough = p.thetap > 1e-8f;
+  c.tan_tp = c.rough ? tanf(p.thetap) : 0.f;
+  c.cot_tp = c.rough ? 1.0f / c.tan_tp : 0.f;
+  c.ch = c.rough ? 1.0f / sqrtf(1.0f + kPi * c.tan_tp * c.tan_tp) : 1.f;
+  c.K = hk_porosity(p.phi);
+  return c;
+}
+// hapke_reflectance for the ray kernels: the same expressions in the same
+// order, with the terms that depend on the parameters alone (tan, cot and
+// chi of theta-bar, the porosity factor) taken from HapkeConsts, computed
+// once per scene on the host by the same float expressions.  The refinement
+// decisions sit on radiance last bits (a segment whose residual is at the
+// tolerance), so everything else keeps the exact operations.
+__device__ inline float hapke_fast(float cosi, float cose, float cosg, float albedo, const dvs_hapke &p,
+                                   const HapkeConsts &c) {
+  if (cosi <= 0 || cose <= 0) return 0;
+  cosi = clampf(cosi, 0, 1); cose = clampf(cose, 0, 1); cosg = clampf(cosg, -1, 1);
+  const float sini = sqrtf(fmaxf(0, 1 - cosi * cosi)), sine = sqrtf(fmaxf(0, 1 - cose * cose));
+  const float tg = hk_safe_tan(hk_safe_acos(cosg) * 0.5f);
+  const float psi = hk_safe_acos((cosg - cosi * cose) / (sini * sine + 1e-6f));
+  const float pg = hk_phase(cosg, p.b, p.c);
+  const float BS = hk_shoe(tg, p.hS), BC = hk_cboe(tg, p.hC), K = c.K;
+  float mu0, mue, S;
+  if (!c.rough) {
+    mu0 = cosi; mue = cose; S = 1.0f;
+  } else {
+    const float tan_tp = c.tan_tp, cot_tp = c.cot_tp, ch = c.ch;
+    const float sh = sinf(psi * 0.5f), shv = sh * sh;
+    const float cpsi = cosf(psi);
+    const float e1i = hk_E1(cot_tp, cosi, sini), e1e = hk_E1(cot_tp, cose, sine);
+    const float e2i = hk_E2(cot_tp, cosi, sini), e2e = hk_E2(cot_tp, cose, sine);
+    const float eta_i = ch * (cosi + sini * tan_tp * e2i / (2.0f - e1i));
+    const float eta_e = ch * (cose + sine * tan_tp * e2e / (2.0f - e1e));
+    const float fpsi = expf(-2.0f * tanf(psi * 0.5f));
+    if (cosi >= cose) {
+      const float den = 2.0f - e1e - (psi / kPi) * e1i;
+      mu0 = ch * (cosi + sini * tan_tp * (cpsi * e2e + shv * e2i) / den);
+      mue = ch * (cose + sine * tan_tp * (e2e - shv * e2i) / den);
+      S = mue / eta_e * cosi / eta_i * ch / (1.0f - fpsi + fpsi * ch * (cosi / eta_i));
+    } else {
+      const float den = 2.0f - e1i - (psi / kPi) * e1e;
+      mu0 = ch * (cosi + sini * tan_tp * (e2i - shv * e2e) / den);
+      mue = ch * (cose + sine * tan_tp * (cpsi * e2i + shv * e2e) / den);
+      S = mue / eta_e * cosi / eta_i * ch / (1.0f - fpsi + fpsi * ch * (cose / eta_e));
+    }
+  }
+  const float M = hk_H(mu0 / K, albedo) * hk_H(mue / K, albedo) - 1.0f;
+  const float LS = mu0 / (mu0 + mue + 1e-6f);
+  return (albedo / (4.0f * kPi)) * LS * K * (pg * (1.0f + BS * p.B_S0) + M) * (1.0f + BC * p.B_C0) * S;
+}
+__device__ __forceinline__ float hapke_directional_fast(f3 normal, f3 light, f3 view, float albedo,
+                                                        const dvs_hapke &p, const HapkeConsts &c) {
+  const f3 n = normalize(normal), l = normalize(light), v = normalize(view);
+  return hapke_fast(dot(n, l), dot(n, v), dot(l, v), albedo, p, c);
+}
 __host__ __device__ __forceinline__ float material_albedo(float value, float fallback) {
   return __builtin_isfinite(value) && value >= 0 && value <= 1 ? value : fallback;
 }
@@ -724,7 +792,8 @@ __device__ inline double lit_radiance(const DevScene &s, const Hit &hit, const S
                                  : s.hapke.w;
   const double a = 1.0 + (double)(r.px * r.px) + (double)(r.py * r.py);
   const double cos4 = 1.0 / (a * a);
-  return (double)fmaxf(0.f, hapke_directional(f.normal, f.light, neg(r.ray), albedo, s.hapke) * pose.intensity) * cos4;
+  return (double)fmaxf(0.f, hapke_directional_fast(f.normal, f.light, neg(r.ray), albedo, s.hapke, s.hk) *
+                                pose.intensity) * cos4;
 }
 
 template <bool SHADE, class Count = NoCount>

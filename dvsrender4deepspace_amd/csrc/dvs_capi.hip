@@ -49,6 +49,7 @@ DevScene make_view(const Geometry &g, const dvs_camera &camera, const dvs_hapke 
   v.samples = camera.samples;
   v.fx = camera.fx; v.fy = camera.fy; v.cx = camera.cx; v.cy = camera.cy;
   v.hapke = hapke;
+  v.hk = dvsdev::hapke_constants(hapke);
   v.packet = g.ordered_ranges ? 1 : 0;
   return v;
 }

@@ -435,10 +435,12 @@ __global__ __launch_bounds__(256) void staged_scatter_kernel(const char *mirror,
 }
 
 // ---- unit kernels -------------------------------------------------------
+// The ray kernels' form of the reflectance (hapke_fast), whose constants
+// the host computes as for a scene.
 __global__ void hapke_kernel(int64_t n, const float *ci, const float *ce, const float *cg, const float *alb,
-                             dvs_hapke h, float *out) {
+                             dvs_hapke h, HapkeConsts c, float *out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = hapke_reflectance(ci[i], ce[i], cg[i], alb[i], h);
+  if (i < n) out[i] = hapke_fast(ci[i], ce[i], cg[i], alb[i], h, c);
 }
 __global__ void invert_hapke_kernel(int64_t n, const float *inc, const float *emi, const float *phase,
                                     const float *target, dvs_hapke h, float *out) {
@@ -721,7 +723,7 @@ hipError_t launch_staged_scatter(const char *mirror, const ScatterArgs &a, hipSt
 hipError_t launch_hapke(int64_t n, const float *ci, const float *ce, const float *cg, const float *alb,
                         const dvs_hapke &h, float *out, hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  hapke_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, ci, ce, cg, alb, h, out);
+  hapke_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, ci, ce, cg, alb, h, hapke_constants(h), out);
   return hipGetLastError();
 }
 hipError_t launch_invert_hapke(int64_t n, const float *inc, const float *emi, const float *phase, const float *target,

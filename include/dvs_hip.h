@@ -506,7 +506,9 @@ int32_t dvs_advance_pixel(dvs_pixel_state *state, const dvs_pixel_params *params
 
 /* ---- Device kernels exposed for unit parity (no reference counterpart;
  * they expose the reference's __host__ __device__ functions run on the GPU) */
-/* hapke_reflectance (include/dvs/hapke.hpp:131-151) over n inputs. */
+/* hapke_reflectance (include/dvs/hapke.hpp:131-151) over n inputs, in the
+ * form the ray kernels evaluate (the parameter-only terms computed once on
+ * the host). */
 int dvs_hapke_eval(const float *cosi, const float *cose, const float *cosg, const float *albedo,
                    int64_t n, const dvs_hapke *hapke, float *out);
 /* invert_hapke_if (include/dvs/hapke.hpp:160-172) over n inputs (the
