@@ -396,22 +396,24 @@ class Flight:
         from dvsrender4deepspace_amd import workloads
         t0 = time.perf_counter()
         while True:
-            ivs = workloads.horizon_intervals(w, self.traj, self.t_prev, self.step, self.batch)
-            check = float(ivs["t0"][0]) >= self.next_check
-            ok, ready = self.sensor.process_adaptive(ivs, d.DEFAULT_TOLERANCE, check)
+            # the window's horizon times; the sensor evaluates their states
+            # on the device inside the window (dvs_sensor_process_adaptive_times)
+            times = workloads.horizon_grid(w, self.t_prev, self.step, self.batch)
+            check = float(times[0]) >= self.next_check
+            ok, ready = self.sensor.process_adaptive_times(self.traj, times, d.DEFAULT_TOLERANCE, check)
             if ok:
                 break
             if self.batch > 1:
                 self.batch = max(1, self.batch // 2)
                 continue
-            ready = self.sensor.process(ivs)
+            ready = self.sensor.process(self.traj.intervals(times))
             break
         if check:
-            self.next_check = float(ivs["t1"][-1]) + d.CHECK_PERIOD
+            self.next_check = float(times[-1]) + d.CHECK_PERIOD
         dt = time.perf_counter() - t0
-        self.step += len(ivs)
-        self.t_prev = float(ivs["t1"][-1])
-        self.windows.append((dt, self.sensor.stats()["comparator_events"], ivs, check))
+        self.step += len(times) - 1
+        self.t_prev = float(times[-1])
+        self.windows.append((dt, self.sensor.stats()["comparator_events"], times, check))
         return dt
 
     def run_until(self, horizon: int):
@@ -708,7 +710,7 @@ def request_mode(args) -> dict:
         gpu = {"wall_s": sum(x[0] for x in first_windows), "events": first_windows[-1][1] - events_before}
         log(f"cpu baseline (oracle) on the first {k} windows ...")
         result["cpu_baseline"] = cpu_baseline(w, (v, n, nodes, root, leaf_idx),
-                                              [(x[2], x[3]) for x in first_windows], gpu)
+                                              [(traj.intervals(x[2]), x[3]) for x in first_windows], gpu)
     if not args.no_served:
         log("served vs plain ...")
         result["served_vs_plain"] = served_compare(args, args.served_requests, tuple(args.served_horizons))

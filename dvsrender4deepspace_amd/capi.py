@@ -247,6 +247,8 @@ def load_library() -> ctypes.CDLL:
                                          ctypes.POINTER(_I32)]),
         "dvs_batch_feed": (ctypes.c_int, [_P, _I32, _P, _I64]),
         "dvs_batch_feed_trajectory": (ctypes.c_int, [_P, _I32, _P, _P, _I64]),
+        "dvs_sensor_process_adaptive_times": (ctypes.c_int, [_P, _P, _P, _I32, ctypes.c_double, _I32,
+                                                             ctypes.POINTER(_I32), ctypes.POINTER(_I64)]),
         "dvs_batch_set_tuning": (ctypes.c_int, [_P, ctypes.POINTER(_TuningC)]),
         "dvs_batch_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(_TuningC)]),
         "dvs_sensor_set_tuning": (ctypes.c_int, [_P, ctypes.POINTER(_TuningC)]),
@@ -530,6 +532,19 @@ class Sensor:
         _check(load_library().dvs_sensor_process_adaptive(self.handle, _ptr(iv), len(iv), float(tolerance),
                                                           int(check_convergence), ctypes.byref(ok),
                                                           ctypes.byref(ready)))
+        return bool(ok.value), self._take(ready.value)
+
+    def process_adaptive_times(self, trajectory: "Trajectory", times, tolerance: float = 1e-2,
+                               check_convergence: bool = False) -> tuple[bool, np.ndarray]:
+        """process_adaptive over the intervals between consecutive `times`,
+        their states evaluated from `trajectory` on the device inside the
+        window (dvs_sensor_process_adaptive_times): the events
+        process_adaptive(trajectory.intervals(times)) gives."""
+        t = np.ascontiguousarray(times, dtype=np.float64)
+        ready, ok = _I64(), _I32()
+        _check(load_library().dvs_sensor_process_adaptive_times(self.handle, trajectory.handle, _ptr(t), len(t),
+                                                                float(tolerance), int(check_convergence),
+                                                                ctypes.byref(ok), ctypes.byref(ready)))
         return bool(ok.value), self._take(ready.value)
 
     def enable_diagnostics(self) -> None:

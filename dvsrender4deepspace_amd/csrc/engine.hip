@@ -1910,6 +1910,32 @@ int dvs_sensor_process_adaptive(dvs_sensor s, const dvs_optical_interval *ivs, i
   });
 }
 
+int dvs_sensor_process_adaptive_times(dvs_sensor s, dvs_trajectory trajectory, const double *times, int32_t n,
+                                      double tolerance, int32_t check_convergence, int32_t *accepted,
+                                      int64_t *ready_count) {
+  return guarded([&] {
+    if (!s) fail(DVS_E_ARGUMENT, "null sensor");
+    if (accepted) *accepted = 0;
+    if (!times || n < 2) fail(DVS_E_ARGUMENT, "empty adaptive optical horizon");
+    Segment &seg = s->engine->segment(s->id);
+    seg.clear_queue();
+    seg.fed_until = seg.last_time;
+    bool ok = false;
+    try {
+      s->engine->feed_trajectory(s->id, trajectory, times, n);
+      ok = s->engine->step_adaptive_once(s->id, tolerance, check_convergence != 0);
+    } catch (...) {
+      seg.clear_queue();
+      seg.fed_until = seg.last_time;
+      throw;
+    }
+    seg.clear_queue();
+    seg.fed_until = seg.last_time;
+    if (accepted) *accepted = ok ? 1 : 0;
+    if (ready_count) *ready_count = (int64_t)seg.ready.size();
+  });
+}
+
 int dvs_sensor_enable_diagnostics(dvs_sensor s) {
   return guarded([&] {
     if (!s) fail(DVS_E_ARGUMENT, "null sensor");

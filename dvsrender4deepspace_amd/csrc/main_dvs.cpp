@@ -7,9 +7,10 @@
 // the first window and every 0.25 s, a rejected window retried at half the
 // size and a rejected one-horizon window integrated natively, writes every
 // 1/imu_fps.  The trajectory states of a window's horizons are evaluated in
-// one device launch (CameraTrajectory::intervals) instead of one host
-// evaluation per horizon; each time is still evaluated once, at float(t),
-// and consecutive intervals share their common state.
+// one device launch instead of one host evaluation per horizon (inside the
+// adaptive window itself, or by CameraTrajectory::intervals for native
+// windows); each time is still evaluated once, at float(t), and consecutive
+// intervals share their common state.
 #include <algorithm>
 #include <cmath>
 #include <filesystem>
@@ -93,28 +94,29 @@ int run(int argc, char **argv) {
   while (step < total_steps) {
     window_times(step, previous_time, times);
     if (times.size() < 2) break;
-    intervals = traj.intervals(times);
     if (!initialized) {
       // the state at start_time, as the first interval starts from it
-      raycaster.initialize(intervals.front().start, start_time);
+      raycaster.initialize(traj.intervals({times[0], times[1]}).front().start, start_time);
       std::cout << "Initialized continuous photocurrent/circuit state from scene rays.\n";
       initialized = true;
     }
     const double first_time = previous_time;
     if (adaptive) {
+      // the window's states are evaluated on the device inside the window
       const bool check = first_time >= next_check;
-      if (!raycaster.advance_adaptive(intervals, kDefaultOpticalLogTolerance, events, check)) {
+      if (!raycaster.advance_adaptive(traj, times, kDefaultOpticalLogTolerance, events, check)) {
         if (batch > 1) {
           batch = std::max(1U, batch / 2);
           continue;
         }
-        raycaster.advance_many(intervals, events);
+        raycaster.advance_many(traj.intervals(times), events);
       }
       if (check) next_check = times.back() + .25;
     } else {
+      intervals = traj.intervals(times);
       raycaster.advance_many(intervals, events);
     }
-    const size_t advanced = intervals.size();
+    const size_t advanced = times.size() - 1;
     previous_time = times.back();
     std::move(events.begin(), events.end(), std::back_inserter(events_buffer));
     events.clear();

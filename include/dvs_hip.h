@@ -372,6 +372,17 @@ int dvs_batch_feed(dvs_batch batch, int32_t segment, const dvs_optical_interval 
  * intervals.  No reference counterpart: the reference feeds host intervals. */
 int dvs_batch_feed_trajectory(dvs_batch batch, int32_t segment, dvs_trajectory trajectory, const double *times,
                               int64_t n);
+/* dvs_sensor_process_adaptive over the n - 1 intervals between consecutive
+ * times (times[0] the end of what the sensor has integrated, strictly
+ * increasing) whose states the sensor evaluates from `trajectory` on the
+ * device inside the window's launch sequence: the window bin/dvs builds
+ * (apps/main_dvs.cpp:103-116) without the host evaluating, downloading and
+ * uploading its intervals.  The same events as dvs_sensor_process_adaptive
+ * fed dvs_trajectory_intervals of the same times.  No reference
+ * counterpart. */
+int dvs_sensor_process_adaptive_times(dvs_sensor sensor, dvs_trajectory trajectory, const double *times, int32_t n,
+                                      double tolerance, int32_t check_convergence, int32_t *accepted,
+                                      int64_t *ready_count);
 /* Launch shapes of the adaptive solver's kernels, for autotuning; results
  * do not depend on them.  No reference counterpart.
  *   rate_cells     circuit cells each photocurrent thread draws in one run

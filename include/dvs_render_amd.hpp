@@ -368,6 +368,9 @@ class CameraTrajectory {
     return out;
   }
 
+  // The C-ABI trajectory (for the windows the sensor evaluates on the device).
+  dvs_trajectory native_handle() const { return handle_.get(); }
+
  private:
   struct Range {
     float start = 0, end = 0;
@@ -509,6 +512,20 @@ class DvsRayCaster {
                                               reinterpret_cast<const dvs_optical_interval *>(intervals.data()),
                                               (int32_t)intervals.size(), tolerance, check_convergence ? 1 : 0,
                                               &accepted, &ready));
+    take(events, ready);
+    return accepted != 0;
+  }
+  // advance_adaptive over the intervals between consecutive `times` of
+  // `trajectory`, their states evaluated on the device inside the window
+  // (dvs_sensor_process_adaptive_times): the events advance_adaptive gives
+  // for trajectory.intervals(times).  No reference counterpart.
+  bool advance_adaptive(const CameraTrajectory &trajectory, const std::vector<double> &times, double tolerance,
+                        std::vector<Event> &events, bool check_convergence = false) {
+    int32_t accepted = 0;
+    int64_t ready = 0;
+    detail::check(dvs_sensor_process_adaptive_times(sensor_, trajectory.native_handle(), times.data(),
+                                                    (int32_t)times.size(), tolerance, check_convergence ? 1 : 0,
+                                                    &accepted, &ready));
     take(events, ready);
     return accepted != 0;
   }

@@ -82,3 +82,47 @@ def test_many_intervals_and_reuse(gpu, oracle):
     assert np.array_equal(a[:999], b)
     pick = np.random.default_rng(1).choice(len(times) - 1, 4000, replace=False)
     check_against(a["start"][pick], gpu.optical_states(o.evaluate(times[pick].astype(np.float32))))
+
+
+def test_sensor_windows_from_trajectory_times_equal_host_intervals(gpu):
+    """dvs_sensor_process_adaptive_times (states evaluated on the device
+    inside the window) gives the events of dvs_sensor_process_adaptive fed
+    dvs_trajectory_intervals of the same times, byte for byte, over bin/dvs
+    windows of the small request's flight with the receptor check on the
+    first window; a time grid that does not start where the sensor stands
+    is refused."""
+    import scenes
+    from dvsrender4deepspace_amd import driver, workloads
+    w = scenes.fast_pass(workloads.small_variant(workloads.PLAIN_SMALL, size=96, craters=6, samples=2))
+    v, n, nodes, root, idx = scenes.workload_mesh(w)
+    scene = gpu.Scene(nodes, root, v, n, idx, w.camera, w.hapke)
+    traj = workloads.trajectory(w)
+    grids, t, step = [], w.start_time, 0
+    for k in range(6):
+        times = workloads.horizon_grid(w, t, step, 20)
+        grids.append(times)
+        t, step = float(times[-1]), step + len(times) - 1
+    out = []
+    for form in ("intervals", "times"):
+        sensor = gpu.Sensor(scene, w.physic)
+        sensor.initialize(w.initial_state(), w.start_time)
+        got, oks = [], []
+        for k, times in enumerate(grids):
+            for check in ((True, False) if k == 0 else (False,)):
+                if form == "times":
+                    ok, ev = sensor.process_adaptive_times(traj, times, driver.DEFAULT_TOLERANCE, check)
+                else:
+                    ok, ev = sensor.process_adaptive(traj.intervals(times), driver.DEFAULT_TOLERANCE, check)
+                oks.append(ok)
+                if ok:
+                    break
+            assert ok
+            got.append(ev)
+        got.append(sensor.drain())
+        out.append((np.concatenate(got), oks))
+        if form == "times":
+            with pytest.raises(capi.InvalidArgument, match="positive and contiguous"):
+                sensor.process_adaptive_times(traj, grids[0], driver.DEFAULT_TOLERANCE, False)
+    assert out[0][1] == out[1][1]
+    assert len(out[0][0]) > 20
+    assert out[0][0].tobytes() == out[1][0].tobytes()
