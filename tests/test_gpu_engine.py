@@ -232,3 +232,55 @@ def test_sensor_errors_match_reference(gpu):
         t.process(fast.intervals(0, 40))
     with pytest.raises(capi.InvalidArgument, match="invalid sensor capacity"):
         gpu.Sensor(g, dataclasses.replace(w.physic, max_events_per_step=65))
+
+
+@pytest.mark.parametrize("adaptive", [False, True])
+def test_mixed_geometry_waves_equal_alone(gpu, adaptive):
+    """Segments of odd pixel counts on alternating terrains, so that waves of
+    the shared launches hold sub-rays of two geometries (the wave-packet
+    traversal serves them geometry by geometry): each segment's events in
+    the mix equal the segment run alone."""
+    specs = _segments()
+    base, other = specs[0][0], specs[2][0]
+    odd = [
+        (base, capi.Camera(33, 21, 30.0, 30.0, 16.5, 10.5, 2), base.physic, 30),
+        (other, capi.Camera(27, 19, 28.0, 28.0, 13.5, 9.5, 2),
+         dataclasses.replace(other.physic, threshold_pos=0.05, threshold_neg=0.05, max_events_per_step=64), 30),
+        (base, capi.Camera(35, 17, 32.0, 32.0, 17.5, 8.5, 2), dataclasses.replace(base.physic, random_seed=5), 30),
+        (other, capi.Camera(29, 23, 30.0, 30.0, 14.5, 11.5, 2),
+         dataclasses.replace(other.physic, photon_shot_noise=1, reset_thermal_noise=1), 30),
+    ]
+    assert any((c.width * c.height * c.samples ** 2) % 64 for _, c, _, _ in odd)
+    geo, views = _scenes(gpu, odd)
+    per_call = 10
+    batch = gpu.Batch(horizons_per_step=per_call)
+    ids = []
+    for (w, cam, phys, n), view in zip(odd, views):
+        sid = batch.add(view, phys, w.initial_state(), w.start_time)
+        if adaptive:
+            batch.set_adaptive(sid, 1e-2, per_call, 0.25)
+        batch.feed(sid, w.intervals(0, n))
+        ids.append(sid)
+    got = {sid: [] for sid in ids}
+    while batch.step():
+        for sid in ids:
+            got[sid].append(batch.take(sid))
+    total = 0
+    for sid, (w, cam, phys, n), view in zip(ids, odd, views):
+        got[sid] += [batch.take(sid), batch.drain(sid)]
+        mine = np.concatenate(got[sid])
+        if adaptive:
+            alone = gpu.Batch(horizons_per_step=per_call)
+            a = alone.add(view, phys, w.initial_state(), w.start_time)
+            alone.set_adaptive(a, 1e-2, per_call, 0.25)
+            alone.feed(a, w.intervals(0, n))
+            parts = []
+            while alone.step():
+                parts.append(alone.take(a))
+            ref = np.concatenate(parts + [alone.take(a), alone.drain(a)])
+            alone.close()
+        else:
+            ref = _alone(gpu, view, w, phys, n, per_call)
+        total += len(ref)
+        assert mine.tobytes() == ref.tobytes(), f"segment {sid} differs in the mix"
+    assert total > 50
