@@ -643,13 +643,21 @@ __device__ inline int circuit_cells(dvs_pixel_state &st, const CircuitParams &p,
 #pragma unroll
     for (int j = 0; j < kCellPrefetch; ++j) r[j] = *reinterpret_cast<const double2 *>(L + j * kCellWords + 2);
     int k[kCellPrefetch];
+    if (CODED) {
+      // every code of the block's cells tabulated (0, 1 or 2) in one test:
+      // no byte with a bit above the lowest two, nor both of them (3); k =
+      // code & 3 then picks the map (an untabulated code's index reads
+      // words of the next cell in the staged block: the bulk result is then
+      // not used)
+      static_assert(kMappedElectrons == 2, "codes 0..2 are tabulated");
+      const uint64_t live = n >= kCellPrefetch ? ~0ull : (1ull << (8 * n)) - 1;
+      const uint64_t w = use_codes & live;
+      ok = ok && (w & 0xFCFCFCFCFCFCFCFCull) == 0 && (w & (w >> 1) & 0x0101010101010101ull) == 0;
 #pragma unroll
-    for (int j = 0; j < kCellPrefetch; ++j) {
-      if (CODED) {
-        const int code = (int)((use_codes >> (8 * j)) & 0xff);
-        k[j] = code <= kMappedElectrons ? code : 0;
-        ok = ok && (j >= n || code <= kMappedElectrons);
-      } else {
+      for (int j = 0; j < kCellPrefetch; ++j) k[j] = (int)((use_codes >> (8 * j)) & 3);
+    } else {
+#pragma unroll
+      for (int j = 0; j < kCellPrefetch; ++j) {
         const double rate = use[j];
         const bool one = rate == r[j].x, two = rate == r[j].y;
         k[j] = one ? 1 : two ? 2 : 0;
@@ -658,11 +666,19 @@ __device__ inline int circuit_cells(dvs_pixel_state &st, const CircuitParams &p,
     }
 #pragma unroll
     for (int j = 0; j < kCellPrefetch; ++j) aq[j] = *reinterpret_cast<const double2 *>(L + j * kCellWords + 4 + 2 * k[j]);
+    if (n == kCellPrefetch) {  // a whole block: no cell past the range
 #pragma unroll
-    for (int j = 0; j < kCellPrefetch; ++j) {
-      if (j < n) {
+      for (int j = 0; j < kCellPrefetch; ++j) {
         y = aq[j].x * y + aq[j].y;
         in_range = in_range && y > s.y_pos && y < s.y_neg;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < kCellPrefetch; ++j) {
+        if (j < n) {
+          y = aq[j].x * y + aq[j].y;
+          in_range = in_range && y > s.y_pos && y < s.y_neg;
+        }
       }
     }
     const double first_t0 = L[0], last_t1 = L[(n - 1) * kCellWords + 1];
