@@ -221,6 +221,7 @@ def load_library() -> ctypes.CDLL:
                                            ctypes.POINTER(_I64)]),
         "dvs_raster_align": (ctypes.c_int, [_P, _P, _I32, _I32, _P, _P, _P, _I32, _I32, _P]),
         "dvs_sort_eval": (ctypes.c_int, [_P, _P, _I64, _I32, _I32, _P, _P]),
+        "dvs_sort_bucketed_eval": (ctypes.c_int, [_P, _P, _I64, ctypes.c_uint64, _I32, _I32, _P, _P, _P]),
         "dvs_scene_stats": (ctypes.c_int, [_P, ctypes.POINTER(_StatsC)]),
         "dvs_frame_traversal_stats": (ctypes.c_int, [_P, _P, _P]),
         "dvs_sensor_create": (ctypes.c_int, [_P, ctypes.POINTER(_PhysicC), _I32, _I32, ctypes.POINTER(_P)]),
@@ -1007,6 +1008,22 @@ def sort_eval(keys, values, begin_bit: int = 0, end_bit: int = 64):
     ko, vo = np.empty_like(k), np.empty_like(v)
     _check(load_library().dvs_sort_eval(_ptr(k), _ptr(v), len(k), begin_bit, end_bit, _ptr(ko), _ptr(vo)))
     return ko, vo
+
+
+def sort_bucketed_eval(keys, values, key_lo: int, key_bits: int, bucket_bits: int):
+    """The event sort's bucketed form: (keys, values, overflow) for keys in
+    [key_lo, key_lo + 2**key_bits), sorted stably by the top bucket_bits of
+    key - key_lo, then by the whole key inside each bucket; overflow is True
+    when a bucket held more than 64 pairs."""
+    k = np.ascontiguousarray(keys, dtype=np.uint64)
+    v = np.ascontiguousarray(values, dtype=np.uint32)
+    if len(k) != len(v):
+        raise ValueError("one value per key")
+    ko, vo = np.empty_like(k), np.empty_like(v)
+    flag = np.zeros(1, dtype=np.int32)
+    _check(load_library().dvs_sort_bucketed_eval(_ptr(k), _ptr(v), len(k), int(key_lo), key_bits, bucket_bits,
+                                                 _ptr(ko), _ptr(vo), _ptr(flag)))
+    return ko, vo, bool(flag[0])
 
 
 def raster_mesh(dem, x_coords, y_coords, albedo=None):

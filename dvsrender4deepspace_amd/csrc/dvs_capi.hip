@@ -620,6 +620,39 @@ int dvs_sort_eval(const uint64_t *keys, const uint32_t *values, int64_t n, int32
   });
 }
 
+int dvs_sort_bucketed_eval(const uint64_t *keys, const uint32_t *values, int64_t n, uint64_t key_lo,
+                           int32_t key_bits, int32_t bucket_bits, uint64_t *keys_out, uint32_t *values_out,
+                           int32_t *overflow) {
+  return guarded([&] {
+    if (n < 0 || (n > 0 && (!keys || !values || !keys_out || !values_out)) || !overflow || n > INT32_MAX ||
+        key_bits < 0 || key_bits > 64 || bucket_bits <= 0 || bucket_bits > 64)
+      fail(DVS_E_ARGUMENT, "invalid sort arguments");
+    *overflow = 0;
+    if (n == 0) return;
+    DeviceArray<unsigned long long> k, k2;
+    DeviceArray<unsigned> v, v2;
+    DeviceArray<int> flag;
+    DeviceArray<unsigned char> temp;
+    k.upload(reinterpret_cast<const unsigned long long *>(keys), n, "sort keys");
+    v.upload(values, n, "sort values");
+    k2.reserve(n, "sort keys");
+    v2.reserve(n, "sort values");
+    const int zero = 0;
+    flag.upload(&zero, 1, "sort flag");
+    size_t bytes = 0;
+    hip_check(sort_pairs_u64_bucketed(nullptr, bytes, k.ptr, k2.ptr, v.ptr, v2.ptr, n, key_lo, key_bits, bucket_bits,
+                                      flag.ptr, nullptr),
+              "sort size");
+    temp.reserve(bytes, "sort scratch");
+    hip_check(sort_pairs_u64_bucketed(temp.ptr, bytes, k.ptr, k2.ptr, v.ptr, v2.ptr, n, key_lo, key_bits, bucket_bits,
+                                      flag.ptr, nullptr),
+              "sort");
+    hip_check(hipMemcpy(keys_out, k2.ptr, n * sizeof(uint64_t), hipMemcpyDeviceToHost), "sort output");
+    hip_check(hipMemcpy(values_out, v2.ptr, n * sizeof(uint32_t), hipMemcpyDeviceToHost), "sort output");
+    hip_check(hipMemcpy(overflow, flag.ptr, sizeof(int), hipMemcpyDeviceToHost), "sort output");
+  });
+}
+
 int dvs_noise_eval(uint64_t seed, int64_t pixels, double lambda, int32_t draws, double *poisson, double *normal) {
   return guarded([&] {
     if (pixels < 0 || draws < 0) fail(DVS_E_ARGUMENT, "invalid noise arguments");

@@ -153,6 +153,9 @@ struct Readout {
   void drain(std::vector<dvs_event> &ready) { deliver(count, ready); }
 };
 
+// Bits of the bucket the event sort orders by first (two radix passes).
+constexpr int kTimeBucketBits = 16;
+
 // The order-preserving 64-bit key of an IEEE double (as the compaction
 // kernel builds it for event times).
 inline unsigned long long time_key(double t) {
@@ -994,22 +997,37 @@ class Engine {
     keyflag_.reserve(1, "event key flag");
     // Sort by time (stably, so ties keep the compaction's (segment, y, x)
     // order), then stably by segment: (segment, t, y, x); permute the
-    // records and copy them (and their segments) to the host.
+    // records and copy them (and their segments) to the host.  Over the
+    // window's span the time sort takes two passes: by the top 16 of the
+    // key bits that vary (a bucket is a few tens of nanoseconds), then
+    // inside each bucket by the whole key; a bucket of more than 64 events
+    // raises the key flag, and the events are then sorted on all bits.
     auto sort_events = [&](int end_bit) {
-      size_t bytes = 0;
+      size_t bytes = 0, bucketed = 0;
       hip_check(sort_pairs_u64(nullptr, bytes, keys_.ptr, keys2_.ptr, order_.ptr, order2_.ptr, event_room, 0, 64, st),
                 "sort size");
+      hip_check(sort_pairs_u64_bucketed(nullptr, bucketed, keys_.ptr, keys2_.ptr, order_.ptr, order2_.ptr, event_room,
+                                        kmin, 64, kTimeBucketBits, keyflag_.ptr, st),
+                "sort size");
+      bytes = std::max(bytes, bucketed);
       temp_.reserve_ahead(bytes, "sort scratch");
-      hip_check(sort_pairs_u64(temp_.ptr, bytes, keys_.ptr, keys2_.ptr, order_.ptr, order2_.ptr, total, 0, end_bit,
-                               st),
-                "event sort");
-      const unsigned *final_order = order2_.ptr;
-      if (nseg > 1) {
-        hip_check(launch_segment_keys(total, order2_.ptr, ev_seg_.ptr, keys_.ptr, st), "segment keys");
-        hip_check(sort_pairs_u64(temp_.ptr, bytes, keys_.ptr, keys2_.ptr, order2_.ptr, order_.ptr, total, 0, seg_bits,
+      if (end_bit < 64 && end_bit > kTimeBucketBits)
+        hip_check(sort_pairs_u64_bucketed(temp_.ptr, bytes, keys_.ptr, keys2_.ptr, order_.ptr, order2_.ptr, total,
+                                          kmin, end_bit, kTimeBucketBits, keyflag_.ptr, st),
+                  "event sort");
+      else
+        hip_check(sort_pairs_u64(temp_.ptr, bytes, keys_.ptr, keys2_.ptr, order_.ptr, order2_.ptr, total, 0, end_bit,
                                  st),
                   "event sort");
-        final_order = order_.ptr;
+      const unsigned *final_order = order2_.ptr;
+      if (nseg > 1) {
+        // in buffers of their own: the time keys and the compaction order
+        // stay for a sort on all bits after a raised key flag
+        hip_check(launch_segment_keys(total, order2_.ptr, ev_seg_.ptr, seg_keys_.ptr, st), "segment keys");
+        hip_check(sort_pairs_u64(temp_.ptr, bytes, seg_keys_.ptr, seg_keys2_.ptr, order2_.ptr, seg_order_.ptr, total,
+                                 0, seg_bits, st),
+                  "event sort");
+        final_order = seg_order_.ptr;
       }
       hip_check(launch_permute_multi(segtab_.ptr, total, final_order, ev_time_.ptr, ev_seg_.ptr, ev_pixel_.ptr,
                                      ev_pol_.ptr, ev_depth_.ptr, events_out_.ptr, events_seg_.ptr, st),
@@ -1031,6 +1049,11 @@ class Engine {
       ev_poses_.reserve_ahead(event_room, "event poses");
       keys_.reserve_ahead(event_room, "event keys");
       keys2_.reserve_ahead(event_room, "event keys");
+      if (nseg > 1) {
+        seg_keys_.reserve_ahead(event_room, "event segment keys");
+        seg_keys2_.reserve_ahead(event_room, "event segment keys");
+        seg_order_.reserve_ahead(event_room, "event order");
+      }
       order_.reserve_ahead(event_room, "event order");
       order2_.reserve_ahead(event_room, "event order");
       events_out_.reserve_ahead(event_room, "events");
@@ -1634,8 +1657,8 @@ class Engine {
   DeviceArray<CellMaps> maps_;
   DeviceArray<Pose> poses_, ev_poses_;
   DeviceArray<float> ev_depth_;
-  DeviceArray<unsigned long long> keys_, keys2_, counter_;
-  DeviceArray<unsigned> order_, order2_;
+  DeviceArray<unsigned long long> keys_, keys2_, seg_keys_, seg_keys2_, counter_;
+  DeviceArray<unsigned> order_, order2_, seg_order_;
   DeviceArray<dvs_event> events_out_;
   DeviceArray<unsigned char> temp_;
   hipGraphExec_t graph_exec_ = nullptr;

@@ -18,6 +18,12 @@
 // the running count from round to round.  Equal keys keep their input
 // order, so sorting by time after compaction in (segment, y, x) order gives
 // the (segment, t, y, x) order of the readout.
+//
+// Bucketed sort, for keys known to lie in [lo, lo + 2^bits): the radix
+// passes above over the top bucket_bits of (key - lo) only, then one pass in
+// which every item ranks itself among its bucket's items by the whole key.
+// A bucket of more than 64 items is reported to the caller, which then sorts
+// on all bits.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -107,14 +113,19 @@ __global__ __launch_bounds__(kThreads) void tile_scan_kernel(const int *in, int 
 }
 
 // The digit of a pass: bits [shift, shift + 8) clipped to the sort's end bit
-// (digit_mask), so that bits at or above end_bit never take part.
-__device__ inline unsigned digit_of(unsigned long long key, int shift, unsigned digit_mask) {
-  return (unsigned)(key >> shift) & digit_mask;
+// (digit_mask), so that bits at or above end_bit never take part.  A
+// bucketed sort sorts by (key - sub) >> pre instead of the key itself.
+struct KeyView {
+  unsigned long long sub;
+  int pre;
+};
+__device__ inline unsigned digit_of(unsigned long long key, int shift, unsigned digit_mask, KeyView kv) {
+  return (unsigned)(((key - kv.sub) >> kv.pre) >> shift) & digit_mask;
 }
 
 // hist[d * tiles + tile] = number of keys of the tile with digit d.
 __global__ __launch_bounds__(kThreads) void digit_hist_kernel(const unsigned long long *keys, long long n, int shift,
-                                                              unsigned digit_mask, int tiles, int *hist) {
+                                                              unsigned digit_mask, KeyView kv, int tiles, int *hist) {
   __shared__ int counts[kDigits];
   counts[threadIdx.x] = 0;
   __syncthreads();
@@ -122,7 +133,7 @@ __global__ __launch_bounds__(kThreads) void digit_hist_kernel(const unsigned lon
 #pragma unroll
   for (int r = 0; r < kItems; ++r) {
     const long long i = base + r * kThreads + threadIdx.x;
-    if (i < n) atomicAdd(&counts[digit_of(keys[i], shift, digit_mask)], 1);
+    if (i < n) atomicAdd(&counts[digit_of(keys[i], shift, digit_mask, kv)], 1);
   }
   __syncthreads();
   hist[(long long)threadIdx.x * tiles + blockIdx.x] = counts[threadIdx.x];
@@ -132,8 +143,8 @@ __global__ __launch_bounds__(kThreads) void digit_hist_kernel(const unsigned lon
 // of `hist`.
 __global__ __launch_bounds__(kThreads) void digit_scatter_kernel(const unsigned long long *keys_in,
                                                                  const unsigned *vals_in, long long n, int shift,
-                                                                 unsigned digit_mask, int tiles, const int *hist,
-                                                                 const int *incl,
+                                                                 unsigned digit_mask, KeyView kv, int tiles,
+                                                                 const int *hist, const int *incl,
                                                                  unsigned long long *keys_out, unsigned *vals_out) {
   __shared__ int base[kDigits];
   __shared__ int run[kDigits];
@@ -154,7 +165,7 @@ __global__ __launch_bounds__(kThreads) void digit_scatter_kernel(const unsigned 
     const bool valid = i < n;
     const unsigned long long key = valid ? keys_in[i] : 0ull;
     const unsigned val = valid ? vals_in[i] : 0u;
-    const unsigned d = digit_of(key, shift, digit_mask);
+    const unsigned d = digit_of(key, shift, digit_mask, kv);
     unsigned long long peers = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
@@ -194,6 +205,39 @@ __global__ void copy_pairs_kernel(const unsigned long long *keys_in, const unsig
   }
 }
 
+// After a stable sort by bucket b(key) = (key - sub) >> pre: every item
+// finds its bucket's extent (at most kBucketCap items either side), ranks
+// itself among the bucket's items by (key, position) and lands at the
+// bucket's start plus that rank, out of place.  Equal keys keep their order,
+// so the output is the stable sort by the whole key.  An item of a bucket
+// of more than kBucketCap items is copied in place and sets *flag (the
+// caller then sorts on all key bits).
+constexpr int kBucketCap = 64;
+__global__ __launch_bounds__(256) void bucket_rank_kernel(const unsigned long long *keys_in, const unsigned *vals_in,
+                                                          long long n, KeyView kv, unsigned long long *keys_out,
+                                                          unsigned *vals_out, int *flag) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned long long key = keys_in[i];
+  const unsigned long long b = (key - kv.sub) >> kv.pre;
+  long long s = i, e = i + 1;
+  while (s > 0 && i - s < kBucketCap && ((keys_in[s - 1] - kv.sub) >> kv.pre) == b) --s;
+  while (e < n && e - s <= kBucketCap && ((keys_in[e] - kv.sub) >> kv.pre) == b) ++e;
+  const bool over = (s > 0 && ((keys_in[s - 1] - kv.sub) >> kv.pre) == b) || e - s > kBucketCap;
+  long long at = i;
+  if (over) {
+    atomicOr(flag, 1);
+  } else {
+    at = s;
+    for (long long j = s; j < e; ++j) {
+      const unsigned long long k = keys_in[j];
+      at += (k < key || (k == key && j < i)) ? 1 : 0;
+    }
+  }
+  keys_out[at] = key;
+  vals_out[at] = vals_in[i];
+}
+
 inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 inline long long tiles_of(long long n) { return (n + kTile - 1) / kTile; }
 
@@ -225,9 +269,10 @@ hipError_t inclusive_scan_int(void *temp, size_t &temp_bytes, const int *in, int
   return scan_launch(in, out, (long long)n, static_cast<int *>(temp), st);
 }
 
-hipError_t sort_pairs_u64(void *temp, size_t &temp_bytes, const unsigned long long *keys_in,
-                          unsigned long long *keys_out, const unsigned *values_in, unsigned *values_out, size_t n,
-                          int begin_bit, int end_bit, hipStream_t st) {
+namespace {
+hipError_t sort_passes(void *temp, size_t &temp_bytes, const unsigned long long *keys_in, unsigned long long *keys_out,
+                       const unsigned *values_in, unsigned *values_out, size_t n, int begin_bit, int end_bit,
+                       KeyView kv, hipStream_t st) {
   const long long tiles = std::max<long long>(tiles_of((long long)n), 1);
   const size_t hist_n = (size_t)kDigits * tiles;
   const size_t b_keys = align256(sizeof(unsigned long long) * std::max<size_t>(n, 1));
@@ -264,17 +309,58 @@ hipError_t sort_pairs_u64(void *temp, size_t &temp_bytes, const unsigned long lo
     const int shift = begin_bit + 8 * pass;
     const int width = std::min(8, end_bit - shift);
     const unsigned digit_mask = (1u << width) - 1u;
-    digit_hist_kernel<<<(unsigned)tiles, kThreads, 0, st>>>(src_k, count, shift, digit_mask, (int)tiles, hist);
+    digit_hist_kernel<<<(unsigned)tiles, kThreads, 0, st>>>(src_k, count, shift, digit_mask, kv, (int)tiles, hist);
     hipError_t e = scan_launch(hist, incl, (long long)hist_n, sums, st);
     if (e != hipSuccess) return e;
-    digit_scatter_kernel<<<(unsigned)tiles, kThreads, 0, st>>>(src_k, src_v, count, shift, digit_mask, (int)tiles,
-                                                               hist, incl, dst_k, dst_v);
+    digit_scatter_kernel<<<(unsigned)tiles, kThreads, 0, st>>>(src_k, src_v, count, shift, digit_mask, kv,
+                                                               (int)tiles, hist, incl, dst_k, dst_v);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     src_k = dst_k;
     src_v = dst_v;
   }
   return hipSuccess;
+}
+}  // namespace
+
+hipError_t sort_pairs_u64(void *temp, size_t &temp_bytes, const unsigned long long *keys_in,
+                          unsigned long long *keys_out, const unsigned *values_in, unsigned *values_out, size_t n,
+                          int begin_bit, int end_bit, hipStream_t st) {
+  return sort_passes(temp, temp_bytes, keys_in, keys_out, values_in, values_out, n, begin_bit, end_bit,
+                     KeyView{0, 0}, st);
+}
+
+hipError_t sort_pairs_u64_bucketed(void *temp, size_t &temp_bytes, const unsigned long long *keys_in,
+                                   unsigned long long *keys_out, const unsigned *values_in, unsigned *values_out,
+                                   size_t n, unsigned long long key_lo, int key_bits, int bucket_bits, int *flag,
+                                   hipStream_t st) {
+  if (key_bits < 0 || key_bits > 64 || bucket_bits <= 0 || bucket_bits > 64) return hipErrorInvalidValue;
+  const int pre = key_bits > bucket_bits ? key_bits - bucket_bits : 0;
+  const KeyView kv{key_lo, pre};
+  const int bits = std::min(bucket_bits, key_bits);
+  // [bucket-sorted keys][their values][the radix passes' scratch]
+  const size_t b_keys = align256(sizeof(unsigned long long) * std::max<size_t>(n, 1));
+  const size_t b_vals = align256(sizeof(unsigned) * std::max<size_t>(n, 1));
+  size_t pass_bytes = 0;
+  hipError_t e = sort_passes(nullptr, pass_bytes, keys_in, keys_out, values_in, values_out, n, 0, bits, kv, st);
+  if (e != hipSuccess) return e;
+  if (!temp) {
+    temp_bytes = b_keys + b_vals + pass_bytes;
+    return hipSuccess;
+  }
+  if (temp_bytes < b_keys + b_vals + pass_bytes || !flag) return hipErrorInvalidValue;
+  if (n == 0) return hipSuccess;
+  if (pre == 0)  // the bucket is the whole key
+    return sort_passes(static_cast<char *>(temp) + b_keys + b_vals, pass_bytes, keys_in, keys_out, values_in,
+                       values_out, n, 0, bits, kv, st);
+  char *p = static_cast<char *>(temp);
+  unsigned long long *bkeys = reinterpret_cast<unsigned long long *>(p);
+  unsigned *bvals = reinterpret_cast<unsigned *>(p + b_keys);
+  e = sort_passes(p + b_keys + b_vals, pass_bytes, keys_in, bkeys, values_in, bvals, n, 0, bits, kv, st);
+  if (e != hipSuccess) return e;
+  bucket_rank_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(bkeys, bvals, (long long)n, kv, keys_out,
+                                                                   values_out, flag);
+  return hipGetLastError();
 }
 
 }  // namespace dvsamd

@@ -553,6 +553,13 @@ int dvs_noise_eval(uint64_t seed, int64_t pixels, double lambda, int32_t draws, 
 int dvs_scan_eval(const int32_t *in, int64_t n, int32_t *out);
 int dvs_sort_eval(const uint64_t *keys, const uint32_t *values, int64_t n, int32_t begin_bit, int32_t end_bit,
                   uint64_t *keys_out, uint32_t *values_out);
+/* The event sort's bucketed form over keys in [key_lo, key_lo + 2^key_bits):
+ * stably by the top bucket_bits of (key - key_lo), then inside each bucket by
+ * the whole key.  *overflow = 1 when a bucket held more than 64 pairs (that
+ * bucket is then left in input order). */
+int dvs_sort_bucketed_eval(const uint64_t *keys, const uint32_t *values, int64_t n, uint64_t key_lo,
+                           int32_t key_bits, int32_t bucket_bits, uint64_t *keys_out, uint32_t *values_out,
+                           int32_t *overflow);
 /* Nearest / shadow queries of n rays against a scene (traversal parity). */
 int dvs_trace_rays(dvs_scene scene, const float *origins, const float *directions, int64_t n,
                    int32_t *hit, int32_t *triangle, float *t, float *u, float *v);

@@ -1,8 +1,9 @@
 """GPU: the readout on the device -- the event compaction's hand-written
-prefix sum and stable radix sort (scan_sort.hip) against numpy, and the
-device FIFO (readout.hip, src/event_readout.cpp:6-34) against the oracle's
-FIFO with a backlog of thousands of events: many refills of the staged
-chunks, ring growth, drops once the queue is full, then the final drain.
+prefix sum, stable radix sort and bucketed time sort (scan_sort.hip)
+against numpy, and the device FIFO (readout.hip,
+src/event_readout.cpp:6-34) against the oracle's FIFO with a backlog of
+thousands of events: many refills of the staged chunks, ring growth, drops
+once the queue is full, then the final drain.
 """
 from __future__ import annotations
 
@@ -53,6 +54,75 @@ def test_sort_of_event_time_keys(gpu):
     bits = int(lo ^ hi).bit_length()
     assert bits < 45
     ko, vo = gpu.sort_eval(keys, np.arange(len(keys), dtype=np.uint32), 0, bits)
+    assert np.array_equal(vo, np.argsort(times, kind="stable"))
+
+
+def _bucketed_expectation(keys, lo, key_bits, bucket_bits):
+    """Stable order by bucket, then by key inside buckets of at most 64 keys
+    (larger buckets keep their input order), and whether one was larger."""
+    bucket = (keys - np.uint64(lo)) >> np.uint64(key_bits - bucket_bits)
+    by_bucket = np.argsort(bucket, kind="stable")
+    _, first, sizes = np.unique(bucket[by_bucket], return_index=True, return_counts=True)
+    order = by_bucket.copy()
+    for f, c in zip(first, sizes):
+        if c <= 64:
+            part = by_bucket[f:f + c]
+            order[f:f + c] = part[np.argsort(keys[part], kind="stable")]
+    return order, bool((sizes > 64).any())
+
+
+@pytest.mark.parametrize("n", [1, 300, 100_000])
+def test_bucketed_sort_is_stable_sort(gpu, n):
+    """Keys in a 2^40 span with clusters and ties: the two bucket passes and
+    the in-bucket ranks give the stable sort by the whole key."""
+    rng = np.random.default_rng(n + 7)
+    lo = 0xBFF4000000000000
+    keys = np.uint64(lo) + rng.integers(0, 2**40, n, dtype=np.uint64)
+    near = rng.random(n) < 0.2  # clusters a few keys wide around one key
+    keys[near] = keys[0] + rng.integers(0, 2**22, int(near.sum()), dtype=np.uint64)
+    keys[rng.random(n) < 0.1] = keys[-1]  # ties
+    keys = np.minimum(keys, np.uint64(lo + 2**40 - 1))
+    values = np.arange(n, dtype=np.uint32)
+    ko, vo, over = gpu.sort_bucketed_eval(keys, values, lo, 40, 16)
+    order, expect_over = _bucketed_expectation(keys, lo, 40, 16)
+    assert over == expect_over
+    assert np.array_equal(vo, values[order])
+    assert np.array_equal(ko, keys[order])
+    if not over:
+        assert np.array_equal(vo, np.argsort(keys, kind="stable"))
+
+
+@pytest.mark.parametrize("size", [64, 65, 500])
+def test_bucketed_sort_bucket_limit(gpu, size):
+    """A bucket of 64 keys is sorted inside; one of 65 or more is left in
+    input order and reported, while the other buckets are still sorted."""
+    rng = np.random.default_rng(size)
+    lo, key_bits, bucket_bits = 1 << 50, 32, 12
+    width = 1 << (key_bits - bucket_bits)
+    spread = np.uint64(lo) + rng.integers(0, 2**key_bits, 5000, dtype=np.uint64)
+    full = np.uint64(lo + 77 * width) + rng.integers(0, width, size, dtype=np.uint64)
+    keys = rng.permutation(np.concatenate([spread, full]))
+    values = np.arange(len(keys), dtype=np.uint32)
+    ko, vo, over = gpu.sort_bucketed_eval(keys, values, lo, key_bits, bucket_bits)
+    order, expect_over = _bucketed_expectation(keys, lo, key_bits, bucket_bits)
+    assert expect_over == (size + int(((spread - np.uint64(lo)) // np.uint64(width) == 77).sum()) > 64)
+    assert over == expect_over
+    assert np.array_equal(vo, values[order])
+    assert np.array_equal(ko, keys[order])
+
+
+def test_bucketed_sort_of_event_time_keys(gpu):
+    """The event sort's form: time keys of one window from the key of its
+    start, 16 bucket bits."""
+    rng = np.random.default_rng(6)
+    t0, t1 = 1.25, 1.252
+    times = np.sort(rng.uniform(t0, t1, 50_000))
+    times = rng.permutation(np.concatenate([times, times[:5000]]))  # ties
+    keys = times.view(np.uint64) | np.uint64(1 << 63)
+    lo, hi = (np.array([t0, t1]).view(np.uint64) | np.uint64(1 << 63))
+    bits = int(lo ^ hi).bit_length()
+    ko, vo, over = gpu.sort_bucketed_eval(keys, np.arange(len(keys), dtype=np.uint32), int(lo), bits, 16)
+    assert not over
     assert np.array_equal(vo, np.argsort(times, kind="stable"))
 
 
