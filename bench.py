@@ -64,21 +64,21 @@ SLICES = 20
 
 # Per-unit counter figures of this build's kernels on this workload, from
 # rocprofv3 --pmc runs of `python3 bench.py --steps 1 --warmup 0
-# --no-cpu-baseline --no-served` (the flight's first twentieth; the
+# --no-cpu-baseline --no-served --no-batched` (the flight's first twentieth; the
 # summary with its commands is profiles/adaptive_window_davis346_counters.md):
 # VALU instructions (SQ_INSTS_VALU), VALUBusy, lane utilisation
 # (VALUUtilization), L2 hit rate (TCC_HIT / (TCC_HIT + TCC_MISS)) and HBM
 # bytes (FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0: a 1 GiB device copy reads
 # FETCH_SIZE 0.5 GiB and WRITE_SIZE 1 GiB), per unit of each kernel's work.
 PROFILE = {
-    "probe": {"unit": "traced pixel sample", "valu_insts": 1757.05, "valu_busy": 1.273, "lane_util": 0.867,
-              "l2_hit": 0.871, "fetch": 293.49, "write": 23.73},
-    "query": {"unit": "traced pixel sample", "valu_insts": 2082.73, "valu_busy": 0.574, "lane_util": 0.775,
-              "l2_hit": 0.830, "fetch": 660.0, "write": 12.0},
-    "rates": {"unit": "pixel cell", "valu_insts": 2.2363, "valu_busy": 0.821, "lane_util": 0.903,
-              "l2_hit": 0.682, "fetch": 0.1578, "write": 1.001},
-    "integrate": {"unit": "pixel cell", "valu_insts": 0.4639, "valu_busy": 0.268, "lane_util": 0.581,
-                  "l2_hit": 0.789, "fetch": 1.1044, "write": 0.0366},
+    "probe": {"unit": "traced pixel sample", "valu_insts": 1698.36, "valu_busy": 1.273, "lane_util": 0.864,
+              "l2_hit": 0.871, "fetch": 293.38, "write": 23.70},
+    "query": {"unit": "traced pixel sample", "valu_insts": 2019.91, "valu_busy": 0.565, "lane_util": 0.778,
+              "l2_hit": 0.830, "fetch": 659.29, "write": 12.00},
+    "rates": {"unit": "pixel cell", "valu_insts": 2.2363, "valu_busy": 0.828, "lane_util": 0.903,
+              "l2_hit": 0.683, "fetch": 0.1519, "write": 1.0010},
+    "integrate": {"unit": "pixel cell", "valu_insts": 0.3773, "valu_busy": 0.249, "lane_util": 0.508,
+                  "l2_hit": 0.787, "fetch": 1.1097, "write": 0.0366},
 }
 
 
