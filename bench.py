@@ -385,9 +385,10 @@ class Flight:
     halved (the smaller window is kept), a rejected one-horizon window
     integrated natively."""
 
-    def __init__(self, w, traj, sensor, batch):
+    def __init__(self, w, traj, sensor, batch, ahead=True):
         from dvsrender4deepspace_amd import driver
         self.w, self.traj, self.sensor, self.batch, self.driver = w, traj, sensor, batch, driver
+        self.ahead = ahead  # queue each expected next window's setup (dvs_sensor_expect_adaptive_times)
         self.step, self.t_prev, self.next_check = 0, w.start_time, w.start_time
         self.windows = []  # (seconds, comparator events, intervals, checked)
 
@@ -400,6 +401,13 @@ class Flight:
             # on the device inside the window (dvs_sensor_process_adaptive_times)
             times = workloads.horizon_grid(w, self.t_prev, self.step, self.batch)
             check = float(times[0]) >= self.next_check
+            # the next window if this one is accepted: its setup is queued
+            # on the device before this call returns
+            step_next = self.step + len(times) - 1
+            if self.ahead and step_next < w.total_steps:
+                nxt = workloads.horizon_grid(w, float(times[-1]), step_next, self.batch)
+                if len(nxt) > 1:
+                    self.sensor.expect_adaptive_times(self.traj, nxt, d.DEFAULT_TOLERANCE)
             ok, ready = self.sensor.process_adaptive_times(self.traj, times, d.DEFAULT_TOLERANCE, check)
             if ok:
                 break

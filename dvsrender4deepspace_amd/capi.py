@@ -154,7 +154,7 @@ class _StatsC(ctypes.Structure):
                 ("query_ms", ctypes.c_double), ("probe_launches", ctypes.c_double),
                 ("query_launches", ctypes.c_double), ("probe_samples", ctypes.c_uint64),
                 ("query_samples", ctypes.c_uint64), ("readout_tasks", ctypes.c_uint64),
-                ("readout_serial_tasks", ctypes.c_uint64)]
+                ("readout_serial_tasks", ctypes.c_uint64), ("windows_set_up_ahead", ctypes.c_uint64)]
 
 
 class _RefineC(ctypes.Structure):
@@ -250,6 +250,7 @@ def load_library() -> ctypes.CDLL:
         "dvs_batch_feed_trajectory": (ctypes.c_int, [_P, _I32, _P, _P, _I64]),
         "dvs_sensor_process_adaptive_times": (ctypes.c_int, [_P, _P, _P, _I32, ctypes.c_double, _I32,
                                                              ctypes.POINTER(_I32), ctypes.POINTER(_I64)]),
+        "dvs_sensor_expect_adaptive_times": (ctypes.c_int, [_P, _P, _P, _I32, ctypes.c_double]),
         "dvs_batch_set_tuning": (ctypes.c_int, [_P, ctypes.POINTER(_TuningC)]),
         "dvs_batch_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(_TuningC)]),
         "dvs_sensor_set_tuning": (ctypes.c_int, [_P, ctypes.POINTER(_TuningC)]),
@@ -547,6 +548,17 @@ class Sensor:
                                                                 float(tolerance), int(check_convergence),
                                                                 ctypes.byref(ok), ctypes.byref(ready)))
         return bool(ok.value), self._take(ready.value)
+
+    def expect_adaptive_times(self, trajectory: "Trajectory", times, tolerance: float = 1e-2) -> None:
+        """The window the next process_adaptive_times call will most likely
+        run (dvs_sensor_expect_adaptive_times).  When the current call's window
+        is accepted, this window's setup (its horizon states, cell tables and
+        initial probes) is queued on the device before the call returns; a
+        next call with other times, trajectory or tolerance runs its own.
+        Events are the same either way.  `times` of length 0 forgets it."""
+        t = np.ascontiguousarray(times, dtype=np.float64)
+        _check(load_library().dvs_sensor_expect_adaptive_times(self.handle, trajectory.handle if len(t) else None,
+                                                                 _ptr(t), len(t), float(tolerance)))
 
     def enable_diagnostics(self) -> None:
         """EventSensor::enable_numerical_diagnostics: every later adaptive

@@ -406,6 +406,27 @@ int gpu(const std::string &out) {
     moving.advance_many(ivs, ev);
     moving.drain(ev);
     expect(moving.dropped_events() == 0, "trajectory window advances");
+    // two adaptive windows of 100 horizons along the trajectory, the second
+    // expected (set up on the device at the end of the first) or not: the
+    // same events
+    std::vector<std::vector<dvs::Event>> runs;
+    for (int form = 0; form < 2; ++form) {
+      dvs::DvsRayCaster caster(scene, config);
+      caster.initialize(trajectory.evaluate(0), 0.0);
+      std::vector<dvs::Event> got;
+      const std::vector<double> first(times.begin(), times.begin() + 101), second(times.begin() + 100, times.end());
+      if (form == 1) caster.expect_adaptive(trajectory, second, 1e-2);
+      bool ok = caster.advance_adaptive(trajectory, first, 1e-2, got);
+      ok = ok && caster.advance_adaptive(trajectory, second, 1e-2, got);
+      caster.drain(got);
+      expect(ok, "adaptive trajectory windows accepted");
+      runs.push_back(got);
+    }
+    bool equal = runs[0].size() == runs[1].size();
+    for (size_t k = 0; equal && k < runs[0].size(); ++k)
+      equal = runs[0][k].timestamp == runs[1][k].timestamp && runs[0][k].x == runs[1][k].x &&
+              runs[0][k].y == runs[1][k].y && runs[0][k].polarity == runs[1][k].polarity;
+    expect(equal, "a window set up ahead gives the same events");
   }
   {
     // EventSensor / OpticalSampler on the same scene view, against the

@@ -19,6 +19,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <limits>
 #include <map>
 #include <memory>
@@ -235,6 +236,28 @@ struct StepSegment {
   bool check = false;
 };
 
+// A queued adaptive window setup (Engine::setup_adaptive): what the rest of
+// the window needs, and for a setup queued ahead of its window, the window
+// and the segment state it was made from.
+struct AdaptSetup {
+  Timer step_t, refine_t, probe_t;
+  size_t lds = 0;
+  int pcur = 0;
+  std::vector<SegDev> tables;
+  std::vector<AdaptDev> ad;
+  std::vector<StepSegment> work;
+  int cell_total = 0, capacity = 0;
+  uint64_t cells_done = 0;
+};
+
+// The window a caller expects to run next on a segment.
+struct Expected {
+  int id = -1;
+  dvs_trajectory trajectory = nullptr;
+  std::vector<double> times;
+  double tolerance = 0;
+};
+
 struct GraphKey {
   std::vector<long long> shape;
   bool operator==(const GraphKey &o) const { return shape == o.shape; }
@@ -299,6 +322,9 @@ class Engine {
   void initialize(int id, const dvs_optical_state &pose, double timestamp) {
     Segment &s = segment(id);
     if (!std::isfinite(timestamp) || s.initialized) fail(DVS_E_ARGUMENT, "invalid sensor initialization");
+    // the poses, radiance and counters below are buffers a queued setup uses
+    ahead_.reset();
+    expected_ = Expected{};
     DeviceArray<dvs_optical_state> state;
     state.upload(&pose, 1, "initial pose", stream_);
     poses_.reserve(1, "initial pose");
@@ -352,6 +378,25 @@ class Engine {
 
   // Queues the n - 1 intervals between consecutive times, their states to be
   // evaluated from `trajectory` on the device when their step runs.
+  // The window the next adaptive-times call on segment `id` will most
+  // likely run (n == 0 forgets it).  Kept until the end of the next
+  // adaptive-times call.
+  void expect_adaptive(int id, dvs_trajectory trajectory, const double *times, int64_t n, double tolerance) {
+    (void)segment(id);
+    expected_ = Expected{};
+    if (n == 0) return;
+    if (!trajectory || !times || n < 2 || !std::isfinite(tolerance) || tolerance <= 0)
+      fail(DVS_E_ARGUMENT, "invalid expected adaptive window");
+    for (int64_t k = 0; k + 1 < n; ++k)
+      if (!std::isfinite(times[k]) || !std::isfinite(times[k + 1]) || times[k + 1] <= times[k])
+        fail(DVS_E_ARGUMENT, "sensor intervals must be positive and contiguous");
+    expected_.id = id;
+    expected_.trajectory = trajectory;
+    expected_.times.assign(times, times + n);
+    expected_.tolerance = tolerance;
+  }
+  void forget_expected() { expected_ = Expected{}; }
+
   void feed_trajectory(int id, dvs_trajectory trajectory, const double *times, int64_t n) {
     Segment &s = segment(id);
     if (!s.initialized || n < 2 || !times || !trajectory) fail(DVS_E_ARGUMENT, "sensor not initialized or no intervals");
@@ -598,9 +643,14 @@ class Engine {
   // Cell tables of a step: every segment's intervals and circuit cells
   // (generate_pixels' cell split, event_sensor.cu:37-40), pixel and circuit
   // block offsets.  Returns the trace kernels' LDS bytes.
-  size_t prepare(std::vector<StepSegment> &work, int pixel_total, int cell_total, int iv_total) {
+  // `staged_done`: the caller knows the previous step's staged copies have
+  // run (a host read queued after them has returned), so the staging memory
+  // is free without waiting for the stream.
+  size_t prepare(std::vector<StepSegment> &work, int pixel_total, int cell_total, int iv_total,
+                 bool staged_done = false) {
     hipStream_t st = stream_;
-    stream_wait(st, "step start");  // the previous step's staged copies are done
+    ahead_.reset();  // a step overwrites the tables a queued setup left
+    if (!staged_done) stream_wait(st, "step start");  // the previous step's staged copies are done
     staging_.reset();
     std::vector<dvs_optical_interval> ivs;
     std::vector<double> t0s, t1s;
@@ -949,8 +999,11 @@ class Engine {
   // Compaction by a device prefix sum over every segment's pixel counts,
   // event-time depth (native steps), device sort by (segment, t, y, x) and
   // each accepted segment's readout FIFO.  Returns the events time (ms).
+  // `before_wait`: queues more work on the stream once the step's last
+  // copies are queued, before the host waits for them (the next window's
+  // setup); the collect's own host read has returned by then.
   double collect(std::vector<StepSegment> &work, int pixel_total, size_t lds, bool depth,
-                 const std::vector<char> &accepted) {
+                 const std::vector<char> &accepted, const std::function<void()> &before_wait = nullptr) {
     hipStream_t st = stream_;
     const int nseg = (int)work.size();
     Timer events_t;
@@ -1104,6 +1157,7 @@ class Engine {
     hip_check(hipEventRecord(events_t.stop, st), "record");
     host_words_.ptr[0] = -1;
     hip_check(hipMemcpyAsync(host_words_.ptr, status_.ptr, sizeof(int), hipMemcpyDeviceToHost, st), "copy status");
+    if (before_wait) before_wait();
     wait_words(host_words_.ptr, 1, -1, st, "copy events");
     // The event records, their segments, the ray counter and the readout's
     // results arrive by other copies than the polled status word; only the
@@ -1180,45 +1234,45 @@ class Engine {
   // prepare_exact_cells, sample_cells and EventSensor::process_adaptive).
   // accepted[i] == 0 when segment i's convergence gate rejected its window;
   // such a window changes no state.
-  void run_adaptive(std::vector<StepSegment> &work, int pixel_total, int cell_total, int iv_total,
-                    std::vector<char> &accepted) {
+  // The part of an adaptive window that reads no pixel state: cell tables,
+  // the window's horizon states, cell poses and fractions, and the five
+  // initial probes of every pixel.  Queued on the stream; the returned
+  // record carries what the rest of the window needs.
+  // `ahead`: queued for the window after the current one, before the current
+  // one's last copies have run; cell_base is then the segment's cell count
+  // once the current window is integrated.
+  std::unique_ptr<AdaptSetup> setup_adaptive(std::vector<StepSegment> &work, int pixel_total, int cell_total,
+                                             int iv_total, bool ahead = false, uint64_t cell_base = 0) {
     hipStream_t st = stream_;
-    Timer step_t, refine_t, circuit_t;
-    hip_check(hipEventRecord(step_t.start, st), "record");
+    auto su = std::make_unique<AdaptSetup>();
+    hip_check(hipEventRecord(su->step_t.start, st), "record");
     ++step_serial_;
-    const size_t lds = prepare(work, pixel_total, cell_total, iv_total);
+    const size_t lds = prepare(work, pixel_total, cell_total, iv_total, ahead);
     last_lds_ = lds;
     last_nseg_ = (int)work.size();
-    for (int i = 0; i < (int)work.size(); ++i) {
-      Segment &s = *work[i].seg;
-      s.window_step = step_serial_;
-      s.window_index = i;
-      s.window_pixel_offset = work[i].pixel_offset;
-      s.window_ivs = (int)work[i].ivs.size();
-      s.window_start = work[i].ivs.front().t0;
-      s.window_end = work[i].ivs.back().t1;
-    }
     const int nseg = (int)work.size();
     const int P = pixel_total;
-    const int samples = work[0].seg->scene->view.samples, R = samples * samples, slots = 256 / R;
-    accepted.assign(nseg, 1);
-    std::vector<SegDev> tables(nseg);
-    std::vector<AdaptDev> ad(nseg);
+    const int samples = work[0].seg->scene->view.samples;
+    su->lds = lds;
+    su->tables.resize(nseg);
+    su->ad.resize(nseg);
     for (int i = 0; i < nseg; ++i) {
       StepSegment &w = work[i];
       Segment &s = *w.seg;
       if (!s.capacity) s.capacity = adaptive_capacity(s.npix);
-      tables[i] = table_entry(w, 0, w.cells, nullptr);
-      AdaptDev &a = ad[i];
+      su->tables[i] = table_entry(w, 0, w.cells, nullptr);
+      AdaptDev &a = su->ad[i];
       a = AdaptDev{};
       a.start = w.ivs.front().t0;
       a.end = w.ivs.back().t1;
       a.tolerance = w.tolerance;
-      a.cell_base = s.cells_done;
+      a.cell_base = ahead ? cell_base : s.cells_done;
       a.capacity = s.capacity;
       a.accepted = 1;
       a.cell_offset = w.cell_offset;
     }
+    std::vector<SegDev> &tables = su->tables;
+    std::vector<AdaptDev> &ad = su->ad;
     segtab_.upload(tables.data(), tables.size(), "segment table", st, staging_);
     adapt_.upload(ad.data(), ad.size(), "adaptive windows", st, staging_);
     staging_.flush(st);
@@ -1228,7 +1282,7 @@ class Engine {
     hip_check(launch_adapt_cell_frac(cell_total, cell_t0_.ptr, cell_t1_.ptr, cellbeg_.ptr, nseg, adapt_.ptr,
                                      cell_frac_.ptr, st),
               "cell fractions");
-    hip_check(hipEventRecord(refine_t.start, st), "record");
+    hip_check(hipEventRecord(su->refine_t.start, st), "record");
     // Five initial probes per pixel.  The probes alternate between two
     // buffers across windows: a start probe whose pose equals the previous
     // window's end pose takes that probe's radiance, provided the segments
@@ -1248,16 +1302,104 @@ class Engine {
                                        probe_reuse_.ptr, st),
               "probe poses");
     probes_[pcur].reserve((size_t)P * 5, "optical probes");
-    std::vector<std::unique_ptr<Timer>> query_t;
-    Timer probe_t;
-    hip_check(hipEventRecord(probe_t.start, st), "record");
+    hip_check(hipEventRecord(su->probe_t.start, st), "record");
     hip_check(launch_adapt_trace(dvsdev::LAYOUT_PROBE, segtab_.ptr, pbegin_.ptr, nseg, samples, (long long)P * 5,
                                  nullptr, nullptr, probe_poses_.ptr, lds, probes_[pcur].ptr, status_.ptr, counter_.ptr,
                                  st, probe_reuse_.ptr, probes_[pprev].ptr, tuning_.trace_threads, wave_),
               "refine pixel optical launch");
-    hip_check(hipEventRecord(probe_t.stop, st), "record");
+    hip_check(hipEventRecord(su->probe_t.stop, st), "record");
     probe_layout_ = std::move(layout);
     probe_cur_ = pcur;
+    su->pcur = pcur;
+    return su;
+  }
+
+  // The setup of the window the caller expects next (expect_adaptive),
+  // queued in the collect step of the accepted window before it, once that
+  // window's last copies are queued and before the host waits for them: the
+  // device goes on from one window to the next while the host finishes the
+  // first and turns to the next call.  `end_time` and `cells_done` are the
+  // segment's once the current window is integrated.  The next window uses
+  // it when everything the setup read is unchanged (ahead_matches); any
+  // other step replaces it.
+  void setup_ahead(int id, double end_time, uint64_t cells_done) {
+    Expected e = std::move(expected_);
+    expected_ = Expected{};
+    Segment &s = segment(id);
+    if (e.id != id || e.times.size() < 2 || s.diagnostics || e.times[0] != end_time) return;
+    StepSegment w;
+    w.seg = &s;
+    for (size_t k = 0; k + 1 < e.times.size(); ++k) {
+      dvs_optical_interval iv{};
+      iv.t0 = e.times[k];
+      iv.t1 = e.times[k + 1];
+      w.ivs.push_back(iv);
+      w.src.push_back(e.trajectory);
+    }
+    w.tolerance = e.tolerance;
+    try {
+      check_adaptive_window(w);
+    } catch (...) {
+      return;  // the call that runs this window reports its error itself
+    }
+    int cells = 0;
+    for (auto &iv : w.ivs) cells += (int)std::ceil((iv.t1 - iv.t0) / s.photo.integration_step);
+    std::vector<StepSegment> work;
+    work.push_back(std::move(w));
+    std::unique_ptr<AdaptSetup> su = setup_adaptive(work, s.npix, cells, (int)work[0].ivs.size(), true, cells_done);
+    su->work = std::move(work);
+    su->cell_total = cells;
+    su->cells_done = cells_done;
+    su->capacity = s.capacity;
+    ahead_ = std::move(su);  // after prepare(), which drops any earlier one
+  }
+
+
+  // Whether the queued setup `su` is that of window `work` now: the same
+  // segment, intervals, trajectory and tolerance, and the segment's cell
+  // count and capacity unchanged since it was queued.
+  static bool ahead_matches(const AdaptSetup &su, const std::vector<StepSegment> &work, int cell_total) {
+    if (work.size() != 1 || su.work.size() != 1 || su.cell_total != cell_total) return false;
+    const StepSegment &a = su.work[0], &b = work[0];
+    if (a.seg != b.seg || a.ivs.size() != b.ivs.size() || a.src != b.src || a.tolerance != b.tolerance) return false;
+    if (su.cells_done != b.seg->cells_done || su.capacity != b.seg->capacity) return false;
+    for (size_t k = 0; k < a.ivs.size(); ++k)
+      if (a.ivs[k].t0 != b.ivs[k].t0 || a.ivs[k].t1 != b.ivs[k].t1 || b.src[k] == nullptr) return false;
+    return true;
+  }
+
+  void run_adaptive(std::vector<StepSegment> &work, int pixel_total, int cell_total, int iv_total,
+                    std::vector<char> &accepted) {
+    hipStream_t st = stream_;
+    std::unique_ptr<AdaptSetup> su;
+    if (ahead_ && ahead_matches(*ahead_, work, cell_total)) {
+      su = std::move(ahead_);
+      work[0].cells = su->work[0].cells;
+      stats.windows_set_up_ahead += 1;
+    } else {
+      su = setup_adaptive(work, pixel_total, cell_total, iv_total);
+    }
+    ahead_.reset();
+    Timer &step_t = su->step_t, &refine_t = su->refine_t, &probe_t = su->probe_t;
+    Timer circuit_t;
+    const size_t lds = su->lds;
+    for (int i = 0; i < (int)work.size(); ++i) {
+      Segment &s = *work[i].seg;
+      s.window_step = step_serial_;
+      s.window_index = i;
+      s.window_pixel_offset = work[i].pixel_offset;
+      s.window_ivs = (int)work[i].ivs.size();
+      s.window_start = work[i].ivs.front().t0;
+      s.window_end = work[i].ivs.back().t1;
+    }
+    const int nseg = (int)work.size();
+    const int P = pixel_total;
+    const int samples = work[0].seg->scene->view.samples, R = samples * samples, slots = 256 / R;
+    accepted.assign(nseg, 1);
+    std::vector<AdaptDev> &ad = su->ad;
+    for (int i = 0; i < nseg; ++i) ad[i].tolerance = work[i].tolerance;
+    const int pcur = su->pcur;
+    std::vector<std::unique_ptr<Timer>> query_t;
     for (auto *a : {&bound_, &flags_, &bisects_, &exact_terms_, &kcount_, &koff_, &ecount_, &eoff_, &fb_index_,
                     &fb_scan_})
       a->reserve(P, "optical refinement state");
@@ -1609,7 +1751,14 @@ class Engine {
     }
     hip_check(hipEventRecord(circuit_t.stop, st), "record");
     hip_check(hipEventRecord(step_t.stop, st), "record");
-    const double events_ms = collect(work, pixel_total, lds, false, accepted);
+    std::function<void()> ahead;
+    if (nseg == 1 && accepted[0] && expected_.id == work[0].seg->id) {
+      const int id = work[0].seg->id;
+      const double end_time = work[0].ivs.back().t1;
+      const uint64_t cells_after = work[0].seg->cells_done + (uint64_t)work[0].cells;
+      ahead = [this, id, end_time, cells_after] { setup_ahead(id, end_time, cells_after); };
+    }
+    const double events_ms = collect(work, pixel_total, lds, false, accepted, ahead);
     // Statistics.
     const double refine_ms = refine_t.ms(), photo_circuit_ms = circuit_t.ms(), step_ms = step_t.ms();
     double circuit_ms = 0;
@@ -1672,6 +1821,9 @@ class Engine {
   DeviceArray<unsigned> kbucket_;  // per-pixel segment words of the knot order
   DeviceArray<double> cell_frac_, qrad_, erad_, gate_rad_, gate_err_, rates_;
   DeviceArray<uint64_t> codes_;
+  // the setup of an expected window queued ahead of it, and the expectation
+  std::unique_ptr<AdaptSetup> ahead_;
+  Expected expected_;
   // probe radiance and end poses of the last two adaptive windows, and the
   // segment layout (id, pixels) of the last one
   DeviceArray<double> probes_[2];
@@ -1950,12 +2102,24 @@ int dvs_sensor_process_adaptive_times(dvs_sensor s, dvs_trajectory trajectory, c
     } catch (...) {
       seg.clear_queue();
       seg.fed_until = seg.last_time;
+      s->engine->forget_expected();
       throw;
     }
     seg.clear_queue();
     seg.fed_until = seg.last_time;
     if (accepted) *accepted = ok ? 1 : 0;
     if (ready_count) *ready_count = (int64_t)seg.ready.size();
+    // an expected next window was set up in the window's collect step, or not at all
+    s->engine->forget_expected();
+  });
+}
+
+int dvs_sensor_expect_adaptive_times(dvs_sensor s, dvs_trajectory trajectory, const double *times, int32_t n,
+                                     double tolerance) {
+  return guarded([&] {
+    if (!s) fail(DVS_E_ARGUMENT, "null sensor");
+    if (n < 0) fail(DVS_E_ARGUMENT, "invalid expected adaptive window");
+    s->engine->expect_adaptive(s->id, trajectory, times, n, tolerance);
   });
 }
 

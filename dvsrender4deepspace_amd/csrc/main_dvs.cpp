@@ -88,7 +88,7 @@ int run(int argc, char **argv) {
   const float write_interval = 1.0f / config.scene.imu_fps;
   double previous_time = start_time, next_check = start_time;
   size_t step = 0;
-  std::vector<double> times;
+  std::vector<double> times, next_times;
   std::vector<dvs::OpticalInterval> intervals;
   bool initialized = false;
   while (step < total_steps) {
@@ -102,8 +102,12 @@ int run(int argc, char **argv) {
     }
     const double first_time = previous_time;
     if (adaptive) {
-      // the window's states are evaluated on the device inside the window
+      // the window's states are evaluated on the device inside the window;
+      // the next window (if this one is accepted) is set up on the device
+      // before this call returns
       const bool check = first_time >= next_check;
+      window_times(step + times.size() - 1, times.back(), next_times);
+      if (next_times.size() >= 2) raycaster.expect_adaptive(traj, next_times, kDefaultOpticalLogTolerance);
       if (!raycaster.advance_adaptive(traj, times, kDefaultOpticalLogTolerance, events, check)) {
         if (batch > 1) {
           batch = std::max(1U, batch / 2);

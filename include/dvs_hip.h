@@ -146,6 +146,9 @@ typedef struct {
   /* readout FIFO on the device: segment-steps run, and those the scan form
    * handed to the serial recurrence */
   uint64_t readout_tasks, readout_serial_tasks;
+  /* adaptive windows whose setup was queued ahead, at the end of the window
+   * before them (dvs_sensor_expect_adaptive_times) */
+  uint64_t windows_set_up_ahead;
 } dvs_stats;
 
 /* OpticalRefinementStats (include/dvs/optical_sampler.hpp:16) of the last
@@ -383,6 +386,24 @@ int dvs_batch_feed_trajectory(dvs_batch batch, int32_t segment, dvs_trajectory t
 int dvs_sensor_process_adaptive_times(dvs_sensor sensor, dvs_trajectory trajectory, const double *times, int32_t n,
                                       double tolerance, int32_t check_convergence, int32_t *accepted,
                                       int64_t *ready_count);
+/* The window the next dvs_sensor_process_adaptive_times call on this sensor
+ * will most likely run (its times, trajectory and tolerance; n = 0 forgets
+ * it).  When the current call's window is accepted, the expected window's
+ * setup (its horizon states, cell tables and the five initial probes of
+ * every pixel, none of which reads pixel state) is queued on the device
+ * before that call returns, so the device works on it while the host turns
+ * to the next call.  The next call uses it when its window is the expected
+ * one and the sensor has done nothing else in between; otherwise it is
+ * dropped and the window is set up as usual.  Events and pixel states are
+ * the same either way.  The expectation lasts until the end of the next
+ * adaptive-times call.  With diagnostics enabled nothing is queued ahead;
+ * after a setup was queued, dvs_sensor_verify_optics and
+ * dvs_sensor_verify_receptor report no window to verify.  No reference
+ * counterpart: it is for window loops such as bin/dvs's
+ * (apps/main_dvs.cpp:96-116), whose next window is known before the current
+ * one ends unless the current one is rejected. */
+int dvs_sensor_expect_adaptive_times(dvs_sensor sensor, dvs_trajectory trajectory, const double *times, int32_t n,
+                                     double tolerance);
 /* Launch shapes of the adaptive solver's kernels, for autotuning; results
  * do not depend on them.  No reference counterpart.
  *   rate_cells     circuit cells each photocurrent thread draws in one run

@@ -529,6 +529,15 @@ class DvsRayCaster {
     take(events, ready);
     return accepted != 0;
   }
+  // The window the next advance_adaptive(trajectory, times, ...) call will
+  // most likely run: when the current call's window is accepted, its setup
+  // is queued on the device before that call returns
+  // (dvs_sensor_expect_adaptive_times).  Events are the same either way; an
+  // empty `times` forgets it.  No reference counterpart.
+  void expect_adaptive(const CameraTrajectory &trajectory, const std::vector<double> &times, double tolerance) {
+    detail::check(dvs_sensor_expect_adaptive_times(sensor_, times.empty() ? nullptr : trajectory.native_handle(),
+                                                   times.data(), (int32_t)times.size(), tolerance));
+  }
   ReceptorProbeResult last_convergence() const {
     ReceptorProbeResult r{};
     detail::check(dvs_sensor_last_convergence(sensor_, &r));

@@ -126,3 +126,53 @@ def test_sensor_windows_from_trajectory_times_equal_host_intervals(gpu):
     assert out[0][1] == out[1][1]
     assert len(out[0][0]) > 20
     assert out[0][0].tobytes() == out[1][0].tobytes()
+
+
+def test_windows_set_up_ahead_equal_plain_windows(gpu):
+    """dvs_sensor_expect_adaptive_times: a window whose setup was queued at
+    the end of the window before it gives the events of the same window set
+    up in its own call, byte for byte, over bin/dvs windows of the small
+    request's flight with receptor checks at windows 0 and 5 and a native
+    window (which drops a queued setup) at window 4.  An expectation the
+    next call does not match (another tolerance) is not used."""
+    import scenes
+    from dvsrender4deepspace_amd import driver, workloads
+    w = scenes.fast_pass(workloads.small_variant(workloads.PLAIN_SMALL, size=96, craters=6, samples=2))
+    v, n, nodes, root, idx = scenes.workload_mesh(w)
+    scene = gpu.Scene(nodes, root, v, n, idx, w.camera, w.hapke)
+    traj = workloads.trajectory(w)
+    grids, t, step = [], w.start_time, 0
+    for k in range(10):
+        times = workloads.horizon_grid(w, t, step, 20)
+        grids.append(times)
+        t, step = float(times[-1]), step + len(times) - 1
+    tol = driver.DEFAULT_TOLERANCE
+    out = {}
+    for form in ("plain", "ahead", "other tolerance"):
+        print(f"form {form}", flush=True)
+        sensor = gpu.Sensor(scene, w.physic)
+        sensor.initialize(w.initial_state(), w.start_time)
+        got, oks = [], []
+        for k, times in enumerate(grids):
+            if k == 4:  # a native window between adaptive ones
+                got.append(sensor.process(traj.intervals(times)))
+                continue
+            for check in ((True, False) if k in (0, 5) else (False,)):
+                if form != "plain" and k + 1 < len(grids):  # window 4's setup is dropped
+                    sensor.expect_adaptive_times(traj, grids[k + 1], tol if form == "ahead" else 2 * tol)
+                ok, ev = sensor.process_adaptive_times(traj, times, tol, check)
+                oks.append(ok)
+                if ok:
+                    break
+            assert ok
+            got.append(ev)
+        got.append(sensor.drain())
+        out[form] = (np.concatenate(got), oks, sensor.stats()["windows_set_up_ahead"])
+        sensor.close()
+    print({k: (len(v[0]), v[2]) for k, v in out.items()}, flush=True)
+    assert len(out["plain"][0]) > 20
+    assert out["plain"][2] == 0 and out["other tolerance"][2] == 0
+    assert out["ahead"][2] >= 5
+    for form in ("ahead", "other tolerance"):
+        assert out[form][1] == out["plain"][1]
+        assert out[form][0].tobytes() == out["plain"][0].tobytes()
