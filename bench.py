@@ -71,12 +71,12 @@ SLICES = 20
 # bytes (FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0: a 1 GiB device copy reads
 # FETCH_SIZE 0.5 GiB and WRITE_SIZE 1 GiB), per unit of each kernel's work.
 PROFILE = {
-    "probe": {"unit": "traced pixel sample", "valu_insts": 1698.36, "valu_busy": 1.273, "lane_util": 0.864,
-              "l2_hit": 0.871, "fetch": 293.38, "write": 23.70},
-    "query": {"unit": "traced pixel sample", "valu_insts": 2019.91, "valu_busy": 0.565, "lane_util": 0.778,
-              "l2_hit": 0.830, "fetch": 659.29, "write": 12.00},
-    "rates": {"unit": "pixel cell", "valu_insts": 2.2363, "valu_busy": 0.828, "lane_util": 0.903,
-              "l2_hit": 0.683, "fetch": 0.1519, "write": 1.0010},
+    "probe": {"unit": "traced pixel sample", "valu_insts": 1698.36, "valu_busy": 1.260, "lane_util": 0.864,
+              "l2_hit": 0.871, "fetch": 293.39, "write": 23.70},
+    "query": {"unit": "traced pixel sample", "valu_insts": 2019.82, "valu_busy": 0.568, "lane_util": 0.778,
+              "l2_hit": 0.830, "fetch": 659.44, "write": 12.00},
+    "rates": {"unit": "pixel cell", "valu_insts": 2.2363, "valu_busy": 0.817, "lane_util": 0.903,
+              "l2_hit": 0.683, "fetch": 0.1520, "write": 1.0010},
     "integrate": {"unit": "pixel cell", "valu_insts": 0.3773, "valu_busy": 0.249, "lane_util": 0.508,
                   "l2_hit": 0.787, "fetch": 1.1097, "write": 0.0366},
 }
