@@ -1053,7 +1053,7 @@ class Engine {
     // records and copy them (and their segments) to the host.  Over the
     // window's span the time sort takes two passes: by the top 16 of the
     // key bits that vary (a bucket is a few tens of nanoseconds), then
-    // inside each bucket by the whole key; a bucket of more than 64 events
+    // inside each bucket by the whole key; a bucket of more than 1024 events
     // raises the key flag, and the events are then sorted on all bits.
     auto sort_events = [&](int end_bit) {
       size_t bytes = 0, bucketed = 0;

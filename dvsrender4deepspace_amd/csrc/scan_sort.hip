@@ -206,24 +206,46 @@ __global__ void copy_pairs_kernel(const unsigned long long *keys_in, const unsig
 }
 
 // After a stable sort by bucket b(key) = (key - sub) >> pre: every item
-// finds its bucket's extent (at most kBucketCap items either side), ranks
-// itself among the bucket's items by (key, position) and lands at the
+// finds its bucket's extent (the buckets are contiguous and ascending: a
+// short walk, then a binary search, at most kBucketCap items either side),
+// ranks itself among the bucket's items by (key, position) and lands at the
 // bucket's start plus that rank, out of place.  Equal keys keep their order,
 // so the output is the stable sort by the whole key.  An item of a bucket
 // of more than kBucketCap items is copied in place and sets *flag (the
 // caller then sorts on all key bits).
-constexpr int kBucketCap = 64;
+constexpr int kBucketCap = 1024;
+constexpr int kBucketWalk = 8;  // linear steps before the binary search
 __global__ __launch_bounds__(256) void bucket_rank_kernel(const unsigned long long *keys_in, const unsigned *vals_in,
                                                           long long n, KeyView kv, unsigned long long *keys_out,
                                                           unsigned *vals_out, int *flag) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const unsigned long long key = keys_in[i];
-  const unsigned long long b = (key - kv.sub) >> kv.pre;
-  long long s = i, e = i + 1;
-  while (s > 0 && i - s < kBucketCap && ((keys_in[s - 1] - kv.sub) >> kv.pre) == b) --s;
-  while (e < n && e - s <= kBucketCap && ((keys_in[e] - kv.sub) >> kv.pre) == b) ++e;
-  const bool over = (s > 0 && ((keys_in[s - 1] - kv.sub) >> kv.pre) == b) || e - s > kBucketCap;
+  auto bucket = [&](long long j) { return (keys_in[j] - kv.sub) >> kv.pre; };
+  const unsigned long long b = bucket(i);
+  // first index s of the bucket within [i - kBucketCap, i]
+  long long s = i;
+  for (int k = 0; k < kBucketWalk && s > 0 && i - s < kBucketCap && bucket(s - 1) == b; ++k) --s;
+  if (s > 0 && i - s < kBucketCap && bucket(s - 1) == b) {
+    long long lo = i - kBucketCap > 0 ? i - kBucketCap : 0, hi = s - 1;  // bucket(hi) == b
+    while (lo < hi) {
+      const long long m = lo + (hi - lo) / 2;
+      if (bucket(m) == b) hi = m; else lo = m + 1;
+    }
+    s = lo;
+  }
+  // one past the last index e of the bucket within [i + 1, s + kBucketCap + 1]
+  long long e = i + 1;
+  for (int k = 0; k < kBucketWalk && e < n && e - s <= kBucketCap && bucket(e) == b; ++k) ++e;
+  if (e < n && e - s <= kBucketCap && bucket(e) == b) {
+    long long lo = e, hi = s + kBucketCap + 1 < n ? s + kBucketCap + 1 : n;  // bucket(lo) == b
+    while (hi - lo > 1) {  // invariant: bucket(lo) == b, and hi == n or bucket(hi) != b or hi is past the cap
+      const long long m = lo + (hi - lo) / 2;
+      if (bucket(m) == b) lo = m; else hi = m;
+    }
+    e = lo + 1;
+  }
+  const bool over = (s > 0 && bucket(s - 1) == b) || e - s > kBucketCap || (e < n && bucket(e) == b);
   long long at = i;
   if (over) {
     atomicOr(flag, 1);

@@ -57,18 +57,21 @@ def test_sort_of_event_time_keys(gpu):
     assert np.array_equal(vo, np.argsort(times, kind="stable"))
 
 
+BUCKET_CAP = 1024  # keys a bucket may hold and still be sorted inside
+
+
 def _bucketed_expectation(keys, lo, key_bits, bucket_bits):
-    """Stable order by bucket, then by key inside buckets of at most 64 keys
-    (larger buckets keep their input order), and whether one was larger."""
+    """Stable order by bucket, then by key inside buckets of at most 1024
+    keys (larger buckets keep their input order), and whether one was larger."""
     bucket = (keys - np.uint64(lo)) >> np.uint64(key_bits - bucket_bits)
     by_bucket = np.argsort(bucket, kind="stable")
     _, first, sizes = np.unique(bucket[by_bucket], return_index=True, return_counts=True)
     order = by_bucket.copy()
     for f, c in zip(first, sizes):
-        if c <= 64:
+        if c <= BUCKET_CAP:
             part = by_bucket[f:f + c]
             order[f:f + c] = part[np.argsort(keys[part], kind="stable")]
-    return order, bool((sizes > 64).any())
+    return order, bool((sizes > BUCKET_CAP).any())
 
 
 @pytest.mark.parametrize("n", [1, 300, 100_000])
@@ -92,10 +95,11 @@ def test_bucketed_sort_is_stable_sort(gpu, n):
         assert np.array_equal(vo, np.argsort(keys, kind="stable"))
 
 
-@pytest.mark.parametrize("size", [64, 65, 500])
+@pytest.mark.parametrize("size", [9, 64, 1023, 1024, 1025, 3000])
 def test_bucketed_sort_bucket_limit(gpu, size):
-    """A bucket of 64 keys is sorted inside; one of 65 or more is left in
-    input order and reported, while the other buckets are still sorted."""
+    """A bucket of up to 1024 keys is sorted inside (its extent found by a
+    short walk, then a binary search); one of 1025 or more is left in input
+    order and reported, while the other buckets are still sorted."""
     rng = np.random.default_rng(size)
     lo, key_bits, bucket_bits = 1 << 50, 32, 12
     width = 1 << (key_bits - bucket_bits)
@@ -105,7 +109,7 @@ def test_bucketed_sort_bucket_limit(gpu, size):
     values = np.arange(len(keys), dtype=np.uint32)
     ko, vo, over = gpu.sort_bucketed_eval(keys, values, lo, key_bits, bucket_bits)
     order, expect_over = _bucketed_expectation(keys, lo, key_bits, bucket_bits)
-    assert expect_over == (size + int(((spread - np.uint64(lo)) // np.uint64(width) == 77).sum()) > 64)
+    assert expect_over == (size + int(((spread - np.uint64(lo)) // np.uint64(width) == 77).sum()) > BUCKET_CAP)
     assert over == expect_over
     assert np.array_equal(vo, values[order])
     assert np.array_equal(ko, keys[order])
