@@ -135,3 +135,14 @@ def test_error_codes_without_gpu_paths():
     """Argument errors are reported before any device work."""
     with pytest.raises(capi.InvalidArgument):
         capi.bvh_build(np.zeros((3, 3), np.float32), np.array([[0, 1, 5]], np.uint32))
+
+
+def test_expected_window_argument_errors_without_gpu_paths():
+    """dvs_sensor_expect_adaptive_times refuses a missing sensor and a
+    negative length before any device work (error 3, invalid argument)."""
+    lib = capi.load_library()
+    times = np.array([0.0, 1e-5, 2e-5])
+    ptr = times.ctypes.data_as(ctypes.c_void_p)
+    assert lib.dvs_sensor_expect_adaptive_times(None, None, ptr, 3, 1e-2) == 3
+    assert "null sensor" in lib.dvs_last_error().decode()
+    assert lib.dvs_sensor_expect_adaptive_times(None, None, ptr, -1, 1e-2) == 3
